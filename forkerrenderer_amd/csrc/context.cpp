@@ -1,0 +1,1001 @@
+// context.cpp -- the fgl_* C ABI: device resources, ForkerGL-style state, pass
+// orchestration on one HIP stream, readback, scene rendering.
+#include <hip/hip_runtime.h>
+
+#include <cfloat>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/fgl.h"
+#include "fgl_kernels.hpp"
+#include "host_io.hpp"
+
+using namespace fgl;
+
+namespace {
+
+struct DevBuf {
+    void *p = nullptr;
+    size_t cap = 0;
+};
+
+struct ModelRes {
+    float *pos = nullptr, *uv = nullptr, *nrm = nullptr, *tan = nullptr;
+    int *vi = nullptr, *ti = nullptr, *ni = nullptr, *fmat = nullptr;
+    int n_pos = 0, n_faces = 0;
+    bool has_tan = false, pbr = false;
+};
+
+struct TimedKernel {
+    std::string name;
+    hipEvent_t a, b;
+};
+
+}  // namespace
+
+struct fgl_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::string err;
+    int TS = 16;
+
+    std::vector<DevTexture> textures;
+    std::vector<void *> texture_mem;
+    DevBuf d_textures;
+    bool textures_dirty = true;
+    std::vector<DevMaterial> materials;
+    DevBuf d_materials;
+    bool materials_dirty = true;
+    std::vector<ModelRes> models;
+
+    // ForkerGL state
+    int W = 0, H = 0, SW = 0, SH = 0;
+    float viewport[16], shadow_viewport[16], view[16], proj[16], light_space[16];
+    float eye[3] = {0.f, 0.f, 0.f};
+    std::vector<fgl_light> lights;
+    int shadow_on = 1;
+    fgl_shadow_cfg shadow_cfg;
+
+    // targets
+    DevBuf depth, fb, gbuf, ao, blur_tmp, shadow, shadow_depth, img;
+    int img_w = 0, img_h = 0;
+
+    // current pass
+    int pass = -1;
+    struct Rec {
+        int model;
+        float m[16];
+    };
+    std::vector<Rec> draws;
+
+    // pass scratch
+    DevBuf d_draws, d_tri_base, d_vert_base, screen, rect, counts, offs, scan_tmp, keys, vals,
+        keys_alt, vals_alt, sort_tmp, tile_start, tile_end;
+    std::vector<DevDraw> last_draws;
+    std::vector<int> last_tri_base, last_vert_base;
+    uint32_t *h_total = nullptr;
+
+    // timing
+    bool timing = false;
+    std::vector<TimedKernel> timed;
+};
+
+static int fail(fgl_ctx *c, int code, const std::string &msg)
+{
+    if (c) c->err = msg;
+    return code;
+}
+
+#define HIPCHK(ctx, expr)                                                                   \
+    do {                                                                                    \
+        hipError_t e_ = (expr);                                                             \
+        if (e_ != hipSuccess)                                                               \
+            return fail(ctx, FGL_ERR_DEVICE, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+static int ensure(fgl_ctx *c, DevBuf &b, size_t bytes)
+{
+    if (bytes <= b.cap && b.p) return FGL_OK;
+    if (b.p) {
+        hipStreamSynchronize(c->stream);
+        hipFree(b.p);
+        b.p = nullptr;
+        b.cap = 0;
+    }
+    size_t want = bytes < 256 ? 256 : bytes;
+    if (hipMalloc(&b.p, want) != hipSuccess) {
+        b.p = nullptr;
+        return fail(c, FGL_ERR_NOMEM, "hipMalloc of " + std::to_string(want) + " bytes failed");
+    }
+    b.cap = want;
+    return FGL_OK;
+}
+
+static void free_buf(DevBuf &b)
+{
+    if (b.p) hipFree(b.p);
+    b.p = nullptr;
+    b.cap = 0;
+}
+
+// ---------------- timing ----------------
+
+struct KernelTimer {
+    fgl_ctx *c;
+    TimedKernel tk;
+    bool on;
+    KernelTimer(fgl_ctx *ctx, const char *name) : c(ctx), on(ctx->timing)
+    {
+        if (!on) return;
+        tk.name = name;
+        hipEventCreate(&tk.a);
+        hipEventCreate(&tk.b);
+        hipEventRecord(tk.a, c->stream);
+    }
+    ~KernelTimer()
+    {
+        if (!on) return;
+        hipEventRecord(tk.b, c->stream);
+        c->timed.push_back(tk);
+    }
+};
+
+template <typename T>
+static int upload_array(fgl_ctx *c, T **dst, const T *src, size_t n)
+{
+    *dst = nullptr;
+    HIPCHK(c, hipMalloc((void **)dst, (n ? n : 1) * sizeof(T)));
+    if (n) HIPCHK(c, hipMemcpy(*dst, src, n * sizeof(T), hipMemcpyHostToDevice));
+    return FGL_OK;
+}
+
+extern "C" {
+
+int fgl_ctx_create(int device, fgl_ctx **out)
+{
+    if (!out) return FGL_ERR_INVALID;
+    *out = nullptr;
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) return FGL_ERR_DEVICE;
+    if (device < 0 || device >= n) return FGL_ERR_INVALID;
+    if (hipSetDevice(device) != hipSuccess) return FGL_ERR_DEVICE;
+    fgl_ctx *c = new fgl_ctx();
+    c->device = device;
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+        delete c;
+        return FGL_ERR_DEVICE;
+    }
+    if (hipHostMalloc((void **)&c->h_total, sizeof(uint32_t) * 4, hipHostMallocDefault) != hipSuccess) {
+        hipStreamDestroy(c->stream);
+        delete c;
+        return FGL_ERR_NOMEM;
+    }
+    float I[16] = {1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1};
+    std::memcpy(c->viewport, I, sizeof(I));
+    std::memcpy(c->shadow_viewport, I, sizeof(I));
+    std::memcpy(c->view, I, sizeof(I));
+    std::memcpy(c->proj, I, sizeof(I));
+    std::memcpy(c->light_space, I, sizeof(I));
+    std::memset(&c->shadow_cfg, 0, sizeof(c->shadow_cfg));
+    *out = c;
+    return FGL_OK;
+}
+
+void fgl_ctx_destroy(fgl_ctx *c)
+{
+    if (!c) return;
+    hipSetDevice(c->device);
+    hipStreamSynchronize(c->stream);
+    for (void *p : c->texture_mem) hipFree(p);
+    for (auto &m : c->models) {
+        hipFree(m.pos); hipFree(m.uv); hipFree(m.nrm);
+        if (m.tan) hipFree(m.tan);
+        hipFree(m.vi); hipFree(m.ti); hipFree(m.ni); hipFree(m.fmat);
+    }
+    DevBuf *bufs[] = {&c->d_textures, &c->d_materials, &c->depth, &c->fb, &c->gbuf, &c->ao,
+                      &c->blur_tmp, &c->shadow, &c->shadow_depth, &c->img, &c->d_draws,
+                      &c->d_tri_base, &c->d_vert_base, &c->screen, &c->rect, &c->counts,
+                      &c->offs, &c->scan_tmp, &c->keys, &c->vals, &c->keys_alt, &c->vals_alt,
+                      &c->sort_tmp, &c->tile_start, &c->tile_end};
+    for (DevBuf *b : bufs) free_buf(*b);
+    for (auto &t : c->timed) { hipEventDestroy(t.a); hipEventDestroy(t.b); }
+    if (c->h_total) hipHostFree(c->h_total);
+    hipStreamDestroy(c->stream);
+    delete c;
+}
+
+const char *fgl_last_error(const fgl_ctx *c) { return c ? c->err.c_str() : "null context"; }
+
+int fgl_sync(fgl_ctx *c)
+{
+    if (!c) return FGL_ERR_INVALID;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return FGL_OK;
+}
+
+int fgl_set_tile_size(fgl_ctx *c, int tile)
+{
+    if (!c) return FGL_ERR_INVALID;
+    if (tile != 8 && tile != 16 && tile != 32) return fail(c, FGL_ERR_INVALID, "tile must be 8, 16 or 32");
+    c->TS = tile;
+    return FGL_OK;
+}
+
+// ---------------- resources ----------------
+
+int fgl_upload_texture(fgl_ctx *c, int w, int h, int bpp, const uint8_t *bytes, int wrap,
+                       int filter, int *out_id)
+{
+    if (!c || !bytes || !out_id) return FGL_ERR_INVALID;
+    if (w <= 0 || h <= 0 || (bpp != 1 && bpp != 3 && bpp != 4) || wrap < 0 || wrap > 3 ||
+        filter < 0 || filter > 1)
+        return fail(c, FGL_ERR_INVALID, "bad texture parameters");
+    std::vector<uint32_t> packed((size_t)w * h);
+    for (size_t i = 0; i < packed.size(); ++i) {
+        const uint8_t *p = bytes + i * bpp;
+        uint32_t b = p[0], g = bpp > 1 ? p[1] : 0, r = bpp > 2 ? p[2] : 0;
+        packed[i] = r | (g << 8) | (b << 16);
+    }
+    void *mem = nullptr;
+    HIPCHK(c, hipMalloc(&mem, packed.size() * 4));
+    HIPCHK(c, hipMemcpy(mem, packed.data(), packed.size() * 4, hipMemcpyHostToDevice));
+    c->texture_mem.push_back(mem);
+    DevTexture t;
+    t.texels = (const uint32_t *)mem;
+    t.w = w; t.h = h; t.wrap = wrap; t.filter = filter;
+    c->textures.push_back(t);
+    c->textures_dirty = true;
+    *out_id = (int)c->textures.size() - 1;
+    return FGL_OK;
+}
+
+int fgl_upload_material(fgl_ctx *c, const fgl_material *m, int *out_id)
+{
+    if (!c || !m || !out_id) return FGL_ERR_INVALID;
+    const int32_t ids[] = {m->map_kd, m->map_ks, m->map_ke, m->map_bump, m->map_base,
+                           m->map_rough, m->map_metal, m->map_ao, m->map_pbr_ke};
+    for (int id : ids)
+        if (id < -1 || id >= (int)c->textures.size())
+            return fail(c, FGL_ERR_INVALID, "material references an unknown texture id");
+    DevMaterial d;
+    static_assert(sizeof(DevMaterial) == sizeof(fgl_material), "material layout");
+    std::memcpy(&d, m, sizeof(d));
+    c->materials.push_back(d);
+    c->materials_dirty = true;
+    *out_id = (int)c->materials.size() - 1;
+    return FGL_OK;
+}
+
+int fgl_upload_model(fgl_ctx *c, const fgl_model_desc *d, int *out_id)
+{
+    if (!c || !d || !out_id) return FGL_ERR_INVALID;
+    if (d->n_pos <= 0 || d->n_uv <= 0 || d->n_nrm <= 0 || d->n_faces < 0 || !d->pos || !d->uv ||
+        !d->nrm || (d->n_faces > 0 && (!d->vi || !d->ti || !d->ni || !d->face_material)))
+        return fail(c, FGL_ERR_INVALID, "bad model description");
+    size_t nf = (size_t)d->n_faces;
+    for (size_t i = 0; i < 3 * nf; ++i)
+        if (d->vi[i] < 0 || d->vi[i] >= d->n_pos || d->ti[i] < 0 || d->ti[i] >= d->n_uv ||
+            d->ni[i] < 0 || d->ni[i] >= d->n_nrm)
+            return fail(c, FGL_ERR_INVALID, "face index out of range");
+    for (size_t f = 0; f < nf; ++f)
+        if (d->face_material[f] < 0 || d->face_material[f] >= (int)c->materials.size())
+            return fail(c, FGL_ERR_INVALID, "face material out of range");
+    ModelRes m;
+    int rc;
+    if ((rc = upload_array(c, &m.pos, d->pos, 3 * (size_t)d->n_pos))) return rc;
+    if ((rc = upload_array(c, &m.uv, d->uv, 2 * (size_t)d->n_uv))) return rc;
+    if ((rc = upload_array(c, &m.nrm, d->nrm, 3 * (size_t)d->n_nrm))) return rc;
+    if (d->tan && (rc = upload_array(c, &m.tan, d->tan, 3 * (size_t)d->n_pos))) return rc;
+    if ((rc = upload_array(c, &m.vi, (const int *)d->vi, 3 * nf))) return rc;
+    if ((rc = upload_array(c, &m.ti, (const int *)d->ti, 3 * nf))) return rc;
+    if ((rc = upload_array(c, &m.ni, (const int *)d->ni, 3 * nf))) return rc;
+    if ((rc = upload_array(c, &m.fmat, (const int *)d->face_material, nf))) return rc;
+    m.n_pos = d->n_pos;
+    m.n_faces = d->n_faces;
+    m.has_tan = d->tan != nullptr;
+    m.pbr = d->support_pbr != 0;
+    c->models.push_back(m);
+    *out_id = (int)c->models.size() - 1;
+    return FGL_OK;
+}
+
+// ---------------- state ----------------
+
+static int fill_f32(fgl_ctx *c, DevBuf &b, float v, size_t n)
+{
+    launch_fill_f32((float *)b.p, v, n, c->stream);
+    HIPCHK(c, hipGetLastError());
+    return FGL_OK;
+}
+
+int fgl_init_targets(fgl_ctx *c, int w, int h)
+{
+    if (!c) return FGL_ERR_INVALID;
+    if (w <= 0 || h <= 0 || w > 16384 || h > 16384) return fail(c, FGL_ERR_INVALID, "bad target size");
+    size_t P = (size_t)w * h;
+    int rc;
+    if ((rc = ensure(c, c->depth, P * 4)) || (rc = ensure(c, c->fb, 3 * P * 4)) ||
+        (rc = ensure(c, c->gbuf, kGbufPlanes * P * 4)) || (rc = ensure(c, c->ao, P * 4)) ||
+        (rc = ensure(c, c->blur_tmp, P * 4)))
+        return rc;
+    c->W = w;
+    c->H = h;
+    if ((rc = fill_f32(c, c->fb, 0.f, 3 * P)) || (rc = fill_f32(c, c->depth, FLT_MAX, P)) ||
+        (rc = fill_f32(c, c->gbuf, 0.f, kGbufPlanes * P)) || (rc = fill_f32(c, c->ao, 1.f, P)))
+        return rc;
+    return FGL_OK;
+}
+
+int fgl_init_shadow_buffer(fgl_ctx *c, int w, int h)
+{
+    if (!c) return FGL_ERR_INVALID;
+    if (w <= 0 || h <= 0 || w > 16384 || h > 16384) return fail(c, FGL_ERR_INVALID, "bad shadow size");
+    size_t P = (size_t)w * h;
+    int rc;
+    if ((rc = ensure(c, c->shadow, P * 4)) || (rc = ensure(c, c->shadow_depth, P * 4))) return rc;
+    c->SW = w;
+    c->SH = h;
+    if ((rc = fill_f32(c, c->shadow, 0.f, P)) || (rc = fill_f32(c, c->shadow_depth, FLT_MAX, P)))
+        return rc;
+    return FGL_OK;
+}
+
+int fgl_clear_color(fgl_ctx *c, float r, float g, float b)
+{
+    if (!c || !c->fb.p) return fail(c, FGL_ERR_INVALID, "targets not initialised");
+    size_t P = (size_t)c->W * c->H;
+    launch_fill_f32((float *)c->fb.p, r, P, c->stream);
+    launch_fill_f32((float *)c->fb.p + P, g, P, c->stream);
+    launch_fill_f32((float *)c->fb.p + 2 * P, b, P, c->stream);
+    HIPCHK(c, hipGetLastError());
+    return FGL_OK;
+}
+
+int fgl_set_viewport(fgl_ctx *c, int shadow, int x, int y, int w, int h)
+{
+    if (!c) return FGL_ERR_INVALID;
+    make_viewport(x, y, w, h, shadow ? c->shadow_viewport : c->viewport);
+    return FGL_OK;
+}
+
+int fgl_set_matrix(fgl_ctx *c, int which, const float m[16])
+{
+    if (!c || !m) return FGL_ERR_INVALID;
+    float *dst = which == FGL_MAT_VIEW ? c->view
+               : which == FGL_MAT_PROJ ? c->proj
+               : which == FGL_MAT_LIGHT_SPACE ? c->light_space : nullptr;
+    if (!dst) return fail(c, FGL_ERR_INVALID, "unknown matrix");
+    std::memcpy(dst, m, 16 * sizeof(float));
+    return FGL_OK;
+}
+
+int fgl_set_eye(fgl_ctx *c, const float eye[3])
+{
+    if (!c || !eye) return FGL_ERR_INVALID;
+    std::memcpy(c->eye, eye, 3 * sizeof(float));
+    return FGL_OK;
+}
+
+int fgl_set_lights(fgl_ctx *c, int n, const fgl_light *lights)
+{
+    if (!c || n < 0 || n > FGL_MAX_LIGHTS || (n && !lights)) return fail(c, FGL_ERR_INVALID, "bad lights");
+    c->lights.assign(lights, lights + n);
+    return FGL_OK;
+}
+
+int fgl_set_shadow(fgl_ctx *c, int on, const fgl_shadow_cfg *cfg)
+{
+    if (!c) return FGL_ERR_INVALID;
+    if (cfg) {
+        if (cfg->mode < 0 || cfg->mode > 3 || cfg->n_blocker < 0 || cfg->n_blocker > FGL_MAX_DISK ||
+            cfg->n_pcf < 0 || cfg->n_pcf > FGL_MAX_DISK || (cfg->mode == 3 && (cfg->grid < 1 || cfg->grid > 9)) ||
+            ((cfg->mode == 1 || cfg->mode == 2) && cfg->n_pcf < 1))
+            return fail(c, FGL_ERR_INVALID, "bad shadow filter configuration");
+        c->shadow_cfg = *cfg;
+    }
+    c->shadow_on = on ? 1 : 0;
+    return FGL_OK;
+}
+
+static void build_uniforms(fgl_ctx *c, FrameUniforms &u)
+{
+    std::memset(&u, 0, sizeof(u));
+    std::memcpy(u.view, c->view, sizeof(u.view));
+    std::memcpy(u.proj, c->proj, sizeof(u.proj));
+    std::memcpy(u.light_space, c->light_space, sizeof(u.light_space));
+    std::memcpy(u.viewport, c->viewport, sizeof(u.viewport));
+    std::memcpy(u.shadow_viewport, c->shadow_viewport, sizeof(u.shadow_viewport));
+    mat_mul(c->proj, c->view, u.vp);
+    std::memcpy(u.eye, c->eye, sizeof(u.eye));
+    u.shadow_on = c->shadow_on;
+    u.n_lights = (int)c->lights.size();
+    for (int i = 0; i < u.n_lights; ++i) {
+        u.lights[i].type = c->lights[i].type;
+        std::memcpy(u.lights[i].vec, c->lights[i].vec, sizeof(float) * 3);
+        std::memcpy(u.lights[i].color, c->lights[i].color, sizeof(float) * 3);
+    }
+    u.W = c->W;
+    u.H = c->H;
+    u.SW = c->SW;
+    u.SH = c->SH;
+    const fgl_shadow_cfg &s = c->shadow_cfg;
+    u.shadow.mode = s.mode;
+    u.shadow.grid = s.grid;
+    u.shadow.n_blocker = s.n_blocker;
+    u.shadow.n_pcf = s.n_pcf;
+    u.shadow.pcf_filter = s.pcf_filter;
+    u.shadow.area_light = s.area_light;
+    u.shadow.blocker_filter = s.blocker_filter;
+    u.shadow.pcss_filter = s.pcss_filter;
+    std::memcpy(u.shadow.blocker_disk, s.blocker_disk, sizeof(u.shadow.blocker_disk));
+    std::memcpy(u.shadow.pcf_disk, s.pcf_disk, sizeof(u.shadow.pcf_disk));
+}
+
+static int upload_tables(fgl_ctx *c)
+{
+    int rc;
+    if (c->textures_dirty) {
+        if ((rc = ensure(c, c->d_textures, (c->textures.size() + 1) * sizeof(DevTexture)))) return rc;
+        if (!c->textures.empty())
+            HIPCHK(c, hipMemcpyAsync(c->d_textures.p, c->textures.data(),
+                                     c->textures.size() * sizeof(DevTexture),
+                                     hipMemcpyHostToDevice, c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        c->textures_dirty = false;
+    }
+    if (c->materials_dirty) {
+        if ((rc = ensure(c, c->d_materials, (c->materials.size() + 1) * sizeof(DevMaterial)))) return rc;
+        if (!c->materials.empty())
+            HIPCHK(c, hipMemcpyAsync(c->d_materials.p, c->materials.data(),
+                                     c->materials.size() * sizeof(DevMaterial),
+                                     hipMemcpyHostToDevice, c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        c->materials_dirty = false;
+    }
+    return FGL_OK;
+}
+
+// ---------------- passes ----------------
+
+static int flush_pass(fgl_ctx *c);
+
+int fgl_begin_pass(fgl_ctx *c, int pass)
+{
+    if (!c) return FGL_ERR_INVALID;
+    if (pass != FGL_PASS_FORWARD && pass != FGL_PASS_GEOMETRY && pass != FGL_PASS_LIGHTING &&
+        pass != FGL_PASS_SHADOW)
+        return fail(c, FGL_ERR_INVALID, "unknown pass");
+    int rc = flush_pass(c);
+    if (rc) return rc;
+    c->pass = pass;
+    return FGL_OK;
+}
+
+int fgl_draw_model(fgl_ctx *c, int model, const float model_matrix[16])
+{
+    if (!c || !model_matrix) return FGL_ERR_INVALID;
+    if (model < 0 || model >= (int)c->models.size()) return fail(c, FGL_ERR_INVALID, "unknown model");
+    if (c->pass != FGL_PASS_FORWARD && c->pass != FGL_PASS_GEOMETRY && c->pass != FGL_PASS_SHADOW)
+        return fail(c, FGL_ERR_INVALID, "no raster pass begun");
+    fgl_ctx::Rec r;
+    r.model = model;
+    std::memcpy(r.m, model_matrix, sizeof(r.m));
+    c->draws.push_back(r);
+    return FGL_OK;
+}
+
+static int bits_for(uint32_t n)
+{
+    int b = 0;
+    while (b < 32 && (n >> b) != 0) ++b;
+    return b;
+}
+
+static int flush_pass(fgl_ctx *c)
+{
+    if (c->draws.empty()) return FGL_OK;
+    const int pass = c->pass == FGL_PASS_SHADOW ? kPassShadow
+                   : c->pass == FGL_PASS_GEOMETRY ? kPassGeometry : kPassForward;
+    const bool shadow = pass == kPassShadow;
+    const int W = shadow ? c->SW : c->W, H = shadow ? c->SH : c->H;
+    if (W <= 0 || H <= 0 || (shadow && !c->shadow.p) || (!shadow && !c->depth.p)) {
+        c->draws.clear();
+        return fail(c, FGL_ERR_INVALID, "render targets not initialised for this pass");
+    }
+    if (pass == kPassForward && c->shadow_on && !c->shadow.p) {
+        c->draws.clear();
+        return fail(c, FGL_ERR_INVALID, "forward pass with shadows needs a shadow buffer");
+    }
+    int rc = upload_tables(c);
+    if (rc) return rc;
+
+    // draw table in submission order
+    std::vector<DevDraw> dd(c->draws.size());
+    std::vector<int> tri_base(c->draws.size() + 1), vert_base(c->draws.size() + 1);
+    long long nt = 0, nv = 0;
+    for (size_t i = 0; i < c->draws.size(); ++i) {
+        const ModelRes &m = c->models[c->draws[i].model];
+        DevDraw &d = dd[i];
+        std::memset(&d, 0, sizeof(d));
+        d.pos = m.pos; d.uv = m.uv; d.nrm = m.nrm; d.tan = m.tan;
+        d.vi = m.vi; d.ti = m.ti; d.ni = m.ni; d.face_mat = m.fmat;
+        d.n_pos = m.n_pos;
+        d.n_faces = m.n_faces;
+        d.vert_base = (int)nv;
+        d.tri_base = (int)nt;
+        d.has_tangents = m.has_tan;
+        d.support_pbr = m.pbr;
+        std::memcpy(d.model, c->draws[i].m, sizeof(d.model));
+        make_normal_matrix(d.model, d.normal);
+        mat_mul(c->light_space, d.model, d.lm);
+        tri_base[i] = (int)nt;
+        vert_base[i] = (int)nv;
+        nt += m.n_faces;
+        nv += m.n_pos;
+    }
+    tri_base[c->draws.size()] = (int)nt;
+    vert_base[c->draws.size()] = (int)nv;
+    const int n_draws = (int)c->draws.size();
+    c->draws.clear();
+    if (nt > 0x7fffffffLL || nv > 0x7fffffffLL) return fail(c, FGL_ERR_CAPACITY, "pass too large");
+    if (nt == 0) return FGL_OK;
+
+    if (dd.size() != c->last_draws.size() ||
+        std::memcmp(dd.data(), c->last_draws.data(), dd.size() * sizeof(DevDraw)) != 0 ||
+        tri_base != c->last_tri_base || vert_base != c->last_vert_base) {
+        if ((rc = ensure(c, c->d_draws, dd.size() * sizeof(DevDraw))) ||
+            (rc = ensure(c, c->d_tri_base, tri_base.size() * 4)) ||
+            (rc = ensure(c, c->d_vert_base, vert_base.size() * 4)))
+            return rc;
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        HIPCHK(c, hipMemcpy(c->d_draws.p, dd.data(), dd.size() * sizeof(DevDraw), hipMemcpyHostToDevice));
+        HIPCHK(c, hipMemcpy(c->d_tri_base.p, tri_base.data(), tri_base.size() * 4, hipMemcpyHostToDevice));
+        HIPCHK(c, hipMemcpy(c->d_vert_base.p, vert_base.data(), vert_base.size() * 4, hipMemcpyHostToDevice));
+        c->last_draws = dd;
+        c->last_tri_base = tri_base;
+        c->last_vert_base = vert_base;
+    }
+
+    FrameUniforms u;
+    build_uniforms(c, u);
+    const int TS = c->TS;
+    const int tiles_x = (W + TS - 1) / TS, tiles_y = (H + TS - 1) / TS;
+    const uint32_t ntiles = (uint32_t)(tiles_x * tiles_y);
+    if ((rc = ensure(c, c->screen, (size_t)nv * sizeof(int4))) ||
+        (rc = ensure(c, c->rect, (size_t)nt * sizeof(uint2))) ||
+        (rc = ensure(c, c->counts, (size_t)nt * 4)) || (rc = ensure(c, c->offs, ((size_t)nt + 1) * 4)) ||
+        (rc = ensure(c, c->scan_tmp, scan_tmp_words((size_t)nt) * 4)) ||
+        (rc = ensure(c, c->tile_start, (size_t)ntiles * 4)) ||
+        (rc = ensure(c, c->tile_end, (size_t)ntiles * 4)))
+        return rc;
+    const DevDraw *ddraws = (const DevDraw *)c->d_draws.p;
+    {
+        KernelTimer t(c, "vertex");
+        launch_vertex(shadow, ddraws, (const int *)c->d_vert_base.p, n_draws, (int)nv, u,
+                      (int4 *)c->screen.p, c->stream);
+    }
+    {
+        KernelTimer t(c, "setup");
+        launch_setup(ddraws, (const int *)c->d_tri_base.p, n_draws, (const int4 *)c->screen.p,
+                     (int)nt, W, H, TS, (uint2 *)c->rect.p, (uint32_t *)c->counts.p, c->stream);
+    }
+    {
+        KernelTimer t(c, "scan");
+        launch_exclusive_scan((const uint32_t *)c->counts.p, (uint32_t *)c->offs.p, (size_t)nt,
+                              (uint32_t *)c->scan_tmp.p, c->stream);
+    }
+    HIPCHK(c, hipMemcpyAsync(c->h_total, (uint32_t *)c->offs.p + nt, 4, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    const uint32_t npairs = *c->h_total;
+    if ((rc = ensure(c, c->keys, (size_t)npairs * 4 + 4)) || (rc = ensure(c, c->vals, (size_t)npairs * 4 + 4)) ||
+        (rc = ensure(c, c->keys_alt, (size_t)npairs * 4 + 4)) ||
+        (rc = ensure(c, c->vals_alt, (size_t)npairs * 4 + 4)) ||
+        (rc = ensure(c, c->sort_tmp, radix_tmp_words(npairs) * 4)))
+        return rc;
+    {
+        KernelTimer t(c, "emit");
+        launch_emit((const uint32_t *)c->offs.p, (int)nt, (const uint2 *)c->rect.p, tiles_x, npairs,
+                    (uint32_t *)c->keys.p, (uint32_t *)c->vals.p, c->stream);
+    }
+    {
+        KernelTimer t(c, "sort");
+        launch_radix_sort((uint32_t *)c->keys.p, (uint32_t *)c->vals.p, (uint32_t *)c->keys_alt.p,
+                          (uint32_t *)c->vals_alt.p, npairs, bits_for(ntiles - 1),
+                          (uint32_t *)c->sort_tmp.p, c->stream);
+    }
+    {
+        KernelTimer t(c, "tile_ranges");
+        launch_fill_u32((uint32_t *)c->tile_start.p, 0, ntiles, c->stream);
+        launch_fill_u32((uint32_t *)c->tile_end.p, 0, ntiles, c->stream);
+        launch_tile_ranges((const uint32_t *)c->keys.p, npairs, (uint32_t *)c->tile_start.p,
+                           (uint32_t *)c->tile_end.p, c->stream);
+    }
+    RasterArgs a;
+    std::memset(&a, 0, sizeof(a));
+    a.draws = ddraws;
+    a.tri_base = (const int *)c->d_tri_base.p;
+    a.n_draws = n_draws;
+    a.screen = (const int4 *)c->screen.p;
+    a.vals = (const uint32_t *)c->vals.p;
+    a.tile_start = (const uint32_t *)c->tile_start.p;
+    a.tile_end = (const uint32_t *)c->tile_end.p;
+    a.W = W;
+    a.H = H;
+    a.tiles_x = tiles_x;
+    a.tiles_y = tiles_y;
+    a.depth = shadow ? (float *)c->shadow_depth.p : (float *)c->depth.p;
+    a.out0 = shadow ? (float *)c->shadow.p : (float *)c->fb.p;
+    a.gbuf = (float *)c->gbuf.p;
+    a.materials = (const DevMaterial *)c->d_materials.p;
+    a.textures = (const DevTexture *)c->d_textures.p;
+    a.shadow_map = (const float *)c->shadow.p;
+    a.u = u;
+    {
+        KernelTimer t(c, shadow ? "raster_shadow" : pass == kPassGeometry ? "raster_gbuffer" : "raster_forward");
+        launch_raster(pass, TS, a, c->stream);
+    }
+    HIPCHK(c, hipGetLastError());
+    return FGL_OK;
+}
+
+int fgl_end_pass(fgl_ctx *c)
+{
+    if (!c) return FGL_ERR_INVALID;
+    return flush_pass(c);
+}
+
+int fgl_draw_screen_space_pixels(fgl_ctx *c)
+{
+    if (!c) return FGL_ERR_INVALID;
+    int rc = flush_pass(c);
+    if (rc) return rc;
+    if (!c->gbuf.p) return fail(c, FGL_ERR_INVALID, "targets not initialised");
+    if (c->shadow_on && !c->shadow.p) return fail(c, FGL_ERR_INVALID, "shadows on but no shadow buffer");
+    FrameUniforms u;
+    build_uniforms(c, u);
+    KernelTimer t(c, "lighting");
+    launch_lighting(u, (const float *)c->gbuf.p, (const float *)c->ao.p, (const float *)c->shadow.p,
+                    (float *)c->fb.p, c->stream);
+    HIPCHK(c, hipGetLastError());
+    return FGL_OK;
+}
+
+int fgl_ssao(fgl_ctx *c, const fgl_ssao_cfg *cfg)
+{
+    if (!c || !cfg) return FGL_ERR_INVALID;
+    if (cfg->n_kernel < 1 || cfg->n_kernel > 64 || cfg->blur < 0 || cfg->blur > 2 ||
+        (cfg->blur == 2 && (cfg->blur_radius < 1 || cfg->blur_radius > 8)))
+        return fail(c, FGL_ERR_INVALID, "bad SSAO configuration");
+    int rc = flush_pass(c);
+    if (rc) return rc;
+    if (!c->gbuf.p) return fail(c, FGL_ERR_INVALID, "targets not initialised");
+    FrameUniforms u;
+    build_uniforms(c, u);
+    SsaoArgs sa;
+    sa.n_kernel = cfg->n_kernel;
+    std::memcpy(sa.kernel, cfg->kernel, sizeof(sa.kernel));
+    std::memcpy(sa.noise, cfg->noise, sizeof(sa.noise));
+    sa.radius = cfg->radius;
+    sa.bias = cfg->bias;
+    sa.range = cfg->range;
+    {
+        KernelTimer t(c, "ssao");
+        launch_ssao(u, sa, (const float *)c->depth.p, (const float *)c->gbuf.p, (float *)c->ao.p, c->stream);
+    }
+    if (cfg->blur == 1) {
+        KernelTimer t(c, "blur_reference");
+        launch_blur_recursive((float *)c->ao.p, c->W, c->H, c->stream);
+    } else if (cfg->blur == 2) {
+        KernelTimer t(c, "blur_gaussian");
+        launch_blur_separable((float *)c->ao.p, (float *)c->blur_tmp.p, c->W, c->H,
+                              cfg->blur_weights, cfg->blur_radius, c->stream);
+    }
+    HIPCHK(c, hipGetLastError());
+    return FGL_OK;
+}
+
+int fgl_resolve(fgl_ctx *c, int k)
+{
+    if (!c) return FGL_ERR_INVALID;
+    int rc = flush_pass(c);
+    if (rc) return rc;
+    if (!c->fb.p) return fail(c, FGL_ERR_INVALID, "targets not initialised");
+    if (k < 1 || k > 16 || c->W / k <= 0 || c->H / k <= 0) return fail(c, FGL_ERR_INVALID, "bad SSAA factor");
+    int ow = c->W / k, oh = c->H / k;
+    if ((rc = ensure(c, c->img, (size_t)ow * oh * 3))) return rc;
+    c->img_w = ow;
+    c->img_h = oh;
+    KernelTimer t(c, "resolve");
+    launch_resolve((const float *)c->fb.p, c->W, c->H, k, (uint8_t *)c->img.p, c->stream);
+    HIPCHK(c, hipGetLastError());
+    return FGL_OK;
+}
+
+// ---------------- readback ----------------
+
+static int target_ptr(fgl_ctx *c, int which, void **p, size_t *bytes)
+{
+    size_t P = (size_t)c->W * c->H, SP = (size_t)c->SW * c->SH;
+    switch (which) {
+    case FGL_TARGET_FRAME: *p = c->fb.p; *bytes = 3 * P * 4; break;
+    case FGL_TARGET_DEPTH: *p = c->depth.p; *bytes = P * 4; break;
+    case FGL_TARGET_SHADOW: *p = c->shadow.p; *bytes = SP * 4; break;
+    case FGL_TARGET_SHADOW_DEPTH: *p = c->shadow_depth.p; *bytes = SP * 4; break;
+    case FGL_TARGET_GBUFFER: *p = c->gbuf.p; *bytes = kGbufPlanes * P * 4; break;
+    case FGL_TARGET_AO: *p = c->ao.p; *bytes = P * 4; break;
+    case FGL_TARGET_IMAGE: *p = c->img.p; *bytes = (size_t)c->img_w * c->img_h * 3; break;
+    default: return fail(c, FGL_ERR_INVALID, "unknown target");
+    }
+    if (!*p) return fail(c, FGL_ERR_INVALID, "target not initialised");
+    return FGL_OK;
+}
+
+int fgl_target_bytes(fgl_ctx *c, int which, size_t *out)
+{
+    if (!c || !out) return FGL_ERR_INVALID;
+    void *p;
+    return target_ptr(c, which, &p, out);
+}
+
+int fgl_read_target(fgl_ctx *c, int which, void *dst, size_t bytes)
+{
+    if (!c || !dst) return FGL_ERR_INVALID;
+    int rc = flush_pass(c);
+    if (rc) return rc;
+    void *p;
+    size_t n;
+    if ((rc = target_ptr(c, which, &p, &n))) return rc;
+    if (bytes != n) return fail(c, FGL_ERR_INVALID, "size mismatch: target has " + std::to_string(n) + " bytes");
+    HIPCHK(c, hipMemcpyAsync(dst, p, n, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return FGL_OK;
+}
+
+int fgl_write_target(fgl_ctx *c, int which, const void *src, size_t bytes)
+{
+    if (!c || !src) return FGL_ERR_INVALID;
+    if (which == FGL_TARGET_IMAGE) return fail(c, FGL_ERR_INVALID, "image target is read-only");
+    int rc = flush_pass(c);
+    if (rc) return rc;
+    void *p;
+    size_t n;
+    if ((rc = target_ptr(c, which, &p, &n))) return rc;
+    if (bytes != n) return fail(c, FGL_ERR_INVALID, "size mismatch: target has " + std::to_string(n) + " bytes");
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    HIPCHK(c, hipMemcpy(p, src, n, hipMemcpyHostToDevice));
+    return FGL_OK;
+}
+
+int fgl_enable_timing(fgl_ctx *c, int on)
+{
+    if (!c) return FGL_ERR_INVALID;
+    c->timing = on != 0;
+    return FGL_OK;
+}
+
+int fgl_read_timings(fgl_ctx *c, char (*names)[32], float *ms, int cap, int *out_n)
+{
+    if (!c || !out_n) return FGL_ERR_INVALID;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    int n = 0;
+    for (auto &t : c->timed) {
+        if (n < cap && names && ms) {
+            float v = 0.f;
+            (void)hipEventElapsedTime(&v, t.a, t.b);
+            std::snprintf(names[n], 32, "%s", t.name.c_str());
+            ms[n] = v;
+        }
+        ++n;
+        (void)hipEventDestroy(t.a);
+        (void)hipEventDestroy(t.b);
+    }
+    c->timed.clear();
+    *out_n = n;
+    return FGL_OK;
+}
+
+// ---------------- scenes ----------------
+
+struct fgl_scene {
+    HostScene s;
+};
+
+int fgl_scene_load(const char *path, fgl_scene **out, char *err, size_t err_len)
+{
+    if (!path || !out) return FGL_ERR_INVALID;
+    *out = nullptr;
+    fgl_scene *sc = new fgl_scene();
+    std::string e;
+    if (!load_scene(path, sc->s, e)) {
+        if (err && err_len) std::snprintf(err, err_len, "%s", e.c_str());
+        delete sc;
+        return FGL_ERR_IO;
+    }
+    *out = sc;
+    return FGL_OK;
+}
+
+void fgl_scene_free(fgl_scene *s) { delete s; }
+
+int fgl_scene_info_get(const fgl_scene *s, fgl_scene_info *out)
+{
+    if (!s || !out) return FGL_ERR_INVALID;
+    *out = s->s.info;
+    return FGL_OK;
+}
+
+int fgl_scene_model(const fgl_scene *s, int i, fgl_model_desc *d, float mm[16])
+{
+    if (!s || !d || i < 0 || i >= (int)s->s.models.size()) return FGL_ERR_INVALID;
+    const HostModel &m = *s->s.models[i];
+    d->n_pos = (int)m.pos.size() / 3;
+    d->n_uv = (int)m.uv.size() / 2;
+    d->n_nrm = (int)m.nrm.size() / 3;
+    d->n_faces = m.n_faces();
+    d->pos = m.pos.data();
+    d->uv = m.uv.data();
+    d->nrm = m.nrm.data();
+    d->tan = m.has_tangents ? m.tan.data() : nullptr;
+    d->vi = m.vi.data();
+    d->ti = m.ti.data();
+    d->ni = m.ni.data();
+    d->face_material = m.face_mat.data();
+    d->support_pbr = m.support_pbr;
+    if (mm) std::memcpy(mm, s->s.model_mats[i].data(), 16 * sizeof(float));
+    return FGL_OK;
+}
+
+int fgl_scene_material(const fgl_scene *s, int i, fgl_material *out)
+{
+    if (!s || !out || i < 0 || i >= (int)s->s.materials.size()) return FGL_ERR_INVALID;
+    *out = s->s.materials[i];
+    return FGL_OK;
+}
+
+int fgl_scene_texture(const fgl_scene *s, int i, int *w, int *h, int *bpp, int *wrap, int *filter,
+                      const uint8_t **bytes)
+{
+    if (!s || i < 0 || i >= (int)s->s.textures.size()) return FGL_ERR_INVALID;
+    const HostTexture &t = s->s.textures[i];
+    if (w) *w = t.w;
+    if (h) *h = t.h;
+    if (bpp) *bpp = t.bpp;
+    if (wrap) *wrap = t.wrap;
+    if (filter) *filter = t.filter;
+    if (bytes) *bytes = t.bytes.data();
+    return FGL_OK;
+}
+
+static int upload_scene(fgl_ctx *c, fgl_scene *sc)
+{
+    HostScene &s = sc->s;
+    if (s.uploaded_ctx == c) return FGL_OK;
+    int rc;
+    std::vector<int> tex_id(s.textures.size()), mat_id(s.materials.size());
+    for (size_t i = 0; i < s.textures.size(); ++i) {
+        const HostTexture &t = s.textures[i];
+        if ((rc = fgl_upload_texture(c, t.w, t.h, t.bpp, t.bytes.data(), t.wrap, t.filter, &tex_id[i])))
+            return rc;
+    }
+    for (size_t i = 0; i < s.materials.size(); ++i) {
+        fgl_material m = s.materials[i];
+        int32_t *maps[] = {&m.map_kd, &m.map_ks, &m.map_ke, &m.map_bump, &m.map_base,
+                           &m.map_rough, &m.map_metal, &m.map_ao, &m.map_pbr_ke};
+        for (int32_t *p : maps)
+            if (*p >= 0) *p = tex_id[*p];
+        if ((rc = fgl_upload_material(c, &m, &mat_id[i]))) return rc;
+    }
+    s.dev_models.clear();
+    std::vector<std::pair<const HostModel *, int>> done;
+    for (size_t i = 0; i < s.models.size(); ++i) {
+        const HostModel *hm = s.models[i].get();
+        int id = -1;
+        for (auto &p : done)
+            if (p.first == hm) id = p.second;
+        if (id < 0) {
+            std::vector<int32_t> fm(hm->face_mat.size());
+            for (size_t f = 0; f < fm.size(); ++f) fm[f] = mat_id[hm->face_mat[f]];
+            fgl_model_desc d;
+            fgl_scene_model(sc, (int)i, &d, nullptr);
+            d.face_material = fm.data();
+            if ((rc = fgl_upload_model(c, &d, &id))) return rc;
+            done.push_back({hm, id});
+        }
+        s.dev_models.push_back(id);
+    }
+    s.uploaded_ctx = c;
+    return FGL_OK;
+}
+
+// Render::Render (src/render.cpp:40-58) with the scene's extension settings.
+int fgl_render_scene(fgl_ctx *c, fgl_scene *sc)
+{
+    if (!c || !sc) return FGL_ERR_INVALID;
+    int rc;
+    if ((rc = upload_scene(c, sc))) return rc;
+    const fgl_scene_info &I = sc->s.info;
+    const int k = I.ssaa_on ? I.ssaa_k : 1;
+    if (k < 1) return fail(c, FGL_ERR_INVALID, "bad SSAA factor");
+    const int W = I.width * k, H = I.height * k;
+    const float ratio = (float)I.width / I.height;
+    // Preconfigure (render.cpp:33-38)
+    fgl_set_viewport(c, 0, 0, 0, W, H);
+    const int S = I.shadow_map_size > 0 ? I.shadow_map_size : 0;
+    fgl_set_viewport(c, 1, 0, 0, S ? S : W, S ? S : H);
+    fgl_set_lights(c, I.n_lights, I.lights);
+    fgl_set_eye(c, I.eye);
+    if ((rc = fgl_set_shadow(c, I.shadow_on, &I.shadow))) return rc;
+    // DoShadowPass (render.cpp:60-97)
+    if (I.shadow_on) {
+        if ((rc = fgl_init_shadow_buffer(c, S ? S : W, S ? S : H))) return rc;
+        float V[16], P[16], L[16];
+        const float origin[3] = {0.f, 0.f, 0.f}, up[3] = {0.f, 1.f, 0.f};
+        make_lookat(I.lights[0].vec, origin, up, V);
+        make_ortho(-3.0f * ratio, 3.0f * ratio, -3.0f, 3.0f, 0.1f, 20.f, P);
+        mat_mul(P, V, L);
+        fgl_set_matrix(c, FGL_MAT_LIGHT_SPACE, L);
+        fgl_begin_pass(c, FGL_PASS_SHADOW);
+        for (size_t i = 0; i < sc->s.models.size(); ++i)
+            if ((rc = fgl_draw_model(c, sc->s.dev_models[i], sc->s.model_mats[i].data()))) return rc;
+        if ((rc = fgl_end_pass(c))) return rc;
+    }
+    // camera (render.cpp:106-115)
+    float V[16], P[16];
+    const float up[3] = {0.f, 1.f, 0.f};
+    make_lookat(I.eye, I.look_at, up, V);
+    if (I.camera_ortho) make_ortho(-1.f * ratio, 1.f * ratio, -1.f, 1.f, 0.01f, 20.f, P);
+    else make_perspective(45.f, ratio, 0.01f, 20.f, P);
+    fgl_set_matrix(c, FGL_MAT_VIEW, V);
+    fgl_set_matrix(c, FGL_MAT_PROJ, P);
+    if ((rc = fgl_init_targets(c, W, H))) return rc;
+    if (!I.deferred) {
+        // DoForwardPass (render.cpp:99-157)
+        if ((rc = fgl_clear_color(c, 0.12f, 0.12f, 0.12f))) return rc;
+        fgl_begin_pass(c, FGL_PASS_FORWARD);
+        for (size_t i = 0; i < sc->s.models.size(); ++i)
+            if ((rc = fgl_draw_model(c, sc->s.dev_models[i], sc->s.model_mats[i].data()))) return rc;
+        if ((rc = fgl_end_pass(c))) return rc;
+    } else {
+        // DoGeometryPass + DoLightingPass (render.cpp:159-212)
+        fgl_begin_pass(c, FGL_PASS_GEOMETRY);
+        for (size_t i = 0; i < sc->s.models.size(); ++i)
+            if ((rc = fgl_draw_model(c, sc->s.dev_models[i], sc->s.model_mats[i].data()))) return rc;
+        if ((rc = fgl_end_pass(c))) return rc;
+        fgl_begin_pass(c, FGL_PASS_LIGHTING);
+        if (I.ssao_on && (rc = fgl_ssao(c, &I.ssao))) return rc;
+        if ((rc = fgl_draw_screen_space_pixels(c))) return rc;
+    }
+    // DoSSAA (render.cpp:291-343); k = 1 quantises only
+    return fgl_resolve(c, k);
+}
+
+// ---------------- host helpers ----------------
+
+void fgl_make_model_matrix(float tx, float ty, float tz, float r, float s, float out[16])
+{
+    make_model_matrix(tx, ty, tz, r, s, out);
+}
+void fgl_make_lookat(const float eye[3], const float center[3], const float up[3], float out[16])
+{
+    make_lookat(eye, center, up, out);
+}
+void fgl_make_perspective(float fov, float aspect, float n, float f, float out[16])
+{
+    make_perspective(fov, aspect, n, f, out);
+}
+void fgl_make_ortho(float l, float r, float b, float t, float n, float f, float out[16])
+{
+    make_ortho(l, r, b, t, n, f, out);
+}
+void fgl_mat_mul(const float a[16], const float b[16], float out[16]) { mat_mul(a, b, out); }
+
+int fgl_tga_write(const char *path, int w, int h, int bpp, const uint8_t *bytes, int rle)
+{
+    if (!path || !bytes || w <= 0 || h <= 0 || (bpp != 1 && bpp != 3 && bpp != 4)) return FGL_ERR_INVALID;
+    return tga_write(path, w, h, bpp, bytes, rle != 0) ? FGL_OK : FGL_ERR_IO;
+}
+
+}  // extern "C"

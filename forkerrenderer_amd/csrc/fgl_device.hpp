@@ -1,0 +1,413 @@
+// fgl_device.hpp -- device-resident scene data and the per-pixel programs of
+// the reference's shaders, written for gfx950.
+#pragma once
+
+#include "fgl_math.hpp"
+
+namespace fgl {
+
+constexpr int kMaxLights = 4;
+constexpr int kMaxDisk = 64;
+
+// Texture in HBM: one packed texel per uint32 (r | g<<8 | b<<16), row y at y*w.
+// TGAImage::Get's byte order (B,G,R; grey sets only b) is resolved at upload
+// (src/tgaimage.h:149-181).
+struct DevTexture {
+    const uint32_t *texels;
+    int w, h, wrap, filter;
+};
+
+struct DevMaterial {
+    float ka[3], kd[3], ks[3], ke[3];
+    int map_kd, map_ks, map_ke, map_bump;
+    float pbr_ke[3], albedo[3], roughness, metalness;
+    int map_base, map_rough, map_metal, map_ao, map_pbr_ke;
+};
+
+// One draw of a pass: a model's SoA attribute arrays, face index streams and
+// uniforms.  vert_base/tri_base place it in the pass-wide vertex/triangle order.
+struct DevDraw {
+    const float *pos, *uv, *nrm, *tan;
+    const int *vi, *ti, *ni, *face_mat;
+    int n_pos, n_faces, vert_base, tri_base;
+    int has_tangents, support_pbr, pad0, pad1;
+    float model[16];
+    float normal[9];
+    float lm[16];  // light_space * model (DepthShader, src/shaders/depthshader.h:23-25)
+};
+
+struct DevLight {
+    int type;
+    float vec[3];
+    float color[3];
+};
+
+struct DevShadowCfg {
+    int mode, grid, n_blocker, n_pcf;
+    float pcf_filter, area_light;
+    double blocker_filter, pcss_filter;
+    float blocker_disk[2 * kMaxDisk];
+    float pcf_disk[2 * kMaxDisk];
+};
+
+struct FrameUniforms {
+    float view[16], proj[16], light_space[16], viewport[16], shadow_viewport[16];
+    float vp[16];  // proj * view (SetViewProjectionMatrix, src/render.cpp:115)
+    float eye[3];
+    int shadow_on;
+    int n_lights, W, H, SW, SH;
+    DevLight lights[kMaxLights];
+    DevShadowCfg shadow;
+};
+
+// ---------------- textures (src/texture.h:41-145) ----------------
+
+__device__ __forceinline__ Vec3 texel_rgb(const DevTexture &t, int x, int y)
+{
+    if (x < 0 || y < 0 || x >= t.w || y >= t.h) return vec3(0.f, 0.f, 0.f);
+    uint32_t p = t.texels[(size_t)x + (size_t)y * t.w];
+    return vec3((float)(p & 255u), (float)((p >> 8) & 255u), (float)((p >> 16) & 255u));
+}
+
+__device__ __forceinline__ Vec2 wrap_coord(const DevTexture &t, Vec2 c)
+{
+    if (t.wrap == 1) return vec2(c.x - floorf(c.x), c.y - floorf(c.y));
+    if (t.wrap == 2) {
+        int xi = trunc_int(floorf(c.x)), yi = trunc_int(floorf(c.y));
+        float rx = c.x - (float)xi, ry = c.y - (float)yi;
+        return vec2((xi % 2 == 0) ? rx : 1.f - rx, (yi % 2 == 0) ? ry : 1.f - ry);
+    }
+    if (t.wrap == 3) return vec2(clampf(c.x, 0.f, 1.f), clampf(c.y, 0.f, 1.f));
+    return c;
+}
+
+__device__ __forceinline__ Vec3 filter_color(const DevTexture &t, Vec2 c)
+{
+    float w = (float)(t.w - 0.001), h = (float)(t.h - 0.001);
+    if (t.filter == 1) {
+        float px = c.x * w, py = c.y * h;
+        float tlx = floorf(px - 0.5f), tly = floorf(py - 0.5f);
+        float tx = px - (tlx + 0.5f);
+        float ty = py - (tly + 0.5f);
+        int s0x = trunc_int(tlx), s0y = trunc_int(tly);
+        int s1x = trunc_int(tlx + 1.f), s2y = trunc_int(tly + 1.f);
+        int s1y = s0y, s2x = s0x, s3x = s1x, s3y = s2y;
+        if (t.wrap != 0) {
+            s0x = clampi(s0x, 0, t.w - 1); s0y = clampi(s0y, 0, t.h - 1);
+            s1x = clampi(s1x, 0, t.w - 1); s1y = clampi(s1y, 0, t.h - 1);
+            s2x = clampi(s2x, 0, t.w - 1); s2y = clampi(s2y, 0, t.h - 1);
+            s3x = clampi(s3x, 0, t.w - 1); s3y = clampi(s3y, 0, t.h - 1);
+        }
+        Vec3 c0 = texel_rgb(t, s0x, s0y), c1 = texel_rgb(t, s1x, s1y);
+        Vec3 c2 = texel_rgb(t, s2x, s2y), c3 = texel_rgb(t, s3x, s3y);
+        Vec3 cx1 = lerp3(tx, c0, c1), cx2 = lerp3(tx, c2, c3);
+        return lerp3(ty, cx1, cx2);
+    }
+    return texel_rgb(t, trunc_int(floorf(c.x * w)), trunc_int(floorf(c.y * h)));
+}
+
+// Texture::Sample / SampleFloat (src/texture.h:41-51)
+__device__ __forceinline__ Vec3 tex_sample(const DevTexture *texs, int id, Vec2 uv)
+{
+    const DevTexture &t = texs[id];
+    return divs3(filter_color(t, wrap_coord(t, uv)), 255.f);
+}
+__device__ __forceinline__ float tex_samplef(const DevTexture *texs, int id, Vec2 uv)
+{
+    const DevTexture &t = texs[id];
+    return filter_color(t, wrap_coord(t, uv)).z / 255.f;
+}
+
+// ---------------- vertex programs ----------------
+
+// Varyings of one vertex after ProcessVertex with perspective-correct
+// interpolation enabled (src/shaders/gshader.h:41-92).
+struct Varying {
+    Vec3 ws, nws, tws, lndc;
+    Vec2 uv;
+    float oow;
+};
+
+__device__ __forceinline__ Varying main_vertex(const DevDraw &d, const FrameUniforms &u, int vi,
+                                               int ti, int ni)
+{
+    Varying o;
+    Vec4 p = vec4(d.pos[3 * vi], d.pos[3 * vi + 1], d.pos[3 * vi + 2], 1.f);
+    Vec4 ws = mv4(d.model, p);
+    Vec4 vs = mv4(u.view, ws);
+    Vec4 cs = mv4(u.proj, vs);
+    Vec2 uv = vec2(d.uv[2 * ti], d.uv[2 * ti + 1]);
+    Vec3 n = norm3(vec3(d.nrm[3 * ni], d.nrm[3 * ni + 1], d.nrm[3 * ni + 2]));
+    Vec3 nws = mv3(d.normal, n);
+    Vec3 tws = vec3(0.f, 0.f, 0.f);
+    if (d.has_tangents) {
+        Vec3 t = norm3(vec3(d.tan[3 * vi], d.tan[3 * vi + 1], d.tan[3 * vi + 2]));
+        tws = mv3(d.normal, t);
+    }
+    Vec4 l = vec4(0.f, 0.f, 0.f, 0.f);
+    if (u.shadow_on) {
+        l = mv4(u.light_space, ws);
+        l = divs4(l, l.w);
+    }
+    float oow = 1.f / cs.w;
+    ws = scale4(ws, oow);
+    uv.x *= oow;
+    uv.y *= oow;
+    nws = scale3(nws, oow);
+    if (d.has_tangents) tws = scale3(tws, oow);
+    if (u.shadow_on) l = scale4(l, oow);
+    o.ws = vec3(ws.x, ws.y, ws.z);
+    o.uv = uv;
+    o.nws = nws;
+    o.tws = tws;
+    o.lndc = vec3(l.x, l.y, l.z);
+    o.oow = oow;
+    return o;
+}
+
+// Clip-space -> NDC of the main shaders: positionCS / positionCS.w.
+__device__ __forceinline__ Vec4 main_ndc(const DevDraw &d, const FrameUniforms &u, int vi)
+{
+    Vec4 p = vec4(d.pos[3 * vi], d.pos[3 * vi + 1], d.pos[3 * vi + 2], 1.f);
+    Vec4 cs = mv4(u.proj, mv4(u.view, mv4(d.model, p)));
+    return divs4(cs, cs.w);
+}
+
+// DepthShader::ProcessVertex (src/shaders/depthshader.h:21-28)
+__device__ __forceinline__ Vec4 depth_ndc(const DevDraw &d, int vi)
+{
+    Vec4 p = vec4(d.pos[3 * vi], d.pos[3 * vi + 1], d.pos[3 * vi + 2], 1.f);
+    Vec4 cs = mv4(d.lm, p);
+    return divs4(cs, cs.w);
+}
+
+// ---------------- fragment geometry (gshader.h:95-141) ----------------
+
+struct FragGeom {
+    Vec3 pos, normal, lndc;
+    Vec2 uv;
+};
+
+__device__ __forceinline__ FragGeom frag_geometry(const DevDraw &d, const FrameUniforms &u,
+                                                  const DevTexture *texs, const Varying &v0,
+                                                  const Varying &v1, const Varying &v2, Vec3 b,
+                                                  int normal_map)
+{
+    FragGeom g;
+    Vec3 pws = interp3(v0.ws, v1.ws, v2.ws, b);
+    Vec2 uv = interp2(v0.uv, v1.uv, v2.uv, b);
+    Vec3 nws = interp3(v0.nws, v1.nws, v2.nws, b);
+    float w = 1.f / dot3(vec3(v0.oow, v1.oow, v2.oow), b);
+    pws = scale3(pws, w);
+    uv.x *= w;
+    uv.y *= w;
+    nws = scale3(nws, w);
+    Vec3 N = norm3(nws);
+    Vec3 normal = N;
+    if (d.has_tangents && normal_map >= 0) {
+        Vec3 tws = scale3(interp3(v0.tws, v1.tws, v2.tws, b), w);
+        Vec3 T = norm3(add3(tws, vec3(0.001f, 0.001f, 0.001f)));
+        T = norm3(sub3(T, scale3(N, dot3(T, N))));
+        Vec3 B = norm3(cross3(N, T));
+        Vec3 s = tex_sample(texs, normal_map, uv);
+        s = norm3(sub3(scale3(s, 2.f), vec3(1.f, 1.f, 1.f)));
+        normal = norm3(vec3(dot3(vec3(T.x, B.x, N.x), s), dot3(vec3(T.y, B.y, N.y), s),
+                            dot3(vec3(T.z, B.z, N.z), s)));
+    }
+    g.pos = pws;
+    g.normal = normal;
+    g.uv = uv;
+    g.lndc = vec3(0.f, 0.f, 0.f);
+    if (u.shadow_on) g.lndc = scale3(interp3(v0.lndc, v1.lndc, v2.lndc, b), w);
+    return g;
+}
+
+// ---------------- shadows (src/shaders/shadow.cpp:23-131) ----------------
+
+__device__ __forceinline__ float sample_shadow(const float *map, int SW, int SH, float u, float v)
+{
+    // A NaN coordinate is treated as out of the map (the reference would
+    // index with INT_MIN).
+    if (!(u >= 0.f && u <= 1.f && v >= 0.f && v <= 1.f)) return __builtin_inff();
+    int w = (int)((float)SW - 0.001f);
+    int h = (int)((float)SH - 0.001f);
+    int x = (int)((float)w * u);
+    int y = (int)((float)h * v);
+    float dd = map[(size_t)x + (size_t)y * SW];
+    return ((double)dd < 0.001) ? 1.f : dd;
+}
+
+__device__ __forceinline__ float pcf_disk(const float *map, int SW, int SH, Vec3 sc, float bias,
+                                          float filter, const float *disk, int n)
+{
+    float vis = 0.f;
+    float inv = 1.f / (float)n;
+    for (int i = 0; i < n; ++i) {
+        float s = sample_shadow(map, SW, SH, sc.x + disk[2 * i] * filter,
+                                sc.y + disk[2 * i + 1] * filter);
+        if (sc.z <= s + bias) vis += inv;
+    }
+    return vis;
+}
+
+__device__ __forceinline__ float shadow_visibility(const FrameUniforms &u, const float *map,
+                                                   Vec3 ndc, Vec3 n, Vec3 l)
+{
+    const DevShadowCfg &c = u.shadow;
+    Vec3 sc = add3(scale3(ndc, 0.5f), vec3(0.5f, 0.5f, 0.5f));
+    float bias = smax(0.009f * (1.f - dot3(n, l)), 0.007f);
+    if (c.mode == 1) return pcf_disk(map, u.SW, u.SH, sc, bias, c.pcf_filter, c.pcf_disk, c.n_pcf);
+    if (c.mode == 2) {
+        float blocker = 0.f, nb = 0.f;
+        for (int i = 0; i < c.n_blocker; ++i) {
+            float su = sc.x + (float)((double)c.blocker_disk[2 * i] * c.blocker_filter);
+            float sv = sc.y + (float)((double)c.blocker_disk[2 * i + 1] * c.blocker_filter);
+            float s = sample_shadow(map, u.SW, u.SH, su, sv);
+            if (sc.z > s + bias) {
+                blocker += s;
+                nb += 1.f;
+            }
+        }
+        float dB = (nb < 1.f) ? 0.f : blocker / nb;
+        if ((double)dB < 0.001) return 1.f;
+        float pen = (sc.z - dB) * c.area_light / dB;
+        float filter = (float)(c.pcss_filter * (double)pen);
+        return pcf_disk(map, u.SW, u.SH, sc, bias, filter, c.pcf_disk, c.n_pcf);
+    }
+    if (c.mode == 3) {
+        int r = c.grid / 2;
+        float inv = 1.f / (float)(c.grid * c.grid);
+        float tu = 1.f / (float)u.SW, tv = 1.f / (float)u.SH;
+        float vis = 0.f;
+        for (int dy = -r; dy <= r; ++dy)
+            for (int dx = -r; dx <= r; ++dx) {
+                float s = sample_shadow(map, u.SW, u.SH, sc.x + (float)dx * tu, sc.y + (float)dy * tv);
+                if (sc.z <= s + bias) vis += inv;
+            }
+        return vis;
+    }
+    float s = sample_shadow(map, u.SW, u.SH, sc.x, sc.y);
+    return (sc.z <= s + bias) ? 1.f : 0.f;
+}
+
+// ---------------- lighting (phongshader.h:171-215, pbrshader.h:182-288) ----------------
+
+__device__ __forceinline__ Vec3 pow3f(Vec3 a, float p)
+{
+    return vec3(powf(a.x, p), powf(a.y, p), powf(a.z, p));
+}
+
+__device__ __forceinline__ Vec3 tonemap_gamma(Vec3 c)
+{
+    c = vec3(c.x / (c.x + 1.f), c.y / (c.y + 1.f), c.z / (c.z + 1.f));
+    return clamp01_3(pow3f(c, 1.f / 2.2f));
+}
+
+__device__ __forceinline__ Vec3 bp_term(Vec3 L, Vec3 H, Vec3 N, float vis, Vec3 dl, Vec3 el,
+                                        float ao, float ks, float shin, Vec3 lc, int shadow_on,
+                                        bool with_emissive)
+{
+    float diff = smax(0.f, dot3(L, N));
+    float spec = powf(smax(0.f, dot3(H, N)), shin);
+    Vec3 diffuse = scale3(scale3(dl, diff), ao);
+    Vec3 specular = scale3(vec3(ks, ks, ks), spec);
+    if (shadow_on) {
+        float sh = (1 - vis) * 0.6f;
+        float vv = 1 - sh;
+        diffuse = scale3(diffuse, vv);
+        specular = scale3(specular, vv);
+    }
+    Vec3 t = add3(diffuse, specular);
+    if (with_emissive) t = add3(t, el);
+    return mul3(t, lc);
+}
+
+__device__ __forceinline__ Vec3 pbr_lo(Vec3 L, Vec3 V, Vec3 H, Vec3 N, float vis, Vec3 al,
+                                       float metal, float rough, Vec3 rad, int shadow_on)
+{
+    const float InvPi = (float)0.31830988618379067154;
+    float NdotV = smax(dot3(N, V), 0.f);
+    float NdotL = smax(dot3(N, L), 0.f);
+    float NdotH = smax(dot3(N, H), 0.f);
+    float HdotV = smax(dot3(H, V), 0.f);
+    Vec3 F0 = lerp3(metal, vec3(0.04f, 0.04f, 0.04f), al);
+    float a = rough * rough;
+    float a2 = a * a;
+    float NdotH2 = NdotH * NdotH;
+    float den = (NdotH2 * (a2 - 1.f) + 1.f);
+    float den2 = den * den;
+    float NDF = a2 * InvPi / den2;
+    float ka = rough + 1.f;
+    float k = ka * ka / 8.f;
+    float g1 = NdotV / (NdotV * (1 - k) + k);
+    float g2 = NdotL / (NdotL * (1 - k) + k);
+    float G = g1 * g2;
+    float omh = smax(1.f - HdotV, 0.f);
+    float p5 = powf(omh, 5.f);
+    Vec3 F = add3(F0, scale3(sub3(vec3(1.f, 1.f, 1.f), F0), p5));
+    Vec3 DGF = scale3(F, NDF * G);
+    float denom = 4 * NdotV * NdotL + 0.001f;
+    Vec3 spec = divs3(DGF, denom);
+    Vec3 kd = scale3(sub3(vec3(1.f, 1.f, 1.f), F), 1.f - metal);
+    Vec3 brdf = add3(scale3(mul3(kd, al), InvPi), spec);
+    Vec3 Lo = scale3(mul3(brdf, rad), NdotL);
+    if (shadow_on) {
+        float sh = (1 - vis) * 0.6f;
+        float vv = 1 - sh;
+        Lo = scale3(Lo, vv);
+    }
+    return Lo;
+}
+
+__device__ __forceinline__ Vec3 light_dir(const DevLight &l, Vec3 P)
+{
+    if (l.type == 0) return norm3(sub3(vec3(l.vec[0], l.vec[1], l.vec[2]), P));
+    return norm3(vec3(-l.vec[0], -l.vec[1], -l.vec[2]));
+}
+
+// One surface point lit by every light.  With a single point light this is
+// the reference's CalculateLight; `deferred_quirk` passes viewDir in the
+// Blinn-Phong halfway slot as DrawScreenSpacePixels does (forkergl.cpp:369).
+__device__ __forceinline__ Vec3 shade_point(const FrameUniforms &u, const float *smap, bool pbr,
+                                            bool deferred_quirk, Vec3 P, Vec3 N, Vec3 lndc,
+                                            Vec3 albedo, Vec3 emissive, Vec3 param)
+{
+    const float gamma = (float)2.2;
+    Vec3 Vd = norm3(sub3(vec3(u.eye[0], u.eye[1], u.eye[2]), P));
+    if (!pbr) {
+        Vec3 dl = pow3f(albedo, gamma);
+        Vec3 el = pow3f(emissive, gamma);
+        Vec3 c = scale3(mul3(vec3(0.3f, 0.3f, 0.3f), dl), param.x);
+        for (int i = 0; i < u.n_lights; ++i) {
+            const DevLight &l = u.lights[i];
+            Vec3 L = light_dir(l, P);
+            float vis = 1.f;
+            if (!u.shadow_on) vis = 0.f;
+            else if (i == 0) vis = shadow_visibility(u, smap, lndc, N, L);
+            Vec3 H = deferred_quirk ? Vd : norm3(add3(L, Vd));
+            c = add3(c, bp_term(L, H, N, vis, dl, el, param.x, param.y, param.z,
+                                vec3(l.color[0], l.color[1], l.color[2]), u.shadow_on, i == 0));
+        }
+        return tonemap_gamma(c);
+    }
+    Vec3 al = pow3f(albedo, gamma);
+    Vec3 el = pow3f(emissive, gamma);
+    Vec3 Lo = vec3(0.f, 0.f, 0.f);
+    for (int i = 0; i < u.n_lights; ++i) {
+        const DevLight &l = u.lights[i];
+        Vec3 L = light_dir(l, P);
+        float vis = 1.f;
+        if (!u.shadow_on) vis = 0.f;
+        else if (i == 0) vis = shadow_visibility(u, smap, lndc, N, L);
+        Vec3 H = norm3(add3(L, Vd));
+        Vec3 lo = pbr_lo(L, Vd, H, N, vis, al, param.y, param.z,
+                         vec3(l.color[0], l.color[1], l.color[2]), u.shadow_on);
+        Lo = (i == 0) ? lo : add3(Lo, lo);
+    }
+    Vec3 color = Lo;
+    color = add3(color, scale3(mul3(vec3(0.3f, 0.3f, 0.3f), al), param.x));
+    color = add3(color, el);
+    return tonemap_gamma(color);
+}
+
+}  // namespace fgl

@@ -1,0 +1,73 @@
+// fgl_kernels.hpp -- launch interface between the host context and the
+// gfx950 kernels (raster.hip, sort.hip, shade.hip).
+#pragma once
+
+#include "fgl_device.hpp"
+
+namespace fgl {
+
+constexpr int kPassForward = 0;
+constexpr int kPassGeometry = 1;
+constexpr int kPassShadow = 3;
+constexpr int kGbufPlanes = 19;
+
+struct RasterArgs {
+    const DevDraw *draws;
+    const int *tri_base;
+    int n_draws;
+    const int4 *screen;
+    const uint32_t *vals;
+    const uint32_t *tile_start, *tile_end;
+    int W, H, tiles_x, tiles_y;
+    float *depth;
+    float *out0;   // shadow value plane (shadow pass) or 3 colour planes (forward)
+    float *gbuf;   // 19 planes (geometry pass)
+    const DevMaterial *materials;
+    const DevTexture *textures;
+    const float *shadow_map;
+    FrameUniforms u;
+};
+
+struct SsaoArgs {
+    int n_kernel;
+    float kernel[3 * 64];
+    float noise[3 * 16];
+    float radius, bias, range;
+};
+
+// raster.hip
+void launch_vertex(bool shadow, const DevDraw *draws, const int *vert_base, int n_draws,
+                   int n_total, const FrameUniforms &u, int4 *out, hipStream_t s);
+void launch_setup(const DevDraw *draws, const int *tri_base, int n_draws, const int4 *screen,
+                  int n_tri, int W, int H, int TS, uint2 *rect, uint32_t *count, hipStream_t s);
+void launch_emit(const uint32_t *off, int n_tri, const uint2 *rect, int tiles_x,
+                 uint32_t n_pairs, uint32_t *keys, uint32_t *vals, hipStream_t s);
+void launch_fill_u32(uint32_t *p, uint32_t v, size_t n, hipStream_t s);
+void launch_fill_f32(float *p, float v, size_t n, hipStream_t s);
+void launch_tile_ranges(const uint32_t *keys, uint32_t n, uint32_t *start, uint32_t *end,
+                        hipStream_t s);
+void launch_raster(int pass, int TS, const RasterArgs &a, hipStream_t s);
+
+// sort.hip
+// Exclusive scan of n uint32 values into out (out may alias in); the total
+// is written to out[n].  tmp needs scan_tmp_words(n) words.
+size_t scan_tmp_words(size_t n);
+void launch_exclusive_scan(const uint32_t *in, uint32_t *out, size_t n, uint32_t *tmp,
+                           hipStream_t s);
+// Stable LSD radix sort of (key, value) pairs by the low `bits` key bits.
+// Result ends in (keys, vals); alt buffers are scratch of the same size.
+size_t radix_tmp_words(size_t n);
+void launch_radix_sort(uint32_t *keys, uint32_t *vals, uint32_t *keys_alt, uint32_t *vals_alt,
+                       size_t n, int bits, uint32_t *tmp, hipStream_t s);
+
+// shade.hip
+void launch_lighting(const FrameUniforms &u, const float *gbuf, const float *ao,
+                     const float *shadow_map, float *fb, hipStream_t s);
+void launch_ssao(const FrameUniforms &u, const SsaoArgs &sa, const float *depth,
+                 const float *gbuf, float *ao, hipStream_t s);
+void launch_blur_recursive(float *ao, int W, int H, hipStream_t s);
+void launch_blur_separable(float *ao, float *tmp, int W, int H, const float *w, int R,
+                           hipStream_t s);
+void launch_resolve(const float *fb, int W, int H, int k, uint8_t *img, hipStream_t s);
+
+}  // namespace fgl

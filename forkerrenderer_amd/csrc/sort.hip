@@ -1,0 +1,232 @@
+// sort.hip -- device-wide exclusive scan and stable LSD radix sort for gfx950
+// (64-lane wavefronts).
+//
+// The binning step needs every screen tile's triangle list in submission
+// order.  Pairs (tile, triangle) are produced in triangle order; a stable
+// sort by tile therefore yields exactly that.  Each radix pass ranks the
+// items of a 2048-item block with wave ballots (eight ballots give the mask
+// of lanes holding the same 8-bit digit; popcount of the lower lanes is the
+// rank inside the wave) and LDS per-wave digit counts for the order across
+// waves and rounds -- no atomics decide any position.
+#include "fgl_kernels.hpp"
+
+namespace fgl {
+
+constexpr int kScanBlock = 256;
+constexpr int kScanItems = 8;
+constexpr int kScanTile = kScanBlock * kScanItems;  // 2048
+
+__device__ __forceinline__ uint32_t wave_inclusive_scan(uint32_t x, int lane)
+{
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        uint32_t y = __shfl_up(x, o, 64);
+        if (lane >= o) x += y;
+    }
+    return x;
+}
+
+// Exclusive scan of one value per thread over a 256-thread block.
+__device__ __forceinline__ uint32_t block_exclusive_scan(uint32_t v, uint32_t *lds4,
+                                                         uint32_t &total)
+{
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    uint32_t inc = wave_inclusive_scan(v, lane);
+    if (lane == 63) lds4[wave] = inc;
+    __syncthreads();
+    uint32_t prefix = 0, tot = 0;
+#pragma unroll
+    for (int w = 0; w < kScanBlock / 64; ++w) {
+        uint32_t c = lds4[w];
+        if (w < wave) prefix += c;
+        tot += c;
+    }
+    __syncthreads();
+    total = tot;
+    return prefix + inc - v;
+}
+
+__global__ __launch_bounds__(kScanBlock) void k_scan_reduce(const uint32_t *__restrict__ in,
+                                                            size_t n, uint32_t *__restrict__ part)
+{
+    __shared__ uint32_t lds4[4];
+    size_t base = (size_t)blockIdx.x * kScanTile + (size_t)threadIdx.x * kScanItems;
+    uint32_t s = 0;
+#pragma unroll
+    for (int i = 0; i < kScanItems; ++i)
+        if (base + i < n) s += in[base + i];
+    uint32_t total;
+    block_exclusive_scan(s, lds4, total);
+    if (threadIdx.x == 0) part[blockIdx.x] = total;
+}
+
+// Single workgroup: exclusive scan of the block partials, total to out[n].
+__global__ __launch_bounds__(kScanBlock) void k_scan_partials(uint32_t *__restrict__ part,
+                                                              int nparts,
+                                                              uint32_t *__restrict__ out_total)
+{
+    __shared__ uint32_t lds4[4];
+    uint32_t carry = 0;
+    for (int base = 0; base < nparts; base += kScanBlock) {
+        int i = base + threadIdx.x;
+        uint32_t v = i < nparts ? part[i] : 0u;
+        uint32_t total;
+        uint32_t ex = block_exclusive_scan(v, lds4, total);
+        if (i < nparts) part[i] = carry + ex;
+        carry += total;
+    }
+    if (threadIdx.x == 0) *out_total = carry;
+}
+
+__global__ __launch_bounds__(kScanBlock) void k_scan_down(const uint32_t *in, uint32_t *out,
+                                                          size_t n,
+                                                          const uint32_t *__restrict__ part)
+{
+    __shared__ uint32_t lds4[4];
+    size_t base = (size_t)blockIdx.x * kScanTile + (size_t)threadIdx.x * kScanItems;
+    uint32_t v[kScanItems];
+    uint32_t s = 0;
+#pragma unroll
+    for (int i = 0; i < kScanItems; ++i) {
+        v[i] = (base + i < n) ? in[base + i] : 0u;
+        s += v[i];
+    }
+    uint32_t total;
+    uint32_t ex = block_exclusive_scan(s, lds4, total) + part[blockIdx.x];
+#pragma unroll
+    for (int i = 0; i < kScanItems; ++i) {
+        if (base + i < n) out[base + i] = ex;
+        ex += v[i];
+    }
+}
+
+size_t scan_tmp_words(size_t n) { return (n + kScanTile - 1) / kScanTile + 1; }
+
+void launch_exclusive_scan(const uint32_t *in, uint32_t *out, size_t n, uint32_t *tmp,
+                           hipStream_t s)
+{
+    int nb = (int)((n + kScanTile - 1) / kScanTile);
+    if (n == 0) {
+        hipLaunchKernelGGL(k_scan_partials, dim3(1), dim3(kScanBlock), 0, s, tmp, 0, out + n);
+        return;
+    }
+    hipLaunchKernelGGL(k_scan_reduce, dim3(nb), dim3(kScanBlock), 0, s, in, n, tmp);
+    hipLaunchKernelGGL(k_scan_partials, dim3(1), dim3(kScanBlock), 0, s, tmp, nb, out + n);
+    hipLaunchKernelGGL(k_scan_down, dim3(nb), dim3(kScanBlock), 0, s, in, out, n, tmp);
+}
+
+// ---------------- radix sort ----------------
+
+constexpr int kRadixBlock = 256;
+constexpr int kRadixRounds = 8;
+constexpr int kRadixTile = kRadixBlock * kRadixRounds;  // 2048 items, striped by round
+
+__global__ __launch_bounds__(kRadixBlock) void k_radix_hist(const uint32_t *__restrict__ keys,
+                                                            uint32_t n, int shift, int nb,
+                                                            uint32_t *__restrict__ hist)
+{
+    __shared__ uint32_t h[256];
+    h[threadIdx.x] = 0;
+    __syncthreads();
+    const uint32_t base = blockIdx.x * kRadixTile;
+#pragma unroll
+    for (int r = 0; r < kRadixRounds; ++r) {
+        uint32_t idx = base + r * kRadixBlock + threadIdx.x;
+        if (idx < n) atomicAdd(&h[(keys[idx] >> shift) & 255u], 1u);
+    }
+    __syncthreads();
+    hist[(size_t)threadIdx.x * nb + blockIdx.x] = h[threadIdx.x];
+}
+
+__global__ __launch_bounds__(kRadixBlock) void k_radix_scatter(
+    const uint32_t *__restrict__ kin, const uint32_t *__restrict__ vin,
+    uint32_t *__restrict__ kout, uint32_t *__restrict__ vout, uint32_t n, int shift, int nb,
+    const uint32_t *__restrict__ hist)
+{
+    __shared__ uint32_t goff[256];
+    __shared__ uint32_t dbase[256];
+    __shared__ uint32_t wcnt[kRadixBlock / 64][256];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    goff[tid] = hist[(size_t)tid * nb + blockIdx.x];
+    dbase[tid] = 0;
+#pragma unroll
+    for (int w = 0; w < kRadixBlock / 64; ++w) wcnt[w][tid] = 0;
+    __syncthreads();
+    const unsigned long long lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    const uint32_t base = blockIdx.x * kRadixTile;
+    for (int r = 0; r < kRadixRounds; ++r) {
+        const uint32_t idx = base + r * kRadixBlock + tid;
+        const bool valid = idx < n;
+        const uint32_t k = valid ? kin[idx] : 0u;
+        const uint32_t v = valid ? vin[idx] : 0u;
+        const uint32_t d = (k >> shift) & 255u;
+        unsigned long long m = __ballot(valid);
+#pragma unroll
+        for (int bit = 0; bit < 8; ++bit) {
+            const bool on = (d >> bit) & 1u;
+            const unsigned long long bm = __ballot(on);
+            m &= on ? bm : ~bm;
+        }
+        const uint32_t rank = (uint32_t)__popcll(m & lt);
+        if (valid && rank == 0) wcnt[wave][d] = (uint32_t)__popcll(m);
+        __syncthreads();
+        if (valid) {
+            uint32_t pos = goff[d] + dbase[d] + rank;
+            for (int w = 0; w < wave; ++w) pos += wcnt[w][d];
+            kout[pos] = k;
+            vout[pos] = v;
+        }
+        __syncthreads();
+        uint32_t add = 0;
+#pragma unroll
+        for (int w = 0; w < kRadixBlock / 64; ++w) {
+            add += wcnt[w][tid];
+            wcnt[w][tid] = 0;
+        }
+        dbase[tid] += add;
+        __syncthreads();
+    }
+}
+
+size_t radix_tmp_words(size_t n)
+{
+    size_t nb = (n + kRadixTile - 1) / kRadixTile;
+    size_t h = 256 * nb;
+    return h + 1 + scan_tmp_words(h);
+}
+
+// Returns 0 when the sorted result is in (keys, vals), 1 when in the alt buffers.
+int launch_radix_sort_impl(uint32_t *keys, uint32_t *vals, uint32_t *keys_alt, uint32_t *vals_alt,
+                           size_t n, int bits, uint32_t *tmp, hipStream_t s)
+{
+    if (n <= 1 || bits <= 0) return 0;
+    int nb = (int)((n + kRadixTile - 1) / kRadixTile);
+    size_t h = 256 * (size_t)nb;
+    uint32_t *hist = tmp;
+    uint32_t *stmp = tmp + h + 1;
+    uint32_t *ki = keys, *vi = vals, *ko = keys_alt, *vo = vals_alt;
+    int passes = (bits + 7) / 8;
+    for (int p = 0; p < passes; ++p) {
+        int shift = 8 * p;
+        hipLaunchKernelGGL(k_radix_hist, dim3(nb), dim3(kRadixBlock), 0, s, ki, (uint32_t)n, shift,
+                           nb, hist);
+        launch_exclusive_scan(hist, hist, h, stmp, s);
+        hipLaunchKernelGGL(k_radix_scatter, dim3(nb), dim3(kRadixBlock), 0, s, ki, vi, ko, vo,
+                           (uint32_t)n, shift, nb, hist);
+        uint32_t *t;
+        t = ki; ki = ko; ko = t;
+        t = vi; vi = vo; vo = t;
+    }
+    return (passes & 1) ? 1 : 0;
+}
+
+void launch_radix_sort(uint32_t *keys, uint32_t *vals, uint32_t *keys_alt, uint32_t *vals_alt,
+                       size_t n, int bits, uint32_t *tmp, hipStream_t s)
+{
+    if (launch_radix_sort_impl(keys, vals, keys_alt, vals_alt, n, bits, tmp, s)) {
+        hipMemcpyAsync(keys, keys_alt, n * sizeof(uint32_t), hipMemcpyDeviceToDevice, s);
+        hipMemcpyAsync(vals, vals_alt, n * sizeof(uint32_t), hipMemcpyDeviceToDevice, s);
+    }
+}
+
+}  // namespace fgl

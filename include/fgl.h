@@ -1,0 +1,237 @@
+/*
+ * fgl.h -- C ABI of the MI355X raster/shading path ("forkergl on gfx950").
+ *
+ * Plain pointers and sizes only; no C++ or HIP types cross this boundary.
+ * Every entry point returns an int status (FGL_OK = 0, negative = error
+ * kind) and records a message readable through fgl_last_error().
+ *
+ * The ABI replaces the static state and draw calls of the reference's
+ * `struct ForkerGL` (src/forkergl.h:15-81), the per-face submission loop of
+ * `Mesh::Draw` (src/mesh.cpp:10-25) and the per-pixel loops of the frame
+ * driver (src/render.cpp:214-343).  Each declaration names the reference
+ * interface it replaces.
+ *
+ * Threading: one host thread per context.  Work is queued on the context's
+ * HIP stream; fgl_read_target / fgl_sync synchronise.  Contexts are
+ * independent.
+ */
+#ifndef FGL_H
+#define FGL_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ---- */
+#define FGL_OK 0
+#define FGL_ERR_INVALID (-1)   /* bad argument, wrong state */
+#define FGL_ERR_DEVICE (-2)    /* HIP runtime failure / no device */
+#define FGL_ERR_IO (-3)        /* file missing or malformed */
+#define FGL_ERR_NOMEM (-4)     /* device or host allocation failed */
+#define FGL_ERR_CAPACITY (-5)  /* a fixed capacity was exceeded */
+
+/* ---- enums mirroring the reference ---- */
+/* ForkerGL::PassType (src/forkergl.h:25-31) */
+#define FGL_PASS_FORWARD 0
+#define FGL_PASS_GEOMETRY 1
+#define FGL_PASS_LIGHTING 2
+#define FGL_PASS_SHADOW 3
+/* Texture::WrapMode / FilterMode (src/texture.h:15-27) */
+#define FGL_WRAP_NONE 0
+#define FGL_WRAP_REPEAT 1
+#define FGL_WRAP_MIRROR 2
+#define FGL_WRAP_CLAMP 3
+#define FGL_FILTER_NEAREST 0
+#define FGL_FILTER_LINEAR 1
+/* Matrices held by ForkerGL / passed as shader uniforms */
+#define FGL_MAT_VIEW 0
+#define FGL_MAT_PROJ 1
+#define FGL_MAT_LIGHT_SPACE 2
+/* Shadow filters (src/shaders/shadow.h:14-23; modes 1-3 take explicit sample sets) */
+#define FGL_SHADOW_HARD 0
+#define FGL_SHADOW_PCF_DISK 1
+#define FGL_SHADOW_PCSS 2
+#define FGL_SHADOW_PCF_GRID 3
+/* Render targets (ForkerGL's public buffers, src/forkergl.h:41-54) */
+#define FGL_TARGET_FRAME 0        /* FrameBuffer: 3 float planes            */
+#define FGL_TARGET_DEPTH 1        /* DepthBuffer: 1 float plane             */
+#define FGL_TARGET_SHADOW 2       /* ShadowBuffer: 1 float plane (SW x SH)  */
+#define FGL_TARGET_GBUFFER 3      /* Normal, WorldPos, LightSpaceNDC, Albedo,
+                                     Emissive, Param (3 planes each),
+                                     ShadingType (1 plane): 19 planes       */
+#define FGL_TARGET_AO 4           /* AmbientOcclusionGBuffer: 1 plane       */
+#define FGL_TARGET_IMAGE 5        /* 8-bit B,G,R image after quantise/SSAA  */
+#define FGL_TARGET_SHADOW_DEPTH 6 /* depth buffer of the shadow pass        */
+
+#define FGL_MAX_LIGHTS 4
+#define FGL_MAX_DISK 64
+
+typedef struct fgl_ctx fgl_ctx;
+
+/* Material + PBRMaterial pair from one "newmtl" (src/materials/material.h:13-41,
+ * pbrmaterial.h:13-45).  Texture ids are from fgl_upload_texture, -1 if absent. */
+typedef struct {
+    float ka[3], kd[3], ks[3], ke[3];
+    int32_t map_kd, map_ks, map_ke, map_bump;
+    float pbr_ke[3], albedo[3], roughness, metalness;
+    int32_t map_base, map_rough, map_metal, map_ao, map_pbr_ke;
+} fgl_material;
+
+/* A loaded Model (src/model.h:15-67): attribute arrays plus the faces of all
+ * its meshes concatenated in draw order (std::map order of mesh names,
+ * src/model.cpp:77-90), one material id per face. */
+typedef struct {
+    int32_t n_pos, n_uv, n_nrm, n_faces;
+    const float *pos;       /* n_pos*3 */
+    const float *uv;        /* n_uv*2 */
+    const float *nrm;       /* n_nrm*3 */
+    const float *tan;       /* n_pos*3, NULL when the model has no tangents */
+    const int32_t *vi, *ti, *ni;   /* n_faces*3 each: face-vertex indices */
+    const int32_t *face_material;  /* n_faces */
+    int32_t support_pbr;           /* Model::SupportPBR (src/model.cpp:52-68) */
+} fgl_model_desc;
+
+typedef struct {
+    int32_t type;       /* 0 point (casts the shadow map), 1 directional */
+    float vec[3];       /* position, or direction of travel */
+    float color[3];
+} fgl_light;
+
+typedef struct {
+    int32_t mode;               /* FGL_SHADOW_* */
+    int32_t grid;               /* PCF_GRID: n (odd, <= 9) */
+    int32_t n_blocker, n_pcf;   /* <= FGL_MAX_DISK */
+    float blocker_disk[2 * FGL_MAX_DISK];
+    float pcf_disk[2 * FGL_MAX_DISK];
+    float pcf_filter;           /* PCF_DISK filter size (float)   */
+    double blocker_filter;      /* PCSS blocker search size (FP64 literal in the reference) */
+    double pcss_filter;         /* PCF_FILTER_SIZE used by PCSS (FP64 literal) */
+    float area_light;           /* AREA_LIGHT_SIZE */
+} fgl_shadow_cfg;
+
+typedef struct {
+    int32_t n_kernel;           /* <= 64 */
+    float kernel[3 * 64];
+    float noise[3 * 16];        /* 4 x 4 tile */
+    float radius, bias, range;
+    int32_t blur;               /* 0 none, 1 reference recursive 9-tap, 2 separable */
+    int32_t blur_radius;        /* separable: taps = 2*R+1, R <= 8 */
+    float blur_weights[9];
+} fgl_ssao_cfg;
+
+/* ---- context ---- */
+int fgl_ctx_create(int device, fgl_ctx **out);
+void fgl_ctx_destroy(fgl_ctx *ctx);
+const char *fgl_last_error(const fgl_ctx *ctx);
+int fgl_sync(fgl_ctx *ctx);
+/* Raster tuning: tile edge in pixels (8, 16 or 32).  Results do not depend
+ * on it (tile ownership keeps submission order per pixel). */
+int fgl_set_tile_size(fgl_ctx *ctx, int tile);
+
+/* ---- resources (replace Model::Load's in-memory products) ---- */
+/* Texture (src/texture.h:29-39): bpp 1/3/4, bytes in B,G,R[,A] order after
+ * the loader's flips, texel (x,y) at (x + y*w)*bpp. */
+int fgl_upload_texture(fgl_ctx *ctx, int w, int h, int bpp, const uint8_t *bytes, int wrap,
+                       int filter, int *out_id);
+int fgl_upload_material(fgl_ctx *ctx, const fgl_material *m, int *out_id);
+int fgl_upload_model(fgl_ctx *ctx, const fgl_model_desc *d, int *out_id);
+
+/* ---- state (ForkerGL statics, src/forkergl.cpp:55-132) ---- */
+/* InitFrameBuffer/InitDepthBuffer/InitGeometryBuffers (forkergl.cpp:55-81) */
+int fgl_init_targets(fgl_ctx *ctx, int w, int h);
+/* InitShadowBuffer (forkergl.cpp:65-68); the reference uses the frame size */
+int fgl_init_shadow_buffer(fgl_ctx *ctx, int w, int h);
+/* ClearColor (forkergl.cpp:84-87) */
+int fgl_clear_color(fgl_ctx *ctx, float r, float g, float b);
+/* SetViewportMatrix (forkergl.cpp:89-102); shadow != 0 sets the shadow pass viewport */
+int fgl_set_viewport(fgl_ctx *ctx, int shadow, int x, int y, int w, int h);
+/* SetViewProjectionMatrix / SetLightSpaceMatrix and the shaders' uView/uProjection
+ * uniforms (forkergl.cpp:109-127; render.cpp:106-131); row-major 4x4 */
+int fgl_set_matrix(fgl_ctx *ctx, int which, const float m[16]);
+/* uEyePos / uPointLight (src/shaders/phongshader.h:26-27) generalised to a light list */
+int fgl_set_eye(fgl_ctx *ctx, const float eye[3]);
+int fgl_set_lights(fgl_ctx *ctx, int n, const fgl_light *lights);
+/* Shadow::SetShadowStatus + compile-time filter selection (src/shaders/shadow.h:11-45) */
+int fgl_set_shadow(fgl_ctx *ctx, int on, const fgl_shadow_cfg *cfg);
+
+/* ---- drawing ---- */
+/* ForkerGL::SetPassType (forkergl.cpp:129-132). */
+int fgl_begin_pass(fgl_ctx *ctx, int pass);
+/* Model::Render(shader) -> Mesh::Draw -> ProcessVertex x3 -> DrawTriangle per face
+ * (src/model.cpp:77-90, src/mesh.cpp:10-25, src/forkergl.cpp:239-261).  The shader
+ * kind follows the pass: DepthShader (shadow), GShader (geometry), Blinn-Phong or
+ * PBR by Model::SupportPBR (forward).  Draws are recorded and rasterised in
+ * submission order by fgl_end_pass (or by any read), all draws of a pass in one
+ * batched launch sequence. */
+int fgl_draw_model(fgl_ctx *ctx, int model, const float model_matrix[16]);
+int fgl_end_pass(fgl_ctx *ctx);
+/* ForkerGL::DrawScreenSpacePixels (forkergl.cpp:326-380): deferred lighting */
+int fgl_draw_screen_space_pixels(fgl_ctx *ctx);
+/* Render::DoSSAO (render.cpp:214-289) with a deterministic sample kernel, then the
+ * configured blur (Buffer1f::TwoPassGaussianBlurDenoised, buffer.cpp:59-98, or separable) */
+int fgl_ssao(fgl_ctx *ctx, const fgl_ssao_cfg *cfg);
+/* Buffer3f::GenerateImage (buffer.cpp:113-126) + Render::DoSSAA (render.cpp:291-343):
+ * k = 1 quantises only; the image is (W/k) x (H/k) B,G,R bytes. */
+int fgl_resolve(fgl_ctx *ctx, int k);
+
+/* ---- readback / upload of targets ---- */
+/* Copies a target to host memory.  `bytes` must equal the target's size:
+ * planes * W * H * 4 for float targets, (W/k)*(H/k)*3 for FGL_TARGET_IMAGE. */
+int fgl_read_target(fgl_ctx *ctx, int which, void *dst, size_t bytes);
+/* Overwrites a float target (tests feed G-buffers, depth or AO directly). */
+int fgl_write_target(fgl_ctx *ctx, int which, const void *src, size_t bytes);
+/* Size in bytes of a target as it is now. */
+int fgl_target_bytes(fgl_ctx *ctx, int which, size_t *out_bytes);
+
+/* ---- per-pass kernel timing (HIP events on the context stream) ---- */
+/* Returns the number of timed kernels recorded since the last call; names and
+ * milliseconds are written to the arrays (up to cap entries). */
+int fgl_enable_timing(fgl_ctx *ctx, int on);
+int fgl_read_timings(fgl_ctx *ctx, char (*names)[32], float *ms, int cap, int *out_n);
+
+/* ---- scene files: .scene/.obj/.mtl/.tga (src/scene.cpp, src/model.cpp, src/tgaimage.cpp) ---- */
+typedef struct fgl_scene fgl_scene;
+typedef struct {
+    int32_t width, height, ssaa_on, ssaa_k, ssao_on, shadow_on, deferred;
+    int32_t camera_ortho;
+    float eye[3], look_at[3];
+    int32_t n_lights;
+    fgl_light lights[FGL_MAX_LIGHTS];
+    int32_t n_models, n_materials, n_textures;
+    /* extensions (extra .scene keys the reference parser ignores) */
+    int32_t shadow_map_size;      /* 0: frame size (reference) */
+    fgl_shadow_cfg shadow;
+    fgl_ssao_cfg ssao;
+} fgl_scene_info;
+
+int fgl_scene_load(const char *path, fgl_scene **out, char *err, size_t err_len);
+void fgl_scene_free(fgl_scene *s);
+int fgl_scene_info_get(const fgl_scene *s, fgl_scene_info *out);
+/* Model i as uploaded (pointers stay valid until fgl_scene_free) and its model matrix */
+int fgl_scene_model(const fgl_scene *s, int i, fgl_model_desc *out, float model_matrix[16]);
+int fgl_scene_material(const fgl_scene *s, int i, fgl_material *out);
+/* Texture i: size, bpp, wrap, filter and a pointer to its bytes */
+int fgl_scene_texture(const fgl_scene *s, int i, int *w, int *h, int *bpp, int *wrap,
+                      int *filter, const uint8_t **bytes);
+/* Render::Render (render.cpp:40-58) on the GPU: shadow pass, forward or
+ * geometry + [SSAO + blur] + lighting, then quantise/SSAA. Uploads the scene
+ * on first use. */
+int fgl_render_scene(fgl_ctx *ctx, fgl_scene *s);
+
+/* ---- host matrix builders (src/geometry.cpp:60-179), shared by the CLI and tests ---- */
+void fgl_make_model_matrix(float tx, float ty, float tz, float roty_deg, float scale, float out[16]);
+void fgl_make_lookat(const float eye[3], const float center[3], const float up[3], float out[16]);
+void fgl_make_perspective(float fov, float aspect, float n, float f, float out[16]);
+void fgl_make_ortho(float l, float r, float b, float t, float n, float f, float out[16]);
+void fgl_mat_mul(const float a[16], const float b[16], float out[16]);
+
+/* ---- TGA codec (src/tgaimage.cpp:43-300) ---- */
+int fgl_tga_write(const char *path, int w, int h, int bpp, const uint8_t *bytes, int rle);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
