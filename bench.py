@@ -55,8 +55,10 @@ def algorithmic_bytes(name, I, ntri, npos):
     S = I.shadow_map_size if I.shadow_map_size > 0 else None
     SP = S * S if S else P
     tab = {
-        "raster_gbuffer": 80 * P + 12 * ntri + 44 * npos,        # G-buffer + depth write, indices, vertices
-        "raster_shadow": 8 * SP + 12 * ntri + 12 * npos,          # shadow depth + value, indices, positions
+        "raster": 8 * P + 16 * ntri + 16 * npos,                  # depth + winner write; triangle and vertex records
+        "gbuffer_resolve": 80 * P + 12 * ntri + 44 * npos,        # winner read, G-buffer write; indices, vertices
+        "forward_resolve": 16 * P + 12 * ntri + 44 * npos,        # winner read, colour write; indices, vertices
+        "raster_shadow": 8 * SP + 16 * ntri + 16 * npos,          # shadow depth + value; triangle, vertex records
         "lighting": 92 * P + 4 * SP,                              # 80 read + 12 write, shadow map once
         "ssao": 32 * P,                                           # normal, position, depth, AO
         "blur_gaussian": 16 * P,
@@ -68,8 +70,14 @@ def algorithmic_bytes(name, I, ntri, npos):
 
 def run_gpu(sc, steps, warmup, tile):
     import forkerrenderer_amd as F
+    from forkerrenderer_amd.autotune import autotune
     ctx = F.Context(0)
-    ctx.set_tile_size(tile)
+    tuned = None
+    if tile:
+        ctx.set_tile_size(tile)
+    else:
+        tuned = autotune(ctx, sc)
+        print(f"[bench] autotune {tuned[0]} {tuned[1]}", file=sys.stderr, flush=True)
     for _ in range(warmup):
         ctx.render_scene(sc)
     ctx.sync()
@@ -88,7 +96,7 @@ def run_gpu(sc, steps, warmup, tile):
     per = {}
     for name, ms in timings:
         per.setdefault(name, []).append(ms)
-    return (t1 - t0) / steps, per, img
+    return (t1 - t0) / steps, per, img, tuned
 
 
 def cpu_baseline(sc, band_frac):
@@ -150,10 +158,19 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--config", default="c3")
-    ap.add_argument("--tile", type=int, default=16)
+    ap.add_argument("--tile", type=int, default=0, help="tile edge 8/16/32; 0 = autotune")
     ap.add_argument("--cpu-band", type=float, default=1.0 / 32)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-calibrate", type=int, default=0, metavar="MIB",
+                    help="only run a plain copy of MIB MiB (calibrates HBM byte counters) and exit")
     a = ap.parse_args()
+    if a.copy_calibrate:
+        import forkerrenderer_amd as F
+        ctx = F.Context(0)
+        nbytes = a.copy_calibrate << 20
+        ms = ctx.copy_benchmark(nbytes, 20)
+        print(json.dumps({"copy_bytes": nbytes, "ms": ms, "GBps_read_plus_write": 2 * nbytes / (ms * 1e-3) / 1e9}))
+        return
     if a.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; multi-GPU sharding is described in "
                                    "DESIGN.md and not launched", "n_gpus": a.gpus}))
@@ -164,7 +181,7 @@ def main():
     I = sc.info
     ntri, npos = scene_stats(sc)
     print(f"[bench] scene {p}: {ntri} triangles, {npos} vertices", file=sys.stderr, flush=True)
-    sec, per, img = run_gpu(sc, a.steps, a.warmup, a.tile)
+    sec, per, img, tuned = run_gpu(sc, a.steps, a.warmup, a.tile)
     print(f"[bench] gpu {sec * 1e3:.3f} ms/frame", file=sys.stderr, flush=True)
     kernels = {}
     for name, v in per.items():
@@ -192,7 +209,7 @@ def main():
         "config": {"workload": f"{a.config}: {I.width}x{I.height} SSAA {k}x, deferred PBR, "
                                f"{'PCSS' if I.shadow.mode == 2 else 'PCF'} {I.shadow_map_size}^2 shadow map, "
                                f"SSAO {I.ssao.n_kernel} + blur, {ntri} triangles",
-                   "internal_resolution": [I.width * k, I.height * k], "triangles": ntri, "tile": a.tile},
+                   "internal_resolution": [I.width * k, I.height * k], "triangles": ntri, "tile": a.tile if a.tile else (tuned[0] if tuned else None), "autotune_ms": tuned[1] if tuned else None},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None, "bytes_per_launch": bytes_launch,
                      "ms_per_launch": ms_launch},

@@ -88,6 +88,8 @@ def lib():
             "fgl_last_error": (C.c_char_p, [P]),
             "fgl_sync": (C.c_int, [P]),
             "fgl_set_tile_size": (C.c_int, [P, C.c_int]),
+            "fgl_set_pass_tile_size": (C.c_int, [P, C.c_int, C.c_int]),
+            "fgl_get_pass_tile_size": (C.c_int, [P, C.c_int, C.POINTER(C.c_int)]),
             "fgl_upload_texture": (C.c_int, [P, C.c_int, C.c_int, C.c_int, P, C.c_int, C.c_int, C.POINTER(C.c_int)]),
             "fgl_upload_material": (C.c_int, [P, C.POINTER(Material), C.POINTER(C.c_int)]),
             "fgl_upload_model": (C.c_int, [P, C.POINTER(ModelDesc), C.POINTER(C.c_int)]),
@@ -124,6 +126,7 @@ def lib():
             "fgl_make_ortho": (None, [C.c_float] * 6 + [P]),
             "fgl_mat_mul": (None, [P, P, P]),
             "fgl_tga_write": (C.c_int, [C.c_char_p, C.c_int, C.c_int, C.c_int, P, C.c_int]),
+            "fgl_copy_benchmark": (C.c_int, [P, C.c_size_t, C.c_int, C.POINTER(C.c_float)]),
         }
         for name, (res, args) in sigs.items():
             f = getattr(L, name)
@@ -334,6 +337,11 @@ class Context:
         n = C.c_int()
         self._chk(lib().fgl_read_timings(self._h, names, ms, cap, C.byref(n)), "fgl_read_timings")
         return [(bytes(names[i]).split(b"\0")[0].decode(), float(ms[i])) for i in range(min(n.value, cap))]
+
+    def copy_benchmark(self, nbytes, iters=20):
+        ms = C.c_float()
+        self._chk(lib().fgl_copy_benchmark(self._h, nbytes, iters, C.byref(ms)), "fgl_copy_benchmark")
+        return ms.value
 
     def render_scene(self, scene):
         self._chk(lib().fgl_render_scene(self._h, scene._h), "fgl_render_scene")

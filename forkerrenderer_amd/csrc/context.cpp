@@ -60,8 +60,13 @@ struct fgl_ctx {
     fgl_shadow_cfg shadow_cfg;
 
     // targets
-    DevBuf depth, fb, gbuf, ao, blur_tmp, shadow, shadow_depth, img;
+    DevBuf depth, fb, gbuf, ao, blur_tmp, shadow, shadow_depth, img, vis;
     int img_w = 0, img_h = 0;
+    // Clears requested by fgl_init_* / fgl_clear_color that have not been
+    // written yet: the next pass that covers every pixel applies them.
+    bool pend_depth = false, pend_gbuf = false, pend_fb = false, pend_ao = false, pend_shadow = false;
+    float fb_clear[3] = {0.f, 0.f, 0.f};
+    int TS_shadow = 16;
 
     // current pass
     int pass = -1;
@@ -72,10 +77,9 @@ struct fgl_ctx {
     std::vector<Rec> draws;
 
     // pass scratch
-    DevBuf d_draws, d_tri_base, d_vert_base, screen, rect, counts, offs, scan_tmp, keys, vals,
-        keys_alt, vals_alt, sort_tmp, tile_start, tile_end;
+    DevBuf d_draws, screen, rect, counts, offs, scan_tmp, keys, vals, keys_alt, vals_alt, sort_tmp,
+        tile_start, tile_end, tri_v, big_list, counters;
     std::vector<DevDraw> last_draws;
-    std::vector<int> last_tri_base, last_vert_base;
     uint32_t *h_total = nullptr;
 
     // timing
@@ -196,10 +200,10 @@ void fgl_ctx_destroy(fgl_ctx *c)
         hipFree(m.vi); hipFree(m.ti); hipFree(m.ni); hipFree(m.fmat);
     }
     DevBuf *bufs[] = {&c->d_textures, &c->d_materials, &c->depth, &c->fb, &c->gbuf, &c->ao,
-                      &c->blur_tmp, &c->shadow, &c->shadow_depth, &c->img, &c->d_draws,
-                      &c->d_tri_base, &c->d_vert_base, &c->screen, &c->rect, &c->counts,
-                      &c->offs, &c->scan_tmp, &c->keys, &c->vals, &c->keys_alt, &c->vals_alt,
-                      &c->sort_tmp, &c->tile_start, &c->tile_end};
+                      &c->blur_tmp, &c->shadow, &c->shadow_depth, &c->img, &c->vis, &c->d_draws,
+                      &c->screen, &c->rect, &c->counts, &c->offs, &c->scan_tmp, &c->keys,
+                      &c->vals, &c->keys_alt, &c->vals_alt, &c->sort_tmp, &c->tile_start,
+                      &c->tile_end, &c->tri_v, &c->big_list, &c->counters};
     for (DevBuf *b : bufs) free_buf(*b);
     for (auto &t : c->timed) { hipEventDestroy(t.a); hipEventDestroy(t.b); }
     if (c->h_total) hipHostFree(c->h_total);
@@ -221,6 +225,24 @@ int fgl_set_tile_size(fgl_ctx *c, int tile)
     if (!c) return FGL_ERR_INVALID;
     if (tile != 8 && tile != 16 && tile != 32) return fail(c, FGL_ERR_INVALID, "tile must be 8, 16 or 32");
     c->TS = tile;
+    c->TS_shadow = tile;
+    return FGL_OK;
+}
+
+int fgl_set_pass_tile_size(fgl_ctx *c, int pass, int tile)
+{
+    if (!c) return FGL_ERR_INVALID;
+    if (tile != 8 && tile != 16 && tile != 32) return fail(c, FGL_ERR_INVALID, "tile must be 8, 16 or 32");
+    if (pass == FGL_PASS_SHADOW) c->TS_shadow = tile;
+    else if (pass == FGL_PASS_FORWARD || pass == FGL_PASS_GEOMETRY) c->TS = tile;
+    else return fail(c, FGL_ERR_INVALID, "pass has no raster stage");
+    return FGL_OK;
+}
+
+int fgl_get_pass_tile_size(fgl_ctx *c, int pass, int *tile)
+{
+    if (!c || !tile) return FGL_ERR_INVALID;
+    *tile = (pass == FGL_PASS_SHADOW) ? c->TS_shadow : c->TS;
     return FGL_OK;
 }
 
@@ -304,28 +326,46 @@ int fgl_upload_model(fgl_ctx *c, const fgl_model_desc *d, int *out_id)
 
 // ---------------- state ----------------
 
-static int fill_f32(fgl_ctx *c, DevBuf &b, float v, size_t n)
+static int flush_pass(fgl_ctx *c);
+
+// Writes the clear values still pending on the targets named by the flags.
+static int materialize(fgl_ctx *c, bool depth, bool gbuf, bool fb, bool ao, bool shadow)
 {
-    launch_fill_f32((float *)b.p, v, n, c->stream);
+    const size_t P = (size_t)c->W * c->H, SP = (size_t)c->SW * c->SH;
+    if (depth && c->pend_depth) { launch_fill_f32((float *)c->depth.p, FLT_MAX, P, c->stream); c->pend_depth = false; }
+    if (gbuf && c->pend_gbuf) { launch_fill_f32((float *)c->gbuf.p, 0.f, kGbufPlanes * P, c->stream); c->pend_gbuf = false; }
+    if (fb && c->pend_fb) {
+        for (int k = 0; k < 3; ++k) launch_fill_f32((float *)c->fb.p + k * P, c->fb_clear[k], P, c->stream);
+        c->pend_fb = false;
+    }
+    if (ao && c->pend_ao) { launch_fill_f32((float *)c->ao.p, 1.f, P, c->stream); c->pend_ao = false; }
+    if (shadow && c->pend_shadow) {
+        launch_fill_f32((float *)c->shadow.p, 0.f, SP, c->stream);
+        launch_fill_f32((float *)c->shadow_depth.p, FLT_MAX, SP, c->stream);
+        c->pend_shadow = false;
+    }
     HIPCHK(c, hipGetLastError());
     return FGL_OK;
 }
 
+// InitFrameBuffer (zero), InitDepthBuffer (FLT_MAX), InitGeometryBuffers
+// (zero, AO one): recorded, then written by the first pass that covers the
+// targets (or materialised before any partial use).
 int fgl_init_targets(fgl_ctx *c, int w, int h)
 {
     if (!c) return FGL_ERR_INVALID;
     if (w <= 0 || h <= 0 || w > 16384 || h > 16384) return fail(c, FGL_ERR_INVALID, "bad target size");
+    int rc = flush_pass(c);
+    if (rc) return rc;
     size_t P = (size_t)w * h;
-    int rc;
     if ((rc = ensure(c, c->depth, P * 4)) || (rc = ensure(c, c->fb, 3 * P * 4)) ||
         (rc = ensure(c, c->gbuf, kGbufPlanes * P * 4)) || (rc = ensure(c, c->ao, P * 4)) ||
-        (rc = ensure(c, c->blur_tmp, P * 4)))
+        (rc = ensure(c, c->blur_tmp, P * 4)) || (rc = ensure(c, c->vis, P * 4)))
         return rc;
     c->W = w;
     c->H = h;
-    if ((rc = fill_f32(c, c->fb, 0.f, 3 * P)) || (rc = fill_f32(c, c->depth, FLT_MAX, P)) ||
-        (rc = fill_f32(c, c->gbuf, 0.f, kGbufPlanes * P)) || (rc = fill_f32(c, c->ao, 1.f, P)))
-        return rc;
+    c->pend_depth = c->pend_gbuf = c->pend_fb = c->pend_ao = true;
+    c->fb_clear[0] = c->fb_clear[1] = c->fb_clear[2] = 0.f;
     return FGL_OK;
 }
 
@@ -333,24 +373,25 @@ int fgl_init_shadow_buffer(fgl_ctx *c, int w, int h)
 {
     if (!c) return FGL_ERR_INVALID;
     if (w <= 0 || h <= 0 || w > 16384 || h > 16384) return fail(c, FGL_ERR_INVALID, "bad shadow size");
+    int rc = flush_pass(c);
+    if (rc) return rc;
     size_t P = (size_t)w * h;
-    int rc;
     if ((rc = ensure(c, c->shadow, P * 4)) || (rc = ensure(c, c->shadow_depth, P * 4))) return rc;
     c->SW = w;
     c->SH = h;
-    if ((rc = fill_f32(c, c->shadow, 0.f, P)) || (rc = fill_f32(c, c->shadow_depth, FLT_MAX, P)))
-        return rc;
+    c->pend_shadow = true;
     return FGL_OK;
 }
 
 int fgl_clear_color(fgl_ctx *c, float r, float g, float b)
 {
     if (!c || !c->fb.p) return fail(c, FGL_ERR_INVALID, "targets not initialised");
-    size_t P = (size_t)c->W * c->H;
-    launch_fill_f32((float *)c->fb.p, r, P, c->stream);
-    launch_fill_f32((float *)c->fb.p + P, g, P, c->stream);
-    launch_fill_f32((float *)c->fb.p + 2 * P, b, P, c->stream);
-    HIPCHK(c, hipGetLastError());
+    int rc = flush_pass(c);
+    if (rc) return rc;
+    c->fb_clear[0] = r;
+    c->fb_clear[1] = g;
+    c->fb_clear[2] = b;
+    c->pend_fb = true;  // overwrites every pixel, so it replaces any pending clear
     return FGL_OK;
 }
 
@@ -514,8 +555,8 @@ static int flush_pass(fgl_ctx *c)
 
     // draw table in submission order
     std::vector<DevDraw> dd(c->draws.size());
-    std::vector<int> tri_base(c->draws.size() + 1), vert_base(c->draws.size() + 1);
     long long nt = 0, nv = 0;
+    int max_pos = 0, max_faces = 0;
     for (size_t i = 0; i < c->draws.size(); ++i) {
         const ModelRes &m = c->models[c->draws[i].model];
         DevDraw &d = dd[i];
@@ -531,37 +572,40 @@ static int flush_pass(fgl_ctx *c)
         std::memcpy(d.model, c->draws[i].m, sizeof(d.model));
         make_normal_matrix(d.model, d.normal);
         mat_mul(c->light_space, d.model, d.lm);
-        tri_base[i] = (int)nt;
-        vert_base[i] = (int)nv;
         nt += m.n_faces;
         nv += m.n_pos;
+        max_pos = m.n_pos > max_pos ? m.n_pos : max_pos;
+        max_faces = m.n_faces > max_faces ? m.n_faces : max_faces;
     }
-    tri_base[c->draws.size()] = (int)nt;
-    vert_base[c->draws.size()] = (int)nv;
     const int n_draws = (int)c->draws.size();
     c->draws.clear();
     if (nt > 0x7fffffffLL || nv > 0x7fffffffLL) return fail(c, FGL_ERR_CAPACITY, "pass too large");
+    // Whether this pass starts from the cleared targets: then it writes
+    // every pixel (winner or clear value) and no separate clear runs.
+    bool fresh;
+    if (shadow) {
+        fresh = c->pend_shadow;
+    } else if (pass == kPassGeometry) {
+        if (c->pend_depth != c->pend_gbuf && (rc = materialize(c, true, true, false, false, false))) return rc;
+        fresh = c->pend_depth;
+    } else {
+        if (c->pend_depth != c->pend_fb && (rc = materialize(c, true, false, true, false, false))) return rc;
+        if (c->shadow_on && (rc = materialize(c, false, false, false, false, true))) return rc;
+        fresh = c->pend_depth;
+    }
     if (nt == 0) return FGL_OK;
 
     if (dd.size() != c->last_draws.size() ||
-        std::memcmp(dd.data(), c->last_draws.data(), dd.size() * sizeof(DevDraw)) != 0 ||
-        tri_base != c->last_tri_base || vert_base != c->last_vert_base) {
-        if ((rc = ensure(c, c->d_draws, dd.size() * sizeof(DevDraw))) ||
-            (rc = ensure(c, c->d_tri_base, tri_base.size() * 4)) ||
-            (rc = ensure(c, c->d_vert_base, vert_base.size() * 4)))
-            return rc;
+        std::memcmp(dd.data(), c->last_draws.data(), dd.size() * sizeof(DevDraw)) != 0) {
+        if ((rc = ensure(c, c->d_draws, dd.size() * sizeof(DevDraw)))) return rc;
         HIPCHK(c, hipStreamSynchronize(c->stream));
         HIPCHK(c, hipMemcpy(c->d_draws.p, dd.data(), dd.size() * sizeof(DevDraw), hipMemcpyHostToDevice));
-        HIPCHK(c, hipMemcpy(c->d_tri_base.p, tri_base.data(), tri_base.size() * 4, hipMemcpyHostToDevice));
-        HIPCHK(c, hipMemcpy(c->d_vert_base.p, vert_base.data(), vert_base.size() * 4, hipMemcpyHostToDevice));
         c->last_draws = dd;
-        c->last_tri_base = tri_base;
-        c->last_vert_base = vert_base;
     }
 
     FrameUniforms u;
     build_uniforms(c, u);
-    const int TS = c->TS;
+    const int TS = shadow ? c->TS_shadow : c->TS;
     const int tiles_x = (W + TS - 1) / TS, tiles_y = (H + TS - 1) / TS;
     const uint32_t ntiles = (uint32_t)(tiles_x * tiles_y);
     if ((rc = ensure(c, c->screen, (size_t)nv * sizeof(int4))) ||
@@ -569,18 +613,22 @@ static int flush_pass(fgl_ctx *c)
         (rc = ensure(c, c->counts, (size_t)nt * 4)) || (rc = ensure(c, c->offs, ((size_t)nt + 1) * 4)) ||
         (rc = ensure(c, c->scan_tmp, scan_tmp_words((size_t)nt) * 4)) ||
         (rc = ensure(c, c->tile_start, (size_t)ntiles * 4)) ||
-        (rc = ensure(c, c->tile_end, (size_t)ntiles * 4)))
+        (rc = ensure(c, c->tile_end, (size_t)ntiles * 4)) ||
+        (rc = ensure(c, c->tri_v, (size_t)nt * sizeof(int4))) ||
+        (rc = ensure(c, c->big_list, (size_t)nt * 4)) || (rc = ensure(c, c->counters, 64)))
         return rc;
     const DevDraw *ddraws = (const DevDraw *)c->d_draws.p;
+    uint32_t *big_count = (uint32_t *)c->counters.p;
     {
         KernelTimer t(c, "vertex");
-        launch_vertex(shadow, ddraws, (const int *)c->d_vert_base.p, n_draws, (int)nv, u,
-                      (int4 *)c->screen.p, c->stream);
+        launch_vertex(shadow, ddraws, n_draws, max_pos, u, (int4 *)c->screen.p, c->stream);
     }
     {
         KernelTimer t(c, "setup");
-        launch_setup(ddraws, (const int *)c->d_tri_base.p, n_draws, (const int4 *)c->screen.p,
-                     (int)nt, W, H, TS, (uint2 *)c->rect.p, (uint32_t *)c->counts.p, c->stream);
+        launch_fill_u32(big_count, 0, 1, c->stream);
+        launch_setup(ddraws, n_draws, max_faces, (const int4 *)c->screen.p, W, H, TS,
+                     (uint2 *)c->rect.p, (uint32_t *)c->counts.p, (int4 *)c->tri_v.p,
+                     (uint32_t *)c->big_list.p, big_count, c->stream);
     }
     {
         KernelTimer t(c, "scan");
@@ -588,8 +636,9 @@ static int flush_pass(fgl_ctx *c)
                               (uint32_t *)c->scan_tmp.p, c->stream);
     }
     HIPCHK(c, hipMemcpyAsync(c->h_total, (uint32_t *)c->offs.p + nt, 4, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipMemcpyAsync(c->h_total + 1, big_count, 4, hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
-    const uint32_t npairs = *c->h_total;
+    const uint32_t npairs = c->h_total[0], n_big = c->h_total[1];
     if ((rc = ensure(c, c->keys, (size_t)npairs * 4 + 4)) || (rc = ensure(c, c->vals, (size_t)npairs * 4 + 4)) ||
         (rc = ensure(c, c->keys_alt, (size_t)npairs * 4 + 4)) ||
         (rc = ensure(c, c->vals_alt, (size_t)npairs * 4 + 4)) ||
@@ -597,8 +646,9 @@ static int flush_pass(fgl_ctx *c)
         return rc;
     {
         KernelTimer t(c, "emit");
-        launch_emit((const uint32_t *)c->offs.p, (int)nt, (const uint2 *)c->rect.p, tiles_x, npairs,
-                    (uint32_t *)c->keys.p, (uint32_t *)c->vals.p, c->stream);
+        launch_emit((const uint32_t *)c->offs.p, (int)nt, (const uint2 *)c->rect.p, tiles_x,
+                    (const uint32_t *)c->big_list.p, n_big, (uint32_t *)c->keys.p,
+                    (uint32_t *)c->vals.p, c->stream);
     }
     {
         KernelTimer t(c, "sort");
@@ -616,9 +666,9 @@ static int flush_pass(fgl_ctx *c)
     RasterArgs a;
     std::memset(&a, 0, sizeof(a));
     a.draws = ddraws;
-    a.tri_base = (const int *)c->d_tri_base.p;
     a.n_draws = n_draws;
     a.screen = (const int4 *)c->screen.p;
+    a.tri_v = (const int4 *)c->tri_v.p;
     a.vals = (const uint32_t *)c->vals.p;
     a.tile_start = (const uint32_t *)c->tile_start.p;
     a.tile_end = (const uint32_t *)c->tile_end.p;
@@ -626,16 +676,43 @@ static int flush_pass(fgl_ctx *c)
     a.H = H;
     a.tiles_x = tiles_x;
     a.tiles_y = tiles_y;
+    a.fresh = fresh;
     a.depth = shadow ? (float *)c->shadow_depth.p : (float *)c->depth.p;
-    a.out0 = shadow ? (float *)c->shadow.p : (float *)c->fb.p;
-    a.gbuf = (float *)c->gbuf.p;
-    a.materials = (const DevMaterial *)c->d_materials.p;
-    a.textures = (const DevTexture *)c->d_textures.p;
-    a.shadow_map = (const float *)c->shadow.p;
-    a.u = u;
+    a.vis = (int *)c->vis.p;
+    a.shadow_out = (float *)c->shadow.p;
     {
-        KernelTimer t(c, shadow ? "raster_shadow" : pass == kPassGeometry ? "raster_gbuffer" : "raster_forward");
-        launch_raster(pass, TS, a, c->stream);
+        KernelTimer t(c, shadow ? "raster_shadow" : "raster");
+        launch_raster(shadow, TS, a, c->stream);
+    }
+    if (shadow) {
+        c->pend_shadow = false;
+    } else {
+        ResolveArgs r;
+        std::memset(&r, 0, sizeof(r));
+        r.draws = ddraws;
+        r.screen = a.screen;
+        r.tri_v = a.tri_v;
+        r.vis = a.vis;
+        r.W = W;
+        r.H = H;
+        r.fresh = fresh;
+        std::memcpy(r.clear, c->fb_clear, sizeof(r.clear));
+        r.gbuf = (float *)c->gbuf.p;
+        r.fb = (float *)c->fb.p;
+        r.materials = (const DevMaterial *)c->d_materials.p;
+        r.textures = (const DevTexture *)c->d_textures.p;
+        r.shadow_map = (const float *)c->shadow.p;
+        r.u = u;
+        if (pass == kPassGeometry) {
+            KernelTimer t(c, "gbuffer_resolve");
+            launch_gbuffer_resolve(r, c->stream);
+            c->pend_gbuf = false;
+        } else {
+            KernelTimer t(c, "forward_resolve");
+            launch_forward_resolve(r, c->stream);
+            c->pend_fb = false;
+        }
+        c->pend_depth = false;
     }
     HIPCHK(c, hipGetLastError());
     return FGL_OK;
@@ -654,11 +731,15 @@ int fgl_draw_screen_space_pixels(fgl_ctx *c)
     if (rc) return rc;
     if (!c->gbuf.p) return fail(c, FGL_ERR_INVALID, "targets not initialised");
     if (c->shadow_on && !c->shadow.p) return fail(c, FGL_ERR_INVALID, "shadows on but no shadow buffer");
+    if ((rc = materialize(c, false, true, false, false, c->shadow_on != 0))) return rc;
     FrameUniforms u;
     build_uniforms(c, u);
-    KernelTimer t(c, "lighting");
-    launch_lighting(u, (const float *)c->gbuf.p, (const float *)c->ao.p, (const float *)c->shadow.p,
-                    (float *)c->fb.p, c->stream);
+    {
+        KernelTimer t(c, "lighting");
+        launch_lighting(u, (const float *)c->gbuf.p, (const float *)c->ao.p, c->pend_ao ? 1 : 0,
+                        (const float *)c->shadow.p, (float *)c->fb.p, c->stream);
+    }
+    c->pend_fb = false;  // every pixel written
     HIPCHK(c, hipGetLastError());
     return FGL_OK;
 }
@@ -672,6 +753,8 @@ int fgl_ssao(fgl_ctx *c, const fgl_ssao_cfg *cfg)
     int rc = flush_pass(c);
     if (rc) return rc;
     if (!c->gbuf.p) return fail(c, FGL_ERR_INVALID, "targets not initialised");
+    if ((rc = materialize(c, true, true, false, false, false))) return rc;
+    c->pend_ao = false;  // every pixel written
     FrameUniforms u;
     build_uniforms(c, u);
     SsaoArgs sa;
@@ -705,6 +788,7 @@ int fgl_resolve(fgl_ctx *c, int k)
     if (!c->fb.p) return fail(c, FGL_ERR_INVALID, "targets not initialised");
     if (k < 1 || k > 16 || c->W / k <= 0 || c->H / k <= 0) return fail(c, FGL_ERR_INVALID, "bad SSAA factor");
     int ow = c->W / k, oh = c->H / k;
+    if ((rc = materialize(c, false, false, true, false, false))) return rc;
     if ((rc = ensure(c, c->img, (size_t)ow * oh * 3))) return rc;
     c->img_w = ow;
     c->img_h = oh;
@@ -749,6 +833,10 @@ int fgl_read_target(fgl_ctx *c, int which, void *dst, size_t bytes)
     size_t n;
     if ((rc = target_ptr(c, which, &p, &n))) return rc;
     if (bytes != n) return fail(c, FGL_ERR_INVALID, "size mismatch: target has " + std::to_string(n) + " bytes");
+    if ((rc = materialize(c, which == FGL_TARGET_DEPTH, which == FGL_TARGET_GBUFFER,
+                          which == FGL_TARGET_FRAME, which == FGL_TARGET_AO,
+                          which == FGL_TARGET_SHADOW || which == FGL_TARGET_SHADOW_DEPTH)))
+        return rc;
     HIPCHK(c, hipMemcpyAsync(dst, p, n, hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
     return FGL_OK;
@@ -764,6 +852,14 @@ int fgl_write_target(fgl_ctx *c, int which, const void *src, size_t bytes)
     size_t n;
     if ((rc = target_ptr(c, which, &p, &n))) return rc;
     if (bytes != n) return fail(c, FGL_ERR_INVALID, "size mismatch: target has " + std::to_string(n) + " bytes");
+    // the whole target is replaced; a shadow target's companion is materialised
+    if (which == FGL_TARGET_SHADOW || which == FGL_TARGET_SHADOW_DEPTH) {
+        if ((rc = materialize(c, false, false, false, false, true))) return rc;
+    }
+    if (which == FGL_TARGET_DEPTH) c->pend_depth = false;
+    if (which == FGL_TARGET_GBUFFER) c->pend_gbuf = false;
+    if (which == FGL_TARGET_FRAME) c->pend_fb = false;
+    if (which == FGL_TARGET_AO) c->pend_ao = false;
     HIPCHK(c, hipStreamSynchronize(c->stream));
     HIPCHK(c, hipMemcpy(p, src, n, hipMemcpyHostToDevice));
     return FGL_OK;
@@ -794,6 +890,34 @@ int fgl_read_timings(fgl_ctx *c, char (*names)[32], float *ms, int cap, int *out
     }
     c->timed.clear();
     *out_n = n;
+    return FGL_OK;
+}
+
+int fgl_copy_benchmark(fgl_ctx *c, size_t bytes, int iters, float *ms_per_iter)
+{
+    if (!c || bytes < 16 || iters < 1) return FGL_ERR_INVALID;
+    void *a = nullptr, *b = nullptr;
+    HIPCHK(c, hipMalloc(&a, bytes));
+    if (hipMalloc(&b, bytes) != hipSuccess) {
+        hipFree(a);
+        return fail(c, FGL_ERR_NOMEM, "copy buffer");
+    }
+    launch_fill_f32((float *)a, 1.f, bytes / 4, c->stream);
+    hipEvent_t e0, e1;
+    (void)hipEventCreate(&e0);
+    (void)hipEventCreate(&e1);
+    launch_copy(a, b, bytes, c->stream);
+    (void)hipEventRecord(e0, c->stream);
+    for (int i = 0; i < iters; ++i) launch_copy(a, b, bytes, c->stream);
+    (void)hipEventRecord(e1, c->stream);
+    (void)hipEventSynchronize(e1);
+    float ms = 0.f;
+    (void)hipEventElapsedTime(&ms, e0, e1);
+    if (ms_per_iter) *ms_per_iter = ms / iters;
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    hipFree(a);
+    hipFree(b);
     return FGL_OK;
 }
 

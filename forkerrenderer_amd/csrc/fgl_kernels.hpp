@@ -10,18 +10,33 @@ constexpr int kPassForward = 0;
 constexpr int kPassGeometry = 1;
 constexpr int kPassShadow = 3;
 constexpr int kGbufPlanes = 19;
+// Triangles whose tile rectangle has more tiles than this are emitted by a
+// whole workgroup each (k_emit_big) instead of by one thread.
+constexpr int kSmallTriTiles = 16;
 
 struct RasterArgs {
     const DevDraw *draws;
-    const int *tri_base;
     int n_draws;
-    const int4 *screen;
-    const uint32_t *vals;
+    const int4 *screen;        // per vertex: X, Y, z bits, exact flag
+    const int4 *tri_v;         // per triangle: 3 pass-wide vertex ids, draw id
+    const uint32_t *vals;      // sorted pair values (triangle ids)
     const uint32_t *tile_start, *tile_end;
     int W, H, tiles_x, tiles_y;
+    int fresh;                 // targets still hold their clear values: start from them
     float *depth;
-    float *out0;   // shadow value plane (shadow pass) or 3 colour planes (forward)
-    float *gbuf;   // 19 planes (geometry pass)
+    int *vis;                  // winning triangle per pixel, -1 if none (this pass)
+    float *shadow_out;         // shadow pass: ShadowBuffer plane
+};
+
+struct ResolveArgs {
+    const DevDraw *draws;
+    const int4 *screen;
+    const int4 *tri_v;
+    const int *vis;
+    int W, H, fresh;
+    float clear[3];            // forward pass background colour
+    float *gbuf;               // 19 planes (geometry pass)
+    float *fb;                 // 3 planes (forward pass)
     const DevMaterial *materials;
     const DevTexture *textures;
     const float *shadow_map;
@@ -36,17 +51,21 @@ struct SsaoArgs {
 };
 
 // raster.hip
-void launch_vertex(bool shadow, const DevDraw *draws, const int *vert_base, int n_draws,
-                   int n_total, const FrameUniforms &u, int4 *out, hipStream_t s);
-void launch_setup(const DevDraw *draws, const int *tri_base, int n_draws, const int4 *screen,
-                  int n_tri, int W, int H, int TS, uint2 *rect, uint32_t *count, hipStream_t s);
+void launch_vertex(bool shadow, const DevDraw *draws, int n_draws, int max_pos,
+                   const FrameUniforms &u, int4 *out, hipStream_t s);
+void launch_setup(const DevDraw *draws, int n_draws, int max_faces, const int4 *screen, int W,
+                  int H, int TS, uint2 *rect, uint32_t *count, int4 *tri_v, uint32_t *big_list,
+                  uint32_t *big_count, hipStream_t s);
 void launch_emit(const uint32_t *off, int n_tri, const uint2 *rect, int tiles_x,
-                 uint32_t n_pairs, uint32_t *keys, uint32_t *vals, hipStream_t s);
+                 const uint32_t *big_list, uint32_t n_big, uint32_t *keys, uint32_t *vals,
+                 hipStream_t s);
 void launch_fill_u32(uint32_t *p, uint32_t v, size_t n, hipStream_t s);
 void launch_fill_f32(float *p, float v, size_t n, hipStream_t s);
 void launch_tile_ranges(const uint32_t *keys, uint32_t n, uint32_t *start, uint32_t *end,
                         hipStream_t s);
-void launch_raster(int pass, int TS, const RasterArgs &a, hipStream_t s);
+void launch_raster(bool shadow, int TS, const RasterArgs &a, hipStream_t s);
+void launch_gbuffer_resolve(const ResolveArgs &a, hipStream_t s);
+void launch_forward_resolve(const ResolveArgs &a, hipStream_t s);
 
 // sort.hip
 // Exclusive scan of n uint32 values into out (out may alias in); the total
@@ -61,7 +80,7 @@ void launch_radix_sort(uint32_t *keys, uint32_t *vals, uint32_t *keys_alt, uint3
                        size_t n, int bits, uint32_t *tmp, hipStream_t s);
 
 // shade.hip
-void launch_lighting(const FrameUniforms &u, const float *gbuf, const float *ao,
+void launch_lighting(const FrameUniforms &u, const float *gbuf, const float *ao, int ao_is_one,
                      const float *shadow_map, float *fb, hipStream_t s);
 void launch_ssao(const FrameUniforms &u, const SsaoArgs &sa, const float *depth,
                  const float *gbuf, float *ao, hipStream_t s);
@@ -69,5 +88,6 @@ void launch_blur_recursive(float *ao, int W, int H, hipStream_t s);
 void launch_blur_separable(float *ao, float *tmp, int W, int H, const float *w, int R,
                            hipStream_t s);
 void launch_resolve(const float *fb, int W, int H, int k, uint8_t *img, hipStream_t s);
+void launch_copy(const void *src, void *dst, size_t bytes, hipStream_t s);
 
 }  // namespace fgl

@@ -8,7 +8,7 @@ namespace fgl {
 // ForkerGL::DrawScreenSpacePixels (src/forkergl.cpp:326-380): read the
 // G-buffer (76 B/px) + AO (4 B/px), light, write the frame buffer (12 B/px).
 __global__ __launch_bounds__(256) void k_lighting(FrameUniforms u, const float *__restrict__ g,
-                                                  const float *__restrict__ ao,
+                                                  const float *__restrict__ ao, int ao_is_one,
                                                   const float *__restrict__ smap,
                                                   float *__restrict__ fb)
 {
@@ -24,7 +24,8 @@ __global__ __launch_bounds__(256) void k_lighting(FrameUniforms u, const float *
     Vec3 em = vec3(g[12 * P + i], g[13 * P + i], g[14 * P + i]);
     Vec3 pa = vec3(g[15 * P + i], g[16 * P + i], g[17 * P + i]);
     float type = g[18 * P + i];
-    pa.x *= ao[i];
+    // AmbientOcclusionGBuffer starts at 1 (forkergl.cpp:80); x * 1.f == x.
+    if (!ao_is_one) pa.x *= ao[i];
     Vec3 c = shade_point(u, smap, !(type < 0.5f), true, Pw, N, L, al, em, pa);
     fb[i] = c.x;
     fb[P + i] = c.y;
@@ -167,13 +168,30 @@ __global__ __launch_bounds__(256) void k_resolve(const float *__restrict__ fb, i
     o[2] = (uint8_t)(int)((float)s[2] / k2);
 }
 
+// Plain 16-byte copy: the reference point for HBM byte counters.
+__global__ __launch_bounds__(256) void k_copy(const float4 *__restrict__ src, float4 *__restrict__ dst,
+                                              size_t n)
+{
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) dst[i] = src[i];
+}
+
+void launch_copy(const void *src, void *dst, size_t bytes, hipStream_t s)
+{
+    size_t n = bytes / 16;
+    if (n == 0) return;
+    hipLaunchKernelGGL(k_copy, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, (const float4 *)src,
+                       (float4 *)dst, n);
+}
+
 static inline dim3 grid2(int W, int H) { return dim3((W + 63) / 64, (H + 3) / 4); }
 
-void launch_lighting(const FrameUniforms &u, const float *gbuf, const float *ao,
+void launch_lighting(const FrameUniforms &u, const float *gbuf, const float *ao, int ao_is_one,
                      const float *shadow_map, float *fb, hipStream_t s)
 {
     if (u.W <= 0 || u.H <= 0) return;
-    hipLaunchKernelGGL(k_lighting, grid2(u.W, u.H), dim3(256), 0, s, u, gbuf, ao, shadow_map, fb);
+    hipLaunchKernelGGL(k_lighting, grid2(u.W, u.H), dim3(256), 0, s, u, gbuf, ao, ao_is_one,
+                       shadow_map, fb);
 }
 
 void launch_ssao(const FrameUniforms &u, const SsaoArgs &sa, const float *depth,

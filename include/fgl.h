@@ -130,6 +130,10 @@ int fgl_sync(fgl_ctx *ctx);
 /* Raster tuning: tile edge in pixels (8, 16 or 32).  Results do not depend
  * on it (tile ownership keeps submission order per pixel). */
 int fgl_set_tile_size(fgl_ctx *ctx, int tile);
+/* Per-pass tile edge (FGL_PASS_SHADOW, or FORWARD/GEOMETRY for the main pass);
+ * the autotuner (forkerrenderer_amd/autotune.py) sets these. */
+int fgl_set_pass_tile_size(fgl_ctx *ctx, int pass, int tile);
+int fgl_get_pass_tile_size(fgl_ctx *ctx, int pass, int *tile);
 
 /* ---- resources (replace Model::Load's in-memory products) ---- */
 /* Texture (src/texture.h:29-39): bpp 1/3/4, bytes in B,G,R[,A] order after
@@ -191,6 +195,11 @@ int fgl_target_bytes(fgl_ctx *ctx, int which, size_t *out_bytes);
  * milliseconds are written to the arrays (up to cap entries). */
 int fgl_enable_timing(fgl_ctx *ctx, int on);
 int fgl_read_timings(fgl_ctx *ctx, char (*names)[32], float *ms, int cap, int *out_n);
+
+/* Diagnostic with no counterpart in the reference: a plain device-to-device
+ * copy of `bytes` run `iters` times, used to calibrate HBM byte counters and
+ * bandwidth figures.  Allocates and frees its own buffers. */
+int fgl_copy_benchmark(fgl_ctx *ctx, size_t bytes, int iters, float *ms_per_iter);
 
 /* ---- scene files: .scene/.obj/.mtl/.tga (src/scene.cpp, src/model.cpp, src/tgaimage.cpp) ---- */
 typedef struct fgl_scene fgl_scene;

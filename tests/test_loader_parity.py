@@ -1,0 +1,132 @@
+"""Host formats: the product's C++ .scene/.obj/.mtl/.tga parsers against the
+oracle's independent Python restatement, array for array (bit-exact), on
+generated scenes and on the reference's own sample assets (when the reference
+tree is present; it is not on GPU machines)."""
+import os
+
+import numpy as np
+import pytest
+
+import forkerrenderer_amd as F
+from forkerrenderer_amd import scenegen as G
+from oracle import sceneload
+from oracle.objload import tga_read
+
+REF = "/root/reference/obj"
+
+
+def _cmp_models(prod, orc):
+    assert len(prod) == len(orc)
+    for (pm, pM), (om, oM) in zip(prod, orc):
+        for k in ("pos", "uv", "nrm", "vi", "ti", "ni", "face_material"):
+            a, b = np.asarray(pm[k]), np.asarray(om[k])
+            assert a.shape == b.shape, k
+            assert np.array_equal(a.view(np.uint32) if a.dtype == np.float32 else a,
+                                  b.view(np.uint32) if b.dtype == np.float32 else b), k
+        assert (pm["tan"] is None) == (om["tan"] is None)
+        if pm["tan"] is not None:
+            assert np.array_equal(pm["tan"].view(np.uint32), om["tan"].view(np.uint32)), "tangents"
+        assert pm["support_pbr"] == om["support_pbr"]
+        assert np.array_equal(np.asarray(pM, np.float32).view(np.uint32),
+                              np.asarray(oM, np.float32).view(np.uint32)), "model matrix"
+
+
+def _cmp_scene(path):
+    sc = F.Scene(path)
+    I = sc.info
+    prod_models = [sc.model(i) for i in range(I.n_models)]
+    o = sceneload.load_scene(path)
+    _cmp_models(prod_models, o["models"])
+    assert I.n_materials == len(o["materials"])
+    for i, om in enumerate(o["materials"]):
+        pm = sc.material(i)
+        for k, v in om.items():
+            pv = getattr(pm, k)
+            pv = list(pv) if hasattr(pv, "__len__") else pv
+            assert np.allclose(np.float32(pv), np.float32(v), rtol=0, atol=0), (i, k, pv, v)
+    assert I.n_textures == len(o["textures"])
+    for i, ot in enumerate(o["textures"]):
+        pt = sc.texture(i)
+        assert np.array_equal(pt["bytes"], ot["bytes"]), i
+        assert pt["wrap"] == ot["wrap"] and pt["filter"] == ot["filter"]
+    oi = o["info"]
+    assert (I.width, I.height, I.ssaa_on, I.ssaa_k, I.ssao_on, I.shadow_on, I.deferred) == \
+        (oi["width"], oi["height"], oi["ssaa_on"], oi["ssaa_k"], oi["ssao_on"], oi["shadow_on"], oi["deferred"])
+    assert list(I.eye) == oi["eye"] and list(I.look_at) == oi["look_at"]
+    assert list(I.lights[0].vec) + list(I.lights[0].color) == oi["point"]
+    return sc
+
+
+@pytest.mark.parametrize("seed", [0, 1, 2])
+def test_generated_tiny_scene_loader(scene_dir, seed):
+    p = G.tiny_scene(os.path.join(scene_dir, f"ld{seed}"), seed=seed, subdiv=2, tex=8, pbr=seed != 1)
+    _cmp_scene(p)
+
+
+def test_config1_loader(scene_dir):
+    p = G.config1(os.path.join(scene_dir, "c1"))
+    sc = _cmp_scene(p)
+    m, _ = sc.model(1)
+    assert len(m["vi"]) == 64 * 32 * 2 - 128  # UV sphere without pole degenerates
+
+
+def _ref_scene(d, models, extra=""):
+    lines = ["mode deferred", "screen 64 48", "light point 2 5 5 1 1 1", "camera persp -1 1 1 0 0 -1", extra]
+    for m, norm, gen in models:
+        lines.append(f"model {REF}/{m} {norm} {gen} 0 0 -1 30 1")
+    os.makedirs(d, exist_ok=True)
+    p = os.path.join(d, "ref.scene")
+    with open(p, "w") as fh:
+        fh.write("\n".join(lines) + "\n")
+    return p
+
+
+@pytest.mark.skipif(not os.path.isdir(REF), reason="reference assets not present on this machine")
+@pytest.mark.parametrize("models", [
+    [("plane/plane.obj", "false", "false")],
+    [("brickwall/brickwall.obj", "true", "true")],
+    [("great_sword/great_sword.obj", "true", "true")],   # quads: fan triangulation
+    [("mary/mary.obj", "true", "true")],                 # two groups, map order != file order
+    [("catbox/catbox.obj", "true", "false")],
+    [("african_head/head.obj", "true", "true"), ("african_head/eye_inner.obj", "true", "true")],
+], ids=lambda m: m[0][0].split("/")[0])
+def test_reference_assets_loader(scene_dir, models):
+    p = _ref_scene(os.path.join(scene_dir, "ref_" + models[0][0].split("/")[0]), models, "texture repeat linear")
+    _cmp_scene(p)
+
+
+@pytest.mark.skipif(not os.path.isdir(REF), reason="reference assets not present on this machine")
+def test_mary_mesh_order_is_map_order():
+    """The reference draws meshes in std::map (name) order, not file order."""
+    import re
+    txt = open(f"{REF}/mary/mary.obj").read()
+    names = re.findall(r"^g (\S+)", txt, flags=re.M)
+    assert names != sorted(names), "asset no longer exercises map order"
+
+
+def test_tga_roundtrip_and_rle_packets(tmp_path):
+    rng = np.random.default_rng(0)
+    img = rng.integers(0, 4, (37, 53, 3), dtype=np.uint8)  # few values -> runs and raw packets
+    img[5:9] = 7
+    for rle in (False, True):
+        p = str(tmp_path / f"t{int(rle)}.tga")
+        F.tga_write(p, img, rle=rle)
+        back = tga_read(p)   # oracle decoder, bottom-left origin -> flipped to top-first rows
+        assert np.array_equal(back[::-1], img)
+        raw = open(p, "rb").read()
+        assert raw[2] == (10 if rle else 2) and raw[17] == 0
+        assert raw[-18:] == b"TRUEVISION-XFILE.\0"
+    grey = rng.integers(0, 255, (9, 11), dtype=np.uint8)
+    p = str(tmp_path / "g.tga")
+    F.tga_write(p, grey, rle=True)
+    assert np.array_equal(tga_read(p)[::-1, :, 0], grey)
+
+
+def test_rle_packet_structure(tmp_path):
+    """RLE stream of a row with a run of 5 equal pixels then 3 distinct ones:
+    one run packet (0x80 | 4) and one raw packet of 3 (count-1 = 2)."""
+    row = np.array([[[1, 1, 1]] * 5 + [[2, 2, 2], [3, 3, 3], [4, 4, 4]]], np.uint8)
+    p = str(tmp_path / "r.tga")
+    F.tga_write(p, row, rle=True)
+    body = open(p, "rb").read()[18:-26]
+    assert body == bytes([0x80 | 4, 1, 1, 1, 2, 2, 2, 2, 3, 3, 3, 4, 4, 4])
