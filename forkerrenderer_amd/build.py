@@ -59,6 +59,15 @@ def build(verbose=False):
         print(r.stdout + r.stderr, file=sys.stderr)
         raise RuntimeError("link failed")
     os.replace(tmp, LIB)
+    # command-line renderer (the reference's ForkerRenderer <scene>)
+    exe = os.path.join(LIBDIR, "ForkerRendererAMD")
+    cmd = ["g++", "-O2", "-std=c++17", "-I", os.path.join(ROOT, "include"), os.path.join(CSRC, "main.cpp"),
+           "-L", LIBDIR, "-lforkergl", "-Wl,-rpath,$ORIGIN", "-o", exe + ".tmp"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        print(r.stdout + r.stderr, file=sys.stderr)
+        raise RuntimeError("CLI link failed")
+    os.replace(exe + ".tmp", exe)
     return LIB
 
 

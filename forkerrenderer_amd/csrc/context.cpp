@@ -369,6 +369,29 @@ int fgl_init_targets(fgl_ctx *c, int w, int h)
     return FGL_OK;
 }
 
+// One of InitFrameBuffer / InitDepthBuffer / InitGeometryBuffers
+// (forkergl.cpp:55-81).  The main targets share one size; initialising one
+// at a new size re-initialises all of them.
+int fgl_init_buffer(fgl_ctx *c, int which, int w, int h)
+{
+    if (!c) return FGL_ERR_INVALID;
+    if (which == FGL_TARGET_SHADOW) return fgl_init_shadow_buffer(c, w, h);
+    if (which != FGL_TARGET_FRAME && which != FGL_TARGET_DEPTH && which != FGL_TARGET_GBUFFER)
+        return fail(c, FGL_ERR_INVALID, "fgl_init_buffer: FRAME, DEPTH, GBUFFER or SHADOW");
+    if (!c->depth.p || w != c->W || h != c->H) return fgl_init_targets(c, w, h);
+    int rc = flush_pass(c);
+    if (rc) return rc;
+    if (which == FGL_TARGET_FRAME) {
+        c->pend_fb = true;
+        c->fb_clear[0] = c->fb_clear[1] = c->fb_clear[2] = 0.f;
+    } else if (which == FGL_TARGET_DEPTH) {
+        c->pend_depth = true;
+    } else {
+        c->pend_gbuf = c->pend_ao = true;
+    }
+    return FGL_OK;
+}
+
 int fgl_init_shadow_buffer(fgl_ctx *c, int w, int h)
 {
     if (!c) return FGL_ERR_INVALID;

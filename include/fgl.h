@@ -148,6 +148,9 @@ int fgl_upload_model(fgl_ctx *ctx, const fgl_model_desc *d, int *out_id);
 int fgl_init_targets(fgl_ctx *ctx, int w, int h);
 /* InitShadowBuffer (forkergl.cpp:65-68); the reference uses the frame size */
 int fgl_init_shadow_buffer(fgl_ctx *ctx, int w, int h);
+/* InitFrameBuffer / InitDepthBuffer / InitGeometryBuffers one at a time
+ * (forkergl.cpp:55-81): which = FGL_TARGET_FRAME, _DEPTH, _GBUFFER (with AO) or _SHADOW */
+int fgl_init_buffer(fgl_ctx *ctx, int which, int w, int h);
 /* ClearColor (forkergl.cpp:84-87) */
 int fgl_clear_color(fgl_ctx *ctx, float r, float g, float b);
 /* SetViewportMatrix (forkergl.cpp:89-102); shadow != 0 sets the shadow pass viewport */
