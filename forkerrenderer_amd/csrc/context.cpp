@@ -10,6 +10,7 @@
 #include <vector>
 
 #include "../../include/fgl.h"
+#include "ctx_internal.hpp"
 #include "fgl_kernels.hpp"
 #include "host_io.hpp"
 
@@ -943,6 +944,37 @@ int fgl_copy_benchmark(fgl_ctx *c, size_t bytes, int iters, float *ms_per_iter)
     hipFree(b);
     return FGL_OK;
 }
+
+}  // extern "C"
+
+namespace fgl {
+
+int ctx_view(fgl_ctx *c, CtxView *v)
+{
+    int rc = flush_pass(c);
+    if (rc) return rc;
+    if (!c->depth.p) return fail(c, FGL_ERR_INVALID, "targets not initialised");
+    if ((rc = materialize(c, true, true, false, true, false))) return rc;
+    v->stream = c->stream;
+    v->W = c->W;
+    v->H = c->H;
+    v->depth = (float *)c->depth.p;
+    v->gbuf = (float *)c->gbuf.p;
+    v->ao = (float *)c->ao.p;
+    v->fb = (float *)c->fb.p;
+    std::memcpy(v->view, c->view, sizeof(v->view));
+    std::memcpy(v->proj, c->proj, sizeof(v->proj));
+    std::memcpy(v->viewport, c->viewport, sizeof(v->viewport));
+    return FGL_OK;
+}
+
+void ctx_ao_written(fgl_ctx *c) { c->pend_ao = false; }
+
+int ctx_fail(fgl_ctx *c, int code, const char *msg) { return fail(c, code, msg); }
+
+}  // namespace fgl
+
+extern "C" {
 
 // ---------------- scenes ----------------
 

@@ -1,2 +1,685 @@
-// nao.hip -- neural ambient occlusion (placeholder translation unit).
-#include "fgl_kernels.hpp"
+// nao.hip -- neural ambient occlusion for gfx950: a small convolutional
+// network that stands in for SSAO hemisphere sampling plus blur.
+//
+// Network (per pixel, 'same' 3x3 convolutions with zero padding):
+//   in  7 features (view depth, view normal xyz, view position xyz),
+//       stored as 16 BF16 channels (9 zero)
+//   L1  conv3x3 16->32 + ReLU      L2..L4 conv3x3 32->32 + ReLU
+//   L5  conv1x1 32->1 + sigmoid
+// Convolutions are implicit GEMMs on MFMA (v_mfma_f32_16x16x16_bf16):
+// M = pixels, N = 32 output channels, K = 9 taps x Cin, BF16 operands, FP32
+// accumulation, BF16 NHWC activations.  Backward: the data gradient is the
+// same kernel with tap-flipped, transposed weights and a ReLU-mask epilogue;
+// the weight gradient is an MFMA GEMM over pixels (co x pixels times
+// pixels x (tap, ci)) with per-block partial sums reduced in a fixed order
+// (bitwise reproducible training, no atomics).  Adam keeps FP32 master
+// weights.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "ctx_internal.hpp"
+#include "fgl_math.hpp"
+
+namespace fgl {
+namespace nao {
+
+typedef short v4s __attribute__((ext_vector_type(4)));
+typedef float v4f __attribute__((ext_vector_type(4)));
+
+constexpr int C0 = 16;    // stored input channels (7 real)
+constexpr int CH = 32;    // hidden channels
+constexpr int K1 = 9 * C0, K2 = 9 * CH;
+// flat FP32 parameter layout
+constexpr int OFF_W1 = 0, OFF_B1 = OFF_W1 + CH * K1;
+constexpr int OFF_W2 = OFF_B1 + CH, OFF_B2 = OFF_W2 + CH * K2;
+constexpr int OFF_W3 = OFF_B2 + CH, OFF_B3 = OFF_W3 + CH * K2;
+constexpr int OFF_W4 = OFF_B3 + CH, OFF_B4 = OFF_W4 + CH * K2;
+constexpr int OFF_W5 = OFF_B4 + CH, OFF_B5 = OFF_W5 + CH;
+constexpr int NPARAM = OFF_B5 + 1;
+constexpr int WG_ROWS = 16;   // rows of pixels per weight-gradient workgroup
+
+__host__ __device__ __forceinline__ uint16_t f2bf(float f)
+{
+    uint32_t u;
+    std::memcpy(&u, &f, 4);
+    u += 0x7FFFu + ((u >> 16) & 1u);
+    return (uint16_t)(u >> 16);
+}
+__host__ __device__ __forceinline__ float bf2f(uint16_t h)
+{
+    uint32_t u = (uint32_t)h << 16;
+    float f;
+    std::memcpy(&f, &u, 4);
+    return f;
+}
+
+// ---------------- features ----------------
+
+// View-space features of one G-buffer pixel (zeros where nothing was drawn):
+// depth d = -z_view / 10, normal = V3 * N, position = V * (P,1) / 10.
+__device__ __forceinline__ void pixel_features(const float *V, const float *depth, const float *g,
+                                               size_t P, size_t i, float f[7], bool &covered)
+{
+    covered = depth[i] < 3.0e38f;
+    if (!covered) {
+        for (int c = 0; c < 7; ++c) f[c] = 0.f;
+        return;
+    }
+    Vec3 N = vec3(g[i], g[P + i], g[2 * P + i]);
+    Vec3 Pw = vec3(g[3 * P + i], g[4 * P + i], g[5 * P + i]);
+    Vec4 vp = mv4(V, vec4(Pw.x, Pw.y, Pw.z, 1.f));
+    Vec3 vn = vec3(dot3(vec3(V[0], V[1], V[2]), N), dot3(vec3(V[4], V[5], V[6]), N),
+                   dot3(vec3(V[8], V[9], V[10]), N));
+    f[0] = -vp.z * 0.1f;
+    f[1] = vn.x; f[2] = vn.y; f[3] = vn.z;
+    f[4] = vp.x * 0.1f; f[5] = vp.y * 0.1f; f[6] = vp.z * 0.1f;
+}
+
+struct Mat16 {
+    float m[16];
+};
+
+// Full frame -> features [H][W][16] (BF16) + coverage mask.
+__global__ __launch_bounds__(256) void k_features(Mat16 V, const float *__restrict__ depth,
+                                                  const float *__restrict__ g, int W, int H,
+                                                  uint16_t *__restrict__ feat,
+                                                  uint8_t *__restrict__ mask)
+{
+    const int x = blockIdx.x * 64 + (threadIdx.x & 63), y = blockIdx.y * 4 + (threadIdx.x >> 6);
+    if (x >= W || y >= H) return;
+    const size_t P = (size_t)W * H, i = (size_t)x + (size_t)y * W;
+    float f[7];
+    bool cov;
+    pixel_features(V.m, depth, g, P, i, f, cov);
+    uint16_t *o = feat + i * C0;
+    for (int c = 0; c < C0; ++c) o[c] = c < 7 ? f2bf(f[c]) : (uint16_t)0;
+    if (mask) mask[i] = cov ? 1 : 0;
+}
+
+// One crop of the current frame into batch slot `slot`: features, AO target,
+// mask.  Crop origin (x0, y0), size S x S, inside the frame.
+__global__ __launch_bounds__(256) void k_crop(Mat16 V, const float *__restrict__ depth,
+                                              const float *__restrict__ g,
+                                              const float *__restrict__ ao, int W, int H, int x0,
+                                              int y0, int S, int slot,
+                                              uint16_t *__restrict__ feat,
+                                              float *__restrict__ target,
+                                              uint8_t *__restrict__ mask)
+{
+    const int cx = blockIdx.x * 64 + (threadIdx.x & 63), cy = blockIdx.y * 4 + (threadIdx.x >> 6);
+    if (cx >= S || cy >= S) return;
+    const size_t P = (size_t)W * H, i = (size_t)(x0 + cx) + (size_t)(y0 + cy) * W;
+    const size_t o = (size_t)slot * S * S + (size_t)cy * S + cx;
+    float f[7];
+    bool cov;
+    pixel_features(V.m, depth, g, P, i, f, cov);
+    for (int c = 0; c < C0; ++c) feat[o * C0 + c] = c < 7 ? f2bf(f[c]) : (uint16_t)0;
+    target[o] = ao[i];
+    mask[o] = cov ? 1 : 0;
+}
+
+// ---------------- 3x3 convolution on MFMA ----------------
+
+// MODE 0: out = bf16(relu(conv(in) + bias)).
+// MODE 1 (data gradient): out = bf16(conv(in) * (act > 0)) with tap-flipped,
+// transposed weights; `in` is the layer's output gradient.
+// Workgroup: 4 wavefronts, each 64 consecutive pixels of one row x 32 channels.
+template <int CIN, int MODE>
+__global__ __launch_bounds__(256) void k_conv3x3(const uint16_t *__restrict__ in,
+                                                 const uint16_t *__restrict__ Wt,
+                                                 const float *__restrict__ bias,
+                                                 const uint16_t *__restrict__ act,
+                                                 uint16_t *__restrict__ out, int W, int H)
+{
+    constexpr int K = 9 * CIN, KS = K / 16, CG = CIN / 16;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int y = blockIdx.y * 4 + wave, n = blockIdx.z, xb = blockIdx.x * 64;
+    if (y >= H) return;
+    const int lr = lane & 15, lq = lane >> 4;
+    v4s bfr[KS][2];
+#pragma unroll
+    for (int ks = 0; ks < KS; ++ks)
+#pragma unroll
+        for (int nb = 0; nb < 2; ++nb)
+            bfr[ks][nb] = *(const v4s *)(Wt + (size_t)(nb * 16 + lr) * K + ks * 16 + lq * 4);
+    v4f acc[4][2];
+#pragma unroll
+    for (int mb = 0; mb < 4; ++mb)
+#pragma unroll
+        for (int nb = 0; nb < 2; ++nb) acc[mb][nb] = v4f{0.f, 0.f, 0.f, 0.f};
+    const size_t img = (size_t)n * H * W;
+#pragma unroll
+    for (int ks = 0; ks < KS; ++ks) {
+        const int tap = ks / CG, cg = ks % CG;
+        const int yy = y + tap / 3 - 1;
+        const bool rowok = yy >= 0 && yy < H;
+#pragma unroll
+        for (int mb = 0; mb < 4; ++mb) {
+            const int xx = xb + mb * 16 + lr + tap % 3 - 1;
+            v4s a = v4s{0, 0, 0, 0};
+            if (rowok && xx >= 0 && xx < W)
+                a = *(const v4s *)(in + (img + (size_t)yy * W + xx) * CIN + cg * 16 + lq * 4);
+            acc[mb][0] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, bfr[ks][0], acc[mb][0], 0, 0, 0);
+            acc[mb][1] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, bfr[ks][1], acc[mb][1], 0, 0, 0);
+        }
+    }
+#pragma unroll
+    for (int mb = 0; mb < 4; ++mb)
+#pragma unroll
+        for (int nb = 0; nb < 2; ++nb)
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                const int px = xb + mb * 16 + lq * 4 + i;
+                if (px >= W) continue;
+                const int co = nb * 16 + lr;
+                const size_t o = (img + (size_t)y * W + px) * CH + co;
+                float v = acc[mb][nb][i];
+                if (MODE == 0) {
+                    v += bias[co];
+                    v = v > 0.f ? v : 0.f;
+                } else {
+                    v = act[o] != 0 ? v : 0.f;
+                }
+                out[o] = f2bf(v);
+            }
+}
+
+// Weight gradient: dW[co][tap*CIN+ci] = sum_p dout[p][co] * in[p+off(tap)][ci]
+// and db[co] = sum_p dout[p][co], per workgroup over a 64 x WG_ROWS pixel
+// tile, written as partials [block][...] for a fixed-order reduction.
+template <int CIN>
+__global__ __launch_bounds__(256) void k_conv_wgrad(const uint16_t *__restrict__ in,
+                                                    const uint16_t *__restrict__ dout,
+                                                    float *__restrict__ part_dw,
+                                                    float *__restrict__ part_db, int W, int H)
+{
+    constexpr int K = 9 * CIN, NB = K / 16;
+    constexpr int NBW = (NB + 3) / 4;  // n-blocks per wavefront
+    __shared__ uint16_t s_out[4 * 64 * CH];
+    __shared__ uint16_t s_in[6 * 66 * CIN];
+    __shared__ float s_db[4][CH];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, lr = lane & 15, lq = lane >> 4;
+    const int xb = blockIdx.x * 64, yb = blockIdx.y * WG_ROWS, n = blockIdx.z;
+    const size_t img = (size_t)n * H * W;
+    v4f acc[2][NBW];
+#pragma unroll
+    for (int mb = 0; mb < 2; ++mb)
+#pragma unroll
+        for (int j = 0; j < NBW; ++j) acc[mb][j] = v4f{0.f, 0.f, 0.f, 0.f};
+    float dbacc = 0.f;
+    for (int r0 = 0; r0 < WG_ROWS; r0 += 4) {
+        const int y0 = yb + r0;
+        // stage output gradients: 4 rows x 64 px x 32 ch
+        for (int e = tid; e < 4 * 64 * CH / 4; e += 256) {
+            const int pix = e / (CH / 4), c4 = e % (CH / 4);
+            const int yy = y0 + pix / 64, xx = xb + pix % 64;
+            uint64_t v = 0;
+            if (yy < H && xx < W) v = *(const uint64_t *)(dout + (img + (size_t)yy * W + xx) * CH + c4 * 4);
+            *(uint64_t *)(s_out + pix * CH + c4 * 4) = v;
+        }
+        // stage inputs with a one-pixel halo: 6 rows x 66 px x CIN
+        for (int e = tid; e < 6 * 66 * CIN / 4; e += 256) {
+            const int pix = e / (CIN / 4), c4 = e % (CIN / 4);
+            const int yy = y0 - 1 + pix / 66, xx = xb - 1 + pix % 66;
+            uint64_t v = 0;
+            if (yy >= 0 && yy < H && xx >= 0 && xx < W)
+                v = *(const uint64_t *)(in + (img + (size_t)yy * W + xx) * CIN + c4 * 4);
+            *(uint64_t *)(s_in + pix * CIN + c4 * 4) = v;
+        }
+        __syncthreads();
+        if (tid < CH)
+            for (int p = 0; p < 4 * 64; ++p) dbacc += bf2f(s_out[p * CH + tid]);
+        // K dimension of this GEMM = 256 pixels = 16 steps of 16
+        for (int ps = 0; ps < 16; ++ps) {
+            v4s a[2];
+#pragma unroll
+            for (int mb = 0; mb < 2; ++mb)
+#pragma unroll
+                for (int j = 0; j < 4; ++j)
+                    a[mb][j] = (short)s_out[(ps * 16 + lq * 4 + j) * CH + mb * 16 + lr];
+#pragma unroll
+            for (int jb = 0; jb < NBW; ++jb) {
+                const int nb = wave + 4 * jb;
+                if (nb >= NB) break;
+                const int k = nb * 16 + lr, tap = k / CIN, ci = k % CIN;
+                const int dy = tap / 3, dx = tap % 3;  // halo-shifted offsets
+                v4s b;
+#pragma unroll
+                for (int j = 0; j < 4; ++j) {
+                    const int pix = ps * 16 + lq * 4 + j;
+                    const int pr = pix / 64, pc = pix % 64;
+                    b[j] = (short)s_in[((pr + dy) * 66 + pc + dx) * CIN + ci];
+                }
+#pragma unroll
+                for (int mb = 0; mb < 2; ++mb)
+                    acc[mb][jb] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a[mb], b, acc[mb][jb], 0, 0, 0);
+            }
+        }
+        __syncthreads();
+    }
+    const size_t blk = ((size_t)n * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x;
+    float *pd = part_dw + blk * CH * K;
+#pragma unroll
+    for (int jb = 0; jb < NBW; ++jb) {
+        const int nb = wave + 4 * jb;
+        if (nb >= NB) break;
+#pragma unroll
+        for (int mb = 0; mb < 2; ++mb)
+#pragma unroll
+            for (int i = 0; i < 4; ++i) pd[(size_t)(mb * 16 + lq * 4 + i) * K + nb * 16 + lr] = acc[mb][jb][i];
+    }
+    if (tid < CH) part_db[blk * CH + tid] = dbacc;
+    (void)s_db;
+}
+
+// out[i] = sum over b of part[b*len + i], b ascending (fixed order).
+__global__ __launch_bounds__(256) void k_reduce_cols(const float *__restrict__ part, int nb, int len,
+                                                     float *__restrict__ out)
+{
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= len) return;
+    float s = 0.f;
+    for (int b = 0; b < nb; ++b) s += part[(size_t)b * len + i];
+    out[i] = s;
+}
+
+// ---------------- head: 1x1 conv 32->1 + sigmoid ----------------
+
+__device__ __forceinline__ float head_z(const float *w5, float b5, const uint16_t *a)
+{
+    float z = b5;
+    for (int c = 0; c < CH; ++c) z += w5[c] * bf2f(a[c]);
+    return z;
+}
+
+__global__ __launch_bounds__(256) void k_head_fwd(const float *__restrict__ params,
+                                                  const uint16_t *__restrict__ a4,
+                                                  const uint8_t *__restrict__ mask, size_t n,
+                                                  float *__restrict__ y)
+{
+    const size_t p = (size_t)blockIdx.x * 256 + threadIdx.x;
+    if (p >= n) return;
+    const float z = head_z(params + OFF_W5, params[OFF_B5], a4 + p * CH);
+    float v = 1.f / (1.f + expf(-z));
+    // background pixels keep AO = 1 (AmbientOcclusionGBuffer's initial value)
+    if (mask && !mask[p]) v = 1.f;
+    y[p] = v;
+}
+
+// Loss = sum over covered pixels of (y - t)^2 / n_total.  Writes the data
+// gradient of a4 and per-block partials of [dW5 (32), db5, loss, |err|, count].
+constexpr int HEAD_PART = CH + 4;
+__global__ __launch_bounds__(256) void k_head_bwd(const float *__restrict__ params,
+                                                  const uint16_t *__restrict__ a4,
+                                                  const float *__restrict__ target,
+                                                  const uint8_t *__restrict__ mask, size_t n,
+                                                  uint16_t *__restrict__ da4,
+                                                  float *__restrict__ part)
+{
+    __shared__ float red[256][HEAD_PART + 1];
+    const size_t p = (size_t)blockIdx.x * 256 + threadIdx.x;
+    float loc[HEAD_PART];
+    for (int k = 0; k < HEAD_PART; ++k) loc[k] = 0.f;
+    if (p < n) {
+        const uint16_t *a = a4 + p * CH;
+        const float *w5 = params + OFF_W5;
+        const float z = head_z(w5, params[OFF_B5], a);
+        const float y = 1.f / (1.f + expf(-z));
+        const bool m = mask[p] != 0;
+        const float diff = y - target[p];
+        const float g = m ? 2.f * diff / (float)n : 0.f;
+        const float dz = g * y * (1.f - y);
+        for (int c = 0; c < CH; ++c) {
+            const float av = bf2f(a[c]);
+            da4[p * CH + c] = f2bf(av > 0.f ? dz * w5[c] : 0.f);
+            loc[c] = dz * av;
+        }
+        loc[CH] = dz;
+        loc[CH + 1] = m ? diff * diff : 0.f;
+        loc[CH + 2] = m ? fabsf(diff) : 0.f;
+        loc[CH + 3] = m ? 1.f : 0.f;
+    }
+    for (int k = 0; k < HEAD_PART; ++k) red[threadIdx.x][k] = loc[k];
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s)
+            for (int k = 0; k < HEAD_PART; ++k) red[threadIdx.x][k] += red[threadIdx.x + s][k];
+        __syncthreads();
+    }
+    if (threadIdx.x < HEAD_PART) part[(size_t)blockIdx.x * HEAD_PART + threadIdx.x] = red[0][threadIdx.x];
+}
+
+// ---------------- weights, Adam ----------------
+
+// FP32 master -> BF16 forward layouts [co][tap*Cin+ci] and the data-gradient
+// layouts [ci][tap'*32+co] with tap' = 8 - tap (layers 2..4).
+__global__ __launch_bounds__(256) void k_prep_weights(const float *__restrict__ params,
+                                                      uint16_t *__restrict__ w1,
+                                                      uint16_t *__restrict__ w234,
+                                                      uint16_t *__restrict__ wt234)
+{
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i < CH * K1) w1[i] = f2bf(params[OFF_W1 + i]);
+    if (i < 3 * CH * K2) {
+        const int l = i / (CH * K2), r = i % (CH * K2);
+        const int offs[3] = {OFF_W2, OFF_W3, OFF_W4};
+        const float v = params[offs[l] + r];
+        w234[i] = f2bf(v);
+        const int co = r / K2, k = r % K2, tap = k / CH, ci = k % CH;
+        wt234[(size_t)l * CH * K2 + (size_t)ci * K2 + (8 - tap) * CH + co] = f2bf(v);
+    }
+}
+
+struct AdamArgs {
+    float lr, b1, b2, eps, c1, c2;  // c1 = 1 - b1^t, c2 = 1 - b2^t
+};
+
+__global__ __launch_bounds__(256) void k_adam(float *__restrict__ w, const float *__restrict__ g,
+                                              float *__restrict__ m, float *__restrict__ v, int n,
+                                              AdamArgs a)
+{
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    const float gi = g[i];
+    const float mi = a.b1 * m[i] + (1.f - a.b1) * gi;
+    const float vi = a.b2 * v[i] + (1.f - a.b2) * gi * gi;
+    m[i] = mi;
+    v[i] = vi;
+    const float mh = mi / a.c1, vh = vi / a.c2;
+    w[i] -= a.lr * mh / (sqrtf(vh) + a.eps);
+}
+
+}  // namespace nao
+}  // namespace fgl
+
+// ======================= host object and C ABI =======================
+
+using namespace fgl;
+using namespace fgl::nao;
+
+struct fgl_nao {
+    fgl_ctx *ctx = nullptr;
+    int batch = 0, crop = 0;
+    int step = 0;
+    // parameters (FP32 master), gradients, Adam moments
+    float *params = nullptr, *grads = nullptr, *m = nullptr, *v = nullptr;
+    uint16_t *w1 = nullptr, *w234 = nullptr, *wt234 = nullptr;
+    // training batch
+    uint16_t *feat = nullptr, *a[4] = {nullptr, nullptr, nullptr, nullptr};
+    uint16_t *d[4] = {nullptr, nullptr, nullptr, nullptr};
+    float *target = nullptr, *y = nullptr;
+    uint8_t *mask = nullptr;
+    float *part = nullptr, *part_db = nullptr, *head_part = nullptr, *head_red = nullptr;
+    size_t part_cap = 0;
+    // inference buffers (grow-only)
+    uint16_t *ifeat = nullptr, *ia = nullptr, *ib = nullptr;
+    uint8_t *imask = nullptr;
+    size_t icap = 0;
+    float *h_stats = nullptr;
+};
+
+static int nao_alloc(void **p, size_t bytes)
+{
+    return hipMalloc(p, bytes < 256 ? 256 : bytes) == hipSuccess ? FGL_OK : FGL_ERR_NOMEM;
+}
+
+static inline dim3 g2(int W, int H) { return dim3((W + 63) / 64, (H + 3) / 4); }
+
+extern "C" {
+
+int fgl_nao_param_count(void) { return NPARAM; }
+
+int fgl_nao_create(fgl_ctx *ctx, int batch, int crop, fgl_nao **out)
+{
+    if (!ctx || !out || batch < 1 || crop < 16 || crop % 64 != 0 || crop % WG_ROWS != 0)
+        return FGL_ERR_INVALID;
+    fgl_nao *n = new fgl_nao();
+    n->ctx = ctx;
+    n->batch = batch;
+    n->crop = crop;
+    const size_t px = (size_t)batch * crop * crop;
+    int rc = FGL_OK;
+    rc |= nao_alloc((void **)&n->params, NPARAM * 4);
+    rc |= nao_alloc((void **)&n->grads, NPARAM * 4);
+    rc |= nao_alloc((void **)&n->m, NPARAM * 4);
+    rc |= nao_alloc((void **)&n->v, NPARAM * 4);
+    rc |= nao_alloc((void **)&n->w1, CH * K1 * 2);
+    rc |= nao_alloc((void **)&n->w234, 3 * CH * K2 * 2);
+    rc |= nao_alloc((void **)&n->wt234, 3 * CH * K2 * 2);
+    rc |= nao_alloc((void **)&n->feat, px * C0 * 2);
+    for (int l = 0; l < 4; ++l) {
+        rc |= nao_alloc((void **)&n->a[l], px * CH * 2);
+        rc |= nao_alloc((void **)&n->d[l], px * CH * 2);
+    }
+    rc |= nao_alloc((void **)&n->target, px * 4);
+    rc |= nao_alloc((void **)&n->y, px * 4);
+    rc |= nao_alloc((void **)&n->mask, px);
+    const size_t nblk = (size_t)(crop / 64) * (crop / WG_ROWS) * batch;
+    n->part_cap = nblk;
+    rc |= nao_alloc((void **)&n->part, nblk * CH * K2 * 4);
+    rc |= nao_alloc((void **)&n->part_db, nblk * CH * 4);
+    const size_t hb = (px + 255) / 256;
+    rc |= nao_alloc((void **)&n->head_part, hb * HEAD_PART * 4);
+    rc |= nao_alloc((void **)&n->head_red, HEAD_PART * 4);
+    if (hipHostMalloc((void **)&n->h_stats, 64 * 4, hipHostMallocDefault) != hipSuccess) rc = FGL_ERR_NOMEM;
+    if (rc != FGL_OK) {
+        delete n;
+        return FGL_ERR_NOMEM;
+    }
+    hipMemset(n->m, 0, NPARAM * 4);
+    hipMemset(n->v, 0, NPARAM * 4);
+    hipMemset(n->params, 0, NPARAM * 4);
+    hipDeviceSynchronize();
+    *out = n;
+    return FGL_OK;
+}
+
+void fgl_nao_destroy(fgl_nao *n)
+{
+    if (!n) return;
+    hipDeviceSynchronize();
+    void *ps[] = {n->params, n->grads, n->m, n->v, n->w1, n->w234, n->wt234, n->feat, n->target, n->y,
+                  n->mask, n->part, n->part_db, n->head_part, n->head_red, n->ifeat, n->ia, n->ib,
+                  n->imask};
+    for (void *p : ps)
+        if (p) hipFree(p);
+    for (int l = 0; l < 4; ++l) {
+        hipFree(n->a[l]);
+        hipFree(n->d[l]);
+    }
+    if (n->h_stats) hipHostFree(n->h_stats);
+    delete n;
+}
+
+static int prep(fgl_nao *n, hipStream_t s)
+{
+    const int total = 3 * CH * K2;
+    hipLaunchKernelGGL(k_prep_weights, dim3((total + 255) / 256), dim3(256), 0, s, n->params, n->w1,
+                       n->w234, n->wt234);
+    return hipGetLastError() == hipSuccess ? FGL_OK : FGL_ERR_DEVICE;
+}
+
+// Parameters: FP32 master weights in the flat layout of fgl_nao_param_count();
+// state = [params | adam m | adam v] (3 * count floats) + step.
+int fgl_nao_set_state(fgl_nao *n, const float *state, int with_adam, int step)
+{
+    if (!n || !state) return FGL_ERR_INVALID;
+    if (hipMemcpy(n->params, state, NPARAM * 4, hipMemcpyHostToDevice) != hipSuccess) return FGL_ERR_DEVICE;
+    if (with_adam) {
+        hipMemcpy(n->m, state + NPARAM, NPARAM * 4, hipMemcpyHostToDevice);
+        hipMemcpy(n->v, state + 2 * NPARAM, NPARAM * 4, hipMemcpyHostToDevice);
+        n->step = step;
+    } else {
+        hipMemset(n->m, 0, NPARAM * 4);
+        hipMemset(n->v, 0, NPARAM * 4);
+        n->step = 0;
+    }
+    hipDeviceSynchronize();
+    return prep(n, 0) == FGL_OK && hipDeviceSynchronize() == hipSuccess ? FGL_OK : FGL_ERR_DEVICE;
+}
+
+int fgl_nao_get_state(fgl_nao *n, float *state, int *step)
+{
+    if (!n || !state) return FGL_ERR_INVALID;
+    CtxView cv;
+    if (ctx_view(n->ctx, &cv) == FGL_OK) hipStreamSynchronize(cv.stream);
+    hipMemcpy(state, n->params, NPARAM * 4, hipMemcpyDeviceToHost);
+    hipMemcpy(state + NPARAM, n->m, NPARAM * 4, hipMemcpyDeviceToHost);
+    hipMemcpy(state + 2 * NPARAM, n->v, NPARAM * 4, hipMemcpyDeviceToHost);
+    if (step) *step = n->step;
+    return FGL_OK;
+}
+
+// Copies crops of the context's current frame (depth, G-buffer, AO as the
+// target) into batch slots slot0 .. slot0+count-1; xy holds crop origins.
+int fgl_nao_stage_crops(fgl_nao *n, int slot0, int count, const int *xy)
+{
+    if (!n || !xy || slot0 < 0 || count < 0 || slot0 + count > n->batch) return FGL_ERR_INVALID;
+    CtxView cv;
+    int rc = ctx_view(n->ctx, &cv);
+    if (rc) return rc;
+    Mat16 V;
+    std::memcpy(V.m, cv.view, sizeof(V.m));
+    for (int i = 0; i < count; ++i) {
+        const int x0 = xy[2 * i], y0 = xy[2 * i + 1];
+        if (x0 < 0 || y0 < 0 || x0 + n->crop > cv.W || y0 + n->crop > cv.H)
+            return ctx_fail(n->ctx, FGL_ERR_INVALID, "crop outside the frame");
+        hipLaunchKernelGGL(k_crop, g2(n->crop, n->crop), dim3(256), 0, cv.stream, V, cv.depth, cv.gbuf, cv.ao,
+                           cv.W, cv.H, x0, y0, n->crop, slot0 + i, n->feat, n->target, n->mask);
+    }
+    return hipGetLastError() == hipSuccess ? FGL_OK : FGL_ERR_DEVICE;
+}
+
+// Forward of conv layers 1..4 on images [N][H][W] in place of the activations.
+static void forward_convs(fgl_nao *n, hipStream_t s, const uint16_t *feat, uint16_t **acts, int W, int H,
+                          int N)
+{
+    dim3 grid((W + 63) / 64, (H + 3) / 4, N);
+    hipLaunchKernelGGL((k_conv3x3<C0, 0>), grid, dim3(256), 0, s, feat, n->w1, n->params + OFF_B1,
+                       (const uint16_t *)nullptr, acts[0], W, H);
+    const int boffs[3] = {OFF_B2, OFF_B3, OFF_B4};
+    for (int l = 0; l < 3; ++l)
+        hipLaunchKernelGGL((k_conv3x3<CH, 0>), grid, dim3(256), 0, s, acts[l], n->w234 + (size_t)l * CH * K2,
+                           n->params + boffs[l], (const uint16_t *)nullptr, acts[l + 1], W, H);
+}
+
+static int run_batch(fgl_nao *n, bool update, float lr, float *stats)
+{
+    CtxView cv;
+    int rc = ctx_view(n->ctx, &cv);
+    if (rc) return rc;
+    hipStream_t s = cv.stream;
+    const int S = n->crop, N = n->batch;
+    const size_t px = (size_t)N * S * S;
+    forward_convs(n, s, n->feat, n->a, S, S, N);
+    // head forward + loss + data gradient of a4
+    const int hb = (int)((px + 255) / 256);
+    hipLaunchKernelGGL(k_head_bwd, dim3(hb), dim3(256), 0, s, n->params, n->a[3], n->target, n->mask, px,
+                       n->d[3], n->head_part);
+    hipLaunchKernelGGL(k_reduce_cols, dim3(1), dim3(256), 0, s, n->head_part, hb, HEAD_PART, n->head_red);
+    hipMemcpyAsync(n->grads + OFF_W5, n->head_red, CH * 4, hipMemcpyDeviceToDevice, s);
+    hipMemcpyAsync(n->grads + OFF_B5, n->head_red + CH, 4, hipMemcpyDeviceToDevice, s);
+    hipMemcpyAsync(n->h_stats, n->head_red + CH + 1, 3 * 4, hipMemcpyDeviceToHost, s);
+    // layers 4..1: weight gradient from (input, output gradient), then the
+    // data gradient of the layer input (not needed for layer 1)
+    const dim3 wg(S / 64, S / WG_ROWS, N);
+    const int nblk = (int)(wg.x * wg.y * wg.z);
+    const int woffs[4] = {OFF_W1, OFF_W2, OFF_W3, OFF_W4}, boffs[4] = {OFF_B1, OFF_B2, OFF_B3, OFF_B4};
+    const dim3 cgrid((S + 63) / 64, (S + 3) / 4, N);
+    for (int l = 3; l >= 0; --l) {
+        const int K = l == 0 ? K1 : K2;
+        if (l == 0)
+            hipLaunchKernelGGL((k_conv_wgrad<C0>), wg, dim3(256), 0, s, n->feat, n->d[0], n->part, n->part_db, S, S);
+        else
+            hipLaunchKernelGGL((k_conv_wgrad<CH>), wg, dim3(256), 0, s, n->a[l - 1], n->d[l], n->part, n->part_db, S, S);
+        hipLaunchKernelGGL(k_reduce_cols, dim3((CH * K + 255) / 256), dim3(256), 0, s, n->part, nblk, CH * K,
+                           n->grads + woffs[l]);
+        hipLaunchKernelGGL(k_reduce_cols, dim3(1), dim3(256), 0, s, n->part_db, nblk, CH, n->grads + boffs[l]);
+        if (l > 0)
+            hipLaunchKernelGGL((k_conv3x3<CH, 1>), cgrid, dim3(256), 0, s, n->d[l],
+                               n->wt234 + (size_t)(l - 1) * CH * K2, (const float *)nullptr, n->a[l - 1],
+                               n->d[l - 1], S, S);
+    }
+    if (update) {
+        n->step += 1;
+        AdamArgs a;
+        a.lr = lr;
+        a.b1 = 0.9f;
+        a.b2 = 0.999f;
+        a.eps = 1e-8f;
+        a.c1 = (float)(1.0 - std::pow(0.9, n->step));
+        a.c2 = (float)(1.0 - std::pow(0.999, n->step));
+        hipLaunchKernelGGL(k_adam, dim3((NPARAM + 255) / 256), dim3(256), 0, s, n->params, n->grads, n->m, n->v,
+                           NPARAM, a);
+        if ((rc = prep(n, s))) return rc;
+    }
+    if (hipStreamSynchronize(s) != hipSuccess) return FGL_ERR_DEVICE;
+    if (stats) {
+        stats[0] = n->h_stats[0];  // sum of squared errors over covered pixels
+        stats[1] = n->h_stats[1];  // sum of |errors|
+        stats[2] = n->h_stats[2];  // covered pixel count
+    }
+    return FGL_OK;
+}
+
+// One Adam step on the staged batch.  stats: [sum sq err, sum abs err, covered].
+int fgl_nao_train_step(fgl_nao *n, float lr, float *stats)
+{
+    if (!n) return FGL_ERR_INVALID;
+    return run_batch(n, true, lr, stats);
+}
+
+// Gradients of the staged batch without updating (for gradient checks).
+int fgl_nao_gradients(fgl_nao *n, float *grads, float *stats)
+{
+    if (!n || !grads) return FGL_ERR_INVALID;
+    int rc = run_batch(n, false, 0.f, stats);
+    if (rc) return rc;
+    return hipMemcpy(grads, n->grads, NPARAM * 4, hipMemcpyDeviceToHost) == hipSuccess ? FGL_OK : FGL_ERR_DEVICE;
+}
+
+// Copies the staged batch to the host: features (N*S*S*16 BF16), target, mask.
+int fgl_nao_read_batch(fgl_nao *n, uint16_t *feat, float *target, uint8_t *mask)
+{
+    if (!n) return FGL_ERR_INVALID;
+    hipDeviceSynchronize();
+    const size_t px = (size_t)n->batch * n->crop * n->crop;
+    if (feat) hipMemcpy(feat, n->feat, px * C0 * 2, hipMemcpyDeviceToHost);
+    if (target) hipMemcpy(target, n->target, px * 4, hipMemcpyDeviceToHost);
+    if (mask) hipMemcpy(mask, n->mask, px, hipMemcpyDeviceToHost);
+    return FGL_OK;
+}
+
+// Neural AO of the context's current frame (depth + G-buffer) into its AO
+// target: features, four MFMA convolutions, sigmoid head.
+int fgl_nao_infer(fgl_nao *n)
+{
+    if (!n) return FGL_ERR_INVALID;
+    CtxView cv;
+    int rc = ctx_view(n->ctx, &cv);
+    if (rc) return rc;
+    const size_t px = (size_t)cv.W * cv.H;
+    if (px > n->icap) {
+        hipStreamSynchronize(cv.stream);
+        if (n->ifeat) { hipFree(n->ifeat); hipFree(n->ia); hipFree(n->ib); hipFree(n->imask); }
+        if (nao_alloc((void **)&n->ifeat, px * C0 * 2) || nao_alloc((void **)&n->ia, px * CH * 2) ||
+            nao_alloc((void **)&n->ib, px * CH * 2) || nao_alloc((void **)&n->imask, px))
+            return FGL_ERR_NOMEM;
+        n->icap = px;
+    }
+    Mat16 V;
+    std::memcpy(V.m, cv.view, sizeof(V.m));
+    hipLaunchKernelGGL(k_features, g2(cv.W, cv.H), dim3(256), 0, cv.stream, V, cv.depth, cv.gbuf, cv.W, cv.H,
+                       n->ifeat, n->imask);
+    uint16_t *acts[4] = {n->ia, n->ib, n->ia, n->ib};
+    forward_convs(n, cv.stream, n->ifeat, acts, cv.W, cv.H, 1);
+    hipLaunchKernelGGL(k_head_fwd, dim3((unsigned)((px + 255) / 256)), dim3(256), 0, cv.stream, n->params,
+                       n->ib, n->imask, px, cv.ao);
+    ctx_ao_written(n->ctx);
+    return hipGetLastError() == hipSuccess ? FGL_OK : FGL_ERR_DEVICE;
+}
+
+}  // extern "C"

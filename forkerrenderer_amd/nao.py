@@ -1,0 +1,403 @@
+"""Neural ambient occlusion: dataset rendering, training, evaluation and
+inference driven through the C ABI (fgl_nao_*).
+
+The training pairs are G-buffer/AO frames the plain pipeline renders on the
+GPU: random layouts of displaced icospheres on a ground plane (the 1280x800
+deferred-PBR generator) seen from random camera poses; the target is the
+plain kernel SSAO + sigma=2 Gaussian blur.  A training step takes 16 random
+256x256 crops (4 frames x 4 crops), runs the network's forward and backward
+kernels and one Adam step (lr 1e-3, betas 0.9/0.999, eps 1e-8).
+"""
+import ctypes as C
+import math
+import os
+
+import numpy as np
+
+from . import (PASS_GEOMETRY, SsaoCfg, Context, FglError, lib, make_lookat, make_model_matrix,
+               make_perspective, scenegen)
+
+NPARAM = None
+
+
+def _bind():
+    L = lib()
+    if getattr(L, "_nao_bound", False):
+        return L
+    P = C.c_void_p
+    sig = {
+        "fgl_nao_param_count": (C.c_int, []),
+        "fgl_nao_create": (C.c_int, [P, C.c_int, C.c_int, C.POINTER(P)]),
+        "fgl_nao_destroy": (None, [P]),
+        "fgl_nao_set_state": (C.c_int, [P, P, C.c_int, C.c_int]),
+        "fgl_nao_get_state": (C.c_int, [P, P, C.POINTER(C.c_int)]),
+        "fgl_nao_stage_crops": (C.c_int, [P, C.c_int, C.c_int, P]),
+        "fgl_nao_train_step": (C.c_int, [P, C.c_float, P]),
+        "fgl_nao_gradients": (C.c_int, [P, P, P]),
+        "fgl_nao_read_batch": (C.c_int, [P, P, P, P]),
+        "fgl_nao_infer": (C.c_int, [P]),
+    }
+    for k, (r, a) in sig.items():
+        f = getattr(L, k)
+        f.restype = r
+        f.argtypes = a
+    L._nao_bound = True
+    return L
+
+
+def param_count():
+    return _bind().fgl_nao_param_count()
+
+
+def init_params(seed):
+    """He-normal convolution weights (layer 1 fan-in counts its 7 real input
+    channels; the 9 padding channels stay zero), zero biases, head N(0, 1/32),
+    head bias 2 so the untrained AO starts near 0.88."""
+    rng = np.random.default_rng(seed)
+    ch = 32
+    w1 = np.zeros((ch, 9, 16))
+    w1[:, :, :7] = rng.normal(0, math.sqrt(2.0 / 63), (ch, 9, 7))
+    parts = [w1.reshape(-1), np.zeros(ch)]
+    for _ in range(3):
+        parts += [rng.normal(0, math.sqrt(2.0 / 288), ch * 288), np.zeros(ch)]
+    parts += [rng.normal(0, math.sqrt(1.0 / ch), ch), np.array([2.0])]
+    p = np.concatenate(parts).astype(np.float32)
+    assert p.size == param_count()
+    return p
+
+
+class NeuralAO:
+    def __init__(self, ctx, batch=16, crop=256, seed=0, params=None):
+        L = _bind()
+        self.ctx = ctx
+        self.batch, self.crop = batch, crop
+        self._h = C.c_void_p()
+        rc = L.fgl_nao_create(ctx._h, batch, crop, C.byref(self._h))
+        if rc != 0:
+            raise FglError(f"fgl_nao_create failed: {rc}")
+        self.set_params(init_params(seed) if params is None else params)
+
+    def close(self):
+        if self._h:
+            _bind().fgl_nao_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_params(self, p):
+        n = param_count()
+        st = np.zeros(3 * n, np.float32)
+        st[:n] = p
+        self._chk(_bind().fgl_nao_set_state(self._h, st.ctypes.data_as(C.c_void_p), 0, 0), "set_state")
+
+    def state(self):
+        n = param_count()
+        st = np.zeros(3 * n, np.float32)
+        step = C.c_int()
+        self._chk(_bind().fgl_nao_get_state(self._h, st.ctypes.data_as(C.c_void_p), C.byref(step)), "get_state")
+        return st, step.value
+
+    def load_state(self, st, step):
+        st = np.ascontiguousarray(st, np.float32)
+        self._chk(_bind().fgl_nao_set_state(self._h, st.ctypes.data_as(C.c_void_p), 1, int(step)), "set_state")
+
+    def params(self):
+        return self.state()[0][:param_count()].copy()
+
+    def _chk(self, rc, what):
+        if rc != 0:
+            raise FglError(f"fgl_nao_{what} failed ({rc}): {lib().fgl_last_error(self.ctx._h).decode()}")
+
+    def stage(self, slot0, xy):
+        xy = np.ascontiguousarray(np.asarray(xy, np.int32).reshape(-1, 2))
+        self._chk(_bind().fgl_nao_stage_crops(self._h, slot0, len(xy), xy.ctypes.data_as(C.c_void_p)), "stage")
+
+    def train_step(self, lr=1e-3):
+        st = np.zeros(3, np.float32)
+        self._chk(_bind().fgl_nao_train_step(self._h, lr, st.ctypes.data_as(C.c_void_p)), "train_step")
+        return st  # sum sq err, sum abs err, covered pixels
+
+    def gradients(self):
+        g = np.zeros(param_count(), np.float32)
+        st = np.zeros(3, np.float32)
+        self._chk(_bind().fgl_nao_gradients(self._h, g.ctypes.data_as(C.c_void_p), st.ctypes.data_as(C.c_void_p)),
+                  "gradients")
+        return g, st
+
+    def read_batch(self):
+        px = self.batch * self.crop * self.crop
+        feat = np.zeros(px * 16, np.uint16)
+        tgt = np.zeros(px, np.float32)
+        mask = np.zeros(px, np.uint8)
+        self._chk(_bind().fgl_nao_read_batch(self._h, feat.ctypes.data_as(C.c_void_p), tgt.ctypes.data_as(C.c_void_p),
+                                             mask.ctypes.data_as(C.c_void_p)), "read_batch")
+        s = (self.batch, self.crop, self.crop)
+        return feat.reshape(*s, 16), tgt.reshape(s), mask.reshape(s)
+
+    def infer(self):
+        """Neural AO of the context's current frame into its AO target."""
+        self._chk(_bind().fgl_nao_infer(self._h), "infer")
+
+
+# ---------------- training data: the plain pipeline on random scenes ----------------
+
+def ssao_cfg(seed=2, n=64, radius=0.5, sigma=2.0):
+    """Config-2 SSAO: 64-sample kernel r=0.5, bias 5e-4, range 1e-2, 4x4 noise,
+    sigma=2 separable blur (the same sets the .scene keys produce)."""
+    import tempfile
+    from . import Scene
+    with tempfile.TemporaryDirectory() as d:
+        p = os.path.join(d, "cfg.scene")
+        with open(p, "w") as fh:
+            fh.write(f"light point 0 5 0 1 1 1\nssaokernel {n} {radius} 0.0005 0.01 {seed}\n"
+                     f"aoblur gaussian {sigma}\n")
+        sc = Scene(p)
+        return SsaoCfg.from_buffer_copy(sc.info.ssao)
+
+
+class SceneSet:
+    """Meshes for the training/evaluation scenes, uploaded once into a context."""
+
+    def __init__(self, ctx, workdir, subdiv=5, seed=100):
+        os.makedirs(workdir, exist_ok=True)
+        v, n, uv, f = scenegen.displaced_icosphere(subdiv, seed, 0.1, 1.0)
+        from . import Material
+        m = Material()
+        m.kd[:] = (0.8, 0.8, 0.8)
+        m.albedo[:] = (0.8, 0.8, 0.8)
+        m.ks[:] = (0.3, 0.3, 0.3)
+        for fld in ("map_kd", "map_ks", "map_ke", "map_bump", "map_base", "map_rough", "map_metal", "map_ao",
+                    "map_pbr_ke"):
+            setattr(m, fld, -1)
+        mid = ctx.upload_material(m)
+        f32 = np.float32
+        self.sphere = ctx.upload_model({"pos": v.astype(f32), "uv": uv.astype(f32), "nrm": n.astype(f32),
+                                        "tan": None, "vi": f, "ti": f, "ni": f,
+                                        "face_material": np.full(len(f), mid, np.int32), "support_pbr": False})
+        gp, gn, guv, gf = scenegen.plane_group(4.0)
+        self.ground = ctx.upload_model({"pos": gp, "uv": guv, "nrm": gn, "tan": None, "vi": gf, "ti": gf, "ni": gf,
+                                        "face_material": np.full(len(gf), mid, np.int32), "support_pbr": False})
+
+
+def layout(seed):
+    """A random scene of the deferred-PBR generator family: 1-6 displaced
+    icospheres (radius 0.3-1.0) on a ground plane, seeded."""
+    rng = np.random.default_rng(seed)
+    objs = []
+    for _ in range(int(rng.integers(1, 7))):
+        r = float(rng.uniform(0.3, 1.0))
+        x, z = rng.uniform(-2.2, 2.2, 2)
+        y = float(rng.uniform(0.55, 1.0)) * r   # resting on / sunk into the ground
+        objs.append(make_model_matrix((x, y, z), float(rng.uniform(0, 360)), r))
+    ground = make_model_matrix((0.0, 0.0, 0.0), 0.0, 4.0)
+    return ground, objs
+
+
+def camera(seed):
+    rng = np.random.default_rng(seed)
+    r = rng.uniform(3.5, 7.0)
+    el = math.radians(rng.uniform(12, 55))
+    az = rng.uniform(0, 2 * math.pi)
+    eye = (r * math.cos(el) * math.cos(az), r * math.sin(el), r * math.cos(el) * math.sin(az))
+    at = tuple(rng.uniform(-0.6, 0.6, 3) * (1, 0.3, 1))
+    return eye, at
+
+
+def render_frame(ctx, sset, frame_seed, W, H, cfg, plain_ao=True):
+    """G-buffer pass of a random layout + camera; optionally the plain SSAO +
+    blur into the AO target.  Shadows off (the AO pass does not use them)."""
+    ground, objs = layout(frame_seed // 1000)
+    eye, at = camera(frame_seed)
+    V = make_lookat(eye, at)
+    P = make_perspective(45.0, np.float32(W) / np.float32(H), 0.01, 20.0)
+    from . import MAT_PROJ, MAT_VIEW
+    ctx.set_matrix(MAT_VIEW, V)
+    ctx.set_matrix(MAT_PROJ, P)
+    ctx.set_eye(eye)
+    ctx.set_shadow(False)
+    ctx.set_viewport(0, 0, W, H)
+    ctx.init_targets(W, H)
+    ctx.begin_pass(PASS_GEOMETRY)
+    ctx.draw_model(sset.ground, ground)
+    for M in objs:
+        ctx.draw_model(sset.sphere, M)
+    ctx.end_pass()
+    if plain_ao:
+        ctx.ssao(cfg)
+
+
+def frame_seeds(split, n):
+    """Training frames: layouts 0..7 x 500 poses; evaluation frames: layouts
+    100..103 x 50 poses (held out)."""
+    if split == "train":
+        return [l * 1000 + p for l in range(8) for p in range(n // 8)]
+    return [(100 + l) * 1000 + p for l in range(4) for p in range(n // 4)]
+
+
+def step_batch(step, seed, frames, W, H, crop, batch):
+    """The frames and crop origins of training step `step`: a pure function
+    of (seed, step), so a run resumed from a checkpoint sees the same data.
+    4 frames per step, batch/4 crops each."""
+    order = np.random.default_rng(seed).permutation(frame_seeds("train", frames))
+    rng = np.random.default_rng([seed, step])
+    per = batch // 4
+    out = []
+    for j in range(4):
+        fs = int(order[(4 * step + j) % len(order)])
+        xy = np.stack([rng.integers(0, W - crop + 1, per), rng.integers(0, H - crop + 1, per)], 1)
+        out.append((fs, xy))
+    return out
+
+
+def train(nao, ctx, sset, steps, start=0, seed=0, W=1280, H=800, frames=4000, lr=1e-3, log=None, cfg=None):
+    """Steps start .. start+steps-1 of a training run; returns per-step
+    (sum sq err, sum |err|, covered pixels)."""
+    cfg = cfg if cfg is not None else ssao_cfg()
+    per = nao.batch // 4
+    hist = []
+    for step in range(start, start + steps):
+        for j, (fs, xy) in enumerate(step_batch(step, seed, frames, W, H, nao.crop, nao.batch)):
+            render_frame(ctx, sset, fs, W, H, cfg)
+            nao.stage(j * per, xy)
+        sse, sae, cnt = nao.train_step(lr)
+        hist.append((float(sse), float(sae), float(cnt)))
+        if log and (step % 50 == 0 or step == start + steps - 1):
+            log(f"step {step}: mse {sse / max(cnt, 1):.5f} mae {sae / max(cnt, 1):.4f}")
+    return hist
+
+
+def save_checkpoint(path, nao, seed, frames, W=1280, H=800):
+    """Weights, Adam moments, step and the data recipe (seed, frame count,
+    frame size, crop, batch) in one .npz."""
+    st, step = nao.state()
+    np.savez(path, state=st, step=np.int64(step), seed=np.int64(seed), frames=np.int64(frames),
+             size=np.array([W, H], np.int64), crop=np.int64(nao.crop), batch=np.int64(nao.batch),
+             nparam=np.int64(param_count()))
+
+
+def load_checkpoint(path):
+    z = np.load(path)
+    if int(z["nparam"]) != param_count():
+        raise FglError(f"{path}: parameter count {int(z['nparam'])} != {param_count()}")
+    return {k: z[k] for k in z.files}
+
+
+DEFAULT_WEIGHTS = os.path.join(os.path.dirname(os.path.abspath(__file__)), "data", "nao_weights.npz")
+
+
+def load_weights(path=DEFAULT_WEIGHTS):
+    """Trained parameters (flat float32) from a checkpoint."""
+    return load_checkpoint(path)["state"][:param_count()].copy()
+
+
+def audit_fallback(nao_ao, plain_ao, cover, frame_index, tile=64, every=16, threshold=0.05):
+    """1-in-`every` tile audit: tiles t with t % every == frame_index % every
+    are compared with plain SSAO; a tile whose MAE over covered pixels exceeds
+    `threshold` falls back to the plain result.  Returns (ao, audited, failed)."""
+    H, W = nao_ao.shape
+    out = nao_ao.copy()
+    audited = failed = 0
+    ty, tx = (H + tile - 1) // tile, (W + tile - 1) // tile
+    for t in range(ty * tx):
+        if t % every != frame_index % every:
+            continue
+        y0, x0 = (t // tx) * tile, (t % tx) * tile
+        sl = (slice(y0, y0 + tile), slice(x0, x0 + tile))
+        c = cover[sl]
+        if not c.any():
+            continue
+        audited += 1
+        if np.abs(nao_ao[sl] - plain_ao[sl])[c].mean() > threshold:
+            failed += 1
+            out[sl] = plain_ao[sl]
+    return out, audited, failed
+
+
+def evaluate(nao, ctx, sset, n_frames=200, W=1280, H=800, cfg=None, unblurred=False):
+    """MAE / max error of neural AO against plain SSAO+blur on held-out frames
+    (covered pixels), the audit fallback rate and the MAE after fallback.
+    With unblurred=True also the MAE of SSAO without its blur against the
+    same target (the cheaper plain alternative)."""
+    from . import TARGET_AO, TARGET_DEPTH
+    cfg = cfg if cfg is not None else ssao_cfg()
+    raw = None
+    if unblurred:
+        raw = SsaoCfg.from_buffer_copy(cfg)
+        raw.blur = 0
+    sae = sae_fb = sae_raw = cnt = 0.0
+    mx = 0.0
+    audited = failed = 0
+    for i, fs in enumerate(frame_seeds("eval", n_frames)):
+        if raw is not None:
+            render_frame(ctx, sset, fs, W, H, raw)
+            noblur = ctx.read(TARGET_AO).reshape(H, W)
+            ctx.ssao(cfg)
+        else:
+            render_frame(ctx, sset, fs, W, H, cfg)
+        plain = ctx.read(TARGET_AO).reshape(H, W)
+        cover = ctx.read(TARGET_DEPTH).reshape(H, W) < 3e38
+        nao.infer()
+        pred = ctx.read(TARGET_AO).reshape(H, W)
+        err = np.abs(pred - plain)[cover]
+        sae += float(err.sum())
+        cnt += err.size
+        mx = max(mx, float(err.max()) if err.size else 0.0)
+        fixed, a, f = audit_fallback(pred, plain, cover, i)
+        sae_fb += float(np.abs(fixed - plain)[cover].sum())
+        if raw is not None:
+            sae_raw += float(np.abs(noblur - plain)[cover].sum())
+        audited += a
+        failed += f
+    out = {"mae": sae / max(cnt, 1), "max_err": mx, "fallback_rate": failed / max(audited, 1),
+           "mae_after_fallback": sae_fb / max(cnt, 1), "frames": n_frames, "audited_tiles": audited}
+    if raw is not None:
+        out["mae_unblurred_ssao"] = sae_raw / max(cnt, 1)
+    return out
+
+
+def main(argv=None):
+    """python -m forkerrenderer_amd.nao --steps N --out ckpt.npz: trains from
+    scratch (or --resume), evaluates on the held-out frames, writes the
+    checkpoint and a JSON line of metrics."""
+    import argparse
+    import json
+    import time
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--steps", type=int, default=2000)
+    ap.add_argument("--seed", type=int, default=0)
+    ap.add_argument("--frames", type=int, default=4000)
+    ap.add_argument("--eval-frames", type=int, default=200)
+    ap.add_argument("--lr", type=float, default=1e-3)
+    ap.add_argument("--out", required=True)
+    ap.add_argument("--resume", default=None)
+    ap.add_argument("--workdir", default="_scenes/_nao")
+    a = ap.parse_args(argv)
+    ctx = Context(0)
+    sset = SceneSet(ctx, a.workdir)
+    cfg = ssao_cfg()
+    nao = NeuralAO(ctx, seed=a.seed)
+    start = 0
+    if a.resume:
+        ck = load_checkpoint(a.resume)
+        nao.load_state(ck["state"], int(ck["step"]))
+        start = int(ck["step"])
+    base = evaluate(nao, ctx, sset, min(a.eval_frames, 40), cfg=cfg) if start == 0 else None
+    t0 = time.time()
+    hist = train(nao, ctx, sset, a.steps, start=start, seed=a.seed, frames=a.frames, lr=a.lr, cfg=cfg,
+                 log=lambda s: print(s, f"({time.time() - t0:.1f}s)", flush=True))
+    t_train = time.time() - t0
+    save_checkpoint(a.out, nao, a.seed, a.frames)
+    ev = evaluate(nao, ctx, sset, a.eval_frames, cfg=cfg, unblurred=True)
+    rec = {"steps": start + a.steps, "train_s": t_train, "final_train_mse": hist[-1][0] / max(hist[-1][2], 1),
+           "eval": ev, "untrained_eval40": base}
+    print(json.dumps(rec), flush=True)
+    nao.close()
+    ctx.close()
+
+
+if __name__ == "__main__":
+    main()

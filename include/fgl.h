@@ -233,6 +233,29 @@ int fgl_scene_texture(const fgl_scene *s, int i, int *w, int *h, int *bpp, int *
  * on first use. */
 int fgl_render_scene(fgl_ctx *ctx, fgl_scene *s);
 
+/* ---- neural ambient occlusion (new; no counterpart in the reference) ----
+ * A BF16 convolutional network that replaces Render::DoSSAO + blur
+ * (src/render.cpp:214-289): 7 view-space features -> 4 x conv3x3 (32 ch,
+ * ReLU) -> conv1x1 + sigmoid.  Training runs on the device (forward,
+ * backward, Adam) on crops of frames the plain pipeline rendered. */
+typedef struct fgl_nao fgl_nao;
+int fgl_nao_param_count(void);
+int fgl_nao_create(fgl_ctx *ctx, int batch, int crop, fgl_nao **out);
+void fgl_nao_destroy(fgl_nao *n);
+/* state = [params | Adam m | Adam v] (3 * param_count floats); with_adam = 0
+ * loads params only and resets Adam */
+int fgl_nao_set_state(fgl_nao *n, const float *state, int with_adam, int step);
+int fgl_nao_get_state(fgl_nao *n, float *state, int *step);
+/* crops (origins xy[2*i]) of the context's current frame -- depth, G-buffer
+ * and AO target -- into batch slots slot0.. */
+int fgl_nao_stage_crops(fgl_nao *n, int slot0, int count, const int *xy);
+/* one Adam step on the staged batch; stats = [sum sq err, sum |err|, covered] */
+int fgl_nao_train_step(fgl_nao *n, float lr, float *stats);
+int fgl_nao_gradients(fgl_nao *n, float *grads, float *stats);
+int fgl_nao_read_batch(fgl_nao *n, uint16_t *feat_bf16, float *target, uint8_t *mask);
+/* neural AO of the context's current frame into its AO target */
+int fgl_nao_infer(fgl_nao *n);
+
 /* ---- host matrix builders (src/geometry.cpp:60-179), shared by the CLI and tests ---- */
 void fgl_make_model_matrix(float tx, float ty, float tz, float roty_deg, float scale, float out[16]);
 void fgl_make_lookat(const float eye[3], const float center[3], const float up[3], float out[16]);

@@ -1,0 +1,129 @@
+"""CPU oracle of the neural-AO network (TEST INFRASTRUCTURE).
+
+Forward and backward of: conv3x3 16->32 ReLU, 3x conv3x3 32->32 ReLU,
+conv1x1 32->1 sigmoid, 'same' zero padding, loss = sum over covered pixels of
+(y - t)^2 / n_pixels.  Sums are evaluated in float64.  With bf16=True the
+values the device keeps in BF16 (weights of layers 1-4, activations, data
+gradients) are rounded to BF16 at the same points; with bf16=False the
+network is plain float64, which the finite-difference test uses to pin the
+backward formulas.
+
+Parameter layout (flat, float32 on the device): W1 [32][9][16], b1[32],
+W2..W4 [32][9][32], b2..b4 [32], W5 [32], b5 [1]; k = tap*Cin + ci with
+tap = (dy+1)*3 + (dx+1).
+"""
+import numpy as np
+
+C0, CH = 16, 32
+K1, K2 = 9 * C0, 9 * CH
+SIZES = [("W1", CH * K1), ("b1", CH), ("W2", CH * K2), ("b2", CH), ("W3", CH * K2), ("b3", CH),
+         ("W4", CH * K2), ("b4", CH), ("W5", CH), ("b5", 1)]
+NPARAM = sum(s for _, s in SIZES)
+
+
+def split(params):
+    out, o = {}, 0
+    for name, s in SIZES:
+        out[name] = np.asarray(params[o:o + s], np.float64)
+        o += s
+    return out
+
+
+def bf16_round(x):
+    """float -> bf16 (round to nearest even) -> float, as the device's f2bf."""
+    x32 = np.ascontiguousarray(np.asarray(x, np.float32))
+    u = x32.view(np.uint32).astype(np.uint64)
+    u = (u + 0x7FFF + ((u >> 16) & 1)) & 0xFFFFFFFF
+    r = ((u >> 16) << 16).astype(np.uint32)
+    return r.view(np.float32).astype(np.float64)
+
+
+def bf16_bits_to_float(b):
+    return (np.asarray(b, np.uint32) << 16).view(np.float32).astype(np.float64)
+
+
+def _patches(x):
+    """x: [N,H,W,C] -> [N,H,W,9,C] with zero padding (tap order dy, dx)."""
+    N, H, W, Cc = x.shape
+    xp = np.zeros((N, H + 2, W + 2, Cc), x.dtype)
+    xp[:, 1:-1, 1:-1] = x
+    taps = [xp[:, 1 + dy:1 + dy + H, 1 + dx:1 + dx + W] for dy in (-1, 0, 1) for dx in (-1, 0, 1)]
+    return np.stack(taps, axis=3)
+
+
+def conv3x3(x, W, b):
+    """x [N,H,W,Cin], W [Co][9*Cin] -> [N,H,W,Co]"""
+    Co = W.shape[0]
+    p = _patches(x).reshape(x.shape[0], x.shape[1], x.shape[2], -1)
+    out = p @ W.reshape(Co, -1).T
+    return out + (b if b is not None else 0.0)
+
+
+def conv3x3_transpose(d, W, cin):
+    """Gradient w.r.t. the input of conv3x3: d [N,H,W,Co], W [Co][9*cin]."""
+    Co = W.shape[0]
+    Wr = W.reshape(Co, 9, cin)
+    Wf = Wr[:, ::-1, :].transpose(2, 1, 0).reshape(cin, 9 * Co)   # flipped taps, transposed
+    return conv3x3(d, Wf, None)
+
+
+def forward(params, feat, bf16=True):
+    P = split(params)
+    r = bf16_round if bf16 else (lambda a: np.asarray(a, np.float64))
+    x = np.asarray(feat, np.float64)
+    acts = []
+    for l in range(4):
+        W = r(P[f"W{l + 1}"]).reshape(CH, -1)
+        z = conv3x3(x, W, P[f"b{l + 1}"])
+        a = r(np.maximum(z, 0.0))
+        acts.append(a)
+        x = a
+    z5 = acts[-1] @ P["W5"] + P["b5"][0]
+    y = 1.0 / (1.0 + np.exp(-z5))
+    return acts, y
+
+
+def loss_and_grads(params, feat, target, mask, bf16=True):
+    """Returns (loss, flat gradient float64, stats (sse, sae, count))."""
+    P = split(params)
+    r = bf16_round if bf16 else (lambda a: np.asarray(a, np.float64))
+    acts, y = forward(params, feat, bf16)
+    n = y.size
+    m = np.asarray(mask, bool)
+    t = np.asarray(target, np.float64)
+    diff = y - t
+    loss = float(np.sum(np.where(m, diff * diff, 0.0)) / n)
+    dz = np.where(m, 2.0 * diff / n, 0.0) * y * (1.0 - y)
+    g = {}
+    a4 = acts[3]
+    g["W5"] = np.einsum("nhw,nhwc->c", dz, a4)
+    g["b5"] = np.array([dz.sum()])
+    d = r(np.where(a4 > 0, dz[..., None] * P["W5"], 0.0))
+    ins = [np.asarray(feat, np.float64)] + acts[:3]
+    for l in range(3, -1, -1):
+        cin = C0 if l == 0 else CH
+        pat = _patches(ins[l]).reshape(d.shape[0], d.shape[1], d.shape[2], -1)
+        g[f"W{l + 1}"] = np.einsum("nhwo,nhwk->ok", d, pat).reshape(-1)
+        g[f"b{l + 1}"] = d.sum(axis=(0, 1, 2))
+        if l > 0:
+            W = r(P[f"W{l + 1}"]).reshape(CH, -1)
+            dx = conv3x3_transpose(d, W, cin)
+            d = r(np.where(ins[l] > 0, dx, 0.0))
+    flat = np.concatenate([np.asarray(g[name], np.float64).reshape(-1) for name, _ in SIZES])
+    stats = (float(np.sum(np.where(m, diff * diff, 0))), float(np.sum(np.where(m, np.abs(diff), 0))),
+             float(m.sum()))
+    return loss, flat, stats
+
+
+def init_params(seed):
+    """He-normal init for the convolutions (fan-in counts the 7 real input
+    channels of layer 1), small head, b5 = 2 (AO starts near 0.88)."""
+    rng = np.random.default_rng(seed)
+    parts = []
+    w1 = np.zeros((CH, 9, C0))
+    w1[:, :, :7] = rng.normal(0, np.sqrt(2.0 / (9 * 7)), (CH, 9, 7))
+    parts += [w1.reshape(-1), np.zeros(CH)]
+    for _ in range(3):
+        parts += [rng.normal(0, np.sqrt(2.0 / K2), CH * K2), np.zeros(CH)]
+    parts += [rng.normal(0, np.sqrt(1.0 / CH), CH), np.array([2.0])]
+    return np.concatenate(parts).astype(np.float32)

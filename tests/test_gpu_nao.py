@@ -1,0 +1,196 @@
+"""Neural AO on the GPU against the CPU network oracle (oracle/nao_ref.py):
+staged features, forward, gradients, the Adam step, deterministic training,
+checkpoint resume, and the audit fallback."""
+import numpy as np
+import pytest
+
+import forkerrenderer_amd as F
+from forkerrenderer_amd import nao as NA
+from oracle import nao_ref as R
+
+pytestmark = pytest.mark.gpu
+
+W, H, CROP, BATCH = 320, 256, 64, 4
+XY = [(0, 0), (256, 192), (131, 70), (40, 150)]
+
+
+@pytest.fixture(scope="module")
+def world(ctx, tmp_path_factory):
+    sset = NA.SceneSet(ctx, str(tmp_path_factory.mktemp("nao")), subdiv=4)
+    cfg = NA.ssao_cfg()
+    NA.render_frame(ctx, sset, 5003, W, H, cfg)
+    frame = {"depth": ctx.read(F.TARGET_DEPTH).reshape(H, W),
+             "gbuf": ctx.read(F.TARGET_GBUFFER).reshape(F.GBUF_PLANES, H, W),
+             "ao": ctx.read(F.TARGET_AO).reshape(H, W),
+             "view": NA.make_lookat(*NA.camera(5003)).astype(np.float32)}
+    assert (frame["depth"] < 3e38).mean() > 0.3, "test frame should be mostly covered"
+    return sset, cfg, frame
+
+
+def _params(seed=7):
+    p = R.init_params(seed)
+    rng = np.random.default_rng(seed)
+    offs = np.cumsum([0] + [s for _, s in R.SIZES])
+    for k, (name, s) in enumerate(R.SIZES):
+        if name.startswith("b"):
+            p[offs[k]:offs[k + 1]] += rng.normal(0, 0.05, s).astype(np.float32)
+    return p.astype(np.float32)
+
+
+def features_f32(frame):
+    """7 view-space features in float32, operation order of the device."""
+    f32 = np.float32
+    V = frame["view"].reshape(4, 4)
+    g = frame["gbuf"]
+    N = [g[0], g[1], g[2]]
+    P = [g[3], g[4], g[5], np.ones_like(g[3])]
+
+    def dot(row, v):
+        r = np.zeros_like(v[0], dtype=f32)
+        for a, b in zip(row, v):
+            r = r + f32(a) * b
+        return r
+    vp = [dot(V[r], P) for r in range(3)]
+    vn = [dot(V[r, :3], N) for r in range(3)]
+    feat = [-vp[2] * f32(0.1), vn[0], vn[1], vn[2], vp[0] * f32(0.1), vp[1] * f32(0.1), vp[2] * f32(0.1)]
+    cov = frame["depth"] < 3e38
+    out = np.zeros((H, W, 16), np.float32)
+    for c in range(7):
+        out[..., c] = np.where(cov, feat[c], f32(0))
+    return out, cov
+
+
+def bf16_bits(x):
+    u = np.ascontiguousarray(x, np.float32).view(np.uint32).astype(np.uint64)
+    return (((u + 0x7FFF + ((u >> 16) & 1)) >> 16) & 0xFFFF).astype(np.uint16)
+
+
+@pytest.fixture(scope="module")
+def staged(ctx, world):
+    sset, cfg, frame = world
+    net = NA.NeuralAO(ctx, batch=BATCH, crop=CROP, params=_params())
+    NA.render_frame(ctx, sset, 5003, W, H, cfg)
+    net.stage(0, XY)
+    feat, tgt, mask = net.read_batch()
+    yield net, feat, tgt, mask
+    net.close()
+
+
+def test_staged_crops_match_gbuffer(world, staged):
+    _, _, frame = world
+    _, feat, tgt, mask = staged
+    ref, cov = features_f32(frame)
+    refb = bf16_bits(ref)
+    for i, (x, y) in enumerate(XY):
+        sl = (slice(y, y + CROP), slice(x, x + CROP))
+        assert np.array_equal(feat[i], refb[sl]), f"crop {i}"
+        assert np.array_equal(tgt[i], frame["ao"][sl])
+        assert np.array_equal(mask[i].astype(bool), cov[sl])
+
+
+def test_gradients_match_oracle(staged):
+    net, feat, tgt, mask = staged
+    p = _params()
+    net.set_params(p)
+    g, st = net.gradients()
+    featf = R.bf16_bits_to_float(feat)
+    loss, gr, (sse, sae, cnt) = R.loss_and_grads(p, featf, tgt, mask, bf16=True)
+    assert st[2] == cnt
+    assert abs(st[0] - sse) <= 1e-4 * sse + 1e-6
+    assert abs(st[1] - sae) <= 1e-4 * sae + 1e-6
+    offs = np.cumsum([0] + [s for _, s in R.SIZES])
+    for k, (name, _) in enumerate(R.SIZES):
+        a, b = g[offs[k]:offs[k + 1]].astype(np.float64), gr[offs[k]:offs[k + 1]]
+        rel = np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-30)
+        # FP32 (device) vs FP64 (oracle) sums can move a BF16 rounding by one
+        # unit (2^-8 relative) at a few activations / data gradients.
+        assert rel < 5e-3, (name, rel)
+
+
+def test_forward_matches_oracle(ctx, world, staged):
+    sset, cfg, frame = world
+    net = staged[0]
+    p = _params()
+    net.set_params(p)
+    NA.render_frame(ctx, sset, 5003, W, H, cfg)
+    net.infer()
+    y = ctx.read(F.TARGET_AO).reshape(H, W)
+    ref, cov = features_f32(frame)
+    featf = R.bf16_bits_to_float(bf16_bits(ref)).reshape(1, H, W, 16)
+    _, yr = R.forward(p, featf, bf16=True)
+    yr = yr[0]
+    assert np.all(y[~cov] == 1.0)
+    err = np.abs(y - yr)[cov]
+    assert err.max() < 2e-2 and err.mean() < 2e-4, (err.max(), err.mean())
+
+
+def test_adam_step_matches_numpy(staged):
+    net = staged[0]
+    p = _params()
+    net.set_params(p)
+    g, _ = net.gradients()
+    lr = np.float32(1e-3)
+    net.train_step(float(lr))
+    p1 = net.params()
+    f = np.float32
+    m = f(0.9) * f(0) + (f(1) - f(0.9)) * g
+    v = f(0.999) * f(0) + (f(1) - f(0.999)) * g * g
+    c1, c2 = f(1.0 - 0.9), f(1.0 - 0.999)
+    want = p - lr * (m / c1) / (np.sqrt(v / c2) + f(1e-8))
+    assert np.array_equal(p1.view(np.uint32), want.astype(np.float32).view(np.uint32))
+
+
+def test_training_is_deterministic_and_resumes_bitwise(ctx, world):
+    sset, cfg, _ = world
+    NA.render_frame(ctx, sset, 5003, W, H, cfg)
+
+    def fresh():
+        n = NA.NeuralAO(ctx, batch=BATCH, crop=CROP, params=_params())
+        n.stage(0, XY)
+        return n
+    a = fresh()
+    for _ in range(4):
+        a.train_step(1e-3)
+    b = fresh()
+    for _ in range(2):
+        b.train_step(1e-3)
+    st, step = b.state()
+    b.close()
+    c = fresh()
+    c.load_state(st, step)
+    for _ in range(2):
+        c.train_step(1e-3)
+    pa, sa = a.state()
+    pc, sc = c.state()
+    assert sa == sc == 4
+    assert np.array_equal(pa.view(np.uint32), pc.view(np.uint32))
+    a.close()
+    c.close()
+
+
+def test_training_reduces_loss(ctx, world):
+    sset, cfg, _ = world
+    NA.render_frame(ctx, sset, 5003, W, H, cfg)
+    n = NA.NeuralAO(ctx, batch=BATCH, crop=CROP, seed=1)
+    n.stage(0, XY)
+    first = n.train_step(1e-3)
+    for _ in range(60):
+        last = n.train_step(1e-3)
+    n.close()
+    assert last[0] < 0.5 * first[0], (first, last)
+
+
+def test_audit_falls_back_on_bad_weights(ctx, world):
+    sset, cfg, _ = world
+    p = R.init_params(0)
+    p[-1] = -30.0   # head bias: AO ~ 0 everywhere
+    n = NA.NeuralAO(ctx, batch=1, crop=64, params=p)
+    NA.render_frame(ctx, sset, 5003, W, H, cfg)
+    plain = ctx.read(F.TARGET_AO).reshape(H, W)
+    cover = ctx.read(F.TARGET_DEPTH).reshape(H, W) < 3e38
+    n.infer()
+    pred = ctx.read(F.TARGET_AO).reshape(H, W)
+    n.close()
+    fixed, audited, failed = NA.audit_fallback(pred, plain, cover, 0, every=1)
+    assert audited > 0 and failed >= 0.9 * audited, (audited, failed)
+    assert np.abs(fixed - plain)[cover].mean() < np.abs(pred - plain)[cover].mean()
