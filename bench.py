@@ -33,7 +33,7 @@ def gen_scene(cfg):
     d = os.path.join(ROOT, "_scenes", cfg)
     p = os.path.join(d, f"{cfg}.scene")
     if not os.path.exists(p):
-        {"c1": G.config1, "c2": G.config2, "c3": G.config3}[cfg](d)
+        {"c1": G.config1, "c2": G.config2, "c3": G.config3, "c4": G.config4}[cfg](d)
     return p
 
 
@@ -99,6 +99,59 @@ def run_gpu(sc, steps, warmup, tile):
     return (t1 - t0) / steps, per, img, tuned
 
 
+def run_neural_ao(steps, warmup):
+    """Neural AO against plain SSAO + blur on the 1920x1080 frame without SSAA
+    (the default scene at SSAA 1x): per-frame kernel time of each, and the
+    AO error of the trained network on that frame (covered pixels)."""
+    import forkerrenderer_amd as F
+    from forkerrenderer_amd import nao as NA
+    if not os.path.exists(NA.DEFAULT_WEIGHTS):
+        return {"error": "no trained weights at " + os.path.relpath(NA.DEFAULT_WEIGHTS, ROOT)}
+    ck = NA.load_checkpoint(NA.DEFAULT_WEIGHTS)
+    sc = F.Scene(gen_scene("c4"))
+    I = sc.info
+    ctx = F.Context(0)
+    net = NA.NeuralAO(ctx, batch=1, crop=64, params=ck["state"][:NA.param_count()])
+    for _ in range(warmup):
+        ctx.render_scene(sc)
+        net.infer()
+    ctx.sync()
+    ctx.read_timings()
+    ctx.enable_timing(True)
+    for _ in range(steps):
+        ctx.render_scene(sc)
+        net.infer()
+    ctx.sync()
+    per = {}
+    for name, ms in ctx.read_timings():
+        per[name] = per.get(name, 0.0) + ms / steps
+    ctx.enable_timing(False)
+    ctx.render_scene(sc)
+    W, H = I.width, I.height
+    plain = ctx.read(F.TARGET_AO).reshape(H, W)
+    cover = ctx.read(F.TARGET_DEPTH).reshape(H, W) < 3e38
+    net.infer()
+    pred = ctx.read(F.TARGET_AO).reshape(H, W)
+    err = np.abs(pred - plain)[cover]
+    fixed, audited, failed = NA.audit_fallback(pred, plain, cover, 0)
+    net.close()
+    ctx.close()
+    plain_ms = per.get("ssao", 0.0) + per.get("blur_gaussian", 0.0) + per.get("blur_reference", 0.0)
+    nao_ms = per.get("nao_features", 0.0) + per.get("nao_conv", 0.0) + per.get("nao_head", 0.0)
+    px = W * H
+    conv_tflops = px * 59392 / (per["nao_conv"] * 1e-3) / 1e12 if per.get("nao_conv") else None
+    held = json.loads(str(ck["eval_json"])) if "eval_json" in ck else None
+    return {"resolution": [W, H], "plain_ssao_blur_ms": plain_ms, "neural_ao_ms": nao_ms,
+            "speedup": plain_ms / nao_ms if nao_ms else None,
+            "kernels_ms": {k: v for k, v in per.items() if k.startswith("nao") or k in ("ssao", "blur_gaussian")},
+            "conv_roofline": {"bound": "mfma", "achieved": conv_tflops, "peak": 2500.0, "unit": "TFLOP/s",
+                              "frac": conv_tflops / 2500.0 if conv_tflops else None,
+                              "flop_per_px": 59392},
+            "frame_mae": float(err.mean()), "frame_max_err": float(err.max()),
+            "frame_fallback_tiles": [failed, audited],
+            "heldout_eval": held, "train_steps": int(ck["step"])}
+
+
 def cpu_baseline(sc, band_frac):
     """Oracle on a band of rows, scaled: est = t_empty + (t_band - t_empty) * rows_total/rows_band
     per pass, where t_empty is the same pass with an empty row window (per-triangle work)."""
@@ -161,6 +214,7 @@ def main():
     ap.add_argument("--tile", type=int, default=0, help="tile edge 8/16/32; 0 = autotune")
     ap.add_argument("--cpu-band", type=float, default=1.0 / 32)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-nao", action="store_true", help="skip the neural-AO vs plain SSAO comparison")
     ap.add_argument("--copy-calibrate", type=int, default=0, metavar="MIB",
                     help="only run a plain copy of MIB MiB (calibrates HBM byte counters) and exit")
     a = ap.parse_args()
@@ -217,6 +271,8 @@ def main():
         "cpu_baseline": cpu,
         "speedup_vs_cpu": (None if cpu is None else (1.0 / sec) / cpu["value"]),
     }
+    if not a.no_nao:
+        line["neural_ao"] = run_neural_ao(max(5, a.steps // 2), a.warmup)
     print(json.dumps(line))
 
 

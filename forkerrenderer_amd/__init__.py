@@ -231,6 +231,29 @@ class Context:
         self._chk(lib().fgl_upload_material(self._h, C.byref(m), C.byref(out)), "fgl_upload_material")
         return out.value
 
+    def upload_scene(self, sc):
+        """Uploads every texture, material and model of a loaded Scene;
+        returns [(model id, model matrix)] in the scene's model order."""
+        tid = []
+        for i in range(sc.info.n_textures):
+            t = sc.texture(i)
+            tid.append(self.upload_texture(t["bytes"], t["wrap"], t["filter"]))
+        mid = []
+        for i in range(sc.info.n_materials):
+            mm = sc.material(i)
+            for f in ("map_kd", "map_ks", "map_ke", "map_bump", "map_base", "map_rough", "map_metal", "map_ao",
+                      "map_pbr_ke"):
+                v = getattr(mm, f)
+                if v >= 0:
+                    setattr(mm, f, tid[v])
+            mid.append(self.upload_material(mm))
+        ids = []
+        for i in range(sc.info.n_models):
+            md, M = sc.model(i)
+            md["face_material"] = np.array([mid[x] for x in md["face_material"]], np.int32)
+            ids.append((self.upload_model(md), M))
+        return ids
+
     def upload_model(self, model):
         """model: dict with pos (n,3), uv (n,2), nrm (n,3), tan (n,3) or None,
         vi/ti/ni (f,3), face_material (f,), support_pbr."""

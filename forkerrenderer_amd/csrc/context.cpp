@@ -684,7 +684,7 @@ static int flush_pass(fgl_ctx *c)
         KernelTimer t(c, "tile_ranges");
         launch_fill_u32((uint32_t *)c->tile_start.p, 0, ntiles, c->stream);
         launch_fill_u32((uint32_t *)c->tile_end.p, 0, ntiles, c->stream);
-        launch_tile_ranges((const uint32_t *)c->keys.p, npairs, (uint32_t *)c->tile_start.p,
+        launch_tile_ranges((const uint32_t *)c->keys.p, npairs, nullptr, (uint32_t *)c->tile_start.p,
                            (uint32_t *)c->tile_end.p, c->stream);
     }
     RasterArgs a;
@@ -726,14 +726,13 @@ static int flush_pass(fgl_ctx *c)
         r.materials = (const DevMaterial *)c->d_materials.p;
         r.textures = (const DevTexture *)c->d_textures.p;
         r.shadow_map = (const float *)c->shadow.p;
-        r.u = u;
         if (pass == kPassGeometry) {
             KernelTimer t(c, "gbuffer_resolve");
-            launch_gbuffer_resolve(r, c->stream);
+            launch_gbuffer_resolve(r, u, c->stream);
             c->pend_gbuf = false;
         } else {
             KernelTimer t(c, "forward_resolve");
-            launch_forward_resolve(r, c->stream);
+            launch_forward_resolve(r, u, c->stream);
             c->pend_fb = false;
         }
         c->pend_depth = false;
@@ -971,6 +970,9 @@ int ctx_view(fgl_ctx *c, CtxView *v)
 void ctx_ao_written(fgl_ctx *c) { c->pend_ao = false; }
 
 int ctx_fail(fgl_ctx *c, int code, const char *msg) { return fail(c, code, msg); }
+
+CtxTimer::CtxTimer(fgl_ctx *c, const char *name) : impl_(new KernelTimer(c, name)) {}
+CtxTimer::~CtxTimer() { delete static_cast<KernelTimer *>(impl_); }
 
 }  // namespace fgl
 

@@ -27,4 +27,18 @@ int ctx_view(fgl_ctx *c, CtxView *v);
 void ctx_ao_written(fgl_ctx *c);
 int ctx_fail(fgl_ctx *c, int code, const char *msg);
 
+// Scoped HIP-event timing of the kernels enqueued on the context stream
+// while it lives, reported by fgl_read_timings under `name` (a string
+// literal) when timing is enabled.
+class CtxTimer {
+public:
+    CtxTimer(fgl_ctx *c, const char *name);
+    ~CtxTimer();
+    CtxTimer(const CtxTimer &) = delete;
+    CtxTimer &operator=(const CtxTimer &) = delete;
+
+private:
+    void *impl_;
+};
+
 }  // namespace fgl

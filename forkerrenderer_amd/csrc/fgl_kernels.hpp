@@ -40,7 +40,6 @@ struct ResolveArgs {
     const DevMaterial *materials;
     const DevTexture *textures;
     const float *shadow_map;
-    FrameUniforms u;
 };
 
 struct SsaoArgs {
@@ -61,11 +60,12 @@ void launch_emit(const uint32_t *off, int n_tri, const uint2 *rect, int tiles_x,
                  hipStream_t s);
 void launch_fill_u32(uint32_t *p, uint32_t v, size_t n, hipStream_t s);
 void launch_fill_f32(float *p, float v, size_t n, hipStream_t s);
-void launch_tile_ranges(const uint32_t *keys, uint32_t n, uint32_t *start, uint32_t *end,
-                        hipStream_t s);
+// n_dev (optional): device-side pair count, used as min(n, *n_dev)
+void launch_tile_ranges(const uint32_t *keys, uint32_t n, const uint32_t *n_dev, uint32_t *start,
+                        uint32_t *end, hipStream_t s);
 void launch_raster(bool shadow, int TS, const RasterArgs &a, hipStream_t s);
-void launch_gbuffer_resolve(const ResolveArgs &a, hipStream_t s);
-void launch_forward_resolve(const ResolveArgs &a, hipStream_t s);
+void launch_gbuffer_resolve(const ResolveArgs &a, const FrameUniforms &u, hipStream_t s);
+void launch_forward_resolve(const ResolveArgs &a, const FrameUniforms &u, hipStream_t s);
 
 // sort.hip
 // Exclusive scan of n uint32 values into out (out may alias in); the total

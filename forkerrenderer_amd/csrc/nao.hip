@@ -672,12 +672,21 @@ int fgl_nao_infer(fgl_nao *n)
     }
     Mat16 V;
     std::memcpy(V.m, cv.view, sizeof(V.m));
-    hipLaunchKernelGGL(k_features, g2(cv.W, cv.H), dim3(256), 0, cv.stream, V, cv.depth, cv.gbuf, cv.W, cv.H,
-                       n->ifeat, n->imask);
+    {
+        CtxTimer t(n->ctx, "nao_features");
+        hipLaunchKernelGGL(k_features, g2(cv.W, cv.H), dim3(256), 0, cv.stream, V, cv.depth, cv.gbuf, cv.W,
+                           cv.H, n->ifeat, n->imask);
+    }
     uint16_t *acts[4] = {n->ia, n->ib, n->ia, n->ib};
-    forward_convs(n, cv.stream, n->ifeat, acts, cv.W, cv.H, 1);
-    hipLaunchKernelGGL(k_head_fwd, dim3((unsigned)((px + 255) / 256)), dim3(256), 0, cv.stream, n->params,
-                       n->ib, n->imask, px, cv.ao);
+    {
+        CtxTimer t(n->ctx, "nao_conv");
+        forward_convs(n, cv.stream, n->ifeat, acts, cv.W, cv.H, 1);
+    }
+    {
+        CtxTimer t(n->ctx, "nao_head");
+        hipLaunchKernelGGL(k_head_fwd, dim3((unsigned)((px + 255) / 256)), dim3(256), 0, cv.stream, n->params,
+                           n->ib, n->imask, px, cv.ao);
+    }
     ctx_ao_written(n->ctx);
     return hipGetLastError() == hipSuccess ? FGL_OK : FGL_ERR_DEVICE;
 }

@@ -1,0 +1,347 @@
+// raster_core.hpp -- per-vertex, per-triangle, per-tile and per-pixel bodies
+// of the raster pipeline, shared by the single-view kernels (raster.hip) and
+// the batched multi-view kernels (batch.hip), so both run the same
+// arithmetic.
+//
+// Exactness: coverage is the reference's Barycentric (src/geometry.cpp:20-56).
+// When every window coordinate satisfies |X|,|Y| <= 8191 the FP32 differences
+// and FP64 cross products are exact integers, so they are computed in int32
+// and only u = cx*inv, v = cy*inv, 1-(u+v) run in FP64 -- the same doubles.
+// Otherwise the literal FP32/FP64 sequence runs.
+#pragma once
+
+#include "fgl_kernels.hpp"
+
+namespace fgl {
+
+constexpr int kExactLimit = 8191;
+
+// Window coordinates of one vertex: Viewport * NDC with all four terms of
+// each row, truncated like x86 cvttss2si (src/forkergl.cpp:244-249).
+__device__ __forceinline__ int4 to_screen(const float *vp, Vec4 ndc)
+{
+    Vec4 c = mv4(vp, ndc);
+    int X = trunc_int(c.x), Y = trunc_int(c.y);
+    int exact = (X >= -kExactLimit && X <= kExactLimit && Y >= -kExactLimit && Y <= kExactLimit);
+    return make_int4(X, Y, __float_as_int(c.z), exact);
+}
+
+// BoundBox::GenerateBoundBox (src/forkergl.cpp:145-163)
+__device__ __forceinline__ void clamped_bbox(int4 a, int4 b, int4 c, int W, int H, int &minx,
+                                             int &miny, int &maxx, int &maxy)
+{
+    minx = clampi(imin(a.x, imin(b.x, c.x)), 0, W - 1);
+    miny = clampi(imin(a.y, imin(b.y, c.y)), 0, H - 1);
+    maxx = clampi(imax(a.x, imax(b.x, c.x)), 0, W - 1);
+    maxy = clampi(imax(a.y, imax(b.y, c.y)), 0, H - 1);
+}
+
+// Cross-product z of Barycentric, which does not depend on the pixel:
+// |cz| > 1e-2 decides whether the triangle can cover anything.
+__device__ __forceinline__ bool nondegenerate_general(int4 a, int4 b, int4 c)
+{
+    float Ax = (float)a.x, Ay = (float)a.y, Bx = (float)b.x, By = (float)b.y;
+    float Cx = (float)c.x, Cy = (float)c.y;
+    double s0x = (double)(Bx - Ax), s0y = (double)(Cx - Ax);
+    double s1x = (double)(By - Ay), s1y = (double)(Cy - Ay);
+    double uz = s0x * s1y - s0y * s1x;
+    return fabs(uz) > 1e-2;
+}
+
+// Triangle setup of face f of draw d (pass-wide triangle id g): clamped
+// bbox, degenerate / off-screen cull, screen-tile rectangle and count.
+__device__ __forceinline__ void setup_tri(const DevDraw &dr, int d, int f,
+                                          const int4 *__restrict__ screen, int W, int H, int TS,
+                                          uint2 *__restrict__ rect, uint32_t *__restrict__ count,
+                                          int4 *__restrict__ tri_v,
+                                          uint32_t *__restrict__ big_list,
+                                          uint32_t *__restrict__ big_count)
+{
+    const int g = dr.tri_base + f;
+    const int v0 = dr.vert_base + dr.vi[3 * f], v1 = dr.vert_base + dr.vi[3 * f + 1];
+    const int v2 = dr.vert_base + dr.vi[3 * f + 2];
+    const int4 a = screen[v0], b = screen[v1], c = screen[v2];
+    int minx, miny, maxx, maxy;
+    clamped_bbox(a, b, c, W, H, minx, miny, maxx, maxy);
+    const bool exact = a.w && b.w && c.w && W <= 8192 && H <= 8192;
+    bool keep;
+    if (exact) {
+        int cz = (b.x - a.x) * (c.y - a.y) - (c.x - a.x) * (b.y - a.y);
+        int ux0 = imin(a.x, imin(b.x, c.x)), ux1 = imax(a.x, imax(b.x, c.x));
+        int uy0 = imin(a.y, imin(b.y, c.y)), uy1 = imax(a.y, imax(b.y, c.y));
+        // Every clamped-bbox pixel of an off-screen triangle lies strictly
+        // outside it, and in the exact regime one computed weight is then
+        // strictly negative: nothing can be covered.
+        keep = cz != 0 && !(ux1 < 0 || ux0 > W - 1 || uy1 < 0 || uy0 > H - 1);
+    } else {
+        keep = nondegenerate_general(a, b, c);
+    }
+    const uint32_t tx0 = minx / TS, tx1 = maxx / TS, ty0 = miny / TS, ty1 = maxy / TS;
+    const uint32_t n = keep ? (tx1 - tx0 + 1) * (ty1 - ty0 + 1) : 0u;
+    rect[g] = make_uint2(tx0 | (tx1 << 16), ty0 | (ty1 << 16));
+    count[g] = n;
+    tri_v[g] = make_int4(v0, v1, v2, d);
+    // The big-triangle list only decides which workgroup writes a triangle's
+    // pairs; their positions come from the scan, so its order is irrelevant.
+    if (n > (uint32_t)kSmallTriTiles) big_list[atomicAdd(big_count, 1u)] = (uint32_t)g;
+}
+
+// Pair j of triangle t: key = tile_base + tile index, row-major over the
+// triangle's tile rectangle.
+__device__ __forceinline__ uint32_t pair_key(uint2 r, uint32_t j, int tiles_x, uint32_t tile_base)
+{
+    const uint32_t tx0 = r.x & 0xffff, tx1 = r.x >> 16, ty0 = r.y & 0xffff;
+    const uint32_t wx = tx1 - tx0 + 1;
+    return tile_base + (ty0 + j / wx) * (uint32_t)tiles_x + tx0 + j % wx;
+}
+
+// Per-triangle record staged in LDS by the rasteriser.
+struct TriL {
+    int X0, Y0, X1, Y1, X2, Y2;
+    int s0x, s0y, s1x, s1y;  // B-A, C-A (x then y); valid when exact
+    float z0, z1, z2;
+    int bb0, bb1;            // minx | miny << 16, maxx | maxy << 16
+    int cz;                  // exact cross product z
+    int exact;
+    int g;
+    double inv;              // 1.0 / cz (exact regime)
+};
+
+__device__ __forceinline__ void stage_tri(TriL &t, int g, int4 a, int4 b, int4 c, int W, int H)
+{
+    t.X0 = a.x; t.Y0 = a.y; t.X1 = b.x; t.Y1 = b.y; t.X2 = c.x; t.Y2 = c.y;
+    t.z0 = __int_as_float(a.z); t.z1 = __int_as_float(b.z); t.z2 = __int_as_float(c.z);
+    int minx, miny, maxx, maxy;
+    clamped_bbox(a, b, c, W, H, minx, miny, maxx, maxy);
+    t.bb0 = minx | (miny << 16);
+    t.bb1 = maxx | (maxy << 16);
+    t.exact = a.w && b.w && c.w && W <= 8192 && H <= 8192;
+    t.g = g;
+    if (t.exact) {
+        t.s0x = b.x - a.x; t.s0y = c.x - a.x; t.s1x = b.y - a.y; t.s1y = c.y - a.y;
+        t.cz = t.s0x * t.s1y - t.s0y * t.s1x;
+        t.inv = 1.0 / (double)t.cz;
+    } else {
+        t.s0x = t.s0y = t.s1x = t.s1y = t.cz = 0;
+        t.inv = 0.0;
+    }
+}
+
+// Coverage of pixel (px,py) by a staged triangle; on success the three
+// barycentric weights exactly as Barycentric returns them.
+__device__ __forceinline__ bool cover(const TriL &t, int px, int py, float &b0, float &b1, float &b2)
+{
+    if (t.exact) {
+        int s0z = t.X0 - px, s1z = t.Y0 - py;
+        int cx = t.s0y * s1z - s0z * t.s1y;
+        int cy = s0z * t.s1x - t.s0x * s1z;
+        // exact sign pre-test: weights cx/cz, cy/cz and (cz-cx-cy)/cz
+        if (t.cz > 0) {
+            if (cx < 0 || cy < 0 || cx + cy > t.cz) return false;
+        } else {
+            if (cx > 0 || cy > 0 || cx + cy < t.cz) return false;
+        }
+        double u = (double)cx * t.inv, v = (double)cy * t.inv;
+        float r0 = (float)(1.0 - (u + v)), r1 = (float)u, r2 = (float)v;
+        if (r0 < 0.f || r1 < 0.f || r2 < 0.f) return false;
+        b0 = r0; b1 = r1; b2 = r2;
+        return true;
+    }
+    return barycentric_ref(t.X0, t.Y0, t.X1, t.Y1, t.X2, t.Y2, px, py, &b0, &b1, &b2);
+}
+
+// XCD-aware tile order: workgroups are dealt round-robin to the 8 XCDs, so
+// give each XCD a contiguous run of tiles (neighbouring tiles share
+// triangles and vertex data in that XCD's L2).
+__device__ __forceinline__ int xcd_remap(int bid, int n)
+{
+    const int X = 8;
+    int per = n / X, rem = n % X;
+    int xcd = bid % X, local = bid / X;
+    int start = xcd * per + imin(xcd, rem);
+    return start + local;
+}
+
+// One TS x TS tile of one view.  With 4 wavefronts each owns a quadrant of
+// QS = TS/2; with one wavefront it owns the tile.  Lane pixel k of a
+// wavefront: lin = lane + 64k, (lin % QS, lin / QS) inside the quadrant.
+// Chunks of the tile's triangle list are staged in LDS; each wavefront keeps,
+// by ballot, the triangles whose bbox meets its quadrant and walks them in
+// list order; each lane folds the depth test of src/forkergl.cpp:180-199 for
+// its pixels in submission order (no atomics) and records the winner.
+// `a.tile_start` / `a.tile_end` are indexed by the view-local tile.
+template <int TS, int BLOCK, bool SHADOW>
+__device__ __forceinline__ void raster_tile(const RasterArgs &a, int tile, TriL *lds)
+{
+    constexpr int NW = BLOCK / 64;
+    constexpr int QS = (NW == 1) ? TS : TS / 2;
+    constexpr int NPX = QS * QS / 64;
+    const int tx = tile % a.tiles_x, ty = tile / a.tiles_x;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int qx0 = tx * TS + (NW == 1 ? 0 : (wave & 1) * QS);
+    const int qy0 = ty * TS + (NW == 1 ? 0 : (wave >> 1) * QS);
+    int px[NPX], py[NPX], win[NPX];
+    float best[NPX];
+    bool act[NPX];
+#pragma unroll
+    for (int k = 0; k < NPX; ++k) {
+        const int lin = lane + 64 * k;
+        px[k] = qx0 + lin % QS;
+        py[k] = qy0 + lin / QS;
+        act[k] = px[k] < a.W && py[k] < a.H;
+        win[k] = -1;
+        best[k] = (act[k] && !a.fresh) ? a.depth[(size_t)px[k] + (size_t)py[k] * a.W] : 3.40282347e+38f;
+    }
+    const int rx1 = qx0 + QS - 1, ry1 = qy0 + QS - 1;
+    const uint32_t start = a.tile_start[tile], end = a.tile_end[tile];
+    for (uint32_t base = start; base < end; base += BLOCK) {
+        const int n = imin(BLOCK, (int)(end - base));
+        if (tid < n) {
+            const int g = (int)a.vals[base + tid];
+            const int4 tv = a.tri_v[g];
+            stage_tri(lds[tid], g, a.screen[tv.x], a.screen[tv.y], a.screen[tv.z], a.W, a.H);
+        }
+        __syncthreads();
+        // Ballot per 64 staged triangles: does its bbox meet this quadrant?
+        unsigned long long mask[NW];
+#pragma unroll
+        for (int j = 0; j < NW; ++j) {
+            const int s = j * 64 + lane;
+            bool hit = false;
+            if (s < n) {
+                const TriL &t = lds[s];
+                const int minx = t.bb0 & 0xffff, miny = t.bb0 >> 16;
+                const int maxx = t.bb1 & 0xffff, maxy = t.bb1 >> 16;
+                hit = !(maxx < qx0 || minx > rx1 || maxy < qy0 || miny > ry1);
+            }
+            mask[j] = __ballot(hit);
+        }
+#pragma unroll
+        for (int j = 0; j < NW; ++j) {
+            unsigned long long m = mask[j];
+            while (m) {
+                const int i = j * 64 + __builtin_ctzll(m);
+                m &= m - 1;
+                const TriL &t = lds[i];
+                const int minx = t.bb0 & 0xffff, miny = t.bb0 >> 16;
+                const int maxx = t.bb1 & 0xffff, maxy = t.bb1 >> 16;
+#pragma unroll
+                for (int k = 0; k < NPX; ++k) {
+                    if (act[k] && px[k] >= minx && px[k] <= maxx && py[k] >= miny && py[k] <= maxy) {
+                        float b0, b1, b2;
+                        if (cover(t, px[k], py[k], b0, b1, b2)) {
+                            // Dot(bary, depths) and the test of forkergl.cpp:180-199
+                            float dd = 0.f;
+                            dd += b0 * t.z0;
+                            dd += b1 * t.z1;
+                            dd += b2 * t.z2;
+                            if (!(dd >= best[k])) {
+                                best[k] = dd;
+                                win[k] = t.g;
+                            }
+                        }
+                    }
+                }
+            }
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int k = 0; k < NPX; ++k) {
+        if (!act[k]) continue;
+        const size_t i = (size_t)px[k] + (size_t)py[k] * a.W;
+        if (win[k] >= 0 || a.fresh) a.depth[i] = best[k];
+        if (!SHADOW) {
+            a.vis[i] = win[k];
+        } else if (win[k] >= 0) {
+            // DepthShader::ProcessFragment (depthshader.h:30-36)
+            const int4 tv = a.tri_v[win[k]];
+            const DevDraw &dr = a.draws[tv.w];
+            TriL t;
+            stage_tri(t, win[k], a.screen[tv.x], a.screen[tv.y], a.screen[tv.z], a.W, a.H);
+            float b0 = 0.f, b1 = 0.f, b2 = 0.f;
+            cover(t, px[k], py[k], b0, b1, b2);
+            const int f = win[k] - dr.tri_base;
+            Vec4 n0 = depth_ndc(dr, dr.vi[3 * f]);
+            Vec4 n1 = depth_ndc(dr, dr.vi[3 * f + 1]);
+            Vec4 n2 = depth_ndc(dr, dr.vi[3 * f + 2]);
+            float z = dot3(vec3(n0.z, n1.z, n2.z), vec3(b0, b1, b2));
+            a.shadow_out[i] = z * 0.5f + 0.5f;
+        } else if (a.fresh) {
+            a.shadow_out[i] = 0.f;
+        }
+    }
+}
+
+// Barycentric weights of the winning triangle at (px,py), recomputed with the
+// same staged arithmetic as the coverage loop.
+__device__ __forceinline__ Vec3 winner_bary(const int4 *__restrict__ screen,
+                                            const int4 *__restrict__ tri_v, int g, int W, int H,
+                                            int px, int py, int4 &tv)
+{
+    tv = tri_v[g];
+    TriL t;
+    stage_tri(t, g, screen[tv.x], screen[tv.y], screen[tv.z], W, H);
+    float b0 = 0.f, b1 = 0.f, b2 = 0.f;
+    cover(t, px, py, b0, b1, b2);
+    return vec3(b0, b1, b2);
+}
+
+// Geometry pass resolve of pixel (x,y): GShader::ProcessFragment + the seven
+// buffer writes (gshader.h:95-202, forkergl.cpp:211-223) for the visible
+// fragment; a pixel with no fragment gets the cleared values when `fresh`.
+__device__ __forceinline__ void gbuffer_px(const ResolveArgs &a, const FrameUniforms &u, int x, int y)
+{
+    const size_t i = (size_t)x + (size_t)y * a.W;
+    const size_t P = (size_t)a.W * a.H;
+    float *gb = a.gbuf;
+    const int g = a.vis[i];
+    if (g < 0) {
+        if (a.fresh)
+            for (int c = 0; c < kGbufPlanes; ++c) gb[c * P + i] = 0.f;
+        return;
+    }
+    int4 tv;
+    const Vec3 b = winner_bary(a.screen, a.tri_v, g, a.W, a.H, x, y, tv);
+    const DevDraw &dr = a.draws[tv.w];
+    const int f = g - dr.tri_base;
+    Varying v0 = main_vertex(dr, u, dr.vi[3 * f], dr.ti[3 * f], dr.ni[3 * f]);
+    Varying v1 = main_vertex(dr, u, dr.vi[3 * f + 1], dr.ti[3 * f + 1], dr.ni[3 * f + 1]);
+    Varying v2 = main_vertex(dr, u, dr.vi[3 * f + 2], dr.ti[3 * f + 2], dr.ni[3 * f + 2]);
+    const DevMaterial &m = a.materials[dr.face_mat[f]];
+    FragGeom fg = frag_geometry(dr, u, a.textures, v0, v1, v2, b, m.map_bump);
+    Vec3 albedo, emissive, param;
+    float type;
+    if (dr.support_pbr) {
+        albedo = m.map_base >= 0 ? tex_sample(a.textures, m.map_base, fg.uv)
+                                 : vec3(m.albedo[0], m.albedo[1], m.albedo[2]);
+        emissive = m.map_pbr_ke >= 0 ? tex_sample(a.textures, m.map_pbr_ke, fg.uv)
+                                     : vec3(m.ke[0], m.ke[1], m.ke[2]);
+        float rough = m.map_rough >= 0 ? tex_samplef(a.textures, m.map_rough, fg.uv) : m.roughness;
+        float metal = m.map_metal >= 0 ? tex_samplef(a.textures, m.map_metal, fg.uv) : m.metalness;
+        float ao = m.map_ao >= 0 ? tex_samplef(a.textures, m.map_ao, fg.uv) : 1.f;
+        param = vec3(ao, metal, rough);
+        type = 1.f;
+    } else {
+        emissive = m.map_ke >= 0 ? tex_sample(a.textures, m.map_ke, fg.uv)
+                                 : vec3(m.ke[0], m.ke[1], m.ke[2]);
+        albedo = m.map_kd >= 0 ? tex_sample(a.textures, m.map_kd, fg.uv)
+                               : vec3(m.kd[0], m.kd[1], m.kd[2]);
+        float shin = m.map_ks >= 0 ? tex_samplef(a.textures, m.map_ks, fg.uv) + 5 : 1.f;
+        param = vec3(1.f, m.ks[0], shin);
+        type = 0.f;
+    }
+    gb[0 * P + i] = fg.normal.x; gb[1 * P + i] = fg.normal.y; gb[2 * P + i] = fg.normal.z;
+    gb[3 * P + i] = fg.pos.x; gb[4 * P + i] = fg.pos.y; gb[5 * P + i] = fg.pos.z;
+    if (u.shadow_on) {
+        gb[6 * P + i] = fg.lndc.x; gb[7 * P + i] = fg.lndc.y; gb[8 * P + i] = fg.lndc.z;
+    } else if (a.fresh) {
+        gb[6 * P + i] = 0.f; gb[7 * P + i] = 0.f; gb[8 * P + i] = 0.f;
+    }
+    gb[9 * P + i] = albedo.x; gb[10 * P + i] = albedo.y; gb[11 * P + i] = albedo.z;
+    gb[12 * P + i] = emissive.x; gb[13 * P + i] = emissive.y; gb[14 * P + i] = emissive.z;
+    gb[15 * P + i] = param.x; gb[16 * P + i] = param.y; gb[17 * P + i] = param.z;
+    gb[18 * P + i] = type;
+}
+
+}  // namespace fgl

@@ -17,8 +17,6 @@ import numpy as np
 from . import (PASS_GEOMETRY, SsaoCfg, Context, FglError, lib, make_lookat, make_model_matrix,
                make_perspective, scenegen)
 
-NPARAM = None
-
 
 def _bind():
     L = lib()
@@ -160,44 +158,53 @@ def ssao_cfg(seed=2, n=64, radius=0.5, sigma=2.0):
 
 
 class SceneSet:
-    """Meshes for the training/evaluation scenes, uploaded once into a context."""
+    """Assets of the deferred-PBR scene generator (displaced icosphere of
+    radius 1, ground plane, 2048^2-class albedo / roughness / metalness /
+    normal maps with generated tangents), uploaded once into a context."""
 
-    def __init__(self, ctx, workdir, subdiv=5, seed=100):
+    def __init__(self, ctx, workdir, subdiv=5, seed=100, tex_size=512):
+        from . import Scene
         os.makedirs(workdir, exist_ok=True)
-        v, n, uv, f = scenegen.displaced_icosphere(subdiv, seed, 0.1, 1.0)
-        from . import Material
-        m = Material()
-        m.kd[:] = (0.8, 0.8, 0.8)
-        m.albedo[:] = (0.8, 0.8, 0.8)
-        m.ks[:] = (0.3, 0.3, 0.3)
-        for fld in ("map_kd", "map_ks", "map_ke", "map_bump", "map_base", "map_rough", "map_metal", "map_ao",
-                    "map_pbr_ke"):
-            setattr(m, fld, -1)
-        mid = ctx.upload_material(m)
-        f32 = np.float32
-        self.sphere = ctx.upload_model({"pos": v.astype(f32), "uv": uv.astype(f32), "nrm": n.astype(f32),
-                                        "tan": None, "vi": f, "ti": f, "ni": f,
-                                        "face_material": np.full(len(f), mid, np.int32), "support_pbr": False})
-        gp, gn, guv, gf = scenegen.plane_group(4.0)
-        self.ground = ctx.upload_model({"pos": gp, "uv": guv, "nrm": gn, "tan": None, "vi": gf, "ti": gf, "ni": gf,
-                                        "face_material": np.full(len(gf), mid, np.int32), "support_pbr": False})
+        scenegen._pbr_scene_assets(workdir, seed, subdiv, tex_size)
+        p = os.path.join(workdir, "assets.scene")
+        with open(p, "w") as fh:
+            fh.write(f"mode deferred\nlight point 2 5 5 3 3 3\ntexture repeat linear\n"
+                     f"model {workdir}/ground.obj false true 0 0 0 0 1\n"
+                     f"model {workdir}/sphere.obj false true 0 0 0 0 1\n")
+        (self.ground, _), (self.sphere, _) = ctx.upload_scene(Scene(p))
 
 
 def layout(seed):
-    """A random scene of the deferred-PBR generator family: 1-6 displaced
-    icospheres (radius 0.3-1.0) on a ground plane, seeded."""
+    """A random scene, seeded: a ground plane of half-size 4 and displaced
+    icospheres in one of three arrangements -- one large sphere (radius
+    0.8-1.2), 1-6 scattered spheres (radius 0.3-1.0), or an n x n grid
+    (n 3-8, spacing 0.8) of small spheres (radius 0.15-0.45) -- resting on
+    or sunk into the ground."""
     rng = np.random.default_rng(seed)
+    kind = int(rng.integers(0, 3))
     objs = []
-    for _ in range(int(rng.integers(1, 7))):
-        r = float(rng.uniform(0.3, 1.0))
-        x, z = rng.uniform(-2.2, 2.2, 2)
-        y = float(rng.uniform(0.55, 1.0)) * r   # resting on / sunk into the ground
-        objs.append(make_model_matrix((x, y, z), float(rng.uniform(0, 360)), r))
+    if kind == 0:
+        r = float(rng.uniform(0.8, 1.2))
+        objs.append(((0.0, float(rng.uniform(0.55, 1.0)) * r, 0.0), float(rng.uniform(0, 360)), r))
+    elif kind == 1:
+        for _ in range(int(rng.integers(1, 7))):
+            r = float(rng.uniform(0.3, 1.0))
+            x, z = rng.uniform(-2.2, 2.2, 2)
+            objs.append(((float(x), float(rng.uniform(0.55, 1.0)) * r, float(z)), float(rng.uniform(0, 360)), r))
+    else:
+        n = int(rng.integers(3, 9))
+        for gz in range(n):
+            for gx in range(n):
+                r = float(rng.uniform(0.15, 0.45))
+                x, z = (gx - (n - 1) / 2) * 0.8, (gz - (n - 1) / 2) * 0.8
+                objs.append(((x, float(rng.uniform(0.55, 1.6)) * r, z), float(rng.uniform(0, 360)), r))
     ground = make_model_matrix((0.0, 0.0, 0.0), 0.0, 4.0)
-    return ground, objs
+    return ground, [make_model_matrix(t, rot, r) for t, rot, r in objs]
 
 
 def camera(seed):
+    """Random orbit pose: distance 3.5-7, elevation 12-55 degrees, any azimuth,
+    target jittered around the origin."""
     rng = np.random.default_rng(seed)
     r = rng.uniform(3.5, 7.0)
     el = math.radians(rng.uniform(12, 55))
@@ -231,11 +238,13 @@ def render_frame(ctx, sset, frame_seed, W, H, cfg, plain_ao=True):
 
 
 def frame_seeds(split, n):
-    """Training frames: layouts 0..7 x 500 poses; evaluation frames: layouts
-    100..103 x 50 poses (held out)."""
+    """Frame seeds layout * 1000 + pose.  Training: layouts 0..399, frame i
+    is layout i % 400, pose i // 400 (4000 frames = 10 poses each).
+    Evaluation: held-out layouts 1000..1019, frame i is layout 1000 + i % 20,
+    pose i // 20 (200 frames = 10 poses each)."""
     if split == "train":
-        return [l * 1000 + p for l in range(8) for p in range(n // 8)]
-    return [(100 + l) * 1000 + p for l in range(4) for p in range(n // 4)]
+        return [(i % 400) * 1000 + i // 400 for i in range(n)]
+    return [(1000 + i % 20) * 1000 + i // 20 for i in range(n)]
 
 
 def step_batch(step, seed, frames, W, H, crop, batch):
@@ -253,30 +262,47 @@ def step_batch(step, seed, frames, W, H, crop, batch):
     return out
 
 
-def train(nao, ctx, sset, steps, start=0, seed=0, W=1280, H=800, frames=4000, lr=1e-3, log=None, cfg=None):
-    """Steps start .. start+steps-1 of a training run; returns per-step
-    (sum sq err, sum |err|, covered pixels)."""
+def lr_at(step, total, base=1e-3):
+    """Adam learning rate: `base` for the first 3/4 of the run, then a linear
+    decay to base/100 at the last step."""
+    t0 = 0.75 * total
+    if total <= 0 or step < t0:
+        return base
+    return base * (1.0 - 0.99 * min(1.0, (step - t0) / max(total - t0, 1.0)))
+
+
+def train(nao, ctx, sset, steps, start=0, total=None, seed=0, W=1280, H=800, frames=4000, lr=1e-3, log=None,
+          cfg=None):
+    """Steps start .. start+steps-1 of a training run of `total` steps
+    (default start+steps); returns per-step (sum sq err, sum |err|, covered
+    pixels)."""
     cfg = cfg if cfg is not None else ssao_cfg()
+    total = start + steps if total is None else total
     per = nao.batch // 4
     hist = []
     for step in range(start, start + steps):
         for j, (fs, xy) in enumerate(step_batch(step, seed, frames, W, H, nao.crop, nao.batch)):
             render_frame(ctx, sset, fs, W, H, cfg)
             nao.stage(j * per, xy)
-        sse, sae, cnt = nao.train_step(lr)
+        sse, sae, cnt = nao.train_step(lr_at(step, total, lr))
         hist.append((float(sse), float(sae), float(cnt)))
-        if log and (step % 50 == 0 or step == start + steps - 1):
-            log(f"step {step}: mse {sse / max(cnt, 1):.5f} mae {sae / max(cnt, 1):.4f}")
+        if log and (step % 500 == 0 or step == start + steps - 1):
+            k = hist[-min(len(hist), 100):]
+            n = max(sum(h[2] for h in k), 1)
+            log(f"step {step}: lr {lr_at(step, total, lr):.2e} mse(100) {sum(h[0] for h in k) / n:.5f} "
+                f"mae(100) {sum(h[1] for h in k) / n:.4f}")
     return hist
 
 
-def save_checkpoint(path, nao, seed, frames, W=1280, H=800):
+def save_checkpoint(path, nao, seed, frames, W=1280, H=800, evaluation=None):
     """Weights, Adam moments, step and the data recipe (seed, frame count,
-    frame size, crop, batch) in one .npz."""
+    frame size, crop, batch) in one .npz, plus the held-out evaluation as
+    JSON text when given."""
+    import json
     st, step = nao.state()
     np.savez(path, state=st, step=np.int64(step), seed=np.int64(seed), frames=np.int64(frames),
              size=np.array([W, H], np.int64), crop=np.int64(nao.crop), batch=np.int64(nao.batch),
-             nparam=np.int64(param_count()))
+             nparam=np.int64(param_count()), eval_json=np.array(json.dumps(evaluation or {})))
 
 
 def load_checkpoint(path):
@@ -367,7 +393,7 @@ def main(argv=None):
     import json
     import time
     ap = argparse.ArgumentParser()
-    ap.add_argument("--steps", type=int, default=2000)
+    ap.add_argument("--total", type=int, default=8000, help="steps of the whole run (the lr schedule's length)")
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--frames", type=int, default=4000)
     ap.add_argument("--eval-frames", type=int, default=200)
@@ -387,12 +413,18 @@ def main(argv=None):
         start = int(ck["step"])
     base = evaluate(nao, ctx, sset, min(a.eval_frames, 40), cfg=cfg) if start == 0 else None
     t0 = time.time()
-    hist = train(nao, ctx, sset, a.steps, start=start, seed=a.seed, frames=a.frames, lr=a.lr, cfg=cfg,
-                 log=lambda s: print(s, f"({time.time() - t0:.1f}s)", flush=True))
+    hist = []
+    chunk = 2000
+    for s0 in range(start, a.total, chunk):
+        n = min(chunk, a.total - s0)
+        hist += train(nao, ctx, sset, n, start=s0, total=a.total, seed=a.seed, frames=a.frames, lr=a.lr, cfg=cfg,
+                      log=lambda s: print(s, f"({time.time() - t0:.1f}s)", flush=True))
+        ev = evaluate(nao, ctx, sset, 20, cfg=cfg)
+        print(f"step {s0 + n}: held-out MAE (20 frames) {ev['mae']:.4f}", flush=True)
     t_train = time.time() - t0
-    save_checkpoint(a.out, nao, a.seed, a.frames)
     ev = evaluate(nao, ctx, sset, a.eval_frames, cfg=cfg, unblurred=True)
-    rec = {"steps": start + a.steps, "train_s": t_train, "final_train_mse": hist[-1][0] / max(hist[-1][2], 1),
+    save_checkpoint(a.out, nao, a.seed, a.frames, evaluation=ev)
+    rec = {"steps": a.total, "train_s": t_train, "final_train_mse": hist[-1][0] / max(hist[-1][2], 1),
            "eval": ev, "untrained_eval40": base}
     print(json.dumps(rec), flush=True)
     nao.close()

@@ -269,7 +269,7 @@ model {d}/sphere.obj false true 0 0 0 0 1
     return p
 
 
-def config3(d, seed=3, subdiv=6, tex_size=2048, width=1920, height=1080, ssaa=2, grid=8):
+def config3(d, seed=3, subdiv=6, tex_size=2048, width=1920, height=1080, ssaa=2, grid=8, name="c3"):
     """Heavy deferred PBR at 1920x1080 with SSAA 2x, PCSS 16/32 area 0.05, kernel
     SSAO as config 2, an 8x8 grid of displaced icospheres with seeded scale and
     rotation, one directional and one point light."""
@@ -301,10 +301,15 @@ camera persp 0 3.6 5.4 0 -0.3 0
 texture repeat linear
 model {d}/ground.obj false true 0 -0.45 0 0 4
 """ + "\n".join(lines) + "\n"
-    p = os.path.join(d, "c3.scene")
+    p = os.path.join(d, f"{name}.scene")
     with open(p, "w") as fh:
         fh.write(scene)
     return p
+
+
+def config4(d, seed=3):
+    """Config 3 without SSAA (1920x1080 internal): the neural-AO comparison frame."""
+    return config3(d, seed=seed, ssaa=1, name="c4")
 
 
 def tiny_scene(d, seed=0, width=64, height=48, mode="deferred", shadow="hard", ssao=False,

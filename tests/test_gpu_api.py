@@ -10,30 +10,9 @@ import pytest
 import forkerrenderer_amd as F
 from forkerrenderer_amd import scenegen as G
 import oracle as O
-from tests.util import assert_bitexact, gpu_frame, oracle_frame, scene_arrays
+from tests.util import assert_bitexact, gpu_frame, oracle_frame
 
 pytestmark = pytest.mark.gpu
-
-
-def _upload(ctx, sc):
-    models, mats, texs = scene_arrays(sc)
-    tid = [ctx.upload_texture(t["bytes"], t["wrap"], t["filter"]) for t in texs]
-    mid = []
-    for m in mats:
-        mm = F.Material()
-        F.C.memmove(F.C.byref(mm), F.C.byref(m), F.C.sizeof(F.Material))
-        for f in ("map_kd", "map_ks", "map_ke", "map_bump", "map_base", "map_rough", "map_metal", "map_ao",
-                  "map_pbr_ke"):
-            v = getattr(mm, f)
-            if v >= 0:
-                setattr(mm, f, tid[v])
-        mid.append(ctx.upload_material(mm))
-    ids = []
-    for md, M in models:
-        d = dict(md)
-        d["face_material"] = np.array([mid[x] for x in md["face_material"]], np.int32)
-        ids.append((ctx.upload_model(d), M))
-    return ids
 
 
 def test_reads_before_any_pass_return_clear_values(ctx):
@@ -55,7 +34,7 @@ def test_draws_split_over_flushes_equal_one_pass(ctx, scene_dir):
     sc = F.Scene(p)
     want = oracle_frame(sc)
     gpu_frame(ctx, sc)  # sets the context's matrices, lights, shadow map exactly as the scene does
-    ids = _upload(ctx, sc)
+    ids = ctx.upload_scene(sc)
     I = sc.info
     outs = []
     for split in (False, True):
