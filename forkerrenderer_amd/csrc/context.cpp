@@ -559,6 +559,22 @@ static int bits_for(uint32_t n)
     return b;
 }
 
+// Draw record of one model instance (bases left zero): Model::Render's
+// per-draw state -- model matrix, its normal matrix, light * model.
+static void fill_draw(fgl_ctx *c, const ModelRes &m, const float M[16], DevDraw &d)
+{
+    std::memset(&d, 0, sizeof(d));
+    d.pos = m.pos; d.uv = m.uv; d.nrm = m.nrm; d.tan = m.tan;
+    d.vi = m.vi; d.ti = m.ti; d.ni = m.ni; d.face_mat = m.fmat;
+    d.n_pos = m.n_pos;
+    d.n_faces = m.n_faces;
+    d.has_tangents = m.has_tan;
+    d.support_pbr = m.pbr;
+    std::memcpy(d.model, M, sizeof(d.model));
+    make_normal_matrix(d.model, d.normal);
+    mat_mul(c->light_space, d.model, d.lm);
+}
+
 static int flush_pass(fgl_ctx *c)
 {
     if (c->draws.empty()) return FGL_OK;
@@ -584,18 +600,9 @@ static int flush_pass(fgl_ctx *c)
     for (size_t i = 0; i < c->draws.size(); ++i) {
         const ModelRes &m = c->models[c->draws[i].model];
         DevDraw &d = dd[i];
-        std::memset(&d, 0, sizeof(d));
-        d.pos = m.pos; d.uv = m.uv; d.nrm = m.nrm; d.tan = m.tan;
-        d.vi = m.vi; d.ti = m.ti; d.ni = m.ni; d.face_mat = m.fmat;
-        d.n_pos = m.n_pos;
-        d.n_faces = m.n_faces;
+        fill_draw(c, m, c->draws[i].m, d);
         d.vert_base = (int)nv;
         d.tri_base = (int)nt;
-        d.has_tangents = m.has_tan;
-        d.support_pbr = m.pbr;
-        std::memcpy(d.model, c->draws[i].m, sizeof(d.model));
-        make_normal_matrix(d.model, d.normal);
-        mat_mul(c->light_space, d.model, d.lm);
         nt += m.n_faces;
         nv += m.n_pos;
         max_pos = m.n_pos > max_pos ? m.n_pos : max_pos;
@@ -971,7 +978,28 @@ void ctx_ao_written(fgl_ctx *c) { c->pend_ao = false; }
 
 int ctx_fail(fgl_ctx *c, int code, const char *msg) { return fail(c, code, msg); }
 
-CtxTimer::CtxTimer(fgl_ctx *c, const char *name) : impl_(new KernelTimer(c, name)) {}
+void ctx_uniforms(fgl_ctx *c, FrameUniforms *u) { build_uniforms(c, *u); }
+
+int ctx_draw_record(fgl_ctx *c, int model, const float M[16], DevDraw *out)
+{
+    if (model < 0 || model >= (int)c->models.size()) return fail(c, FGL_ERR_INVALID, "unknown model");
+    fill_draw(c, c->models[model], M, *out);
+    return FGL_OK;
+}
+
+int ctx_tables(fgl_ctx *c, const DevMaterial **materials, const DevTexture **textures)
+{
+    int rc = upload_tables(c);
+    if (rc) return rc;
+    *materials = (const DevMaterial *)c->d_materials.p;
+    *textures = (const DevTexture *)c->d_textures.p;
+    return FGL_OK;
+}
+
+hipStream_t ctx_stream(fgl_ctx *c) { return c->stream; }
+bool ctx_timing(fgl_ctx *c) { return c->timing; }
+
+CtxTimer::CtxTimer(fgl_ctx *c, const char *name) : impl_(name ? new KernelTimer(c, name) : nullptr) {}
 CtxTimer::~CtxTimer() { delete static_cast<KernelTimer *>(impl_); }
 
 }  // namespace fgl

@@ -5,6 +5,7 @@
 #include <hip/hip_runtime.h>
 
 #include "../../include/fgl.h"
+#include "fgl_kernels.hpp"
 
 namespace fgl {
 
@@ -27,9 +28,40 @@ int ctx_view(fgl_ctx *c, CtxView *v);
 void ctx_ao_written(fgl_ctx *c);
 int ctx_fail(fgl_ctx *c, int code, const char *msg);
 
+// FrameUniforms from the context's current ForkerGL state (what the next
+// pass would use).
+void ctx_uniforms(fgl_ctx *c, FrameUniforms *u);
+// Draw record of `model` under model matrix M with the context's current
+// light-space matrix: geometry pointers, normal matrix, light * model.  The
+// pass-wide bases (vert_base, tri_base) and the view index are left zero.
+int ctx_draw_record(fgl_ctx *c, int model, const float M[16], DevDraw *out);
+// Uploads pending texture / material tables; returns their device copies.
+int ctx_tables(fgl_ctx *c, const DevMaterial **materials, const DevTexture **textures);
+hipStream_t ctx_stream(fgl_ctx *c);
+bool ctx_timing(fgl_ctx *c);
+
+// Per-view pixel layout of a batch of views, for the batched per-pixel
+// kernels outside the batch engine (neural AO): the view's targets start at
+// pixel px_off of the batch buffers; its 64 x 4 pixel blocks are blocks
+// blk_base .. of the batch-wide block index.
+struct BatchPixView {
+    int W, H;
+    uint32_t blk_base, pad;
+    size_t px_off;
+    float view[16];
+};
+
+// Neural AO of every view of a batch (nao.hip): features from the batch
+// depth / G-buffer, four convolutions, head into the batch AO target.
+// nao_reserve allocates the batch-sized activations beforehand (the infer
+// call then allocates nothing, so it can be captured in a graph).
+int nao_reserve(fgl_nao *n, size_t total_px);
+int nao_infer_batch(fgl_nao *n, hipStream_t s, const BatchPixView *views, const uint8_t *blk_view,
+                    uint32_t n_blocks, size_t total_px, const float *depth, const float *gbuf, float *ao);
+
 // Scoped HIP-event timing of the kernels enqueued on the context stream
 // while it lives, reported by fgl_read_timings under `name` (a string
-// literal) when timing is enabled.
+// literal) when timing is enabled; a null name times nothing.
 class CtxTimer {
 public:
     CtxTimer(fgl_ctx *c, const char *name);

@@ -30,7 +30,9 @@ struct DevDraw {
     const float *pos, *uv, *nrm, *tan;
     const int *vi, *ti, *ni, *face_mat;
     int n_pos, n_faces, vert_base, tri_base;
-    int has_tangents, support_pbr, pad0, pad1;
+    int has_tangents, support_pbr;
+    int view;      // batched views: index of the draw's view (0 otherwise)
+    int pad0;
     float model[16];
     float normal[9];
     float lm[16];  // light_space * model (DepthShader, src/shaders/depthshader.h:23-25)

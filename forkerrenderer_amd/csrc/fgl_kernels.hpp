@@ -78,6 +78,10 @@ void launch_exclusive_scan(const uint32_t *in, uint32_t *out, size_t n, uint32_t
 size_t radix_tmp_words(size_t n);
 void launch_radix_sort(uint32_t *keys, uint32_t *vals, uint32_t *keys_alt, uint32_t *vals_alt,
                        size_t n, int bits, uint32_t *tmp, hipStream_t s);
+// The passes only: returns 1 when the result is in the alt buffers.  With
+// n_dev, n is the capacity and min(n, *n_dev) items are sorted.
+int launch_radix_sort_impl(uint32_t *keys, uint32_t *vals, uint32_t *keys_alt, uint32_t *vals_alt,
+                           size_t n, const uint32_t *n_dev, int bits, uint32_t *tmp, hipStream_t s);
 
 // shade.hip
 void launch_lighting(const FrameUniforms &u, const float *gbuf, const float *ao, int ao_is_one,

@@ -127,17 +127,18 @@ __global__ __launch_bounds__(256) void k_crop(Mat16 V, const float *__restrict__
 // MODE 1 (data gradient): out = bf16(conv(in) * (act > 0)) with tap-flipped,
 // transposed weights; `in` is the layer's output gradient.
 // Workgroup: 4 wavefronts, each 64 consecutive pixels of one row x 32 channels.
+// One wavefront: pixels xb .. xb+63 of row y of the image starting at pixel
+// `img` of `in` / `out` (W x H, NHWC).
 template <int CIN, int MODE>
-__global__ __launch_bounds__(256) void k_conv3x3(const uint16_t *__restrict__ in,
-                                                 const uint16_t *__restrict__ Wt,
-                                                 const float *__restrict__ bias,
-                                                 const uint16_t *__restrict__ act,
-                                                 uint16_t *__restrict__ out, int W, int H)
+__device__ __forceinline__ void conv_row(const uint16_t *__restrict__ in,
+                                         const uint16_t *__restrict__ Wt,
+                                         const float *__restrict__ bias,
+                                         const uint16_t *__restrict__ act,
+                                         uint16_t *__restrict__ out, int W, int H, size_t img,
+                                         int y, int xb)
 {
     constexpr int K = 9 * CIN, KS = K / 16, CG = CIN / 16;
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int y = blockIdx.y * 4 + wave, n = blockIdx.z, xb = blockIdx.x * 64;
-    if (y >= H) return;
+    const int lane = threadIdx.x & 63;
     const int lr = lane & 15, lq = lane >> 4;
     v4s bfr[KS][2];
 #pragma unroll
@@ -150,7 +151,6 @@ __global__ __launch_bounds__(256) void k_conv3x3(const uint16_t *__restrict__ in
     for (int mb = 0; mb < 4; ++mb)
 #pragma unroll
         for (int nb = 0; nb < 2; ++nb) acc[mb][nb] = v4f{0.f, 0.f, 0.f, 0.f};
-    const size_t img = (size_t)n * H * W;
 #pragma unroll
     for (int ks = 0; ks < KS; ++ks) {
         const int tap = ks / CG, cg = ks % CG;
@@ -185,6 +185,46 @@ __global__ __launch_bounds__(256) void k_conv3x3(const uint16_t *__restrict__ in
                 }
                 out[o] = f2bf(v);
             }
+}
+
+template <int CIN, int MODE>
+__global__ __launch_bounds__(256) void k_conv3x3(const uint16_t *__restrict__ in,
+                                                 const uint16_t *__restrict__ Wt,
+                                                 const float *__restrict__ bias,
+                                                 const uint16_t *__restrict__ act,
+                                                 uint16_t *__restrict__ out, int W, int H)
+{
+    const int y = blockIdx.y * 4 + (threadIdx.x >> 6);
+    if (y >= H) return;
+    conv_row<CIN, MODE>(in, Wt, bias, act, out, W, H, (size_t)blockIdx.z * H * W, y, blockIdx.x * 64);
+}
+
+// Batched views: workgroup b is 64 x 4 pixel block b of the batch (view
+// blk_view[b], local block b - blk_base, row-major over the view's blocks).
+__device__ __forceinline__ bool batch_block(const BatchPixView *__restrict__ views,
+                                            const uint8_t *__restrict__ blk_view, int &v, int &x,
+                                            int &y)
+{
+    v = blk_view[blockIdx.x];
+    const BatchPixView &bv = views[v];
+    const int lb = (int)(blockIdx.x - bv.blk_base), nbx = (bv.W + 63) / 64;
+    x = (lb % nbx) * 64 + (threadIdx.x & 63);
+    y = (lb / nbx) * 4 + (threadIdx.x >> 6);
+    return y < bv.H;
+}
+
+template <int CIN>
+__global__ __launch_bounds__(256) void k_conv3x3_batch(const BatchPixView *__restrict__ views,
+                                                       const uint8_t *__restrict__ blk_view,
+                                                       const uint16_t *__restrict__ in,
+                                                       const uint16_t *__restrict__ Wt,
+                                                       const float *__restrict__ bias,
+                                                       uint16_t *__restrict__ out)
+{
+    int v, x, y;
+    if (!batch_block(views, blk_view, v, x, y)) return;
+    const BatchPixView &bv = views[v];
+    conv_row<CIN, 0>(in, Wt, bias, nullptr, out, bv.W, bv.H, bv.px_off, y, x - (threadIdx.x & 63));
 }
 
 // Weight gradient: dW[co][tap*CIN+ci] = sum_p dout[p][co] * in[p+off(tap)][ci]
@@ -309,6 +349,42 @@ __global__ __launch_bounds__(256) void k_head_fwd(const float *__restrict__ para
     y[p] = v;
 }
 
+__global__ __launch_bounds__(256) void k_features_batch(const BatchPixView *__restrict__ views,
+                                                        const uint8_t *__restrict__ blk_view,
+                                                        const float *__restrict__ depth,
+                                                        const float *__restrict__ gbuf,
+                                                        uint16_t *__restrict__ feat,
+                                                        uint8_t *__restrict__ mask)
+{
+    int v, x, y;
+    if (!batch_block(views, blk_view, v, x, y)) return;
+    const BatchPixView &bv = views[v];
+    if (x >= bv.W) return;
+    const size_t P = (size_t)bv.W * bv.H, i = (size_t)x + (size_t)y * bv.W;
+    float f[7];
+    bool cov;
+    pixel_features(bv.view, depth + bv.px_off, gbuf + kGbufPlanes * bv.px_off, P, i, f, cov);
+    uint16_t *o = feat + (bv.px_off + i) * C0;
+    for (int c = 0; c < C0; ++c) o[c] = c < 7 ? f2bf(f[c]) : (uint16_t)0;
+    mask[bv.px_off + i] = cov ? 1 : 0;
+}
+
+__global__ __launch_bounds__(256) void k_head_batch(const BatchPixView *__restrict__ views,
+                                                    const uint8_t *__restrict__ blk_view,
+                                                    const float *__restrict__ params,
+                                                    const uint16_t *__restrict__ a4,
+                                                    const uint8_t *__restrict__ mask,
+                                                    float *__restrict__ ao)
+{
+    int v, x, y;
+    if (!batch_block(views, blk_view, v, x, y)) return;
+    const BatchPixView &bv = views[v];
+    if (x >= bv.W) return;
+    const size_t p = bv.px_off + (size_t)x + (size_t)y * bv.W;
+    const float z = head_z(params + OFF_W5, params[OFF_B5], a4 + p * CH);
+    ao[p] = mask[p] ? 1.f / (1.f + expf(-z)) : 1.f;
+}
+
 // Loss = sum over covered pixels of (y - t)^2 / n_total.  Writes the data
 // gradient of a4 and per-block partials of [dW5 (32), db5, loss, |err|, count].
 constexpr int HEAD_PART = CH + 4;
@@ -427,6 +503,49 @@ static int nao_alloc(void **p, size_t bytes)
 }
 
 static inline dim3 g2(int W, int H) { return dim3((W + 63) / 64, (H + 3) / 4); }
+
+namespace fgl {
+
+int nao_reserve(fgl_nao *n, size_t px)
+{
+    if (px <= n->icap) return FGL_OK;
+    hipStreamSynchronize(ctx_stream(n->ctx));
+    if (n->ifeat) {
+        hipFree(n->ifeat); hipFree(n->ia); hipFree(n->ib); hipFree(n->imask);
+        n->ifeat = n->ia = n->ib = nullptr;
+        n->imask = nullptr;
+        n->icap = 0;
+    }
+    if (nao_alloc((void **)&n->ifeat, px * C0 * 2) || nao_alloc((void **)&n->ia, px * CH * 2) ||
+        nao_alloc((void **)&n->ib, px * CH * 2) || nao_alloc((void **)&n->imask, px))
+        return FGL_ERR_NOMEM;
+    n->icap = px;
+    return FGL_OK;
+}
+
+int nao_infer_batch(fgl_nao *n, hipStream_t s, const BatchPixView *views, const uint8_t *blk_view,
+                    uint32_t n_blocks, size_t total_px, const float *depth, const float *gbuf, float *ao)
+{
+    if (total_px > n->icap) return FGL_ERR_INVALID;
+    if (n_blocks == 0) return FGL_OK;
+    const dim3 g(n_blocks), b(256);
+    hipLaunchKernelGGL(k_features_batch, g, b, 0, s, views, blk_view, depth, gbuf, n->ifeat, n->imask);
+    hipLaunchKernelGGL((k_conv3x3_batch<C0>), g, b, 0, s, views, blk_view, (const uint16_t *)n->ifeat, n->w1,
+                       n->params + OFF_B1, n->ia);
+    const int boffs[3] = {OFF_B2, OFF_B3, OFF_B4};
+    uint16_t *src = n->ia, *dst = n->ib;
+    for (int l = 0; l < 3; ++l) {
+        hipLaunchKernelGGL((k_conv3x3_batch<CH>), g, b, 0, s, views, blk_view, (const uint16_t *)src,
+                           n->w234 + (size_t)l * CH * K2, n->params + boffs[l], dst);
+        uint16_t *t = src;
+        src = dst;
+        dst = t;
+    }
+    hipLaunchKernelGGL(k_head_batch, g, b, 0, s, views, blk_view, n->params, (const uint16_t *)src, n->imask, ao);
+    return hipGetLastError() == hipSuccess ? FGL_OK : FGL_ERR_DEVICE;
+}
+
+}  // namespace fgl
 
 extern "C" {
 
@@ -662,14 +781,7 @@ int fgl_nao_infer(fgl_nao *n)
     int rc = ctx_view(n->ctx, &cv);
     if (rc) return rc;
     const size_t px = (size_t)cv.W * cv.H;
-    if (px > n->icap) {
-        hipStreamSynchronize(cv.stream);
-        if (n->ifeat) { hipFree(n->ifeat); hipFree(n->ia); hipFree(n->ib); hipFree(n->imask); }
-        if (nao_alloc((void **)&n->ifeat, px * C0 * 2) || nao_alloc((void **)&n->ia, px * CH * 2) ||
-            nao_alloc((void **)&n->ib, px * CH * 2) || nao_alloc((void **)&n->imask, px))
-            return FGL_ERR_NOMEM;
-        n->icap = px;
-    }
+    if ((rc = nao_reserve(n, px))) return rc;
     Mat16 V;
     std::memcpy(V.m, cv.view, sizeof(V.m));
     {

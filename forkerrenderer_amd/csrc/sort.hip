@@ -121,10 +121,14 @@ constexpr int kRadixBlock = 256;
 constexpr int kRadixRounds = 8;
 constexpr int kRadixTile = kRadixBlock * kRadixRounds;  // 2048 items, striped by round
 
+// n is the item count, or the capacity when n_dev holds the device-side
+// count (then min(n, *n_dev) items are sorted).
 __global__ __launch_bounds__(kRadixBlock) void k_radix_hist(const uint32_t *__restrict__ keys,
-                                                            uint32_t n, int shift, int nb,
+                                                            uint32_t n, const uint32_t *__restrict__ n_dev,
+                                                            int shift, int nb,
                                                             uint32_t *__restrict__ hist)
 {
+    if (n_dev) n = min(n, *n_dev);
     __shared__ uint32_t h[256];
     h[threadIdx.x] = 0;
     __syncthreads();
@@ -140,9 +144,10 @@ __global__ __launch_bounds__(kRadixBlock) void k_radix_hist(const uint32_t *__re
 
 __global__ __launch_bounds__(kRadixBlock) void k_radix_scatter(
     const uint32_t *__restrict__ kin, const uint32_t *__restrict__ vin,
-    uint32_t *__restrict__ kout, uint32_t *__restrict__ vout, uint32_t n, int shift, int nb,
-    const uint32_t *__restrict__ hist)
+    uint32_t *__restrict__ kout, uint32_t *__restrict__ vout, uint32_t n,
+    const uint32_t *__restrict__ n_dev, int shift, int nb, const uint32_t *__restrict__ hist)
 {
+    if (n_dev) n = min(n, *n_dev);
     __shared__ uint32_t goff[256];
     __shared__ uint32_t dbase[256];
     __shared__ uint32_t wcnt[kRadixBlock / 64][256];
@@ -195,9 +200,10 @@ size_t radix_tmp_words(size_t n)
     return h + 1 + scan_tmp_words(h);
 }
 
-// Returns 0 when the sorted result is in (keys, vals), 1 when in the alt buffers.
+// Returns 0 when the sorted result is in (keys, vals), 1 when in the alt
+// buffers.  With n_dev, n is the capacity and *n_dev the item count.
 int launch_radix_sort_impl(uint32_t *keys, uint32_t *vals, uint32_t *keys_alt, uint32_t *vals_alt,
-                           size_t n, int bits, uint32_t *tmp, hipStream_t s)
+                           size_t n, const uint32_t *n_dev, int bits, uint32_t *tmp, hipStream_t s)
 {
     if (n <= 1 || bits <= 0) return 0;
     int nb = (int)((n + kRadixTile - 1) / kRadixTile);
@@ -208,11 +214,11 @@ int launch_radix_sort_impl(uint32_t *keys, uint32_t *vals, uint32_t *keys_alt, u
     int passes = (bits + 7) / 8;
     for (int p = 0; p < passes; ++p) {
         int shift = 8 * p;
-        hipLaunchKernelGGL(k_radix_hist, dim3(nb), dim3(kRadixBlock), 0, s, ki, (uint32_t)n, shift,
-                           nb, hist);
+        hipLaunchKernelGGL(k_radix_hist, dim3(nb), dim3(kRadixBlock), 0, s, ki, (uint32_t)n, n_dev,
+                           shift, nb, hist);
         launch_exclusive_scan(hist, hist, h, stmp, s);
         hipLaunchKernelGGL(k_radix_scatter, dim3(nb), dim3(kRadixBlock), 0, s, ki, vi, ko, vo,
-                           (uint32_t)n, shift, nb, hist);
+                           (uint32_t)n, n_dev, shift, nb, hist);
         uint32_t *t;
         t = ki; ki = ko; ko = t;
         t = vi; vi = vo; vo = t;
@@ -223,7 +229,7 @@ int launch_radix_sort_impl(uint32_t *keys, uint32_t *vals, uint32_t *keys_alt, u
 void launch_radix_sort(uint32_t *keys, uint32_t *vals, uint32_t *keys_alt, uint32_t *vals_alt,
                        size_t n, int bits, uint32_t *tmp, hipStream_t s)
 {
-    if (launch_radix_sort_impl(keys, vals, keys_alt, vals_alt, n, bits, tmp, s)) {
+    if (launch_radix_sort_impl(keys, vals, keys_alt, vals_alt, n, nullptr, bits, tmp, s)) {
         hipMemcpyAsync(keys, keys_alt, n * sizeof(uint32_t), hipMemcpyDeviceToDevice, s);
         hipMemcpyAsync(vals, vals_alt, n * sizeof(uint32_t), hipMemcpyDeviceToDevice, s);
     }

@@ -256,6 +256,61 @@ int fgl_nao_read_batch(fgl_nao *n, uint16_t *feat_bf16, float *target, uint8_t *
 /* neural AO of the context's current frame into its AO target */
 int fgl_nao_infer(fgl_nao *n);
 
+/* ---- views and batched views (new; no counterpart in the reference) ----
+ * A view is one deferred frame of Render::Render (src/render.cpp:24-58)
+ * without SSAA: optional shadow pass (light-space ortho +-3*aspect, near 0.1,
+ * far 20, looking from lights[0] at the origin), geometry pass (perspective
+ * 45 deg, 0.01..20), ambient occlusion (none or neural), lighting
+ * (ForkerGL::DrawScreenSpacePixels) and quantisation, over a list of model
+ * instances already uploaded into the context. */
+#define FGL_AO_NONE 0
+#define FGL_AO_NEURAL 2
+typedef struct {
+    int32_t width, height;     /* frame size (pixels) */
+    int32_t shadow_size;       /* shadow map edge (square); used when shadow_on */
+    int32_t shadow_on;
+    float eye[3], look_at[3];
+    int32_t n_lights;
+    fgl_light lights[FGL_MAX_LIGHTS];
+    fgl_shadow_cfg shadow;
+    int32_t n_draws;
+    const int32_t *models;     /* n_draws model ids */
+    const float *matrices;     /* n_draws row-major 4x4 model matrices */
+} fgl_view_desc;
+
+/* One view through the context's ForkerGL calls (the single-view path);
+ * nao is used when ao_mode == FGL_AO_NEURAL.  Results in the context targets. */
+int fgl_render_view(fgl_ctx *ctx, const fgl_view_desc *view, int ao_mode, fgl_nao *nao);
+
+/* A batch renders many views of different sizes and meshes together: every
+ * pass is one launch over all views (persistent rasteriser workgroups pull
+ * (view, tile) items from a device queue), the pass sequence is captured in
+ * a HIP graph and replayed.  Results equal fgl_render_view's bit for bit. */
+typedef struct fgl_batch fgl_batch;
+int fgl_batch_create(fgl_ctx *ctx, int n_views, const fgl_view_desc *views, int ao_mode, fgl_nao *nao,
+                     fgl_batch **out);
+void fgl_batch_destroy(fgl_batch *b);
+/* raster tile edges (8, 16, 32) of the shadow and geometry passes */
+int fgl_batch_set_tiles(fgl_batch *b, int shadow_tile, int main_tile);
+/* persistent rasteriser workgroups per compute unit (1..16) */
+int fgl_batch_set_occupancy(fgl_batch *b, int wg_per_cu);
+/* (tile, triangle) pair capacity per pass; 0 = measured + margin.  A pass
+ * that overflows is detected by fgl_batch_sync and re-rendered. */
+int fgl_batch_set_capacity(fgl_batch *b, uint32_t shadow_pairs, uint32_t main_pairs);
+int fgl_batch_use_graph(fgl_batch *b, int on);
+/* enqueues one render of every view (graph replay when enabled) */
+int fgl_batch_render(fgl_batch *b);
+/* waits; re-renders when a pass overflowed its capacity.  *rerenders (optional)
+ * receives the number of such re-renders. */
+int fgl_batch_sync(fgl_batch *b, int *rerenders);
+/* target of one view, as fgl_read_target (FRAME, DEPTH, GBUFFER, AO, IMAGE,
+ * SHADOW, SHADOW_DEPTH) */
+int fgl_batch_read(fgl_batch *b, int view, int which, void *dst, size_t bytes);
+/* stats[0..9]: (tile, triangle) pairs of the last measured render (shadow,
+ * main), pair capacities (shadow, main), triangles (shadow, main), pixels,
+ * views, persistent rasteriser workgroups, main-pass tiles */
+int fgl_batch_stats(fgl_batch *b, uint64_t stats[10]);
+
 /* ---- host matrix builders (src/geometry.cpp:60-179), shared by the CLI and tests ---- */
 void fgl_make_model_matrix(float tx, float ty, float tz, float roty_deg, float scale, float out[16]);
 void fgl_make_lookat(const float eye[3], const float center[3], const float up[3], float out[16]);
