@@ -152,6 +152,81 @@ def run_neural_ao(steps, warmup):
             "heldout_eval": held, "train_steps": int(ck["step"])}
 
 
+def run_batched_views(reps, n_views=64):
+    """The uneven-views workload (64 views, 256^2 .. 1920x1080, icosphere
+    subdivision 4..8, deferred PBR + 5x5 PCF + neural AO) as one batch:
+    aggregate Mpix/s and Mtri/s of graph replays after a small autotune over
+    tile sizes and rasteriser occupancy, and per-view latency of the same
+    views rendered one at a time (render_view, synchronised per view)."""
+    import forkerrenderer_amd as F
+    from forkerrenderer_amd import nao as NA
+    from forkerrenderer_amd import views as VW
+    ctx = F.Context(0)
+    params = NA.load_weights() if os.path.exists(NA.DEFAULT_WEIGHTS) else NA.init_params(0)
+    net = NA.NeuralAO(ctx, batch=1, crop=64, params=params)
+    t0 = time.perf_counter()
+    views = VW.uneven_views(ctx, n_views=n_views)
+    t_setup = time.perf_counter() - t0
+    b = VW.Batch(ctx, views, VW.AO_NEURAL, net)
+    b.render()
+    b.sync()
+
+    def timed(k):
+        b.render()
+        b.sync()
+        t = time.perf_counter()
+        for _ in range(k):
+            b.render()
+        b.sync()
+        return (time.perf_counter() - t) / k
+
+    tune = {}
+    for tiles in ((16, 16), (8, 16), (16, 8), (8, 8), (32, 16)):
+        for occ in (2, 4, 8):
+            b.set_tiles(*tiles)
+            b.set_occupancy(occ)
+            b.render()
+            b.sync()
+            tune[f"{tiles[0]}/{tiles[1]}/{occ}"] = timed(2) * 1e3
+    best = min(tune, key=tune.get)
+    ts, tm, occ = (int(x) for x in best.split("/"))
+    b.set_tiles(ts, tm)
+    b.set_occupancy(occ)
+    b.render()
+    b.sync()
+    sec = timed(reps)
+    st = b.stats()
+    # per-pass kernel time of one eager batch render
+    ctx.read_timings()
+    ctx.enable_timing(True)
+    b.render()
+    b.sync()
+    per = {}
+    for name, ms in ctx.read_timings():
+        per[name] = per.get(name, 0.0) + ms
+    ctx.enable_timing(False)
+    b.close()
+    lat = []
+    for v in views:
+        VW.render_view(ctx, v, VW.AO_NEURAL, net)
+        ctx.sync()
+        t = time.perf_counter()
+        VW.render_view(ctx, v, VW.AO_NEURAL, net)
+        ctx.sync()
+        lat.append((time.perf_counter() - t) * 1e3)
+    net.close()
+    ctx.close()
+    lat = np.array(lat)
+    return {"views": n_views, "pixels": st["pixels"], "triangles_per_pass": st["tris_main"],
+            "ms_per_batch": sec * 1e3, "mpix_per_s": st["pixels"] / sec / 1e6,
+            "mtri_per_s": st["tris_main"] / sec / 1e6, "tiles_shadow_main_occupancy": best,
+            "autotune_ms": tune, "pairs": [st["pairs_shadow"], st["pairs_main"]],
+            "kernels_ms": per,
+            "single_view_latency_ms": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99)),
+                                       "sum": float(lat.sum())},
+            "batch_speedup_vs_sequential": float(lat.sum() / (sec * 1e3)), "setup_s": t_setup}
+
+
 def cpu_baseline(sc, band_frac):
     """Oracle on a band of rows, scaled: est = t_empty + (t_band - t_empty) * rows_total/rows_band
     per pass, where t_empty is the same pass with an empty row window (per-triangle work)."""
@@ -215,6 +290,7 @@ def main():
     ap.add_argument("--cpu-band", type=float, default=1.0 / 32)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-nao", action="store_true", help="skip the neural-AO vs plain SSAO comparison")
+    ap.add_argument("--no-views", action="store_true", help="skip the batched uneven-views workload")
     ap.add_argument("--copy-calibrate", type=int, default=0, metavar="MIB",
                     help="only run a plain copy of MIB MiB (calibrates HBM byte counters) and exit")
     a = ap.parse_args()
@@ -273,6 +349,8 @@ def main():
     }
     if not a.no_nao:
         line["neural_ao"] = run_neural_ao(max(5, a.steps // 2), a.warmup)
+    if not a.no_views:
+        line["batched_views"] = run_batched_views(max(3, a.steps // 4))
     print(json.dumps(line))
 
 

@@ -6,6 +6,7 @@
 //       stored as 16 BF16 channels (9 zero)
 //   L1  conv3x3 16->32 + ReLU      L2..L4 conv3x3 32->32 + ReLU
 //   L5  conv1x1 32->1 + sigmoid
+//   the 3x3 layers are dilated by 1, 2, 4, 8 (receptive field 31 x 31)
 // Convolutions are implicit GEMMs on MFMA (v_mfma_f32_16x16x16_bf16):
 // M = pixels, N = 32 output channels, K = 9 taps x Cin, BF16 operands, FP32
 // accumulation, BF16 NHWC activations.  Backward: the data gradient is the
@@ -40,6 +41,10 @@ constexpr int OFF_W4 = OFF_B3 + CH, OFF_B4 = OFF_W4 + CH * K2;
 constexpr int OFF_W5 = OFF_B4 + CH, OFF_B5 = OFF_W5 + CH;
 constexpr int NPARAM = OFF_B5 + 1;
 constexpr int WG_ROWS = 16;   // rows of pixels per weight-gradient workgroup
+// Dilation of the four 3x3 convolutions (taps at -d, 0, +d): a 31 x 31
+// receptive field at the cost of four plain 3x3 layers.
+constexpr int kMaxDil = 8;
+__host__ __device__ constexpr int layer_dil(int l) { return l == 0 ? 1 : l == 1 ? 2 : l == 2 ? 4 : 8; }
 
 __host__ __device__ __forceinline__ uint16_t f2bf(float f)
 {
@@ -135,7 +140,7 @@ __device__ __forceinline__ void conv_row(const uint16_t *__restrict__ in,
                                          const float *__restrict__ bias,
                                          const uint16_t *__restrict__ act,
                                          uint16_t *__restrict__ out, int W, int H, size_t img,
-                                         int y, int xb)
+                                         int y, int xb, int dil)
 {
     constexpr int K = 9 * CIN, KS = K / 16, CG = CIN / 16;
     const int lane = threadIdx.x & 63;
@@ -154,11 +159,11 @@ __device__ __forceinline__ void conv_row(const uint16_t *__restrict__ in,
 #pragma unroll
     for (int ks = 0; ks < KS; ++ks) {
         const int tap = ks / CG, cg = ks % CG;
-        const int yy = y + tap / 3 - 1;
+        const int yy = y + (tap / 3 - 1) * dil;
         const bool rowok = yy >= 0 && yy < H;
 #pragma unroll
         for (int mb = 0; mb < 4; ++mb) {
-            const int xx = xb + mb * 16 + lr + tap % 3 - 1;
+            const int xx = xb + mb * 16 + lr + (tap % 3 - 1) * dil;
             v4s a = v4s{0, 0, 0, 0};
             if (rowok && xx >= 0 && xx < W)
                 a = *(const v4s *)(in + (img + (size_t)yy * W + xx) * CIN + cg * 16 + lq * 4);
@@ -187,16 +192,84 @@ __device__ __forceinline__ void conv_row(const uint16_t *__restrict__ in,
             }
 }
 
+// The same row with v_mfma_f32_16x16x32_bf16: one instruction covers 32 K
+// values (one tap of 32 channels, or two taps of 16), A fragments are 16-byte
+// loads (lane l: pixel l % 16, channels 8 * (l / 16) .. + 8 of its tap), so
+// a wavefront reads 1 KB contiguous per fragment.
+typedef __bf16 v8bf __attribute__((ext_vector_type(8)));
+
+template <int CIN, int MODE>
+__device__ __forceinline__ void conv_row_k32(const uint16_t *__restrict__ in,
+                                             const uint16_t *__restrict__ Wt,
+                                             const float *__restrict__ bias,
+                                             const uint16_t *__restrict__ act,
+                                             uint16_t *__restrict__ out, int W, int H, size_t img,
+                                             int y, int xb, int dil)
+{
+    constexpr int K = 9 * CIN, KC = (K + 31) / 32;
+    const int lane = threadIdx.x & 63;
+    const int lr = lane & 15, lq = lane >> 4;
+    v8bf bfr[KC][2];
+#pragma unroll
+    for (int c = 0; c < KC; ++c)
+#pragma unroll
+        for (int nb = 0; nb < 2; ++nb) {
+            const int k0 = c * 32 + lq * 8;
+            if (k0 < K) bfr[c][nb] = *(const v8bf *)(Wt + (size_t)(nb * 16 + lr) * K + k0);
+            else bfr[c][nb] = v8bf{};
+        }
+    v4f acc[4][2];
+#pragma unroll
+    for (int mb = 0; mb < 4; ++mb)
+#pragma unroll
+        for (int nb = 0; nb < 2; ++nb) acc[mb][nb] = v4f{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int c = 0; c < KC; ++c) {
+        const int k0 = c * 32 + lq * 8;
+        const int tap = k0 / CIN, ci0 = k0 % CIN;
+        const int yy = y + (tap / 3 - 1) * dil;
+        const bool rowok = k0 < K && yy >= 0 && yy < H;
+#pragma unroll
+        for (int mb = 0; mb < 4; ++mb) {
+            const int xx = xb + mb * 16 + lr + (tap % 3 - 1) * dil;
+            v8bf a = v8bf{};
+            if (rowok && xx >= 0 && xx < W)
+                a = *(const v8bf *)(in + (img + (size_t)yy * W + xx) * CIN + ci0);
+            acc[mb][0] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, bfr[c][0], acc[mb][0], 0, 0, 0);
+            acc[mb][1] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, bfr[c][1], acc[mb][1], 0, 0, 0);
+        }
+    }
+#pragma unroll
+    for (int mb = 0; mb < 4; ++mb)
+#pragma unroll
+        for (int nb = 0; nb < 2; ++nb)
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                const int px = xb + mb * 16 + lq * 4 + i;
+                if (px >= W) continue;
+                const int co = nb * 16 + lr;
+                const size_t o = (img + (size_t)y * W + px) * CH + co;
+                float v = acc[mb][nb][i];
+                if (MODE == 0) {
+                    v += bias[co];
+                    v = v > 0.f ? v : 0.f;
+                } else {
+                    v = act[o] != 0 ? v : 0.f;
+                }
+                out[o] = f2bf(v);
+            }
+}
+
 template <int CIN, int MODE>
 __global__ __launch_bounds__(256) void k_conv3x3(const uint16_t *__restrict__ in,
                                                  const uint16_t *__restrict__ Wt,
                                                  const float *__restrict__ bias,
                                                  const uint16_t *__restrict__ act,
-                                                 uint16_t *__restrict__ out, int W, int H)
+                                                 uint16_t *__restrict__ out, int W, int H, int dil)
 {
     const int y = blockIdx.y * 4 + (threadIdx.x >> 6);
     if (y >= H) return;
-    conv_row<CIN, MODE>(in, Wt, bias, act, out, W, H, (size_t)blockIdx.z * H * W, y, blockIdx.x * 64);
+    conv_row<CIN, MODE>(in, Wt, bias, act, out, W, H, (size_t)blockIdx.z * H * W, y, blockIdx.x * 64, dil);
 }
 
 // Batched views: workgroup b is 64 x 4 pixel block b of the batch (view
@@ -219,28 +292,32 @@ __global__ __launch_bounds__(256) void k_conv3x3_batch(const BatchPixView *__res
                                                        const uint16_t *__restrict__ in,
                                                        const uint16_t *__restrict__ Wt,
                                                        const float *__restrict__ bias,
-                                                       uint16_t *__restrict__ out)
+                                                       uint16_t *__restrict__ out, int dil)
 {
     int v, x, y;
     if (!batch_block(views, blk_view, v, x, y)) return;
     const BatchPixView &bv = views[v];
-    conv_row<CIN, 0>(in, Wt, bias, nullptr, out, bv.W, bv.H, bv.px_off, y, x - (threadIdx.x & 63));
+    conv_row<CIN, 0>(in, Wt, bias, nullptr, out, bv.W, bv.H, bv.px_off, y, x - (threadIdx.x & 63), dil);
 }
 
 // Weight gradient: dW[co][tap*CIN+ci] = sum_p dout[p][co] * in[p+off(tap)][ci]
 // and db[co] = sum_p dout[p][co], per workgroup over a 64 x WG_ROWS pixel
-// tile, written as partials [block][...] for a fixed-order reduction.
+// tile, written as partials [block][...] for a fixed-order reduction.  Tap
+// offsets are (dy, dx) * dil.  Two rows of pixels per stage: their output
+// gradients, and the six input rows the three tap rows reach, each with a
+// dil-pixel halo left and right.
 template <int CIN>
 __global__ __launch_bounds__(256) void k_conv_wgrad(const uint16_t *__restrict__ in,
                                                     const uint16_t *__restrict__ dout,
                                                     float *__restrict__ part_dw,
-                                                    float *__restrict__ part_db, int W, int H)
+                                                    float *__restrict__ part_db, int W, int H, int dil)
 {
     constexpr int K = 9 * CIN, NB = K / 16;
     constexpr int NBW = (NB + 3) / 4;  // n-blocks per wavefront
-    __shared__ uint16_t s_out[4 * 64 * CH];
-    __shared__ uint16_t s_in[6 * 66 * CIN];
-    __shared__ float s_db[4][CH];
+    constexpr int SR = 2;              // pixel rows per stage
+    __shared__ uint16_t s_out[SR * 64 * CH];
+    __shared__ uint16_t s_in[3 * SR * (64 + 2 * kMaxDil) * CIN];
+    const int SW = 64 + 2 * dil;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, lr = lane & 15, lq = lane >> 4;
     const int xb = blockIdx.x * 64, yb = blockIdx.y * WG_ROWS, n = blockIdx.z;
     const size_t img = (size_t)n * H * W;
@@ -250,20 +327,22 @@ __global__ __launch_bounds__(256) void k_conv_wgrad(const uint16_t *__restrict__
 #pragma unroll
         for (int j = 0; j < NBW; ++j) acc[mb][j] = v4f{0.f, 0.f, 0.f, 0.f};
     float dbacc = 0.f;
-    for (int r0 = 0; r0 < WG_ROWS; r0 += 4) {
+    for (int r0 = 0; r0 < WG_ROWS; r0 += SR) {
         const int y0 = yb + r0;
-        // stage output gradients: 4 rows x 64 px x 32 ch
-        for (int e = tid; e < 4 * 64 * CH / 4; e += 256) {
+        // output gradients: SR rows x 64 px x 32 ch
+        for (int e = tid; e < SR * 64 * CH / 4; e += 256) {
             const int pix = e / (CH / 4), c4 = e % (CH / 4);
             const int yy = y0 + pix / 64, xx = xb + pix % 64;
             uint64_t v = 0;
             if (yy < H && xx < W) v = *(const uint64_t *)(dout + (img + (size_t)yy * W + xx) * CH + c4 * 4);
             *(uint64_t *)(s_out + pix * CH + c4 * 4) = v;
         }
-        // stage inputs with a one-pixel halo: 6 rows x 66 px x CIN
-        for (int e = tid; e < 6 * 66 * CIN / 4; e += 256) {
+        // inputs: stage row (dyi * SR + r) is image row y0 + r + (dyi - 1) * dil,
+        // column c is image column xb - dil + c
+        for (int e = tid; e < 3 * SR * SW * CIN / 4; e += 256) {
             const int pix = e / (CIN / 4), c4 = e % (CIN / 4);
-            const int yy = y0 - 1 + pix / 66, xx = xb - 1 + pix % 66;
+            const int srow = pix / SW, col = pix % SW;
+            const int yy = y0 + srow % SR + (srow / SR - 1) * dil, xx = xb - dil + col;
             uint64_t v = 0;
             if (yy >= 0 && yy < H && xx >= 0 && xx < W)
                 v = *(const uint64_t *)(in + (img + (size_t)yy * W + xx) * CIN + c4 * 4);
@@ -271,9 +350,9 @@ __global__ __launch_bounds__(256) void k_conv_wgrad(const uint16_t *__restrict__
         }
         __syncthreads();
         if (tid < CH)
-            for (int p = 0; p < 4 * 64; ++p) dbacc += bf2f(s_out[p * CH + tid]);
-        // K dimension of this GEMM = 256 pixels = 16 steps of 16
-        for (int ps = 0; ps < 16; ++ps) {
+            for (int p = 0; p < SR * 64; ++p) dbacc += bf2f(s_out[p * CH + tid]);
+        // K dimension of this GEMM = SR * 64 pixels in steps of 16
+        for (int ps = 0; ps < SR * 4; ++ps) {
             v4s a[2];
 #pragma unroll
             for (int mb = 0; mb < 2; ++mb)
@@ -285,13 +364,13 @@ __global__ __launch_bounds__(256) void k_conv_wgrad(const uint16_t *__restrict__
                 const int nb = wave + 4 * jb;
                 if (nb >= NB) break;
                 const int k = nb * 16 + lr, tap = k / CIN, ci = k % CIN;
-                const int dy = tap / 3, dx = tap % 3;  // halo-shifted offsets
+                const int dyi = tap / 3, dxi = tap % 3;
                 v4s b;
 #pragma unroll
                 for (int j = 0; j < 4; ++j) {
                     const int pix = ps * 16 + lq * 4 + j;
                     const int pr = pix / 64, pc = pix % 64;
-                    b[j] = (short)s_in[((pr + dy) * 66 + pc + dx) * CIN + ci];
+                    b[j] = (short)s_in[((dyi * SR + pr) * SW + pc + dxi * dil) * CIN + ci];
                 }
 #pragma unroll
                 for (int mb = 0; mb < 2; ++mb)
@@ -312,7 +391,6 @@ __global__ __launch_bounds__(256) void k_conv_wgrad(const uint16_t *__restrict__
             for (int i = 0; i < 4; ++i) pd[(size_t)(mb * 16 + lq * 4 + i) * K + nb * 16 + lr] = acc[mb][jb][i];
     }
     if (tid < CH) part_db[blk * CH + tid] = dbacc;
-    (void)s_db;
 }
 
 // out[i] = sum over b of part[b*len + i], b ascending (fixed order).
@@ -531,12 +609,12 @@ int nao_infer_batch(fgl_nao *n, hipStream_t s, const BatchPixView *views, const 
     const dim3 g(n_blocks), b(256);
     hipLaunchKernelGGL(k_features_batch, g, b, 0, s, views, blk_view, depth, gbuf, n->ifeat, n->imask);
     hipLaunchKernelGGL((k_conv3x3_batch<C0>), g, b, 0, s, views, blk_view, (const uint16_t *)n->ifeat, n->w1,
-                       n->params + OFF_B1, n->ia);
+                       n->params + OFF_B1, n->ia, layer_dil(0));
     const int boffs[3] = {OFF_B2, OFF_B3, OFF_B4};
     uint16_t *src = n->ia, *dst = n->ib;
     for (int l = 0; l < 3; ++l) {
         hipLaunchKernelGGL((k_conv3x3_batch<CH>), g, b, 0, s, views, blk_view, (const uint16_t *)src,
-                           n->w234 + (size_t)l * CH * K2, n->params + boffs[l], dst);
+                           n->w234 + (size_t)l * CH * K2, n->params + boffs[l], dst, layer_dil(l + 1));
         uint16_t *t = src;
         src = dst;
         dst = t;
@@ -678,11 +756,11 @@ static void forward_convs(fgl_nao *n, hipStream_t s, const uint16_t *feat, uint1
 {
     dim3 grid((W + 63) / 64, (H + 3) / 4, N);
     hipLaunchKernelGGL((k_conv3x3<C0, 0>), grid, dim3(256), 0, s, feat, n->w1, n->params + OFF_B1,
-                       (const uint16_t *)nullptr, acts[0], W, H);
+                       (const uint16_t *)nullptr, acts[0], W, H, layer_dil(0));
     const int boffs[3] = {OFF_B2, OFF_B3, OFF_B4};
     for (int l = 0; l < 3; ++l)
         hipLaunchKernelGGL((k_conv3x3<CH, 0>), grid, dim3(256), 0, s, acts[l], n->w234 + (size_t)l * CH * K2,
-                           n->params + boffs[l], (const uint16_t *)nullptr, acts[l + 1], W, H);
+                           n->params + boffs[l], (const uint16_t *)nullptr, acts[l + 1], W, H, layer_dil(l + 1));
 }
 
 static int run_batch(fgl_nao *n, bool update, float lr, float *stats)
@@ -711,16 +789,18 @@ static int run_batch(fgl_nao *n, bool update, float lr, float *stats)
     for (int l = 3; l >= 0; --l) {
         const int K = l == 0 ? K1 : K2;
         if (l == 0)
-            hipLaunchKernelGGL((k_conv_wgrad<C0>), wg, dim3(256), 0, s, n->feat, n->d[0], n->part, n->part_db, S, S);
+            hipLaunchKernelGGL((k_conv_wgrad<C0>), wg, dim3(256), 0, s, n->feat, n->d[0], n->part, n->part_db, S, S,
+                               layer_dil(0));
         else
-            hipLaunchKernelGGL((k_conv_wgrad<CH>), wg, dim3(256), 0, s, n->a[l - 1], n->d[l], n->part, n->part_db, S, S);
+            hipLaunchKernelGGL((k_conv_wgrad<CH>), wg, dim3(256), 0, s, n->a[l - 1], n->d[l], n->part, n->part_db, S, S,
+                               layer_dil(l));
         hipLaunchKernelGGL(k_reduce_cols, dim3((CH * K + 255) / 256), dim3(256), 0, s, n->part, nblk, CH * K,
                            n->grads + woffs[l]);
         hipLaunchKernelGGL(k_reduce_cols, dim3(1), dim3(256), 0, s, n->part_db, nblk, CH, n->grads + boffs[l]);
         if (l > 0)
             hipLaunchKernelGGL((k_conv3x3<CH, 1>), cgrid, dim3(256), 0, s, n->d[l],
                                n->wt234 + (size_t)(l - 1) * CH * K2, (const float *)nullptr, n->a[l - 1],
-                               n->d[l - 1], S, S);
+                               n->d[l - 1], S, S, layer_dil(l));
     }
     if (update) {
         n->step += 1;

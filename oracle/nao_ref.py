@@ -1,8 +1,9 @@
 """CPU oracle of the neural-AO network (TEST INFRASTRUCTURE).
 
 Forward and backward of: conv3x3 16->32 ReLU, 3x conv3x3 32->32 ReLU,
-conv1x1 32->1 sigmoid, 'same' zero padding, loss = sum over covered pixels of
-(y - t)^2 / n_pixels.  Sums are evaluated in float64.  With bf16=True the
+conv1x1 32->1 sigmoid, 'same' zero padding, the four 3x3 convolutions
+dilated by DIL = (1, 2, 4, 8) (taps at offsets -d, 0, +d: receptive field
+31 x 31), loss = sum over covered pixels of (y - t)^2 / n_pixels.  Sums are evaluated in float64.  With bf16=True the
 values the device keeps in BF16 (weights of layers 1-4, activations, data
 gradients) are rounded to BF16 at the same points; with bf16=False the
 network is plain float64, which the finite-difference test uses to pin the
@@ -19,6 +20,7 @@ K1, K2 = 9 * C0, 9 * CH
 SIZES = [("W1", CH * K1), ("b1", CH), ("W2", CH * K2), ("b2", CH), ("W3", CH * K2), ("b3", CH),
          ("W4", CH * K2), ("b4", CH), ("W5", CH), ("b5", 1)]
 NPARAM = sum(s for _, s in SIZES)
+DIL = (1, 2, 4, 8)
 
 
 def split(params):
@@ -42,29 +44,31 @@ def bf16_bits_to_float(b):
     return (np.asarray(b, np.uint32) << 16).view(np.float32).astype(np.float64)
 
 
-def _patches(x):
-    """x: [N,H,W,C] -> [N,H,W,9,C] with zero padding (tap order dy, dx)."""
+def _patches(x, dil=1):
+    """x: [N,H,W,C] -> [N,H,W,9,C] with zero padding (tap order dy, dx; tap
+    offsets -dil, 0, +dil)."""
     N, H, W, Cc = x.shape
-    xp = np.zeros((N, H + 2, W + 2, Cc), x.dtype)
-    xp[:, 1:-1, 1:-1] = x
-    taps = [xp[:, 1 + dy:1 + dy + H, 1 + dx:1 + dx + W] for dy in (-1, 0, 1) for dx in (-1, 0, 1)]
+    d = dil
+    xp = np.zeros((N, H + 2 * d, W + 2 * d, Cc), x.dtype)
+    xp[:, d:d + H, d:d + W] = x
+    taps = [xp[:, d + dy * d:d + dy * d + H, d + dx * d:d + dx * d + W] for dy in (-1, 0, 1) for dx in (-1, 0, 1)]
     return np.stack(taps, axis=3)
 
 
-def conv3x3(x, W, b):
+def conv3x3(x, W, b, dil=1):
     """x [N,H,W,Cin], W [Co][9*Cin] -> [N,H,W,Co]"""
     Co = W.shape[0]
-    p = _patches(x).reshape(x.shape[0], x.shape[1], x.shape[2], -1)
+    p = _patches(x, dil).reshape(x.shape[0], x.shape[1], x.shape[2], -1)
     out = p @ W.reshape(Co, -1).T
     return out + (b if b is not None else 0.0)
 
 
-def conv3x3_transpose(d, W, cin):
+def conv3x3_transpose(d, W, cin, dil=1):
     """Gradient w.r.t. the input of conv3x3: d [N,H,W,Co], W [Co][9*cin]."""
     Co = W.shape[0]
     Wr = W.reshape(Co, 9, cin)
     Wf = Wr[:, ::-1, :].transpose(2, 1, 0).reshape(cin, 9 * Co)   # flipped taps, transposed
-    return conv3x3(d, Wf, None)
+    return conv3x3(d, Wf, None, dil)
 
 
 def forward(params, feat, bf16=True):
@@ -74,7 +78,7 @@ def forward(params, feat, bf16=True):
     acts = []
     for l in range(4):
         W = r(P[f"W{l + 1}"]).reshape(CH, -1)
-        z = conv3x3(x, W, P[f"b{l + 1}"])
+        z = conv3x3(x, W, P[f"b{l + 1}"], DIL[l])
         a = r(np.maximum(z, 0.0))
         acts.append(a)
         x = a
@@ -102,12 +106,12 @@ def loss_and_grads(params, feat, target, mask, bf16=True):
     ins = [np.asarray(feat, np.float64)] + acts[:3]
     for l in range(3, -1, -1):
         cin = C0 if l == 0 else CH
-        pat = _patches(ins[l]).reshape(d.shape[0], d.shape[1], d.shape[2], -1)
+        pat = _patches(ins[l], DIL[l]).reshape(d.shape[0], d.shape[1], d.shape[2], -1)
         g[f"W{l + 1}"] = np.einsum("nhwo,nhwk->ok", d, pat).reshape(-1)
         g[f"b{l + 1}"] = d.sum(axis=(0, 1, 2))
         if l > 0:
             W = r(P[f"W{l + 1}"]).reshape(CH, -1)
-            dx = conv3x3_transpose(d, W, cin)
+            dx = conv3x3_transpose(d, W, cin, DIL[l])
             d = r(np.where(ins[l] > 0, dx, 0.0))
     flat = np.concatenate([np.asarray(g[name], np.float64).reshape(-1) for name, _ in SIZES])
     stats = (float(np.sum(np.where(m, diff * diff, 0))), float(np.sum(np.where(m, np.abs(diff), 0))),

@@ -15,7 +15,7 @@ def test_bf16_rounding_fixed_points():
 
 def test_backward_matches_finite_differences():
     rng = np.random.default_rng(0)
-    N, H, W = 2, 6, 7
+    N, H, W = 1, 19, 21   # larger than the widest dilated tap reach
     feat = np.zeros((N, H, W, 16))
     feat[..., :7] = rng.normal(0, 1, (N, H, W, 7))
     target = rng.uniform(0, 1, (N, H, W))
@@ -46,12 +46,34 @@ def test_backward_matches_finite_differences():
 
 
 def test_transposed_convolution_is_adjoint():
-    """<conv(x), d> == <x, conv_transpose(d)> for random x, d (the data
-    gradient is the adjoint of the forward convolution)."""
+    """<conv(x), d> == <x, conv_transpose(d)> for random x, d at every
+    dilation (the data gradient is the adjoint of the forward convolution)."""
     rng = np.random.default_rng(1)
-    x = rng.normal(size=(1, 5, 6, 32))
-    d = rng.normal(size=(1, 5, 6, 32))
-    Wt = rng.normal(size=(32, 9 * 32))
-    lhs = np.sum(R.conv3x3(x, Wt, None) * d)
-    rhs = np.sum(x * R.conv3x3_transpose(d, Wt, 32))
-    assert abs(lhs - rhs) < 1e-9 * max(1.0, abs(lhs))
+    for dil in R.DIL:
+        x = rng.normal(size=(1, 11, 13, 32))
+        d = rng.normal(size=(1, 11, 13, 32))
+        Wt = rng.normal(size=(32, 9 * 32))
+        lhs = np.sum(R.conv3x3(x, Wt, None, dil) * d)
+        rhs = np.sum(x * R.conv3x3_transpose(d, Wt, 32, dil))
+        assert abs(lhs - rhs) < 1e-9 * max(1.0, abs(lhs)), dil
+
+
+def test_dilated_convolution_against_direct_sum():
+    """conv3x3 at dilation d equals the textbook direct sum
+    out[y,x,o] = sum_{dy,dx,c} W[o, (dy+1)*3+dx+1, c] * x[y+d*dy, x+d*dx, c]
+    (zero outside), on a small image."""
+    rng = np.random.default_rng(2)
+    x = rng.normal(size=(1, 9, 10, 3))
+    Wt = rng.normal(size=(2, 9 * 3))
+    for dil in (1, 2, 4):
+        got = R.conv3x3(x, Wt, None, dil)[0]
+        want = np.zeros((9, 10, 2))
+        for yy in range(9):
+            for xx in range(10):
+                for dy in (-1, 0, 1):
+                    for dx in (-1, 0, 1):
+                        sy, sx = yy + dil * dy, xx + dil * dx
+                        if 0 <= sy < 9 and 0 <= sx < 10:
+                            t = (dy + 1) * 3 + dx + 1
+                            want[yy, xx] += Wt[:, t * 3:(t + 1) * 3] @ x[0, sy, sx]
+        assert np.allclose(got, want, atol=1e-12), dil
