@@ -7,7 +7,7 @@
 //   L1  conv3x3 16->32 + ReLU      L2..L4 conv3x3 32->32 + ReLU
 //   L5  conv1x1 32->1 + sigmoid
 //   the 3x3 layers are dilated by 1, 2, 4, 8 (receptive field 31 x 31)
-// Convolutions are implicit GEMMs on MFMA (v_mfma_f32_16x16x16_bf16):
+// Convolutions are implicit GEMMs on MFMA (v_mfma_f32_16x16x32_bf16):
 // M = pixels, N = 32 output channels, K = 9 taps x Cin, BF16 operands, FP32
 // accumulation, BF16 NHWC activations.  Backward: the data gradient is the
 // same kernel with tap-flipped, transposed weights and a ReLU-mask epilogue;
@@ -63,24 +63,29 @@ __host__ __device__ __forceinline__ float bf2f(uint16_t h)
 
 // ---------------- features ----------------
 
-// View-space features of one G-buffer pixel (zeros where nothing was drawn):
-// depth d = -z_view / 10, normal = V3 * N, position = V * (P,1) / 10.
+// View-space features of one G-buffer pixel: depth d = -z_view / 10,
+// normal = V3 * N, position = V * (P,1) / 10.  Pixels where nothing was
+// drawn, or whose G-buffer holds a non-finite normal or position (a
+// degenerate tangent frame makes the mapped normal NaN), are uncovered: zero
+// features, AO 1 -- what kernel SSAO yields there too -- and no part in the
+// loss, so one such pixel cannot poison a weight gradient.
 __device__ __forceinline__ void pixel_features(const float *V, const float *depth, const float *g,
                                                size_t P, size_t i, float f[7], bool &covered)
 {
     covered = depth[i] < 3.0e38f;
-    if (!covered) {
-        for (int c = 0; c < 7; ++c) f[c] = 0.f;
-        return;
+    if (covered) {
+        Vec3 N = vec3(g[i], g[P + i], g[2 * P + i]);
+        Vec3 Pw = vec3(g[3 * P + i], g[4 * P + i], g[5 * P + i]);
+        Vec4 vp = mv4(V, vec4(Pw.x, Pw.y, Pw.z, 1.f));
+        Vec3 vn = vec3(dot3(vec3(V[0], V[1], V[2]), N), dot3(vec3(V[4], V[5], V[6]), N),
+                       dot3(vec3(V[8], V[9], V[10]), N));
+        f[0] = -vp.z * 0.1f;
+        f[1] = vn.x; f[2] = vn.y; f[3] = vn.z;
+        f[4] = vp.x * 0.1f; f[5] = vp.y * 0.1f; f[6] = vp.z * 0.1f;
+        for (int c = 0; c < 7; ++c) covered = covered && isfinite(f[c]);
     }
-    Vec3 N = vec3(g[i], g[P + i], g[2 * P + i]);
-    Vec3 Pw = vec3(g[3 * P + i], g[4 * P + i], g[5 * P + i]);
-    Vec4 vp = mv4(V, vec4(Pw.x, Pw.y, Pw.z, 1.f));
-    Vec3 vn = vec3(dot3(vec3(V[0], V[1], V[2]), N), dot3(vec3(V[4], V[5], V[6]), N),
-                   dot3(vec3(V[8], V[9], V[10]), N));
-    f[0] = -vp.z * 0.1f;
-    f[1] = vn.x; f[2] = vn.y; f[3] = vn.z;
-    f[4] = vp.x * 0.1f; f[5] = vp.y * 0.1f; f[6] = vp.z * 0.1f;
+    if (!covered)
+        for (int c = 0; c < 7; ++c) f[c] = 0.f;
 }
 
 struct Mat16 {
@@ -131,134 +136,109 @@ __global__ __launch_bounds__(256) void k_crop(Mat16 V, const float *__restrict__
 // MODE 0: out = bf16(relu(conv(in) + bias)).
 // MODE 1 (data gradient): out = bf16(conv(in) * (act > 0)) with tap-flipped,
 // transposed weights; `in` is the layer's output gradient.
-// Workgroup: 4 wavefronts, each 64 consecutive pixels of one row x 32 channels.
+//
+// Implicit GEMM on v_mfma_f32_16x16x32_bf16 with D = W x patches: M = 16
+// output channels, N = 16 pixels, K = 32 (one tap of 32 channels, or two taps
+// of 16).  A (weights) stays in registers for the wavefront's whole life; B
+// (patches) fragments are 16-byte loads -- lane l: pixel l % 16, channels
+// 8 * (l / 16) .. + 8 of its tap -- so a wavefront reads 1 KB contiguous per
+// fragment; each lane ends with 4 consecutive output channels of one pixel,
+// stored as one 8-byte write.
+typedef __bf16 v8bf __attribute__((ext_vector_type(8)));
+
+template <int CIN>
+struct ConvWeights {
+    static constexpr int K = 9 * CIN, KC = (K + 31) / 32;
+    v8bf w[KC][2];  // [k chunk][16-channel block]
+};
+
+template <int CIN>
+__device__ __forceinline__ void load_weights(ConvWeights<CIN> &cw, const uint16_t *__restrict__ Wt)
+{
+    constexpr int K = ConvWeights<CIN>::K, KC = ConvWeights<CIN>::KC;
+    const int lane = threadIdx.x & 63, lr = lane & 15, lq = lane >> 4;
+#pragma unroll
+    for (int c = 0; c < KC; ++c)
+#pragma unroll
+        for (int cb = 0; cb < 2; ++cb) {
+            const int k0 = c * 32 + lq * 8;
+            if (k0 < K) cw.w[c][cb] = *(const v8bf *)(Wt + (size_t)(cb * 16 + lr) * K + k0);
+            else cw.w[c][cb] = v8bf{};
+        }
+}
+
 // One wavefront: pixels xb .. xb+63 of row y of the image starting at pixel
-// `img` of `in` / `out` (W x H, NHWC).
+// `img` of `in` / `out` (W x H, NHWC); taps at (dy, dx) * dil.
 template <int CIN, int MODE>
-__device__ __forceinline__ void conv_row(const uint16_t *__restrict__ in,
-                                         const uint16_t *__restrict__ Wt,
+__device__ __forceinline__ void conv_row(const ConvWeights<CIN> &cw, const uint16_t *__restrict__ in,
                                          const float *__restrict__ bias,
                                          const uint16_t *__restrict__ act,
                                          uint16_t *__restrict__ out, int W, int H, size_t img,
                                          int y, int xb, int dil)
 {
-    constexpr int K = 9 * CIN, KS = K / 16, CG = CIN / 16;
+    constexpr int K = ConvWeights<CIN>::K, KC = ConvWeights<CIN>::KC;
     const int lane = threadIdx.x & 63;
     const int lr = lane & 15, lq = lane >> 4;
-    v4s bfr[KS][2];
-#pragma unroll
-    for (int ks = 0; ks < KS; ++ks)
-#pragma unroll
-        for (int nb = 0; nb < 2; ++nb)
-            bfr[ks][nb] = *(const v4s *)(Wt + (size_t)(nb * 16 + lr) * K + ks * 16 + lq * 4);
     v4f acc[4][2];
 #pragma unroll
-    for (int mb = 0; mb < 4; ++mb)
+    for (int pb = 0; pb < 4; ++pb)
 #pragma unroll
-        for (int nb = 0; nb < 2; ++nb) acc[mb][nb] = v4f{0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-    for (int ks = 0; ks < KS; ++ks) {
-        const int tap = ks / CG, cg = ks % CG;
-        const int yy = y + (tap / 3 - 1) * dil;
-        const bool rowok = yy >= 0 && yy < H;
-#pragma unroll
-        for (int mb = 0; mb < 4; ++mb) {
-            const int xx = xb + mb * 16 + lr + (tap % 3 - 1) * dil;
-            v4s a = v4s{0, 0, 0, 0};
-            if (rowok && xx >= 0 && xx < W)
-                a = *(const v4s *)(in + (img + (size_t)yy * W + xx) * CIN + cg * 16 + lq * 4);
-            acc[mb][0] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, bfr[ks][0], acc[mb][0], 0, 0, 0);
-            acc[mb][1] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, bfr[ks][1], acc[mb][1], 0, 0, 0);
-        }
-    }
-#pragma unroll
-    for (int mb = 0; mb < 4; ++mb)
-#pragma unroll
-        for (int nb = 0; nb < 2; ++nb)
-#pragma unroll
-            for (int i = 0; i < 4; ++i) {
-                const int px = xb + mb * 16 + lq * 4 + i;
-                if (px >= W) continue;
-                const int co = nb * 16 + lr;
-                const size_t o = (img + (size_t)y * W + px) * CH + co;
-                float v = acc[mb][nb][i];
-                if (MODE == 0) {
-                    v += bias[co];
-                    v = v > 0.f ? v : 0.f;
-                } else {
-                    v = act[o] != 0 ? v : 0.f;
-                }
-                out[o] = f2bf(v);
-            }
-}
-
-// The same row with v_mfma_f32_16x16x32_bf16: one instruction covers 32 K
-// values (one tap of 32 channels, or two taps of 16), A fragments are 16-byte
-// loads (lane l: pixel l % 16, channels 8 * (l / 16) .. + 8 of its tap), so
-// a wavefront reads 1 KB contiguous per fragment.
-typedef __bf16 v8bf __attribute__((ext_vector_type(8)));
-
-template <int CIN, int MODE>
-__device__ __forceinline__ void conv_row_k32(const uint16_t *__restrict__ in,
-                                             const uint16_t *__restrict__ Wt,
-                                             const float *__restrict__ bias,
-                                             const uint16_t *__restrict__ act,
-                                             uint16_t *__restrict__ out, int W, int H, size_t img,
-                                             int y, int xb, int dil)
-{
-    constexpr int K = 9 * CIN, KC = (K + 31) / 32;
-    const int lane = threadIdx.x & 63;
-    const int lr = lane & 15, lq = lane >> 4;
-    v8bf bfr[KC][2];
-#pragma unroll
-    for (int c = 0; c < KC; ++c)
-#pragma unroll
-        for (int nb = 0; nb < 2; ++nb) {
-            const int k0 = c * 32 + lq * 8;
-            if (k0 < K) bfr[c][nb] = *(const v8bf *)(Wt + (size_t)(nb * 16 + lr) * K + k0);
-            else bfr[c][nb] = v8bf{};
-        }
-    v4f acc[4][2];
-#pragma unroll
-    for (int mb = 0; mb < 4; ++mb)
-#pragma unroll
-        for (int nb = 0; nb < 2; ++nb) acc[mb][nb] = v4f{0.f, 0.f, 0.f, 0.f};
+        for (int cb = 0; cb < 2; ++cb) acc[pb][cb] = v4f{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
     for (int c = 0; c < KC; ++c) {
         const int k0 = c * 32 + lq * 8;
         const int tap = k0 / CIN, ci0 = k0 % CIN;
         const int yy = y + (tap / 3 - 1) * dil;
         const bool rowok = k0 < K && yy >= 0 && yy < H;
+        v8bf b[4];
 #pragma unroll
-        for (int mb = 0; mb < 4; ++mb) {
-            const int xx = xb + mb * 16 + lr + (tap % 3 - 1) * dil;
-            v8bf a = v8bf{};
-            if (rowok && xx >= 0 && xx < W)
-                a = *(const v8bf *)(in + (img + (size_t)yy * W + xx) * CIN + ci0);
-            acc[mb][0] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, bfr[c][0], acc[mb][0], 0, 0, 0);
-            acc[mb][1] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, bfr[c][1], acc[mb][1], 0, 0, 0);
+        for (int pb = 0; pb < 4; ++pb) {
+            const int xx = xb + pb * 16 + lr + (tap % 3 - 1) * dil;
+            b[pb] = v8bf{};
+            if (rowok && xx >= 0 && xx < W) b[pb] = *(const v8bf *)(in + (img + (size_t)yy * W + xx) * CIN + ci0);
+        }
+#pragma unroll
+        for (int pb = 0; pb < 4; ++pb) {
+            acc[pb][0] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(cw.w[c][0], b[pb], acc[pb][0], 0, 0, 0);
+            acc[pb][1] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(cw.w[c][1], b[pb], acc[pb][1], 0, 0, 0);
         }
     }
+    // D[co][px]: lane holds channels cb*16 + 4*lq .. +3 of pixel xb + pb*16 + lr
 #pragma unroll
-    for (int mb = 0; mb < 4; ++mb)
+    for (int pb = 0; pb < 4; ++pb) {
+        const int px = xb + pb * 16 + lr;
+        if (px >= W) continue;
 #pragma unroll
-        for (int nb = 0; nb < 2; ++nb)
+        for (int cb = 0; cb < 2; ++cb) {
+            const int co = cb * 16 + lq * 4;
+            const size_t o = (img + (size_t)y * W + px) * CH + co;
+            uint16_t r[4];
+            if (MODE == 0) {
+                const float4 bv = *(const float4 *)(bias + co);
+                const float bb[4] = {bv.x, bv.y, bv.z, bv.w};
 #pragma unroll
-            for (int i = 0; i < 4; ++i) {
-                const int px = xb + mb * 16 + lq * 4 + i;
-                if (px >= W) continue;
-                const int co = nb * 16 + lr;
-                const size_t o = (img + (size_t)y * W + px) * CH + co;
-                float v = acc[mb][nb][i];
-                if (MODE == 0) {
-                    v += bias[co];
-                    v = v > 0.f ? v : 0.f;
-                } else {
-                    v = act[o] != 0 ? v : 0.f;
+                for (int i = 0; i < 4; ++i) {
+                    const float v = acc[pb][cb][i] + bb[i];
+                    r[i] = f2bf(v > 0.f ? v : 0.f);
                 }
-                out[o] = f2bf(v);
+            } else {
+                const uint64_t am = *(const uint64_t *)(act + o);
+#pragma unroll
+                for (int i = 0; i < 4; ++i) {
+                    const uint16_t ai = (uint16_t)(am >> (16 * i));
+                    r[i] = f2bf(ai != 0 ? acc[pb][cb][i] : 0.f);
+                }
             }
+            *(uint64_t *)(out + o) = (uint64_t)r[0] | ((uint64_t)r[1] << 16) | ((uint64_t)r[2] << 32) |
+                                     ((uint64_t)r[3] << 48);
+        }
+    }
 }
+
+// Rows per wavefront: the weights are loaded once and used for kConvRows
+// rows.  Workgroup: 4 wavefronts x kConvRows rows of 64 pixels; wave w owns
+// rows 4 r + w of its workgroup's 4 * kConvRows rows.
+constexpr int kConvRows = 4;
 
 template <int CIN, int MODE>
 __global__ __launch_bounds__(256) void k_conv3x3(const uint16_t *__restrict__ in,
@@ -267,37 +247,56 @@ __global__ __launch_bounds__(256) void k_conv3x3(const uint16_t *__restrict__ in
                                                  const uint16_t *__restrict__ act,
                                                  uint16_t *__restrict__ out, int W, int H, int dil)
 {
-    const int y = blockIdx.y * 4 + (threadIdx.x >> 6);
-    if (y >= H) return;
-    conv_row<CIN, MODE>(in, Wt, bias, act, out, W, H, (size_t)blockIdx.z * H * W, y, blockIdx.x * 64, dil);
+    ConvWeights<CIN> cw;
+    load_weights<CIN>(cw, Wt);
+    const size_t img = (size_t)blockIdx.z * H * W;
+    for (int r = 0; r < kConvRows; ++r) {
+        const int y = (blockIdx.y * kConvRows + r) * 4 + (threadIdx.x >> 6);
+        if (y >= H) break;
+        conv_row<CIN, MODE>(cw, in, bias, act, out, W, H, img, y, blockIdx.x * 64, dil);
+    }
 }
 
-// Batched views: workgroup b is 64 x 4 pixel block b of the batch (view
-// blk_view[b], local block b - blk_base, row-major over the view's blocks).
-__device__ __forceinline__ bool batch_block(const BatchPixView *__restrict__ views,
-                                            const uint8_t *__restrict__ blk_view, int &v, int &x,
-                                            int &y)
+// Batched views: 64 x 4 pixel block b of the batch belongs to view
+// blk_view[b] (local block b - blk_base, row-major over the view's blocks).
+__device__ __forceinline__ bool batch_block_at(const BatchPixView *__restrict__ views,
+                                               const uint8_t *__restrict__ blk_view, uint32_t blk,
+                                               int &v, int &x, int &y)
 {
-    v = blk_view[blockIdx.x];
+    v = blk_view[blk];
     const BatchPixView &bv = views[v];
-    const int lb = (int)(blockIdx.x - bv.blk_base), nbx = (bv.W + 63) / 64;
+    const int lb = (int)(blk - bv.blk_base), nbx = (bv.W + 63) / 64;
     x = (lb % nbx) * 64 + (threadIdx.x & 63);
     y = (lb / nbx) * 4 + (threadIdx.x >> 6);
     return y < bv.H;
 }
 
+__device__ __forceinline__ bool batch_block(const BatchPixView *__restrict__ views,
+                                            const uint8_t *__restrict__ blk_view, int &v, int &x,
+                                            int &y)
+{
+    return batch_block_at(views, blk_view, blockIdx.x, v, x, y);
+}
+
+// Workgroup g covers batch blocks g * kConvRows .. + kConvRows - 1.
 template <int CIN>
 __global__ __launch_bounds__(256) void k_conv3x3_batch(const BatchPixView *__restrict__ views,
-                                                       const uint8_t *__restrict__ blk_view,
+                                                       const uint8_t *__restrict__ blk_view, uint32_t n_blocks,
                                                        const uint16_t *__restrict__ in,
                                                        const uint16_t *__restrict__ Wt,
                                                        const float *__restrict__ bias,
                                                        uint16_t *__restrict__ out, int dil)
 {
-    int v, x, y;
-    if (!batch_block(views, blk_view, v, x, y)) return;
-    const BatchPixView &bv = views[v];
-    conv_row<CIN, 0>(in, Wt, bias, nullptr, out, bv.W, bv.H, bv.px_off, y, x - (threadIdx.x & 63), dil);
+    ConvWeights<CIN> cw;
+    load_weights<CIN>(cw, Wt);
+    for (int r = 0; r < kConvRows; ++r) {
+        const uint32_t blk = blockIdx.x * kConvRows + r;
+        if (blk >= n_blocks) break;
+        int v, x, y;
+        if (!batch_block_at(views, blk_view, blk, v, x, y)) continue;
+        const BatchPixView &bv = views[v];
+        conv_row<CIN, 0>(cw, in, bias, nullptr, out, bv.W, bv.H, bv.px_off, y, x - (threadIdx.x & 63), dil);
+    }
 }
 
 // Weight gradient: dW[co][tap*CIN+ci] = sum_p dout[p][co] * in[p+off(tap)][ci]
@@ -608,12 +607,13 @@ int nao_infer_batch(fgl_nao *n, hipStream_t s, const BatchPixView *views, const 
     if (n_blocks == 0) return FGL_OK;
     const dim3 g(n_blocks), b(256);
     hipLaunchKernelGGL(k_features_batch, g, b, 0, s, views, blk_view, depth, gbuf, n->ifeat, n->imask);
-    hipLaunchKernelGGL((k_conv3x3_batch<C0>), g, b, 0, s, views, blk_view, (const uint16_t *)n->ifeat, n->w1,
-                       n->params + OFF_B1, n->ia, layer_dil(0));
+    const dim3 gc((n_blocks + kConvRows - 1) / kConvRows);
+    hipLaunchKernelGGL((k_conv3x3_batch<C0>), gc, b, 0, s, views, blk_view, n_blocks, (const uint16_t *)n->ifeat,
+                       n->w1, n->params + OFF_B1, n->ia, layer_dil(0));
     const int boffs[3] = {OFF_B2, OFF_B3, OFF_B4};
     uint16_t *src = n->ia, *dst = n->ib;
     for (int l = 0; l < 3; ++l) {
-        hipLaunchKernelGGL((k_conv3x3_batch<CH>), g, b, 0, s, views, blk_view, (const uint16_t *)src,
+        hipLaunchKernelGGL((k_conv3x3_batch<CH>), gc, b, 0, s, views, blk_view, n_blocks, (const uint16_t *)src,
                            n->w234 + (size_t)l * CH * K2, n->params + boffs[l], dst, layer_dil(l + 1));
         uint16_t *t = src;
         src = dst;
@@ -754,7 +754,7 @@ int fgl_nao_stage_crops(fgl_nao *n, int slot0, int count, const int *xy)
 static void forward_convs(fgl_nao *n, hipStream_t s, const uint16_t *feat, uint16_t **acts, int W, int H,
                           int N)
 {
-    dim3 grid((W + 63) / 64, (H + 3) / 4, N);
+    dim3 grid((W + 63) / 64, (H + 4 * kConvRows - 1) / (4 * kConvRows), N);
     hipLaunchKernelGGL((k_conv3x3<C0, 0>), grid, dim3(256), 0, s, feat, n->w1, n->params + OFF_B1,
                        (const uint16_t *)nullptr, acts[0], W, H, layer_dil(0));
     const int boffs[3] = {OFF_B2, OFF_B3, OFF_B4};
@@ -785,7 +785,7 @@ static int run_batch(fgl_nao *n, bool update, float lr, float *stats)
     const dim3 wg(S / 64, S / WG_ROWS, N);
     const int nblk = (int)(wg.x * wg.y * wg.z);
     const int woffs[4] = {OFF_W1, OFF_W2, OFF_W3, OFF_W4}, boffs[4] = {OFF_B1, OFF_B2, OFF_B3, OFF_B4};
-    const dim3 cgrid((S + 63) / 64, (S + 3) / 4, N);
+    const dim3 cgrid((S + 63) / 64, (S + 4 * kConvRows - 1) / (4 * kConvRows), N);
     for (int l = 3; l >= 0; --l) {
         const int K = l == 0 ? K1 : K2;
         if (l == 0)

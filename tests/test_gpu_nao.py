@@ -54,10 +54,40 @@ def features_f32(frame):
     vn = [dot(V[r, :3], N) for r in range(3)]
     feat = [-vp[2] * f32(0.1), vn[0], vn[1], vn[2], vp[0] * f32(0.1), vp[1] * f32(0.1), vp[2] * f32(0.1)]
     cov = frame["depth"] < 3e38
+    with np.errstate(invalid="ignore"):
+        for c in range(7):
+            cov &= np.isfinite(feat[c])   # non-finite G-buffer values count as uncovered
     out = np.zeros((H, W, 16), np.float32)
     for c in range(7):
         out[..., c] = np.where(cov, feat[c], f32(0))
     return out, cov
+
+
+def test_nonfinite_gbuffer_pixels_are_uncovered(ctx, world):
+    """A NaN normal or position in the G-buffer gives zero features and mask
+    0 (it must not reach the weight gradients), and AO 1 at inference."""
+    sset, cfg, frame = world
+    NA.render_frame(ctx, sset, 5003, W, H, cfg)
+    g = ctx.read(F.TARGET_GBUFFER).reshape(F.GBUF_PLANES, H, W).copy()
+    cover = frame["depth"] < 3e38
+    ys, xs = np.nonzero(cover)
+    picks = [(int(ys[k]), int(xs[k])) for k in (0, len(ys) // 2, len(ys) - 1)]
+    g[0, picks[0][0], picks[0][1]] = np.nan      # normal x
+    g[4, picks[1][0], picks[1][1]] = np.inf      # position y
+    g[2, picks[2][0], picks[2][1]] = np.nan      # normal z
+    ctx.write(F.TARGET_GBUFFER, g)
+    n = NA.NeuralAO(ctx, batch=1, crop=64, params=_params())
+    n.infer()
+    ao = ctx.read(F.TARGET_AO).reshape(H, W)
+    for y, x in picks:
+        assert ao[y, x] == 1.0
+        x0, y0 = min(max(x - 32, 0), W - 64), min(max(y - 32, 0), H - 64)
+        n.stage(0, [(x0, y0)])
+        feat, _, mask = n.read_batch()
+        assert mask[0, y - y0, x - x0] == 0 and not feat[0, y - y0, x - x0].any()
+    g_grad, st = n.gradients()
+    assert np.all(np.isfinite(g_grad))
+    n.close()
 
 
 def bf16_bits(x):
