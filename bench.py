@@ -68,12 +68,28 @@ def algorithmic_bytes(name, I, ntri, npos):
     return tab.get(name)
 
 
-def run_gpu(sc, steps, warmup, tile):
+def measured_traffic(kernel):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 PMC
+    summary (FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0, calibrated on a plain
+    copy), or None."""
+    import re
+    p = os.path.join(ROOT, "profiles", f"{kernel}_c3_hbm_traffic.md")
+    if not os.path.exists(p):
+        return None
+    m = re.search(r"traffic_bytes_per_launch:\s*(\d+)", open(p).read())
+    return int(m.group(1)) if m else None
+
+
+def run_gpu(sc, steps, warmup, tile, tiles=None):
     import forkerrenderer_amd as F
     from forkerrenderer_amd.autotune import autotune
     ctx = F.Context(0)
     tuned = None
-    if tile:
+    if tiles:
+        ctx.set_pass_tile_size(F.PASS_SHADOW, tiles[0])
+        ctx.set_pass_tile_size(F.PASS_GEOMETRY, tiles[1])
+        tuned = ({"shadow": tiles[0], "main": tiles[1]}, None)
+    elif tile:
         ctx.set_tile_size(tile)
     else:
         tuned = autotune(ctx, sc)
@@ -287,6 +303,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--config", default="c3")
     ap.add_argument("--tile", type=int, default=0, help="tile edge 8/16/32; 0 = autotune")
+    ap.add_argument("--tiles", type=int, nargs=2, metavar=("SHADOW", "MAIN"), default=None,
+                    help="per-pass tile edges instead of autotuning")
     ap.add_argument("--cpu-band", type=float, default=1.0 / 32)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-nao", action="store_true", help="skip the neural-AO vs plain SSAO comparison")
@@ -311,7 +329,7 @@ def main():
     I = sc.info
     ntri, npos = scene_stats(sc)
     print(f"[bench] scene {p}: {ntri} triangles, {npos} vertices", file=sys.stderr, flush=True)
-    sec, per, img, tuned = run_gpu(sc, a.steps, a.warmup, a.tile)
+    sec, per, img, tuned = run_gpu(sc, a.steps, a.warmup, a.tile, a.tiles)
     print(f"[bench] gpu {sec * 1e3:.3f} ms/frame", file=sys.stderr, flush=True)
     kernels = {}
     for name, v in per.items():
@@ -341,7 +359,9 @@ def main():
                                f"SSAO {I.ssao.n_kernel} + blur, {ntri} triangles",
                    "internal_resolution": [I.width * k, I.height * k], "triangles": ntri, "tile": a.tile if a.tile else (tuned[0] if tuned else None), "autotune_ms": tuned[1] if tuned else None},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None, "bytes_per_launch": bytes_launch,
+                     "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": measured_traffic(dom) if a.config == "c3" else None,
+                     "bytes_per_launch": bytes_launch,
                      "ms_per_launch": ms_launch},
         "kernels": kernels,
         "cpu_baseline": cpu,

@@ -276,6 +276,9 @@ class Context:
         return out.value
 
     # ---- state ----
+    def set_pass_tile_size(self, pass_, t):
+        self._chk(lib().fgl_set_pass_tile_size(self._h, pass_, t), "fgl_set_pass_tile_size")
+
     def set_tile_size(self, t):
         self._chk(lib().fgl_set_tile_size(self._h, t), "fgl_set_tile_size")
 
