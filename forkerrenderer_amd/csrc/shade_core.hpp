@@ -34,9 +34,12 @@ __device__ __forceinline__ void lighting_px(const FrameUniforms &u, const float 
 
 // Kernel SSAO at pixel (x,y): the deterministic replacement for
 // Render::DoSSAO's RNG hemisphere samples (src/render.cpp:214-289).  Tangent
-// frame from the 4x4 noise vector, n kernel offsets, projection to window
-// space exactly as the reference does, depth gather, bias and range check;
-// off-screen taps add no occlusion; ao = (1-occ)^3.
+// frame from the 4x4 noise vector; the clip-space sample
+// VP * (P + radius * (T kx + B ky + N kz), 1) linearised per pixel as
+// c0 + radius * (kx cT + ky cB + kz cN) (c0 = VP * (P, 1), cT|cB|cN =
+// VP * (dir, 0)), evaluated with fmaf in a fixed order; window coordinates =
+// viewport diagonal * (clip / w) + viewport offset; depth gather, bias and
+// range check; off-screen taps add no occlusion; ao = (1 - count / n)^3.
 __device__ __forceinline__ void ssao_px(const FrameUniforms &u, const SsaoArgs &sa,
                                         const float *__restrict__ depth,
                                         const float *__restrict__ g, float *__restrict__ ao,
@@ -46,29 +49,39 @@ __device__ __forceinline__ void ssao_px(const FrameUniforms &u, const SsaoArgs &
     const size_t i = (size_t)x + (size_t)y * u.W;
     Vec3 N = vec3(g[i], g[P + i], g[2 * P + i]);
     Vec3 Pw = vec3(g[3 * P + i], g[4 * P + i], g[5 * P + i]);
-    float fd = depth[i];
-    int ni = (x & 3) + 4 * (y & 3);
+    const float fd = depth[i];
+    const int ni = (x & 3) + 4 * (y & 3);
     Vec3 r = vec3(sa.noise[3 * ni], sa.noise[3 * ni + 1], sa.noise[3 * ni + 2]);
     Vec3 T = norm3(sub3(r, scale3(N, dot3(r, N))));
     Vec3 B = cross3(N, T);
-    const float scale = 1.f / (float)sa.n_kernel;
-    float occ = 0.f;
-    for (int s = 0; s < sa.n_kernel; ++s) {
-        Vec3 k = vec3(sa.kernel[3 * s], sa.kernel[3 * s + 1], sa.kernel[3 * s + 2]);
-        Vec3 d = add3(add3(scale3(T, k.x), scale3(B, k.y)), scale3(N, k.z));
-        Vec3 S = add3(Pw, scale3(d, sa.radius));
-        Vec4 cs = mv4(u.vp, vec4(S.x, S.y, S.z, 1.f));
-        Vec4 nd = divs4(cs, cs.w);
-        Vec4 ss = mv4(u.viewport, nd);
-        int sx = trunc_int(ss.x), sy = trunc_int(ss.y);
-        if (sx < 0 || sy < 0 || sx >= u.W || sy >= u.H) continue;
-        float cached = depth[(size_t)sx + (size_t)sy * u.W];
-        if (ss.z >= cached + sa.bias) {
-            float rc = fabsf(fd - cached) < sa.range ? 1.f : 0.f;
-            occ += scale * rc;
-        }
+    const Vec4 c0 = mv4(u.vp, vec4(Pw.x, Pw.y, Pw.z, 1.f));
+    float cT[4], cB[4], cN[4];
+    const float cc[4] = {c0.x, c0.y, c0.z, c0.w};
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        const Vec3 row = vec3(u.vp[4 * j], u.vp[4 * j + 1], u.vp[4 * j + 2]);
+        cT[j] = dot3(row, T);
+        cB[j] = dot3(row, B);
+        cN[j] = dot3(row, N);
     }
-    float a = 1.f - occ;
+    const float *vp = u.viewport;
+    const float radius = sa.radius;
+    int count = 0;
+    for (int s = 0; s < sa.n_kernel; ++s) {
+        const float kx = sa.kernel[3 * s], ky = sa.kernel[3 * s + 1], kz = sa.kernel[3 * s + 2];
+        float cs[4];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) cs[j] = fmaf(radius, fmaf(kz, cN[j], fmaf(ky, cB[j], kx * cT[j])), cc[j]);
+        const float inv = 1.f / cs[3];
+        const float sxf = fmaf(vp[0], cs[0] * inv, vp[3]);
+        const float syf = fmaf(vp[5], cs[1] * inv, vp[7]);
+        const float sz = fmaf(vp[10], cs[2] * inv, vp[11]);
+        const int sx = trunc_int(sxf), sy = trunc_int(syf);
+        if (sx < 0 || sy < 0 || sx >= u.W || sy >= u.H) continue;
+        const float cached = depth[(size_t)sx + (size_t)sy * u.W];
+        if (sz >= cached + sa.bias && fabsf(fd - cached) < sa.range) ++count;
+    }
+    const float a = 1.f - (float)count * (1.f / (float)sa.n_kernel);
     ao[i] = (a * a) * a;
 }
 

@@ -940,9 +940,16 @@ void orc_ssao_kernel(const orc_frame *fr, int W, int H, const float *depth,
                      int n_kernel, const float *noise, float radius, float bias, float range,
                      float *ao)
 {
+    /* Kernel SSAO with the projection linearised per pixel: the clip-space
+     * sample VP * (P + radius * (T kx + B ky + N kz), 1) is evaluated as
+     * c0 + radius * (kx cT + ky cB + kz cN) with c0 = VP * (P, 1) and cT,
+     * cB, cN = VP * (T|B|N, 0), each component by fmaf in the order below;
+     * window x, y, z = viewport diagonal * (clip / w) + viewport offset by
+     * fmaf; occlusion as Render::DoSSAO tests it (render.cpp:250-275). */
     float VP[16];
     orc_mat_mul(fr->proj, fr->view, VP);
-    float scale = 1.f / (float)n_kernel;
+    const float *vp = fr->viewport;
+    const float scale = 1.f / (float)n_kernel;
     for (int y = g_y0 > 0 ? g_y0 : 0; y < H && y < g_y1; ++y)
         for (int x = 0; x < W; ++x) {
             size_t i = (size_t)x + (size_t)y * W;
@@ -953,23 +960,31 @@ void orc_ssao_kernel(const orc_frame *fr, int W, int H, const float *depth,
             V3 r = v3(noise[3 * ni], noise[3 * ni + 1], noise[3 * ni + 2]);
             V3 T = norm3(sub3(r, scale3(N, dot3(r, N))));
             V3 B = cross3(N, T);
-            float occ = 0.f;
+            V4 c0 = mv4(VP, (V4){P.x, P.y, P.z, 1.f});
+            float cT[4], cB[4], cN[4], cc[4];
+            for (int j = 0; j < 4; ++j) {
+                V3 row = v3(VP[4 * j], VP[4 * j + 1], VP[4 * j + 2]);
+                cT[j] = dot3(row, T);
+                cB[j] = dot3(row, B);
+                cN[j] = dot3(row, N);
+            }
+            cc[0] = c0.x; cc[1] = c0.y; cc[2] = c0.z; cc[3] = c0.w;
+            int count = 0;
             for (int s = 0; s < n_kernel; ++s) {
-                V3 k = v3(kernel[3 * s], kernel[3 * s + 1], kernel[3 * s + 2]);
-                V3 d = add3(add3(scale3(T, k.x), scale3(B, k.y)), scale3(N, k.z));
-                V3 S = add3(P, scale3(d, radius));
-                V4 cs = mv4(VP, (V4){S.x, S.y, S.z, 1.f});
-                V4 nd = divs4(cs, cs.w);
-                V4 ss = mv4(fr->viewport, nd);
-                int sx = orc_trunc_int(ss.x), sy = orc_trunc_int(ss.y);
+                const float kx = kernel[3 * s], ky = kernel[3 * s + 1], kz = kernel[3 * s + 2];
+                float cs[4];
+                for (int j = 0; j < 4; ++j)
+                    cs[j] = fmaf(radius, fmaf(kz, cN[j], fmaf(ky, cB[j], kx * cT[j])), cc[j]);
+                const float inv = 1.f / cs[3];
+                const float sxf = fmaf(vp[0], cs[0] * inv, vp[3]);
+                const float syf = fmaf(vp[5], cs[1] * inv, vp[7]);
+                const float sz = fmaf(vp[10], cs[2] * inv, vp[11]);
+                int sx = orc_trunc_int(sxf), sy = orc_trunc_int(syf);
                 if (sx < 0 || sy < 0 || sx >= W || sy >= H) continue;
                 float cached = depth[(size_t)sx + (size_t)sy * W];
-                if (ss.z >= cached + bias) {
-                    float rc = fabsf(fd - cached) < range ? 1.f : 0.f;
-                    occ += scale * rc;
-                }
+                if (sz >= cached + bias && fabsf(fd - cached) < range) ++count;
             }
-            float a = 1.f - occ;
+            float a = 1.f - (float)count * scale;
             ao[i] = (a * a) * a;
         }
 }

@@ -368,7 +368,7 @@ def test_equal_depth_tie_goes_to_first_submitted():
 
 def test_kernel_ssao_flat_facing_plane_is_unoccluded():
     """A plane facing the camera: every hemisphere tap lies in front of it, so
-    AO is exactly 1; a concave corner is occluded."""
+    AO is exactly 1."""
     W, H = 40, 30
     quad = [(-3, -3, -2), (3, -3, -2), (3, 3, -2), (-3, 3, -2)]
     md, M = _draw(quad, [(0, 1, 2), (0, 2, 3)])
@@ -391,3 +391,76 @@ def test_kernel_ssao_flat_facing_plane_is_unoccluded():
     cfg.n_kernel, cfg.radius, cfg.bias, cfg.range = 16, 0.3, 0.0005, 0.01
     ao = O.ssao_kernel(fr, W, H, depth, g, cfg)
     assert np.all(ao[depth < 3e38] == 1.0)
+
+
+def _ssao_cfg(n, seed):
+    class Cfg:
+        pass
+    rng = np.random.default_rng(seed)
+    cfg = Cfg()
+    k = rng.normal(size=(n, 3)); k[:, 2] = np.abs(k[:, 2]); k /= np.linalg.norm(k, axis=1, keepdims=True)
+    cfg.kernel = (k * rng.uniform(0.1, 1, (n, 1))).astype(np.float32).reshape(-1).tolist() + [0.0] * (192 - 3 * n)
+    cfg.noise = np.concatenate([rng.uniform(-1, 1, (16, 2)), np.zeros((16, 1))], 1).astype(np.float32).reshape(-1).tolist()
+    cfg.n_kernel, cfg.radius, cfg.bias, cfg.range = n, 0.5, 0.0005, 0.01
+    return cfg
+
+
+def test_kernel_ssao_matches_direct_fp64_projection():
+    """The oracle's linearised kernel SSAO (clip = c0 + r (kx cT + ky cB + kz
+    cN), fmaf, window = diagonal * clip / w + offset) against the textbook
+    definition evaluated in float64: sample S = P + r (T kx + B ky + N kz),
+    window = Viewport * (VP * (S, 1)) / w, the same gather and tests.  Only
+    samples within rounding of a threshold may differ; a dropped term, wrong
+    sign or swapped axis changes most pixels.  The scene is a concave corner
+    (floor and wall), so many taps are occluded."""
+    W, H = 48, 36
+    floor = [(-3, -1, 1), (3, -1, 1), (3, -1, -4), (-3, -1, -4)]
+    wall = [(-3, -1, -2), (3, -1, -2), (3, 3, -2), (-3, 3, -2)]
+    md0, M = _draw(floor, [(0, 1, 2), (0, 2, 3)])
+    md1, _ = _draw(wall, [(0, 1, 2), (0, 2, 3)])
+    m = O.Material(); m.map_kd = m.map_ks = m.map_ke = m.map_bump = -1
+    m.map_base = m.map_rough = m.map_metal = m.map_ao = m.map_pbr_ke = -1
+    sc = O.OracleScene([(md0, M), (md1, M)], [m], [])
+    V = O.make_lookat([0.3, 0.8, 1.5], [0, -0.5, -2])
+    P = O.make_perspective(45.0, W / H, 0.01, 20.0)
+    vp = O.make_viewport(0, 0, W, H)
+    fr = O.make_frame(V, P, np.eye(4), vp, vp, [0.3, 0.8, 1.5], 0, None, [])
+    depth, g = O.geometry_pass(sc, fr, W, H)
+    cfg = _ssao_cfg(32, 4)
+    ao = O.ssao_kernel(fr, W, H, depth, g, cfg).reshape(H, W)
+    VP = O.mat_mul(P, V).astype(np.float64).reshape(4, 4)
+    VPt = np.asarray(vp, np.float64).reshape(4, 4)
+    k = np.array(cfg.kernel[:96], np.float64).reshape(32, 3)
+    nz = np.array(cfg.noise, np.float64).reshape(16, 3)
+    d2 = depth.reshape(H, W)
+    cov = d2 < 3e38
+    diff = []
+    occluded = 0
+    for y in range(H):
+        for x in range(W):
+            if not cov[y, x]:
+                continue
+            i = x + y * W
+            N = g[0:3, i].astype(np.float64)
+            Pw = g[3:6, i].astype(np.float64)
+            r = nz[(x & 3) + 4 * (y & 3)]
+            T = r - N * (r @ N); T /= np.linalg.norm(T)
+            B = np.cross(N, T)
+            cnt = 0
+            for s in range(32):
+                S = Pw + cfg.radius * (T * k[s, 0] + B * k[s, 1] + N * k[s, 2])
+                c = VP @ np.append(S, 1.0)
+                w = VPt @ (c / c[3])
+                sx, sy = math.trunc(w[0]), math.trunc(w[1])
+                if sx < 0 or sy < 0 or sx >= W or sy >= H:
+                    continue
+                cached = float(d2[sy, sx])
+                if w[2] >= cached + cfg.bias and abs(float(d2[y, x]) - cached) < cfg.range:
+                    cnt += 1
+            want = (1.0 - cnt / 32) ** 3
+            got_cnt = round(32 * (1 - float(ao[y, x]) ** (1 / 3)))
+            occluded += cnt > 0
+            diff.append(abs(got_cnt - cnt))
+    diff = np.array(diff)
+    assert occluded > 0.3 * len(diff)
+    assert np.mean(diff == 0) > 0.97 and diff.max() <= 2, (np.mean(diff == 0), diff.max())
