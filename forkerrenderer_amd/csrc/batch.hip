@@ -93,7 +93,7 @@ __global__ void k_bv_overflow(const uint32_t *__restrict__ total, uint32_t cap, 
 template <int TS, int BLOCK, bool SHADOW>
 __global__ __launch_bounds__(BLOCK) void k_bv_raster(BvRasterArgs b)
 {
-    __shared__ TriL lds[BLOCK];
+    __shared__ RasterLds<TS, BLOCK> lds;
     __shared__ uint32_t s_item;
     constexpr int P = SHADOW ? kBvShadow : kBvMain;
     for (;;) {
@@ -121,7 +121,7 @@ __global__ __launch_bounds__(BLOCK) void k_bv_raster(BvRasterArgs b)
         a.depth = SHADOW ? b.shadow_depth + v.spx_off : b.depth + v.px_off;
         a.vis = b.vis + v.px_off;
         a.shadow_out = b.shadow + v.spx_off;
-        raster_tile<TS, BLOCK, SHADOW>(a, tile, lds);
+        raster_tile_fast<TS, BLOCK, SHADOW>(a, tile, lds);
     }
 }
 

@@ -108,9 +108,9 @@ __global__ __launch_bounds__(256) void k_tile_ranges(const uint32_t *__restrict_
 template <int TS, int BLOCK, bool SHADOW>
 __global__ __launch_bounds__(BLOCK) void k_raster(RasterArgs a)
 {
-    __shared__ TriL lds[BLOCK];
+    __shared__ RasterLds<TS, BLOCK> lds;
     const int tile = xcd_remap(blockIdx.x, a.tiles_x * a.tiles_y);
-    raster_tile<TS, BLOCK, SHADOW>(a, tile, lds);
+    raster_tile_fast<TS, BLOCK, SHADOW>(a, tile, lds);
 }
 
 __global__ __launch_bounds__(256) void k_gbuffer_resolve(ResolveArgs a, FrameUniforms u)

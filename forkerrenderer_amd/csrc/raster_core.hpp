@@ -273,6 +273,195 @@ __device__ __forceinline__ void raster_tile(const RasterArgs &a, int tile, TriL 
     }
 }
 
+// ---------------- order-independent tile rasteriser ----------------
+//
+// The submission-order depth fold of src/forkergl.cpp:180-199 (replace when
+// !(d >= best)) picks, for every pixel, the first triangle -- lowest
+// pass-wide id g -- among those of minimum depth, as long as no depth is NaN
+// (with -0 and +0 equal).  That makes it a minimum over 64-bit keys
+// (order(depth) << 32 | g + 1) and lets any lane fold any triangle into any
+// pixel with an LDS atomicMin.  Triangles whose bbox covers at most
+// kSmallTriArea pixels of the tile are rasterised one per lane (the common
+// case for dense meshes, where a wave-per-triangle walk would leave most
+// lanes idle); larger ones one per wave, a lane per pixel.  A NaN depth
+// anywhere in the tile (initial or computed) re-runs the tile with the
+// sequential fold above.
+
+constexpr int kSmallTriArea = 16;
+
+// Monotone map of float depth to uint32 with -0 == +0.
+__device__ __forceinline__ uint32_t depth_order(float d)
+{
+    const uint32_t u = __float_as_uint(d == 0.f ? 0.f : d);
+    return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+}
+
+__device__ __forceinline__ float depth_from_order(uint32_t k)
+{
+    return __uint_as_float((k & 0x80000000u) ? (k & 0x7fffffffu) : ~k);
+}
+
+template <int TS, int BLOCK>
+struct RasterLds {
+    TriL tri[BLOCK];
+    unsigned long long key[TS * TS];
+    int nan_seen;
+};
+
+// Depth of staged triangle t at a covered pixel: Dot(bary, depths) in the
+// reference's order.
+__device__ __forceinline__ float tri_depth(const TriL &t, float b0, float b1, float b2)
+{
+    float dd = 0.f;
+    dd += b0 * t.z0;
+    dd += b1 * t.z1;
+    dd += b2 * t.z2;
+    return dd;
+}
+
+template <int TS, int BLOCK, bool SHADOW>
+__device__ __forceinline__ void raster_tile_fast(const RasterArgs &a, int tile, RasterLds<TS, BLOCK> &L)
+{
+    constexpr int NW = BLOCK / 64;
+    constexpr int QS = (NW == 1) ? TS : TS / 2;
+    constexpr int NPX = QS * QS / 64;
+    const int tx = tile % a.tiles_x, ty = tile / a.tiles_x;
+    const int x0 = tx * TS, y0 = ty * TS;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    if (tid == 0) L.nan_seen = 0;
+    __syncthreads();
+    for (int p = tid; p < TS * TS; p += BLOCK) {
+        const int px = x0 + p % TS, py = y0 + p / TS;
+        float init = 3.40282347e+38f;
+        if (!a.fresh && px < a.W && py < a.H) init = a.depth[(size_t)px + (size_t)py * a.W];
+        if (init != init) L.nan_seen = 1;
+        L.key[p] = (unsigned long long)depth_order(init) << 32;
+    }
+    const int qx0 = x0 + (NW == 1 ? 0 : (wave & 1) * QS);
+    const int qy0 = y0 + (NW == 1 ? 0 : (wave >> 1) * QS);
+    const int rx1 = qx0 + QS - 1, ry1 = qy0 + QS - 1;
+    const uint32_t start = a.tile_start[tile], end = a.tile_end[tile];
+    for (uint32_t base = start; base < end; base += BLOCK) {
+        const int n = imin(BLOCK, (int)(end - base));
+        __syncthreads();   // previous chunk's triangles fully consumed
+        if (tid < n) {
+            const int g = (int)a.vals[base + tid];
+            const int4 tv = a.tri_v[g];
+            stage_tri(L.tri[tid], g, a.screen[tv.x], a.screen[tv.y], a.screen[tv.z], a.W, a.H);
+        }
+        __syncthreads();
+        // small triangles: one per lane
+        bool small = false;
+        if (tid < n) {
+            const TriL &t = L.tri[tid];
+            const int bx0 = imax(t.bb0 & 0xffff, x0), bx1 = imin(t.bb1 & 0xffff, x0 + TS - 1);
+            const int by0 = imax(t.bb0 >> 16, y0), by1 = imin(t.bb1 >> 16, y0 + TS - 1);
+            const int area = (bx1 >= bx0 && by1 >= by0) ? (bx1 - bx0 + 1) * (by1 - by0 + 1) : 0;
+            small = area <= kSmallTriArea;
+            if (area > 0 && small) {
+                const unsigned long long gk = (unsigned long long)(uint32_t)(t.g + 1);
+                for (int py = by0; py <= by1; ++py)
+                    for (int px = bx0; px <= bx1; ++px) {
+                        float b0, b1, b2;
+                        if (!cover(t, px, py, b0, b1, b2)) continue;
+                        const float dd = tri_depth(t, b0, b1, b2);
+                        if (dd != dd) {
+                            L.nan_seen = 1;
+                            continue;
+                        }
+                        atomicMin(&L.key[(py - y0) * TS + (px - x0)],
+                                  ((unsigned long long)depth_order(dd) << 32) | gk);
+                    }
+            }
+        }
+        // large triangles: one per wave over its quadrant, a lane per pixel
+        unsigned long long mask[NW];
+#pragma unroll
+        for (int j = 0; j < NW; ++j) {
+            const int s = j * 64 + lane;
+            bool hit = false;
+            if (s < n) {
+                const TriL &t = L.tri[s];
+                const int minx = t.bb0 & 0xffff, miny = t.bb0 >> 16;
+                const int maxx = t.bb1 & 0xffff, maxy = t.bb1 >> 16;
+                const int bx0 = imax(minx, x0), bx1 = imin(maxx, x0 + TS - 1);
+                const int by0 = imax(miny, y0), by1 = imin(maxy, y0 + TS - 1);
+                const int area = (bx1 >= bx0 && by1 >= by0) ? (bx1 - bx0 + 1) * (by1 - by0 + 1) : 0;
+                hit = area > kSmallTriArea && !(maxx < qx0 || minx > rx1 || maxy < qy0 || miny > ry1);
+            }
+            mask[j] = __ballot(hit);
+        }
+        (void)small;
+#pragma unroll
+        for (int j = 0; j < NW; ++j) {
+            unsigned long long m = mask[j];
+            while (m) {
+                const int i = j * 64 + __builtin_ctzll(m);
+                m &= m - 1;
+                const TriL &t = L.tri[i];
+                const int minx = t.bb0 & 0xffff, miny = t.bb0 >> 16;
+                const int maxx = t.bb1 & 0xffff, maxy = t.bb1 >> 16;
+                const unsigned long long gk = (unsigned long long)(uint32_t)(t.g + 1);
+#pragma unroll
+                for (int k = 0; k < NPX; ++k) {
+                    const int lin = lane + 64 * k;
+                    const int px = qx0 + lin % QS, py = qy0 + lin / QS;
+                    if (px < a.W && py < a.H && px >= minx && px <= maxx && py >= miny && py <= maxy) {
+                        float b0, b1, b2;
+                        if (cover(t, px, py, b0, b1, b2)) {
+                            const float dd = tri_depth(t, b0, b1, b2);
+                            if (dd != dd) L.nan_seen = 1;
+                            else
+                                atomicMin(&L.key[(py - y0) * TS + (px - x0)],
+                                          ((unsigned long long)depth_order(dd) << 32) | gk);
+                        }
+                    }
+                }
+            }
+        }
+    }
+    __syncthreads();
+    if (L.nan_seen) {
+        raster_tile<TS, BLOCK, SHADOW>(a, tile, L.tri);
+        return;
+    }
+    for (int p = tid; p < TS * TS; p += BLOCK) {
+        const int px = x0 + p % TS, py = y0 + p / TS;
+        if (px >= a.W || py >= a.H) continue;
+        const size_t i = (size_t)px + (size_t)py * a.W;
+        const unsigned long long key = L.key[p];
+        const int win = (int)(uint32_t)(key & 0xffffffffu) - 1;
+        if (win >= 0) {
+            float dd = depth_from_order((uint32_t)(key >> 32));
+            TriL t;
+            float b0 = 0.f, b1 = 0.f, b2 = 0.f;
+            const int4 tv = a.tri_v[win];
+            if (dd == 0.f || SHADOW) {
+                stage_tri(t, win, a.screen[tv.x], a.screen[tv.y], a.screen[tv.z], a.W, a.H);
+                cover(t, px, py, b0, b1, b2);
+                if (dd == 0.f) dd = tri_depth(t, b0, b1, b2);   // keeps the sign of a zero
+            }
+            a.depth[i] = dd;
+            if (!SHADOW) {
+                a.vis[i] = win;
+            } else {
+                // DepthShader::ProcessFragment (depthshader.h:30-36)
+                const DevDraw &dr = a.draws[tv.w];
+                const int f = win - dr.tri_base;
+                Vec4 n0 = depth_ndc(dr, dr.vi[3 * f]);
+                Vec4 n1 = depth_ndc(dr, dr.vi[3 * f + 1]);
+                Vec4 n2 = depth_ndc(dr, dr.vi[3 * f + 2]);
+                float z = dot3(vec3(n0.z, n1.z, n2.z), vec3(b0, b1, b2));
+                a.shadow_out[i] = z * 0.5f + 0.5f;
+            }
+        } else {
+            if (a.fresh) a.depth[i] = 3.40282347e+38f;
+            if (!SHADOW) a.vis[i] = -1;
+            else if (a.fresh) a.shadow_out[i] = 0.f;
+        }
+    }
+}
+
 // Barycentric weights of the winning triangle at (px,py), recomputed with the
 // same staged arithmetic as the coverage loop.
 __device__ __forceinline__ Vec3 winner_bary(const int4 *__restrict__ screen,
