@@ -128,7 +128,8 @@ struct fgl_batch {
     bool have_shadow = false;
 
     // device
-    Buf d_views, d_draws[2], d_items[2], d_blk_view, d_pix_views;
+    Buf d_views, d_draws[2], d_items[2], d_blk_view, d_pix_views, d_conv_items[4];
+    uint32_t n_conv_items[4] = {0, 0, 0, 0};
     Buf depth, vis, gbuf, ao, fb, img, shadow, shadow_depth;
     Buf screen, rect, counts, offs, scan_tmp, tri_v, big_list, tile_start, tile_end;
     Buf keys[2], vals[2], keys_alt[2], vals_alt[2], sort_tmp[2];
@@ -158,6 +159,7 @@ static int plan(fgl_batch *b)
     std::vector<uint32_t> items[2];
     std::vector<uint8_t> blk_view;
     std::vector<BatchPixView> pix(b->n);
+    std::vector<uint2> conv_items[4];
     size_t px = 0, spx = 0;
     uint32_t tiles[2] = {0, 0}, blocks = 0;
     b->have_shadow = false;
@@ -201,6 +203,7 @@ static int plan(fgl_batch *b)
         pix[i].pad = 0;
         pix[i].px_off = dv.px_off;
         std::memcpy(pix[i].view, dv.u.view, sizeof(pix[i].view));
+        if (b->ao_mode == FGL_AO_NEURAL) nao_conv_items(i, v.width, v.height, conv_items);
         // draw records: light * model with this view's light space
         for (int k = 0; k < v.n_draws; ++k) {
             DevDraw d;
@@ -259,7 +262,17 @@ static int plan(fgl_batch *b)
         (!blk_view.empty() && hipMemcpy(b->d_blk_view.p, blk_view.data(), blk_view.size(), hipMemcpyHostToDevice)) ||
         hipMemcpy(b->d_pix_views.p, pix.data(), b->n * sizeof(BatchPixView), hipMemcpyHostToDevice))
         return bfail(b, FGL_ERR_DEVICE, "upload failed");
-    if (b->ao_mode == FGL_AO_NEURAL && (rc = nao_reserve(b->nao, px))) return bfail(b, rc, "neural AO buffers");
+    if (b->ao_mode == FGL_AO_NEURAL) {
+        if ((rc = nao_reserve(b->nao, px))) return bfail(b, rc, "neural AO buffers");
+        for (int l = 0; l < 4; ++l) {
+            b->n_conv_items[l] = (uint32_t)conv_items[l].size();
+            if (grow(b->d_conv_items[l], (conv_items[l].size() + 1) * sizeof(uint2)))
+                return bfail(b, FGL_ERR_NOMEM, "batch allocation failed");
+            if (!conv_items[l].empty() && hipMemcpy(b->d_conv_items[l].p, conv_items[l].data(),
+                                                    conv_items[l].size() * sizeof(uint2), hipMemcpyHostToDevice))
+                return bfail(b, FGL_ERR_DEVICE, "upload failed");
+        }
+    }
     b->measured = false;
     return FGL_OK;
 }
@@ -397,8 +410,15 @@ static int enqueue(fgl_batch *b, bool measure, bool timed)
     }
     if (b->ao_mode == FGL_AO_NEURAL) {
         CtxTimer t(b->ctx, timed ? "batch_neural_ao" : nullptr);
-        if ((rc = nao_infer_batch(b->nao, s, (const BatchPixView *)b->d_pix_views.p, pa.blk_view, b->n_blocks,
-                                  b->total_px, (const float *)b->depth.p, pa.gbuf, pa.ao)))
+        NaoBatchTables nt;
+        nt.views = (const BatchPixView *)b->d_pix_views.p;
+        nt.blk_view = pa.blk_view;
+        nt.n_blocks = b->n_blocks;
+        for (int l = 0; l < 4; ++l) {
+            nt.items[l] = (const uint2 *)b->d_conv_items[l].p;
+            nt.n_items[l] = b->n_conv_items[l];
+        }
+        if ((rc = nao_infer_batch(b->nao, s, nt, b->total_px, (const float *)b->depth.p, pa.gbuf, pa.ao)))
             return bfail(b, rc, "batched neural AO failed");
     }
     {
@@ -515,6 +535,7 @@ void fgl_batch_destroy(fgl_batch *b)
     if (!b) return;
     hipStreamSynchronize(ctx_stream(b->ctx));
     drop_graph(b);
+    for (int l = 0; l < 4; ++l) release(b->d_conv_items[l]);
     Buf *bufs[] = {&b->d_views, &b->d_draws[0], &b->d_draws[1], &b->d_items[0], &b->d_items[1], &b->d_blk_view,
                    &b->d_pix_views, &b->depth, &b->vis, &b->gbuf, &b->ao, &b->fb, &b->img, &b->shadow,
                    &b->shadow_depth, &b->screen, &b->rect, &b->counts, &b->offs, &b->scan_tmp, &b->tri_v,

@@ -4,6 +4,8 @@
 
 #include <hip/hip_runtime.h>
 
+#include <vector>
+
 #include "../../include/fgl.h"
 #include "fgl_kernels.hpp"
 
@@ -55,9 +57,18 @@ struct BatchPixView {
 // depth / G-buffer, four convolutions, head into the batch AO target.
 // nao_reserve allocates the batch-sized activations beforehand (the infer
 // call then allocates nothing, so it can be captured in a graph).
+struct NaoBatchTables {
+    const BatchPixView *views;
+    const uint8_t *blk_view;     // view of each 64 x 4 pixel block
+    uint32_t n_blocks;
+    const uint2 *items[4];       // convolution tiles of each layer (nao_conv_items)
+    uint32_t n_items[4];
+};
 int nao_reserve(fgl_nao *n, size_t total_px);
-int nao_infer_batch(fgl_nao *n, hipStream_t s, const BatchPixView *views, const uint8_t *blk_view,
-                    uint32_t n_blocks, size_t total_px, const float *depth, const float *gbuf, float *ao);
+// Appends the convolution tiles of a W x H view to the four per-layer lists.
+void nao_conv_items(int view, int W, int H, std::vector<uint2> items[4]);
+int nao_infer_batch(fgl_nao *n, hipStream_t s, const NaoBatchTables &t, size_t total_px, const float *depth,
+                    const float *gbuf, float *ao);
 
 // Scoped HIP-event timing of the kernels enqueued on the context stream
 // while it lives, reported by fgl_read_timings under `name` (a string

@@ -167,94 +167,245 @@ __device__ __forceinline__ void load_weights(ConvWeights<CIN> &cw, const uint16_
         }
 }
 
-// One wavefront: pixels xb .. xb+63 of row y of the image starting at pixel
-// `img` of `in` / `out` (W x H, NHWC); taps at (dy, dx) * dil.
-template <int CIN, int MODE>
-__device__ __forceinline__ void conv_row(const ConvWeights<CIN> &cw, const uint16_t *__restrict__ in,
-                                         const float *__restrict__ bias,
-                                         const uint16_t *__restrict__ act,
-                                         uint16_t *__restrict__ out, int W, int H, size_t img,
-                                         int y, int xb, int dil)
+// ---- phase-tiled convolution with an LDS input tile ----
+//
+// A 3x3 convolution dilated by d only connects pixels of one phase (x mod d,
+// y mod d): on that phase's grid it is a plain 3x3 convolution.  A workgroup
+// takes a 64 x 8 tile of one phase grid, stages the tile plus a one-pixel
+// halo (66 x 10 pixels, all channels) in LDS once -- every input pixel is
+// then read from HBM/L2 about 1.3 times instead of 9 -- and its four
+// wavefronts compute two rows each from LDS.  Pixels are padded to 80 bytes
+// in LDS so the 16-byte fragment reads of 16 consecutive pixels spread over
+// the banks.
+constexpr int kTileW = 64, kTileH = 8;
+constexpr int kStageW = kTileW + 2, kStageH = kTileH + 2;
+constexpr int kStagePix = 40;  // bf16 per staged pixel (32 channels + 8 pad)
+
+// Tile geometry: image base pixel, phase, tile column / row.
+struct ConvTile {
+    size_t img;
+    int W, H, phx, phy, bx, by;
+};
+
+template <int CIN>
+struct ConvStage {
+    static constexpr int CPP = CIN / 8;                                  // 16-byte chunks per pixel
+    static constexpr int N = (kStageH * kStageW * CPP + 255) / 256;      // chunks per thread
+    uint4 v[N];
+};
+
+// Global -> registers: this thread's chunks of the tile plus halo (zeros
+// outside the phase grid).
+template <int CIN>
+__device__ __forceinline__ void stage_load(ConvStage<CIN> &st, const uint16_t *__restrict__ in,
+                                           const ConvTile &t, int dil)
 {
-    constexpr int K = ConvWeights<CIN>::K, KC = ConvWeights<CIN>::KC;
-    const int lane = threadIdx.x & 63;
-    const int lr = lane & 15, lq = lane >> 4;
-    v4f acc[4][2];
+    constexpr int CPP = ConvStage<CIN>::CPP, TOT = kStageH * kStageW * CPP;
+    const int Wp = (t.W - t.phx + dil - 1) / dil, Hp = (t.H - t.phy + dil - 1) / dil;
+    const int i0 = t.bx * kTileW - 1, j0 = t.by * kTileH - 1;
 #pragma unroll
-    for (int pb = 0; pb < 4; ++pb)
-#pragma unroll
-        for (int cb = 0; cb < 2; ++cb) acc[pb][cb] = v4f{0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-    for (int c = 0; c < KC; ++c) {
-        const int k0 = c * 32 + lq * 8;
-        const int tap = k0 / CIN, ci0 = k0 % CIN;
-        const int yy = y + (tap / 3 - 1) * dil;
-        const bool rowok = k0 < K && yy >= 0 && yy < H;
-        v8bf b[4];
-#pragma unroll
-        for (int pb = 0; pb < 4; ++pb) {
-            const int xx = xb + pb * 16 + lr + (tap % 3 - 1) * dil;
-            b[pb] = v8bf{};
-            if (rowok && xx >= 0 && xx < W) b[pb] = *(const v8bf *)(in + (img + (size_t)yy * W + xx) * CIN + ci0);
-        }
-#pragma unroll
-        for (int pb = 0; pb < 4; ++pb) {
-            acc[pb][0] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(cw.w[c][0], b[pb], acc[pb][0], 0, 0, 0);
-            acc[pb][1] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(cw.w[c][1], b[pb], acc[pb][1], 0, 0, 0);
-        }
-    }
-    // D[co][px]: lane holds channels cb*16 + 4*lq .. +3 of pixel xb + pb*16 + lr
-#pragma unroll
-    for (int pb = 0; pb < 4; ++pb) {
-        const int px = xb + pb * 16 + lr;
-        if (px >= W) continue;
-#pragma unroll
-        for (int cb = 0; cb < 2; ++cb) {
-            const int co = cb * 16 + lq * 4;
-            const size_t o = (img + (size_t)y * W + px) * CH + co;
-            uint16_t r[4];
-            if (MODE == 0) {
-                const float4 bv = *(const float4 *)(bias + co);
-                const float bb[4] = {bv.x, bv.y, bv.z, bv.w};
-#pragma unroll
-                for (int i = 0; i < 4; ++i) {
-                    const float v = acc[pb][cb][i] + bb[i];
-                    r[i] = f2bf(v > 0.f ? v : 0.f);
-                }
-            } else {
-                const uint64_t am = *(const uint64_t *)(act + o);
-#pragma unroll
-                for (int i = 0; i < 4; ++i) {
-                    const uint16_t ai = (uint16_t)(am >> (16 * i));
-                    r[i] = f2bf(ai != 0 ? acc[pb][cb][i] : 0.f);
-                }
-            }
-            *(uint64_t *)(out + o) = (uint64_t)r[0] | ((uint64_t)r[1] << 16) | ((uint64_t)r[2] << 32) |
-                                     ((uint64_t)r[3] << 48);
+    for (int q = 0; q < ConvStage<CIN>::N; ++q) {
+        const int e = (int)threadIdx.x + 256 * q;
+        st.v[q] = make_uint4(0u, 0u, 0u, 0u);
+        if (e < TOT) {
+            const int pix = e / CPP, c = e % CPP;
+            const int i = i0 + pix % kStageW, j = j0 + pix / kStageW;
+            if (i >= 0 && i < Wp && j >= 0 && j < Hp)
+                st.v[q] = *(const uint4 *)(in + (t.img + (size_t)(t.phy + j * dil) * t.W + t.phx + i * dil) * CIN +
+                                           c * 8);
         }
     }
 }
 
-// Rows per wavefront: the weights are loaded once and used for kConvRows
-// rows.  Workgroup: 4 wavefronts x kConvRows rows of 64 pixels; wave w owns
-// rows 4 r + w of its workgroup's 4 * kConvRows rows.
-constexpr int kConvRows = 4;
-
-template <int CIN, int MODE>
-__global__ __launch_bounds__(256) void k_conv3x3(const uint16_t *__restrict__ in,
-                                                 const uint16_t *__restrict__ Wt,
-                                                 const float *__restrict__ bias,
-                                                 const uint16_t *__restrict__ act,
-                                                 uint16_t *__restrict__ out, int W, int H, int dil)
+template <int CIN>
+__device__ __forceinline__ void stage_store(const ConvStage<CIN> &st, uint16_t *s_in)
 {
+    constexpr int CPP = ConvStage<CIN>::CPP, TOT = kStageH * kStageW * CPP;
+#pragma unroll
+    for (int q = 0; q < ConvStage<CIN>::N; ++q) {
+        const int e = (int)threadIdx.x + 256 * q;
+        if (e < TOT) *(uint4 *)(s_in + (e / CPP) * kStagePix + (e % CPP) * 8) = st.v[q];
+    }
+}
+
+// The staged tile through the 3x3 taps: wave w computes rows 2w, 2w+1.
+template <int CIN, int MODE>
+__device__ __forceinline__ void conv_tile_compute(const ConvWeights<CIN> &cw, const uint16_t *s_in,
+                                                  const float *__restrict__ bias, const uint16_t *__restrict__ act,
+                                                  uint16_t *__restrict__ out, const ConvTile &t, int dil)
+{
+    constexpr int K = ConvWeights<CIN>::K, KC = ConvWeights<CIN>::KC;
+    const int Wp = (t.W - t.phx + dil - 1) / dil, Hp = (t.H - t.phy + dil - 1) / dil;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, lr = lane & 15, lq = lane >> 4;
+#pragma unroll 1
+    for (int r = 0; r < kTileH / 4; ++r) {
+        const int jr = wave * (kTileH / 4) + r, j = t.by * kTileH + jr;
+        if (j >= Hp) break;
+        v4f acc[4][2];
+#pragma unroll
+        for (int pb = 0; pb < 4; ++pb)
+#pragma unroll
+            for (int cb = 0; cb < 2; ++cb) acc[pb][cb] = v4f{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int c = 0; c < KC; ++c) {
+            const int k0 = c * 32 + lq * 8;
+            const int tap = imin(k0 / CIN, 8), ci0 = k0 % CIN;
+            const int srow = jr + tap / 3, scol = lr + tap % 3;
+            v8bf b[4];
+#pragma unroll
+            for (int pb = 0; pb < 4; ++pb) {
+                b[pb] = v8bf{};
+                if (k0 < K) b[pb] = *(const v8bf *)(s_in + (srow * kStageW + pb * 16 + scol) * kStagePix + ci0);
+            }
+#pragma unroll
+            for (int pb = 0; pb < 4; ++pb) {
+                acc[pb][0] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(cw.w[c][0], b[pb], acc[pb][0], 0, 0, 0);
+                acc[pb][1] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(cw.w[c][1], b[pb], acc[pb][1], 0, 0, 0);
+            }
+        }
+        const int y = t.phy + j * dil;
+#pragma unroll
+        for (int pb = 0; pb < 4; ++pb) {
+            const int i = t.bx * kTileW + pb * 16 + lr;
+            if (i >= Wp) continue;
+            const int x = t.phx + i * dil;
+#pragma unroll
+            for (int cb = 0; cb < 2; ++cb) {
+                const int co = cb * 16 + lq * 4;
+                const size_t o = (t.img + (size_t)y * t.W + x) * CH + co;
+                uint16_t rr[4];
+                if (MODE == 0) {
+                    const float4 bv = *(const float4 *)(bias + co);
+                    const float bb[4] = {bv.x, bv.y, bv.z, bv.w};
+#pragma unroll
+                    for (int q = 0; q < 4; ++q) {
+                        const float v = acc[pb][cb][q] + bb[q];
+                        rr[q] = f2bf(v > 0.f ? v : 0.f);
+                    }
+                } else {
+                    const uint64_t am = *(const uint64_t *)(act + o);
+#pragma unroll
+                    for (int q = 0; q < 4; ++q) rr[q] = f2bf((uint16_t)(am >> (16 * q)) != 0 ? acc[pb][cb][q] : 0.f);
+                }
+                *(uint64_t *)(out + o) = (uint64_t)rr[0] | ((uint64_t)rr[1] << 16) | ((uint64_t)rr[2] << 32) |
+                                         ((uint64_t)rr[3] << 48);
+            }
+        }
+    }
+}
+
+// Tile t of a single-image-set launch: tiles_x x tiles_y tiles per phase
+// grid, dil^2 phases per image.
+__device__ __forceinline__ ConvTile single_tile(int t, int W, int H, int dil, int tiles_x, int tiles_y)
+{
+    const int per_phase = tiles_x * tiles_y, per_img = per_phase * dil * dil;
+    const int n = t / per_img, rem = t % per_img, ph = rem / per_phase, k = rem % per_phase;
+    ConvTile ct;
+    ct.img = (size_t)n * H * W;
+    ct.W = W;
+    ct.H = H;
+    ct.phx = ph % dil;
+    ct.phy = ph / dil;
+    ct.bx = k % tiles_x;
+    ct.by = k / tiles_x;
+    return ct;
+}
+
+__device__ __forceinline__ ConvTile batch_tile(const BatchPixView *__restrict__ views, const uint2 *__restrict__ items,
+                                               int t, int dil)
+{
+    const uint2 it = items[t];
+    const BatchPixView &bv = views[it.x & 0xff];
+    const int ph = (int)(it.x >> 8);
+    ConvTile ct;
+    ct.img = bv.px_off;
+    ct.W = bv.W;
+    ct.H = bv.H;
+    ct.phx = ph % dil;
+    ct.phy = ph / dil;
+    ct.bx = (int)(it.y & 0xffff);
+    ct.by = (int)(it.y >> 16);
+    return ct;
+}
+
+// Persistent workgroups stride over the tiles; the next tile's global reads
+// are in flight while the current tile is computed from LDS.
+template <int CIN, int MODE, bool BATCH>
+__global__ __launch_bounds__(256) void k_conv3x3_tiled(const uint16_t *__restrict__ in,
+                                                       const uint16_t *__restrict__ Wt,
+                                                       const float *__restrict__ bias,
+                                                       const uint16_t *__restrict__ act,
+                                                       uint16_t *__restrict__ out, int W, int H, int dil,
+                                                       int n_tiles, int tiles_x, int tiles_y,
+                                                       const BatchPixView *__restrict__ views,
+                                                       const uint2 *__restrict__ items)
+{
+    __shared__ __attribute__((aligned(16))) uint16_t s_in[kStageH * kStageW * kStagePix];
     ConvWeights<CIN> cw;
     load_weights<CIN>(cw, Wt);
-    const size_t img = (size_t)blockIdx.z * H * W;
-    for (int r = 0; r < kConvRows; ++r) {
-        const int y = (blockIdx.y * kConvRows + r) * 4 + (threadIdx.x >> 6);
-        if (y >= H) break;
-        conv_row<CIN, MODE>(cw, in, bias, act, out, W, H, img, y, blockIdx.x * 64, dil);
+    int t = blockIdx.x;
+    if (t >= n_tiles) return;
+    ConvTile cur = BATCH ? batch_tile(views, items, t, dil) : single_tile(t, W, H, dil, tiles_x, tiles_y);
+    ConvStage<CIN> st;
+    stage_load<CIN>(st, in, cur, dil);
+    for (;;) {
+        __syncthreads();   // the previous tile is no longer read from LDS
+        stage_store<CIN>(st, s_in);
+        __syncthreads();
+        const int nt = t + (int)gridDim.x;
+        ConvTile nxt = cur;
+        if (nt < n_tiles) {
+            nxt = BATCH ? batch_tile(views, items, nt, dil) : single_tile(nt, W, H, dil, tiles_x, tiles_y);
+            stage_load<CIN>(st, in, nxt, dil);
+        }
+        conv_tile_compute<CIN, MODE>(cw, s_in, bias, act, out, cur, dil);
+        if (nt >= n_tiles) break;
+        t = nt;
+        cur = nxt;
     }
+}
+
+static inline void conv_tiles(int W, int H, int N, int dil, int &tx, int &ty, int &n)
+{
+    const int Wp = (W + dil - 1) / dil, Hp = (H + dil - 1) / dil;
+    tx = (Wp + kTileW - 1) / kTileW;
+    ty = (Hp + kTileH - 1) / kTileH;
+    n = tx * ty * dil * dil * N;
+}
+
+// Persistent grid: two workgroups per compute unit (the LDS tile and the
+// registers of the weights and the prefetch allow two).
+static int g_conv_wgs = 0;
+
+static inline unsigned conv_workgroups(int n_tiles)
+{
+    if (g_conv_wgs == 0) {
+        int dev = 0, cu = 0;
+        hipGetDevice(&dev);
+        if (hipDeviceGetAttribute(&cu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cu < 1) cu = 256;
+        g_conv_wgs = 2 * cu;
+    }
+    return (unsigned)(n_tiles < g_conv_wgs ? n_tiles : g_conv_wgs);
+}
+
+template <int CIN, int MODE>
+static void launch_conv(hipStream_t s, const uint16_t *in, const uint16_t *Wt, const float *bias,
+                        const uint16_t *act, uint16_t *out, int W, int H, int N, int dil)
+{
+    int tx, ty, n;
+    conv_tiles(W, H, N, dil, tx, ty, n);
+    if (n == 0) return;
+    hipLaunchKernelGGL((k_conv3x3_tiled<CIN, MODE, false>), dim3(conv_workgroups(n)), dim3(256), 0, s, in, Wt, bias,
+                       act, out, W, H, dil, n, tx, ty, (const BatchPixView *)nullptr, (const uint2 *)nullptr);
+}
+
+template <int CIN>
+static void launch_conv_batch(hipStream_t s, const BatchPixView *views, const uint2 *items, int n, const uint16_t *in,
+                              const uint16_t *Wt, const float *bias, uint16_t *out, int dil)
+{
+    if (n == 0) return;
+    hipLaunchKernelGGL((k_conv3x3_tiled<CIN, 0, true>), dim3(conv_workgroups(n)), dim3(256), 0, s, in, Wt, bias,
+                       (const uint16_t *)nullptr, out, 0, 0, dil, n, 0, 0, views, items);
 }
 
 // Batched views: 64 x 4 pixel block b of the batch belongs to view
@@ -276,27 +427,6 @@ __device__ __forceinline__ bool batch_block(const BatchPixView *__restrict__ vie
                                             int &y)
 {
     return batch_block_at(views, blk_view, blockIdx.x, v, x, y);
-}
-
-// Workgroup g covers batch blocks g * kConvRows .. + kConvRows - 1.
-template <int CIN>
-__global__ __launch_bounds__(256) void k_conv3x3_batch(const BatchPixView *__restrict__ views,
-                                                       const uint8_t *__restrict__ blk_view, uint32_t n_blocks,
-                                                       const uint16_t *__restrict__ in,
-                                                       const uint16_t *__restrict__ Wt,
-                                                       const float *__restrict__ bias,
-                                                       uint16_t *__restrict__ out, int dil)
-{
-    ConvWeights<CIN> cw;
-    load_weights<CIN>(cw, Wt);
-    for (int r = 0; r < kConvRows; ++r) {
-        const uint32_t blk = blockIdx.x * kConvRows + r;
-        if (blk >= n_blocks) break;
-        int v, x, y;
-        if (!batch_block_at(views, blk_view, blk, v, x, y)) continue;
-        const BatchPixView &bv = views[v];
-        conv_row<CIN, 0>(cw, in, bias, nullptr, out, bv.W, bv.H, bv.px_off, y, x - (threadIdx.x & 63), dil);
-    }
 }
 
 // Weight gradient: dW[co][tap*CIN+ci] = sum_p dout[p][co] * in[p+off(tap)][ci]
@@ -600,26 +730,38 @@ int nao_reserve(fgl_nao *n, size_t px)
     return FGL_OK;
 }
 
-int nao_infer_batch(fgl_nao *n, hipStream_t s, const BatchPixView *views, const uint8_t *blk_view,
-                    uint32_t n_blocks, size_t total_px, const float *depth, const float *gbuf, float *ao)
+void nao_conv_items(int view, int W, int H, std::vector<uint2> items[4])
+{
+    for (int l = 0; l < 4; ++l) {
+        const int d = layer_dil(l);
+        int tx, ty, n;
+        conv_tiles(W, H, 1, d, tx, ty, n);
+        for (int ph = 0; ph < d * d; ++ph)
+            for (int by = 0; by < ty; ++by)
+                for (int bx = 0; bx < tx; ++bx)
+                    items[l].push_back(make_uint2((uint32_t)view | ((uint32_t)ph << 8), (uint32_t)bx | ((uint32_t)by << 16)));
+    }
+}
+
+int nao_infer_batch(fgl_nao *n, hipStream_t s, const NaoBatchTables &t, size_t total_px, const float *depth,
+                    const float *gbuf, float *ao)
 {
     if (total_px > n->icap) return FGL_ERR_INVALID;
-    if (n_blocks == 0) return FGL_OK;
-    const dim3 g(n_blocks), b(256);
-    hipLaunchKernelGGL(k_features_batch, g, b, 0, s, views, blk_view, depth, gbuf, n->ifeat, n->imask);
-    const dim3 gc((n_blocks + kConvRows - 1) / kConvRows);
-    hipLaunchKernelGGL((k_conv3x3_batch<C0>), gc, b, 0, s, views, blk_view, n_blocks, (const uint16_t *)n->ifeat,
-                       n->w1, n->params + OFF_B1, n->ia, layer_dil(0));
+    if (t.n_blocks == 0) return FGL_OK;
+    const dim3 g(t.n_blocks), b(256);
+    hipLaunchKernelGGL(k_features_batch, g, b, 0, s, t.views, t.blk_view, depth, gbuf, n->ifeat, n->imask);
+    launch_conv_batch<C0>(s, t.views, t.items[0], (int)t.n_items[0], (const uint16_t *)n->ifeat, n->w1,
+                          n->params + OFF_B1, n->ia, layer_dil(0));
     const int boffs[3] = {OFF_B2, OFF_B3, OFF_B4};
     uint16_t *src = n->ia, *dst = n->ib;
     for (int l = 0; l < 3; ++l) {
-        hipLaunchKernelGGL((k_conv3x3_batch<CH>), gc, b, 0, s, views, blk_view, n_blocks, (const uint16_t *)src,
-                           n->w234 + (size_t)l * CH * K2, n->params + boffs[l], dst, layer_dil(l + 1));
-        uint16_t *t = src;
+        launch_conv_batch<CH>(s, t.views, t.items[l + 1], (int)t.n_items[l + 1], (const uint16_t *)src,
+                              n->w234 + (size_t)l * CH * K2, n->params + boffs[l], dst, layer_dil(l + 1));
+        uint16_t *tmp = src;
         src = dst;
-        dst = t;
+        dst = tmp;
     }
-    hipLaunchKernelGGL(k_head_batch, g, b, 0, s, views, blk_view, n->params, (const uint16_t *)src, n->imask, ao);
+    hipLaunchKernelGGL(k_head_batch, g, b, 0, s, t.views, t.blk_view, n->params, (const uint16_t *)src, n->imask, ao);
     return hipGetLastError() == hipSuccess ? FGL_OK : FGL_ERR_DEVICE;
 }
 
@@ -754,13 +896,11 @@ int fgl_nao_stage_crops(fgl_nao *n, int slot0, int count, const int *xy)
 static void forward_convs(fgl_nao *n, hipStream_t s, const uint16_t *feat, uint16_t **acts, int W, int H,
                           int N)
 {
-    dim3 grid((W + 63) / 64, (H + 4 * kConvRows - 1) / (4 * kConvRows), N);
-    hipLaunchKernelGGL((k_conv3x3<C0, 0>), grid, dim3(256), 0, s, feat, n->w1, n->params + OFF_B1,
-                       (const uint16_t *)nullptr, acts[0], W, H, layer_dil(0));
+    launch_conv<C0, 0>(s, feat, n->w1, n->params + OFF_B1, nullptr, acts[0], W, H, N, layer_dil(0));
     const int boffs[3] = {OFF_B2, OFF_B3, OFF_B4};
     for (int l = 0; l < 3; ++l)
-        hipLaunchKernelGGL((k_conv3x3<CH, 0>), grid, dim3(256), 0, s, acts[l], n->w234 + (size_t)l * CH * K2,
-                           n->params + boffs[l], (const uint16_t *)nullptr, acts[l + 1], W, H, layer_dil(l + 1));
+        launch_conv<CH, 0>(s, acts[l], n->w234 + (size_t)l * CH * K2, n->params + boffs[l], nullptr, acts[l + 1], W, H,
+                           N, layer_dil(l + 1));
 }
 
 static int run_batch(fgl_nao *n, bool update, float lr, float *stats)
@@ -785,7 +925,6 @@ static int run_batch(fgl_nao *n, bool update, float lr, float *stats)
     const dim3 wg(S / 64, S / WG_ROWS, N);
     const int nblk = (int)(wg.x * wg.y * wg.z);
     const int woffs[4] = {OFF_W1, OFF_W2, OFF_W3, OFF_W4}, boffs[4] = {OFF_B1, OFF_B2, OFF_B3, OFF_B4};
-    const dim3 cgrid((S + 63) / 64, (S + 4 * kConvRows - 1) / (4 * kConvRows), N);
     for (int l = 3; l >= 0; --l) {
         const int K = l == 0 ? K1 : K2;
         if (l == 0)
@@ -798,9 +937,8 @@ static int run_batch(fgl_nao *n, bool update, float lr, float *stats)
                            n->grads + woffs[l]);
         hipLaunchKernelGGL(k_reduce_cols, dim3(1), dim3(256), 0, s, n->part_db, nblk, CH, n->grads + boffs[l]);
         if (l > 0)
-            hipLaunchKernelGGL((k_conv3x3<CH, 1>), cgrid, dim3(256), 0, s, n->d[l],
-                               n->wt234 + (size_t)(l - 1) * CH * K2, (const float *)nullptr, n->a[l - 1],
-                               n->d[l - 1], S, S, layer_dil(l));
+            launch_conv<CH, 1>(s, n->d[l], n->wt234 + (size_t)(l - 1) * CH * K2, nullptr, n->a[l - 1], n->d[l - 1], S,
+                               S, N, layer_dil(l));
     }
     if (update) {
         n->step += 1;
