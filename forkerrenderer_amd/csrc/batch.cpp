@@ -128,7 +128,8 @@ struct fgl_batch {
     bool have_shadow = false;
 
     // device
-    Buf d_views, d_draws[2], d_items[2], d_blk_view, d_pix_views, d_conv_items[4];
+    Buf d_views, d_draws[2], d_items[2], d_blk_view, d_pix_views, d_conv_items[4], d_vblocks[2], d_fblocks[2];
+    int n_vblocks[2] = {0, 0}, n_fblocks[2] = {0, 0};
     uint32_t n_conv_items[4] = {0, 0, 0, 0};
     Buf depth, vis, gbuf, ao, fb, img, shadow, shadow_depth;
     Buf screen, rect, counts, offs, scan_tmp, tri_v, big_list, tile_start, tile_end;
@@ -248,6 +249,18 @@ static int plan(fgl_batch *b)
         grow(b->tile_end, (size_t)mtiles * 4) || grow(b->counters, 256))
         return bfail(b, FGL_ERR_NOMEM, "batch allocation failed");
     for (int p = 0; p < 2; ++p) {
+        std::vector<uint2> vb, fb;
+        for (size_t k = 0; k < dd[p].size(); ++k) {
+            for (int v0 = 0; v0 < dd[p][k].n_pos; v0 += 256) vb.push_back(make_uint2((uint32_t)k, (uint32_t)v0));
+            for (int f0 = 0; f0 < dd[p][k].n_faces; f0 += 256) fb.push_back(make_uint2((uint32_t)k, (uint32_t)f0));
+        }
+        b->n_vblocks[p] = (int)vb.size();
+        b->n_fblocks[p] = (int)fb.size();
+        if (grow(b->d_vblocks[p], (vb.size() + 1) * sizeof(uint2)) || grow(b->d_fblocks[p], (fb.size() + 1) * sizeof(uint2)))
+            return bfail(b, FGL_ERR_NOMEM, "batch allocation failed");
+        if ((!vb.empty() && hipMemcpy(b->d_vblocks[p].p, vb.data(), vb.size() * sizeof(uint2), hipMemcpyHostToDevice)) ||
+            (!fb.empty() && hipMemcpy(b->d_fblocks[p].p, fb.data(), fb.size() * sizeof(uint2), hipMemcpyHostToDevice)))
+            return bfail(b, FGL_ERR_DEVICE, "upload failed");
         if (grow(b->d_draws[p], (dd[p].size() + 1) * sizeof(DevDraw)) ||
             grow(b->d_items[p], (items[p].size() + 1) * 4))
             return bfail(b, FGL_ERR_NOMEM, "batch allocation failed");
@@ -312,9 +325,11 @@ static int enqueue(fgl_batch *b, bool measure, bool timed)
         {
             CtxTimer t(b->ctx, timed ? (p == kBvShadow ? "batch_shadow_binning" : "batch_main_binning") : nullptr);
             if (nt > 0) {
-                launch_bv_vertex(p, draws, views, b->n_draws[p], b->max_pos[p], (int4 *)b->screen.p, s);
+                launch_bv_vertex(p, draws, views, (const uint2 *)b->d_vblocks[p].p, b->n_vblocks[p], (int4 *)b->screen.p,
+                                 s);
                 launch_fill_u32(ctr, 0, 1, s);
-                launch_bv_setup(p, draws, views, b->n_draws[p], b->max_faces[p], (const int4 *)b->screen.p,
+                launch_bv_setup(p, draws, views, (const uint2 *)b->d_fblocks[p].p, b->n_fblocks[p],
+                                (const int4 *)b->screen.p,
                                 b->TS[p], (uint2 *)b->rect.p, (uint32_t *)b->counts.p, (int4 *)b->tri_v.p,
                                 (uint32_t *)b->big_list.p, ctr, s);
                 launch_exclusive_scan((const uint32_t *)b->counts.p, offs, (size_t)nt, (uint32_t *)b->scan_tmp.p, s);
@@ -536,6 +551,10 @@ void fgl_batch_destroy(fgl_batch *b)
     hipStreamSynchronize(ctx_stream(b->ctx));
     drop_graph(b);
     for (int l = 0; l < 4; ++l) release(b->d_conv_items[l]);
+    for (int p = 0; p < 2; ++p) {
+        release(b->d_vblocks[p]);
+        release(b->d_fblocks[p]);
+    }
     Buf *bufs[] = {&b->d_views, &b->d_draws[0], &b->d_draws[1], &b->d_items[0], &b->d_items[1], &b->d_blk_view,
                    &b->d_pix_views, &b->depth, &b->vis, &b->gbuf, &b->ao, &b->fb, &b->img, &b->shadow,
                    &b->shadow_depth, &b->screen, &b->rect, &b->counts, &b->offs, &b->scan_tmp, &b->tri_v,

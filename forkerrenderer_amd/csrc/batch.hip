@@ -16,13 +16,19 @@
 
 namespace fgl {
 
+// Vertex and setup grids are flat lists of 256-item blocks: block b covers
+// items blocks[b].y .. + 255 of draw blocks[b].x (draws differ in size by
+// orders of magnitude in a batch, so a draws x largest-draw grid would be
+// mostly empty).
 template <int PASS>
 __global__ __launch_bounds__(256) void k_bv_vertex(const DevDraw *__restrict__ draws,
                                                    const DevView *__restrict__ views,
+                                                   const uint2 *__restrict__ blocks,
                                                    int4 *__restrict__ out)
 {
-    const DevDraw &dr = draws[blockIdx.y];
-    const int vi = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint2 bk = blocks[blockIdx.x];
+    const DevDraw &dr = draws[bk.x];
+    const int vi = (int)bk.y + threadIdx.x;
     if (vi >= dr.n_pos) return;
     const FrameUniforms &u = views[dr.view].u;
     if (PASS == kBvShadow) out[dr.vert_base + vi] = to_screen(u.shadow_viewport, depth_ndc(dr, vi));
@@ -32,6 +38,7 @@ __global__ __launch_bounds__(256) void k_bv_vertex(const DevDraw *__restrict__ d
 template <int PASS>
 __global__ __launch_bounds__(256) void k_bv_setup(const DevDraw *__restrict__ draws,
                                                   const DevView *__restrict__ views,
+                                                  const uint2 *__restrict__ blocks,
                                                   const int4 *__restrict__ screen, int TS,
                                                   uint2 *__restrict__ rect,
                                                   uint32_t *__restrict__ count,
@@ -39,9 +46,10 @@ __global__ __launch_bounds__(256) void k_bv_setup(const DevDraw *__restrict__ dr
                                                   uint32_t *__restrict__ big_list,
                                                   uint32_t *__restrict__ big_count)
 {
-    const int d = blockIdx.y;
+    const uint2 bk = blocks[blockIdx.x];
+    const int d = (int)bk.x;
     const DevDraw &dr = draws[d];
-    const int f = blockIdx.x * blockDim.x + threadIdx.x;
+    const int f = (int)bk.y + threadIdx.x;
     if (f >= dr.n_faces) return;
     const DevView &v = views[dr.view];
     const int W = PASS == kBvShadow ? v.S : v.W, H = PASS == kBvShadow ? v.S : v.H;
@@ -180,27 +188,26 @@ __global__ __launch_bounds__(256) void k_bv_resolve(BvPixelArgs b)
 
 static inline unsigned nblk(size_t n, unsigned b) { return (unsigned)((n + b - 1) / b); }
 
-void launch_bv_vertex(int pass, const DevDraw *draws, const DevView *views, int n_draws, int max_pos,
+void launch_bv_vertex(int pass, const DevDraw *draws, const DevView *views, const uint2 *blocks, int n_blocks,
                       int4 *screen, hipStream_t s)
 {
-    if (n_draws <= 0 || max_pos <= 0) return;
-    const dim3 g(nblk(max_pos, 256), n_draws);
-    if (pass == kBvShadow) hipLaunchKernelGGL(k_bv_vertex<kBvShadow>, g, dim3(256), 0, s, draws, views, screen);
-    else hipLaunchKernelGGL(k_bv_vertex<kBvMain>, g, dim3(256), 0, s, draws, views, screen);
+    if (n_blocks <= 0) return;
+    if (pass == kBvShadow)
+        hipLaunchKernelGGL(k_bv_vertex<kBvShadow>, dim3(n_blocks), dim3(256), 0, s, draws, views, blocks, screen);
+    else hipLaunchKernelGGL(k_bv_vertex<kBvMain>, dim3(n_blocks), dim3(256), 0, s, draws, views, blocks, screen);
 }
 
-void launch_bv_setup(int pass, const DevDraw *draws, const DevView *views, int n_draws, int max_faces,
+void launch_bv_setup(int pass, const DevDraw *draws, const DevView *views, const uint2 *blocks, int n_blocks,
                      const int4 *screen, int TS, uint2 *rect, uint32_t *count, int4 *tri_v,
                      uint32_t *big_list, uint32_t *big_count, hipStream_t s)
 {
-    if (n_draws <= 0 || max_faces <= 0) return;
-    const dim3 g(nblk(max_faces, 256), n_draws);
+    if (n_blocks <= 0) return;
     if (pass == kBvShadow)
-        hipLaunchKernelGGL(k_bv_setup<kBvShadow>, g, dim3(256), 0, s, draws, views, screen, TS, rect, count,
-                           tri_v, big_list, big_count);
+        hipLaunchKernelGGL(k_bv_setup<kBvShadow>, dim3(n_blocks), dim3(256), 0, s, draws, views, blocks, screen, TS,
+                           rect, count, tri_v, big_list, big_count);
     else
-        hipLaunchKernelGGL(k_bv_setup<kBvMain>, g, dim3(256), 0, s, draws, views, screen, TS, rect, count,
-                           tri_v, big_list, big_count);
+        hipLaunchKernelGGL(k_bv_setup<kBvMain>, dim3(n_blocks), dim3(256), 0, s, draws, views, blocks, screen, TS,
+                           rect, count, tri_v, big_list, big_count);
 }
 
 void launch_bv_emit(int pass, const BvBinArgs &a, hipStream_t s)

@@ -69,9 +69,10 @@ struct BvPixelArgs {
     uint8_t *img;
 };
 
-void launch_bv_vertex(int pass, const DevDraw *draws, const DevView *views, int n_draws, int max_pos,
+// blocks: (draw, first vertex / face) of each 256-item block
+void launch_bv_vertex(int pass, const DevDraw *draws, const DevView *views, const uint2 *blocks, int n_blocks,
                       int4 *screen, hipStream_t s);
-void launch_bv_setup(int pass, const DevDraw *draws, const DevView *views, int n_draws, int max_faces,
+void launch_bv_setup(int pass, const DevDraw *draws, const DevView *views, const uint2 *blocks, int n_blocks,
                      const int4 *screen, int TS, uint2 *rect, uint32_t *count, int4 *tri_v,
                      uint32_t *big_list, uint32_t *big_count, hipStream_t s);
 void launch_bv_emit(int pass, const BvBinArgs &a, hipStream_t s);
