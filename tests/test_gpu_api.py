@@ -91,3 +91,37 @@ def test_forward_config1_full_size(ctx, scene_dir):
     d = np.abs(got["image"].astype(int) - want["image"].astype(int))
     print("c1 image: max diff", d.max(), "fraction within 1 LSB", (d <= 1).mean())
     assert (d <= 1).mean() >= 0.999
+
+
+def test_nan_initial_depth_tiles_fall_back_to_the_sequential_fold(ctx, scene_dir):
+    """The reference's fold `replace when !(d >= best)` lets any fragment
+    replace a NaN depth -- not a minimum -- so a tile whose depth buffer holds
+    a NaN re-runs in submission order.  Depth written with scattered NaNs,
+    then a geometry pass over it: covered pixels equal the fresh render
+    (and the oracle), uncovered ones keep what was written."""
+    d = os.path.join(scene_dir, "nan")
+    p = G.tiny_scene(d, seed=24, width=64, height=48, subdiv=2, tex=16)
+    sc = F.Scene(p)
+    I = sc.info
+    W, H = I.width, I.height
+    want = oracle_frame(sc)
+    fresh = gpu_frame(ctx, sc)          # also leaves the scene's matrices and lights set
+    assert_bitexact("fresh depth", fresh["depth"], want["depth"])
+    ids = ctx.upload_scene(sc)
+    init = np.full(W * H, np.finfo(np.float32).max, np.float32)
+    init[::7] = np.nan
+    covered = want["depth"] < 3e38
+    assert (covered & np.isnan(init)).sum() > 50
+    for tile in (8, 16, 32):
+        ctx.set_tile_size(tile)
+        ctx.init_targets(W, H)
+        ctx.write(F.TARGET_DEPTH, init)
+        ctx.begin_pass(F.PASS_GEOMETRY)
+        for mid, M in ids:
+            ctx.draw_model(mid, M)
+        ctx.end_pass()
+        depth = ctx.read(F.TARGET_DEPTH)
+        gbuf = ctx.read(F.TARGET_GBUFFER).reshape(F.GBUF_PLANES, -1)
+        assert_bitexact(f"covered depth, tile {tile}", depth[covered], want["depth"][covered])
+        assert_bitexact(f"uncovered depth, tile {tile}", depth[~covered], init[~covered])
+        assert_bitexact(f"covered gbuf, tile {tile}", gbuf[:, covered], fresh["gbuf"][:, covered])
