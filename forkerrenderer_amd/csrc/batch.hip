@@ -129,7 +129,7 @@ __global__ __launch_bounds__(BLOCK) void k_bv_raster(BvRasterArgs b)
         a.depth = SHADOW ? b.shadow_depth + v.spx_off : b.depth + v.px_off;
         a.vis = b.vis + v.px_off;
         a.shadow_out = b.shadow + v.spx_off;
-        raster_tile_fast<TS, BLOCK, SHADOW>(a, tile, lds);
+        raster_tile<TS, BLOCK, SHADOW>(a, tile, lds);
     }
 }
 

@@ -9,7 +9,7 @@
 //                 off-screen cull, screen-tile rectangle and count, and the
 //                 triangle's pass-wide vertex ids
 //   scan + emit   (tile, triangle) pairs in triangle order
-//   stable radix sort by tile (sort.hip) -> per-tile lists in submission order
+//   radix sort by tile (sort.hip) -> per-tile triangle lists
 //   k_tile_ranges per-tile [start, end) in the sorted list
 //   k_raster      one workgroup per tile (raster_tile)
 //   resolve       one thread per pixel: the pass's fragment program for the
@@ -110,7 +110,7 @@ __global__ __launch_bounds__(BLOCK) void k_raster(RasterArgs a)
 {
     __shared__ RasterLds<TS, BLOCK> lds;
     const int tile = xcd_remap(blockIdx.x, a.tiles_x * a.tiles_y);
-    raster_tile_fast<TS, BLOCK, SHADOW>(a, tile, lds);
+    raster_tile<TS, BLOCK, SHADOW>(a, tile, lds);
 }
 
 __global__ __launch_bounds__(256) void k_gbuffer_resolve(ResolveArgs a, FrameUniforms u)

@@ -162,130 +162,26 @@ __device__ __forceinline__ int xcd_remap(int bid, int n)
     return start + local;
 }
 
-// One TS x TS tile of one view.  With 4 wavefronts each owns a quadrant of
-// QS = TS/2; with one wavefront it owns the tile.  Lane pixel k of a
-// wavefront: lin = lane + 64k, (lin % QS, lin / QS) inside the quadrant.
-// Chunks of the tile's triangle list are staged in LDS; each wavefront keeps,
-// by ballot, the triangles whose bbox meets its quadrant and walks them in
-// list order; each lane folds the depth test of src/forkergl.cpp:180-199 for
-// its pixels in submission order (no atomics) and records the winner.
-// `a.tile_start` / `a.tile_end` are indexed by the view-local tile.
-template <int TS, int BLOCK, bool SHADOW>
-__device__ __forceinline__ void raster_tile(const RasterArgs &a, int tile, TriL *lds)
-{
-    constexpr int NW = BLOCK / 64;
-    constexpr int QS = (NW == 1) ? TS : TS / 2;
-    constexpr int NPX = QS * QS / 64;
-    const int tx = tile % a.tiles_x, ty = tile / a.tiles_x;
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int qx0 = tx * TS + (NW == 1 ? 0 : (wave & 1) * QS);
-    const int qy0 = ty * TS + (NW == 1 ? 0 : (wave >> 1) * QS);
-    int px[NPX], py[NPX], win[NPX];
-    float best[NPX];
-    bool act[NPX];
-#pragma unroll
-    for (int k = 0; k < NPX; ++k) {
-        const int lin = lane + 64 * k;
-        px[k] = qx0 + lin % QS;
-        py[k] = qy0 + lin / QS;
-        act[k] = px[k] < a.W && py[k] < a.H;
-        win[k] = -1;
-        best[k] = (act[k] && !a.fresh) ? a.depth[(size_t)px[k] + (size_t)py[k] * a.W] : 3.40282347e+38f;
-    }
-    const int rx1 = qx0 + QS - 1, ry1 = qy0 + QS - 1;
-    const uint32_t start = a.tile_start[tile], end = a.tile_end[tile];
-    for (uint32_t base = start; base < end; base += BLOCK) {
-        const int n = imin(BLOCK, (int)(end - base));
-        if (tid < n) {
-            const int g = (int)a.vals[base + tid];
-            const int4 tv = a.tri_v[g];
-            stage_tri(lds[tid], g, a.screen[tv.x], a.screen[tv.y], a.screen[tv.z], a.W, a.H);
-        }
-        __syncthreads();
-        // Ballot per 64 staged triangles: does its bbox meet this quadrant?
-        unsigned long long mask[NW];
-#pragma unroll
-        for (int j = 0; j < NW; ++j) {
-            const int s = j * 64 + lane;
-            bool hit = false;
-            if (s < n) {
-                const TriL &t = lds[s];
-                const int minx = t.bb0 & 0xffff, miny = t.bb0 >> 16;
-                const int maxx = t.bb1 & 0xffff, maxy = t.bb1 >> 16;
-                hit = !(maxx < qx0 || minx > rx1 || maxy < qy0 || miny > ry1);
-            }
-            mask[j] = __ballot(hit);
-        }
-#pragma unroll
-        for (int j = 0; j < NW; ++j) {
-            unsigned long long m = mask[j];
-            while (m) {
-                const int i = j * 64 + __builtin_ctzll(m);
-                m &= m - 1;
-                const TriL &t = lds[i];
-                const int minx = t.bb0 & 0xffff, miny = t.bb0 >> 16;
-                const int maxx = t.bb1 & 0xffff, maxy = t.bb1 >> 16;
-#pragma unroll
-                for (int k = 0; k < NPX; ++k) {
-                    if (act[k] && px[k] >= minx && px[k] <= maxx && py[k] >= miny && py[k] <= maxy) {
-                        float b0, b1, b2;
-                        if (cover(t, px[k], py[k], b0, b1, b2)) {
-                            // Dot(bary, depths) and the test of forkergl.cpp:180-199
-                            float dd = 0.f;
-                            dd += b0 * t.z0;
-                            dd += b1 * t.z1;
-                            dd += b2 * t.z2;
-                            if (!(dd >= best[k])) {
-                                best[k] = dd;
-                                win[k] = t.g;
-                            }
-                        }
-                    }
-                }
-            }
-        }
-        __syncthreads();
-    }
-#pragma unroll
-    for (int k = 0; k < NPX; ++k) {
-        if (!act[k]) continue;
-        const size_t i = (size_t)px[k] + (size_t)py[k] * a.W;
-        if (win[k] >= 0 || a.fresh) a.depth[i] = best[k];
-        if (!SHADOW) {
-            a.vis[i] = win[k];
-        } else if (win[k] >= 0) {
-            // DepthShader::ProcessFragment (depthshader.h:30-36)
-            const int4 tv = a.tri_v[win[k]];
-            const DevDraw &dr = a.draws[tv.w];
-            TriL t;
-            stage_tri(t, win[k], a.screen[tv.x], a.screen[tv.y], a.screen[tv.z], a.W, a.H);
-            float b0 = 0.f, b1 = 0.f, b2 = 0.f;
-            cover(t, px[k], py[k], b0, b1, b2);
-            const int f = win[k] - dr.tri_base;
-            Vec4 n0 = depth_ndc(dr, dr.vi[3 * f]);
-            Vec4 n1 = depth_ndc(dr, dr.vi[3 * f + 1]);
-            Vec4 n2 = depth_ndc(dr, dr.vi[3 * f + 2]);
-            float z = dot3(vec3(n0.z, n1.z, n2.z), vec3(b0, b1, b2));
-            a.shadow_out[i] = z * 0.5f + 0.5f;
-        } else if (a.fresh) {
-            a.shadow_out[i] = 0.f;
-        }
-    }
-}
-
 // ---------------- order-independent tile rasteriser ----------------
 //
-// The submission-order depth fold of src/forkergl.cpp:180-199 (replace when
-// !(d >= best)) picks, for every pixel, the first triangle -- lowest
-// pass-wide id g -- among those of minimum depth, as long as no depth is NaN
-// (with -0 and +0 equal).  That makes it a minimum over 64-bit keys
-// (order(depth) << 32 | g + 1) and lets any lane fold any triangle into any
-// pixel with an LDS atomicMin.  Triangles whose bbox covers at most
-// kSmallTriArea pixels of the tile are rasterised one per lane (the common
-// case for dense meshes, where a wave-per-triangle walk would leave most
-// lanes idle); larger ones one per wave, a lane per pixel.  A NaN depth
-// anywhere in the tile (initial or computed) re-runs the tile with the
-// sequential fold above.
+// The submission-order depth fold of src/forkergl.cpp:180-199 -- replace when
+// !(d >= best), fragments in increasing pass-wide id g -- is reproduced
+// without any order:
+//  * no NaN involved: the winner is the lowest g among fragments of minimum
+//    depth (-0 == +0), or the initial value if it is not beaten: a minimum
+//    over 64-bit keys (order(depth) << 32 | g + 1), initial value g + 1 = 0;
+//  * a NaN fragment always replaces, and whatever follows a NaN replaces it:
+//    with J the highest g of a NaN fragment at the pixel, the winner is the
+//    key minimum over the fragments with g > J (the initial value excluded),
+//    or J itself when there are none;
+//  * a NaN initial depth with no NaN fragment: the key minimum over all
+//    fragments (the initial value excluded), or the initial value.
+// So any lane can fold any triangle into any pixel with LDS atomics, and the
+// tile's triangle list needs no particular order.  Triangles whose bbox
+// covers at most kSmallTriArea pixels of the tile are rasterised one per lane
+// (the common case for dense meshes, where a wave-per-triangle walk would
+// leave most lanes idle); larger ones one per wave, a lane per pixel.  Tiles
+// with a NaN run the fragment walk a second time for the affected pixels.
 
 constexpr int kSmallTriArea = 16;
 
@@ -305,6 +201,8 @@ template <int TS, int BLOCK>
 struct RasterLds {
     TriL tri[BLOCK];
     unsigned long long key[TS * TS];
+    int nan_g[TS * TS];       // highest id of a NaN fragment, -1 if none
+    uint8_t nan_init[TS * TS];
     int nan_seen;
 };
 
@@ -319,8 +217,10 @@ __device__ __forceinline__ float tri_depth(const TriL &t, float b0, float b1, fl
     return dd;
 }
 
-template <int TS, int BLOCK, bool SHADOW>
-__device__ __forceinline__ void raster_tile_fast(const RasterArgs &a, int tile, RasterLds<TS, BLOCK> &L)
+// One walk over the tile's triangles.  PASS 0: every fragment; PASS 1 (NaN
+// tiles): only pixels touched by a NaN, only fragments after their last NaN.
+template <int TS, int BLOCK, int PASS>
+__device__ __forceinline__ void raster_walk(const RasterArgs &a, int tile, RasterLds<TS, BLOCK> &L)
 {
     constexpr int NW = BLOCK / 64;
     constexpr int QS = (NW == 1) ? TS : TS / 2;
@@ -328,19 +228,24 @@ __device__ __forceinline__ void raster_tile_fast(const RasterArgs &a, int tile, 
     const int tx = tile % a.tiles_x, ty = tile / a.tiles_x;
     const int x0 = tx * TS, y0 = ty * TS;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    if (tid == 0) L.nan_seen = 0;
-    __syncthreads();
-    for (int p = tid; p < TS * TS; p += BLOCK) {
-        const int px = x0 + p % TS, py = y0 + p / TS;
-        float init = 3.40282347e+38f;
-        if (!a.fresh && px < a.W && py < a.H) init = a.depth[(size_t)px + (size_t)py * a.W];
-        if (init != init) L.nan_seen = 1;
-        L.key[p] = (unsigned long long)depth_order(init) << 32;
-    }
     const int qx0 = x0 + (NW == 1 ? 0 : (wave & 1) * QS);
     const int qy0 = y0 + (NW == 1 ? 0 : (wave >> 1) * QS);
     const int rx1 = qx0 + QS - 1, ry1 = qy0 + QS - 1;
     const uint32_t start = a.tile_start[tile], end = a.tile_end[tile];
+    auto fold = [&](const TriL &t, int px, int py, float dd) {
+        const int p = (py - y0) * TS + (px - x0);
+        if (PASS == 0) {
+            if (dd != dd) {
+                atomicMax(&L.nan_g[p], t.g);
+                L.nan_seen = 1;
+                return;
+            }
+        } else {
+            const int j = L.nan_g[p];
+            if (!(j >= 0 || L.nan_init[p]) || t.g <= j || dd != dd) return;
+        }
+        atomicMin(&L.key[p], ((unsigned long long)depth_order(dd) << 32) | (unsigned long long)(uint32_t)(t.g + 1));
+    };
     for (uint32_t base = start; base < end; base += BLOCK) {
         const int n = imin(BLOCK, (int)(end - base));
         __syncthreads();   // previous chunk's triangles fully consumed
@@ -351,37 +256,27 @@ __device__ __forceinline__ void raster_tile_fast(const RasterArgs &a, int tile, 
         }
         __syncthreads();
         // small triangles: one per lane
-        bool small = false;
         if (tid < n) {
             const TriL &t = L.tri[tid];
             const int bx0 = imax(t.bb0 & 0xffff, x0), bx1 = imin(t.bb1 & 0xffff, x0 + TS - 1);
             const int by0 = imax(t.bb0 >> 16, y0), by1 = imin(t.bb1 >> 16, y0 + TS - 1);
             const int area = (bx1 >= bx0 && by1 >= by0) ? (bx1 - bx0 + 1) * (by1 - by0 + 1) : 0;
-            small = area <= kSmallTriArea;
-            if (area > 0 && small) {
-                const unsigned long long gk = (unsigned long long)(uint32_t)(t.g + 1);
+            if (area > 0 && area <= kSmallTriArea) {
                 for (int py = by0; py <= by1; ++py)
                     for (int px = bx0; px <= bx1; ++px) {
                         float b0, b1, b2;
-                        if (!cover(t, px, py, b0, b1, b2)) continue;
-                        const float dd = tri_depth(t, b0, b1, b2);
-                        if (dd != dd) {
-                            L.nan_seen = 1;
-                            continue;
-                        }
-                        atomicMin(&L.key[(py - y0) * TS + (px - x0)],
-                                  ((unsigned long long)depth_order(dd) << 32) | gk);
+                        if (cover(t, px, py, b0, b1, b2)) fold(t, px, py, tri_depth(t, b0, b1, b2));
                     }
             }
         }
         // large triangles: one per wave over its quadrant, a lane per pixel
         unsigned long long mask[NW];
 #pragma unroll
-        for (int j = 0; j < NW; ++j) {
-            const int s = j * 64 + lane;
+        for (int jj = 0; jj < NW; ++jj) {
+            const int si = jj * 64 + lane;
             bool hit = false;
-            if (s < n) {
-                const TriL &t = L.tri[s];
+            if (si < n) {
+                const TriL &t = L.tri[si];
                 const int minx = t.bb0 & 0xffff, miny = t.bb0 >> 16;
                 const int maxx = t.bb1 & 0xffff, maxy = t.bb1 >> 16;
                 const int bx0 = imax(minx, x0), bx1 = imin(maxx, x0 + TS - 1);
@@ -389,57 +284,73 @@ __device__ __forceinline__ void raster_tile_fast(const RasterArgs &a, int tile, 
                 const int area = (bx1 >= bx0 && by1 >= by0) ? (bx1 - bx0 + 1) * (by1 - by0 + 1) : 0;
                 hit = area > kSmallTriArea && !(maxx < qx0 || minx > rx1 || maxy < qy0 || miny > ry1);
             }
-            mask[j] = __ballot(hit);
+            mask[jj] = __ballot(hit);
         }
-        (void)small;
 #pragma unroll
-        for (int j = 0; j < NW; ++j) {
-            unsigned long long m = mask[j];
+        for (int jj = 0; jj < NW; ++jj) {
+            unsigned long long m = mask[jj];
             while (m) {
-                const int i = j * 64 + __builtin_ctzll(m);
+                const int i = jj * 64 + __builtin_ctzll(m);
                 m &= m - 1;
                 const TriL &t = L.tri[i];
                 const int minx = t.bb0 & 0xffff, miny = t.bb0 >> 16;
                 const int maxx = t.bb1 & 0xffff, maxy = t.bb1 >> 16;
-                const unsigned long long gk = (unsigned long long)(uint32_t)(t.g + 1);
 #pragma unroll
                 for (int k = 0; k < NPX; ++k) {
                     const int lin = lane + 64 * k;
                     const int px = qx0 + lin % QS, py = qy0 + lin / QS;
                     if (px < a.W && py < a.H && px >= minx && px <= maxx && py >= miny && py <= maxy) {
                         float b0, b1, b2;
-                        if (cover(t, px, py, b0, b1, b2)) {
-                            const float dd = tri_depth(t, b0, b1, b2);
-                            if (dd != dd) L.nan_seen = 1;
-                            else
-                                atomicMin(&L.key[(py - y0) * TS + (px - x0)],
-                                          ((unsigned long long)depth_order(dd) << 32) | gk);
-                        }
+                        if (cover(t, px, py, b0, b1, b2)) fold(t, px, py, tri_depth(t, b0, b1, b2));
                     }
                 }
             }
         }
     }
     __syncthreads();
+}
+
+template <int TS, int BLOCK, bool SHADOW>
+__device__ __forceinline__ void raster_tile(const RasterArgs &a, int tile, RasterLds<TS, BLOCK> &L)
+{
+    const int tx = tile % a.tiles_x, ty = tile / a.tiles_x;
+    const int x0 = tx * TS, y0 = ty * TS;
+    const int tid = threadIdx.x;
+    if (tid == 0) L.nan_seen = 0;
+    __syncthreads();
+    for (int p = tid; p < TS * TS; p += BLOCK) {
+        const int px = x0 + p % TS, py = y0 + p / TS;
+        float init = 3.40282347e+38f;
+        if (!a.fresh && px < a.W && py < a.H) init = a.depth[(size_t)px + (size_t)py * a.W];
+        const bool ni = init != init;
+        if (ni) L.nan_seen = 1;
+        L.nan_init[p] = ni ? 1 : 0;
+        L.nan_g[p] = -1;
+        L.key[p] = ni ? ~0ull : (unsigned long long)depth_order(init) << 32;
+    }
+    raster_walk<TS, BLOCK, 0>(a, tile, L);   // ends with a barrier
     if (L.nan_seen) {
-        raster_tile<TS, BLOCK, SHADOW>(a, tile, L.tri);
-        return;
+        for (int p = tid; p < TS * TS; p += BLOCK)
+            if (L.nan_g[p] >= 0 || L.nan_init[p]) L.key[p] = ~0ull;
+        raster_walk<TS, BLOCK, 1>(a, tile, L);
     }
     for (int p = tid; p < TS * TS; p += BLOCK) {
         const int px = x0 + p % TS, py = y0 + p / TS;
         if (px >= a.W || py >= a.H) continue;
         const size_t i = (size_t)px + (size_t)py * a.W;
         const unsigned long long key = L.key[p];
-        const int win = (int)(uint32_t)(key & 0xffffffffu) - 1;
+        int win = key == ~0ull ? -1 : (int)(uint32_t)(key & 0xffffffffu) - 1;
+        bool from_key = win >= 0;
+        if (!from_key && L.nan_g[p] >= 0) win = L.nan_g[p];   // nothing after the last NaN fragment
         if (win >= 0) {
-            float dd = depth_from_order((uint32_t)(key >> 32));
+            float dd = from_key ? depth_from_order((uint32_t)(key >> 32)) : 0.f;
             TriL t;
             float b0 = 0.f, b1 = 0.f, b2 = 0.f;
             const int4 tv = a.tri_v[win];
-            if (dd == 0.f || SHADOW) {
+            if (!from_key || dd == 0.f || SHADOW) {
                 stage_tri(t, win, a.screen[tv.x], a.screen[tv.y], a.screen[tv.z], a.W, a.H);
                 cover(t, px, py, b0, b1, b2);
-                if (dd == 0.f) dd = tri_depth(t, b0, b1, b2);   // keeps the sign of a zero
+                if (!from_key || dd == 0.f) dd = tri_depth(t, b0, b1, b2);   // NaN, or a zero with its sign
             }
             a.depth[i] = dd;
             if (!SHADOW) {

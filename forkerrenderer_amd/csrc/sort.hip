@@ -1,13 +1,15 @@
 // sort.hip -- device-wide exclusive scan and stable LSD radix sort for gfx950
 // (64-lane wavefronts).
 //
-// The binning step needs every screen tile's triangle list in submission
-// order.  Pairs (tile, triangle) are produced in triangle order; a stable
-// sort by tile therefore yields exactly that.  Each radix pass ranks the
-// items of a 2048-item block with wave ballots (eight ballots give the mask
-// of lanes holding the same 8-bit digit; popcount of the lower lanes is the
-// rank inside the wave) and LDS per-wave digit counts for the order across
-// waves and rounds -- no atomics decide any position.
+// Binning sorts (tile, triangle) pairs by tile.  The rasteriser's depth
+// resolve does not depend on the order inside a tile's list, but the sort is
+// stable anyway (pairs come out in submission order per tile), so the lists
+// are reproducible.  Each radix pass ranks the items of a 2048-item block
+// with wave ballots (eight ballots give the mask of lanes holding the same
+// 8-bit digit; popcount of the lower lanes is the rank inside the wave) and
+// LDS per-wave digit counts for the order across waves and rounds -- no
+// atomics decide any position (device-scope atomics on shared counters are
+// several times slower here than the three sort passes).
 #include "fgl_kernels.hpp"
 
 namespace fgl {

@@ -93,10 +93,9 @@ def test_forward_config1_full_size(ctx, scene_dir):
     assert (d <= 1).mean() >= 0.999
 
 
-def test_nan_initial_depth_tiles_fall_back_to_the_sequential_fold(ctx, scene_dir):
+def test_nan_initial_depth_follows_the_submission_order_fold(ctx, scene_dir):
     """The reference's fold `replace when !(d >= best)` lets any fragment
-    replace a NaN depth -- not a minimum -- so a tile whose depth buffer holds
-    a NaN re-runs in submission order.  Depth written with scattered NaNs,
+    replace a NaN depth -- not a minimum.  Depth written with scattered NaNs,
     then a geometry pass over it: covered pixels equal the fresh render
     (and the oracle), uncovered ones keep what was written."""
     d = os.path.join(scene_dir, "nan")
@@ -125,3 +124,52 @@ def test_nan_initial_depth_tiles_fall_back_to_the_sequential_fold(ctx, scene_dir
         assert_bitexact(f"covered depth, tile {tile}", depth[covered], want["depth"][covered])
         assert_bitexact(f"uncovered depth, tile {tile}", depth[~covered], init[~covered])
         assert_bitexact(f"covered gbuf, tile {tile}", gbuf[:, covered], fresh["gbuf"][:, covered])
+
+
+def _nan_aware_equal(name, got, want):
+    gn, wn = np.isnan(got), np.isnan(want)
+    assert np.array_equal(gn, wn), f"{name}: NaN positions differ ({gn.sum()} vs {wn.sum()})"
+    assert_bitexact(name, got[~gn], want[~wn])
+
+
+def test_nan_fragments_follow_the_submission_order_fold(ctx):
+    """A vertex at NaN truncates to INT_MIN window coordinates with a NaN
+    depth, so its triangles cover real pixels with NaN fragments; an infinite
+    depth gives NaN where its weight is zero.  A NaN fragment always replaces
+    the stored depth and whatever follows it replaces the NaN: the winner is
+    the depth minimum over the fragments after the pixel's last NaN fragment.
+    Shadow pass with identity matrices, every tile size, against the oracle's
+    in-order fold (NaN positions equal, every other value bit-exact)."""
+    rng = np.random.default_rng(8)
+    W, H = 64, 48
+    n = 60
+    pos = rng.uniform(-1.2, 1.2, (n, 3)).astype(np.float32)
+    pos[:, 2] = rng.uniform(-0.9, 0.9, n)
+    pos[3] = np.nan
+    pos[7, 2] = np.inf
+    faces = rng.integers(0, n, (80, 3)).astype(np.int32)
+    md = {"pos": pos, "uv": np.zeros((n, 2), np.float32), "nrm": np.tile([[0, 0, 1]], (n, 1)).astype(np.float32),
+          "tan": None, "vi": faces, "ti": faces, "ni": faces, "face_material": np.zeros(len(faces), np.int32),
+          "support_pbr": False}
+    eye = np.eye(4, dtype=np.float32)
+    sc = O.OracleScene([(md, eye)], [O.Material()], [])
+    vp = O.make_viewport(0, 0, W, H)
+    fr = O.make_frame(eye, eye, eye, vp, vp, [0, 0, 0], 1, None, [])
+    want_d, want_s = O.shadow_pass(sc, fr, W, H)
+    assert np.isnan(want_d).sum() > 50 and (want_d < 3e38).sum() > 1000
+    m = F.Material()
+    for f in ("map_kd", "map_ks", "map_ke", "map_bump", "map_base", "map_rough", "map_metal", "map_ao", "map_pbr_ke"):
+        setattr(m, f, -1)
+    mat = ctx.upload_material(m)
+    md["face_material"] = np.full(len(faces), mat, np.int32)
+    mid = ctx.upload_model(md)
+    ctx.set_viewport(0, 0, W, H, shadow=True)
+    ctx.set_matrix(F.MAT_LIGHT_SPACE, eye)
+    for tile in (8, 16, 32):
+        ctx.set_pass_tile_size(F.PASS_SHADOW, tile)
+        ctx.init_shadow_buffer(W, H)
+        ctx.begin_pass(F.PASS_SHADOW)
+        ctx.draw_model(mid, eye)
+        ctx.end_pass()
+        _nan_aware_equal(f"shadow depth, tile {tile}", ctx.read(F.TARGET_SHADOW_DEPTH), want_d)
+        _nan_aware_equal(f"shadow map, tile {tile}", ctx.read(F.TARGET_SHADOW), want_s)

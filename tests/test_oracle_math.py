@@ -464,3 +464,63 @@ def test_kernel_ssao_matches_direct_fp64_projection():
     diff = np.array(diff)
     assert occluded > 0.3 * len(diff)
     assert np.mean(diff == 0) > 0.97 and diff.max() <= 2, (np.mean(diff == 0), diff.max())
+
+
+def _fold_in_order(init, frags):
+    """src/forkergl.cpp:180-199 at one pixel: fragments (g, depth) in
+    submission order, replace when !(d >= best)."""
+    best, win = np.float32(init), -1
+    for g, d in frags:
+        if not (d >= best):
+            best, win = d, g
+    return best, win
+
+
+def _depth_order(d):
+    u = int(np.float32(0.0 if d == 0 else d).view(np.uint32))
+    return (~u & 0xFFFFFFFF) if u & 0x80000000 else (u | 0x80000000)
+
+
+def _fold_any_order(init, frags):
+    """The rasteriser's order-free statement of the same fold: key minima
+    over (order(depth), g + 1), restricted to the fragments after the last
+    NaN fragment when the pixel has one (or when the stored depth is NaN)."""
+    nan_g = max([g for g, d in frags if np.isnan(d)], default=-1)
+    if nan_g >= 0 or np.isnan(init):
+        keys = [(_depth_order(d), g + 1, d) for g, d in frags if g > nan_g and not np.isnan(d)]
+    else:
+        keys = [(_depth_order(init), 0, np.float32(init))] + [(_depth_order(d), g + 1, d) for g, d in frags]
+    if keys:
+        _, g1, d = min(keys, key=lambda k: (k[0], k[1]))
+        if g1 > 0:
+            return d, g1 - 1
+    if nan_g >= 0:
+        return dict(frags)[nan_g], nan_g
+    return np.float32(init), -1
+
+
+def test_order_free_depth_fold_equals_the_in_order_fold():
+    """Random fragment lists over a value set with ties, signed zeros,
+    infinities and NaNs, random initial depths (NaN included), shuffled ids:
+    the order-free rule picks the same winner and the same depth bits as the
+    in-order fold, whatever order the fragments arrive in."""
+    rng = np.random.default_rng(31)
+    vals = np.array([-1.0, -0.0, 0.0, 0.25, 0.5, 1.0, np.inf, -np.inf, np.nan, -np.nan, np.finfo(np.float32).max],
+                    np.float32)
+    assert vals[9].view(np.uint32) >> 31 == 1   # a NaN with the sign bit set
+    nan_wins = 0
+    for _ in range(20000):
+        n = int(rng.integers(0, 8))
+        ids = np.sort(rng.choice(40, n, replace=False))
+        frags = [(int(g), vals[rng.integers(0, len(vals))]) for g in ids]
+        init = vals[rng.integers(0, len(vals))] if rng.random() < 0.5 else np.finfo(np.float32).max
+        want_d, want_g = _fold_in_order(init, frags)
+        shuffled = [frags[i] for i in rng.permutation(n)]
+        got_d, got_g = _fold_any_order(init, shuffled)
+        assert got_g == want_g, (init, frags)
+        if np.isnan(want_d):
+            assert np.isnan(got_d)
+            nan_wins += 1
+        else:
+            assert np.float32(got_d).view(np.uint32) == np.float32(want_d).view(np.uint32), (init, frags)
+    assert nan_wins > 100
