@@ -1,8 +1,9 @@
 """Raster autotuner: picks the screen-tile edge (8, 16 or 32 pixels; the
 workgroup shape follows: 64 lanes for 8x8, 256 lanes for 16x16 and 32x32)
 separately for the shadow pass and the main pass by timing whole frames of
-the scene on the device.  Results never depend on the choice (tile ownership
-keeps submission order per pixel); only speed does."""
+the scene on the device.  Results never depend on the choice (each pixel's
+depth resolve is a minimum over (depth, submission id) keys, the same for
+any tile shape or list order); only speed does."""
 import itertools
 import time
 
