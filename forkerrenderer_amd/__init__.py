@@ -20,6 +20,7 @@ WRAP_NONE, WRAP_REPEAT, WRAP_MIRROR, WRAP_CLAMP = 0, 1, 2, 3
 FILTER_NEAREST, FILTER_LINEAR = 0, 1
 MAT_VIEW, MAT_PROJ, MAT_LIGHT_SPACE = 0, 1, 2
 SHADOW_HARD, SHADOW_PCF_DISK, SHADOW_PCSS, SHADOW_PCF_GRID = 0, 1, 2, 3
+SHADOW_REF_PCF, SHADOW_REF_PCSS = 4, 5
 TARGET_FRAME, TARGET_DEPTH, TARGET_SHADOW, TARGET_GBUFFER, TARGET_AO, TARGET_IMAGE, TARGET_SHADOW_DEPTH = range(7)
 MAX_LIGHTS, MAX_DISK = 4, 64
 GBUF_PLANES = 19
@@ -59,7 +60,7 @@ class ShadowCfg(C.Structure):
 class SsaoCfg(C.Structure):
     _fields_ = [("n_kernel", C.c_int32), ("kernel", C.c_float * 192), ("noise", C.c_float * 48),
                 ("radius", C.c_float), ("bias", C.c_float), ("range", C.c_float), ("blur", C.c_int32),
-                ("blur_radius", C.c_int32), ("blur_weights", C.c_float * 9)]
+                ("blur_radius", C.c_int32), ("blur_weights", C.c_float * 9), ("stochastic", C.c_int32)]
 
 
 class SceneInfo(C.Structure):
@@ -107,6 +108,9 @@ def lib():
             "fgl_draw_screen_space_pixels": (C.c_int, [P]),
             "fgl_ssao": (C.c_int, [P, C.POINTER(SsaoCfg)]),
             "fgl_resolve": (C.c_int, [P, C.c_int]),
+            "fgl_rng_seed": (C.c_int, [P, C.c_uint32]),
+            "fgl_rng_draws": (C.c_int, [P, C.POINTER(C.c_uint64)]),
+            "fgl_rng_vectors": (C.c_int, [P, C.c_int, C.c_uint32, P]),
             "fgl_read_target": (C.c_int, [P, C.c_int, P, C.c_size_t]),
             "fgl_write_target": (C.c_int, [P, C.c_int, P, C.c_size_t]),
             "fgl_target_bytes": (C.c_int, [P, C.c_int, C.POINTER(C.c_size_t)]),
@@ -330,6 +334,24 @@ class Context:
 
     def ssao(self, cfg):
         self._chk(lib().fgl_ssao(self._h, C.byref(cfg)), "fgl_ssao")
+
+    def rng_seed(self, seed=5489):
+        """Restarts the reference's random stream (std::mt19937(seed)) used by
+        stochastic SSAO and the FGL_SHADOW_REF_* filters."""
+        self._chk(lib().fgl_rng_seed(self._h, seed), "fgl_rng_seed")
+
+    def rng_vectors(self, kind, n):
+        """n RandomVectorInUnitSphere (kind 3) or RandomVectorInUnitDisk
+        (kind 2) results from the stream, shape (n, kind)."""
+        out = np.empty((n, kind), np.float32)
+        self._chk(lib().fgl_rng_vectors(self._h, kind, n, out.ctypes.data_as(C.c_void_p)), "fgl_rng_vectors")
+        return out
+
+    def rng_draws(self):
+        """32-bit draws consumed from the stream since its restart."""
+        n = C.c_uint64()
+        self._chk(lib().fgl_rng_draws(self._h, C.byref(n)), "fgl_rng_draws")
+        return n.value
 
     def resolve(self, k=1):
         self._chk(lib().fgl_resolve(self._h, k), "fgl_resolve")

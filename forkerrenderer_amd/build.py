@@ -19,7 +19,7 @@ LIB = os.path.join(LIBDIR, "libforkergl.so")
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
-SOURCES = ["raster.hip", "sort.hip", "shade.hip", "nao.hip", "batch.hip", "context.cpp", "host_io.cpp", "batch.cpp"]
+SOURCES = ["raster.hip", "sort.hip", "shade.hip", "rng.hip", "nao.hip", "batch.hip", "context.cpp", "host_io.cpp", "batch.cpp"]
 COMMON = [
     "-O3", "-std=c++17", "-fPIC", "-ffp-contract=off",
     "-fno-gpu-flush-denormals-to-zero",

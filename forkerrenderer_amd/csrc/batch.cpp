@@ -94,6 +94,8 @@ int check_view(fgl_ctx *c, const fgl_view_desc &v)
         return ctx_fail(c, FGL_ERR_INVALID, "view size must be 1..8192");
     if (v.shadow_on && (v.shadow_size < 1 || v.shadow_size > 8192 || v.n_lights < 1))
         return ctx_fail(c, FGL_ERR_INVALID, "a shadowed view needs shadow_size 1..8192 and a light");
+    if (v.shadow_on && v.shadow.mode >= FGL_SHADOW_REF_PCF)
+        return ctx_fail(c, FGL_ERR_INVALID, "views take the deterministic shadow filters (modes 0-3)");
     if (v.n_draws < 0 || (v.n_draws && (!v.models || !v.matrices)))
         return ctx_fail(c, FGL_ERR_INVALID, "bad draw list");
     return FGL_OK;

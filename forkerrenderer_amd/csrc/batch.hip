@@ -172,7 +172,8 @@ __global__ __launch_bounds__(256) void k_bv_lighting(BvPixelArgs b)
     int vi, x, y;
     if (!bv_pixel(b, vi, x, y)) return;
     const DevView &v = b.views[vi];
-    lighting_px(v.u, b.gbuf + kGbufPlanes * v.px_off, b.ao + v.px_off, b.ao_is_one, b.shadow + v.spx_off,
+    const ShadowSamples none = {nullptr, nullptr, 0};
+    lighting_px(v.u, b.gbuf + kGbufPlanes * v.px_off, b.ao + v.px_off, b.ao_is_one, b.shadow + v.spx_off, none,
                 b.fb + 3 * v.px_off, x, y);
 }
 

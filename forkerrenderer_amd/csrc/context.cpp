@@ -82,6 +82,10 @@ struct fgl_ctx {
         tile_start, tile_end, tri_v, big_list, counters;
     std::vector<DevDraw> last_draws;
     uint32_t *h_total = nullptr;
+    // the reference's random stream (device MtState), a copy for the PCSS
+    // replay, its sample buffers, the DoSSAO powf table
+    DevBuf rng, rng_saved, rng_samples, rng_offs, ssao_pow3;
+    uint32_t rng_seed = 5489u;
 
     // timing
     bool timing = false;
@@ -148,6 +152,16 @@ struct KernelTimer {
     }
 };
 
+// Restarts the context's random stream at its seed (std::mt19937(seed)).
+static int rng_restart(fgl_ctx *c)
+{
+    MtState st;
+    mt_seed_state(c->rng_seed, st);
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    HIPCHK(c, hipMemcpy(c->rng.p, &st, sizeof(st), hipMemcpyHostToDevice));
+    return FGL_OK;
+}
+
 template <typename T>
 static int upload_array(fgl_ctx *c, T **dst, const T *src, size_t n)
 {
@@ -185,6 +199,21 @@ int fgl_ctx_create(int device, fgl_ctx **out)
     std::memcpy(c->proj, I, sizeof(I));
     std::memcpy(c->light_space, I, sizeof(I));
     std::memset(&c->shadow_cfg, 0, sizeof(c->shadow_cfg));
+    // Render::DoSSAO's 1 - occlusion is k/32 (k = 0..32, exact); its
+    // Pow(x, 3) is the host libm's powf, as in the reference.
+    float pow3[33];
+    for (int k = 0; k <= 32; ++k) {
+        float occ = 0.f;
+        for (int j = 0; j < k; ++j) occ += (1.f / 32) * 1.f;
+        pow3[k] = powf(1.f - occ, 3);
+    }
+    if (ensure(c, c->ssao_pow3, sizeof(pow3)) || ensure(c, c->rng, sizeof(MtState)) ||
+        ensure(c, c->rng_saved, sizeof(MtState)) ||
+        hipMemcpy(c->ssao_pow3.p, pow3, sizeof(pow3), hipMemcpyHostToDevice) != hipSuccess ||
+        rng_restart(c) != FGL_OK) {
+        fgl_ctx_destroy(c);
+        return FGL_ERR_NOMEM;
+    }
     *out = c;
     return FGL_OK;
 }
@@ -202,6 +231,7 @@ void fgl_ctx_destroy(fgl_ctx *c)
     }
     DevBuf *bufs[] = {&c->d_textures, &c->d_materials, &c->depth, &c->fb, &c->gbuf, &c->ao,
                       &c->blur_tmp, &c->shadow, &c->shadow_depth, &c->img, &c->vis, &c->d_draws,
+                      &c->rng, &c->rng_saved, &c->rng_samples, &c->rng_offs, &c->ssao_pow3,
                       &c->screen, &c->rect, &c->counts, &c->offs, &c->scan_tmp, &c->keys,
                       &c->vals, &c->keys_alt, &c->vals_alt, &c->sort_tmp, &c->tile_start,
                       &c->tile_end, &c->tri_v, &c->big_list, &c->counters};
@@ -455,9 +485,10 @@ int fgl_set_shadow(fgl_ctx *c, int on, const fgl_shadow_cfg *cfg)
 {
     if (!c) return FGL_ERR_INVALID;
     if (cfg) {
-        if (cfg->mode < 0 || cfg->mode > 3 || cfg->n_blocker < 0 || cfg->n_blocker > FGL_MAX_DISK ||
+        if (cfg->mode < 0 || cfg->mode > 5 || cfg->n_blocker < 0 || cfg->n_blocker > FGL_MAX_DISK ||
             cfg->n_pcf < 0 || cfg->n_pcf > FGL_MAX_DISK || (cfg->mode == 3 && (cfg->grid < 1 || cfg->grid > 9)) ||
-            ((cfg->mode == 1 || cfg->mode == 2) && cfg->n_pcf < 1))
+            ((cfg->mode == 1 || cfg->mode == 2 || cfg->mode >= 4) && cfg->n_pcf < 1) ||
+            (cfg->mode == 5 && cfg->n_blocker < 1))
             return fail(c, FGL_ERR_INVALID, "bad shadow filter configuration");
         c->shadow_cfg = *cfg;
     }
@@ -589,6 +620,12 @@ static int flush_pass(fgl_ctx *c)
     if (pass == kPassForward && c->shadow_on && !c->shadow.p) {
         c->draws.clear();
         return fail(c, FGL_ERR_INVALID, "forward pass with shadows needs a shadow buffer");
+    }
+    if (pass == kPassForward && c->shadow_on && c->shadow_cfg.mode >= FGL_SHADOW_REF_PCF) {
+        // the forward shaders draw per depth-passing fragment in raster
+        // order, a sequence this tile rasteriser does not replay
+        c->draws.clear();
+        return fail(c, FGL_ERR_INVALID, "stochastic shadow filters run in the deferred lighting pass only");
     }
     int rc = upload_tables(c);
     if (rc) return rc;
@@ -764,10 +801,45 @@ int fgl_draw_screen_space_pixels(fgl_ctx *c)
     if ((rc = materialize(c, false, true, false, false, c->shadow_on != 0))) return rc;
     FrameUniforms u;
     build_uniforms(c, u);
+    ShadowSamples ss = {nullptr, nullptr, 0};
+    const int mode = c->shadow_cfg.mode;
+    if (c->shadow_on && !c->lights.empty() && (mode == FGL_SHADOW_REF_PCF || mode == FGL_SHADOW_REF_PCSS)) {
+        // The filters' unit-disk samples from the stream, pixels in row order.
+        KernelTimer t(c, "shadow_samples");
+        const size_t P = (size_t)c->W * c->H;
+        const uint32_t nb = (uint32_t)c->shadow_cfg.n_blocker, np = (uint32_t)c->shadow_cfg.n_pcf;
+        const size_t most = P * (mode == FGL_SHADOW_REF_PCF ? np : nb + np);
+        if (most > 0xffffffffull) return fail(c, FGL_ERR_CAPACITY, "stochastic shadows: too many samples");
+        if ((rc = ensure(c, c->rng_samples, most * 2 * sizeof(float)))) return rc;
+        MtState *st = (MtState *)c->rng.p;
+        float *pairs = (float *)c->rng_samples.p;
+        if (mode == FGL_SHADOW_REF_PCF) {
+            launch_rng_accept(2, st, pairs, (uint32_t)most, c->stream);
+            ss = {pairs, nullptr, np};
+        } else {
+            // PCSS takes n_pcf more samples only where its blocker search
+            // finds a blocker: draw the most it can need, walk the pixels for
+            // the offsets, then replay the stream from the saved state to
+            // leave it after exactly the samples used (the replay rewrites
+            // the same values).
+            if ((rc = ensure(c, c->rng_offs, (P + 1) * sizeof(uint32_t)))) return rc;
+            uint32_t *offs = (uint32_t *)c->rng_offs.p;
+            HIPCHK(c, hipMemcpyAsync(c->rng_saved.p, st, sizeof(MtState), hipMemcpyDeviceToDevice, c->stream));
+            launch_rng_accept(2, st, pairs, (uint32_t)most, c->stream);
+            launch_pcss_walk(u, (const float *)c->gbuf.p, (const float *)c->shadow.p, pairs, offs, offs + P,
+                             c->stream);
+            HIPCHK(c, hipMemcpyAsync(c->h_total, offs + P, 4, hipMemcpyDeviceToHost, c->stream));
+            HIPCHK(c, hipStreamSynchronize(c->stream));
+            const uint32_t used = c->h_total[0];
+            HIPCHK(c, hipMemcpyAsync(st, c->rng_saved.p, sizeof(MtState), hipMemcpyDeviceToDevice, c->stream));
+            launch_rng_accept(2, st, pairs, used, c->stream);
+            ss = {pairs, offs, 0};
+        }
+    }
     {
         KernelTimer t(c, "lighting");
         launch_lighting(u, (const float *)c->gbuf.p, (const float *)c->ao.p, c->pend_ao ? 1 : 0,
-                        (const float *)c->shadow.p, (float *)c->fb.p, c->stream);
+                        (const float *)c->shadow.p, ss, (float *)c->fb.p, c->stream);
     }
     c->pend_fb = false;  // every pixel written
     HIPCHK(c, hipGetLastError());
@@ -777,7 +849,7 @@ int fgl_draw_screen_space_pixels(fgl_ctx *c)
 int fgl_ssao(fgl_ctx *c, const fgl_ssao_cfg *cfg)
 {
     if (!c || !cfg) return FGL_ERR_INVALID;
-    if (cfg->n_kernel < 1 || cfg->n_kernel > 64 || cfg->blur < 0 || cfg->blur > 2 ||
+    if ((!cfg->stochastic && (cfg->n_kernel < 1 || cfg->n_kernel > 64)) || cfg->blur < 0 || cfg->blur > 2 ||
         (cfg->blur == 2 && (cfg->blur_radius < 1 || cfg->blur_radius > 8)))
         return fail(c, FGL_ERR_INVALID, "bad SSAO configuration");
     int rc = flush_pass(c);
@@ -794,7 +866,16 @@ int fgl_ssao(fgl_ctx *c, const fgl_ssao_cfg *cfg)
     sa.radius = cfg->radius;
     sa.bias = cfg->bias;
     sa.range = cfg->range;
-    {
+    if (cfg->stochastic) {
+        // 32 hemisphere samples per pixel, pixels in row order, from the stream
+        const size_t n = (size_t)c->W * c->H * 32;
+        if (n > 0xffffffffull) return fail(c, FGL_ERR_CAPACITY, "stochastic SSAO: too many samples");
+        if ((rc = ensure(c, c->rng_samples, n * 3 * sizeof(float)))) return rc;
+        KernelTimer t(c, "ssao_reference");
+        launch_rng_accept(3, (MtState *)c->rng.p, (float *)c->rng_samples.p, (uint32_t)n, c->stream);
+        launch_ssao_reference(u, (const float *)c->rng_samples.p, (const float *)c->ssao_pow3.p,
+                              (const float *)c->depth.p, (const float *)c->gbuf.p, (float *)c->ao.p, c->stream);
+    } else {
         KernelTimer t(c, "ssao");
         launch_ssao(u, sa, (const float *)c->depth.p, (const float *)c->gbuf.p, (float *)c->ao.p, c->stream);
     }
@@ -899,6 +980,42 @@ int fgl_enable_timing(fgl_ctx *c, int on)
 {
     if (!c) return FGL_ERR_INVALID;
     c->timing = on != 0;
+    return FGL_OK;
+}
+
+int fgl_rng_seed(fgl_ctx *c, uint32_t seed)
+{
+    if (!c) return FGL_ERR_INVALID;
+    int rc = flush_pass(c);
+    if (rc) return rc;
+    c->rng_seed = seed;
+    return rng_restart(c);
+}
+
+int fgl_rng_vectors(fgl_ctx *c, int kind, uint32_t n, float *out)
+{
+    if (!c || (kind != 2 && kind != 3) || (n && !out)) return FGL_ERR_INVALID;
+    int rc = flush_pass(c);
+    if (rc) return rc;
+    if (n == 0) return FGL_OK;
+    const size_t bytes = (size_t)n * kind * sizeof(float);
+    if ((rc = ensure(c, c->rng_samples, bytes))) return rc;
+    launch_rng_accept(kind, (MtState *)c->rng.p, (float *)c->rng_samples.p, n, c->stream);
+    HIPCHK(c, hipGetLastError());
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    HIPCHK(c, hipMemcpy(out, c->rng_samples.p, bytes, hipMemcpyDeviceToHost));
+    return FGL_OK;
+}
+
+int fgl_rng_draws(fgl_ctx *c, uint64_t *out)
+{
+    if (!c || !out) return FGL_ERR_INVALID;
+    int rc = flush_pass(c);
+    if (rc) return rc;
+    MtState st;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    HIPCHK(c, hipMemcpy(&st, c->rng.p, sizeof(st), hipMemcpyDeviceToHost));
+    *out = st.draws;
     return FGL_OK;
 }
 
@@ -1126,6 +1243,9 @@ int fgl_render_scene(fgl_ctx *c, fgl_scene *sc)
     int rc;
     if ((rc = upload_scene(c, sc))) return rc;
     const fgl_scene_info &I = sc->s.info;
+    // one Render::Render run: the reference's stream starts fresh
+    if ((I.ssao_on && I.ssao.stochastic) || (I.shadow_on && I.shadow.mode >= FGL_SHADOW_REF_PCF))
+        if ((rc = rng_restart(c))) return rc;
     const int k = I.ssaa_on ? I.ssaa_k : 1;
     if (k < 1) return fail(c, FGL_ERR_INVALID, "bad SSAA factor");
     const int W = I.width * k, H = I.height * k;

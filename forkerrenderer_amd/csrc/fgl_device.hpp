@@ -252,28 +252,49 @@ __device__ __forceinline__ float pcf_disk(const float *map, int SW, int SH, Vec3
     return vis;
 }
 
-__device__ __forceinline__ float shadow_visibility(const FrameUniforms &u, const float *map,
-                                                   Vec3 ndc, Vec3 n, Vec3 l)
+// PCSS blocker term (FindAverageBlockDepth, shadow.cpp:65-91): the average
+// depth of the samples that block, 0 when none does.
+__device__ __forceinline__ float pcss_blocker_depth(const FrameUniforms &u, const float *map, Vec3 sc, float bias,
+                                                    const float *disk, int n)
 {
     const DevShadowCfg &c = u.shadow;
-    Vec3 sc = add3(scale3(ndc, 0.5f), vec3(0.5f, 0.5f, 0.5f));
-    float bias = smax(0.009f * (1.f - dot3(n, l)), 0.007f);
-    if (c.mode == 1) return pcf_disk(map, u.SW, u.SH, sc, bias, c.pcf_filter, c.pcf_disk, c.n_pcf);
-    if (c.mode == 2) {
-        float blocker = 0.f, nb = 0.f;
-        for (int i = 0; i < c.n_blocker; ++i) {
-            float su = sc.x + (float)((double)c.blocker_disk[2 * i] * c.blocker_filter);
-            float sv = sc.y + (float)((double)c.blocker_disk[2 * i + 1] * c.blocker_filter);
-            float s = sample_shadow(map, u.SW, u.SH, su, sv);
-            if (sc.z > s + bias) {
-                blocker += s;
-                nb += 1.f;
-            }
+    float blocker = 0.f, nb = 0.f;
+    for (int i = 0; i < n; ++i) {
+        float su = sc.x + (float)((double)disk[2 * i] * c.blocker_filter);
+        float sv = sc.y + (float)((double)disk[2 * i + 1] * c.blocker_filter);
+        float s = sample_shadow(map, u.SW, u.SH, su, sv);
+        if (sc.z > s + bias) {
+            blocker += s;
+            nb += 1.f;
         }
-        float dB = (nb < 1.f) ? 0.f : blocker / nb;
+    }
+    return (nb < 1.f) ? 0.f : blocker / nb;
+}
+
+// Shadow coordinate and slope bias (CalculateShadowVisibility, shadow.cpp:109-117).
+__device__ __forceinline__ Vec3 shadow_coord(Vec3 ndc) { return add3(scale3(ndc, 0.5f), vec3(0.5f, 0.5f, 0.5f)); }
+__device__ __forceinline__ float shadow_bias(Vec3 n, Vec3 l) { return smax(0.009f * (1.f - dot3(n, l)), 0.007f); }
+
+// `rdisk`: this pixel's unit-disk samples from the random stream (modes 4 and
+// 5 only): the PCF samples, or the blocker samples followed by the PCF ones.
+__device__ __forceinline__ float shadow_visibility(const FrameUniforms &u, const float *map,
+                                                   Vec3 ndc, Vec3 n, Vec3 l, const float *rdisk)
+{
+    const DevShadowCfg &c = u.shadow;
+    Vec3 sc = shadow_coord(ndc);
+    float bias = shadow_bias(n, l);
+    if (c.mode == 1) return pcf_disk(map, u.SW, u.SH, sc, bias, c.pcf_filter, c.pcf_disk, c.n_pcf);
+    if (c.mode == 4) return pcf_disk(map, u.SW, u.SH, sc, bias, c.pcf_filter, rdisk, c.n_pcf);
+    if (c.mode == 2 || c.mode == 5) {
+        // (two calls rather than a select between the argument-space sample
+        // arrays and the global stream samples)
+        float dB;
+        if (c.mode == 5) dB = pcss_blocker_depth(u, map, sc, bias, rdisk, c.n_blocker);
+        else dB = pcss_blocker_depth(u, map, sc, bias, c.blocker_disk, c.n_blocker);
         if ((double)dB < 0.001) return 1.f;
         float pen = (sc.z - dB) * c.area_light / dB;
         float filter = (float)(c.pcss_filter * (double)pen);
+        if (c.mode == 5) return pcf_disk(map, u.SW, u.SH, sc, bias, filter, rdisk + 2 * c.n_blocker, c.n_pcf);
         return pcf_disk(map, u.SW, u.SH, sc, bias, filter, c.pcf_disk, c.n_pcf);
     }
     if (c.mode == 3) {
@@ -372,7 +393,8 @@ __device__ __forceinline__ Vec3 light_dir(const DevLight &l, Vec3 P)
 // Blinn-Phong halfway slot as DrawScreenSpacePixels does (forkergl.cpp:369).
 __device__ __forceinline__ Vec3 shade_point(const FrameUniforms &u, const float *smap, bool pbr,
                                             bool deferred_quirk, Vec3 P, Vec3 N, Vec3 lndc,
-                                            Vec3 albedo, Vec3 emissive, Vec3 param)
+                                            Vec3 albedo, Vec3 emissive, Vec3 param,
+                                            const float *rdisk = nullptr)
 {
     const float gamma = (float)2.2;
     Vec3 Vd = norm3(sub3(vec3(u.eye[0], u.eye[1], u.eye[2]), P));
@@ -385,7 +407,7 @@ __device__ __forceinline__ Vec3 shade_point(const FrameUniforms &u, const float 
             Vec3 L = light_dir(l, P);
             float vis = 1.f;
             if (!u.shadow_on) vis = 0.f;
-            else if (i == 0) vis = shadow_visibility(u, smap, lndc, N, L);
+            else if (i == 0) vis = shadow_visibility(u, smap, lndc, N, L, rdisk);
             Vec3 H = deferred_quirk ? Vd : norm3(add3(L, Vd));
             c = add3(c, bp_term(L, H, N, vis, dl, el, param.x, param.y, param.z,
                                 vec3(l.color[0], l.color[1], l.color[2]), u.shadow_on, i == 0));
@@ -400,7 +422,7 @@ __device__ __forceinline__ Vec3 shade_point(const FrameUniforms &u, const float 
         Vec3 L = light_dir(l, P);
         float vis = 1.f;
         if (!u.shadow_on) vis = 0.f;
-        else if (i == 0) vis = shadow_visibility(u, smap, lndc, N, L);
+        else if (i == 0) vis = shadow_visibility(u, smap, lndc, N, L, rdisk);
         Vec3 H = norm3(add3(L, Vd));
         Vec3 lo = pbr_lo(L, Vd, H, N, vis, al, param.y, param.z,
                          vec3(l.color[0], l.color[1], l.color[2]), u.shadow_on);

@@ -83,9 +83,40 @@ void launch_radix_sort(uint32_t *keys, uint32_t *vals, uint32_t *keys_alt, uint3
 int launch_radix_sort_impl(uint32_t *keys, uint32_t *vals, uint32_t *keys_alt, uint32_t *vals_alt,
                            size_t n, const uint32_t *n_dev, int bits, uint32_t *tmp, hipStream_t s);
 
+// rng.hip: the reference's std::mt19937 stream (624 words, the next output
+// index, 32-bit draws consumed).
+struct MtState {
+    uint32_t mt[624];
+    uint32_t idx, pad;
+    unsigned long long draws;
+};
+void mt_seed_state(uint32_t seed, MtState &st);
+// Rejection-sampled unit vectors in stream order: draws_per_try 3 gives
+// RandomVectorInUnitSphere (x, y, z), 2 RandomVectorInUnitDisk (x, y).
+void launch_rng_accept(int draws_per_try, MtState *st, float *out, uint32_t n_accept, hipStream_t s);
+
+// Per-pixel unit-disk samples of the stochastic shadow filters: pixel i's
+// samples start at pair offs[i] of `pairs`, or at i * stride when offs is
+// null (fixed count per pixel).
+struct ShadowSamples {
+    const float *pairs;
+    const uint32_t *offs;
+    uint32_t stride;
+};
+
 // shade.hip
 void launch_lighting(const FrameUniforms &u, const float *gbuf, const float *ao, int ao_is_one,
-                     const float *shadow_map, float *fb, hipStream_t s);
+                     const float *shadow_map, const ShadowSamples &ss, float *fb, hipStream_t s);
+// Render::DoSSAO with the stream's hemisphere samples (32 (x, y, z) per
+// pixel in row order); pow3[c] = powf(1 - c / 32, 3) from the host libm.
+void launch_ssao_reference(const FrameUniforms &u, const float *samples, const float *pow3,
+                           const float *depth, const float *gbuf, float *ao, hipStream_t s);
+// Offsets of the PCSS filter's samples: walks the pixels in row order,
+// running each pixel's blocker search at its offset; a pixel takes n_blocker
+// pairs, plus n_pcf when a blocker is found.  offs[i] per pixel, *total at
+// the end.
+void launch_pcss_walk(const FrameUniforms &u, const float *gbuf, const float *shadow_map, const float *pairs,
+                      uint32_t *offs, uint32_t *total, hipStream_t s);
 void launch_ssao(const FrameUniforms &u, const SsaoArgs &sa, const float *depth,
                  const float *gbuf, float *ao, hipStream_t s);
 void launch_blur_recursive(float *ao, int W, int H, hipStream_t s);

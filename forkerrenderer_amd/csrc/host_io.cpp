@@ -786,11 +786,20 @@ bool load_scene(const std::string &path, HostScene &s, std::string &err)
                 nb = nb < 1 ? 1 : (nb > FGL_MAX_DISK ? FGL_MAX_DISK : nb);
                 np = np < 1 ? 1 : (np > FGL_MAX_DISK ? FGL_MAX_DISK : np);
                 set_shadow_pcss(I.shadow, nb, np, area);
+            } else if (kind == "refpcf" || kind == "refpcss") {
+                // the reference's SOFT_SHADOW_PCF / SOFT_SHADOW_PCSS with its
+                // random disk samples (src/shaders/shadow.h:14-23)
+                set_shadow_pcss(I.shadow, 32, 64, 2.5f);
+                I.shadow.mode = kind == "refpcf" ? FGL_SHADOW_REF_PCF : FGL_SHADOW_REF_PCSS;
             }
+        } else if (starts(line, "ssaokernel reference")) {
+            // Render::DoSSAO itself: RNG hemisphere samples from the stream
+            I.ssao.stochastic = 1;
         } else if (starts(line, "ssaokernel ")) {
             int n = 64;
             float r = 0.5f, b = 0.0005f, rg = 0.01f;
             unsigned long long seed = 0x55A0u;
+            I.ssao.stochastic = 0;
             iss >> trash >> n;
             read_float(iss, r);
             read_float(iss, b);

@@ -7,13 +7,73 @@ namespace fgl {
 
 __global__ __launch_bounds__(256) void k_lighting(FrameUniforms u, const float *__restrict__ g,
                                                   const float *__restrict__ ao, int ao_is_one,
-                                                  const float *__restrict__ smap,
+                                                  const float *__restrict__ smap, ShadowSamples ss,
                                                   float *__restrict__ fb)
 {
     const int x = blockIdx.x * 64 + (threadIdx.x & 63);
     const int y = blockIdx.y * 4 + (threadIdx.x >> 6);
     if (x >= u.W || y >= u.H) return;
-    lighting_px(u, g, ao, ao_is_one, smap, fb, x, y);
+    lighting_px(u, g, ao, ao_is_one, smap, ss, fb, x, y);
+}
+
+__global__ __launch_bounds__(256) void k_ssao_ref(FrameUniforms u, const float *__restrict__ samples,
+                                                  const float *__restrict__ pow3,
+                                                  const float *__restrict__ depth,
+                                                  const float *__restrict__ g, float *__restrict__ ao)
+{
+    const int x = blockIdx.x * 64 + (threadIdx.x & 63);
+    const int y = blockIdx.y * 4 + (threadIdx.x >> 6);
+    if (x >= u.W || y >= u.H) return;
+    ssao_ref_px(u, samples, pow3, depth, g, ao, x, y);
+}
+
+// The PCSS sample offsets: one wavefront walks the pixels in the lighting
+// pass's order (row order); at each pixel lanes evaluate the blocker
+// samples at the pixel's offset in parallel and lane order sums them as
+// FindAverageBlockDepth does (shadow.cpp:65-91), giving the pixel's
+// consumption: n_blocker pairs, plus n_pcf when a blocker is found
+// (PCSS, shadow.cpp:93-106).  Only light 0 is shadowed; with shadows off or
+// no light nothing is consumed.
+__global__ __launch_bounds__(64) void k_pcss_walk(FrameUniforms u, const float *__restrict__ g,
+                                                  const float *__restrict__ smap,
+                                                  const float *__restrict__ pairs,
+                                                  uint32_t *__restrict__ offs, uint32_t *__restrict__ total)
+{
+    const int lane = threadIdx.x;
+    const size_t P = (size_t)u.W * u.H;
+    const DevShadowCfg &c = u.shadow;
+    const bool consume = u.shadow_on && u.n_lights > 0;
+    uint32_t o = 0;
+    for (size_t i = 0; i < P; ++i) {
+        if (lane == 0) offs[i] = o;
+        if (!consume) continue;
+        const Vec3 N = vec3(g[i], g[P + i], g[2 * P + i]);
+        const Vec3 Pw = vec3(g[3 * P + i], g[4 * P + i], g[5 * P + i]);
+        const Vec3 lndc = vec3(g[6 * P + i], g[7 * P + i], g[8 * P + i]);
+        const Vec3 L = light_dir(u.lights[0], Pw);
+        const Vec3 sc = shadow_coord(lndc);
+        const float bias = shadow_bias(N, L);
+        float sv = 0.f;
+        int blk = 0;
+        if (lane < c.n_blocker) {
+            const float *d = pairs + 2 * ((size_t)o + lane);
+            const float su = sc.x + (float)((double)d[0] * c.blocker_filter);
+            const float sw = sc.y + (float)((double)d[1] * c.blocker_filter);
+            sv = sample_shadow(smap, u.SW, u.SH, su, sw);
+            blk = sc.z > sv + bias;
+        }
+        float blocker = 0.f, nb = 0.f;
+        for (int j = 0; j < c.n_blocker; ++j) {
+            const float sj = __shfl(sv, j);
+            if (__shfl(blk, j)) {
+                blocker += sj;
+                nb += 1.f;
+            }
+        }
+        const float dB = (nb < 1.f) ? 0.f : blocker / nb;
+        o += (uint32_t)c.n_blocker + (((double)dB < 0.001) ? 0u : (uint32_t)c.n_pcf);
+    }
+    if (lane == 0) *total = o;
 }
 
 __global__ __launch_bounds__(256) void k_ssao(FrameUniforms u, SsaoArgs sa,
@@ -120,11 +180,24 @@ void launch_copy(const void *src, void *dst, size_t bytes, hipStream_t s)
 static inline dim3 grid2(int W, int H) { return dim3((W + 63) / 64, (H + 3) / 4); }
 
 void launch_lighting(const FrameUniforms &u, const float *gbuf, const float *ao, int ao_is_one,
-                     const float *shadow_map, float *fb, hipStream_t s)
+                     const float *shadow_map, const ShadowSamples &ss, float *fb, hipStream_t s)
 {
     if (u.W <= 0 || u.H <= 0) return;
     hipLaunchKernelGGL(k_lighting, grid2(u.W, u.H), dim3(256), 0, s, u, gbuf, ao, ao_is_one,
-                       shadow_map, fb);
+                       shadow_map, ss, fb);
+}
+
+void launch_ssao_reference(const FrameUniforms &u, const float *samples, const float *pow3,
+                           const float *depth, const float *gbuf, float *ao, hipStream_t s)
+{
+    if (u.W <= 0 || u.H <= 0) return;
+    hipLaunchKernelGGL(k_ssao_ref, grid2(u.W, u.H), dim3(256), 0, s, u, samples, pow3, depth, gbuf, ao);
+}
+
+void launch_pcss_walk(const FrameUniforms &u, const float *gbuf, const float *shadow_map, const float *pairs,
+                      uint32_t *offs, uint32_t *total, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_pcss_walk, dim3(1), dim3(64), 0, s, u, gbuf, shadow_map, pairs, offs, total);
 }
 
 void launch_ssao(const FrameUniforms &u, const SsaoArgs &sa, const float *depth,

@@ -12,8 +12,8 @@ namespace fgl {
 // (12 B/px).
 __device__ __forceinline__ void lighting_px(const FrameUniforms &u, const float *__restrict__ g,
                                             const float *__restrict__ ao, int ao_is_one,
-                                            const float *__restrict__ smap, float *__restrict__ fb,
-                                            int x, int y)
+                                            const float *__restrict__ smap, const ShadowSamples &ss,
+                                            float *__restrict__ fb, int x, int y)
 {
     const size_t P = (size_t)u.W * u.H;
     const size_t i = (size_t)x + (size_t)y * u.W;
@@ -26,7 +26,9 @@ __device__ __forceinline__ void lighting_px(const FrameUniforms &u, const float 
     float type = g[18 * P + i];
     // AmbientOcclusionGBuffer starts at 1 (forkergl.cpp:80); x * 1.f == x.
     if (!ao_is_one) pa.x *= ao[i];
-    Vec3 c = shade_point(u, smap, !(type < 0.5f), true, Pw, N, L, al, em, pa);
+    const float *rdisk = nullptr;
+    if (ss.pairs) rdisk = ss.pairs + 2 * (ss.offs ? (size_t)ss.offs[i] : i * (size_t)ss.stride);
+    Vec3 c = shade_point(u, smap, !(type < 0.5f), true, Pw, N, L, al, em, pa, rdisk);
     fb[i] = c.x;
     fb[P + i] = c.y;
     fb[2 * P + i] = c.z;
@@ -83,6 +85,47 @@ __device__ __forceinline__ void ssao_px(const FrameUniforms &u, const SsaoArgs &
     }
     const float a = 1.f - (float)count * (1.f / (float)sa.n_kernel);
     ao[i] = (a * a) * a;
+}
+
+// Render::DoSSAO (src/render.cpp:214-289) itself at pixel (x,y), with the
+// pixel's 32 hemisphere samples taken from the random stream (samples: 32
+// unit-sphere vectors per pixel, pixels in row order).  Each sample is
+// flipped to the normal's side (RandomVectorInHemisphere), scaled by
+// Lerp(0.1f, 1.0f, |d|^2) = (1 - 0.1f) * 1.0f + 0.1f * |d|^2 (Lerp(t, a, b),
+// src/utility.h:26-29), offset by 0.075 from the position, projected with
+// the view-projection and viewport matrices (v / w is v * (1/w)) and
+// truncated; the depth read is data[x + y * W] in 32-bit int arithmetic, so
+// an off-screen sample reads another pixel, or lies outside the buffer
+// (undefined in the reference; here no occlusion).  Each occluding sample
+// adds 1/32 (exact); ao = powf(1 - occlusion, 3) by table.
+__device__ __forceinline__ void ssao_ref_px(const FrameUniforms &u, const float *__restrict__ samples,
+                                            const float *__restrict__ pow3, const float *__restrict__ depth,
+                                            const float *__restrict__ g, float *__restrict__ ao, int x, int y)
+{
+    const size_t P = (size_t)u.W * u.H;
+    const size_t i = (size_t)x + (size_t)y * u.W;
+    const Vec3 N = vec3(g[i], g[P + i], g[2 * P + i]);
+    const Vec3 Pw = vec3(g[3 * P + i], g[4 * P + i], g[5 * P + i]);
+    const float fd = depth[i];
+    const float *smp = samples + i * 96;
+    int count = 0;
+    for (int s = 0; s < 32; ++s) {
+        Vec3 d = vec3(smp[3 * s], smp[3 * s + 1], smp[3 * s + 2]);
+        if (!(dot3(d, N) > 0.f)) d = vec3(-d.x, -d.y, -d.z);
+        float sc = len3(d);
+        sc = (1 - 0.1f) * 1.0f + 0.1f * (sc * sc);
+        d = scale3(d, sc);
+        const Vec3 ws = add3(Pw, scale3(d, 0.075f));
+        const Vec4 cs = mv4(u.vp, vec4(ws.x, ws.y, ws.z, 1.f));
+        const Vec4 ndc = divs4(cs, cs.w);
+        const Vec4 ss = mv4(u.viewport, ndc);
+        const int sx = trunc_int(ss.x), sy = trunc_int(ss.y);
+        const int idx = (int)((uint32_t)sx + (uint32_t)sy * (uint32_t)u.W);
+        if (idx < 0 || (size_t)idx >= P) continue;
+        const float cached = depth[idx];
+        if (ss.z >= cached + 0.0005f && fabsf(fd - cached) < 0.01f) ++count;
+    }
+    ao[i] = pow3[count];
 }
 
 // Buffer3f::GenerateImage (src/buffer.cpp:113-126) and Render::DoSSAA

@@ -55,6 +55,13 @@ extern "C" {
 #define FGL_SHADOW_PCF_DISK 1
 #define FGL_SHADOW_PCSS 2
 #define FGL_SHADOW_PCF_GRID 3
+/* The reference's own filters with their samples drawn from the reference's
+ * random stream (SOFT_SHADOW_PCF / SOFT_SHADOW_PCSS, src/shaders/shadow.h:14-23,
+ * shadow.cpp:47-106): n_pcf (and n_blocker) RandomVectorInUnitDisk samples per
+ * pixel from the context's std::mt19937 stream; the disk arrays are unused.
+ * Deferred lighting only. */
+#define FGL_SHADOW_REF_PCF 4
+#define FGL_SHADOW_REF_PCSS 5
 /* Render targets (ForkerGL's public buffers, src/forkergl.h:41-54) */
 #define FGL_TARGET_FRAME 0        /* FrameBuffer: 3 float planes            */
 #define FGL_TARGET_DEPTH 1        /* DepthBuffer: 1 float plane             */
@@ -120,6 +127,10 @@ typedef struct {
     int32_t blur;               /* 0 none, 1 reference recursive 9-tap, 2 separable */
     int32_t blur_radius;        /* separable: taps = 2*R+1, R <= 8 */
     float blur_weights[9];
+    int32_t stochastic;         /* 1: Render::DoSSAO exactly (render.cpp:214-289): 32
+                                   RandomVectorInHemisphere samples per pixel from the
+                                   context's random stream, radius 0.075, bias 0.0005,
+                                   range 0.01; the kernel fields are unused */
 } fgl_ssao_cfg;
 
 /* ---- context ---- */
@@ -180,6 +191,21 @@ int fgl_draw_screen_space_pixels(fgl_ctx *ctx);
 /* Render::DoSSAO (render.cpp:214-289) with a deterministic sample kernel, then the
  * configured blur (Buffer1f::TwoPassGaussianBlurDenoised, buffer.cpp:59-98, or separable) */
 int fgl_ssao(fgl_ctx *ctx, const fgl_ssao_cfg *cfg);
+/* The reference's random stream (src/utility.h:90-103: one function-static
+ * std::mt19937, default seed 5489, through uniform_real_distribution<float>;
+ * random vectors src/geometry.h:947-990).  The context owns one such stream on
+ * the device; stochastic SSAO and the FGL_SHADOW_REF_* filters consume it in
+ * the reference's order, and fgl_render_scene of a scene that draws from it
+ * restarts it at the seed first (one Render::Render run per process in the
+ * reference).  fgl_rng_seed sets the
+ * seed and restarts the stream; fgl_rng_draws reports the 32-bit draws
+ * consumed since the restart. */
+int fgl_rng_seed(fgl_ctx *ctx, uint32_t seed);
+int fgl_rng_draws(fgl_ctx *ctx, uint64_t *out_draws);
+/* n results of RandomVectorInUnitSphere (kind 3: x, y, z each) or
+ * RandomVectorInUnitDisk (kind 2: x, y each; src/geometry.h:957-976), drawn
+ * from the context's stream on the device, copied to host memory `out`. */
+int fgl_rng_vectors(fgl_ctx *ctx, int kind, uint32_t n, float *out);
 /* Buffer3f::GenerateImage (buffer.cpp:113-126) + Render::DoSSAA (render.cpp:291-343):
  * k = 1 quantises only; the image is (W/k) x (H/k) B,G,R bytes. */
 int fgl_resolve(fgl_ctx *ctx, int k);

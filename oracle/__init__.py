@@ -95,6 +95,14 @@ def lib():
             "orc_shade_pbr": (None, [P, P, P, P, C.c_float, P, P, P, P, C.c_int, P]),
             "orc_ssao_kernel": (None, [C.POINTER(Frame), C.c_int, C.c_int, P, P, P, P, C.c_int, P,
                                        C.c_float, C.c_float, C.c_float, P]),
+            "orc_ssao_reference": (None, [C.POINTER(Frame), C.c_int, C.c_int, P, P, P, P]),
+            "orc_rng_seed": (None, [C.c_uint]),
+            "orc_rng_draws": (C.c_ulonglong, []),
+            "orc_rng_raw": (None, [C.c_int, P]),
+            "orc_rng_uniform": (None, [C.c_int, P]),
+            "orc_rng_sphere": (None, [C.c_int, P]),
+            "orc_rng_disk": (None, [C.c_int, P]),
+            "orc_rng_hemisphere": (None, [P, P]),
             "orc_blur_recursive": (None, [P, C.c_int, C.c_int]),
             "orc_blur_separable": (None, [P, C.c_int, C.c_int, P, C.c_int, P]),
             "orc_quantize_rgb": (None, [P, C.c_int, C.c_int, P]),
@@ -314,6 +322,53 @@ def ssao_kernel(fr, W, H, depth, gbuf, cfg):
     return ao
 
 
+def ssao_reference(fr, W, H, depth, gbuf):
+    """Render::DoSSAO with the reference's random stream (continues the
+    oracle's stream; see rng_seed)."""
+    n = [np.ascontiguousarray(gbuf[i], np.float32) for i in range(3)]
+    p = [np.ascontiguousarray(gbuf[3 + i], np.float32) for i in range(3)]
+    npp = (C.c_void_p * 3)(*[a.ctypes.data for a in n])
+    ppp = (C.c_void_p * 3)(*[a.ctypes.data for a in p])
+    ao = np.empty(W * H, np.float32)
+    d = _f32(depth)
+    lib().orc_ssao_reference(C.byref(fr), W, H, _p(d), C.cast(npp, C.c_void_p), C.cast(ppp, C.c_void_p), _p(ao))
+    return ao
+
+
+# ---- the reference's random stream (oracle/ref_rng.cpp) ----
+
+def rng_seed(seed=5489):
+    lib().orc_rng_seed(seed)
+
+
+def rng_draws():
+    return int(lib().orc_rng_draws())
+
+
+def rng_raw(n):
+    out = np.empty(n, np.uint32)
+    lib().orc_rng_raw(n, _p(out))
+    return out
+
+
+def rng_uniform(n):
+    out = np.empty(n, np.float32)
+    lib().orc_rng_uniform(n, _p(out))
+    return out
+
+
+def rng_sphere(n):
+    out = np.empty((n, 3), np.float32)
+    lib().orc_rng_sphere(n, _p(out))
+    return out
+
+
+def rng_disk(n):
+    out = np.empty((n, 2), np.float32)
+    lib().orc_rng_disk(n, _p(out))
+    return out
+
+
 def blur_recursive(ao, W, H):
     a = _f32(ao).copy()
     lib().orc_blur_recursive(_p(a), W, H)
@@ -342,11 +397,13 @@ def ssaa_resolve(img, W, H, k):
     return out
 
 
-def render_frame(models, materials, textures, info, shadow_cfg=None, ssao_cfg=None, multilight_lights=None):
+def render_frame(models, materials, textures, info, shadow_cfg=None, ssao_cfg=None, multilight_lights=None,
+                 rng_start=5489):
     """Render::Render on the oracle (src/render.cpp:40-58), with the same
     extension settings the product reads from the .scene file.  `info` is a
     forkerrenderer_amd.SceneInfo-like object.  Returns a dict of buffers."""
     sc = OracleScene(models, materials, textures)
+    rng_seed(rng_start)   # one Render::Render run: the stream starts fresh
     k = info.ssaa_k if info.ssaa_on else 1
     W, H = info.width * k, info.height * k
     ratio = np.float32(info.width) / np.float32(info.height)
@@ -382,7 +439,10 @@ def render_frame(models, materials, textures, info, shadow_cfg=None, ssao_cfg=No
         out["depth"], out["gbuf"] = depth, g
         ao = np.ones(W * H, np.float32)
         if info.ssao_on:
-            ao = ssao_kernel(fr, W, H, depth, g, ssao_cfg)
+            if getattr(ssao_cfg, "stochastic", 0):
+                ao = ssao_reference(fr, W, H, depth, g)
+            else:
+                ao = ssao_kernel(fr, W, H, depth, g, ssao_cfg)
             if ssao_cfg.blur == 1:
                 ao = blur_recursive(ao, W, H)
             elif ssao_cfg.blur == 2:

@@ -17,6 +17,7 @@
 #include <float.h>
 #include <limits.h>
 #include <math.h>
+#include <stdint.h>
 #include <stdlib.h>
 #include <string.h>
 
@@ -513,6 +514,38 @@ float orc_shadow_visibility(const orc_frame *fr, const float *map, int SW, int S
     V3 sc = add3(scale3(ndc, 0.5f), v3(0.5f, 0.5f, 0.5f));
     float bias = smax(0.009f * (1.f - dot3(v3(n3[0], n3[1], n3[2]), v3(l3[0], l3[1], l3[2]))), 0.007f);
     if (c->mode == 1) return pcf_disk(map, SW, SH, sc, bias, c->pcf_filter, c->pcf_disk, c->n_pcf);
+    if (c->mode == 4) {
+        /* SOFT_SHADOW_PCF with the reference's random stream: PCF
+         * (shadow.cpp:47-63) draws n_pcf RandomVectorInUnitDisk samples, one
+         * per iteration (nothing else draws in between, so drawing them
+         * first consumes the stream identically). */
+        float disk[2 * 64];
+        orc_rng_disk(c->n_pcf, disk);
+        return pcf_disk(map, SW, SH, sc, bias, c->pcf_filter, disk, c->n_pcf);
+    }
+    if (c->mode == 5) {
+        /* SOFT_SHADOW_PCSS with the reference's random stream
+         * (shadow.cpp:65-106): n_blocker blocker-search samples, and n_pcf
+         * PCF samples only when a blocker was found. */
+        float bd[2 * 64], pd[2 * 64];
+        orc_rng_disk(c->n_blocker, bd);
+        float blocker = 0.f, nb = 0.f;
+        for (int i = 0; i < c->n_blocker; ++i) {
+            float u = sc.x + (float)((double)bd[2 * i] * c->blocker_filter);
+            float v = sc.y + (float)((double)bd[2 * i + 1] * c->blocker_filter);
+            float s = sample_shadow(map, SW, SH, u, v);
+            if (sc.z > s + bias) {
+                blocker += s;
+                nb += 1.f;
+            }
+        }
+        float dB = (nb < 1.f) ? 0.f : blocker / nb;
+        if ((double)dB < 0.001) return 1.f;
+        float pen = (sc.z - dB) * c->area_light / dB;
+        float filter = (float)(c->pcss_filter * (double)pen);
+        orc_rng_disk(c->n_pcf, pd);
+        return pcf_disk(map, SW, SH, sc, bias, filter, pd, c->n_pcf);
+    }
     if (c->mode == 2) {
         /* FindAverageBlockDepth + PCSS (shadow.cpp:65-106) */
         float blocker = 0.f, nb = 0.f;
@@ -929,6 +962,59 @@ void orc_lighting_pass(const orc_frame *fr, int W, int H, const float *const *g,
 }
 
 /* ---------------- SSAO, blur, resolve ---------------- */
+
+/* Render::DoSSAO (render.cpp:214-289) with the reference's random stream
+ * (ref_rng.cpp): every pixel in row order (y outer, x inner) draws 32
+ * RandomVectorInHemisphere(normal) samples, scaled by
+ * Lerp(0.1f, 1.0f, |d|^2) = (1 - 0.1f) * 1.0f + 0.1f * |d|^2
+ * (Lerp(t, v1, v2), utility.h:26-29), radius 0.075f, projected with the
+ * view-projection and viewport matrices (v / w is v * (1/w)), truncated to
+ * Point2i; the depth read is DepthBuffer.GetValue(x, y) = data[x + y * W]
+ * with 32-bit int arithmetic, which for an off-screen sample still lands
+ * inside the buffer (another pixel) or outside it (undefined in the
+ * reference: here no occlusion).  Occlusion adds 1/32 per sample past the
+ * 0.0005 bias with |frag - cached| < 0.01; ao = powf(1 - occlusion, 3). */
+void orc_ssao_reference(const orc_frame *fr, int W, int H, const float *depth,
+                        const float *const *nrm, const float *const *pos, float *ao)
+{
+    float VP[16];
+    orc_mat_mul(fr->proj, fr->view, VP);
+    const float *vpm = fr->viewport;
+    const int numSample = 32;
+    const float kernelScale = 1.f / numSample;
+    const float radius = 0.075f, rangeCheckRadius = 0.01f, bias = 0.0005f;
+    const int64_t n_px = (int64_t)W * H;
+    for (int y = 0; y < H; ++y)
+        for (int x = 0; x < W; ++x) {
+            size_t i = (size_t)x + (size_t)y * W;
+            V3 P = v3(pos[0][i], pos[1][i], pos[2][i]);
+            float nn[3] = {nrm[0][i], nrm[1][i], nrm[2][i]};
+            float fd = depth[i];
+            float occ = 0.f;
+            for (int s = 0; s < numSample; ++s) {
+                float d[3];
+                orc_rng_hemisphere(nn, d);
+                V3 dir = v3(d[0], d[1], d[2]);
+                float scale = len3(dir);
+                scale = (1 - 0.1f) * 1.0f + 0.1f * (scale * scale);
+                dir = scale3(dir, scale);
+                V3 ws = add3(P, scale3(dir, radius));
+                V4 cs = mv4(VP, (V4){ws.x, ws.y, ws.z, 1.f});
+                V4 ndc = divs4(cs, cs.w);
+                V4 ss = mv4(vpm, ndc);
+                int sx = orc_trunc_int(ss.x), sy = orc_trunc_int(ss.y);
+                int32_t idx = (int32_t)((uint32_t)sx + (uint32_t)sy * (uint32_t)W);
+                if (idx < 0 || (int64_t)idx >= n_px) continue;
+                float cached = depth[idx];
+                if (ss.z >= cached + bias) {
+                    float rc = fabsf(fd - cached) < rangeCheckRadius ? 1.f : 0.f;
+                    occ += kernelScale * rc;
+                }
+            }
+            occ = 1.f - occ;
+            ao[i] = powf(occ, 3);
+        }
+}
 
 /* Kernel SSAO (new, deterministic; the configs' replacement for the
  * RNG-driven Render::DoSSAO, render.cpp:214-289).  Per pixel: a tangent

@@ -68,7 +68,9 @@ typedef struct {
  * (src/shaders/shadow.cpp:38-45); mode 1 is PCF over a fixed disk sample set
  * (the reference's PCF with its RNG samples replaced by a given set,
  * shadow.cpp:47-63); mode 2 is PCSS with fixed blocker/PCF sets
- * (shadow.cpp:65-106); mode 3 is an n x n texel-grid PCF (new). */
+ * (shadow.cpp:65-106); mode 3 is an n x n texel-grid PCF (new); modes 4 and
+ * 5 are the reference's PCF and PCSS drawing their disk samples from the
+ * reference's random stream (ref_rng.cpp; the disk arrays are unused). */
 typedef struct {
     int mode;
     int grid;                     /* mode 3: n (odd) */
@@ -91,6 +93,16 @@ typedef struct {
     int n_lights;
     orc_light lights[4];
 } orc_frame;
+
+/* ---- the reference's random stream (ref_rng.cpp; src/utility.h:90-103,
+ * src/geometry.h:947-990) ---- */
+void orc_rng_seed(unsigned int seed);
+unsigned long long orc_rng_draws(void);
+void orc_rng_raw(int n, unsigned int *out);
+void orc_rng_uniform(int n, float *out);
+void orc_rng_sphere(int n, float *out);
+void orc_rng_disk(int n, float *out);
+void orc_rng_hemisphere(const float n[3], float out[3]);
 
 /* ---- L0 math (src/geometry.h, src/geometry.cpp) ---- */
 void orc_mat_mul(const float *a, const float *b, float *out);
@@ -130,6 +142,10 @@ void orc_shade_blinn_phong(const float *L, const float *H, const float *N, float
 void orc_shade_pbr(const float *L, const float *V, const float *Hv, const float *N, float vis,
                    const float *albedo, const float *emissive, const float *param,
                    const float *radiance, int shadow_on, float *out3);
+/* Render::DoSSAO with the reference's random stream (consumes 32 hemisphere
+ * samples per pixel from it) */
+void orc_ssao_reference(const orc_frame *fr, int W, int H, const float *depth,
+                        const float *const *nrm, const float *const *pos, float *ao);
 void orc_ssao_kernel(const orc_frame *fr, int W, int H, const float *depth,
                      const float *const *normal3, const float *const *pos3,
                      const float *kernel, int n_kernel, const float *noise16,

@@ -1,0 +1,111 @@
+"""The reference's random stream on the device against the oracle: the
+generator's unit-sphere and unit-disk vectors and draw counts (bit-exact,
+across calls that end mid-block), Render::DoSSAO with the stream (AO
+bit-exact), and whole deferred frames with the reference's PCF / PCSS filters
+drawing from the stream (G-buffer and AO bit-exact, the stream consumed to
+the same draw, image within the usual +-1 LSB)."""
+import os
+
+import numpy as np
+import pytest
+
+import forkerrenderer_amd as F
+from forkerrenderer_amd import scenegen as G
+import oracle as O
+from tests.test_oracle_rng import mt_raw, stream_disk, stream_sphere
+from tests.util import assert_bitexact, gpu_frame, lsb_stats, oracle_frame
+
+pytestmark = pytest.mark.gpu
+
+
+def test_device_stream_matches_the_oracle(ctx):
+    ctx.rng_seed(5489)
+    sph = ctx.rng_vectors(3, 20000)
+    raw = mt_raw(400000)
+    want, used = stream_sphere(raw, 20000)
+    assert_bitexact("sphere", sph, want)
+    assert ctx.rng_draws() == used
+    dsk = ctx.rng_vectors(2, 30000)
+    want_d, used_d = stream_disk(raw[used:], 30000)
+    assert_bitexact("disk", dsk, want_d)
+    assert ctx.rng_draws() == used + used_d
+    # short requests that end inside a 624-word block, then a long one
+    ctx.rng_seed(77)
+    O.rng_seed(77)
+    for kind, n in ((3, 1), (2, 1), (3, 7), (2, 5), (3, 150), (2, 333), (3, 5000), (2, 1), (3, 2)):
+        got = ctx.rng_vectors(kind, n)
+        want = O.rng_sphere(n) if kind == 3 else O.rng_disk(n)
+        assert_bitexact(f"kind {kind} n {n}", got, want)
+        assert ctx.rng_draws() == O.rng_draws(), (kind, n)
+    ctx.rng_seed(5489)
+
+
+def _deferred_targets(ctx, sc):
+    """Geometry pass of a scene through the context (the frame's matrices and
+    lights set as fgl_render_scene sets them); the frame's stream starts at
+    the reference's default seed."""
+    ctx.rng_seed(5489)
+    gpu_frame(ctx, sc)
+    return ctx.read(F.TARGET_DEPTH), ctx.read(F.TARGET_GBUFFER).reshape(F.GBUF_PLANES, -1)
+
+
+def test_reference_ssao_matches_the_oracle(ctx, scene_dir):
+    p = G.tiny_scene(os.path.join(scene_dir, "rssao"), seed=31, width=64, height=48, subdiv=3, tex=16,
+                     ssao=True, extra="ssaokernel reference\naoblur none")
+    sc = F.Scene(p)
+    I = sc.info
+    W, H = I.width, I.height
+    depth, g = _deferred_targets(ctx, sc)
+    want = oracle_frame(sc)
+    assert_bitexact("depth", depth, want["depth"])
+    assert_bitexact("gbuf", g, want["gbuf"])
+    # the frame's own SSAO ran with the stream from 5489: AO equals the oracle's
+    ao_frame = ctx.read(F.TARGET_AO)
+    assert_bitexact("frame ao", ao_frame, want["ao"])
+    assert len(np.unique(want["ao"])) > 5, "AO too uniform to test"
+    # again on the same targets, the stream continued by one call
+    fr = O.make_frame(np.asarray(ctx_view(sc)), proj(sc), np.eye(4), O.make_viewport(0, 0, W, H),
+                      O.make_viewport(0, 0, W, H), I.eye, 0, None, [])
+    ctx.rng_seed(1234)
+    O.rng_seed(1234)
+    cfg = F.SsaoCfg.from_buffer_copy(I.ssao)
+    cfg.blur = 0
+    for rep in range(2):
+        ctx.ssao(cfg)
+        got = ctx.read(F.TARGET_AO)
+        ref = O.ssao_reference(fr, W, H, depth, g)
+        assert_bitexact(f"ao call {rep}", got, ref)
+        assert ctx.rng_draws() == O.rng_draws()
+
+
+def ctx_view(sc):
+    I = sc.info
+    return O.make_lookat(list(I.eye), list(I.look_at))
+
+
+def proj(sc):
+    I = sc.info
+    ratio = np.float32(I.width) / np.float32(I.height)
+    return O.make_perspective(45.0, ratio, 0.01, 20.0)
+
+
+@pytest.mark.parametrize("filt", ["refpcf", "refpcss"])
+def test_stochastic_frames_match_the_oracle(ctx, scene_dir, filt):
+    p = G.tiny_scene(os.path.join(scene_dir, "r" + filt), seed=32, width=64, height=48, subdiv=3, tex=16,
+                     ssao=True, extra=f"ssaokernel reference\naoblur reference\nshadowfilter {filt}")
+    sc = F.Scene(p)
+    assert sc.info.shadow.mode == (F.SHADOW_REF_PCF if filt == "refpcf" else F.SHADOW_REF_PCSS)
+    print(f"stochastic frame {filt}", flush=True)
+    ctx.rng_seed(5489)
+    got = gpu_frame(ctx, sc)
+    device_draws = ctx.rng_draws()
+    want = oracle_frame(sc)
+    assert device_draws == O.rng_draws()
+    assert_bitexact("shadow", got["shadow"], want["shadow"])
+    assert_bitexact("gbuf", got["gbuf"], want["gbuf"])
+    assert_bitexact("ao", got["ao"], want["ao"])
+    mx, frac = lsb_stats(got["image"], want["image"])
+    assert frac >= 0.999, (mx, frac)
+    # the filter's samples were consumed after SSAO's
+    P = sc.info.width * sc.info.height
+    assert device_draws > 32 * P * 3

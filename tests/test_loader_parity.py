@@ -130,3 +130,21 @@ def test_rle_packet_structure(tmp_path):
     F.tga_write(p, row, rle=True)
     body = open(p, "rb").read()[18:-26]
     assert body == bytes([0x80 | 4, 1, 1, 1, 2, 2, 2, 2, 3, 3, 3, 4, 4, 4])
+
+
+def test_stochastic_scene_keys(scene_dir):
+    """`ssaokernel reference` selects Render::DoSSAO with the random stream;
+    `shadowfilter refpcf|refpcss` the reference's PCF / PCSS with its
+    parameters (shadow.h:18-22: 64 PCF and 32 blocker samples, filter sizes
+    0.007 and 0.005, area light 2.5)."""
+    for filt, mode in (("refpcf", F.SHADOW_REF_PCF), ("refpcss", F.SHADOW_REF_PCSS)):
+        p = G.tiny_scene(os.path.join(scene_dir, "k" + filt), seed=1, ssao=True,
+                         extra=f"ssaokernel reference\nshadowfilter {filt}")
+        I = F.Scene(p).info
+        assert I.ssao.stochastic == 1
+        s = I.shadow
+        assert (s.mode, s.n_blocker, s.n_pcf) == (mode, 32, 64)
+        assert s.pcf_filter == np.float32(0.007) and s.blocker_filter == 0.005 and s.pcss_filter == 0.007
+        assert s.area_light == np.float32(2.5)
+    p = G.tiny_scene(os.path.join(scene_dir, "kdet"), seed=1, ssao=True)
+    assert F.Scene(p).info.ssao.stochastic == 0
