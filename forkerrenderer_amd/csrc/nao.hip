@@ -92,23 +92,6 @@ struct Mat16 {
     float m[16];
 };
 
-// Full frame -> features [H][W][16] (BF16) + coverage mask.
-__global__ __launch_bounds__(256) void k_features(Mat16 V, const float *__restrict__ depth,
-                                                  const float *__restrict__ g, int W, int H,
-                                                  uint16_t *__restrict__ feat,
-                                                  uint8_t *__restrict__ mask)
-{
-    const int x = blockIdx.x * 64 + (threadIdx.x & 63), y = blockIdx.y * 4 + (threadIdx.x >> 6);
-    if (x >= W || y >= H) return;
-    const size_t P = (size_t)W * H, i = (size_t)x + (size_t)y * W;
-    float f[7];
-    bool cov;
-    pixel_features(V.m, depth, g, P, i, f, cov);
-    uint16_t *o = feat + i * C0;
-    for (int c = 0; c < C0; ++c) o[c] = c < 7 ? f2bf(f[c]) : (uint16_t)0;
-    if (mask) mask[i] = cov ? 1 : 0;
-}
-
 // One crop of the current frame into batch slot `slot`: features, AO target,
 // mask.  Crop origin (x0, y0), size S x S, inside the frame.
 __global__ __launch_bounds__(256) void k_crop(Mat16 V, const float *__restrict__ depth,
@@ -133,9 +116,45 @@ __global__ __launch_bounds__(256) void k_crop(Mat16 V, const float *__restrict__
 
 // ---------------- 3x3 convolution on MFMA ----------------
 
+// First layer with the features computed while staging (inference): the
+// G-buffer planes and depth of the image set, the view matrix (single-image
+// launches; batched views take their own), and the coverage mask it writes.
+struct FeatArgs {
+    const float *depth, *gbuf;
+    uint8_t *mask;
+    Mat16 V;
+};
+
+// Head pre-activation: conv1x1 32 -> 1 over one pixel's BF16 activations, in
+// channel order.
+__device__ __forceinline__ float head_z(const float *w5, float b5, const uint16_t *a)
+{
+    float z = b5;
+    for (int c = 0; c < CH; ++c) z += w5[c] * bf2f(a[c]);
+    return z;
+}
+
+// Head of the network fused into the last convolution (inference): w5 = the
+// 32 head weights followed by the bias; pixels whose mask is 0 keep AO 1.
+struct HeadArgs {
+    const float *w5;
+    const uint8_t *mask;
+    float *ao;
+};
+
 // MODE 0: out = bf16(relu(conv(in) + bias)).
 // MODE 1 (data gradient): out = bf16(conv(in) * (act > 0)) with tap-flipped,
 // transposed weights; `in` is the layer's output gradient.
+// MODE 3 (first layer of inference): as MODE 0 with CIN 16, the input tile
+// computed from the G-buffer while staging (pixel_features, as the training
+// crops get them: 7 features, BF16, channels 7-15 zero except channel 15 =
+// coverage, whose weights are held at zero), and the coverage mask written
+// for the tile's pixels.
+// MODE 2 (last layer of inference): as MODE 0, but each row of outputs goes
+// to the wavefront's LDS row buffer and the head -- head_z, then
+// 1 / (1 + expf(-z)), AO 1 where the mask is 0 (the background keeps
+// AmbientOcclusionGBuffer's initial value) -- writes AO; the layer's
+// activations never reach HBM.
 //
 // Implicit GEMM on v_mfma_f32_16x16x32_bf16 with D = W x patches: M = 16
 // output channels, N = 16 pixels, K = 32 (one tap of 32 channels, or two taps
@@ -185,6 +204,7 @@ constexpr int kStagePix = 40;  // bf16 per staged pixel (32 channels + 8 pad)
 struct ConvTile {
     size_t img;
     int W, H, phx, phy, bx, by;
+    const float *V;   // view matrix (first layer with features)
 };
 
 template <int CIN>
@@ -228,11 +248,100 @@ __device__ __forceinline__ void stage_store(const ConvStage<CIN> &st, uint16_t *
     }
 }
 
+// MODE 3 staging: one thread per staged pixel (3 rounds over the 66 x 10
+// pixels), both 16-byte chunks of it.
+__device__ __forceinline__ void stage_load_feat(ConvStage<C0> &st, const FeatArgs &fa, const ConvTile &t, int dil)
+{
+    static_assert(ConvStage<C0>::N >= 6, "two chunks for each of three pixels");
+    const int Wp = (t.W - t.phx + dil - 1) / dil, Hp = (t.H - t.phy + dil - 1) / dil;
+    const int i0 = t.bx * kTileW - 1, j0 = t.by * kTileH - 1;
+    const size_t P = (size_t)t.W * t.H;
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+        const int pix = (int)threadIdx.x + 256 * q;
+        st.v[2 * q] = make_uint4(0u, 0u, 0u, 0u);
+        st.v[2 * q + 1] = make_uint4(0u, 0u, 0u, 0u);
+        const int i = i0 + pix % kStageW, j = j0 + pix / kStageW;
+        if (pix < kStageW * kStageH && i >= 0 && i < Wp && j >= 0 && j < Hp) {
+            const size_t k = (size_t)(t.phy + j * dil) * t.W + t.phx + i * dil;
+            float f[7];
+            bool cov;
+            pixel_features(t.V, fa.depth + t.img, fa.gbuf + (size_t)kGbufPlanes * t.img, P, k, f, cov);
+            uint16_t b[8];
+#pragma unroll
+            for (int c = 0; c < 7; ++c) b[c] = f2bf(f[c]);
+            b[7] = 0;
+            st.v[2 * q] = make_uint4(b[0] | ((uint32_t)b[1] << 16), b[2] | ((uint32_t)b[3] << 16),
+                                     b[4] | ((uint32_t)b[5] << 16), b[6] | ((uint32_t)b[7] << 16));
+            st.v[2 * q + 1] = make_uint4(0u, 0u, 0u, cov ? 0x3f800000u : 0u);   // channel 15 = 1.0
+        }
+    }
+}
+
+__device__ __forceinline__ void stage_store_feat(const ConvStage<C0> &st, uint16_t *s_in)
+{
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+        const int pix = (int)threadIdx.x + 256 * q;
+        if (pix < kStageW * kStageH) {
+            *(uint4 *)(s_in + pix * kStagePix) = st.v[2 * q];
+            *(uint4 *)(s_in + pix * kStagePix + 8) = st.v[2 * q + 1];
+        }
+    }
+}
+
+// MODE 2 epilogue of one output row (64 pixels of the phase grid row y): the
+// BF16 activations bf16(relu(acc + bias)) of 32 pixels at a time go to this
+// wavefront's LDS rows (32 x 80 B), then lanes 0-31 take one pixel each
+// through the head.  LDS operations of one wavefront execute in order.
+__device__ __forceinline__ void head_row(const v4f (&acc)[4][2], const float *__restrict__ bias,
+                                         const HeadArgs &hd, uint16_t *sh, const ConvTile &t, int y, int Wp,
+                                         int dil)
+{
+    const int lane = threadIdx.x & 63, lr = lane & 15, lq = lane >> 4;
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+#pragma unroll
+        for (int q2 = 0; q2 < 2; ++q2) {
+            const int pb = 2 * h + q2;
+#pragma unroll
+            for (int cb = 0; cb < 2; ++cb) {
+                const int co = cb * 16 + lq * 4;
+                const float4 bv = *(const float4 *)(bias + co);
+                const float bb[4] = {bv.x, bv.y, bv.z, bv.w};
+                uint16_t rr[4];
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    const float v = acc[pb][cb][q] + bb[q];
+                    rr[q] = f2bf(v > 0.f ? v : 0.f);
+                }
+                *(uint64_t *)(sh + (q2 * 16 + lr) * kStagePix + co) =
+                    (uint64_t)rr[0] | ((uint64_t)rr[1] << 16) | ((uint64_t)rr[2] << 32) | ((uint64_t)rr[3] << 48);
+            }
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        const int i = t.bx * kTileW + 32 * h + lane;
+        if (lane < 32 && i < Wp) {
+            const size_t p = t.img + (size_t)y * t.W + t.phx + (size_t)i * dil;
+            const float z = head_z(hd.w5, hd.w5[CH], sh + lane * kStagePix);
+            float v = 1.f / (1.f + expf(-z));
+            if (!hd.mask[p]) v = 1.f;
+            hd.ao[p] = v;
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    }
+}
+
 // The staged tile through the 3x3 taps: wave w computes rows 2w, 2w+1.
 template <int CIN, int MODE>
 __device__ __forceinline__ void conv_tile_compute(const ConvWeights<CIN> &cw, const uint16_t *s_in,
                                                   const float *__restrict__ bias, const uint16_t *__restrict__ act,
-                                                  uint16_t *__restrict__ out, const ConvTile &t, int dil)
+                                                  uint16_t *__restrict__ out, const ConvTile &t, int dil,
+                                                  const HeadArgs &hd, uint16_t *s_head, const FeatArgs &fa)
 {
     constexpr int K = ConvWeights<CIN>::K, KC = ConvWeights<CIN>::KC;
     const int Wp = (t.W - t.phx + dil - 1) / dil, Hp = (t.H - t.phy + dil - 1) / dil;
@@ -264,6 +373,10 @@ __device__ __forceinline__ void conv_tile_compute(const ConvWeights<CIN> &cw, co
             }
         }
         const int y = t.phy + j * dil;
+        if (MODE == 2) {
+            head_row(acc, bias, hd, s_head + wave * (32 * kStagePix), t, y, Wp, dil);
+            continue;
+        }
 #pragma unroll
         for (int pb = 0; pb < 4; ++pb) {
             const int i = t.bx * kTileW + pb * 16 + lr;
@@ -274,7 +387,7 @@ __device__ __forceinline__ void conv_tile_compute(const ConvWeights<CIN> &cw, co
                 const int co = cb * 16 + lq * 4;
                 const size_t o = (t.img + (size_t)y * t.W + x) * CH + co;
                 uint16_t rr[4];
-                if (MODE == 0) {
+                if (MODE == 0 || MODE == 3) {
                     const float4 bv = *(const float4 *)(bias + co);
                     const float bb[4] = {bv.x, bv.y, bv.z, bv.w};
 #pragma unroll
@@ -289,6 +402,11 @@ __device__ __forceinline__ void conv_tile_compute(const ConvWeights<CIN> &cw, co
                 }
                 *(uint64_t *)(out + o) = (uint64_t)rr[0] | ((uint64_t)rr[1] << 16) | ((uint64_t)rr[2] << 32) |
                                          ((uint64_t)rr[3] << 48);
+            }
+            if (MODE == 3 && lq == 0) {
+                // coverage of the pixel, staged as its channel 15
+                const uint16_t cv = s_in[((jr + 1) * kStageW + pb * 16 + lr + 1) * kStagePix + 15];
+                fa.mask[t.img + (size_t)y * t.W + x] = cv != 0 ? 1 : 0;
             }
         }
     }
@@ -308,6 +426,7 @@ __device__ __forceinline__ ConvTile single_tile(int t, int W, int H, int dil, in
     ct.phy = ph / dil;
     ct.bx = k % tiles_x;
     ct.by = k / tiles_x;
+    ct.V = nullptr;
     return ct;
 }
 
@@ -325,40 +444,48 @@ __device__ __forceinline__ ConvTile batch_tile(const BatchPixView *__restrict__ 
     ct.phy = ph / dil;
     ct.bx = (int)(it.y & 0xffff);
     ct.by = (int)(it.y >> 16);
+    ct.V = bv.view;
     return ct;
 }
 
 // Persistent workgroups stride over the tiles; the next tile's global reads
 // are in flight while the current tile is computed from LDS.
 template <int CIN, int MODE, bool BATCH>
-__global__ __launch_bounds__(256) void k_conv3x3_tiled(const uint16_t *__restrict__ in,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k_conv3x3_tiled(const uint16_t *__restrict__ in,
                                                        const uint16_t *__restrict__ Wt,
                                                        const float *__restrict__ bias,
                                                        const uint16_t *__restrict__ act,
                                                        uint16_t *__restrict__ out, int W, int H, int dil,
                                                        int n_tiles, int tiles_x, int tiles_y,
                                                        const BatchPixView *__restrict__ views,
-                                                       const uint2 *__restrict__ items)
+                                                       const uint2 *__restrict__ items, HeadArgs hd,
+                                                       FeatArgs fa)
 {
     __shared__ __attribute__((aligned(16))) uint16_t s_in[kStageH * kStageW * kStagePix];
+    __shared__ __attribute__((aligned(16))) uint16_t s_head[MODE == 2 ? 4 * 32 * kStagePix : 8];
     ConvWeights<CIN> cw;
     load_weights<CIN>(cw, Wt);
     int t = blockIdx.x;
     if (t >= n_tiles) return;
     ConvTile cur = BATCH ? batch_tile(views, items, t, dil) : single_tile(t, W, H, dil, tiles_x, tiles_y);
+    if (MODE == 3 && !BATCH) cur.V = fa.V.m;
     ConvStage<CIN> st;
-    stage_load<CIN>(st, in, cur, dil);
+    if constexpr (MODE == 3) stage_load_feat(st, fa, cur, dil);
+    else stage_load<CIN>(st, in, cur, dil);
     for (;;) {
         __syncthreads();   // the previous tile is no longer read from LDS
-        stage_store<CIN>(st, s_in);
+        if constexpr (MODE == 3) stage_store_feat(st, s_in);
+        else stage_store<CIN>(st, s_in);
         __syncthreads();
         const int nt = t + (int)gridDim.x;
         ConvTile nxt = cur;
         if (nt < n_tiles) {
             nxt = BATCH ? batch_tile(views, items, nt, dil) : single_tile(nt, W, H, dil, tiles_x, tiles_y);
-            stage_load<CIN>(st, in, nxt, dil);
+            if (MODE == 3 && !BATCH) nxt.V = fa.V.m;
+            if constexpr (MODE == 3) stage_load_feat(st, fa, nxt, dil);
+            else stage_load<CIN>(st, in, nxt, dil);
         }
-        conv_tile_compute<CIN, MODE>(cw, s_in, bias, act, out, cur, dil);
+        conv_tile_compute<CIN, MODE>(cw, s_in, bias, act, out, cur, dil, hd, s_head, fa);
         if (nt >= n_tiles) break;
         t = nt;
         cur = nxt;
@@ -396,38 +523,45 @@ static void launch_conv(hipStream_t s, const uint16_t *in, const uint16_t *Wt, c
     conv_tiles(W, H, N, dil, tx, ty, n);
     if (n == 0) return;
     hipLaunchKernelGGL((k_conv3x3_tiled<CIN, MODE, false>), dim3(conv_workgroups(n)), dim3(256), 0, s, in, Wt, bias,
-                       act, out, W, H, dil, n, tx, ty, (const BatchPixView *)nullptr, (const uint2 *)nullptr);
+                       act, out, W, H, dil, n, tx, ty, (const BatchPixView *)nullptr, (const uint2 *)nullptr,
+                       HeadArgs{nullptr, nullptr, nullptr}, FeatArgs{});
 }
 
-template <int CIN>
+// First layer with the features computed while staging (single image set).
+static void launch_conv_feat(hipStream_t s, const FeatArgs &fa, const uint16_t *Wt, const float *bias, uint16_t *out,
+                             int W, int H, int dil)
+{
+    int tx, ty, n;
+    conv_tiles(W, H, 1, dil, tx, ty, n);
+    if (n == 0) return;
+    hipLaunchKernelGGL((k_conv3x3_tiled<C0, 3, false>), dim3(conv_workgroups(n)), dim3(256), 0, s,
+                       (const uint16_t *)nullptr, Wt, bias, (const uint16_t *)nullptr, out, W, H, dil, n, tx, ty,
+                       (const BatchPixView *)nullptr, (const uint2 *)nullptr, HeadArgs{nullptr, nullptr, nullptr}, fa);
+}
+
+// Last layer with the head fused (single image set of N images).
+static void launch_conv_head(hipStream_t s, const uint16_t *in, const uint16_t *Wt, const float *bias,
+                             const HeadArgs &hd, int W, int H, int N, int dil)
+{
+    int tx, ty, n;
+    conv_tiles(W, H, N, dil, tx, ty, n);
+    if (n == 0) return;
+    hipLaunchKernelGGL((k_conv3x3_tiled<CH, 2, false>), dim3(conv_workgroups(n)), dim3(256), 0, s, in, Wt, bias,
+                       (const uint16_t *)nullptr, (uint16_t *)nullptr, W, H, dil, n, tx, ty,
+                       (const BatchPixView *)nullptr, (const uint2 *)nullptr, hd, FeatArgs{});
+}
+
+template <int CIN, int MODE = 0>
 static void launch_conv_batch(hipStream_t s, const BatchPixView *views, const uint2 *items, int n, const uint16_t *in,
-                              const uint16_t *Wt, const float *bias, uint16_t *out, int dil)
+                              const uint16_t *Wt, const float *bias, uint16_t *out, int dil,
+                              const HeadArgs &hd = HeadArgs{nullptr, nullptr, nullptr},
+                              const FeatArgs &fa = FeatArgs{})
 {
     if (n == 0) return;
-    hipLaunchKernelGGL((k_conv3x3_tiled<CIN, 0, true>), dim3(conv_workgroups(n)), dim3(256), 0, s, in, Wt, bias,
-                       (const uint16_t *)nullptr, out, 0, 0, dil, n, 0, 0, views, items);
+    hipLaunchKernelGGL((k_conv3x3_tiled<CIN, MODE, true>), dim3(conv_workgroups(n)), dim3(256), 0, s, in, Wt, bias,
+                       (const uint16_t *)nullptr, out, 0, 0, dil, n, 0, 0, views, items, hd, fa);
 }
 
-// Batched views: 64 x 4 pixel block b of the batch belongs to view
-// blk_view[b] (local block b - blk_base, row-major over the view's blocks).
-__device__ __forceinline__ bool batch_block_at(const BatchPixView *__restrict__ views,
-                                               const uint8_t *__restrict__ blk_view, uint32_t blk,
-                                               int &v, int &x, int &y)
-{
-    v = blk_view[blk];
-    const BatchPixView &bv = views[v];
-    const int lb = (int)(blk - bv.blk_base), nbx = (bv.W + 63) / 64;
-    x = (lb % nbx) * 64 + (threadIdx.x & 63);
-    y = (lb / nbx) * 4 + (threadIdx.x >> 6);
-    return y < bv.H;
-}
-
-__device__ __forceinline__ bool batch_block(const BatchPixView *__restrict__ views,
-                                            const uint8_t *__restrict__ blk_view, int &v, int &x,
-                                            int &y)
-{
-    return batch_block_at(views, blk_view, blockIdx.x, v, x, y);
-}
 
 // Weight gradient: dW[co][tap*CIN+ci] = sum_p dout[p][co] * in[p+off(tap)][ci]
 // and db[co] = sum_p dout[p][co], per workgroup over a 64 x WG_ROWS pixel
@@ -533,64 +667,7 @@ __global__ __launch_bounds__(256) void k_reduce_cols(const float *__restrict__ p
     out[i] = s;
 }
 
-// ---------------- head: 1x1 conv 32->1 + sigmoid ----------------
-
-__device__ __forceinline__ float head_z(const float *w5, float b5, const uint16_t *a)
-{
-    float z = b5;
-    for (int c = 0; c < CH; ++c) z += w5[c] * bf2f(a[c]);
-    return z;
-}
-
-__global__ __launch_bounds__(256) void k_head_fwd(const float *__restrict__ params,
-                                                  const uint16_t *__restrict__ a4,
-                                                  const uint8_t *__restrict__ mask, size_t n,
-                                                  float *__restrict__ y)
-{
-    const size_t p = (size_t)blockIdx.x * 256 + threadIdx.x;
-    if (p >= n) return;
-    const float z = head_z(params + OFF_W5, params[OFF_B5], a4 + p * CH);
-    float v = 1.f / (1.f + expf(-z));
-    // background pixels keep AO = 1 (AmbientOcclusionGBuffer's initial value)
-    if (mask && !mask[p]) v = 1.f;
-    y[p] = v;
-}
-
-__global__ __launch_bounds__(256) void k_features_batch(const BatchPixView *__restrict__ views,
-                                                        const uint8_t *__restrict__ blk_view,
-                                                        const float *__restrict__ depth,
-                                                        const float *__restrict__ gbuf,
-                                                        uint16_t *__restrict__ feat,
-                                                        uint8_t *__restrict__ mask)
-{
-    int v, x, y;
-    if (!batch_block(views, blk_view, v, x, y)) return;
-    const BatchPixView &bv = views[v];
-    if (x >= bv.W) return;
-    const size_t P = (size_t)bv.W * bv.H, i = (size_t)x + (size_t)y * bv.W;
-    float f[7];
-    bool cov;
-    pixel_features(bv.view, depth + bv.px_off, gbuf + kGbufPlanes * bv.px_off, P, i, f, cov);
-    uint16_t *o = feat + (bv.px_off + i) * C0;
-    for (int c = 0; c < C0; ++c) o[c] = c < 7 ? f2bf(f[c]) : (uint16_t)0;
-    mask[bv.px_off + i] = cov ? 1 : 0;
-}
-
-__global__ __launch_bounds__(256) void k_head_batch(const BatchPixView *__restrict__ views,
-                                                    const uint8_t *__restrict__ blk_view,
-                                                    const float *__restrict__ params,
-                                                    const uint16_t *__restrict__ a4,
-                                                    const uint8_t *__restrict__ mask,
-                                                    float *__restrict__ ao)
-{
-    int v, x, y;
-    if (!batch_block(views, blk_view, v, x, y)) return;
-    const BatchPixView &bv = views[v];
-    if (x >= bv.W) return;
-    const size_t p = bv.px_off + (size_t)x + (size_t)y * bv.W;
-    const float z = head_z(params + OFF_W5, params[OFF_B5], a4 + p * CH);
-    ao[p] = mask[p] ? 1.f / (1.f + expf(-z)) : 1.f;
-}
+// ---------------- head backward (training) ----------------
 
 // Loss = sum over covered pixels of (y - t)^2 / n_total.  Writes the data
 // gradient of a4 and per-block partials of [dW5 (32), db5, loss, |err|, count].
@@ -645,7 +722,10 @@ __global__ __launch_bounds__(256) void k_prep_weights(const float *__restrict__ 
                                                       uint16_t *__restrict__ wt234)
 {
     const int i = blockIdx.x * 256 + threadIdx.x;
-    if (i < CH * K1) w1[i] = f2bf(params[OFF_W1 + i]);
+    // layer 1: only the 7 real input channels; the padding channels' weights
+    // are held at zero (their inputs are zero, except channel 15, which
+    // carries the coverage in the fused inference layer)
+    if (i < CH * K1) w1[i] = (i % K1) % C0 < 7 ? f2bf(params[OFF_W1 + i]) : (uint16_t)0;
     if (i < 3 * CH * K2) {
         const int l = i / (CH * K2), r = i % (CH * K2);
         const int offs[3] = {OFF_W2, OFF_W3, OFF_W4};
@@ -698,7 +778,7 @@ struct fgl_nao {
     float *part = nullptr, *part_db = nullptr, *head_part = nullptr, *head_red = nullptr;
     size_t part_cap = 0;
     // inference buffers (grow-only)
-    uint16_t *ifeat = nullptr, *ia = nullptr, *ib = nullptr;
+    uint16_t *ia = nullptr, *ib = nullptr;
     uint8_t *imask = nullptr;
     size_t icap = 0;
     float *h_stats = nullptr;
@@ -717,14 +797,15 @@ int nao_reserve(fgl_nao *n, size_t px)
 {
     if (px <= n->icap) return FGL_OK;
     hipStreamSynchronize(ctx_stream(n->ctx));
-    if (n->ifeat) {
-        hipFree(n->ifeat); hipFree(n->ia); hipFree(n->ib); hipFree(n->imask);
-        n->ifeat = n->ia = n->ib = nullptr;
+    if (n->ia) {
+        hipFree(n->ia); hipFree(n->ib); hipFree(n->imask);
+        n->ia = n->ib = nullptr;
         n->imask = nullptr;
         n->icap = 0;
     }
-    if (nao_alloc((void **)&n->ifeat, px * C0 * 2) || nao_alloc((void **)&n->ia, px * CH * 2) ||
-        nao_alloc((void **)&n->ib, px * CH * 2) || nao_alloc((void **)&n->imask, px))
+    // activations of layers 1-3 (layer 1 reads the G-buffer, layer 4 writes AO)
+    if (nao_alloc((void **)&n->ia, px * CH * 2) || nao_alloc((void **)&n->ib, px * CH * 2) ||
+        nao_alloc((void **)&n->imask, px))
         return FGL_ERR_NOMEM;
     n->icap = px;
     return FGL_OK;
@@ -748,20 +829,21 @@ int nao_infer_batch(fgl_nao *n, hipStream_t s, const NaoBatchTables &t, size_t t
 {
     if (total_px > n->icap) return FGL_ERR_INVALID;
     if (t.n_blocks == 0) return FGL_OK;
-    const dim3 g(t.n_blocks), b(256);
-    hipLaunchKernelGGL(k_features_batch, g, b, 0, s, t.views, t.blk_view, depth, gbuf, n->ifeat, n->imask);
-    launch_conv_batch<C0>(s, t.views, t.items[0], (int)t.n_items[0], (const uint16_t *)n->ifeat, n->w1,
-                          n->params + OFF_B1, n->ia, layer_dil(0));
-    const int boffs[3] = {OFF_B2, OFF_B3, OFF_B4};
-    uint16_t *src = n->ia, *dst = n->ib;
-    for (int l = 0; l < 3; ++l) {
-        launch_conv_batch<CH>(s, t.views, t.items[l + 1], (int)t.n_items[l + 1], (const uint16_t *)src,
-                              n->w234 + (size_t)l * CH * K2, n->params + boffs[l], dst, layer_dil(l + 1));
-        uint16_t *tmp = src;
-        src = dst;
-        dst = tmp;
-    }
-    hipLaunchKernelGGL(k_head_batch, g, b, 0, s, t.views, t.blk_view, n->params, (const uint16_t *)src, n->imask, ao);
+    FeatArgs fa;
+    fa.depth = depth;
+    fa.gbuf = gbuf;
+    fa.mask = n->imask;
+    std::memset(fa.V.m, 0, sizeof(fa.V.m));
+    launch_conv_batch<C0, 3>(s, t.views, t.items[0], (int)t.n_items[0], (const uint16_t *)nullptr, n->w1,
+                             n->params + OFF_B1, n->ia, layer_dil(0), HeadArgs{nullptr, nullptr, nullptr}, fa);
+    launch_conv_batch<CH>(s, t.views, t.items[1], (int)t.n_items[1], (const uint16_t *)n->ia, n->w234,
+                          n->params + OFF_B2, n->ib, layer_dil(1));
+    launch_conv_batch<CH>(s, t.views, t.items[2], (int)t.n_items[2], (const uint16_t *)n->ib,
+                          n->w234 + (size_t)CH * K2, n->params + OFF_B3, n->ia, layer_dil(2));
+    // layer 4 with the head fused
+    launch_conv_batch<CH, 2>(s, t.views, t.items[3], (int)t.n_items[3], (const uint16_t *)n->ia,
+                             n->w234 + (size_t)2 * CH * K2, n->params + OFF_B4, nullptr, layer_dil(3),
+                             HeadArgs{n->params + OFF_W5, n->imask, ao});
     return hipGetLastError() == hipSuccess ? FGL_OK : FGL_ERR_DEVICE;
 }
 
@@ -821,7 +903,7 @@ void fgl_nao_destroy(fgl_nao *n)
     if (!n) return;
     hipDeviceSynchronize();
     void *ps[] = {n->params, n->grads, n->m, n->v, n->w1, n->w234, n->wt234, n->feat, n->target, n->y,
-                  n->mask, n->part, n->part_db, n->head_part, n->head_red, n->ifeat, n->ia, n->ib,
+                  n->mask, n->part, n->part_db, n->head_part, n->head_red, n->ia, n->ib,
                   n->imask};
     for (void *p : ps)
         if (p) hipFree(p);
@@ -1000,22 +1082,22 @@ int fgl_nao_infer(fgl_nao *n)
     if (rc) return rc;
     const size_t px = (size_t)cv.W * cv.H;
     if ((rc = nao_reserve(n, px))) return rc;
-    Mat16 V;
-    std::memcpy(V.m, cv.view, sizeof(V.m));
+    FeatArgs fa;
+    fa.depth = cv.depth;
+    fa.gbuf = cv.gbuf;
+    fa.mask = n->imask;
+    std::memcpy(fa.V.m, cv.view, sizeof(fa.V.m));
     {
-        CtxTimer t(n->ctx, "nao_features");
-        hipLaunchKernelGGL(k_features, g2(cv.W, cv.H), dim3(256), 0, cv.stream, V, cv.depth, cv.gbuf, cv.W,
-                           cv.H, n->ifeat, n->imask);
-    }
-    uint16_t *acts[4] = {n->ia, n->ib, n->ia, n->ib};
-    {
+        // layer 1 with the features computed while staging, layers 2-3, then
+        // layer 4 with the head fused: G-buffer in, AO out
         CtxTimer t(n->ctx, "nao_conv");
-        forward_convs(n, cv.stream, n->ifeat, acts, cv.W, cv.H, 1);
-    }
-    {
-        CtxTimer t(n->ctx, "nao_head");
-        hipLaunchKernelGGL(k_head_fwd, dim3((unsigned)((px + 255) / 256)), dim3(256), 0, cv.stream, n->params,
-                           n->ib, n->imask, px, cv.ao);
+        launch_conv_feat(cv.stream, fa, n->w1, n->params + OFF_B1, n->ia, cv.W, cv.H, layer_dil(0));
+        launch_conv<CH, 0>(cv.stream, n->ia, n->w234, n->params + OFF_B2, nullptr, n->ib, cv.W, cv.H, 1,
+                           layer_dil(1));
+        launch_conv<CH, 0>(cv.stream, n->ib, n->w234 + (size_t)CH * K2, n->params + OFF_B3, nullptr, n->ia, cv.W,
+                           cv.H, 1, layer_dil(2));
+        launch_conv_head(cv.stream, n->ia, n->w234 + (size_t)2 * CH * K2, n->params + OFF_B4,
+                         HeadArgs{n->params + OFF_W5, n->imask, cv.ao}, cv.W, cv.H, 1, layer_dil(3));
     }
     ctx_ao_written(n->ctx);
     return hipGetLastError() == hipSuccess ? FGL_OK : FGL_ERR_DEVICE;
