@@ -153,7 +153,7 @@ def run_neural_ao(steps, warmup):
     net.close()
     ctx.close()
     plain_ms = per.get("ssao", 0.0) + per.get("blur_gaussian", 0.0) + per.get("blur_reference", 0.0)
-    nao_ms = per.get("nao_features", 0.0) + per.get("nao_conv", 0.0) + per.get("nao_head", 0.0)
+    nao_ms = per.get("nao_conv", 0.0)   # the whole network: features, four convolutions, head
     px = W * H
     conv_tflops = px * 59392 / (per["nao_conv"] * 1e-3) / 1e12 if per.get("nao_conv") else None
     held = json.loads(str(ck["eval_json"])) if "eval_json" in ck else None

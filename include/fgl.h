@@ -139,7 +139,8 @@ void fgl_ctx_destroy(fgl_ctx *ctx);
 const char *fgl_last_error(const fgl_ctx *ctx);
 int fgl_sync(fgl_ctx *ctx);
 /* Raster tuning: tile edge in pixels (8, 16 or 32).  Results do not depend
- * on it (tile ownership keeps submission order per pixel). */
+ * on it (each pixel's depth resolve is a minimum over (depth, submission id)
+ * keys, whatever the tile). */
 int fgl_set_tile_size(fgl_ctx *ctx, int tile);
 /* Per-pass tile edge (FGL_PASS_SHADOW, or FORWARD/GEOMETRY for the main pass);
  * the autotuner (forkerrenderer_amd/autotune.py) sets these. */
