@@ -414,10 +414,43 @@ __device__ __forceinline__ void conv_tile_compute(const ConvWeights<CIN> &cw, co
 
 // Tile t of a single-image-set launch: tiles_x x tiles_y tiles per phase
 // grid, dil^2 phases per image.
+// Order of the tiles of one image: a 128-byte line holds two neighbouring
+// pixels of a row (64 B each), which belong to x-phases 2m and 2m+1 when the
+// convolution is dilated.  Tile u and tile u + 8 take the same spatial tile of
+// those two phases: the persistent workgroups stride over tiles by a multiple
+// of 8, so both run on the same XCD at about the same time and the line is
+// fetched from HBM once (in phase order the second half of every line came
+// back a whole phase later, out of L2: 2x the read traffic).  Returns the
+// phase (phy * dil + phx) and the tile within the phase grid.
+__host__ __device__ __forceinline__ void tile_order(int u, int per_phase, int dil, int &ph, int &k)
+{
+    if (dil == 1) {
+        ph = 0;
+        k = u;
+        return;
+    }
+    const int M = per_phase * dil * dil / 2, full = M / 8 * 8;   // tiles of one phase pair
+    int L, lo;
+    if (u < 2 * full) {
+        const int w = u % 16;
+        lo = w / 8;
+        L = 8 * (u / 16) + w % 8;
+    } else {
+        const int v = u - 2 * full;
+        lo = v % 2;
+        L = full + v / 2;
+    }
+    const int half = dil / 2, hp = L / per_phase;
+    k = L % per_phase;
+    ph = (hp / half) * dil + 2 * (hp % half) + lo;
+}
+
 __device__ __forceinline__ ConvTile single_tile(int t, int W, int H, int dil, int tiles_x, int tiles_y)
 {
     const int per_phase = tiles_x * tiles_y, per_img = per_phase * dil * dil;
-    const int n = t / per_img, rem = t % per_img, ph = rem / per_phase, k = rem % per_phase;
+    const int n = t / per_img;
+    int ph, k;
+    tile_order(t % per_img, per_phase, dil, ph, k);
     ConvTile ct;
     ct.img = (size_t)n * H * W;
     ct.W = W;
@@ -817,10 +850,12 @@ void nao_conv_items(int view, int W, int H, std::vector<uint2> items[4])
         const int d = layer_dil(l);
         int tx, ty, n;
         conv_tiles(W, H, 1, d, tx, ty, n);
-        for (int ph = 0; ph < d * d; ++ph)
-            for (int by = 0; by < ty; ++by)
-                for (int bx = 0; bx < tx; ++bx)
-                    items[l].push_back(make_uint2((uint32_t)view | ((uint32_t)ph << 8), (uint32_t)bx | ((uint32_t)by << 16)));
+        for (int u = 0; u < n; ++u) {
+            int ph, k;
+            tile_order(u, tx * ty, d, ph, k);
+            const int bx = k % tx, by = k / tx;
+            items[l].push_back(make_uint2((uint32_t)view | ((uint32_t)ph << 8), (uint32_t)bx | ((uint32_t)by << 16)));
+        }
     }
 }
 
