@@ -109,3 +109,22 @@ def test_stochastic_frames_match_the_oracle(ctx, scene_dir, filt):
     # the filter's samples were consumed after SSAO's
     P = sc.info.width * sc.info.height
     assert device_draws > 32 * P * 3
+
+
+def test_stochastic_modes_are_rejected_where_they_cannot_replay(ctx, scene_dir):
+    """The forward pass draws per depth-passing fragment in raster order, which
+    the tile rasteriser does not replay, and batched views take the
+    deterministic filters: both refuse FGL_SHADOW_REF_* with an error (no
+    device work is started)."""
+    p = G.tiny_scene(os.path.join(scene_dir, "rfwd"), seed=33, mode="forward", extra="shadowfilter refpcss")
+    sc = F.Scene(p)
+    with pytest.raises(F.FglError, match="deferred lighting pass only"):
+        ctx.render_scene(sc)
+    with pytest.raises(F.FglError):
+        ctx.rng_vectors(4, 10)
+    # the context still works afterwards
+    p2 = G.tiny_scene(os.path.join(scene_dir, "rok"), seed=33)
+    sc2 = F.Scene(p2)
+    got = gpu_frame(ctx, sc2)
+    want = oracle_frame(sc2)
+    assert_bitexact("depth after errors", got["depth"], want["depth"])
