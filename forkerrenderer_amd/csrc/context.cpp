@@ -818,16 +818,28 @@ int fgl_draw_screen_space_pixels(fgl_ctx *c)
             ss = {pairs, nullptr, np};
         } else {
             // PCSS takes n_pcf more samples only where its blocker search
-            // finds a blocker: draw the most it can need, walk the pixels for
-            // the offsets, then replay the stream from the saved state to
-            // leave it after exactly the samples used (the replay rewrites
-            // the same values).
-            if ((rc = ensure(c, c->rng_offs, (P + 1) * sizeof(uint32_t)))) return rc;
-            uint32_t *offs = (uint32_t *)c->rng_offs.p;
+            // finds a blocker.  Pixels the shadow map decides alone are
+            // counted in parallel; the others are walked in order over the
+            // most samples the frame can need; then the stream is restored and
+            // replayed to stop after exactly the samples used (the replay
+            // rewrites the same values).
+            const size_t words = scan_tmp_words(P + 1);
+            if ((rc = ensure(c, c->rng_offs, (5 * (P + 1) + words) * sizeof(uint32_t)))) return rc;
+            uint32_t *base = (uint32_t *)c->rng_offs.p, *upos = base + (P + 1), *list = upos + (P + 1);
+            uint32_t *found = list + (P + 1), *offs = found + (P + 1), *tmp = offs + (P + 1);
             HIPCHK(c, hipMemcpyAsync(c->rng_saved.p, st, sizeof(MtState), hipMemcpyDeviceToDevice, c->stream));
             launch_rng_accept(2, st, pairs, (uint32_t)most, c->stream);
-            launch_pcss_walk(u, (const float *)c->gbuf.p, (const float *)c->shadow.p, pairs, offs, offs + P,
-                             c->stream);
+            launch_pcss_classify(u, (const float *)c->gbuf.p, (const float *)c->shadow.p, base, upos, c->stream);
+            launch_exclusive_scan(base, base, P, tmp, c->stream);
+            launch_exclusive_scan(upos, upos, P, tmp, c->stream);
+            launch_pcss_compact(upos, P, list, c->stream);
+            HIPCHK(c, hipMemcpyAsync(c->h_total, upos + P, 4, hipMemcpyDeviceToHost, c->stream));
+            HIPCHK(c, hipStreamSynchronize(c->stream));
+            const uint32_t n_unc = c->h_total[0];
+            launch_pcss_walk_list(u, (const float *)c->gbuf.p, (const float *)c->shadow.p, pairs, base, list, n_unc,
+                                  found, c->stream);
+            launch_exclusive_scan(found, found, n_unc, tmp, c->stream);
+            launch_pcss_offsets(base, upos, found, np, P, offs, c->stream);
             HIPCHK(c, hipMemcpyAsync(c->h_total, offs + P, 4, hipMemcpyDeviceToHost, c->stream));
             HIPCHK(c, hipStreamSynchronize(c->stream));
             const uint32_t used = c->h_total[0];

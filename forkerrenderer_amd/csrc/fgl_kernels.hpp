@@ -111,12 +111,20 @@ void launch_lighting(const FrameUniforms &u, const float *gbuf, const float *ao,
 // pixel in row order); pow3[c] = powf(1 - c / 32, 3) from the host libm.
 void launch_ssao_reference(const FrameUniforms &u, const float *samples, const float *pow3,
                            const float *depth, const float *gbuf, float *ao, hipStream_t s);
-// Offsets of the PCSS filter's samples: walks the pixels in row order,
-// running each pixel's blocker search at its offset; a pixel takes n_blocker
-// pairs, plus n_pcf when a blocker is found.  offs[i] per pixel, *total at
-// the end.
-void launch_pcss_walk(const FrameUniforms &u, const float *gbuf, const float *shadow_map, const float *pairs,
-                      uint32_t *offs, uint32_t *total, hipStream_t s);
+// Offsets of the PCSS filter's samples (a pixel takes n_blocker pairs, plus
+// n_pcf when its blocker search finds a blocker): classify every pixel as
+// decided by the shadow map alone or not (cnt: its pairs when decided,
+// n_blocker otherwise; unc: 1 when undecided), compact the undecided pixels
+// after a scan of unc (upos), walk them in order from the scanned cnt (base),
+// then offs[i] = base[i] + n_pcf * (blockers found by undecided pixels
+// before i) for i = 0..P.
+void launch_pcss_classify(const FrameUniforms &u, const float *gbuf, const float *shadow_map, uint32_t *cnt,
+                          uint32_t *unc, hipStream_t s);
+void launch_pcss_compact(const uint32_t *upos, size_t P, uint32_t *list, hipStream_t s);
+void launch_pcss_walk_list(const FrameUniforms &u, const float *gbuf, const float *shadow_map, const float *pairs,
+                           const uint32_t *base, const uint32_t *list, uint32_t n, uint32_t *found, hipStream_t s);
+void launch_pcss_offsets(const uint32_t *base, const uint32_t *upos, const uint32_t *fpos, uint32_t n_pcf, size_t P,
+                         uint32_t *offs, hipStream_t s);
 void launch_ssao(const FrameUniforms &u, const SsaoArgs &sa, const float *depth,
                  const float *gbuf, float *ao, hipStream_t s);
 void launch_blur_recursive(float *ao, int W, int H, hipStream_t s);

@@ -27,32 +27,111 @@ __global__ __launch_bounds__(256) void k_ssao_ref(FrameUniforms u, const float *
     ssao_ref_px(u, samples, pow3, depth, g, ao, x, y);
 }
 
-// The PCSS sample offsets: one wavefront walks the pixels in the lighting
-// pass's order (row order); at each pixel lanes evaluate the blocker
-// samples at the pixel's offset in parallel and lane order sums them as
-// FindAverageBlockDepth does (shadow.cpp:65-91), giving the pixel's
-// consumption: n_blocker pairs, plus n_pcf when a blocker is found
-// (PCSS, shadow.cpp:93-106).  Only light 0 is shadowed; with shadows off or
-// no light nothing is consumed.
-__global__ __launch_bounds__(64) void k_pcss_walk(FrameUniforms u, const float *__restrict__ g,
-                                                  const float *__restrict__ smap,
-                                                  const float *__restrict__ pairs,
-                                                  uint32_t *__restrict__ offs, uint32_t *__restrict__ total)
+// ---- PCSS sample offsets for the stochastic filter (FGL_SHADOW_REF_PCSS) ----
+//
+// The lighting pass visits pixels in row order and each pixel draws n_blocker
+// unit-disk samples, plus n_pcf more when its blocker search finds a blocker
+// (PCSS, shadow.cpp:93-106), so a pixel's offset into the stream depends on
+// the outcome of every search before it.  Most searches are decided by the
+// shadow map alone: k_pcss_classify checks every texel a sample of the pixel
+// could reach (the square around the disk) and marks the pixel "no blocker
+// for any sample set", "a blocker for any sample set" or "depends on the
+// samples".  Only the last kind is walked in order (k_pcss_walk_list, one
+// wavefront), at offsets that count the decided pixels before it.
+
+// Light 0's shadow coordinate and bias at pixel i, as shade_point computes them.
+__device__ __forceinline__ void pcss_pixel(const FrameUniforms &u, const float *__restrict__ g, size_t P, size_t i,
+                                           Vec3 &sc, float &bias)
+{
+    const Vec3 N = vec3(g[i], g[P + i], g[2 * P + i]);
+    const Vec3 Pw = vec3(g[3 * P + i], g[4 * P + i], g[5 * P + i]);
+    const Vec3 lndc = vec3(g[6 * P + i], g[7 * P + i], g[8 * P + i]);
+    sc = shadow_coord(lndc);
+    bias = shadow_bias(N, light_dir(u.lights[0], Pw));
+}
+
+// cnt[i] = n_blocker + n_pcf for "a blocker for any sample set", n_blocker
+// otherwise; unc[i] = 1 for "depends on the samples".  Nothing is drawn with
+// shadows off or without a light.
+__global__ __launch_bounds__(256) void k_pcss_classify(FrameUniforms u, const float *__restrict__ g,
+                                                       const float *__restrict__ smap, uint32_t *__restrict__ cnt,
+                                                       uint32_t *__restrict__ unc)
+{
+    const size_t P = (size_t)u.W * u.H;
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= P) return;
+    const DevShadowCfg &c = u.shadow;
+    if (!(u.shadow_on && u.n_lights > 0)) {
+        cnt[i] = 0;
+        unc[i] = 0;
+        return;
+    }
+    Vec3 sc;
+    float bias;
+    pcss_pixel(u, g, P, i, sc, bias);
+    // every sample coordinate lies in [sc - F, sc + F], F = (float)blocker_filter
+    // (|(float)(d * filter)| <= (float)filter for |d| < 1; float add and the
+    // texel index are monotone)
+    const float F = (float)c.blocker_filter;
+    const float u0 = sc.x + (-F), u1 = sc.x + F, v0 = sc.y + (-F), v1 = sc.y + F;
+    int cls = 2;
+    if (u0 == u0 && u1 == u1 && v0 == v0 && v1 == v1 && sc.z == sc.z && bias == bias) {
+        const bool inside = u0 >= 0.f && u1 <= 1.f && v0 >= 0.f && v1 <= 1.f;
+        bool any_blk = false, any_clear = !inside;   // samples off the map never block
+        if (u1 >= 0.f && u0 <= 1.f && v1 >= 0.f && v0 <= 1.f) {
+            const int w = (int)((float)u.SW - 0.001f), h = (int)((float)u.SH - 0.001f);
+            const int x0 = (int)((float)w * (u0 < 0.f ? 0.f : u0)), x1 = (int)((float)w * (u1 > 1.f ? 1.f : u1));
+            const int y0 = (int)((float)h * (v0 < 0.f ? 0.f : v0)), y1 = (int)((float)h * (v1 > 1.f ? 1.f : v1));
+            for (int y = y0; y <= y1 && !(any_blk && any_clear); ++y)
+                for (int x = x0; x <= x1; ++x) {
+                    const float d = smap[(size_t)x + (size_t)y * u.SW];
+                    const float s = ((double)d < 0.001) ? 1.f : d;
+                    const bool blk = sc.z > s + bias;
+                    // a blocker depth within reach of 0.001 could average
+                    // below it: not decided
+                    if (blk && !((double)s > 0.0011)) any_clear = true;
+                    any_blk |= blk;
+                    any_clear |= !blk;
+                }
+        }
+        if (!any_blk) cls = 0;
+        else if (!any_clear) cls = 1;
+    }
+    cnt[i] = (uint32_t)c.n_blocker + (cls == 1 ? (uint32_t)c.n_pcf : 0u);
+    unc[i] = cls == 2 ? 1u : 0u;
+}
+
+// list[upos[i]] = i for the undecided pixels (upos = exclusive scan of the
+// undecided flags, total at upos[P]; a flag is upos[i + 1] - upos[i]).
+__global__ __launch_bounds__(256) void k_pcss_compact(const uint32_t *__restrict__ upos, size_t P,
+                                                      uint32_t *__restrict__ list)
+{
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < P && upos[i + 1] != upos[i]) list[upos[i]] = (uint32_t)i;
+}
+
+// The undecided pixels in row order: pixel list[k] draws from
+// base[list[k]] + n_pcf * (blockers found by the undecided pixels before it);
+// lanes evaluate its blocker samples in parallel and lane order sums them as
+// FindAverageBlockDepth does (shadow.cpp:65-91).  found[k] = 1 when it finds
+// a blocker.
+__global__ __launch_bounds__(64) void k_pcss_walk_list(FrameUniforms u, const float *__restrict__ g,
+                                                       const float *__restrict__ smap,
+                                                       const float *__restrict__ pairs,
+                                                       const uint32_t *__restrict__ base,
+                                                       const uint32_t *__restrict__ list, uint32_t n,
+                                                       uint32_t *__restrict__ found)
 {
     const int lane = threadIdx.x;
     const size_t P = (size_t)u.W * u.H;
     const DevShadowCfg &c = u.shadow;
-    const bool consume = u.shadow_on && u.n_lights > 0;
-    uint32_t o = 0;
-    for (size_t i = 0; i < P; ++i) {
-        if (lane == 0) offs[i] = o;
-        if (!consume) continue;
-        const Vec3 N = vec3(g[i], g[P + i], g[2 * P + i]);
-        const Vec3 Pw = vec3(g[3 * P + i], g[4 * P + i], g[5 * P + i]);
-        const Vec3 lndc = vec3(g[6 * P + i], g[7 * P + i], g[8 * P + i]);
-        const Vec3 L = light_dir(u.lights[0], Pw);
-        const Vec3 sc = shadow_coord(lndc);
-        const float bias = shadow_bias(N, L);
+    uint32_t F = 0;
+    for (uint32_t k = 0; k < n; ++k) {
+        const size_t i = list[k];
+        const uint32_t o = base[i] + (uint32_t)c.n_pcf * F;
+        Vec3 sc;
+        float bias;
+        pcss_pixel(u, g, P, i, sc, bias);
         float sv = 0.f;
         int blk = 0;
         if (lane < c.n_blocker) {
@@ -71,9 +150,20 @@ __global__ __launch_bounds__(64) void k_pcss_walk(FrameUniforms u, const float *
             }
         }
         const float dB = (nb < 1.f) ? 0.f : blocker / nb;
-        o += (uint32_t)c.n_blocker + (((double)dB < 0.001) ? 0u : (uint32_t)c.n_pcf);
+        const uint32_t f = ((double)dB < 0.001) ? 0u : 1u;
+        if (lane == 0) found[k] = f;
+        F += f;
     }
-    if (lane == 0) *total = o;
+}
+
+// offs[i] = base[i] + n_pcf * fpos[upos[i]] for i = 0..P (offs[P] = total).
+__global__ __launch_bounds__(256) void k_pcss_offsets(const uint32_t *__restrict__ base,
+                                                      const uint32_t *__restrict__ upos,
+                                                      const uint32_t *__restrict__ fpos, uint32_t n_pcf, size_t P,
+                                                      uint32_t *__restrict__ offs)
+{
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i <= P) offs[i] = base[i] + n_pcf * fpos[upos[i]];
 }
 
 __global__ __launch_bounds__(256) void k_ssao(FrameUniforms u, SsaoArgs sa,
@@ -194,10 +284,33 @@ void launch_ssao_reference(const FrameUniforms &u, const float *samples, const f
     hipLaunchKernelGGL(k_ssao_ref, grid2(u.W, u.H), dim3(256), 0, s, u, samples, pow3, depth, gbuf, ao);
 }
 
-void launch_pcss_walk(const FrameUniforms &u, const float *gbuf, const float *shadow_map, const float *pairs,
-                      uint32_t *offs, uint32_t *total, hipStream_t s)
+void launch_pcss_classify(const FrameUniforms &u, const float *gbuf, const float *shadow_map, uint32_t *cnt,
+                          uint32_t *unc, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_pcss_walk, dim3(1), dim3(64), 0, s, u, gbuf, shadow_map, pairs, offs, total);
+    const size_t P = (size_t)u.W * u.H;
+    if (P == 0) return;
+    hipLaunchKernelGGL(k_pcss_classify, dim3((unsigned)((P + 255) / 256)), dim3(256), 0, s, u, gbuf, shadow_map, cnt,
+                       unc);
+}
+
+void launch_pcss_compact(const uint32_t *upos, size_t P, uint32_t *list, hipStream_t s)
+{
+    if (P == 0) return;
+    hipLaunchKernelGGL(k_pcss_compact, dim3((unsigned)((P + 255) / 256)), dim3(256), 0, s, upos, P, list);
+}
+
+void launch_pcss_walk_list(const FrameUniforms &u, const float *gbuf, const float *shadow_map, const float *pairs,
+                           const uint32_t *base, const uint32_t *list, uint32_t n, uint32_t *found, hipStream_t s)
+{
+    if (n == 0) return;
+    hipLaunchKernelGGL(k_pcss_walk_list, dim3(1), dim3(64), 0, s, u, gbuf, shadow_map, pairs, base, list, n, found);
+}
+
+void launch_pcss_offsets(const uint32_t *base, const uint32_t *upos, const uint32_t *fpos, uint32_t n_pcf, size_t P,
+                         uint32_t *offs, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_pcss_offsets, dim3((unsigned)((P + 1 + 255) / 256)), dim3(256), 0, s, base, upos, fpos, n_pcf,
+                       P, offs);
 }
 
 void launch_ssao(const FrameUniforms &u, const SsaoArgs &sa, const float *depth,
