@@ -26,6 +26,7 @@ struct DevBuf {
 struct ModelRes {
     float *pos = nullptr, *uv = nullptr, *nrm = nullptr, *tan = nullptr;
     int *vi = nullptr, *ti = nullptr, *ni = nullptr, *fmat = nullptr;
+    float *corner = nullptr;   // per face corner: pos, uv, nrm, tan (12 floats)
     int n_pos = 0, n_faces = 0;
     bool has_tan = false, pbr = false;
 };
@@ -227,7 +228,7 @@ void fgl_ctx_destroy(fgl_ctx *c)
     for (auto &m : c->models) {
         hipFree(m.pos); hipFree(m.uv); hipFree(m.nrm);
         if (m.tan) hipFree(m.tan);
-        hipFree(m.vi); hipFree(m.ti); hipFree(m.ni); hipFree(m.fmat);
+        hipFree(m.vi); hipFree(m.ti); hipFree(m.ni); hipFree(m.fmat); hipFree(m.corner);
     }
     DevBuf *bufs[] = {&c->d_textures, &c->d_materials, &c->depth, &c->fb, &c->gbuf, &c->ao,
                       &c->blur_tmp, &c->shadow, &c->shadow_depth, &c->img, &c->vis, &c->d_draws,
@@ -346,6 +347,18 @@ int fgl_upload_model(fgl_ctx *c, const fgl_model_desc *d, int *out_id)
     if ((rc = upload_array(c, &m.ti, (const int *)d->ti, 3 * nf))) return rc;
     if ((rc = upload_array(c, &m.ni, (const int *)d->ni, 3 * nf))) return rc;
     if ((rc = upload_array(c, &m.fmat, (const int *)d->face_material, nf))) return rc;
+    {
+        std::vector<float> corner(3 * nf * 12, 0.f);
+        for (size_t k = 0; k < 3 * nf; ++k) {
+            float *o = &corner[12 * k];
+            const int vi = d->vi[k], ti = d->ti[k], ni = d->ni[k];
+            o[0] = d->pos[3 * vi]; o[1] = d->pos[3 * vi + 1]; o[2] = d->pos[3 * vi + 2];
+            o[3] = d->uv[2 * ti]; o[4] = d->uv[2 * ti + 1];
+            o[5] = d->nrm[3 * ni]; o[6] = d->nrm[3 * ni + 1]; o[7] = d->nrm[3 * ni + 2];
+            if (d->tan) { o[8] = d->tan[3 * vi]; o[9] = d->tan[3 * vi + 1]; o[10] = d->tan[3 * vi + 2]; }
+        }
+        if ((rc = upload_array(c, &m.corner, corner.data(), corner.size()))) return rc;
+    }
     m.n_pos = d->n_pos;
     m.n_faces = d->n_faces;
     m.has_tan = d->tan != nullptr;
@@ -597,6 +610,7 @@ static void fill_draw(fgl_ctx *c, const ModelRes &m, const float M[16], DevDraw 
     std::memset(&d, 0, sizeof(d));
     d.pos = m.pos; d.uv = m.uv; d.nrm = m.nrm; d.tan = m.tan;
     d.vi = m.vi; d.ti = m.ti; d.ni = m.ni; d.face_mat = m.fmat;
+    d.corner = (const float4 *)m.corner;
     d.n_pos = m.n_pos;
     d.n_faces = m.n_faces;
     d.has_tangents = m.has_tan;

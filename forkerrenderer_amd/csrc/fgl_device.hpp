@@ -29,6 +29,11 @@ struct DevMaterial {
 struct DevDraw {
     const float *pos, *uv, *nrm, *tan;
     const int *vi, *ti, *ni, *face_mat;
+    // per face corner (3 per face), the attributes the main vertex programs
+    // read, interleaved as 3 float4: (pos.xyz, uv.x), (uv.y, nrm.xyz),
+    // (tan.xyz, 0) -- one 48-byte record instead of four arrays behind three
+    // index streams
+    const float4 *corner;
     int n_pos, n_faces, vert_base, tri_base;
     int has_tangents, support_pbr;
     int view;      // batched views: index of the draw's view (0 otherwise)
@@ -130,20 +135,24 @@ struct Varying {
     float oow;
 };
 
-__device__ __forceinline__ Varying main_vertex(const DevDraw &d, const FrameUniforms &u, int vi,
-                                               int ti, int ni)
+// BlinnPhong/PBR/G ProcessVertex (phongshader.h:35-85, pbrshader.h:35-85,
+// gshader.h:41-92) of corner k of face f, with the corner's attributes as
+// Mesh::Vert / TexCoord / Normal / Tangent fetch them (mesh.cpp:34-56).
+__device__ __forceinline__ Varying main_vertex(const DevDraw &d, const FrameUniforms &u, int f, int k)
 {
     Varying o;
-    Vec4 p = vec4(d.pos[3 * vi], d.pos[3 * vi + 1], d.pos[3 * vi + 2], 1.f);
+    const float4 *r = d.corner + (size_t)(3 * f + k) * 3;
+    const float4 r0 = r[0], r1 = r[1], r2 = r[2];
+    Vec4 p = vec4(r0.x, r0.y, r0.z, 1.f);
     Vec4 ws = mv4(d.model, p);
     Vec4 vs = mv4(u.view, ws);
     Vec4 cs = mv4(u.proj, vs);
-    Vec2 uv = vec2(d.uv[2 * ti], d.uv[2 * ti + 1]);
-    Vec3 n = norm3(vec3(d.nrm[3 * ni], d.nrm[3 * ni + 1], d.nrm[3 * ni + 2]));
+    Vec2 uv = vec2(r0.w, r1.x);
+    Vec3 n = norm3(vec3(r1.y, r1.z, r1.w));
     Vec3 nws = mv3(d.normal, n);
     Vec3 tws = vec3(0.f, 0.f, 0.f);
     if (d.has_tangents) {
-        Vec3 t = norm3(vec3(d.tan[3 * vi], d.tan[3 * vi + 1], d.tan[3 * vi + 2]));
+        Vec3 t = norm3(vec3(r2.x, r2.y, r2.z));
         tws = mv3(d.normal, t);
     }
     Vec4 l = vec4(0.f, 0.f, 0.f, 0.f);

@@ -143,9 +143,9 @@ __global__ __launch_bounds__(256) void k_forward_resolve(ResolveArgs a, FrameUni
     const Vec3 b = winner_bary(a.screen, a.tri_v, g, a.W, a.H, x, y, tv);
     const DevDraw &dr = a.draws[tv.w];
     const int f = g - dr.tri_base;
-    Varying v0 = main_vertex(dr, u, dr.vi[3 * f], dr.ti[3 * f], dr.ni[3 * f]);
-    Varying v1 = main_vertex(dr, u, dr.vi[3 * f + 1], dr.ti[3 * f + 1], dr.ni[3 * f + 1]);
-    Varying v2 = main_vertex(dr, u, dr.vi[3 * f + 2], dr.ti[3 * f + 2], dr.ni[3 * f + 2]);
+    Varying v0 = main_vertex(dr, u, f, 0);
+    Varying v1 = main_vertex(dr, u, f, 1);
+    Varying v2 = main_vertex(dr, u, f, 2);
     const DevMaterial &m = a.materials[dr.face_mat[f]];
     FragGeom fg = frag_geometry(dr, u, a.textures, v0, v1, v2, b, m.map_bump);
     Vec3 c;
