@@ -391,6 +391,14 @@ def quantize_rgb(fb, W, H):
     return out
 
 
+def quantize_grey(plane, W, H):
+    """Buffer1f::GenerateImage (src/buffer.cpp:20-32) on one float plane."""
+    a = np.ascontiguousarray(plane, np.float32)
+    out = np.empty(W * H, np.uint8)
+    lib().orc_quantize_grey(_p(a), W, H, _p(out))
+    return out
+
+
 def ssaa_resolve(img, W, H, k):
     out = np.empty((W // k) * (H // k) * 3, np.uint8)
     lib().orc_ssaa_resolve(_p(np.ascontiguousarray(img, np.uint8)), W, H, k, _p(out))
