@@ -115,6 +115,7 @@ def lib():
             "fgl_write_target": (C.c_int, [P, C.c_int, P, C.c_size_t]),
             "fgl_target_bytes": (C.c_int, [P, C.c_int, C.POINTER(C.c_size_t)]),
             "fgl_enable_timing": (C.c_int, [P, C.c_int]),
+            "fgl_time_only": (C.c_int, [P, C.c_char_p]),
             "fgl_read_timings": (C.c_int, [P, P, P, C.c_int, C.POINTER(C.c_int)]),
             "fgl_scene_load": (C.c_int, [C.c_char_p, C.POINTER(P), C.c_char_p, C.c_size_t]),
             "fgl_scene_free": (None, [P]),
@@ -376,7 +377,10 @@ class Context:
         a = f32(arr).reshape(-1)
         self._chk(lib().fgl_write_target(self._h, which, _ptr(a), a.nbytes), "fgl_write_target")
 
-    def enable_timing(self, on=True):
+    def enable_timing(self, on=True, only=None):
+        """Kernel timing with HIP events; `only` restricts it to the kernels
+        timed under that name (each event pair costs the stream a few us)."""
+        self._chk(lib().fgl_time_only(self._h, only.encode() if only else None), "fgl_time_only")
         self._chk(lib().fgl_enable_timing(self._h, int(on)), "fgl_enable_timing")
 
     def read_timings(self, cap=4096):

@@ -90,6 +90,7 @@ struct fgl_ctx {
 
     // timing
     bool timing = false;
+    std::string timing_only;   // empty: every timed kernel
     std::vector<TimedKernel> timed;
 };
 
@@ -137,7 +138,8 @@ struct KernelTimer {
     fgl_ctx *c;
     TimedKernel tk;
     bool on;
-    KernelTimer(fgl_ctx *ctx, const char *name) : c(ctx), on(ctx->timing)
+    KernelTimer(fgl_ctx *ctx, const char *name)
+        : c(ctx), on(ctx->timing && (ctx->timing_only.empty() || ctx->timing_only == name))
     {
         if (!on) return;
         tk.name = name;
@@ -1009,6 +1011,13 @@ int fgl_enable_timing(fgl_ctx *c, int on)
     return FGL_OK;
 }
 
+int fgl_time_only(fgl_ctx *c, const char *name)
+{
+    if (!c) return FGL_ERR_INVALID;
+    c->timing_only = name ? name : "";
+    return FGL_OK;
+}
+
 int fgl_rng_seed(fgl_ctx *c, uint32_t seed)
 {
     if (!c) return FGL_ERR_INVALID;
@@ -1140,7 +1149,7 @@ int ctx_tables(fgl_ctx *c, const DevMaterial **materials, const DevTexture **tex
 }
 
 hipStream_t ctx_stream(fgl_ctx *c) { return c->stream; }
-bool ctx_timing(fgl_ctx *c) { return c->timing; }
+bool ctx_timing(fgl_ctx *c) { return c->timing && c->timing_only.empty(); }
 
 CtxTimer::CtxTimer(fgl_ctx *c, const char *name) : impl_(name ? new KernelTimer(c, name) : nullptr) {}
 CtxTimer::~CtxTimer() { delete static_cast<KernelTimer *>(impl_); }

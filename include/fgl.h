@@ -225,6 +225,10 @@ int fgl_target_bytes(fgl_ctx *ctx, int which, size_t *out_bytes);
  * milliseconds are written to the arrays (up to cap entries). */
 int fgl_enable_timing(fgl_ctx *ctx, int on);
 int fgl_read_timings(fgl_ctx *ctx, char (*names)[32], float *ms, int cap, int *out_n);
+/* With timing on, records events only around the kernels timed under `name`
+ * (NULL or "" = all).  Every pair of events costs the stream a few
+ * microseconds, so a throughput run times only the kernel it reports. */
+int fgl_time_only(fgl_ctx *ctx, const char *name);
 
 /* Diagnostic with no counterpart in the reference: a plain device-to-device
  * copy of `bytes` run `iters` times, used to calibrate HBM byte counters and
