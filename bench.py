@@ -61,7 +61,7 @@ def algorithmic_bytes(name, I, ntri, npos):
         "raster_shadow": 8 * SP + 16 * ntri + 16 * npos,          # shadow depth + value; triangle, vertex records
         "lighting": 92 * P + 4 * SP,                              # 80 read + 12 write, shadow map once
         "ssao": 32 * P,                                           # normal, position, depth, AO
-        "blur_gaussian": 16 * P,
+        "blur_gaussian": 8 * P,
         "blur_reference": 16 * P,
         "resolve": 12 * P + 3 * (P // (k * k)),
     }

@@ -6,7 +6,10 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <atomic>
+#include <chrono>
 #include <string>
+#include <utility>
 #include <vector>
 
 #include "../../include/fgl.h"
@@ -83,6 +86,10 @@ struct fgl_ctx {
         tile_start, tile_end, tri_v, big_list, counters;
     std::vector<DevDraw> last_draws;
     uint32_t *h_total = nullptr;
+    // pass counts published by the device into coherent pinned memory
+    // (pair count, big-triangle count, sequence number), polled by the host
+    uint32_t *h_pub = nullptr, *d_pub = nullptr;
+    uint32_t pub_seq = 0;
     // the reference's random stream (device MtState), a copy for the PCSS
     // replay, its sample buffers, the DoSSAO powf table
     DevBuf rng, rng_saved, rng_samples, rng_offs, ssao_pow3;
@@ -155,6 +162,33 @@ struct KernelTimer {
     }
 };
 
+// Two device counters to the host without a stream synchronisation: a
+// one-thread kernel writes them and a fresh sequence number into coherent
+// pinned memory, and the host polls for the sequence number (the GPU then
+// idles only for the poll and the next launch, not for a full
+// synchronisation).  A poll that outlasts a second falls back to
+// synchronising the stream, which reports a failed kernel.
+static int read_counts(fgl_ctx *c, const uint32_t *a, const uint32_t *b, uint32_t &va, uint32_t &vb)
+{
+    const uint32_t seq = ++c->pub_seq;
+    launch_publish2(a, b, seq, c->d_pub, c->stream);
+    HIPCHK(c, hipGetLastError());
+    volatile uint32_t *pub = c->h_pub;
+    auto t0 = std::chrono::steady_clock::now();
+    for (uint32_t spin = 0; pub[2] != seq; ++spin) {
+        if ((spin & 1023u) == 1023u &&
+            std::chrono::steady_clock::now() - t0 > std::chrono::seconds(1)) {
+            HIPCHK(c, hipStreamSynchronize(c->stream));
+            if (pub[2] != seq) return fail(c, FGL_ERR_DEVICE, "pass counts were not published");
+            break;
+        }
+    }
+    std::atomic_thread_fence(std::memory_order_acquire);
+    va = pub[0];
+    vb = pub[1];
+    return FGL_OK;
+}
+
 // Restarts the context's random stream at its seed (std::mt19937(seed)).
 static int rng_restart(fgl_ctx *c)
 {
@@ -195,6 +229,15 @@ int fgl_ctx_create(int device, fgl_ctx **out)
         delete c;
         return FGL_ERR_NOMEM;
     }
+    if (hipHostMalloc((void **)&c->h_pub, 64, hipHostMallocCoherent | hipHostMallocMapped) != hipSuccess ||
+        hipHostGetDevicePointer((void **)&c->d_pub, c->h_pub, 0) != hipSuccess) {
+        if (c->h_pub) hipHostFree(c->h_pub);
+        hipHostFree(c->h_total);
+        hipStreamDestroy(c->stream);
+        delete c;
+        return FGL_ERR_NOMEM;
+    }
+    std::memset(c->h_pub, 0, 64);
     float I[16] = {1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1};
     std::memcpy(c->viewport, I, sizeof(I));
     std::memcpy(c->shadow_viewport, I, sizeof(I));
@@ -241,6 +284,7 @@ void fgl_ctx_destroy(fgl_ctx *c)
     for (DevBuf *b : bufs) free_buf(*b);
     for (auto &t : c->timed) { hipEventDestroy(t.a); hipEventDestroy(t.b); }
     if (c->h_total) hipHostFree(c->h_total);
+    if (c->h_pub) hipHostFree(c->h_pub);
     hipStreamDestroy(c->stream);
     delete c;
 }
@@ -719,10 +763,8 @@ static int flush_pass(fgl_ctx *c)
         launch_exclusive_scan((const uint32_t *)c->counts.p, (uint32_t *)c->offs.p, (size_t)nt,
                               (uint32_t *)c->scan_tmp.p, c->stream);
     }
-    HIPCHK(c, hipMemcpyAsync(c->h_total, (uint32_t *)c->offs.p + nt, 4, hipMemcpyDeviceToHost, c->stream));
-    HIPCHK(c, hipMemcpyAsync(c->h_total + 1, big_count, 4, hipMemcpyDeviceToHost, c->stream));
-    HIPCHK(c, hipStreamSynchronize(c->stream));
-    const uint32_t npairs = c->h_total[0], n_big = c->h_total[1];
+    uint32_t npairs = 0, n_big = 0;
+    if ((rc = read_counts(c, (const uint32_t *)c->offs.p + nt, big_count, npairs, n_big))) return rc;
     if ((rc = ensure(c, c->keys, (size_t)npairs * 4 + 4)) || (rc = ensure(c, c->vals, (size_t)npairs * 4 + 4)) ||
         (rc = ensure(c, c->keys_alt, (size_t)npairs * 4 + 4)) ||
         (rc = ensure(c, c->vals_alt, (size_t)npairs * 4 + 4)) ||
@@ -912,8 +954,10 @@ int fgl_ssao(fgl_ctx *c, const fgl_ssao_cfg *cfg)
         launch_blur_recursive((float *)c->ao.p, c->W, c->H, c->stream);
     } else if (cfg->blur == 2) {
         KernelTimer t(c, "blur_gaussian");
-        launch_blur_separable((float *)c->ao.p, (float *)c->blur_tmp.p, c->W, c->H,
-                              cfg->blur_weights, cfg->blur_radius, c->stream);
+        // one pass into the spare plane, which then becomes the AO target
+        launch_blur_separable((const float *)c->ao.p, (float *)c->blur_tmp.p, c->W, c->H,
+                                 cfg->blur_weights, cfg->blur_radius, c->stream);
+        std::swap(c->ao, c->blur_tmp);
     }
     HIPCHK(c, hipGetLastError());
     return FGL_OK;

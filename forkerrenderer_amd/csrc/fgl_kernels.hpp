@@ -58,6 +58,8 @@ void launch_setup(const DevDraw *draws, int n_draws, int max_faces, const int4 *
 void launch_emit(const uint32_t *off, int n_tri, const uint2 *rect, int tiles_x,
                  const uint32_t *big_list, uint32_t n_big, uint32_t *keys, uint32_t *vals,
                  hipStream_t s);
+// host: a device pointer to host memory; writes *a, *b, then seq.
+void launch_publish2(const uint32_t *a, const uint32_t *b, uint32_t seq, uint32_t *host, hipStream_t s);
 void launch_fill_u32(uint32_t *p, uint32_t v, size_t n, hipStream_t s);
 void launch_fill_f32(float *p, float v, size_t n, hipStream_t s);
 // n_dev (optional): device-side pair count, used as min(n, *n_dev)
@@ -128,8 +130,9 @@ void launch_pcss_offsets(const uint32_t *base, const uint32_t *upos, const uint3
 void launch_ssao(const FrameUniforms &u, const SsaoArgs &sa, const float *depth,
                  const float *gbuf, float *ao, hipStream_t s);
 void launch_blur_recursive(float *ao, int W, int H, hipStream_t s);
-void launch_blur_separable(float *ao, float *tmp, int W, int H, const float *w, int R,
-                           hipStream_t s);
+// Separable Gaussian of radius R (1..8), horizontal then vertical, clamped
+// taps, in one launch from src into dst (distinct buffers).
+void launch_blur_separable(const float *src, float *dst, int W, int H, const float *w, int R, hipStream_t s);
 void launch_resolve(const float *fb, int W, int H, int k, uint8_t *img, hipStream_t s);
 void launch_copy(const void *src, void *dst, size_t bytes, hipStream_t s);
 

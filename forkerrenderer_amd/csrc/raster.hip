@@ -78,6 +78,19 @@ __global__ __launch_bounds__(256) void k_emit_big(const uint32_t *__restrict__ o
     }
 }
 
+// Publishes two device counters to host memory, then the sequence number
+// that tells the host they are there.
+__global__ __launch_bounds__(64) void k_publish2(const uint32_t *__restrict__ a, const uint32_t *__restrict__ b,
+                                                 uint32_t seq, uint32_t *host)
+{
+    if (threadIdx.x != 0) return;
+    const uint32_t va = *a, vb = *b;
+    host[0] = va;
+    host[1] = vb;
+    __threadfence_system();
+    host[2] = seq;
+}
+
 __global__ __launch_bounds__(256) void k_fill_u32(uint32_t *__restrict__ p, uint32_t v, size_t n)
 {
     size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -206,6 +219,11 @@ void launch_emit(const uint32_t *off, int n_tri, const uint2 *rect, int tiles_x,
     if (n_big)
         hipLaunchKernelGGL(k_emit_big, dim3(n_big), dim3(256), 0, s, off, rect, tiles_x, big_list,
                            keys, vals);
+}
+
+void launch_publish2(const uint32_t *a, const uint32_t *b, uint32_t seq, uint32_t *host, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_publish2, dim3(1), dim3(64), 0, s, a, b, seq, host);
 }
 
 void launch_fill_u32(uint32_t *p, uint32_t v, size_t n, hipStream_t s)

@@ -220,26 +220,54 @@ struct BlurWeights {
     int R;
 };
 
-// Separable Gaussian: same tap order as the reference loop, reading a copy.
-template <bool HORIZ>
-__global__ __launch_bounds__(256) void k_blur_sep(const float *__restrict__ src,
-                                                  float *__restrict__ dst, int W, int H,
-                                                  BlurWeights bw)
+// Separable Gaussian in one pass: a workgroup stages a (16 + 2R) x (64 + 2R)
+// block of the source in LDS (coordinates clamped to the image, as the taps
+// are), runs the horizontal taps for its 16 + 2R rows (the rows the vertical
+// taps of its 64 x 16 outputs read, clamped likewise) into a second LDS
+// block, then the vertical taps; each tap sum in the reference loop's order
+// (centre, then +i and -i for i = 1..R), so the result is the
+// horizontal-then-vertical two-pass filter bit for bit, without an
+// intermediate plane in HBM.  Writes dst (not src: neighbouring workgroups
+// read src).
+constexpr int kBlurTW = 64, kBlurTH = 16, kBlurMaxR = 8;
+
+template <int R>
+__global__ __launch_bounds__(256) void k_blur_fused(const float *__restrict__ src, float *__restrict__ dst,
+                                                    int W, int H, BlurWeights bw)
 {
-    const int x = blockIdx.x * 64 + (threadIdx.x & 63);
-    const int y = blockIdx.y * 4 + (threadIdx.x >> 6);
-    if (x >= W || y >= H) return;
-    float r = src[(size_t)x + (size_t)y * W] * bw.w[0];
-    for (int i = 1; i <= bw.R; ++i) {
-        if (HORIZ) {
-            r += src[(size_t)clampi(x + i, 0, W - 1) + (size_t)y * W] * bw.w[i];
-            r += src[(size_t)clampi(x - i, 0, W - 1) + (size_t)y * W] * bw.w[i];
-        } else {
-            r += src[(size_t)x + (size_t)clampi(y + i, 0, H - 1) * W] * bw.w[i];
-            r += src[(size_t)x + (size_t)clampi(y - i, 0, H - 1) * W] * bw.w[i];
-        }
+    __shared__ float s[kBlurTH + 2 * kBlurMaxR][kBlurTW + 2 * kBlurMaxR];
+    __shared__ float t[kBlurTH + 2 * kBlurMaxR][kBlurTW];
+    constexpr int rows = kBlurTH + 2 * R, cols = kBlurTW + 2 * R;
+    const int x0 = blockIdx.x * kBlurTW, y0 = blockIdx.y * kBlurTH;
+    for (int k = threadIdx.x; k < rows * cols; k += 256) {
+        const int j = k / cols, i = k - j * cols;
+        const int gy = clampi(y0 - R + j, 0, H - 1), gx = clampi(x0 - R + i, 0, W - 1);
+        s[j][i] = src[(size_t)gx + (size_t)gy * W];
     }
-    dst[(size_t)x + (size_t)y * W] = r;
+    __syncthreads();
+    for (int k = threadIdx.x; k < rows * kBlurTW; k += 256) {
+        const int j = k / kBlurTW, x = k - j * kBlurTW;
+        float r = s[j][x + R] * bw.w[0];
+#pragma unroll
+        for (int i = 1; i <= R; ++i) {
+            r += s[j][x + R + i] * bw.w[i];
+            r += s[j][x + R - i] * bw.w[i];
+        }
+        t[j][x] = r;
+    }
+    __syncthreads();
+    for (int k = threadIdx.x; k < kBlurTH * kBlurTW; k += 256) {
+        const int yy = k / kBlurTW, x = k - yy * kBlurTW;
+        const int gx = x0 + x, gy = y0 + yy;
+        if (gx >= W || gy >= H) continue;
+        float r = t[yy + R][x] * bw.w[0];
+#pragma unroll
+        for (int i = 1; i <= R; ++i) {
+            r += t[yy + R + i][x] * bw.w[i];
+            r += t[yy + R - i][x] * bw.w[i];
+        }
+        dst[(size_t)gx + (size_t)gy * W] = r;
+    }
 }
 
 __global__ __launch_bounds__(256) void k_resolve(const float *__restrict__ fb, int W, int H, int k,
@@ -327,15 +355,21 @@ void launch_blur_recursive(float *ao, int W, int H, hipStream_t s)
     hipLaunchKernelGGL(k_blur_rec_cols, dim3((W + 63) / 64), dim3(64), 0, s, ao, W, H);
 }
 
-void launch_blur_separable(float *ao, float *tmp, int W, int H, const float *w, int R,
-                           hipStream_t s)
+void launch_blur_separable(const float *src, float *dst, int W, int H, const float *w, int R, hipStream_t s)
 {
     if (W <= 0 || H <= 0) return;
     BlurWeights bw;
     for (int i = 0; i < 9; ++i) bw.w[i] = i <= R ? w[i] : 0.f;
     bw.R = R;
-    hipLaunchKernelGGL(k_blur_sep<true>, grid2(W, H), dim3(256), 0, s, ao, tmp, W, H, bw);
-    hipLaunchKernelGGL(k_blur_sep<false>, grid2(W, H), dim3(256), 0, s, tmp, ao, W, H, bw);
+    const dim3 grid((W + kBlurTW - 1) / kBlurTW, (H + kBlurTH - 1) / kBlurTH);
+    switch (R) {
+#define FGL_BLUR_CASE(r) \
+    case r: hipLaunchKernelGGL(k_blur_fused<r>, grid, dim3(256), 0, s, src, dst, W, H, bw); break;
+        FGL_BLUR_CASE(1) FGL_BLUR_CASE(2) FGL_BLUR_CASE(3) FGL_BLUR_CASE(4)
+        FGL_BLUR_CASE(5) FGL_BLUR_CASE(6) FGL_BLUR_CASE(7) FGL_BLUR_CASE(8)
+#undef FGL_BLUR_CASE
+    default: break;
+    }
 }
 
 void launch_resolve(const float *fb, int W, int H, int k, uint8_t *img, hipStream_t s)
