@@ -116,6 +116,7 @@ def lib():
             "fgl_target_bytes": (C.c_int, [P, C.c_int, C.POINTER(C.c_size_t)]),
             "fgl_enable_timing": (C.c_int, [P, C.c_int]),
             "fgl_time_only": (C.c_int, [P, C.c_char_p]),
+            "fgl_sort_pairs": (C.c_int, [P, P, P, C.c_size_t, C.c_int]),
             "fgl_read_timings": (C.c_int, [P, P, P, C.c_int, C.POINTER(C.c_int)]),
             "fgl_scene_load": (C.c_int, [C.c_char_p, C.POINTER(P), C.c_char_p, C.c_size_t]),
             "fgl_scene_free": (None, [P]),
@@ -376,6 +377,15 @@ class Context:
     def write(self, which, arr):
         a = f32(arr).reshape(-1)
         self._chk(lib().fgl_write_target(self._h, which, _ptr(a), a.nbytes), "fgl_write_target")
+
+    def sort_pairs(self, keys, vals, bits):
+        """The binning's device radix sort (stable, by key) on copies of the
+        arrays; returns the sorted (keys, vals)."""
+        k = np.ascontiguousarray(keys, np.uint32).copy()
+        v = np.ascontiguousarray(vals, np.uint32).copy()
+        assert k.shape == v.shape and k.ndim == 1
+        self._chk(lib().fgl_sort_pairs(self._h, _ptr(k), _ptr(v), k.size, bits), "fgl_sort_pairs")
+        return k, v
 
     def enable_timing(self, on=True, only=None):
         """Kernel timing with HIP events; `only` restricts it to the kernels

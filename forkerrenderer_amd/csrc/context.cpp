@@ -1147,6 +1147,36 @@ int fgl_copy_benchmark(fgl_ctx *c, size_t bytes, int iters, float *ms_per_iter)
     return FGL_OK;
 }
 
+int fgl_sort_pairs(fgl_ctx *c, uint32_t *keys, uint32_t *vals, size_t n, int bits)
+{
+    if (!c || (n && (!keys || !vals)) || n > 0x80000000ull || bits < 1 || bits > 32) return FGL_ERR_INVALID;
+    if (n == 0) return FGL_OK;
+    int rc = flush_pass(c);
+    if (rc) return rc;
+    const size_t bytes = n * 4;
+    const size_t tmp_bytes = radix_tmp_words(n) * 4;
+    void *buf[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
+    const size_t sz[5] = {bytes, bytes, bytes, bytes, tmp_bytes};
+    for (int i = 0; i < 5; ++i)
+        if (hipMalloc(&buf[i], sz[i]) != hipSuccess) {
+            for (int j = 0; j < i; ++j) hipFree(buf[j]);
+            return fail(c, FGL_ERR_NOMEM, "sort buffers");
+        }
+    hipError_t e = hipMemcpyAsync(buf[0], keys, bytes, hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(buf[1], vals, bytes, hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess) {
+        launch_radix_sort((uint32_t *)buf[0], (uint32_t *)buf[1], (uint32_t *)buf[2], (uint32_t *)buf[3], n, bits,
+                          (uint32_t *)buf[4], c->stream);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipMemcpyAsync(keys, buf[0], bytes, hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(vals, buf[1], bytes, hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    for (int i = 0; i < 5; ++i) hipFree(buf[i]);
+    if (e != hipSuccess) return fail(c, FGL_ERR_DEVICE, hipGetErrorString(e));
+    return FGL_OK;
+}
+
 }  // extern "C"
 
 namespace fgl {

@@ -144,6 +144,14 @@ __global__ __launch_bounds__(kRadixBlock) void k_radix_hist(const uint32_t *__re
     hist[(size_t)threadIdx.x * nb + blockIdx.x] = h[threadIdx.x];
 }
 
+// Scatter of one 2048-item block: ranks by wave ballots as above, but the
+// block first reorders its items by digit in LDS and then writes each
+// digit's run with consecutive lanes on consecutive addresses.  Writing
+// straight from the ranks leaves runs of a few items per digit and round
+// (28-byte pieces of 128-byte lines when a block's items spread over a few
+// dozen digits), which costs 3-4x the time of whole-line writes.  The
+// block's count of a digit is the difference of two scanned histogram
+// entries (hist[256 * nb] holds the total).
 __global__ __launch_bounds__(kRadixBlock) void k_radix_scatter(
     const uint32_t *__restrict__ kin, const uint32_t *__restrict__ vin,
     uint32_t *__restrict__ kout, uint32_t *__restrict__ vout, uint32_t n,
@@ -151,10 +159,18 @@ __global__ __launch_bounds__(kRadixBlock) void k_radix_scatter(
 {
     if (n_dev) n = min(n, *n_dev);
     __shared__ uint32_t goff[256];
+    __shared__ uint32_t lstart[256];
     __shared__ uint32_t dbase[256];
     __shared__ uint32_t wcnt[kRadixBlock / 64][256];
+    __shared__ uint32_t lds4[4];
+    __shared__ uint32_t sk[kRadixTile], sv[kRadixTile];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    goff[tid] = hist[(size_t)tid * nb + blockIdx.x];
+    const size_t hi = (size_t)tid * nb + blockIdx.x;
+    const uint32_t g0 = hist[hi];
+    const uint32_t g1 = (blockIdx.x + 1 < (unsigned)nb) ? hist[hi + 1] : hist[(size_t)(tid + 1) * nb];
+    goff[tid] = g0;
+    uint32_t total;
+    lstart[tid] = block_exclusive_scan(g1 - g0, lds4, total);
     dbase[tid] = 0;
 #pragma unroll
     for (int w = 0; w < kRadixBlock / 64; ++w) wcnt[w][tid] = 0;
@@ -178,10 +194,10 @@ __global__ __launch_bounds__(kRadixBlock) void k_radix_scatter(
         if (valid && rank == 0) wcnt[wave][d] = (uint32_t)__popcll(m);
         __syncthreads();
         if (valid) {
-            uint32_t pos = goff[d] + dbase[d] + rank;
+            uint32_t pos = lstart[d] + dbase[d] + rank;
             for (int w = 0; w < wave; ++w) pos += wcnt[w][d];
-            kout[pos] = k;
-            vout[pos] = v;
+            sk[pos] = k;
+            sv[pos] = v;
         }
         __syncthreads();
         uint32_t add = 0;
@@ -192,6 +208,14 @@ __global__ __launch_bounds__(kRadixBlock) void k_radix_scatter(
         }
         dbase[tid] += add;
         __syncthreads();
+    }
+    const uint32_t nvalid = n > base ? min((uint32_t)kRadixTile, n - base) : 0u;
+    for (uint32_t i = tid; i < nvalid; i += kRadixBlock) {
+        const uint32_t k = sk[i];
+        const uint32_t d = (k >> shift) & 255u;
+        const uint32_t pos = goff[d] + (i - lstart[d]);
+        kout[pos] = k;
+        vout[pos] = sv[i];
     }
 }
 

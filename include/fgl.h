@@ -235,6 +235,12 @@ int fgl_time_only(fgl_ctx *ctx, const char *name);
  * bandwidth figures.  Allocates and frees its own buffers. */
 int fgl_copy_benchmark(fgl_ctx *ctx, size_t bytes, int iters, float *ms_per_iter);
 
+/* Diagnostic with no counterpart in the reference: the binning's device
+ * radix sort on caller arrays (n <= 2^31 key/value pairs, keys < 2^bits,
+ * bits 1..32): a stable sort of the pairs by key, in place.  Allocates and
+ * frees its own buffers. */
+int fgl_sort_pairs(fgl_ctx *ctx, uint32_t *keys, uint32_t *vals, size_t n, int bits);
+
 /* ---- scene files: .scene/.obj/.mtl/.tga (src/scene.cpp, src/model.cpp, src/tgaimage.cpp) ---- */
 typedef struct fgl_scene fgl_scene;
 typedef struct {
