@@ -749,11 +749,10 @@ static int flush_pass(fgl_ctx *c)
     uint32_t *big_count = (uint32_t *)c->counters.p;
     {
         KernelTimer t(c, "vertex");
-        launch_vertex(shadow, ddraws, n_draws, max_pos, u, (int4 *)c->screen.p, c->stream);
+        launch_vertex(shadow, ddraws, n_draws, max_pos, u, (int4 *)c->screen.p, big_count, c->stream);
     }
     {
         KernelTimer t(c, "setup");
-        launch_fill_u32(big_count, 0, 1, c->stream);
         launch_setup(ddraws, n_draws, max_faces, (const int4 *)c->screen.p, W, H, TS,
                      (uint2 *)c->rect.p, (uint32_t *)c->counts.p, (int4 *)c->tri_v.p,
                      (uint32_t *)c->big_list.p, big_count, c->stream);
@@ -774,7 +773,8 @@ static int flush_pass(fgl_ctx *c)
         KernelTimer t(c, "emit");
         launch_emit((const uint32_t *)c->offs.p, (int)nt, (const uint2 *)c->rect.p, tiles_x,
                     (const uint32_t *)c->big_list.p, n_big, (uint32_t *)c->keys.p,
-                    (uint32_t *)c->vals.p, c->stream);
+                    (uint32_t *)c->vals.p, (uint32_t *)c->tile_start.p, (uint32_t *)c->tile_end.p,
+                    (int)ntiles, c->stream);
     }
     {
         KernelTimer t(c, "sort");
@@ -784,8 +784,6 @@ static int flush_pass(fgl_ctx *c)
     }
     {
         KernelTimer t(c, "tile_ranges");
-        launch_fill_u32((uint32_t *)c->tile_start.p, 0, ntiles, c->stream);
-        launch_fill_u32((uint32_t *)c->tile_end.p, 0, ntiles, c->stream);
         launch_tile_ranges((const uint32_t *)c->keys.p, npairs, nullptr, (uint32_t *)c->tile_start.p,
                            (uint32_t *)c->tile_end.p, c->stream);
     }

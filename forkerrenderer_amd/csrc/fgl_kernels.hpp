@@ -50,14 +50,18 @@ struct SsaoArgs {
 };
 
 // raster.hip
+// Window records of every (draw, position); zero (may be null): a word set
+// to 0 on the way.
 void launch_vertex(bool shadow, const DevDraw *draws, int n_draws, int max_pos,
-                   const FrameUniforms &u, int4 *out, hipStream_t s);
+                   const FrameUniforms &u, int4 *out, uint32_t *zero, hipStream_t s);
 void launch_setup(const DevDraw *draws, int n_draws, int max_faces, const int4 *screen, int W,
                   int H, int TS, uint2 *rect, uint32_t *count, int4 *tri_v, uint32_t *big_list,
                   uint32_t *big_count, hipStream_t s);
+// Pairs of the pass (offsets from the scan of the per-triangle counts); the
+// first n_clear entries of range_a and range_b are zeroed on the way.
 void launch_emit(const uint32_t *off, int n_tri, const uint2 *rect, int tiles_x,
                  const uint32_t *big_list, uint32_t n_big, uint32_t *keys, uint32_t *vals,
-                 hipStream_t s);
+                 uint32_t *range_a, uint32_t *range_b, int n_clear, hipStream_t s);
 // host: a device pointer to host memory; writes *a, *b, then seq.
 void launch_publish2(const uint32_t *a, const uint32_t *b, uint32_t seq, uint32_t *host, hipStream_t s);
 void launch_fill_u32(uint32_t *p, uint32_t v, size_t n, hipStream_t s);

@@ -19,12 +19,15 @@
 
 namespace fgl {
 
+// One thread per (draw, position); the first thread also zeroes *zero (the
+// pass's big-triangle counter, which setup counts up next).
 template <bool SHADOW>
 __global__ __launch_bounds__(256) void k_vertex(const DevDraw *__restrict__ draws, FrameUniforms u,
-                                                int4 *__restrict__ out)
+                                                int4 *__restrict__ out, uint32_t *__restrict__ zero)
 {
     const DevDraw &dr = draws[blockIdx.y];
     const int vi = blockIdx.x * blockDim.x + threadIdx.x;
+    if (zero && (vi | blockIdx.y) == 0) *zero = 0u;
     if (vi >= dr.n_pos) return;
     if (SHADOW) out[dr.vert_base + vi] = to_screen(u.shadow_viewport, depth_ndc(dr, vi));
     else out[dr.vert_base + vi] = to_screen(u.viewport, main_ndc(dr, u, vi));
@@ -47,12 +50,20 @@ __global__ __launch_bounds__(256) void k_setup(const DevDraw *__restrict__ draws
 
 // Pairs of triangle t at [off[t], off[t+1]), row-major over its tile
 // rectangle; triangles with more than kSmallTriTiles tiles go to k_emit_big.
+// The first n_clear threads also zero the per-tile ranges (range_a/range_b)
+// that k_tile_ranges fills after the sort.
 __global__ __launch_bounds__(256) void k_emit_small(const uint32_t *__restrict__ off, int n_tri,
                                                     const uint2 *__restrict__ rect, int tiles_x,
                                                     uint32_t *__restrict__ keys,
-                                                    uint32_t *__restrict__ vals)
+                                                    uint32_t *__restrict__ vals,
+                                                    uint32_t *__restrict__ range_a,
+                                                    uint32_t *__restrict__ range_b, int n_clear)
 {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < n_clear) {
+        range_a[t] = 0u;
+        range_b[t] = 0u;
+    }
     if (t >= n_tri) return;
     const uint32_t o = off[t], n = off[t + 1] - o;
     if (n == 0 || n > (uint32_t)kSmallTriTiles) return;
@@ -192,12 +203,15 @@ __global__ __launch_bounds__(256) void k_forward_resolve(ResolveArgs a, FrameUni
 static inline unsigned blocks_for(size_t n, unsigned b) { return (unsigned)((n + b - 1) / b); }
 
 void launch_vertex(bool shadow, const DevDraw *draws, int n_draws, int max_pos,
-                   const FrameUniforms &u, int4 *out, hipStream_t s)
+                   const FrameUniforms &u, int4 *out, uint32_t *zero, hipStream_t s)
 {
-    if (n_draws <= 0 || max_pos <= 0) return;
+    if (n_draws <= 0 || max_pos <= 0) {
+        if (zero) launch_fill_u32(zero, 0u, 1, s);
+        return;
+    }
     dim3 grid(blocks_for(max_pos, 256), n_draws);
-    if (shadow) hipLaunchKernelGGL(k_vertex<true>, grid, dim3(256), 0, s, draws, u, out);
-    else hipLaunchKernelGGL(k_vertex<false>, grid, dim3(256), 0, s, draws, u, out);
+    if (shadow) hipLaunchKernelGGL(k_vertex<true>, grid, dim3(256), 0, s, draws, u, out, zero);
+    else hipLaunchKernelGGL(k_vertex<false>, grid, dim3(256), 0, s, draws, u, out, zero);
 }
 
 void launch_setup(const DevDraw *draws, int n_draws, int max_faces, const int4 *screen, int W,
@@ -211,11 +225,12 @@ void launch_setup(const DevDraw *draws, int n_draws, int max_faces, const int4 *
 
 void launch_emit(const uint32_t *off, int n_tri, const uint2 *rect, int tiles_x,
                  const uint32_t *big_list, uint32_t n_big, uint32_t *keys, uint32_t *vals,
-                 hipStream_t s)
+                 uint32_t *range_a, uint32_t *range_b, int n_clear, hipStream_t s)
 {
-    if (n_tri <= 0) return;
-    hipLaunchKernelGGL(k_emit_small, dim3(blocks_for(n_tri, 256)), dim3(256), 0, s, off, n_tri,
-                       rect, tiles_x, keys, vals);
+    const int n = n_tri > n_clear ? n_tri : n_clear;
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_emit_small, dim3(blocks_for(n, 256)), dim3(256), 0, s, off, n_tri,
+                       rect, tiles_x, keys, vals, range_a, range_b, n_clear);
     if (n_big)
         hipLaunchKernelGGL(k_emit_big, dim3(n_big), dim3(256), 0, s, off, rect, tiles_x, big_list,
                            keys, vals);

@@ -48,15 +48,30 @@ __device__ __forceinline__ uint32_t block_exclusive_scan(uint32_t v, uint32_t *l
     return prefix + inc - v;
 }
 
+// A thread's kScanItems consecutive values: two 16-byte loads when all are
+// in range and the array is 16-byte aligned, else one load per value.
+__device__ __forceinline__ void scan_load8(const uint32_t *in, size_t n, size_t base, bool vec, uint32_t *v)
+{
+    if (vec && base + kScanItems <= n) {
+        const uint4 a = *(const uint4 *)(in + base), b = *(const uint4 *)(in + base + 4);
+        v[0] = a.x; v[1] = a.y; v[2] = a.z; v[3] = a.w;
+        v[4] = b.x; v[5] = b.y; v[6] = b.z; v[7] = b.w;
+    } else {
+#pragma unroll
+        for (int i = 0; i < kScanItems; ++i) v[i] = (base + i < n) ? in[base + i] : 0u;
+    }
+}
+
 __global__ __launch_bounds__(kScanBlock) void k_scan_reduce(const uint32_t *__restrict__ in,
                                                             size_t n, uint32_t *__restrict__ part)
 {
     __shared__ uint32_t lds4[4];
     size_t base = (size_t)blockIdx.x * kScanTile + (size_t)threadIdx.x * kScanItems;
+    uint32_t v[kScanItems];
+    scan_load8(in, n, base, ((uintptr_t)in & 15u) == 0, v);
     uint32_t s = 0;
 #pragma unroll
-    for (int i = 0; i < kScanItems; ++i)
-        if (base + i < n) s += in[base + i];
+    for (int i = 0; i < kScanItems; ++i) s += v[i];
     uint32_t total;
     block_exclusive_scan(s, lds4, total);
     if (threadIdx.x == 0) part[blockIdx.x] = total;
@@ -86,19 +101,27 @@ __global__ __launch_bounds__(kScanBlock) void k_scan_down(const uint32_t *in, ui
 {
     __shared__ uint32_t lds4[4];
     size_t base = (size_t)blockIdx.x * kScanTile + (size_t)threadIdx.x * kScanItems;
+    const bool vec = (((uintptr_t)in | (uintptr_t)out) & 15u) == 0;
     uint32_t v[kScanItems];
+    scan_load8(in, n, base, vec, v);
     uint32_t s = 0;
 #pragma unroll
-    for (int i = 0; i < kScanItems; ++i) {
-        v[i] = (base + i < n) ? in[base + i] : 0u;
-        s += v[i];
-    }
+    for (int i = 0; i < kScanItems; ++i) s += v[i];
     uint32_t total;
     uint32_t ex = block_exclusive_scan(s, lds4, total) + part[blockIdx.x];
+    uint32_t o[kScanItems];
 #pragma unroll
     for (int i = 0; i < kScanItems; ++i) {
-        if (base + i < n) out[base + i] = ex;
+        o[i] = ex;
         ex += v[i];
+    }
+    if (vec && base + kScanItems <= n) {
+        *(uint4 *)(out + base) = make_uint4(o[0], o[1], o[2], o[3]);
+        *(uint4 *)(out + base + 4) = make_uint4(o[4], o[5], o[6], o[7]);
+    } else {
+#pragma unroll
+        for (int i = 0; i < kScanItems; ++i)
+            if (base + i < n) out[base + i] = o[i];
     }
 }
 
