@@ -117,6 +117,7 @@ def lib():
             "fgl_enable_timing": (C.c_int, [P, C.c_int]),
             "fgl_time_only": (C.c_int, [P, C.c_char_p]),
             "fgl_sort_pairs": (C.c_int, [P, P, P, C.c_size_t, C.c_int]),
+            "fgl_check_recip": (C.c_int, [P, P, P, P]),
             "fgl_read_timings": (C.c_int, [P, P, P, C.c_int, C.POINTER(C.c_int)]),
             "fgl_scene_load": (C.c_int, [C.c_char_p, C.POINTER(P), C.c_char_p, C.c_size_t]),
             "fgl_scene_free": (None, [P]),
@@ -377,6 +378,13 @@ class Context:
     def write(self, which, arr):
         a = f32(arr).reshape(-1)
         self._chk(lib().fgl_write_target(self._h, which, _ptr(a), a.nbytes), "fgl_write_target")
+
+    def check_recip(self):
+        """(compared, mismatches, fast_path) of the kernels' reciprocal against
+        the IEEE quotient over every non-NaN float."""
+        a, b, f = C.c_uint64(), C.c_uint64(), C.c_uint64()
+        self._chk(lib().fgl_check_recip(self._h, C.byref(a), C.byref(b), C.byref(f)), "fgl_check_recip")
+        return a.value, b.value, f.value
 
     def sort_pairs(self, keys, vals, bits):
         """The binning's device radix sort (stable, by key) on copies of the

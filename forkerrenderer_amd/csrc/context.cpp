@@ -1145,6 +1145,28 @@ int fgl_copy_benchmark(fgl_ctx *c, size_t bytes, int iters, float *ms_per_iter)
     return FGL_OK;
 }
 
+int fgl_check_recip(fgl_ctx *c, uint64_t *compared, uint64_t *mismatches, uint64_t *fast_path)
+{
+    if (!c) return FGL_ERR_INVALID;
+    int rc = flush_pass(c);
+    if (rc) return rc;
+    unsigned long long *d = nullptr, h[3] = {0, 0, 0};
+    HIPCHK(c, hipMalloc(&d, sizeof(h)));
+    hipError_t e = hipMemcpyAsync(d, h, sizeof(h), hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess) {
+        launch_check_recip(d, c->stream);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipMemcpyAsync(h, d, sizeof(h), hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    hipFree(d);
+    if (e != hipSuccess) return fail(c, FGL_ERR_DEVICE, hipGetErrorString(e));
+    if (compared) *compared = h[0];
+    if (mismatches) *mismatches = h[1];
+    if (fast_path) *fast_path = h[2];
+    return FGL_OK;
+}
+
 int fgl_sort_pairs(fgl_ctx *c, uint32_t *keys, uint32_t *vals, size_t n, int bits)
 {
     if (!c || (n && (!keys || !vals)) || n > 0x80000000ull || bits < 1 || bits > 32) return FGL_ERR_INVALID;

@@ -160,7 +160,7 @@ __device__ __forceinline__ Varying main_vertex(const DevDraw &d, const FrameUnif
         l = mv4(u.light_space, ws);
         l = divs4(l, l.w);
     }
-    float oow = 1.f / cs.w;
+    float oow = recip(cs.w);
     ws = scale4(ws, oow);
     uv.x *= oow;
     uv.y *= oow;
@@ -208,7 +208,7 @@ __device__ __forceinline__ FragGeom frag_geometry(const DevDraw &d, const FrameU
     Vec3 pws = interp3(v0.ws, v1.ws, v2.ws, b);
     Vec2 uv = interp2(v0.uv, v1.uv, v2.uv, b);
     Vec3 nws = interp3(v0.nws, v1.nws, v2.nws, b);
-    float w = 1.f / dot3(vec3(v0.oow, v1.oow, v2.oow), b);
+    float w = recip(dot3(vec3(v0.oow, v1.oow, v2.oow), b));
     pws = scale3(pws, w);
     uv.x *= w;
     uv.y *= w;

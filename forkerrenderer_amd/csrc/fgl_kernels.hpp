@@ -137,6 +137,9 @@ void launch_blur_recursive(float *ao, int W, int H, hipStream_t s);
 // Separable Gaussian of radius R (1..8), horizontal then vertical, clamped
 // taps, in one launch from src into dst (distinct buffers).
 void launch_blur_separable(const float *src, float *dst, int W, int H, const float *w, int R, hipStream_t s);
+// counts[0..2] (zeroed by the caller) += floats compared, mismatches of
+// recip() against 1.f / w, floats on recip's fast path.
+void launch_check_recip(unsigned long long *counts, hipStream_t s);
 void launch_resolve(const float *fb, int W, int H, int k, uint8_t *img, hipStream_t s);
 void launch_copy(const void *src, void *dst, size_t bytes, hipStream_t s);
 

@@ -372,6 +372,47 @@ void launch_blur_separable(const float *src, float *dst, int W, int H, const flo
     }
 }
 
+// recip(w) against the IEEE quotient 1.f / w for every non-NaN float w:
+// counts[0] += floats compared, counts[1] += those that differ, counts[2] +=
+// those that took the reciprocal-and-Newton path.
+__global__ __launch_bounds__(256) void k_check_recip(unsigned long long *__restrict__ counts)
+{
+    __shared__ unsigned long long part[3][4];
+    unsigned long long n = 0, bad = 0, fast = 0;
+    const uint64_t stride = (uint64_t)gridDim.x * 256;
+    for (uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x; i < (1ull << 32); i += stride) {
+        const float w = __uint_as_float((uint32_t)i);
+        if (w != w) continue;
+        ++n;
+        const float a = fabsf(w);
+        fast += (a >= 0x1p-125f && a <= 0x1p125f) ? 1u : 0u;
+        bad += __float_as_uint(recip(w)) != __float_as_uint(1.f / w) ? 1u : 0u;
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        n += __shfl_xor(n, o, 64);
+        bad += __shfl_xor(bad, o, 64);
+        fast += __shfl_xor(fast, o, 64);
+    }
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    if (lane == 0) {
+        part[0][wave] = n;
+        part[1][wave] = bad;
+        part[2][wave] = fast;
+    }
+    __syncthreads();
+    if (threadIdx.x < 3) {
+        const unsigned long long v = part[threadIdx.x][0] + part[threadIdx.x][1] + part[threadIdx.x][2] +
+                                     part[threadIdx.x][3];
+        atomicAdd(&counts[threadIdx.x], v);
+    }
+}
+
+void launch_check_recip(unsigned long long *counts, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_check_recip, dim3(4096), dim3(256), 0, s, counts);
+}
+
 void launch_resolve(const float *fb, int W, int H, int k, uint8_t *img, hipStream_t s)
 {
     int ow = W / k, oh = H / k;

@@ -68,19 +68,25 @@ __device__ __forceinline__ void ssao_px(const FrameUniforms &u, const SsaoArgs &
     }
     const float *vp = u.viewport;
     const float radius = sa.radius;
+    // The tap is on screen iff trunc_int(sxf) lies in [0, W): for sxf in
+    // (-1, W) the truncation is in range, and outside it (NaN included) it
+    // is not (trunc_int maps what does not fit an int to INT_MIN).  Testing
+    // the float bounds is that test with fewer operations; W and H are
+    // exact in float.
+    const float Wf = (float)u.W, Hf = (float)u.H;
     int count = 0;
     for (int s = 0; s < sa.n_kernel; ++s) {
         const float kx = sa.kernel[3 * s], ky = sa.kernel[3 * s + 1], kz = sa.kernel[3 * s + 2];
         float cs[4];
 #pragma unroll
         for (int j = 0; j < 4; ++j) cs[j] = fmaf(radius, fmaf(kz, cN[j], fmaf(ky, cB[j], kx * cT[j])), cc[j]);
-        const float inv = 1.f / cs[3];
+        const float inv = recip(cs[3]);
         const float sxf = fmaf(vp[0], cs[0] * inv, vp[3]);
         const float syf = fmaf(vp[5], cs[1] * inv, vp[7]);
+        if (!(sxf > -1.f && sxf < Wf && syf > -1.f && syf < Hf)) continue;
         const float sz = fmaf(vp[10], cs[2] * inv, vp[11]);
-        const int sx = trunc_int(sxf), sy = trunc_int(syf);
-        if (sx < 0 || sy < 0 || sx >= u.W || sy >= u.H) continue;
-        const float cached = depth[(size_t)sx + (size_t)sy * u.W];
+        const uint32_t t = (uint32_t)(int)sxf + (uint32_t)(int)syf * (uint32_t)u.W;
+        const float cached = depth[t];
         if (sz >= cached + sa.bias && fabsf(fd - cached) < sa.range) ++count;
     }
     const float a = 1.f - (float)count * (1.f / (float)sa.n_kernel);

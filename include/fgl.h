@@ -241,6 +241,13 @@ int fgl_copy_benchmark(fgl_ctx *ctx, size_t bytes, int iters, float *ms_per_iter
  * frees its own buffers. */
 int fgl_sort_pairs(fgl_ctx *ctx, uint32_t *keys, uint32_t *vals, size_t n, int bits);
 
+/* Diagnostic with no counterpart in the reference: the kernels' correctly
+ * rounded reciprocal (hardware reciprocal + one fma Newton step for |w| in
+ * [2^-125, 2^125], the IEEE division elsewhere) compared with the IEEE
+ * quotient 1.f / w for every non-NaN float w on this device.  Writes the
+ * number of floats compared, of mismatches, and of floats on the fast path. */
+int fgl_check_recip(fgl_ctx *ctx, uint64_t *compared, uint64_t *mismatches, uint64_t *fast_path);
+
 /* ---- scene files: .scene/.obj/.mtl/.tga (src/scene.cpp, src/model.cpp, src/tgaimage.cpp) ---- */
 typedef struct fgl_scene fgl_scene;
 typedef struct {
