@@ -390,25 +390,15 @@ __device__ __forceinline__ Vec3 winner_bary(const int4 *__restrict__ screen,
 // Geometry pass resolve of pixel (x,y): GShader::ProcessFragment + the seven
 // buffer writes (gshader.h:95-202, forkergl.cpp:211-223) for the visible
 // fragment; a pixel with no fragment gets the cleared values when `fresh`.
-__device__ __forceinline__ void gbuffer_px(const ResolveArgs &a, const FrameUniforms &u, int x, int y)
+// GShader::ProcessFragment for the winner (face f of draw dr, material m,
+// barycentrics b) at pixel i: the 19 G-buffer planes.
+__device__ __forceinline__ void gbuffer_shade(const ResolveArgs &a, const FrameUniforms &u, const DevDraw &dr,
+                                              const DevMaterial &m, int f, const Vec3 &b, size_t i, size_t P)
 {
-    const size_t i = (size_t)x + (size_t)y * a.W;
-    const size_t P = (size_t)a.W * a.H;
     float *gb = a.gbuf;
-    const int g = a.vis[i];
-    if (g < 0) {
-        if (a.fresh)
-            for (int c = 0; c < kGbufPlanes; ++c) gb[c * P + i] = 0.f;
-        return;
-    }
-    int4 tv;
-    const Vec3 b = winner_bary(a.screen, a.tri_v, g, a.W, a.H, x, y, tv);
-    const DevDraw &dr = a.draws[tv.w];
-    const int f = g - dr.tri_base;
     Varying v0 = main_vertex(dr, u, f, 0);
     Varying v1 = main_vertex(dr, u, f, 1);
     Varying v2 = main_vertex(dr, u, f, 2);
-    const DevMaterial &m = a.materials[dr.face_mat[f]];
     FragGeom fg = frag_geometry(dr, u, a.textures, v0, v1, v2, b, m.map_bump);
     Vec3 albedo, emissive, param;
     float type;
@@ -442,6 +432,40 @@ __device__ __forceinline__ void gbuffer_px(const ResolveArgs &a, const FrameUnif
     gb[12 * P + i] = emissive.x; gb[13 * P + i] = emissive.y; gb[14 * P + i] = emissive.z;
     gb[15 * P + i] = param.x; gb[16 * P + i] = param.y; gb[17 * P + i] = param.z;
     gb[18 * P + i] = type;
+}
+
+// G-buffer resolve of pixel (x,y).  When every active lane of the wavefront
+// resolves a face of the same draw with the same material (the common case:
+// neighbouring pixels show the same mesh), the draw and material records are
+// addressed through readfirstlane, so their fields (matrices, flags, texture
+// descriptors) come in as scalar loads instead of one vector load per field.
+__device__ __forceinline__ void gbuffer_px(const ResolveArgs &a, const FrameUniforms &u, int x, int y)
+{
+    const size_t i = (size_t)x + (size_t)y * a.W;
+    const size_t P = (size_t)a.W * a.H;
+    float *gb = a.gbuf;
+    const int g = a.vis[i];
+    if (g < 0) {
+        if (a.fresh)
+            for (int c = 0; c < kGbufPlanes; ++c) gb[c * P + i] = 0.f;
+        return;
+    }
+    int4 tv;
+    const Vec3 b = winner_bary(a.screen, a.tri_v, g, a.W, a.H, x, y, tv);
+    const int d0 = __builtin_amdgcn_readfirstlane(tv.w);
+    if (__ballot(tv.w != d0) == 0ull) {
+        const DevDraw &dr = a.draws[d0];
+        const int f = g - dr.tri_base;
+        const int mi = dr.face_mat[f];
+        const int m0 = __builtin_amdgcn_readfirstlane(mi);
+        if (__ballot(mi != m0) == 0ull) {
+            gbuffer_shade(a, u, dr, a.materials[m0], f, b, i, P);
+            return;
+        }
+    }
+    const DevDraw &dr = a.draws[tv.w];
+    const int f = g - dr.tri_base;
+    gbuffer_shade(a, u, dr, a.materials[dr.face_mat[f]], f, b, i, P);
 }
 
 }  // namespace fgl
