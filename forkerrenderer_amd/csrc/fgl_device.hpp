@@ -55,6 +55,9 @@ struct DevShadowCfg {
     double blocker_filter, pcss_filter;
     float blocker_disk[2 * kMaxDisk];
     float pcf_disk[2 * kMaxDisk];
+    // blocker_disk[i] * blocker_filter in double, narrowed to float (the
+    // reference's per-sample offset; the same for every pixel)
+    float blocker_off[2 * kMaxDisk];
 };
 
 struct FrameUniforms {
@@ -245,7 +248,10 @@ __device__ __forceinline__ float sample_shadow(const float *map, int SW, int SH,
     int x = (int)((float)w * u);
     int y = (int)((float)h * v);
     float dd = map[(size_t)x + (size_t)y * SW];
-    return ((double)dd < 0.001) ? 1.f : dd;
+    // (double)dd < 0.001: 0.001f is the float just above 0.001 (the one just
+    // below is 0.00099999993), so for a float dd the two tests agree, NaN
+    // included.
+    return (dd < 0.001f) ? 1.f : dd;
 }
 
 __device__ __forceinline__ float pcf_disk(const float *map, int SW, int SH, Vec3 sc, float bias,
@@ -262,15 +268,20 @@ __device__ __forceinline__ float pcf_disk(const float *map, int SW, int SH, Vec3
 }
 
 // PCSS blocker term (FindAverageBlockDepth, shadow.cpp:65-91): the average
-// depth of the samples that block, 0 when none does.
+// depth of the samples that block, 0 when none does.  SCALED: `disk` holds
+// the offsets already scaled (blocker_off), else unit-disk samples scaled
+// here in double as the reference does.
+template <bool SCALED>
 __device__ __forceinline__ float pcss_blocker_depth(const FrameUniforms &u, const float *map, Vec3 sc, float bias,
                                                     const float *disk, int n)
 {
     const DevShadowCfg &c = u.shadow;
     float blocker = 0.f, nb = 0.f;
     for (int i = 0; i < n; ++i) {
-        float su = sc.x + (float)((double)disk[2 * i] * c.blocker_filter);
-        float sv = sc.y + (float)((double)disk[2 * i + 1] * c.blocker_filter);
+        const float ou = SCALED ? disk[2 * i] : (float)((double)disk[2 * i] * c.blocker_filter);
+        const float ov = SCALED ? disk[2 * i + 1] : (float)((double)disk[2 * i + 1] * c.blocker_filter);
+        float su = sc.x + ou;
+        float sv = sc.y + ov;
         float s = sample_shadow(map, u.SW, u.SH, su, sv);
         if (sc.z > s + bias) {
             blocker += s;
@@ -298,8 +309,8 @@ __device__ __forceinline__ float shadow_visibility(const FrameUniforms &u, const
         // (two calls rather than a select between the argument-space sample
         // arrays and the global stream samples)
         float dB;
-        if (c.mode == 5) dB = pcss_blocker_depth(u, map, sc, bias, rdisk, c.n_blocker);
-        else dB = pcss_blocker_depth(u, map, sc, bias, c.blocker_disk, c.n_blocker);
+        if (c.mode == 5) dB = pcss_blocker_depth<false>(u, map, sc, bias, rdisk, c.n_blocker);
+        else dB = pcss_blocker_depth<true>(u, map, sc, bias, c.blocker_off, c.n_blocker);
         if ((double)dB < 0.001) return 1.f;
         float pen = (sc.z - dB) * c.area_light / dB;
         float filter = (float)(c.pcss_filter * (double)pen);

@@ -147,30 +147,14 @@ __global__ __launch_bounds__(256) void k_gbuffer_resolve(ResolveArgs a, FrameUni
 
 // Forward pass resolve: BlinnPhongShader / PBRShader::ProcessFragment
 // (phongshader.h:90-169, pbrshader.h:90-180) once for the visible fragment.
-__global__ __launch_bounds__(256) void k_forward_resolve(ResolveArgs a, FrameUniforms u)
+// The forward fragment program for the winner (face f of draw dr, material
+// m, barycentrics b) at pixel i.
+__device__ __forceinline__ void forward_shade(const ResolveArgs &a, const FrameUniforms &u, const DevDraw &dr,
+                                              const DevMaterial &m, int f, const Vec3 &b, size_t i, size_t P)
 {
-    const int x = blockIdx.x * 64 + (threadIdx.x & 63);
-    const int y = blockIdx.y * 4 + (threadIdx.x >> 6);
-    if (x >= a.W || y >= a.H) return;
-    const size_t i = (size_t)x + (size_t)y * a.W;
-    const size_t P = (size_t)a.W * a.H;
-    const int g = a.vis[i];
-    if (g < 0) {
-        if (a.fresh) {
-            a.fb[i] = a.clear[0];
-            a.fb[P + i] = a.clear[1];
-            a.fb[2 * P + i] = a.clear[2];
-        }
-        return;
-    }
-    int4 tv;
-    const Vec3 b = winner_bary(a.screen, a.tri_v, g, a.W, a.H, x, y, tv);
-    const DevDraw &dr = a.draws[tv.w];
-    const int f = g - dr.tri_base;
     Varying v0 = main_vertex(dr, u, f, 0);
     Varying v1 = main_vertex(dr, u, f, 1);
     Varying v2 = main_vertex(dr, u, f, 2);
-    const DevMaterial &m = a.materials[dr.face_mat[f]];
     FragGeom fg = frag_geometry(dr, u, a.textures, v0, v1, v2, b, m.map_bump);
     Vec3 c;
     if (!dr.support_pbr) {
@@ -196,6 +180,42 @@ __global__ __launch_bounds__(256) void k_forward_resolve(ResolveArgs a, FrameUni
     a.fb[i] = c.x;
     a.fb[P + i] = c.y;
     a.fb[2 * P + i] = c.z;
+}
+
+// Forward resolve, one thread per pixel; draw and material records through
+// readfirstlane when the wavefront shares them (as in gbuffer_px).
+__global__ __launch_bounds__(256) void k_forward_resolve(ResolveArgs a, FrameUniforms u)
+{
+    const int x = blockIdx.x * 64 + (threadIdx.x & 63);
+    const int y = blockIdx.y * 4 + (threadIdx.x >> 6);
+    if (x >= a.W || y >= a.H) return;
+    const size_t i = (size_t)x + (size_t)y * a.W;
+    const size_t P = (size_t)a.W * a.H;
+    const int g = a.vis[i];
+    if (g < 0) {
+        if (a.fresh) {
+            a.fb[i] = a.clear[0];
+            a.fb[P + i] = a.clear[1];
+            a.fb[2 * P + i] = a.clear[2];
+        }
+        return;
+    }
+    int4 tv;
+    const Vec3 b = winner_bary(a.screen, a.tri_v, g, a.W, a.H, x, y, tv);
+    const int d0 = __builtin_amdgcn_readfirstlane(tv.w);
+    if (__ballot(tv.w != d0) == 0ull) {
+        const DevDraw &dr = a.draws[d0];
+        const int f = g - dr.tri_base;
+        const int mi = dr.face_mat[f];
+        const int m0 = __builtin_amdgcn_readfirstlane(mi);
+        if (__ballot(mi != m0) == 0ull) {
+            forward_shade(a, u, dr, a.materials[m0], f, b, i, P);
+            return;
+        }
+    }
+    const DevDraw &dr = a.draws[tv.w];
+    const int f = g - dr.tri_base;
+    forward_shade(a, u, dr, a.materials[dr.face_mat[f]], f, b, i, P);
 }
 
 // ---------------- launch wrappers ----------------
