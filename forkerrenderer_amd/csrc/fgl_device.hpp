@@ -335,9 +335,26 @@ __device__ __forceinline__ float shadow_visibility(const FrameUniforms &u, const
 
 // ---------------- lighting (phongshader.h:171-215, pbrshader.h:182-288) ----------------
 
+// powf(x, y) for the non-negative (or NaN) bases of the shading code, as
+// exp2(y * log2(x)) with the hardware base-2 log and exp: a few ulps from the
+// correctly rounded power, which the 8-bit output absorbs (a result moves by
+// at most one 8-bit step, and only at a quantisation boundary), in ~8
+// operations instead of the ~160 of the general powf.  pow(x, 0) = 1;
+// denormal x and results below 2^-126 are rescaled (the hardware log and exp
+// take and give normal floats only).
+__device__ __forceinline__ float pow_pos(float x, float y)
+{
+    if (y == 0.f) return 1.f;
+    const bool sub = x < 0x1p-126f;
+    const float lx = __builtin_amdgcn_logf(sub ? x * 0x1p32f : x) - (sub ? 32.f : 0.f);
+    const float e = y * lx;
+    const bool tiny = e < -126.f;
+    return __builtin_amdgcn_exp2f(tiny ? e + 32.f : e) * (tiny ? 0x1p-32f : 1.f);
+}
+
 __device__ __forceinline__ Vec3 pow3f(Vec3 a, float p)
 {
-    return vec3(powf(a.x, p), powf(a.y, p), powf(a.z, p));
+    return vec3(pow_pos(a.x, p), pow_pos(a.y, p), pow_pos(a.z, p));
 }
 
 __device__ __forceinline__ Vec3 tonemap_gamma(Vec3 c)
@@ -351,7 +368,7 @@ __device__ __forceinline__ Vec3 bp_term(Vec3 L, Vec3 H, Vec3 N, float vis, Vec3 
                                         bool with_emissive)
 {
     float diff = smax(0.f, dot3(L, N));
-    float spec = powf(smax(0.f, dot3(H, N)), shin);
+    float spec = pow_pos(smax(0.f, dot3(H, N)), shin);
     Vec3 diffuse = scale3(scale3(dl, diff), ao);
     Vec3 specular = scale3(vec3(ks, ks, ks), spec);
     if (shadow_on) {
@@ -386,7 +403,7 @@ __device__ __forceinline__ Vec3 pbr_lo(Vec3 L, Vec3 V, Vec3 H, Vec3 N, float vis
     float g2 = NdotL / (NdotL * (1 - k) + k);
     float G = g1 * g2;
     float omh = smax(1.f - HdotV, 0.f);
-    float p5 = powf(omh, 5.f);
+    float p5 = pow_pos(omh, 5.f);
     Vec3 F = add3(F0, scale3(sub3(vec3(1.f, 1.f, 1.f), F0), p5));
     Vec3 DGF = scale3(F, NDF * G);
     float denom = 4 * NdotV * NdotL + 0.001f;
