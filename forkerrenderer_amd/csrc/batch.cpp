@@ -134,7 +134,7 @@ struct fgl_batch {
     int n_vblocks[2] = {0, 0}, n_fblocks[2] = {0, 0};
     uint32_t n_conv_items[4] = {0, 0, 0, 0};
     Buf depth, vis, gbuf, ao, fb, img, shadow, shadow_depth;
-    Buf screen, rect, counts, offs, scan_tmp, tri_v, big_list, tile_start, tile_end;
+    Buf screen, ndc_z, rect, counts, offs, scan_tmp, tri_v, big_list, tile_start, tile_end;
     Buf keys[2], vals[2], keys_alt[2], vals_alt[2], sort_tmp[2];
     Buf counters;   // [0] big count, [1] queue, [2], [3] overflow flags (shadow, main)
     uint32_t *h_flags = nullptr;
@@ -244,7 +244,8 @@ static int plan(fgl_batch *b)
         grow(b->d_pix_views, b->n * sizeof(BatchPixView)) || grow(b->depth, px * 4) || grow(b->vis, px * 4) ||
         grow(b->gbuf, (size_t)kGbufPlanes * px * 4) || grow(b->ao, px * 4) || grow(b->fb, 3 * px * 4) ||
         grow(b->img, 3 * px) || grow(b->shadow, spx * 4) || grow(b->shadow_depth, spx * 4) ||
-        grow(b->screen, (size_t)mv * sizeof(int4)) || grow(b->rect, (size_t)mt * sizeof(uint2)) ||
+        grow(b->screen, (size_t)mv * sizeof(int4)) || grow(b->ndc_z, (size_t)mv * 4) ||
+        grow(b->rect, (size_t)mt * sizeof(uint2)) ||
         grow(b->counts, (size_t)mt * 4) || grow(b->offs, ((size_t)mt + 1) * 4) ||
         grow(b->scan_tmp, scan_tmp_words((size_t)mt) * 4) || grow(b->tri_v, (size_t)mt * sizeof(int4)) ||
         grow(b->big_list, (size_t)mt * 4) || grow(b->tile_start, (size_t)mtiles * 4) ||
@@ -328,7 +329,7 @@ static int enqueue(fgl_batch *b, bool measure, bool timed)
             CtxTimer t(b->ctx, timed ? (p == kBvShadow ? "batch_shadow_binning" : "batch_main_binning") : nullptr);
             if (nt > 0) {
                 launch_bv_vertex(p, draws, views, (const uint2 *)b->d_vblocks[p].p, b->n_vblocks[p], (int4 *)b->screen.p,
-                                 s);
+                                 (float *)b->ndc_z.p, s);
                 launch_fill_u32(ctr, 0, 1, s);
                 launch_bv_setup(p, draws, views, (const uint2 *)b->d_fblocks[p].p, b->n_fblocks[p],
                                 (const int4 *)b->screen.p,
@@ -390,6 +391,7 @@ static int enqueue(fgl_batch *b, bool measure, bool timed)
             ra.draws = draws;
             ra.views = views;
             ra.screen = (const int4 *)b->screen.p;
+            ra.ndc_z = (const float *)b->ndc_z.p;
             ra.tri_v = (const int4 *)b->tri_v.p;
             ra.vals = svals;
             ra.tile_start = (const uint32_t *)b->tile_start.p;
@@ -559,7 +561,7 @@ void fgl_batch_destroy(fgl_batch *b)
     }
     Buf *bufs[] = {&b->d_views, &b->d_draws[0], &b->d_draws[1], &b->d_items[0], &b->d_items[1], &b->d_blk_view,
                    &b->d_pix_views, &b->depth, &b->vis, &b->gbuf, &b->ao, &b->fb, &b->img, &b->shadow,
-                   &b->shadow_depth, &b->screen, &b->rect, &b->counts, &b->offs, &b->scan_tmp, &b->tri_v,
+                   &b->shadow_depth, &b->screen, &b->ndc_z, &b->rect, &b->counts, &b->offs, &b->scan_tmp, &b->tri_v,
                    &b->big_list, &b->tile_start, &b->tile_end, &b->counters};
     for (Buf *x : bufs) release(*x);
     for (int p = 0; p < 2; ++p) {

@@ -24,15 +24,20 @@ template <int PASS>
 __global__ __launch_bounds__(256) void k_bv_vertex(const DevDraw *__restrict__ draws,
                                                    const DevView *__restrict__ views,
                                                    const uint2 *__restrict__ blocks,
-                                                   int4 *__restrict__ out)
+                                                   int4 *__restrict__ out, float *__restrict__ ndc_z)
 {
     const uint2 bk = blocks[blockIdx.x];
     const DevDraw &dr = draws[bk.x];
     const int vi = (int)bk.y + threadIdx.x;
     if (vi >= dr.n_pos) return;
     const FrameUniforms &u = views[dr.view].u;
-    if (PASS == kBvShadow) out[dr.vert_base + vi] = to_screen(u.shadow_viewport, depth_ndc(dr, vi));
-    else out[dr.vert_base + vi] = to_screen(u.viewport, main_ndc(dr, u, vi));
+    if (PASS == kBvShadow) {
+        const Vec4 n = depth_ndc(dr, vi);
+        out[dr.vert_base + vi] = to_screen(u.shadow_viewport, n);
+        ndc_z[dr.vert_base + vi] = n.z;
+    } else {
+        out[dr.vert_base + vi] = to_screen(u.viewport, main_ndc(dr, u, vi));
+    }
 }
 
 template <int PASS>
@@ -129,6 +134,7 @@ __global__ __launch_bounds__(BLOCK) void k_bv_raster(BvRasterArgs b)
         a.depth = SHADOW ? b.shadow_depth + v.spx_off : b.depth + v.px_off;
         a.vis = b.vis + v.px_off;
         a.shadow_out = b.shadow + v.spx_off;
+        a.ndc_z = b.ndc_z;
         raster_tile<TS, BLOCK, SHADOW>(a, tile, lds);
     }
 }
@@ -190,12 +196,15 @@ __global__ __launch_bounds__(256) void k_bv_resolve(BvPixelArgs b)
 static inline unsigned nblk(size_t n, unsigned b) { return (unsigned)((n + b - 1) / b); }
 
 void launch_bv_vertex(int pass, const DevDraw *draws, const DevView *views, const uint2 *blocks, int n_blocks,
-                      int4 *screen, hipStream_t s)
+                      int4 *screen, float *ndc_z, hipStream_t s)
 {
     if (n_blocks <= 0) return;
     if (pass == kBvShadow)
-        hipLaunchKernelGGL(k_bv_vertex<kBvShadow>, dim3(n_blocks), dim3(256), 0, s, draws, views, blocks, screen);
-    else hipLaunchKernelGGL(k_bv_vertex<kBvMain>, dim3(n_blocks), dim3(256), 0, s, draws, views, blocks, screen);
+        hipLaunchKernelGGL(k_bv_vertex<kBvShadow>, dim3(n_blocks), dim3(256), 0, s, draws, views, blocks, screen,
+                           ndc_z);
+    else
+        hipLaunchKernelGGL(k_bv_vertex<kBvMain>, dim3(n_blocks), dim3(256), 0, s, draws, views, blocks, screen,
+                           ndc_z);
 }
 
 void launch_bv_setup(int pass, const DevDraw *draws, const DevView *views, const uint2 *blocks, int n_blocks,

@@ -51,6 +51,7 @@ struct BvRasterArgs {
     uint32_t *queue;          // next item (reset to 0 before the launch)
     float *depth, *shadow_depth, *shadow;
     int *vis;
+    const float *ndc_z;       // shadow pass: per vertex NDC z
 };
 
 struct BvPixelArgs {
@@ -71,7 +72,7 @@ struct BvPixelArgs {
 
 // blocks: (draw, first vertex / face) of each 256-item block
 void launch_bv_vertex(int pass, const DevDraw *draws, const DevView *views, const uint2 *blocks, int n_blocks,
-                      int4 *screen, hipStream_t s);
+                      int4 *screen, float *ndc_z, hipStream_t s);
 void launch_bv_setup(int pass, const DevDraw *draws, const DevView *views, const uint2 *blocks, int n_blocks,
                      const int4 *screen, int TS, uint2 *rect, uint32_t *count, int4 *tri_v,
                      uint32_t *big_list, uint32_t *big_count, hipStream_t s);

@@ -83,7 +83,7 @@ struct fgl_ctx {
 
     // pass scratch
     DevBuf d_draws, screen, rect, counts, offs, scan_tmp, keys, vals, keys_alt, vals_alt, sort_tmp,
-        tile_start, tile_end, tri_v, big_list, counters;
+        tile_start, tile_end, tri_v, big_list, counters, ndc_z;
     std::vector<DevDraw> last_draws;
     uint32_t *h_total = nullptr;
     // pass counts published by the device into coherent pinned memory
@@ -280,7 +280,7 @@ void fgl_ctx_destroy(fgl_ctx *c)
                       &c->rng, &c->rng_saved, &c->rng_samples, &c->rng_offs, &c->ssao_pow3,
                       &c->screen, &c->rect, &c->counts, &c->offs, &c->scan_tmp, &c->keys,
                       &c->vals, &c->keys_alt, &c->vals_alt, &c->sort_tmp, &c->tile_start,
-                      &c->tile_end, &c->tri_v, &c->big_list, &c->counters};
+                      &c->tile_end, &c->tri_v, &c->big_list, &c->counters, &c->ndc_z};
     for (DevBuf *b : bufs) free_buf(*b);
     for (auto &t : c->timed) { hipEventDestroy(t.a); hipEventDestroy(t.b); }
     if (c->h_total) hipHostFree(c->h_total);
@@ -744,13 +744,15 @@ static int flush_pass(fgl_ctx *c)
         (rc = ensure(c, c->tile_start, (size_t)ntiles * 4)) ||
         (rc = ensure(c, c->tile_end, (size_t)ntiles * 4)) ||
         (rc = ensure(c, c->tri_v, (size_t)nt * sizeof(int4))) ||
-        (rc = ensure(c, c->big_list, (size_t)nt * 4)) || (rc = ensure(c, c->counters, 64)))
+        (rc = ensure(c, c->big_list, (size_t)nt * 4)) || (rc = ensure(c, c->counters, 64)) ||
+        (shadow && (rc = ensure(c, c->ndc_z, (size_t)nv * 4))))
         return rc;
     const DevDraw *ddraws = (const DevDraw *)c->d_draws.p;
     uint32_t *big_count = (uint32_t *)c->counters.p;
     {
         KernelTimer t(c, "vertex");
-        launch_vertex(shadow, ddraws, n_draws, max_pos, u, (int4 *)c->screen.p, big_count, c->stream);
+        launch_vertex(shadow, ddraws, n_draws, max_pos, u, (int4 *)c->screen.p,
+                      shadow ? (float *)c->ndc_z.p : nullptr, big_count, c->stream);
     }
     {
         KernelTimer t(c, "setup");
@@ -805,6 +807,7 @@ static int flush_pass(fgl_ctx *c)
     a.depth = shadow ? (float *)c->shadow_depth.p : (float *)c->depth.p;
     a.vis = (int *)c->vis.p;
     a.shadow_out = (float *)c->shadow.p;
+    a.ndc_z = shadow ? (const float *)c->ndc_z.p : nullptr;
     {
         KernelTimer t(c, shadow ? "raster_shadow" : "raster");
         launch_raster(shadow, TS, a, c->stream);

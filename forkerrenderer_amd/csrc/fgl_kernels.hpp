@@ -26,6 +26,7 @@ struct RasterArgs {
     float *depth;
     int *vis;                  // winning triangle per pixel, -1 if none (this pass)
     float *shadow_out;         // shadow pass: ShadowBuffer plane
+    const float *ndc_z;        // shadow pass: per vertex, z of DepthShader::ProcessVertex's NDC
 };
 
 struct ResolveArgs {
@@ -50,10 +51,10 @@ struct SsaoArgs {
 };
 
 // raster.hip
-// Window records of every (draw, position); zero (may be null): a word set
-// to 0 on the way.
+// Window records of every (draw, position); shadow pass: also the NDC z of
+// each vertex into ndc_z; zero (may be null): a word set to 0 on the way.
 void launch_vertex(bool shadow, const DevDraw *draws, int n_draws, int max_pos,
-                   const FrameUniforms &u, int4 *out, uint32_t *zero, hipStream_t s);
+                   const FrameUniforms &u, int4 *out, float *ndc_z, uint32_t *zero, hipStream_t s);
 void launch_setup(const DevDraw *draws, int n_draws, int max_faces, const int4 *screen, int W,
                   int H, int TS, uint2 *rect, uint32_t *count, int4 *tri_v, uint32_t *big_list,
                   uint32_t *big_count, hipStream_t s);

@@ -23,14 +23,20 @@ namespace fgl {
 // pass's big-triangle counter, which setup counts up next).
 template <bool SHADOW>
 __global__ __launch_bounds__(256) void k_vertex(const DevDraw *__restrict__ draws, FrameUniforms u,
-                                                int4 *__restrict__ out, uint32_t *__restrict__ zero)
+                                                int4 *__restrict__ out, float *__restrict__ ndc_z,
+                                                uint32_t *__restrict__ zero)
 {
     const DevDraw &dr = draws[blockIdx.y];
     const int vi = blockIdx.x * blockDim.x + threadIdx.x;
     if (zero && (vi | blockIdx.y) == 0) *zero = 0u;
     if (vi >= dr.n_pos) return;
-    if (SHADOW) out[dr.vert_base + vi] = to_screen(u.shadow_viewport, depth_ndc(dr, vi));
-    else out[dr.vert_base + vi] = to_screen(u.viewport, main_ndc(dr, u, vi));
+    if (SHADOW) {
+        const Vec4 n = depth_ndc(dr, vi);
+        out[dr.vert_base + vi] = to_screen(u.shadow_viewport, n);
+        ndc_z[dr.vert_base + vi] = n.z;
+    } else {
+        out[dr.vert_base + vi] = to_screen(u.viewport, main_ndc(dr, u, vi));
+    }
 }
 
 __global__ __launch_bounds__(256) void k_setup(const DevDraw *__restrict__ draws,
@@ -223,15 +229,15 @@ __global__ __launch_bounds__(256) void k_forward_resolve(ResolveArgs a, FrameUni
 static inline unsigned blocks_for(size_t n, unsigned b) { return (unsigned)((n + b - 1) / b); }
 
 void launch_vertex(bool shadow, const DevDraw *draws, int n_draws, int max_pos,
-                   const FrameUniforms &u, int4 *out, uint32_t *zero, hipStream_t s)
+                   const FrameUniforms &u, int4 *out, float *ndc_z, uint32_t *zero, hipStream_t s)
 {
     if (n_draws <= 0 || max_pos <= 0) {
         if (zero) launch_fill_u32(zero, 0u, 1, s);
         return;
     }
     dim3 grid(blocks_for(max_pos, 256), n_draws);
-    if (shadow) hipLaunchKernelGGL(k_vertex<true>, grid, dim3(256), 0, s, draws, u, out, zero);
-    else hipLaunchKernelGGL(k_vertex<false>, grid, dim3(256), 0, s, draws, u, out, zero);
+    if (shadow) hipLaunchKernelGGL(k_vertex<true>, grid, dim3(256), 0, s, draws, u, out, ndc_z, zero);
+    else hipLaunchKernelGGL(k_vertex<false>, grid, dim3(256), 0, s, draws, u, out, ndc_z, zero);
 }
 
 void launch_setup(const DevDraw *draws, int n_draws, int max_faces, const int4 *screen, int W,

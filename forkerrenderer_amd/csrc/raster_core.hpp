@@ -356,13 +356,9 @@ __device__ __forceinline__ void raster_tile(const RasterArgs &a, int tile, Raste
             if (!SHADOW) {
                 a.vis[i] = win;
             } else {
-                // DepthShader::ProcessFragment (depthshader.h:30-36)
-                const DevDraw &dr = a.draws[tv.w];
-                const int f = win - dr.tri_base;
-                Vec4 n0 = depth_ndc(dr, dr.vi[3 * f]);
-                Vec4 n1 = depth_ndc(dr, dr.vi[3 * f + 1]);
-                Vec4 n2 = depth_ndc(dr, dr.vi[3 * f + 2]);
-                float z = dot3(vec3(n0.z, n1.z, n2.z), vec3(b0, b1, b2));
+                // DepthShader::ProcessFragment (depthshader.h:30-36), with
+                // the vertices' NDC z as the vertex pass computed them
+                float z = dot3(vec3(a.ndc_z[tv.x], a.ndc_z[tv.y], a.ndc_z[tv.z]), vec3(b0, b1, b2));
                 a.shadow_out[i] = z * 0.5f + 0.5f;
             }
         } else {
@@ -387,10 +383,8 @@ __device__ __forceinline__ Vec3 winner_bary(const int4 *__restrict__ screen,
     return vec3(b0, b1, b2);
 }
 
-// Geometry pass resolve of pixel (x,y): GShader::ProcessFragment + the seven
-// buffer writes (gshader.h:95-202, forkergl.cpp:211-223) for the visible
-// fragment; a pixel with no fragment gets the cleared values when `fresh`.
-// GShader::ProcessFragment for the winner (face f of draw dr, material m,
+// GShader::ProcessFragment + the seven buffer writes (gshader.h:95-202,
+// forkergl.cpp:211-223) for the winner (face f of draw dr, material m,
 // barycentrics b) at pixel i: the 19 G-buffer planes.
 __device__ __forceinline__ void gbuffer_shade(const ResolveArgs &a, const FrameUniforms &u, const DevDraw &dr,
                                               const DevMaterial &m, int f, const Vec3 &b, size_t i, size_t P)
@@ -434,7 +428,8 @@ __device__ __forceinline__ void gbuffer_shade(const ResolveArgs &a, const FrameU
     gb[18 * P + i] = type;
 }
 
-// G-buffer resolve of pixel (x,y).  When every active lane of the wavefront
+// Geometry pass resolve of pixel (x,y); a pixel with no fragment gets the
+// cleared values when `fresh`.  When every active lane of the wavefront
 // resolves a face of the same draw with the same material (the common case:
 // neighbouring pixels show the same mesh), the draw and material records are
 // addressed through readfirstlane, so their fields (matrices, flags, texture
