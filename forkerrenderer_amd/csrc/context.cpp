@@ -334,17 +334,23 @@ int fgl_upload_texture(fgl_ctx *c, int w, int h, int bpp, const uint8_t *bytes, 
         filter < 0 || filter > 1)
         return fail(c, FGL_ERR_INVALID, "bad texture parameters");
     std::vector<uint32_t> packed((size_t)w * h);
+    std::vector<uint8_t> zb((size_t)w * h);
     for (size_t i = 0; i < packed.size(); ++i) {
         const uint8_t *p = bytes + i * bpp;
         uint32_t b = p[0], g = bpp > 1 ? p[1] : 0, r = bpp > 2 ? p[2] : 0;
         packed[i] = r | (g << 8) | (b << 16);
+        zb[i] = (uint8_t)b;
     }
-    void *mem = nullptr;
+    void *mem = nullptr, *zmem = nullptr;
     HIPCHK(c, hipMalloc(&mem, packed.size() * 4));
-    HIPCHK(c, hipMemcpy(mem, packed.data(), packed.size() * 4, hipMemcpyHostToDevice));
     c->texture_mem.push_back(mem);
+    HIPCHK(c, hipMalloc(&zmem, zb.size()));
+    c->texture_mem.push_back(zmem);
+    HIPCHK(c, hipMemcpy(mem, packed.data(), packed.size() * 4, hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(zmem, zb.data(), zb.size(), hipMemcpyHostToDevice));
     DevTexture t;
     t.texels = (const uint32_t *)mem;
+    t.zb = (const uint8_t *)zmem;
     t.w = w; t.h = h; t.wrap = wrap; t.filter = filter;
     c->textures.push_back(t);
     c->textures_dirty = true;

@@ -13,7 +13,8 @@ constexpr int kMaxDisk = 64;
 // TGAImage::Get's byte order (B,G,R; grey sets only b) is resolved at upload
 // (src/tgaimage.h:149-181).
 struct DevTexture {
-    const uint32_t *texels;
+    const uint32_t *texels;   // r | g << 8 | b << 16 per texel
+    const uint8_t *zb;        // the b byte alone (what SampleFloat reads), 1 B per texel
     int w, h, wrap, filter;
 };
 
@@ -116,6 +117,42 @@ __device__ __forceinline__ Vec3 filter_color(const DevTexture &t, Vec2 c)
     return texel_rgb(t, trunc_int(floorf(c.x * w)), trunc_int(floorf(c.y * h)));
 }
 
+// filter_color's third channel alone, from the 1-byte plane: the same
+// texel coordinates and the same per-channel arithmetic (lerp3 is
+// component-wise), a quarter of the bytes.
+__device__ __forceinline__ float texel_z(const DevTexture &t, int x, int y)
+{
+    if (x < 0 || y < 0 || x >= t.w || y >= t.h) return 0.f;
+    return (float)t.zb[(size_t)x + (size_t)y * t.w];
+}
+
+__device__ __forceinline__ float lerp1(float t, float a, float b) { return a * (1 - t) + b * t; }
+
+__device__ __forceinline__ float filter_z(const DevTexture &t, Vec2 c)
+{
+    float w = (float)(t.w - 0.001), h = (float)(t.h - 0.001);
+    if (t.filter == 1) {
+        float px = c.x * w, py = c.y * h;
+        float tlx = floorf(px - 0.5f), tly = floorf(py - 0.5f);
+        float tx = px - (tlx + 0.5f);
+        float ty = py - (tly + 0.5f);
+        int s0x = trunc_int(tlx), s0y = trunc_int(tly);
+        int s1x = trunc_int(tlx + 1.f), s2y = trunc_int(tly + 1.f);
+        int s1y = s0y, s2x = s0x, s3x = s1x, s3y = s2y;
+        if (t.wrap != 0) {
+            s0x = clampi(s0x, 0, t.w - 1); s0y = clampi(s0y, 0, t.h - 1);
+            s1x = clampi(s1x, 0, t.w - 1); s1y = clampi(s1y, 0, t.h - 1);
+            s2x = clampi(s2x, 0, t.w - 1); s2y = clampi(s2y, 0, t.h - 1);
+            s3x = clampi(s3x, 0, t.w - 1); s3y = clampi(s3y, 0, t.h - 1);
+        }
+        float c0 = texel_z(t, s0x, s0y), c1 = texel_z(t, s1x, s1y);
+        float c2 = texel_z(t, s2x, s2y), c3 = texel_z(t, s3x, s3y);
+        float cx1 = lerp1(tx, c0, c1), cx2 = lerp1(tx, c2, c3);
+        return lerp1(ty, cx1, cx2);
+    }
+    return texel_z(t, trunc_int(floorf(c.x * w)), trunc_int(floorf(c.y * h)));
+}
+
 // Texture::Sample / SampleFloat (src/texture.h:41-51)
 __device__ __forceinline__ Vec3 tex_sample(const DevTexture *texs, int id, Vec2 uv)
 {
@@ -125,7 +162,7 @@ __device__ __forceinline__ Vec3 tex_sample(const DevTexture *texs, int id, Vec2 
 __device__ __forceinline__ float tex_samplef(const DevTexture *texs, int id, Vec2 uv)
 {
     const DevTexture &t = texs[id];
-    return filter_color(t, wrap_coord(t, uv)).z / 255.f;
+    return filter_z(t, wrap_coord(t, uv)) / 255.f;
 }
 
 // ---------------- vertex programs ----------------
