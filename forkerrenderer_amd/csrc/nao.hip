@@ -533,19 +533,22 @@ static inline void conv_tiles(int W, int H, int N, int dil, int &tx, int &ty, in
     n = tx * ty * dil * dil * N;
 }
 
-// Persistent grid: two workgroups per compute unit (the LDS tile and the
-// registers of the weights and the prefetch allow two).
-static int g_conv_wgs = 0;
+// Persistent grid: `per_cu` workgroups per compute unit -- three for the
+// 16-channel first layer (52 KB of LDS, 152 registers: three fit), two for
+// the 32-channel layers (220-233 registers allow two); more resident
+// workgroups keep more tile loads in flight.
+static int g_cus = 0;
 
-static inline unsigned conv_workgroups(int n_tiles)
+static inline unsigned conv_workgroups(int n_tiles, int per_cu)
 {
-    if (g_conv_wgs == 0) {
+    if (g_cus == 0) {
         int dev = 0, cu = 0;
         hipGetDevice(&dev);
         if (hipDeviceGetAttribute(&cu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cu < 1) cu = 256;
-        g_conv_wgs = 2 * cu;
+        g_cus = cu;
     }
-    return (unsigned)(n_tiles < g_conv_wgs ? n_tiles : g_conv_wgs);
+    const int wgs = per_cu * g_cus;
+    return (unsigned)(n_tiles < wgs ? n_tiles : wgs);
 }
 
 template <int CIN, int MODE>
@@ -555,7 +558,7 @@ static void launch_conv(hipStream_t s, const uint16_t *in, const uint16_t *Wt, c
     int tx, ty, n;
     conv_tiles(W, H, N, dil, tx, ty, n);
     if (n == 0) return;
-    hipLaunchKernelGGL((k_conv3x3_tiled<CIN, MODE, false>), dim3(conv_workgroups(n)), dim3(256), 0, s, in, Wt, bias,
+    hipLaunchKernelGGL((k_conv3x3_tiled<CIN, MODE, false>), dim3(conv_workgroups(n, CIN == C0 ? 3 : 2)), dim3(256), 0, s, in, Wt, bias,
                        act, out, W, H, dil, n, tx, ty, (const BatchPixView *)nullptr, (const uint2 *)nullptr,
                        HeadArgs{nullptr, nullptr, nullptr}, FeatArgs{});
 }
@@ -567,7 +570,7 @@ static void launch_conv_feat(hipStream_t s, const FeatArgs &fa, const uint16_t *
     int tx, ty, n;
     conv_tiles(W, H, 1, dil, tx, ty, n);
     if (n == 0) return;
-    hipLaunchKernelGGL((k_conv3x3_tiled<C0, 3, false>), dim3(conv_workgroups(n)), dim3(256), 0, s,
+    hipLaunchKernelGGL((k_conv3x3_tiled<C0, 3, false>), dim3(conv_workgroups(n, 3)), dim3(256), 0, s,
                        (const uint16_t *)nullptr, Wt, bias, (const uint16_t *)nullptr, out, W, H, dil, n, tx, ty,
                        (const BatchPixView *)nullptr, (const uint2 *)nullptr, HeadArgs{nullptr, nullptr, nullptr}, fa);
 }
@@ -579,7 +582,7 @@ static void launch_conv_head(hipStream_t s, const uint16_t *in, const uint16_t *
     int tx, ty, n;
     conv_tiles(W, H, N, dil, tx, ty, n);
     if (n == 0) return;
-    hipLaunchKernelGGL((k_conv3x3_tiled<CH, 2, false>), dim3(conv_workgroups(n)), dim3(256), 0, s, in, Wt, bias,
+    hipLaunchKernelGGL((k_conv3x3_tiled<CH, 2, false>), dim3(conv_workgroups(n, 2)), dim3(256), 0, s, in, Wt, bias,
                        (const uint16_t *)nullptr, (uint16_t *)nullptr, W, H, dil, n, tx, ty,
                        (const BatchPixView *)nullptr, (const uint2 *)nullptr, hd, FeatArgs{});
 }
@@ -591,7 +594,7 @@ static void launch_conv_batch(hipStream_t s, const BatchPixView *views, const ui
                               const FeatArgs &fa = FeatArgs{})
 {
     if (n == 0) return;
-    hipLaunchKernelGGL((k_conv3x3_tiled<CIN, MODE, true>), dim3(conv_workgroups(n)), dim3(256), 0, s, in, Wt, bias,
+    hipLaunchKernelGGL((k_conv3x3_tiled<CIN, MODE, true>), dim3(conv_workgroups(n, CIN == C0 ? 3 : 2)), dim3(256), 0, s, in, Wt, bias,
                        (const uint16_t *)nullptr, out, 0, 0, dil, n, 0, 0, views, items, hd, fa);
 }
 
