@@ -95,10 +95,12 @@ __device__ __forceinline__ uint32_t pair_key(uint2 r, uint32_t j, int tiles_x, u
     return tile_base + (ty0 + j / wx) * (uint32_t)tiles_x + tx0 + j % wx;
 }
 
-// Per-triangle record staged in LDS by the rasteriser.
+// Per-triangle record of the rasteriser (registers; staged in LDS as TriSoA).
 struct TriL {
-    int X0, Y0, X1, Y1, X2, Y2;
-    int s0x, s0y, s1x, s1y;  // B-A, C-A (x then y); valid when exact
+    int X0, Y0;
+    // exact: B-A, C-A as s0x = Bx-Ax, s0y = Cx-Ax, s1x = By-Ay, s1y = Cy-Ay;
+    // otherwise the other two vertices X1, Y1, X2, Y2
+    int q0, q1, q2, q3;
     float z0, z1, z2;
     int bb0, bb1;            // minx | miny << 16, maxx | maxy << 16
     int cz;                  // exact cross product z
@@ -109,7 +111,7 @@ struct TriL {
 
 __device__ __forceinline__ void stage_tri(TriL &t, int g, int4 a, int4 b, int4 c, int W, int H)
 {
-    t.X0 = a.x; t.Y0 = a.y; t.X1 = b.x; t.Y1 = b.y; t.X2 = c.x; t.Y2 = c.y;
+    t.X0 = a.x; t.Y0 = a.y;
     t.z0 = __int_as_float(a.z); t.z1 = __int_as_float(b.z); t.z2 = __int_as_float(c.z);
     int minx, miny, maxx, maxy;
     clamped_bbox(a, b, c, W, H, minx, miny, maxx, maxy);
@@ -118,11 +120,12 @@ __device__ __forceinline__ void stage_tri(TriL &t, int g, int4 a, int4 b, int4 c
     t.exact = a.w && b.w && c.w && W <= 8192 && H <= 8192;
     t.g = g;
     if (t.exact) {
-        t.s0x = b.x - a.x; t.s0y = c.x - a.x; t.s1x = b.y - a.y; t.s1y = c.y - a.y;
-        t.cz = t.s0x * t.s1y - t.s0y * t.s1x;
+        t.q0 = b.x - a.x; t.q1 = c.x - a.x; t.q2 = b.y - a.y; t.q3 = c.y - a.y;
+        t.cz = t.q0 * t.q3 - t.q1 * t.q2;
         t.inv = 1.0 / (double)t.cz;
     } else {
-        t.s0x = t.s0y = t.s1x = t.s1y = t.cz = 0;
+        t.q0 = b.x; t.q1 = b.y; t.q2 = c.x; t.q3 = c.y;
+        t.cz = 0;
         t.inv = 0.0;
     }
 }
@@ -133,8 +136,8 @@ __device__ __forceinline__ bool cover(const TriL &t, int px, int py, float &b0, 
 {
     if (t.exact) {
         int s0z = t.X0 - px, s1z = t.Y0 - py;
-        int cx = t.s0y * s1z - s0z * t.s1y;
-        int cy = s0z * t.s1x - t.s0x * s1z;
+        int cx = t.q1 * s1z - s0z * t.q3;    // s0y * s1z - s0z * s1y
+        int cy = s0z * t.q2 - t.q0 * s1z;    // s0z * s1x - s0x * s1z
         // exact sign pre-test: weights cx/cz, cy/cz and (cz-cx-cy)/cz
         if (t.cz > 0) {
             if (cx < 0 || cy < 0 || cx + cy > t.cz) return false;
@@ -147,7 +150,7 @@ __device__ __forceinline__ bool cover(const TriL &t, int px, int py, float &b0, 
         b0 = r0; b1 = r1; b2 = r2;
         return true;
     }
-    return barycentric_ref(t.X0, t.Y0, t.X1, t.Y1, t.X2, t.Y2, px, py, &b0, &b1, &b2);
+    return barycentric_ref(t.X0, t.Y0, t.q0, t.q1, t.q2, t.q3, px, py, &b0, &b1, &b2);
 }
 
 // XCD-aware tile order: workgroups are dealt round-robin to the 8 XCDs, so
@@ -197,9 +200,36 @@ __device__ __forceinline__ float depth_from_order(uint32_t k)
     return __uint_as_float((k & 0x80000000u) ? (k & 0x7fffffffu) : ~k);
 }
 
+// The staged triangles of a chunk, one array per field: lanes reading
+// consecutive triangles hit consecutive banks (an array of 64- or 80-byte
+// records puts 8 to 32 lanes on one bank).
+template <int BLOCK>
+struct TriSoA {
+    int X0[BLOCK], Y0[BLOCK], q0[BLOCK], q1[BLOCK], q2[BLOCK], q3[BLOCK];
+    float z0[BLOCK], z1[BLOCK], z2[BLOCK];
+    int bb0[BLOCK], bb1[BLOCK], cz[BLOCK], exact[BLOCK], g[BLOCK];
+    double inv[BLOCK];
+    __device__ __forceinline__ void put(int i, const TriL &t)
+    {
+        X0[i] = t.X0; Y0[i] = t.Y0; q0[i] = t.q0; q1[i] = t.q1; q2[i] = t.q2; q3[i] = t.q3;
+        z0[i] = t.z0; z1[i] = t.z1; z2[i] = t.z2;
+        bb0[i] = t.bb0; bb1[i] = t.bb1; cz[i] = t.cz; exact[i] = t.exact; g[i] = t.g;
+        inv[i] = t.inv;
+    }
+    __device__ __forceinline__ TriL get(int i) const
+    {
+        TriL t;
+        t.X0 = X0[i]; t.Y0 = Y0[i]; t.q0 = q0[i]; t.q1 = q1[i]; t.q2 = q2[i]; t.q3 = q3[i];
+        t.z0 = z0[i]; t.z1 = z1[i]; t.z2 = z2[i];
+        t.bb0 = bb0[i]; t.bb1 = bb1[i]; t.cz = cz[i]; t.exact = exact[i]; t.g = g[i];
+        t.inv = inv[i];
+        return t;
+    }
+};
+
 template <int TS, int BLOCK>
 struct RasterLds {
-    TriL tri[BLOCK];
+    TriSoA<BLOCK> tri;
     unsigned long long key[TS * TS];
     int nan_g[TS * TS];       // highest id of a NaN fragment, -1 if none
     uint8_t nan_init[TS * TS];
@@ -252,12 +282,14 @@ __device__ __forceinline__ void raster_walk(const RasterArgs &a, int tile, Raste
         if (tid < n) {
             const int g = (int)a.vals[base + tid];
             const int4 tv = a.tri_v[g];
-            stage_tri(L.tri[tid], g, a.screen[tv.x], a.screen[tv.y], a.screen[tv.z], a.W, a.H);
+            TriL t;
+            stage_tri(t, g, a.screen[tv.x], a.screen[tv.y], a.screen[tv.z], a.W, a.H);
+            L.tri.put(tid, t);
         }
         __syncthreads();
         // small triangles: one per lane
         if (tid < n) {
-            const TriL &t = L.tri[tid];
+            const TriL t = L.tri.get(tid);
             const int bx0 = imax(t.bb0 & 0xffff, x0), bx1 = imin(t.bb1 & 0xffff, x0 + TS - 1);
             const int by0 = imax(t.bb0 >> 16, y0), by1 = imin(t.bb1 >> 16, y0 + TS - 1);
             const int area = (bx1 >= bx0 && by1 >= by0) ? (bx1 - bx0 + 1) * (by1 - by0 + 1) : 0;
@@ -276,9 +308,9 @@ __device__ __forceinline__ void raster_walk(const RasterArgs &a, int tile, Raste
             const int si = jj * 64 + lane;
             bool hit = false;
             if (si < n) {
-                const TriL &t = L.tri[si];
-                const int minx = t.bb0 & 0xffff, miny = t.bb0 >> 16;
-                const int maxx = t.bb1 & 0xffff, maxy = t.bb1 >> 16;
+                const int bb0 = L.tri.bb0[si], bb1 = L.tri.bb1[si];
+                const int minx = bb0 & 0xffff, miny = bb0 >> 16;
+                const int maxx = bb1 & 0xffff, maxy = bb1 >> 16;
                 const int bx0 = imax(minx, x0), bx1 = imin(maxx, x0 + TS - 1);
                 const int by0 = imax(miny, y0), by1 = imin(maxy, y0 + TS - 1);
                 const int area = (bx1 >= bx0 && by1 >= by0) ? (bx1 - bx0 + 1) * (by1 - by0 + 1) : 0;
@@ -292,7 +324,7 @@ __device__ __forceinline__ void raster_walk(const RasterArgs &a, int tile, Raste
             while (m) {
                 const int i = jj * 64 + __builtin_ctzll(m);
                 m &= m - 1;
-                const TriL &t = L.tri[i];
+                const TriL t = L.tri.get(i);
                 const int minx = t.bb0 & 0xffff, miny = t.bb0 >> 16;
                 const int maxx = t.bb1 & 0xffff, maxy = t.bb1 >> 16;
 #pragma unroll
