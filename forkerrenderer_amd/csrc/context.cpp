@@ -593,7 +593,8 @@ static void build_uniforms(fgl_ctx *c, FrameUniforms &u)
     u.shadow.pcss_filter = s.pcss_filter;
     std::memcpy(u.shadow.blocker_disk, s.blocker_disk, sizeof(u.shadow.blocker_disk));
     std::memcpy(u.shadow.pcf_disk, s.pcf_disk, sizeof(u.shadow.pcf_disk));
-    for (int i = 0; i < 2 * kMaxDisk; ++i) u.shadow.blocker_off[i] = (float)((double)s.blocker_disk[i] * s.blocker_filter);
+    for (int i = 0; i < 2 * kMaxDisk; ++i)
+        u.shadow.blocker_off[i] = (float)((double)s.blocker_disk[i] * s.blocker_filter);
 }
 
 static int upload_tables(fgl_ctx *c)
