@@ -68,6 +68,15 @@ def build(verbose=False):
         print(r.stdout + r.stderr, file=sys.stderr)
         raise RuntimeError("CLI link failed")
     os.replace(exe + ".tmp", exe)
+    # the frame driver written against the C++ facade (include/forkergl_amd.hpp)
+    exe = os.path.join(LIBDIR, "facade_render")
+    cmd = ["g++", "-O2", "-std=c++17", "-I", os.path.join(ROOT, "include"), os.path.join(CSRC, "facade_render.cpp"),
+           "-L", LIBDIR, "-lforkergl", "-Wl,-rpath,$ORIGIN", "-o", exe + ".tmp"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        print(r.stdout + r.stderr, file=sys.stderr)
+        raise RuntimeError("facade program link failed")
+    os.replace(exe + ".tmp", exe)
     return LIB
 
 

@@ -17,6 +17,7 @@
 #pragma once
 
 #include <array>
+#include <cstdint>
 #include <cstring>
 #include <stdexcept>
 #include <string>
@@ -86,6 +87,11 @@ struct ForkerGL {
         Viewport() = m;
     }
     static Mat4 GetViewportMatrix() { return Viewport(); }
+    // Extension (the `shadowmap S` scene key): a shadow map of its own size.
+    static void SetShadowViewportMatrix(int x, int y, int w, int h)
+    {
+        check(Context(), fgl_set_viewport(Context(), 1, x, y, w, h), "SetShadowViewportMatrix");
+    }
     static void SetViewProjectionMatrix(const Mat4 &m) { ViewProjection() = m; }
     static Mat4 GetViewProjectionMatrix() { return ViewProjection(); }
     static void SetLightSpaceMatrix(const Mat4 &m)
@@ -118,7 +124,33 @@ struct ForkerGL {
     static Buffer FrameBuffer() { return Read(FGL_TARGET_FRAME, 3); }
     static Buffer DepthBuffer() { return Read(FGL_TARGET_DEPTH, 1); }
     static Buffer AmbientOcclusionGBuffer() { return Read(FGL_TARGET_AO, 1); }
+    // The seven G-buffers as 19 planes (normal, world position, light-space
+    // NDC, albedo, emissive, param: 3 each; shading type: 1).
+    static Buffer GeometryBuffers() { return Read(FGL_TARGET_GBUFFER, 19); }
+    // AntiAliasedImage (render.cpp:291-343): 8-bit B,G,R after DoSSAA.
+    static std::vector<uint8_t> AntiAliasedImage()
+    {
+        size_t n = 0;
+        check(Context(), fgl_target_bytes(Context(), FGL_TARGET_IMAGE, &n), "target");
+        std::vector<uint8_t> img(n);
+        check(Context(), fgl_read_target(Context(), FGL_TARGET_IMAGE, img.data(), n), "read");
+        return img;
+    }
 };
+
+// Shadow::SetShadowStatus and the shadow.h filter parameters
+// (src/shaders/shadow.cpp:11-21, shadow.h:12-23).
+struct Shadow {
+    static void SetShadowStatus(bool on, const fgl_shadow_cfg &cfg)
+    {
+        check(ForkerGL::Context(), fgl_set_shadow(ForkerGL::Context(), on ? 1 : 0, &cfg), "SetShadowStatus");
+    }
+};
+
+// Render::DoSSAO + the AO blur (src/render.cpp:214-289) and Render::DoSSAA
+// (src/render.cpp:291-343): the frame driver's two screen-space steps.
+inline void DoSSAO(const fgl_ssao_cfg &cfg) { check(ForkerGL::Context(), fgl_ssao(ForkerGL::Context(), &cfg), "DoSSAO"); }
+inline void DoSSAA(int k) { check(ForkerGL::Context(), fgl_resolve(ForkerGL::Context(), k), "DoSSAA"); }
 
 // Shader uniform blocks (src/shaders/*.h).  The fragment programs run on the
 // GPU; the struct fields keep the reference's names.
