@@ -774,6 +774,8 @@ static int flush_pass(fgl_ctx *c)
     }
     uint32_t npairs = 0, n_big = 0;
     if ((rc = read_counts(c, (const uint32_t *)c->offs.p + nt, big_count, npairs, n_big))) return rc;
+    if (npairs == 0xffffffffu)
+        return fail(c, FGL_ERR_CAPACITY, "the pass has 2^32 - 1 or more (tile, triangle) pairs; use a larger tile");
     if ((rc = ensure(c, c->keys, (size_t)npairs * 4 + 4)) || (rc = ensure(c, c->vals, (size_t)npairs * 4 + 4)) ||
         (rc = ensure(c, c->keys_alt, (size_t)npairs * 4 + 4)) ||
         (rc = ensure(c, c->vals_alt, (size_t)npairs * 4 + 4)) ||

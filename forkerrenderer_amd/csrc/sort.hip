@@ -65,25 +65,37 @@ __device__ __forceinline__ void scan_load8(const uint32_t *in, size_t n, size_t 
 __global__ __launch_bounds__(kScanBlock) void k_scan_reduce(const uint32_t *__restrict__ in,
                                                             size_t n, uint32_t *__restrict__ part)
 {
-    __shared__ uint32_t lds4[4];
     size_t base = (size_t)blockIdx.x * kScanTile + (size_t)threadIdx.x * kScanItems;
     uint32_t v[kScanItems];
     scan_load8(in, n, base, ((uintptr_t)in & 15u) == 0, v);
-    uint32_t s = 0;
+    // the block's sum in 64 bits; a sum that does not fit is stored as
+    // 0xffffffff, which makes the total saturate in k_scan_partials
+    unsigned long long s = 0;
 #pragma unroll
     for (int i = 0; i < kScanItems; ++i) s += v[i];
-    uint32_t total;
-    block_exclusive_scan(s, lds4, total);
-    if (threadIdx.x == 0) part[blockIdx.x] = total;
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
+    __shared__ unsigned long long lds8[4];
+    if ((threadIdx.x & 63) == 0) lds8[threadIdx.x >> 6] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const unsigned long long t = lds8[0] + lds8[1] + lds8[2] + lds8[3];
+        part[blockIdx.x] = t >= 0xffffffffull ? 0xffffffffu : (uint32_t)t;
+    }
 }
 
 // Single workgroup: exclusive scan of the block partials, total to out[n].
+// A total of 2^32 - 1 or more is written as 0xffffffff (the offsets have
+// wrapped; callers treat it as an overflow).
 __global__ __launch_bounds__(kScanBlock) void k_scan_partials(uint32_t *__restrict__ part,
                                                               int nparts,
                                                               uint32_t *__restrict__ out_total)
 {
     __shared__ uint32_t lds4[4];
+    __shared__ unsigned long long lds8[4];
     uint32_t carry = 0;
+    unsigned long long carry64 = 0;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     for (int base = 0; base < nparts; base += kScanBlock) {
         int i = base + threadIdx.x;
         uint32_t v = i < nparts ? part[i] : 0u;
@@ -91,8 +103,16 @@ __global__ __launch_bounds__(kScanBlock) void k_scan_partials(uint32_t *__restri
         uint32_t ex = block_exclusive_scan(v, lds4, total);
         if (i < nparts) part[i] = carry + ex;
         carry += total;
+        // the chunk's sum again in 64 bits (the 32-bit one may wrap)
+        unsigned long long w = v;
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) w += __shfl_xor(w, o, 64);
+        if (lane == 0) lds8[wave] = w;
+        __syncthreads();
+        carry64 += lds8[0] + lds8[1] + lds8[2] + lds8[3];
+        __syncthreads();
     }
-    if (threadIdx.x == 0) *out_total = carry;
+    if (threadIdx.x == 0) *out_total = carry64 >= 0xffffffffull ? 0xffffffffu : carry;
 }
 
 __global__ __launch_bounds__(kScanBlock) void k_scan_down(const uint32_t *in, uint32_t *out,

@@ -173,3 +173,44 @@ def test_nan_fragments_follow_the_submission_order_fold(ctx):
         ctx.end_pass()
         _nan_aware_equal(f"shadow depth, tile {tile}", ctx.read(F.TARGET_SHADOW_DEPTH), want_d)
         _nan_aware_equal(f"shadow map, tile {tile}", ctx.read(F.TARGET_SHADOW), want_s)
+
+
+def test_pair_count_overflow_is_an_error(ctx):
+    """140,000 triangles that each cover a 3840 x 2160 target give 1.8e10
+    (tile, triangle) pairs at tile 8, more than the 32-bit pair offsets hold:
+    the pass stops with FGL_ERR_CAPACITY before any pair is written, and the
+    context renders normally afterwards."""
+    n = 140_000
+    tri = np.array([[-3, -3, 0.5], [3, -3, 0.5], [0, 3, 0.5]], np.float32)
+    pos = np.tile(tri, (n, 1))
+    faces = np.arange(3 * n, dtype=np.int32).reshape(n, 3)
+    m = F.Material()
+    for f in ("map_kd", "map_ks", "map_ke", "map_bump", "map_base", "map_rough", "map_metal", "map_ao", "map_pbr_ke"):
+        setattr(m, f, -1)
+    mat = ctx.upload_material(m)
+    md = {"pos": pos, "uv": np.zeros((3 * n, 2), np.float32), "nrm": np.tile([[0, 0, 1]], (3 * n, 1)).astype(np.float32),
+          "tan": None, "vi": faces, "ti": faces, "ni": faces, "face_material": np.full(n, mat, np.int32),
+          "support_pbr": False}
+    mid = ctx.upload_model(md)
+    eye = np.eye(4, dtype=np.float32)
+    W, H = 3840, 2160
+    ctx.set_viewport(0, 0, W, H, shadow=True)
+    ctx.set_matrix(F.MAT_LIGHT_SPACE, eye)
+    ctx.set_pass_tile_size(F.PASS_SHADOW, 8)
+    ctx.init_shadow_buffer(W, H)
+    ctx.begin_pass(F.PASS_SHADOW)
+    ctx.draw_model(mid, eye)
+    print("pair overflow pass", flush=True)
+    with pytest.raises(F.FglError, match="2\\^32"):
+        ctx.end_pass()
+    ctx.set_pass_tile_size(F.PASS_SHADOW, 16)
+    # a pass of one of those triangles works and covers every texel
+    md1 = dict(md, pos=tri, uv=np.zeros((3, 2), np.float32), nrm=np.tile([[0, 0, 1]], (3, 1)).astype(np.float32),
+               vi=faces[:1], ti=faces[:1], ni=faces[:1], face_material=np.full(1, mat, np.int32))
+    one = ctx.upload_model(md1)
+    ctx.init_shadow_buffer(W, H)
+    ctx.begin_pass(F.PASS_SHADOW)
+    ctx.draw_model(one, eye)
+    ctx.end_pass()
+    d = ctx.read(F.TARGET_SHADOW_DEPTH)
+    assert np.all(np.abs(d - np.float32(0.75)) < 1e-6), "every texel covered at window z 0.75"
