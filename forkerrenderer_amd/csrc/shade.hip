@@ -166,6 +166,52 @@ __global__ __launch_bounds__(256) void k_pcss_offsets(const uint32_t *__restrict
     if (i <= P) offs[i] = base[i] + n_pcf * fpos[upos[i]];
 }
 
+// Kernel SSAO over 64 x 16-pixel tiles (1024 threads, a wavefront per row):
+// the workgroup first copies the depth plane around its tile, 32 pixels
+// each side, into LDS (coalesced 16-byte loads), and taps that land there
+// read LDS; the others read the plane.  Each CU's texture units, which every
+// gathered lane goes through, are what bounds the per-tap depth reads.
+constexpr int kSsaoTW = 64, kSsaoTH = 16, kSsaoHalo = 32;
+constexpr int kSsaoLW = kSsaoTW + 2 * kSsaoHalo, kSsaoLH = kSsaoTH + 2 * kSsaoHalo;
+
+__global__ __launch_bounds__(1024) void k_ssao_tiled(FrameUniforms u, SsaoArgs sa,
+                                                     const float *__restrict__ depth,
+                                                     const float *__restrict__ g,
+                                                     float *__restrict__ ao)
+{
+    __shared__ float tile[kSsaoLH * kSsaoLW];
+    const int x0 = blockIdx.x * kSsaoTW, y0 = blockIdx.y * kSsaoTH;
+    const int lx0 = x0 - kSsaoHalo, ly0 = y0 - kSsaoHalo;
+    const bool vec = (u.W & 3) == 0;   // rows start 16-byte aligned
+    for (int k = threadIdx.x; k < kSsaoLH * kSsaoLW / 4; k += 1024) {
+        const int row = k / (kSsaoLW / 4), c4 = k - row * (kSsaoLW / 4);
+        const int gy = ly0 + row, gx = lx0 + 4 * c4;
+        float v[4] = {0.f, 0.f, 0.f, 0.f};   // outside the plane: never read
+        if (gy >= 0 && gy < u.H) {
+            const float *src = depth + (size_t)gy * u.W;
+            if (vec && gx >= 0 && gx + 3 < u.W) {
+                const float4 q = *(const float4 *)(src + gx);
+                v[0] = q.x; v[1] = q.y; v[2] = q.z; v[3] = q.w;
+            } else {
+#pragma unroll
+                for (int e = 0; e < 4; ++e)
+                    if (gx + e >= 0 && gx + e < u.W) v[e] = src[gx + e];
+            }
+        }
+#pragma unroll
+        for (int e = 0; e < 4; ++e) tile[row * kSsaoLW + 4 * c4 + e] = v[e];
+    }
+    __syncthreads();
+    const int x = x0 + (threadIdx.x & 63), y = y0 + (threadIdx.x >> 6);
+    if (x >= u.W || y >= u.H) return;
+    const uint32_t W = (uint32_t)u.W;
+    ssao_px_with(u, sa, depth, g, ao, x, y, [&](int sx, int sy) {
+        const uint32_t tx = (uint32_t)(sx - lx0), ty = (uint32_t)(sy - ly0);
+        if (tx < (uint32_t)kSsaoLW && ty < (uint32_t)kSsaoLH) return tile[ty * kSsaoLW + tx];
+        return depth[(uint32_t)sx + (uint32_t)sy * W];
+    });
+}
+
 __global__ __launch_bounds__(256) void k_ssao(FrameUniforms u, SsaoArgs sa,
                                               const float *__restrict__ depth,
                                               const float *__restrict__ g,
@@ -345,7 +391,8 @@ void launch_ssao(const FrameUniforms &u, const SsaoArgs &sa, const float *depth,
                  const float *gbuf, float *ao, hipStream_t s)
 {
     if (u.W <= 0 || u.H <= 0) return;
-    hipLaunchKernelGGL(k_ssao, grid2(u.W, u.H), dim3(256), 0, s, u, sa, depth, gbuf, ao);
+    hipLaunchKernelGGL(k_ssao_tiled, dim3((u.W + kSsaoTW - 1) / kSsaoTW, (u.H + kSsaoTH - 1) / kSsaoTH), dim3(1024),
+                       0, s, u, sa, depth, gbuf, ao);
 }
 
 void launch_blur_recursive(float *ao, int W, int H, hipStream_t s)

@@ -42,10 +42,13 @@ __device__ __forceinline__ void lighting_px(const FrameUniforms &u, const float 
 // VP * (dir, 0)), evaluated with fmaf in a fixed order; window coordinates =
 // viewport diagonal * (clip / w) + viewport offset; depth gather, bias and
 // range check; off-screen taps add no occlusion; ao = (1 - count / n)^3.
-__device__ __forceinline__ void ssao_px(const FrameUniforms &u, const SsaoArgs &sa,
-                                        const float *__restrict__ depth,
-                                        const float *__restrict__ g, float *__restrict__ ao,
-                                        int x, int y)
+// `tap(sx, sy)` reads the depth at an on-screen tap (from the depth plane,
+// or from a workgroup's copy of the part around it: the same floats).
+template <class Tap>
+__device__ __forceinline__ void ssao_px_with(const FrameUniforms &u, const SsaoArgs &sa,
+                                             const float *__restrict__ depth,
+                                             const float *__restrict__ g, float *__restrict__ ao,
+                                             int x, int y, const Tap &tap)
 {
     const size_t P = (size_t)u.W * u.H;
     const size_t i = (size_t)x + (size_t)y * u.W;
@@ -85,12 +88,21 @@ __device__ __forceinline__ void ssao_px(const FrameUniforms &u, const SsaoArgs &
         const float syf = fmaf(vp[5], cs[1] * inv, vp[7]);
         if (!(sxf > -1.f && sxf < Wf && syf > -1.f && syf < Hf)) continue;
         const float sz = fmaf(vp[10], cs[2] * inv, vp[11]);
-        const uint32_t t = (uint32_t)(int)sxf + (uint32_t)(int)syf * (uint32_t)u.W;
-        const float cached = depth[t];
+        const float cached = tap((int)sxf, (int)syf);
         if (sz >= cached + sa.bias && fabsf(fd - cached) < sa.range) ++count;
     }
     const float a = 1.f - (float)count * (1.f / (float)sa.n_kernel);
     ao[i] = (a * a) * a;
+}
+
+__device__ __forceinline__ void ssao_px(const FrameUniforms &u, const SsaoArgs &sa,
+                                        const float *__restrict__ depth,
+                                        const float *__restrict__ g, float *__restrict__ ao,
+                                        int x, int y)
+{
+    const uint32_t W = (uint32_t)u.W;
+    ssao_px_with(u, sa, depth, g, ao, x, y,
+                 [&](int sx, int sy) { return depth[(uint32_t)sx + (uint32_t)sy * W]; });
 }
 
 // Render::DoSSAO (src/render.cpp:214-289) itself at pixel (x,y), with the
