@@ -212,17 +212,6 @@ __global__ __launch_bounds__(1024) void k_ssao_tiled(FrameUniforms u, SsaoArgs s
     });
 }
 
-__global__ __launch_bounds__(256) void k_ssao(FrameUniforms u, SsaoArgs sa,
-                                              const float *__restrict__ depth,
-                                              const float *__restrict__ g,
-                                              float *__restrict__ ao)
-{
-    const int x = blockIdx.x * 64 + (threadIdx.x & 63);
-    const int y = blockIdx.y * 4 + (threadIdx.x >> 6);
-    if (x >= u.W || y >= u.H) return;
-    ssao_px(u, sa, depth, g, ao, x, y);
-}
-
 // Buffer1f::TwoPassGaussianBlurDenoised (src/buffer.cpp:59-98) is in place and
 // order dependent: the taps left of (or below) a pixel read values already
 // blurred.  One thread owns a whole row (then a whole column) and walks it in

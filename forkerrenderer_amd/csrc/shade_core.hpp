@@ -95,16 +95,6 @@ __device__ __forceinline__ void ssao_px_with(const FrameUniforms &u, const SsaoA
     ao[i] = (a * a) * a;
 }
 
-__device__ __forceinline__ void ssao_px(const FrameUniforms &u, const SsaoArgs &sa,
-                                        const float *__restrict__ depth,
-                                        const float *__restrict__ g, float *__restrict__ ao,
-                                        int x, int y)
-{
-    const uint32_t W = (uint32_t)u.W;
-    ssao_px_with(u, sa, depth, g, ao, x, y,
-                 [&](int sx, int sy) { return depth[(uint32_t)sx + (uint32_t)sy * W]; });
-}
-
 // Render::DoSSAO (src/render.cpp:214-289) itself at pixel (x,y), with the
 // pixel's 32 hemisphere samples taken from the random stream (samples: 32
 // unit-sphere vectors per pixel, pixels in row order).  Each sample is
