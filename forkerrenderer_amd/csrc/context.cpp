@@ -947,6 +947,11 @@ int fgl_ssao(fgl_ctx *c, const fgl_ssao_cfg *cfg)
     sa.radius = cfg->radius;
     sa.bias = cfg->bias;
     sa.range = cfg->range;
+    sa.kmax = 0.f;
+    for (int k = 0; k < 3 * sa.n_kernel; ++k) {
+        const float a = std::fabs(sa.kernel[k]);
+        if (!(a <= sa.kmax)) sa.kmax = a;   // a NaN component makes kmax NaN
+    }
     if (cfg->stochastic) {
         // 32 hemisphere samples per pixel, pixels in row order, from the stream
         const size_t n = (size_t)c->W * c->H * 32;

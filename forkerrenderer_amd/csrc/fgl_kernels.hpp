@@ -48,6 +48,7 @@ struct SsaoArgs {
     float kernel[3 * 64];
     float noise[3 * 16];
     float radius, bias, range;
+    float kmax;   // largest |component| over the n_kernel kernel vectors
 };
 
 // raster.hip

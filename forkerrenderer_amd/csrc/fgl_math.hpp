@@ -50,6 +50,18 @@ FGL_HD float recip(float s)
     return 1.f / s;
 }
 
+// recip(s) for a caller that has shown 2^-125 <= |s| <= 2^125: on the
+// device the reciprocal step alone, without the range test.
+FGL_HD float recip_in_range(float s)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+    const float y = __builtin_amdgcn_rcpf(s);
+    return fmaf(fmaf(-s, y, 1.f), y, y);
+#else
+    return 1.f / s;
+#endif
+}
+
 // float -> int as x86 cvttss2si: truncation; NaN or outside [-2^31, 2^31)
 // gives INT_MIN.  gfx950's v_cvt_i32_f32 saturates instead, so range-check.
 FGL_HD int trunc_int(float v)

@@ -169,8 +169,8 @@ __global__ __launch_bounds__(256) void k_pcss_offsets(const uint32_t *__restrict
 // Kernel SSAO over 64 x 16-pixel tiles (1024 threads, a wavefront per row):
 // the workgroup first copies the depth plane around its tile, 32 pixels
 // each side, into LDS (coalesced 16-byte loads), and taps that land there
-// read LDS; the others read the plane.  Each CU's texture units, which every
-// gathered lane goes through, are what bounds the per-tap depth reads.
+// read LDS; the others read the plane.  About 46 vector ALU operations per
+// tap (projection, reciprocal, bounds, compare) bound the kernel.
 constexpr int kSsaoTW = 64, kSsaoTH = 16, kSsaoHalo = 32;
 constexpr int kSsaoLW = kSsaoTW + 2 * kSsaoHalo, kSsaoLH = kSsaoTH + 2 * kSsaoHalo;
 
@@ -205,10 +205,22 @@ __global__ __launch_bounds__(1024) void k_ssao_tiled(FrameUniforms u, SsaoArgs s
     const int x = x0 + (threadIdx.x & 63), y = y0 + (threadIdx.x >> 6);
     if (x >= u.W || y >= u.H) return;
     const uint32_t W = (uint32_t)u.W;
+    const int lbase = ly0 * kSsaoLW + lx0;
+    // The plane is read through a global-address-space pointer: with two
+    // generic pointers the compiler merges both reads into one flat load,
+    // which sends the LDS taps down the vector-memory path as well.
+    const __attribute__((address_space(1))) float *gdepth =
+        (const __attribute__((address_space(1))) float *)depth;
     ssao_px_with(u, sa, depth, g, ao, x, y, [&](int sx, int sy) {
         const uint32_t tx = (uint32_t)(sx - lx0), ty = (uint32_t)(sy - ly0);
-        if (tx < (uint32_t)kSsaoLW && ty < (uint32_t)kSsaoLH) return tile[ty * kSsaoLW + tx];
-        return depth[(uint32_t)sx + (uint32_t)sy * W];
+        const bool near = tx < (uint32_t)kSsaoLW && ty < (uint32_t)kSsaoLH;
+        // Every lane reads LDS (a far lane at the tile's first word) and far
+        // lanes also read the plane, into another register: the LDS read
+        // does not wait for the plane's.
+        const float dl = tile[near ? sy * kSsaoLW + sx - lbase : 0];
+        float dg = 0.f;
+        if (!near) dg = gdepth[(uint32_t)sx + (uint32_t)sy * W];
+        return near ? dl : dg;
     });
 }
 

@@ -3,6 +3,8 @@
 // kernels (shade.hip) and the batched multi-view kernels (batch.hip).
 #pragma once
 
+#include <type_traits>
+
 #include "fgl_kernels.hpp"
 
 namespace fgl {
@@ -77,20 +79,31 @@ __device__ __forceinline__ void ssao_px_with(const FrameUniforms &u, const SsaoA
     // the float bounds is that test with fewer operations; W and H are
     // exact in float.
     const float Wf = (float)u.W, Hf = (float)u.H;
-    int count = 0;
-    for (int s = 0; s < sa.n_kernel; ++s) {
-        const float kx = sa.kernel[3 * s], ky = sa.kernel[3 * s + 1], kz = sa.kernel[3 * s + 2];
-        float cs[4];
+    // Every tap's w = cc[3] + radius * (kx cT[3] + ky cB[3] + kz cN[3]), so
+    // |w - cc[3]| <= |radius| kmax (|cT[3]| + |cB[3]| + |cN[3]|) up to a few
+    // roundings, which the 1.001 factor covers.  When that interval lies in
+    // [2^-124, 2^124] for every pixel of the wavefront, recip's range test
+    // would pass for every tap and is left out (NaN or inf fails the test).
+    const float dw = fabsf(radius) * sa.kmax * (fabsf(cT[3]) + fabsf(cB[3]) + fabsf(cN[3])) * 1.001f;
+    const bool w_safe = cc[3] - dw >= 0x1p-124f && cc[3] + dw <= 0x1p124f;
+    auto taps = [&](auto in_range) {
+        int count = 0;
+        for (int s = 0; s < sa.n_kernel; ++s) {
+            const float kx = sa.kernel[3 * s], ky = sa.kernel[3 * s + 1], kz = sa.kernel[3 * s + 2];
+            float cs[4];
 #pragma unroll
-        for (int j = 0; j < 4; ++j) cs[j] = fmaf(radius, fmaf(kz, cN[j], fmaf(ky, cB[j], kx * cT[j])), cc[j]);
-        const float inv = recip(cs[3]);
-        const float sxf = fmaf(vp[0], cs[0] * inv, vp[3]);
-        const float syf = fmaf(vp[5], cs[1] * inv, vp[7]);
-        if (!(sxf > -1.f && sxf < Wf && syf > -1.f && syf < Hf)) continue;
-        const float sz = fmaf(vp[10], cs[2] * inv, vp[11]);
-        const float cached = tap((int)sxf, (int)syf);
-        if (sz >= cached + sa.bias && fabsf(fd - cached) < sa.range) ++count;
-    }
+            for (int j = 0; j < 4; ++j) cs[j] = fmaf(radius, fmaf(kz, cN[j], fmaf(ky, cB[j], kx * cT[j])), cc[j]);
+            const float inv = decltype(in_range)::value ? recip_in_range(cs[3]) : recip(cs[3]);
+            const float sxf = fmaf(vp[0], cs[0] * inv, vp[3]);
+            const float syf = fmaf(vp[5], cs[1] * inv, vp[7]);
+            if (!(sxf > -1.f && sxf < Wf && syf > -1.f && syf < Hf)) continue;
+            const float sz = fmaf(vp[10], cs[2] * inv, vp[11]);
+            const float cached = tap((int)sxf, (int)syf);
+            if (sz >= cached + sa.bias && fabsf(fd - cached) < sa.range) ++count;
+        }
+        return count;
+    };
+    const int count = __all(w_safe) ? taps(std::true_type{}) : taps(std::false_type{});
     const float a = 1.f - (float)count * (1.f / (float)sa.n_kernel);
     ao[i] = (a * a) * a;
 }

@@ -45,3 +45,23 @@ def test_frame_parity(ctx, scene_dir, case, tile):
     mx, frac = lsb_stats(got["image"], want["image"])
     assert mx <= 1 or frac >= 0.999, (mx, frac)
     assert frac >= 0.999, frac
+
+
+def test_ssao_far_and_behind_camera_taps(ctx, scene_dir):
+    """Kernel SSAO with a 3-unit radius seen from 2.6 units: taps land far
+    outside the 32-pixel region around each 64x16 tile that the kernel keeps
+    in LDS (read from the depth plane instead), and taps reach behind the
+    camera (w <= 0: the wavefronts holding such pixels take the loop with
+    the reciprocal's range test).  AO bit-exact against the oracle."""
+    d = os.path.join(scene_dir, "ssao_far")
+    p = G.tiny_scene(d, seed=11, width=256, height=160, subdiv=3, tex=16, ssao=True,
+                     extra="ssaokernel 64 3.0 0.0005 0.5 7\naoblur none")
+    sc = F.Scene(p)
+    assert sc.info.ssao.radius == 3.0 and sc.info.ssao.n_kernel == 64
+    got = gpu_frame(ctx, sc, 16)
+    want = oracle_frame(sc)
+    assert_bitexact("depth", got["depth"], want["depth"])
+    assert_bitexact("gbuf", got["gbuf"], want["gbuf"])
+    ao = want["ao"][want["depth"] < 3e38]
+    assert (ao < 1).mean() > 0.2, "too few occluded pixels to test"
+    assert_bitexact("ao", got["ao"], want["ao"])
