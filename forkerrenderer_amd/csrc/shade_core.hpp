@@ -38,14 +38,17 @@ __device__ __forceinline__ void lighting_px(const FrameUniforms &u, const float 
 
 // Kernel SSAO at pixel (x,y): the deterministic replacement for
 // Render::DoSSAO's RNG hemisphere samples (src/render.cpp:214-289).  Tangent
-// frame from the 4x4 noise vector; the clip-space sample
-// VP * (P + radius * (T kx + B ky + N kz), 1) linearised per pixel as
-// c0 + radius * (kx cT + ky cB + kz cN) (c0 = VP * (P, 1), cT|cB|cN =
-// VP * (dir, 0)), evaluated with fmaf in a fixed order; window coordinates =
-// viewport diagonal * (clip / w) + viewport offset; depth gather, bias and
-// range check; off-screen taps add no occlusion; ao = (1 - count / n)^3.
-// `tap(sx, sy)` reads the depth at an on-screen tap (from the depth plane,
-// or from a workgroup's copy of the part around it: the same floats).
+// frame from the 4x4 noise vector; the sample P + radius * (T kx + B ky +
+// N kz) is projected with the view-projection and the viewport's diagonal
+// and offset, linearised per pixel: with c = VP * (P, 1) and cT|cB|cN =
+// VP * (dir, 0), each column is taken to window-space numerators
+// (vp0 clip_x + vp3 clip_w, vp5 clip_y + vp7 clip_w, vp10 clip_z +
+// vp11 clip_w, clip_w; window = numerator / clip_w), the direction columns
+// times the radius, so a tap is e = ((ec + kx eT) + ky eB) + kz eN by fmaf
+// and window x, y, z = e * (1 / e_w).  Depth gather, bias and range check;
+// off-screen taps add no occlusion; ao = (1 - count / n)^3.  `tap(sx, sy)`
+// reads the depth at an on-screen tap (from the depth plane, or from a
+// workgroup's copy of the part around it: the same floats).
 template <class Tap>
 __device__ __forceinline__ void ssao_px_with(const FrameUniforms &u, const SsaoArgs &sa,
                                              const float *__restrict__ depth,
@@ -73,31 +76,48 @@ __device__ __forceinline__ void ssao_px_with(const FrameUniforms &u, const SsaoA
     }
     const float *vp = u.viewport;
     const float radius = sa.radius;
+    float ec[4], eT[4], eB[4], eN[4];
+    ec[0] = fmaf(vp[0], cc[0], vp[3] * cc[3]);
+    ec[1] = fmaf(vp[5], cc[1], vp[7] * cc[3]);
+    ec[2] = fmaf(vp[10], cc[2], vp[11] * cc[3]);
+    ec[3] = cc[3];
+    const float *cols[3] = {cT, cB, cN};
+    float *outs[3] = {eT, eB, eN};
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+        const float *c = cols[q];
+        float *e = outs[q];
+        e[0] = radius * fmaf(vp[0], c[0], vp[3] * c[3]);
+        e[1] = radius * fmaf(vp[5], c[1], vp[7] * c[3]);
+        e[2] = radius * fmaf(vp[10], c[2], vp[11] * c[3]);
+        e[3] = radius * c[3];
+    }
     // The tap is on screen iff trunc_int(sxf) lies in [0, W): for sxf in
     // (-1, W) the truncation is in range, and outside it (NaN included) it
     // is not (trunc_int maps what does not fit an int to INT_MIN).  Testing
     // the float bounds is that test with fewer operations; W and H are
     // exact in float.
     const float Wf = (float)u.W, Hf = (float)u.H;
-    // Every tap's w = cc[3] + radius * (kx cT[3] + ky cB[3] + kz cN[3]), so
-    // |w - cc[3]| <= |radius| kmax (|cT[3]| + |cB[3]| + |cN[3]|) up to a few
-    // roundings, which the 1.001 factor covers.  When that interval lies in
+    // Every tap's e_w differs from ec[3] by at most kmax (|eT[3]| + |eB[3]|
+    // + |eN[3]|) (widened by 0.1 % for that sum's roundings) plus three
+    // roundings relative to ec[3], which the factor 2 between 2^-124 and
+    // recip's 2^-125 (and 2^124, 2^125) covers.  When that interval lies in
     // [2^-124, 2^124] for every pixel of the wavefront, recip's range test
     // would pass for every tap and is left out (NaN or inf fails the test).
-    const float dw = fabsf(radius) * sa.kmax * (fabsf(cT[3]) + fabsf(cB[3]) + fabsf(cN[3])) * 1.001f;
-    const bool w_safe = cc[3] - dw >= 0x1p-124f && cc[3] + dw <= 0x1p124f;
+    const float dw = sa.kmax * (fabsf(eT[3]) + fabsf(eB[3]) + fabsf(eN[3])) * 1.001f;
+    const bool w_safe = ec[3] - dw >= 0x1p-124f && ec[3] + dw <= 0x1p124f;
     auto taps = [&](auto in_range) {
         int count = 0;
         for (int s = 0; s < sa.n_kernel; ++s) {
             const float kx = sa.kernel[3 * s], ky = sa.kernel[3 * s + 1], kz = sa.kernel[3 * s + 2];
-            float cs[4];
+            float e[4];
 #pragma unroll
-            for (int j = 0; j < 4; ++j) cs[j] = fmaf(radius, fmaf(kz, cN[j], fmaf(ky, cB[j], kx * cT[j])), cc[j]);
-            const float inv = decltype(in_range)::value ? recip_in_range(cs[3]) : recip(cs[3]);
-            const float sxf = fmaf(vp[0], cs[0] * inv, vp[3]);
-            const float syf = fmaf(vp[5], cs[1] * inv, vp[7]);
+            for (int j = 0; j < 4; ++j) e[j] = fmaf(kz, eN[j], fmaf(ky, eB[j], fmaf(kx, eT[j], ec[j])));
+            const float inv = decltype(in_range)::value ? recip_in_range(e[3]) : recip(e[3]);
+            const float sxf = e[0] * inv;
+            const float syf = e[1] * inv;
             if (!(sxf > -1.f && sxf < Wf && syf > -1.f && syf < Hf)) continue;
-            const float sz = fmaf(vp[10], cs[2] * inv, vp[11]);
+            const float sz = e[2] * inv;
             const float cached = tap((int)sxf, (int)syf);
             if (sz >= cached + sa.bias && fabsf(fd - cached) < sa.range) ++count;
         }

@@ -1026,12 +1026,13 @@ void orc_ssao_kernel(const orc_frame *fr, int W, int H, const float *depth,
                      int n_kernel, const float *noise, float radius, float bias, float range,
                      float *ao)
 {
-    /* Kernel SSAO with the projection linearised per pixel: the clip-space
-     * sample VP * (P + radius * (T kx + B ky + N kz), 1) is evaluated as
-     * c0 + radius * (kx cT + ky cB + kz cN) with c0 = VP * (P, 1) and cT,
-     * cB, cN = VP * (T|B|N, 0), each component by fmaf in the order below;
-     * window x, y, z = viewport diagonal * (clip / w) + viewport offset by
-     * fmaf; occlusion as Render::DoSSAO tests it (render.cpp:250-275). */
+    /* Kernel SSAO with the projection linearised per pixel: c = VP * (P, 1)
+     * and cT, cB, cN = VP * (T|B|N, 0); each column is taken to window-space
+     * numerators (vp0 clip_x + vp3 clip_w, vp5 clip_y + vp7 clip_w,
+     * vp10 clip_z + vp11 clip_w, clip_w) by fmaf, the direction columns
+     * times the radius; a tap is e = ((ec + kx eT) + ky eB) + kz eN by fmaf
+     * and window x, y, z = e * (1 / e_w); occlusion as Render::DoSSAO tests
+     * it (render.cpp:250-275). */
     float VP[16];
     orc_mat_mul(fr->proj, fr->view, VP);
     const float *vp = fr->viewport;
@@ -1055,16 +1056,29 @@ void orc_ssao_kernel(const orc_frame *fr, int W, int H, const float *depth,
                 cN[j] = dot3(row, N);
             }
             cc[0] = c0.x; cc[1] = c0.y; cc[2] = c0.z; cc[3] = c0.w;
+            float ec[4], e3[3][4];
+            ec[0] = fmaf(vp[0], cc[0], vp[3] * cc[3]);
+            ec[1] = fmaf(vp[5], cc[1], vp[7] * cc[3]);
+            ec[2] = fmaf(vp[10], cc[2], vp[11] * cc[3]);
+            ec[3] = cc[3];
+            const float *cols[3] = {cT, cB, cN};
+            for (int q = 0; q < 3; ++q) {
+                const float *c = cols[q];
+                e3[q][0] = radius * fmaf(vp[0], c[0], vp[3] * c[3]);
+                e3[q][1] = radius * fmaf(vp[5], c[1], vp[7] * c[3]);
+                e3[q][2] = radius * fmaf(vp[10], c[2], vp[11] * c[3]);
+                e3[q][3] = radius * c[3];
+            }
             int count = 0;
             for (int s = 0; s < n_kernel; ++s) {
                 const float kx = kernel[3 * s], ky = kernel[3 * s + 1], kz = kernel[3 * s + 2];
-                float cs[4];
+                float e[4];
                 for (int j = 0; j < 4; ++j)
-                    cs[j] = fmaf(radius, fmaf(kz, cN[j], fmaf(ky, cB[j], kx * cT[j])), cc[j]);
-                const float inv = 1.f / cs[3];
-                const float sxf = fmaf(vp[0], cs[0] * inv, vp[3]);
-                const float syf = fmaf(vp[5], cs[1] * inv, vp[7]);
-                const float sz = fmaf(vp[10], cs[2] * inv, vp[11]);
+                    e[j] = fmaf(kz, e3[2][j], fmaf(ky, e3[1][j], fmaf(kx, e3[0][j], ec[j])));
+                const float inv = 1.f / e[3];
+                const float sxf = e[0] * inv;
+                const float syf = e[1] * inv;
+                const float sz = e[2] * inv;
                 int sx = orc_trunc_int(sxf), sy = orc_trunc_int(syf);
                 if (sx < 0 || sy < 0 || sx >= W || sy >= H) continue;
                 float cached = depth[(size_t)sx + (size_t)sy * W];
