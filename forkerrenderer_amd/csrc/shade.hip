@@ -169,8 +169,8 @@ __global__ __launch_bounds__(256) void k_pcss_offsets(const uint32_t *__restrict
 // Kernel SSAO over 64 x 16-pixel tiles (1024 threads, a wavefront per row):
 // the workgroup first copies the depth plane around its tile, 32 pixels
 // each side, into LDS (coalesced 16-byte loads), and taps that land there
-// read LDS; the others read the plane.  About 46 vector ALU operations per
-// tap (projection, reciprocal, bounds, compare) bound the kernel.
+// read LDS; the others read the plane.  Per tap it issues about 42 vector
+// ALU operations (projection, reciprocal, bounds, address, compare).
 constexpr int kSsaoTW = 64, kSsaoTH = 16, kSsaoHalo = 32;
 constexpr int kSsaoLW = kSsaoTW + 2 * kSsaoHalo, kSsaoLH = kSsaoTH + 2 * kSsaoHalo;
 
