@@ -205,7 +205,6 @@ __global__ __launch_bounds__(1024) void k_ssao_tiled(FrameUniforms u, SsaoArgs s
     const int x = x0 + (threadIdx.x & 63), y = y0 + (threadIdx.x >> 6);
     if (x >= u.W || y >= u.H) return;
     const uint32_t W = (uint32_t)u.W;
-    const int lbase = ly0 * kSsaoLW + lx0;
     // The plane is read through a global-address-space pointer: with two
     // generic pointers the compiler merges both reads into one flat load,
     // which sends the LDS taps down the vector-memory path as well.
@@ -217,7 +216,7 @@ __global__ __launch_bounds__(1024) void k_ssao_tiled(FrameUniforms u, SsaoArgs s
         // Every lane reads LDS (a far lane at the tile's first word) and far
         // lanes also read the plane, into another register: the LDS read
         // does not wait for the plane's.
-        const float dl = tile[near ? sy * kSsaoLW + sx - lbase : 0];
+        const float dl = tile[near ? ty * kSsaoLW + tx : 0u];
         float dg = 0.f;
         if (!near) dg = gdepth[(uint32_t)sx + (uint32_t)sy * W];
         return near ? dl : dg;
