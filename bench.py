@@ -332,8 +332,8 @@ def main():
         print(json.dumps({"copy_bytes": nbytes, "ms": ms, "GBps_read_plus_write": 2 * nbytes / (ms * 1e-3) / 1e9}))
         return
     if a.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU; multi-GPU sharding is described in "
-                                   "DESIGN.md and not launched", "n_gpus": a.gpus}))
+        print(json.dumps({"error": "runs on one GPU only: the frame path is not sharded across GPUs "
+                                   "(--gpus 1)", "n_gpus": a.gpus}))
         sys.exit(2)
     import forkerrenderer_amd as F
     p = gen_scene(a.config)
