@@ -172,6 +172,7 @@ __global__ __launch_bounds__(256) void k_pcss_offsets(const uint32_t *__restrict
 // read LDS; the others read the plane.  Per tap it issues about 42 vector
 // ALU operations (projection, reciprocal, bounds, address, compare).
 constexpr int kSsaoTW = 64, kSsaoTH = 16, kSsaoHalo = 32;
+static_assert(kSsaoTW * kSsaoTH == 1024 && kSsaoHalo % 4 == 0, "one thread per pixel; 16-byte staging");
 constexpr int kSsaoLW = kSsaoTW + 2 * kSsaoHalo, kSsaoLH = kSsaoTH + 2 * kSsaoHalo;
 
 __global__ __launch_bounds__(1024) void k_ssao_tiled(FrameUniforms u, SsaoArgs sa,
@@ -202,7 +203,7 @@ __global__ __launch_bounds__(1024) void k_ssao_tiled(FrameUniforms u, SsaoArgs s
         for (int e = 0; e < 4; ++e) tile[row * kSsaoLW + 4 * c4 + e] = v[e];
     }
     __syncthreads();
-    const int x = x0 + (threadIdx.x & 63), y = y0 + (threadIdx.x >> 6);
+    const int x = x0 + (int)(threadIdx.x % kSsaoTW), y = y0 + (int)(threadIdx.x / kSsaoTW);
     if (x >= u.W || y >= u.H) return;
     const uint32_t W = (uint32_t)u.W;
     // The plane is read through a global-address-space pointer: with two
