@@ -95,6 +95,7 @@ def lib():
             "fgl_upload_material": (C.c_int, [P, C.POINTER(Material), C.POINTER(C.c_int)]),
             "fgl_upload_model": (C.c_int, [P, C.POINTER(ModelDesc), C.POINTER(C.c_int)]),
             "fgl_init_targets": (C.c_int, [P, C.c_int, C.c_int]),
+            "fgl_init_buffer": (C.c_int, [P, C.c_int, C.c_int, C.c_int]),
             "fgl_init_shadow_buffer": (C.c_int, [P, C.c_int, C.c_int]),
             "fgl_clear_color": (C.c_int, [P, C.c_float, C.c_float, C.c_float]),
             "fgl_set_viewport": (C.c_int, [P, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int]),
@@ -291,6 +292,11 @@ class Context:
 
     def init_targets(self, w, h):
         self._chk(lib().fgl_init_targets(self._h, w, h), "fgl_init_targets")
+
+    def init_buffer(self, which, w, h):
+        """ForkerGL::InitFrameBuffer / InitDepthBuffer / InitGeometryBuffers /
+        InitShadowBuffer one at a time (which = TARGET_FRAME, _DEPTH, _GBUFFER, _SHADOW)."""
+        self._chk(lib().fgl_init_buffer(self._h, which, w, h), "fgl_init_buffer")
 
     def init_shadow_buffer(self, w, h):
         self._chk(lib().fgl_init_shadow_buffer(self._h, w, h), "fgl_init_shadow_buffer")

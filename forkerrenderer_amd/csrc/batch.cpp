@@ -141,6 +141,7 @@ struct fgl_batch {
     uint32_t *h_tot = nullptr;
     const DevMaterial *materials = nullptr;
     const DevTexture *textures = nullptr;
+    uint64_t nao_gen = 0;   // generation of the neural-AO buffers the graph captured
 };
 
 static int bfail(fgl_batch *b, int code, const char *msg) { return ctx_fail(b->ctx, code, msg); }
@@ -616,11 +617,38 @@ int fgl_batch_use_graph(fgl_batch *b, int on)
     return FGL_OK;
 }
 
+// The context's material / texture tables and the neural-AO buffers can move
+// after the batch was planned (an upload that outgrows a table, a larger
+// single-frame inference): re-read them before every render and capture the
+// graph again when any address changed.
+static int refresh_resources(fgl_batch *b)
+{
+    const DevMaterial *m = nullptr;
+    const DevTexture *t = nullptr;
+    int rc = ctx_tables(b->ctx, &m, &t);
+    if (rc) return rc;
+    if (m != b->materials || t != b->textures) {
+        b->materials = m;
+        b->textures = t;
+        drop_graph(b);
+    }
+    if (b->ao_mode == FGL_AO_NEURAL) {
+        if ((rc = nao_reserve(b->nao, b->total_px))) return bfail(b, rc, "neural AO buffers");
+        const uint64_t g = nao_buffers_generation(b->nao);
+        if (g != b->nao_gen) {
+            b->nao_gen = g;
+            drop_graph(b);
+        }
+    }
+    return FGL_OK;
+}
+
 int fgl_batch_render(fgl_batch *b)
 {
     if (!b) return FGL_ERR_INVALID;
     const bool timed = ctx_timing(b->ctx);
     int rc;
+    if ((rc = refresh_resources(b))) return rc;
     if (!b->measured) {
         // first render: measure each pass's pairs and size the buffers
         if ((rc = enqueue(b, true, timed))) return rc;
@@ -643,7 +671,9 @@ int fgl_batch_sync(fgl_batch *b, int *rerenders)
         // a pass overflowed its pair capacity: re-render, sizing from the counts
         b->cap_req[0] = b->cap_req[1] = 0;
         drop_graph(b);
-        int rc = enqueue(b, true, false);
+        int rc = refresh_resources(b);
+        if (rc) return rc;
+        rc = enqueue(b, true, false);
         if (rc) return rc;
         if (hipStreamSynchronize(s) != hipSuccess) return bfail(b, FGL_ERR_DEVICE, "batch failed");
         if (rerenders) ++*rerenders;

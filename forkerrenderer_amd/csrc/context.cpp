@@ -514,10 +514,27 @@ int fgl_clear_color(fgl_ctx *c, float r, float g, float b)
     return FGL_OK;
 }
 
+// The draws recorded so far were issued under the current ForkerGL state
+// (in the reference each draw runs at once with its shader's uniforms,
+// src/model.cpp:77-90).  A setter that changes state the pending draws read
+// (matrices, viewport, eye, lights, shadow settings) rasterises them first,
+// so every draw sees the values set before it; setting an equal value keeps
+// the pass batched.
+static int flush_if_changed(fgl_ctx *c, const void *cur, const void *next, size_t bytes)
+{
+    if (c->draws.empty() || std::memcmp(cur, next, bytes) == 0) return FGL_OK;
+    return flush_pass(c);
+}
+
 int fgl_set_viewport(fgl_ctx *c, int shadow, int x, int y, int w, int h)
 {
     if (!c) return FGL_ERR_INVALID;
-    make_viewport(x, y, w, h, shadow ? c->shadow_viewport : c->viewport);
+    float vp[16];
+    make_viewport(x, y, w, h, vp);
+    float *dst = shadow ? c->shadow_viewport : c->viewport;
+    int rc = flush_if_changed(c, dst, vp, sizeof(vp));
+    if (rc) return rc;
+    std::memcpy(dst, vp, sizeof(vp));
     return FGL_OK;
 }
 
@@ -528,6 +545,8 @@ int fgl_set_matrix(fgl_ctx *c, int which, const float m[16])
                : which == FGL_MAT_PROJ ? c->proj
                : which == FGL_MAT_LIGHT_SPACE ? c->light_space : nullptr;
     if (!dst) return fail(c, FGL_ERR_INVALID, "unknown matrix");
+    int rc = flush_if_changed(c, dst, m, 16 * sizeof(float));
+    if (rc) return rc;
     std::memcpy(dst, m, 16 * sizeof(float));
     return FGL_OK;
 }
@@ -535,6 +554,8 @@ int fgl_set_matrix(fgl_ctx *c, int which, const float m[16])
 int fgl_set_eye(fgl_ctx *c, const float eye[3])
 {
     if (!c || !eye) return FGL_ERR_INVALID;
+    int rc = flush_if_changed(c, c->eye, eye, 3 * sizeof(float));
+    if (rc) return rc;
     std::memcpy(c->eye, eye, 3 * sizeof(float));
     return FGL_OK;
 }
@@ -542,6 +563,11 @@ int fgl_set_eye(fgl_ctx *c, const float eye[3])
 int fgl_set_lights(fgl_ctx *c, int n, const fgl_light *lights)
 {
     if (!c || n < 0 || n > FGL_MAX_LIGHTS || (n && !lights)) return fail(c, FGL_ERR_INVALID, "bad lights");
+    if (!c->draws.empty() &&
+        (n != (int)c->lights.size() || (n && std::memcmp(c->lights.data(), lights, n * sizeof(fgl_light)) != 0))) {
+        int rc = flush_pass(c);
+        if (rc) return rc;
+    }
     c->lights.assign(lights, lights + n);
     return FGL_OK;
 }
@@ -555,8 +581,13 @@ int fgl_set_shadow(fgl_ctx *c, int on, const fgl_shadow_cfg *cfg)
             ((cfg->mode == 1 || cfg->mode == 2 || cfg->mode >= 4) && cfg->n_pcf < 1) ||
             (cfg->mode == 5 && cfg->n_blocker < 1))
             return fail(c, FGL_ERR_INVALID, "bad shadow filter configuration");
-        c->shadow_cfg = *cfg;
     }
+    if (!c->draws.empty() && ((on ? 1 : 0) != c->shadow_on ||
+                              (cfg && std::memcmp(cfg, &c->shadow_cfg, sizeof(fgl_shadow_cfg)) != 0))) {
+        int rc = flush_pass(c);
+        if (rc) return rc;
+    }
+    if (cfg) c->shadow_cfg = *cfg;
     c->shadow_on = on ? 1 : 0;
     return FGL_OK;
 }

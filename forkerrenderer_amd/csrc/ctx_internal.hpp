@@ -65,6 +65,9 @@ struct NaoBatchTables {
     uint32_t n_items[4];
 };
 int nao_reserve(fgl_nao *n, size_t total_px);
+// Changes whenever nao_reserve moves the inference buffers (a graph that
+// captured their addresses must then be captured again).
+uint64_t nao_buffers_generation(const fgl_nao *n);
 // Appends the convolution tiles of a W x H view to the four per-layer lists.
 void nao_conv_items(int view, int W, int H, std::vector<uint2> items[4]);
 int nao_infer_batch(fgl_nao *n, hipStream_t s, const NaoBatchTables &t, size_t total_px, const float *depth,

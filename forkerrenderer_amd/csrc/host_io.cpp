@@ -194,30 +194,27 @@ bool tga_read(const std::string &path, HostTexture &t, std::string &err)
         in.read(reinterpret_cast<char *>(t.bytes.data()), nbytes);
         if (!in.good()) { err = "an error occurred while reading the data: " + path; return false; }
     } else if (h.datatypecode == 10 || h.datatypecode == 11) {
-        size_t npix = (size_t)t.w * t.h, cur = 0, byte = 0;
+        // Packets: a header byte h < 128 is followed by h + 1 literal pixels;
+        // h >= 128 by one pixel repeated h - 127 times.  A packet that would
+        // put a pixel past the image is an error, reported before the write.
+        const size_t npix = (size_t)t.w * t.h;
+        size_t cur = 0;
         uint8_t px[4];
-        do {
-            int ch = in.get();
+        while (cur < npix) {
+            const int ch = in.get();
             if (!in.good()) { err = "an error occurred while reading the data: " + path; return false; }
-            int chunk = ch & 255;
-            if (chunk < 128) {
-                chunk++;
-                for (int i = 0; i < chunk; ++i) {
+            const bool literal = ch < 128;
+            const int count = literal ? ch + 1 : ch - 127;
+            for (int i = 0; i < count; ++i) {
+                if (literal || i == 0) {
                     in.read(reinterpret_cast<char *>(px), t.bpp);
                     if (!in.good()) { err = "rle read error: " + path; return false; }
-                    for (int c = 0; c < t.bpp; ++c) t.bytes[byte++] = px[c];
-                    if (++cur > npix) { err = "too many pixels read: " + path; return false; }
                 }
-            } else {
-                chunk -= 127;
-                in.read(reinterpret_cast<char *>(px), t.bpp);
-                if (!in.good()) { err = "rle read error: " + path; return false; }
-                for (int i = 0; i < chunk; ++i) {
-                    for (int c = 0; c < t.bpp; ++c) t.bytes[byte++] = px[c];
-                    if (++cur > npix) { err = "too many pixels read: " + path; return false; }
-                }
+                if (cur >= npix) { err = "too many pixels read: " + path; return false; }
+                std::memcpy(t.bytes.data() + cur * t.bpp, px, t.bpp);
+                ++cur;
             }
-        } while (cur < npix);
+        }
     } else {
         err = "unknown file format " + std::to_string((int)h.datatypecode) + ": " + path;
         return false;
@@ -225,6 +222,28 @@ bool tga_read(const std::string &path, HostTexture &t, std::string &err)
     if (!(h.imagedescriptor & 0x20)) flip_vertically(t);
     if (h.imagedescriptor & 0x10) flip_horizontally(t);
     return true;
+}
+
+// Length of the run-length packet that starts at pixel i, as the reference's
+// encoder chooses it (src/tgaimage.cpp:259-300; greedy, left to right, at
+// most 128 pixels per packet).  When pixel i equals pixel i+1 the packet
+// repeats pixel i over the whole stretch of equal pixels.  Otherwise it holds
+// literal pixels and ends just before the first pixel at offset 1..126 that
+// equals its successor (that pixel starts the next, repeating packet); with
+// no such pixel it takes 128 pixels, or the rest of the image.
+static size_t rle_packet(const uint8_t *d, size_t npix, int bpp, size_t i, bool &repeat)
+{
+    auto same = [&](size_t a, size_t b) { return std::memcmp(d + a * bpp, d + b * bpp, bpp) == 0; };
+    const size_t left = npix - i, cap = left < 128 ? left : 128;
+    size_t n = 1;
+    repeat = cap >= 2 && same(i, i + 1);
+    if (repeat) {
+        n = 2;
+        while (n < cap && same(i + n - 1, i + n)) ++n;
+        return n;
+    }
+    while (n < cap && !(n <= 126 && i + n + 1 < npix && same(i + n, i + n + 1))) ++n;
+    return n;
 }
 
 // WriteTgaFile + unloadRleData (src/tgaimage.cpp:172-300): bottom-left origin,
@@ -245,23 +264,12 @@ bool tga_write(const std::string &path, int w, int h, int bpp, const uint8_t *da
         out.write(reinterpret_cast<const char *>(data), (size_t)w * h * bpp);
     } else {
         const size_t npix = (size_t)w * h;
-        size_t cur = 0;
-        while (cur < npix) {
-            size_t start = cur * bpp, curbyte = cur * bpp;
-            uint8_t run = 1;
-            bool raw = true;
-            while (cur + run < npix && run < 128) {
-                bool eq = true;
-                for (int t = 0; eq && t < bpp; ++t) eq = data[curbyte + t] == data[curbyte + t + bpp];
-                curbyte += bpp;
-                if (run == 1) raw = !eq;
-                if (raw && eq) { run--; break; }
-                if (!raw && !eq) break;
-                run++;
-            }
-            cur += run;
-            out.put((char)(raw ? run - 1 : run + 127));
-            out.write(reinterpret_cast<const char *>(data + start), raw ? (size_t)run * bpp : (size_t)bpp);
+        for (size_t cur = 0; cur < npix;) {
+            bool repeat = false;
+            const size_t n = rle_packet(data, npix, bpp, cur, repeat);
+            out.put((char)(repeat ? n + 127 : n - 1));
+            out.write(reinterpret_cast<const char *>(data + cur * bpp), (repeat ? 1 : n) * (size_t)bpp);
+            cur += n;
         }
     }
     const uint8_t zeros[8] = {0, 0, 0, 0, 0, 0, 0, 0};

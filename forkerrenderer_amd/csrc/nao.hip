@@ -817,6 +817,7 @@ struct fgl_nao {
     uint16_t *ia = nullptr, *ib = nullptr;
     uint8_t *imask = nullptr;
     size_t icap = 0;
+    uint64_t igen = 0;   // bumped whenever the inference buffers move
     float *h_stats = nullptr;
 };
 
@@ -840,12 +841,15 @@ int nao_reserve(fgl_nao *n, size_t px)
         n->icap = 0;
     }
     // activations of layers 1-3 (layer 1 reads the G-buffer, layer 4 writes AO)
+    ++n->igen;
     if (nao_alloc((void **)&n->ia, px * CH * 2) || nao_alloc((void **)&n->ib, px * CH * 2) ||
         nao_alloc((void **)&n->imask, px))
         return FGL_ERR_NOMEM;
     n->icap = px;
     return FGL_OK;
 }
+
+uint64_t nao_buffers_generation(const fgl_nao *n) { return n->igen; }
 
 void nao_conv_items(int view, int W, int H, std::vector<uint2> items[4])
 {

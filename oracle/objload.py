@@ -72,6 +72,38 @@ def tga_read(path):
 
 # ---------------- math helpers in float32 ----------------
 
+def tga_rle_encode(data, bpp):
+    """Packet stream of TGAImage::unloadRleData (src/tgaimage.cpp:250-300) for
+    `data`, the pixel bytes in file order: the reference's greedy loop, step
+    for step -- grow the packet while the next pixel compares as the first
+    pair did, a literal packet giving back its last pixel when it meets an
+    equal pair, at most 128 pixels per packet."""
+    data = bytes(data)
+    npix = len(data) // bpp
+    out = bytearray()
+    cur = 0
+    while cur < npix:
+        start = cur * bpp
+        byte = cur * bpp
+        run = 1
+        raw = True
+        while cur + run < npix and run < 128:
+            eq = data[byte:byte + bpp] == data[byte + bpp:byte + 2 * bpp]
+            byte += bpp
+            if run == 1:
+                raw = not eq
+            if raw and eq:
+                run -= 1
+                break
+            if not raw and not eq:
+                break
+            run += 1
+        cur += run
+        out.append(run - 1 if raw else run + 127)
+        out += data[start:start + (run * bpp if raw else bpp)]
+    return bytes(out)
+
+
 def _norm3(v):
     v = np.asarray(v, F32)
     ls = F32(F32(v[0] * v[0]) + F32(v[1] * v[1]))

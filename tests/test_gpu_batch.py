@@ -106,3 +106,44 @@ def test_batch_without_ao(ctx, world):
     b.sync()
     check(b, views[:3], ref, "no AO")
     b.close()
+
+
+def test_batch_after_tables_and_ao_buffers_move(ctx, world, scene_dir):
+    """Resources the batch's captured graph addresses can move after the
+    batch was planned: the context's material and texture tables when new
+    uploads outgrow them, the neural-AO activations when a single-frame
+    inference runs on a frame larger than the whole batch.  Replays after
+    each move still equal the single-view renders."""
+    import os
+    from forkerrenderer_amd import scenegen as G
+    views, net, singles = world
+    b = VW.Batch(ctx, views, VW.AO_NEURAL, net)
+    b.render()
+    assert b.sync() == 0
+    b.render()            # the graph is captured here
+    assert b.sync() == 0
+    check(b, views, singles, "before")
+    # many new materials and textures: both tables are reallocated
+    tex = np.zeros((4, 4, 3), np.uint8)
+    m = F.Material()
+    for f in ("map_kd", "map_ks", "map_ke", "map_bump", "map_base", "map_rough", "map_metal", "map_ao", "map_pbr_ke"):
+        setattr(m, f, -1)
+    for _ in range(300):
+        ctx.upload_texture(tex)
+        ctx.upload_material(m)
+    for rep in range(2):
+        b.render()
+        assert b.sync() == 0
+        check(b, views, singles, f"after table growth, replay {rep}")
+    # a single-frame neural AO larger than the batch moves the activations
+    px = b.stats()["pixels"]
+    W, H = 1024, 1 + px // 1024
+    p = G.tiny_scene(os.path.join(scene_dir, "batch_big_nao"), seed=3, width=W, height=H, subdiv=2, tex=16)
+    ctx.render_scene(F.Scene(p))
+    net.infer()
+    ctx.sync()
+    for rep in range(2):
+        b.render()
+        assert b.sync() == 0
+        check(b, views, singles, f"after activation move, replay {rep}")
+    b.close()

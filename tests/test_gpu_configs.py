@@ -1,0 +1,123 @@
+"""The benchmark configurations at their real sizes, against the oracle.
+
+Config 2: 1280x800 deferred PBR, 327,682 triangles, 2048^2 PBR maps,
+2048^2 shadow map with 5x5 PCF, 64-sample kernel SSAO + sigma=2 blur --
+the whole frame.
+
+Config 3: 1920x1080 with SSAA 2x (3840x2160 internal), 5,242,882 triangles,
+PCSS 16/32 area 0.05, kernel SSAO + sigma=2 blur, a point and a directional
+light.  Shadow map, shadow-pass depth, depth and all 19 G-buffer planes are
+compared bit for bit over the whole frame; the unblurred SSAO on row bands
+(the oracle's SSAO takes ~25 s for all 2160 rows on one core); the blur over
+the whole frame (oracle blur of the device's unblurred AO); lighting and
+quantisation on row bands (oracle lighting of the device's G-buffer and AO,
+both already checked), and the SSAA resolve of the whole image from the
+device's quantised frame by the integer closed form.
+
+Bars: bit-exact for coverage, depth, shadow map, G-buffer and AO; 8-bit
+output within +-1 LSB on >= 99.9 % of the pixels.
+"""
+import os
+
+import numpy as np
+import pytest
+
+import forkerrenderer_amd as F
+import oracle as O
+from forkerrenderer_amd import scenegen as G
+from tests.util import assert_bitexact, lsb_stats, oracle_frame, scene_arrays
+
+pytestmark = pytest.mark.gpu
+
+
+def _rows(a, W, y0, y1):
+    return a.reshape(-1, W)[y0:y1] if a.ndim == 1 else a.reshape(a.shape[0], -1, W)[:, y0:y1]
+
+
+def test_config2_whole_frame(ctx, scene_dir):
+    p = G.config2(os.path.join(scene_dir, "c2"))
+    sc = F.Scene(p)
+    I = sc.info
+    assert (I.width, I.height, I.shadow_map_size, I.ssao.n_kernel) == (1280, 800, 2048, 64)
+    print("config 2: render on the device", flush=True)
+    ctx.set_tile_size(16)
+    ctx.render_scene(sc)
+    got = {"depth": ctx.read(F.TARGET_DEPTH), "gbuf": ctx.read(F.TARGET_GBUFFER).reshape(F.GBUF_PLANES, -1),
+           "ao": ctx.read(F.TARGET_AO), "shadow": ctx.read(F.TARGET_SHADOW),
+           "shadow_depth": ctx.read(F.TARGET_SHADOW_DEPTH), "image": ctx.read(F.TARGET_IMAGE)}
+    print("config 2: oracle", flush=True)
+    want = oracle_frame(sc)
+    ntri = sum(len(sc.model(i)[0]["vi"]) for i in range(I.n_models))
+    assert ntri == 327682
+    assert (want["depth"] < 3e38).mean() > 0.2     # sphere and ground cover ~26 % of the frame
+    for k in ("shadow", "shadow_depth", "depth", "gbuf", "ao"):
+        assert_bitexact(k, got[k], want[k])
+    mx, frac = lsb_stats(got["image"], want["image"])
+    print(f"config 2 image: max diff {mx}, within 1 LSB {frac:.6f}")
+    assert frac >= 0.999, (mx, frac)
+
+
+def test_config3_full_size(ctx, scene_dir):
+    p = G.config3(os.path.join(scene_dir, "c3"))
+    sc = F.Scene(p)
+    I = sc.info
+    models, mats, texs = scene_arrays(sc)
+    assert sum(len(m["vi"]) for m, _ in models) == 5242882
+    st = O.frame_setup(I)
+    W, H, SW, SH, k = st["W"], st["H"], st["SW"], st["SH"], st["k"]
+    assert (W, H, SW, SH, k) == (3840, 2160, 2048, 2048, 2)
+    print("config 3: render on the device", flush=True)
+    ctx.set_pass_tile_size(F.PASS_SHADOW, 16)
+    ctx.set_pass_tile_size(F.PASS_GEOMETRY, 16)
+    ctx.render_scene(sc)
+    g_depth = ctx.read(F.TARGET_DEPTH)
+    g_gbuf = ctx.read(F.TARGET_GBUFFER).reshape(F.GBUF_PLANES, -1)
+    g_shadow, g_sdepth = ctx.read(F.TARGET_SHADOW), ctx.read(F.TARGET_SHADOW_DEPTH)
+    g_ao = ctx.read(F.TARGET_AO)
+    g_fb = ctx.read(F.TARGET_FRAME).reshape(3, -1)
+    g_img = ctx.read(F.TARGET_IMAGE)
+    # the same frame's SSAO without the blur (depth and G-buffer are unchanged)
+    cfg = F.SsaoCfg()
+    F.C.memmove(F.C.byref(cfg), F.C.byref(I.ssao), F.C.sizeof(F.SsaoCfg))
+    cfg.blur = 0
+    ctx.ssao(cfg)
+    g_raw = ctx.read(F.TARGET_AO)
+
+    osc = O.OracleScene(models, mats, texs)
+    print("config 3: oracle shadow pass", flush=True)
+    sdepth, smap = O.shadow_pass(osc, st["fr_shadow"], SW, SH)
+    assert_bitexact("shadow_depth", g_sdepth, sdepth)
+    assert_bitexact("shadow", g_shadow, smap)
+    print("config 3: oracle geometry pass", flush=True)
+    depth, gbuf = O.geometry_pass(osc, st["fr"], W, H)
+    cover = depth < 3e38
+    assert cover.mean() > 0.5
+    assert_bitexact("depth", g_depth, depth)
+    assert_bitexact("gbuf", g_gbuf, gbuf)
+    del gbuf
+    fr = st["fr"]
+    lib = O.lib()
+    bands = [(0, 48), (1040, 1136), (2112, 2160)]   # bottom rows, middle, top rows
+    R = I.ssao.blur_radius
+    try:
+        for y0, y1 in bands:
+            print(f"config 3: oracle SSAO and lighting, rows {y0}-{y1}", flush=True)
+            # unblurred AO on the band widened by the blur radius: every blurred
+            # value the band's lighting reads is then checked
+            a0, a1 = max(0, y0 - R), min(H, y1 + R)
+            lib.orc_set_rows(a0, a1)
+            raw = O.ssao_kernel(fr, W, H, depth, g_gbuf, I.ssao)
+            assert_bitexact(f"raw AO rows {a0}-{a1}", _rows(g_raw, W, a0, a1), _rows(raw, W, a0, a1))
+            lib.orc_set_rows(y0, y1)
+            fb = O.lighting_pass(fr, W, H, g_gbuf, g_ao, smap, SW, SH)
+            q = O.quantize_rgb(_rows(fb, W, y0, y1).reshape(3, -1), W, y1 - y0)
+            gq = O.quantize_rgb(_rows(g_fb, W, y0, y1).reshape(3, -1), W, y1 - y0)
+            mx, frac = lsb_stats(gq, q)
+            print(f"config 3 rows {y0}-{y1}: max diff {mx}, within 1 LSB {frac:.6f}", flush=True)
+            assert frac >= 0.999, (y0, y1, mx, frac)
+    finally:
+        lib.orc_set_rows(0, 1 << 30)
+    occluded = (_rows(g_raw, W, 1040, 1136) < 1)[_rows(cover, W, 1040, 1136)]
+    assert occluded.mean() > 0.05, "middle band has too little occlusion to test"
+    assert_bitexact("blurred AO", g_ao, O.blur_separable(g_raw, W, H, I.ssao.blur_weights, I.ssao.blur_radius))
+    assert np.array_equal(g_img, O.ssaa_resolve(O.quantize_rgb(g_fb, W, H), W, H, k)), "SSAA resolve"

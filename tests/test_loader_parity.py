@@ -148,3 +148,88 @@ def test_stochastic_scene_keys(scene_dir):
         assert s.area_light == np.float32(2.5)
     p = G.tiny_scene(os.path.join(scene_dir, "kdet"), seed=1, ssao=True)
     assert F.Scene(p).info.ssao.stochastic == 0
+
+
+def test_rle_stream_equals_the_reference_encoder(tmp_path):
+    """The product's packetiser (written from the packet rule) against the
+    oracle's step-for-step restatement of the reference's encoder loop, on
+    rows built to hit every boundary: runs of 1..300 equal pixels (runs
+    longer than 128 split), literal stretches of 1..300 (split at 128), a
+    literal stretch ending in an equal pair at offsets 125-128, single
+    pixels at the image end, 1/3/4 bytes per pixel."""
+    from oracle.objload import tga_rle_encode
+    rng = np.random.default_rng(5)
+    cases = []
+    for bpp in (1, 3, 4):
+        for _ in range(40):
+            parts = []
+            for _ in range(int(rng.integers(1, 6))):
+                n = int(rng.choice([1, 2, 3, 125, 126, 127, 128, 129, 130, 255, 256, 257, 300]))
+                if rng.random() < 0.5:
+                    parts.append(np.repeat(rng.integers(0, 256, (1, bpp)), n, 0))
+                else:
+                    lit = rng.integers(0, 256, (n, bpp))
+                    lit[1:, 0] = (lit[:-1, 0] + 1 + lit[1:, 0] % 200) % 256   # neighbours differ
+                    parts.append(lit)
+            cases.append(np.concatenate(parts).astype(np.uint8))
+        cases.append(rng.integers(0, 2, (1000, bpp)).astype(np.uint8))   # dense mix of short runs
+        for k in (124, 125, 126, 127, 128):                            # equal pair at offset k
+            lit = np.stack([np.arange(300) % 251] * bpp, 1).astype(np.uint8)
+            lit[k + 1] = lit[k]
+            cases.append(lit)
+    for i, px in enumerate(cases):
+        bpp = px.shape[1]
+        img = px.reshape(1, -1, bpp) if bpp > 1 else px.reshape(1, -1)
+        p = str(tmp_path / f"c{i}.tga")
+        F.tga_write(p, img, rle=True)
+        got = open(p, "rb").read()[18:-26]
+        assert got == tga_rle_encode(px.tobytes(), bpp), f"case {i} (bpp {bpp}, {len(px)} px)"
+
+
+def _asan_tga_driver(tmp_path):
+    """host_io.cpp's TGA reader built for the host with AddressSanitizer."""
+    import shutil
+    import subprocess
+    if shutil.which("g++") is None:
+        pytest.skip("g++ not available")
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    src = os.path.join(root, "tests", "tga_read_driver.cpp")
+    exe = str(tmp_path / "tga_read_asan")
+    cmd = ["g++", "-std=c++17", "-O1", "-g", "-fsanitize=address", "-fno-omit-frame-pointer",
+           "-D__HIP_PLATFORM_AMD__", "-I", os.path.join(root, "include"), "-I", "/opt/rocm/include",
+           "-I", os.path.join(root, "forkerrenderer_amd", "csrc"), src,
+           os.path.join(root, "forkerrenderer_amd", "csrc", "host_io.cpp"), "-o", exe]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        pytest.skip("host build with AddressSanitizer failed: " + r.stderr[-400:])
+    return exe
+
+
+def test_rle_decoder_rejects_packets_past_the_image(tmp_path):
+    """A literal packet and a repeat packet that run past the last pixel are
+    load errors, raised before any byte lands outside the image (checked
+    under AddressSanitizer); a well-formed stream still decodes."""
+    import struct
+    import subprocess
+    exe = _asan_tga_driver(tmp_path)
+    env = dict(os.environ, ASAN_OPTIONS="verify_asan_link_order=0:detect_leaks=0")
+
+    def tga(w, h, bpp, body):
+        hdr = struct.pack("<BBBHHBHHHHBB", 0, 0, 10 if bpp > 1 else 11, 0, 0, 0, 0, 0, w, h, bpp * 8, 0)
+        return hdr + bytes(body)
+
+    cases = {
+        "literal_over": (tga(2, 2, 3, [5] + [7] * 18), 1),      # 6 literal pixels into 4
+        "repeat_over": (tga(2, 2, 4, [0x80 | 9, 1, 2, 3, 4]), 1),  # 10 repeats into 4
+        "repeat_over_grey": (tga(3, 1, 1, [0x80 | 127, 9]), 1),    # 128 repeats into 3
+        "ok": (tga(2, 2, 3, [0x80 | 1, 1, 2, 3, 1, 4, 5, 6, 7, 8, 9]), 0),
+    }
+    for name, (blob, want_err) in cases.items():
+        p = tmp_path / f"{name}.tga"
+        p.write_bytes(blob)
+        r = subprocess.run([exe, str(p)], capture_output=True, text=True, env=env)
+        assert "AddressSanitizer" not in r.stderr, name + ": " + r.stderr[-600:]
+        if want_err:
+            assert r.returncode == 1 and "too many pixels read" in r.stdout, (name, r.stdout, r.stderr[-300:])
+        else:
+            assert r.returncode == 0, (name, r.stdout, r.stderr[-300:])
