@@ -26,6 +26,10 @@ MAX_LIGHTS, MAX_DISK = 4, 64
 GBUF_PLANES = 19
 
 
+# status codes (include/fgl.h)
+OK, ERR_INVALID, ERR_DEVICE, ERR_IO, ERR_NOMEM, ERR_CAPACITY = 0, -1, -2, -3, -4, -5
+
+
 class FglError(RuntimeError):
     pass
 

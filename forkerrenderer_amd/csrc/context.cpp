@@ -10,6 +10,7 @@
 #include <chrono>
 #include <string>
 #include <utility>
+#include <mutex>
 #include <vector>
 
 #include "../../include/fgl.h"
@@ -95,11 +96,42 @@ struct fgl_ctx {
     DevBuf rng, rng_saved, rng_samples, rng_offs, ssao_pow3;
     uint32_t rng_seed = 5489u;
 
+    // device address of the allocation-free entry point's scratch array
+    float *noalloc_scratch = nullptr;
+
     // timing
     bool timing = false;
     std::string timing_only;   // empty: every timed kernel
     std::vector<TimedKernel> timed;
 };
+
+// Scratch of fgl_ssao_noalloc: the unblurred AO that the separable blur
+// reads, FGL_SSAO_NOALLOC_MAX_PIXELS floats on every device the library runs
+// on, held until the process ends.  On the host `scratch` names a stand-in,
+// not device memory: host code only ever uses the address that
+// hipGetSymbolAddress returns for the current device (scratch_address).
+static __device__ __align__(256) float scratch[FGL_SSAO_NOALLOC_MAX_PIXELS];
+
+static constexpr int kMaxDevices = 64;
+static std::mutex g_scratch_mu;
+static float *g_scratch_ptr[kMaxDevices] = {};
+
+// Address of `scratch` on `device`, which must be the current device; looked
+// up once per device.
+static hipError_t scratch_address(int device, float **out)
+{
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (device < 0 || device >= kMaxDevices) return hipErrorInvalidDevice;
+    if (!g_scratch_ptr[device]) {
+        void *p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return e;
+        if (!p) return hipErrorInvalidSymbol;
+        g_scratch_ptr[device] = (float *)p;
+    }
+    *out = g_scratch_ptr[device];
+    return hipSuccess;
+}
 
 static int fail(fgl_ctx *c, int code, const std::string &msg)
 {
@@ -220,6 +252,10 @@ int fgl_ctx_create(int device, fgl_ctx **out)
     if (hipSetDevice(device) != hipSuccess) return FGL_ERR_DEVICE;
     fgl_ctx *c = new fgl_ctx();
     c->device = device;
+    if (scratch_address(device, &c->noalloc_scratch) != hipSuccess) {
+        delete c;
+        return FGL_ERR_DEVICE;
+    }
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
         delete c;
         return FGL_ERR_DEVICE;
@@ -1006,6 +1042,53 @@ int fgl_ssao(fgl_ctx *c, const fgl_ssao_cfg *cfg)
                                  cfg->blur_weights, cfg->blur_radius, c->stream);
         std::swap(c->ao, c->blur_tmp);
     }
+    HIPCHK(c, hipGetLastError());
+    return FGL_OK;
+}
+
+int fgl_ssao_noalloc(fgl_ctx *c, int w, int h, const float view[16], const float proj[16],
+                     const float *depth, const float *gbuf, float *ao, const fgl_ssao_cfg *cfg)
+{
+    if (!c) return FGL_ERR_INVALID;
+    if (!cfg || !view || !proj || !depth || !gbuf || !ao) return fail(c, FGL_ERR_INVALID, "null argument");
+    if (w <= 0 || h <= 0) return fail(c, FGL_ERR_INVALID, "bad frame size");
+    if ((long long)w * h > FGL_SSAO_NOALLOC_MAX_PIXELS)
+        return fail(c, FGL_ERR_CAPACITY, "frame larger than FGL_SSAO_NOALLOC_MAX_PIXELS (" +
+                                             std::to_string(FGL_SSAO_NOALLOC_MAX_PIXELS) + " pixels)");
+    if (cfg->stochastic)
+        return fail(c, FGL_ERR_INVALID, "the stochastic SSAO draws from the context's stream: use fgl_ssao");
+    if (cfg->n_kernel < 1 || cfg->n_kernel > 64 || cfg->blur < 0 || cfg->blur > 2 ||
+        (cfg->blur == 2 && (cfg->blur_radius < 1 || cfg->blur_radius > 8)))
+        return fail(c, FGL_ERR_INVALID, "bad SSAO configuration");
+    if (!c->noalloc_scratch) return fail(c, FGL_ERR_DEVICE, "no scratch address for this device");
+    FrameUniforms u;
+    std::memset(&u, 0, sizeof(u));
+    std::memcpy(u.view, view, sizeof(u.view));
+    std::memcpy(u.proj, proj, sizeof(u.proj));
+    mat_mul(proj, view, u.vp);
+    make_viewport(0, 0, w, h, u.viewport);
+    u.W = w;
+    u.H = h;
+    SsaoArgs sa;
+    sa.n_kernel = cfg->n_kernel;
+    std::memcpy(sa.kernel, cfg->kernel, sizeof(sa.kernel));
+    std::memcpy(sa.noise, cfg->noise, sizeof(sa.noise));
+    sa.radius = cfg->radius;
+    sa.bias = cfg->bias;
+    sa.range = cfg->range;
+    sa.kmax = 0.f;
+    for (int k = 0; k < 3 * sa.n_kernel; ++k) {
+        const float a = std::fabs(sa.kernel[k]);
+        if (!(a <= sa.kmax)) sa.kmax = a;
+    }
+    // The separable blur reads the unblurred AO from the scratch array; the
+    // SSAO kernel writes every pixel of it first, so nothing is cleared.
+    float *raw = cfg->blur == 2 ? c->noalloc_scratch : ao;
+    launch_ssao(u, sa, depth, gbuf, raw, c->stream);
+    if (cfg->blur == 1)
+        launch_blur_recursive(ao, w, h, c->stream);
+    else if (cfg->blur == 2)
+        launch_blur_separable(raw, ao, w, h, cfg->blur_weights, cfg->blur_radius, c->stream);
     HIPCHK(c, hipGetLastError());
     return FGL_OK;
 }

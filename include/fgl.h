@@ -192,6 +192,29 @@ int fgl_draw_screen_space_pixels(fgl_ctx *ctx);
 /* Render::DoSSAO (render.cpp:214-289) with a deterministic sample kernel, then the
  * configured blur (Buffer1f::TwoPassGaussianBlurDenoised, buffer.cpp:59-98, or separable) */
 int fgl_ssao(fgl_ctx *ctx, const fgl_ssao_cfg *cfg);
+/* Allocation-free variant of fgl_ssao.  No counterpart in the reference: it is
+ * for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  The inputs
+ * and the output are the caller's device buffers of one w x h frame:
+ *   depth  w*h floats (the geometry pass's depth, ForkerGL::DepthBuffer),
+ *   gbuf   the G-buffer planes laid out as fgl_read_target returns them
+ *          (plane p at gbuf + p*w*h); planes 0-5 (normal, world position)
+ *          are read,
+ *   ao     w*h floats, written.
+ * view / proj are the frame's row-major camera matrices; the viewport is
+ * (0, 0, w, h), as Render::Render sets it.  cfg as fgl_ssao; the stochastic
+ * mode is refused (it draws from the context's random stream).  The
+ * unblurred AO that the separable blur reads is kept in a static device
+ * array of FGL_SSAO_NOALLOC_MAX_PIXELS floats (32 MiB on every device the
+ * library runs on, held until the process ends); larger frames fail with
+ * FGL_ERR_CAPACITY.  Every call on a device shares that array: calls that
+ * overlap on one device (two contexts, two host threads) race.  Work is
+ * enqueued on the context's stream; nothing is allocated or freed, and the
+ * context's targets and pending draws are left as they are.  Results equal
+ * fgl_ssao's on the same frame bit for bit. */
+#define FGL_SSAO_NOALLOC_MAX_PIXELS (1 << 23)
+int fgl_ssao_noalloc(fgl_ctx *ctx, int w, int h, const float view[16], const float proj[16],
+                     const float *depth, const float *gbuf, float *ao, const fgl_ssao_cfg *cfg);
 /* The reference's random stream (src/utility.h:90-103: one function-static
  * std::mt19937, default seed 5489, through uniform_real_distribution<float>;
  * random vectors src/geometry.h:947-990).  The context owns one such stream on
