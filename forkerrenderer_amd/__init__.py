@@ -125,6 +125,10 @@ def lib():
             "fgl_check_recip": (C.c_int, [P, P, P, P]),
             "fgl_read_timings": (C.c_int, [P, P, P, C.c_int, C.POINTER(C.c_int)]),
             "fgl_scene_load": (C.c_int, [C.c_char_p, C.POINTER(P), C.c_char_p, C.c_size_t]),
+            "fgl_scene_load_with": (C.c_int, [C.c_char_p, C.c_int, C.c_int, C.POINTER(P), C.c_char_p, C.c_size_t]),
+            "fgl_scene_upload": (C.c_int, [P, P, P]),
+            "fgl_draw_model_normal": (C.c_int, [P, C.c_int, P, P]),
+            "fgl_make_normal_matrix": (None, [P, P]),
             "fgl_scene_free": (None, [P]),
             "fgl_scene_info_get": (C.c_int, [P, C.POINTER(SceneInfo)]),
             "fgl_scene_model": (C.c_int, [P, C.c_int, C.POINTER(ModelDesc), P]),
@@ -335,6 +339,11 @@ class Context:
     def begin_pass(self, p):
         self._chk(lib().fgl_begin_pass(self._h, p), "fgl_begin_pass")
 
+    def draw_model_normal(self, model_id, model_matrix, normal_matrix):
+        a = f32(model_matrix).reshape(16)
+        n = f32(normal_matrix).reshape(9)
+        self._chk(lib().fgl_draw_model_normal(self._h, model_id, _ptr(a), _ptr(n)), "fgl_draw_model_normal")
+
     def draw_model(self, model_id, model_matrix):
         a = f32(model_matrix).reshape(16)
         self._chk(lib().fgl_draw_model(self._h, model_id, _ptr(a)), "fgl_draw_model")
@@ -430,10 +439,14 @@ class Context:
 class Scene:
     """A .scene file loaded by the library's parsers (src/scene.cpp semantics)."""
 
-    def __init__(self, path):
+    def __init__(self, path, wrap=None, filt=None):
+        """wrap / filt: ForkerGL::TextureWrapping / TextureFiltering when the
+        scene is loaded (default: the reference's NoWrap, Nearest); a
+        `texture` key in the file overrides them."""
         self._h = C.c_void_p()
         err = C.create_string_buffer(1024)
-        rc = lib().fgl_scene_load(os.fsencode(path), C.byref(self._h), err, 1024)
+        rc = lib().fgl_scene_load_with(os.fsencode(path), WRAP_NONE if wrap is None else wrap,
+                                       FILTER_NEAREST if filt is None else filt, C.byref(self._h), err, 1024)
         if rc != FGL_OK:
             raise FglError(f"fgl_scene_load({path}) failed ({rc}): {err.value.decode()}")
         self.info = SceneInfo()

@@ -79,6 +79,8 @@ struct fgl_ctx {
     struct Rec {
         int model;
         float m[16];
+        bool has_normal;   // the shader's uNormalMatrix was given
+        float n[9];
     };
     std::vector<Rec> draws;
 
@@ -713,7 +715,18 @@ int fgl_draw_model(fgl_ctx *c, int model, const float model_matrix[16])
     fgl_ctx::Rec r;
     r.model = model;
     std::memcpy(r.m, model_matrix, sizeof(r.m));
+    r.has_normal = false;
     c->draws.push_back(r);
+    return FGL_OK;
+}
+
+int fgl_draw_model_normal(fgl_ctx *c, int model, const float model_matrix[16], const float normal_matrix[9])
+{
+    if (!c || !normal_matrix) return FGL_ERR_INVALID;
+    int rc = fgl_draw_model(c, model, model_matrix);
+    if (rc) return rc;
+    c->draws.back().has_normal = true;
+    std::memcpy(c->draws.back().n, normal_matrix, sizeof(c->draws.back().n));
     return FGL_OK;
 }
 
@@ -773,6 +786,7 @@ static int flush_pass(fgl_ctx *c)
         const ModelRes &m = c->models[c->draws[i].model];
         DevDraw &d = dd[i];
         fill_draw(c, m, c->draws[i].m, d);
+        if (c->draws[i].has_normal) std::memcpy(d.normal, c->draws[i].n, sizeof(d.normal));
         d.vert_base = (int)nv;
         d.tri_base = (int)nt;
         nt += m.n_faces;
@@ -1392,11 +1406,18 @@ struct fgl_scene {
 
 int fgl_scene_load(const char *path, fgl_scene **out, char *err, size_t err_len)
 {
-    if (!path || !out) return FGL_ERR_INVALID;
+    return fgl_scene_load_with(path, FGL_WRAP_NONE, FGL_FILTER_NEAREST, out, err, err_len);
+}
+
+int fgl_scene_load_with(const char *path, int wrap, int filter, fgl_scene **out, char *err, size_t err_len)
+{
+    if (!path || !out || wrap < FGL_WRAP_NONE || wrap > FGL_WRAP_CLAMP || filter < FGL_FILTER_NEAREST ||
+        filter > FGL_FILTER_LINEAR)
+        return FGL_ERR_INVALID;
     *out = nullptr;
     fgl_scene *sc = new fgl_scene();
     std::string e;
-    if (!load_scene(path, sc->s, e)) {
+    if (!load_scene(path, sc->s, e, wrap, filter)) {
         if (err && err_len) std::snprintf(err, err_len, "%s", e.c_str());
         delete sc;
         return FGL_ERR_IO;
@@ -1497,6 +1518,16 @@ static int upload_scene(fgl_ctx *c, fgl_scene *sc)
     return FGL_OK;
 }
 
+int fgl_scene_upload(fgl_ctx *c, fgl_scene *sc, int32_t *model_ids)
+{
+    if (!c || !sc) return FGL_ERR_INVALID;
+    int rc = upload_scene(c, sc);
+    if (rc) return rc;
+    if (model_ids)
+        for (size_t i = 0; i < sc->s.dev_models.size(); ++i) model_ids[i] = sc->s.dev_models[i];
+    return FGL_OK;
+}
+
 // Render::Render (src/render.cpp:40-58) with the scene's extension settings.
 int fgl_render_scene(fgl_ctx *c, fgl_scene *sc)
 {
@@ -1581,6 +1612,7 @@ void fgl_make_ortho(float l, float r, float b, float t, float n, float f, float 
     make_ortho(l, r, b, t, n, f, out);
 }
 void fgl_mat_mul(const float a[16], const float b[16], float out[16]) { mat_mul(a, b, out); }
+void fgl_make_normal_matrix(const float m[16], float out[9]) { make_normal_matrix(m, out); }
 
 int fgl_tga_write(const char *path, int w, int h, int bpp, const uint8_t *bytes, int rle)
 {

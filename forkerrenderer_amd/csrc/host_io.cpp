@@ -675,7 +675,7 @@ static int parse_wrap(const std::string &s)
     return FGL_WRAP_NONE;
 }
 
-bool load_scene(const std::string &path, HostScene &s, std::string &err)
+bool load_scene(const std::string &path, HostScene &s, std::string &err, int wrap, int filter)
 {
     std::ifstream in(path);
     if (in.fail()) { err = "Failed to open the .scene file: " + path; return false; }
@@ -699,6 +699,8 @@ bool load_scene(const std::string &path, HostScene &s, std::string &err)
     bool have_point = false, have_dir = false;
     fgl_light point{}, dir{};
     LoadState st;
+    st.wrap = wrap;       // ForkerGL::TextureWrapping / TextureFiltering when the scene is
+    st.filter = filter;   // loaded (src/forkergl.cpp:15-16, captured by Model::Load)
     std::map<std::string, std::shared_ptr<HostModel>> cache;
     std::string line;
     while (!in.eof()) {

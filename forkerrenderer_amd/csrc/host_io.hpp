@@ -38,7 +38,9 @@ struct HostScene {
 
 bool tga_read(const std::string &path, HostTexture &t, std::string &err);
 bool tga_write(const std::string &path, int w, int h, int bpp, const uint8_t *bytes, bool rle);
-bool load_scene(const std::string &path, HostScene &s, std::string &err);
+// wrap / filter: the texture modes textures take unless a `texture` key sets
+// them (the reference's ForkerGL statics at load time: NoWrap, Nearest).
+bool load_scene(const std::string &path, HostScene &s, std::string &err, int wrap = 0, int filter = 0);
 
 void make_model_matrix(float tx, float ty, float tz, float roty_deg, float scale, float *out);
 void make_lookat(const float *eye, const float *center, const float *up, float *out);

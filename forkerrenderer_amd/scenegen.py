@@ -313,14 +313,23 @@ def config4(d, seed=3):
 
 
 def tiny_scene(d, seed=0, width=64, height=48, mode="deferred", shadow="hard", ssao=False,
-               subdiv=1, tex=16, ssaa=1, pbr=True, extra=""):
-    """Small scene for parity tests: a displaced icosphere and a ground plane."""
+               subdiv=1, tex=16, ssaa=1, pbr=True, extra="", texture="repeat linear", shadowmap=128, uv_scale=1.0):
+    """Small scene for parity tests: a displaced icosphere and a ground plane.
+    texture: the `texture WRAP FILTER` key, or None for the reference's
+    load-time defaults; shadowmap: the `shadowmap S` key, or None for a
+    frame-sized shadow map (the reference's); uv_scale multiplies the
+    sphere's and the ground's texture coordinates (values outside [0, 1]
+    exercise the wrap modes)."""
     os.makedirs(d, exist_ok=True)
     v, n, uv, f = displaced_icosphere(subdiv, seed, 0.1, 0.8)
     rng = np.random.default_rng(seed)
     mat = "Pbr" if pbr else "Phong"
+    gp, gn, guv, gf = plane_group(2.0)
+    if uv_scale != 1.0:
+        uv = (uv - 0.25) * uv_scale
+        guv = (guv - 0.5) * uv_scale
     write_obj(os.path.join(d, "sphere.obj"), [("Sphere", mat, v, n, uv, f)], "m.mtl")
-    write_obj(os.path.join(d, "ground.obj"), [("Ground", mat, *plane_group(2.0))], "m.mtl")
+    write_obj(os.path.join(d, "ground.obj"), [("Ground", mat, gp, gn, guv, gf)], "m.mtl")
     phong = [("Ka", "0.2 0.2 0.2"), ("Kd", "0.7 0.6 0.5"), ("Ks", "0.4 0.4 0.4"), ("map_Kd", "albedo.tga"),
              ("map_Ks", "roughness.tga"), ("map_Bump", "normal.tga")]
     write_mtl(os.path.join(d, "m.mtl"), [("Pbr", PBR_MTL), ("Phong", phong)])
@@ -331,13 +340,13 @@ screen {width} {height}
 ssaa {'on' if ssaa > 1 else 'off'} {max(ssaa, 1)}
 ssao {'on' if ssao else 'off'}
 shadow on
-shadowmap 128
+{"" if shadowmap is None else f"shadowmap {shadowmap}"}
 shadowfilter {filt}
 ssaokernel 16 0.5 0.0005 0.01 {seed}
 aoblur gaussian 2
 light point 2 5 5 2 2 2
 camera persp 0.3 1.0 2.6 0 -0.2 0
-texture repeat linear
+{"" if texture is None else f"texture {texture}"}
 model {d}/ground.obj false true 0 -0.9 0 0 2
 model {d}/sphere.obj false true 0 0 0 {seed * 37 % 360} 1
 {extra}

@@ -186,6 +186,12 @@ int fgl_begin_pass(fgl_ctx *ctx, int pass);
  * submission order by fgl_end_pass (or by any read), all draws of a pass in one
  * batched launch sequence. */
 int fgl_draw_model(fgl_ctx *ctx, int model, const float model_matrix[16]);
+/* The same draw with the shader's uNormalMatrix given by the caller (row-major
+ * 3x3; src/shaders/gshader.h:26, phongshader.h:26, pbrshader.h:26, applied to
+ * normals and tangents in ProcessVertex, gshader.h:50-54).  fgl_draw_model
+ * uses MakeNormalMatrix(model_matrix) (src/geometry.cpp:60-68), the value
+ * Render::* sets (src/render.cpp:129,146,187). */
+int fgl_draw_model_normal(fgl_ctx *ctx, int model, const float model_matrix[16], const float normal_matrix[9]);
 int fgl_end_pass(fgl_ctx *ctx);
 /* ForkerGL::DrawScreenSpacePixels (forkergl.cpp:326-380): deferred lighting */
 int fgl_draw_screen_space_pixels(fgl_ctx *ctx);
@@ -287,6 +293,17 @@ typedef struct {
 } fgl_scene_info;
 
 int fgl_scene_load(const char *path, fgl_scene **out, char *err, size_t err_len);
+/* Scene(filename) with ForkerGL::TextureWrapping / TextureFiltering set to wrap
+ * / filter when the scene is loaded (src/forkergl.cpp:15-16, 44-52): every
+ * texture Model::Load reads takes those modes (src/model.cpp:425-426) unless
+ * a `texture` key of the scene sets others.  fgl_scene_load uses the
+ * reference's defaults, FGL_WRAP_NONE and FGL_FILTER_NEAREST. */
+int fgl_scene_load_with(const char *path, int wrap, int filter, fgl_scene **out, char *err, size_t err_len);
+/* Uploads the scene's textures, materials and models into the context (once
+ * per context; fgl_render_scene does it on first use); model_ids (n_models
+ * entries, may be NULL) receives the context's id of each of the scene's
+ * model lines. */
+int fgl_scene_upload(fgl_ctx *ctx, fgl_scene *s, int32_t *model_ids);
 void fgl_scene_free(fgl_scene *s);
 int fgl_scene_info_get(const fgl_scene *s, fgl_scene_info *out);
 /* Model i as uploaded (pointers stay valid until fgl_scene_free) and its model matrix */
@@ -384,6 +401,8 @@ void fgl_make_lookat(const float eye[3], const float center[3], const float up[3
 void fgl_make_perspective(float fov, float aspect, float n, float f, float out[16]);
 void fgl_make_ortho(float l, float r, float b, float t, float n, float f, float out[16]);
 void fgl_mat_mul(const float a[16], const float b[16], float out[16]);
+/* MakeNormalMatrix (src/geometry.cpp:60-68): upper 3x3 of the inverse transpose */
+void fgl_make_normal_matrix(const float m[16], float out[9]);
 
 /* ---- TGA codec (src/tgaimage.cpp:43-300) ---- */
 int fgl_tga_write(const char *path, int w, int h, int bpp, const uint8_t *bytes, int rle);

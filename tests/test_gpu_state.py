@@ -117,3 +117,36 @@ def test_eye_and_lights_change_between_forward_draws(ctx, scene_dir):
     assert_bitexact("depth", ctx.read(F.TARGET_DEPTH), want_d)
     mx, frac = lsb_stats(ctx.read(F.TARGET_IMAGE), want_img)
     assert frac >= 0.999, (mx, frac)
+
+
+def test_normal_matrix_uniform_reaches_the_vertex_stage(ctx, scene_dir):
+    """The shaders' uNormalMatrix (src/shaders/gshader.h:26, applied to
+    normals and tangents in ProcessVertex, gshader.h:50-54): a draw given
+    MakeNormalMatrix(M) equals fgl_draw_model's; a draw given another 3x3
+    transforms the G-buffer normals as the oracle does with that matrix."""
+    sc, models, mats, texs = _two_draw_scene(scene_dir, "state_nm", "deferred")
+    I = sc.info
+    st = O.frame_setup(I)
+    W, H = st["W"], st["H"]
+    ctx.render_scene(sc)
+    ids = ctx.upload_scene(sc)
+    want_plain = O.geometry_pass(O.OracleScene(models, mats, texs), st["fr"], W, H)
+    custom = [np.array([[0.9, 0.1, 0.0], [-0.2, 1.1, 0.05], [0.0, -0.1, 0.8]], np.float32),
+              O.make_normal_matrix(ids[1][1])]
+    osc = O.OracleScene(models, mats, texs)
+    assert osc.n_draws == 2
+    for k in range(2):
+        osc.draws[k].normal[:] = custom[k].reshape(9).tolist()
+    want_custom = O.geometry_pass(osc, st["fr"], W, H)
+    assert not np.array_equal(want_custom[1][0:3], want_plain[1][0:3])
+    for normals, want in ((None, want_plain), (custom, want_custom)):
+        ctx.init_targets(W, H)
+        ctx.begin_pass(F.PASS_GEOMETRY)
+        for k, (mid, M) in enumerate(ids):
+            if normals is None:
+                ctx.draw_model_normal(mid, M, O.make_normal_matrix(M))
+            else:
+                ctx.draw_model_normal(mid, M, normals[k])
+        ctx.end_pass()
+        assert_bitexact("depth", ctx.read(F.TARGET_DEPTH), want[0])
+        assert_bitexact("gbuf", ctx.read(F.TARGET_GBUFFER).reshape(F.GBUF_PLANES, -1), want[1])

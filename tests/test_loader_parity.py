@@ -233,3 +233,20 @@ def test_rle_decoder_rejects_packets_past_the_image(tmp_path):
             assert r.returncode == 1 and "too many pixels read" in r.stdout, (name, r.stdout, r.stderr[-300:])
         else:
             assert r.returncode == 0, (name, r.stdout, r.stderr[-300:])
+
+
+def test_texture_modes_are_captured_at_load(scene_dir):
+    """Without a `texture` key, textures take the modes ForkerGL holds when
+    the scene is loaded (src/forkergl.cpp:15-16, src/model.cpp:425-426):
+    NoWrap + Nearest by default, or the modes given to the loader; a
+    `texture` key overrides them."""
+    p = G.tiny_scene(os.path.join(scene_dir, "texdef"), seed=1, texture=None)
+    for args, want in (((), (F.WRAP_NONE, F.FILTER_NEAREST)),
+                       ((F.WRAP_MIRROR, F.FILTER_LINEAR), (F.WRAP_MIRROR, F.FILTER_LINEAR))):
+        sc = F.Scene(p, *args)
+        assert sc.info.n_textures > 0
+        assert all((sc.texture(i)["wrap"], sc.texture(i)["filter"]) == want for i in range(sc.info.n_textures))
+    p = G.tiny_scene(os.path.join(scene_dir, "texkey"), seed=1, texture="clamp linear")
+    sc = F.Scene(p, F.WRAP_REPEAT, F.FILTER_NEAREST)
+    assert all((sc.texture(i)["wrap"], sc.texture(i)["filter"]) == (F.WRAP_CLAMP, F.FILTER_LINEAR)
+               for i in range(sc.info.n_textures))

@@ -1,0 +1,650 @@
+"""Pins for the oracle functions that the GPU parity tests trust: each is
+checked against arithmetic that does not retype it -- a float64
+re-evaluation of the reference's equations, a closed form, or a property --
+on inputs built so that a plausible mistake (a dropped term, a wrong index,
+a transposed operand, a wrong constant, an off-by-one texel) fails.
+
+* main_vertex / frag_geometry (src/shaders/gshader.h:41-202): perspective-
+  correct world position, normal (with the normal matrix and the separate
+  normal index stream), light-space NDC, texture coordinates, the TBN normal
+  map, and window depth, against float64 under a perspective camera where w
+  varies ~3.5x across a triangle.
+* depth_vertex + the shadow value (depthshader.h:21-36).
+* orc_shadow_visibility (src/shaders/shadow.cpp:23-131): hard-shadow bias
+  and texel lookup, grid PCF and disk PCF on a step-edge map, PCSS on a flat
+  blocker, the 0.001 floor, off-map samples.
+* orc_lighting_pass (src/forkergl.cpp:326-380) as a composed pass: viewDir
+  in Blinn-Phong's halfway slot, param.x *= AO, two lights, only light 0
+  shadowed.
+* orc_forward_pass's Blinn-Phong inputs (phongshader.h:152-168): ao = Ka.x,
+  ks = Ks.x, shininess 1 without a specular map.
+* MirroredRepeat (texture.h:69-75) against the textbook mirror.
+* orc_ssao_reference (render.cpp:214-289): per-pixel occlusion counts
+  against a float64 evaluation of DoSSAO with the same random samples.
+* The sample sets the product generates for the extensions (Vogel disks,
+  the SSAO kernel and noise, the Gaussian weights).
+"""
+import ctypes as C
+import math
+
+import numpy as np
+import pytest
+
+import forkerrenderer_amd as F
+import oracle as O
+from forkerrenderer_amd import scenegen as G
+from tests.test_oracle_math import _fp64_pbr, _tex
+
+P_ = C.c_void_p
+I4 = np.eye(4, dtype=np.float32)
+
+
+def _mat(**kw):
+    m = O.Material()
+    for f in ("map_kd", "map_ks", "map_ke", "map_bump", "map_base", "map_rough", "map_metal", "map_ao",
+              "map_pbr_ke"):
+        setattr(m, f, -1)
+    for k, v in kw.items():
+        if isinstance(v, (list, tuple)):
+            getattr(m, k)[:] = v
+        else:
+            setattr(m, k, v)
+    return m
+
+
+def _unit(v):
+    v = np.asarray(v, np.float64)
+    return v / np.linalg.norm(v)
+
+
+# ---------------------------------------------------------------------------
+# vertex + fragment geometry
+
+
+W_G, H_G = 72, 56
+
+
+def _geometry_setup(normal_map):
+    """A quad receding from the camera (w from ~2.7 to ~9.7), with a model
+    matrix (rotation + translation + scale), separate uv / normal index
+    streams, unnormalised normals and tangents, and a light-space matrix."""
+    pos = np.array([(-1.0, 0.0, 1.0), (1.0, 0.0, 1.0), (1.0, 0.2, -4.5), (-1.0, 0.1, -4.5)], np.float32)
+    uv = np.array([(0.05, 0.07), (0.93, 0.11), (0.88, 0.95), (0.09, 0.9)], np.float32)
+    nrm = np.array([(0.1, 2.0, 0.3), (-0.2, 1.5, 0.1), (0.3, 1.0, -0.2), (0.0, 3.0, 0.5)], np.float32)
+    tan = np.array([(2.0, 0.1, 0.0), (1.5, -0.2, 0.1), (1.0, 0.0, 0.3), (3.0, 0.2, -0.1)], np.float32)
+    vi = np.array([(0, 1, 2), (0, 2, 3)], np.int32)
+    ti = np.array([(1, 2, 3), (1, 3, 0)], np.int32)    # uv index stream != position stream
+    ni = np.array([(3, 0, 1), (3, 1, 2)], np.int32)    # normal index stream != position stream
+    md = {"pos": pos, "uv": uv, "nrm": nrm, "tan": tan if normal_map else None, "vi": vi, "ti": ti, "ni": ni,
+          "face_material": np.zeros(2, np.int32), "support_pbr": True}
+    M = O.make_model_matrix((0.2, -0.4, -0.6), 25.0, 1.3)
+    # albedo texture: R = 4x, G = 4y (texel index encoded), nearest, no wrap
+    yy, xx = np.mgrid[0:64, 0:64]
+    alb = np.stack([4 * xx, 4 * yy, np.zeros_like(xx)], -1).astype(np.uint8)
+    rng = np.random.default_rng(12)
+    nmap = np.clip(np.round((_unit_rows(rng.normal([0, 0, 1], [0.35, 0.35, 0.0], (16, 16, 3))) + 1) / 2 * 255),
+                   0, 255).astype(np.uint8)
+    t_alb, k1 = _tex(alb, 0, 0)
+    t_nm, k2 = _tex(nmap, 0, 0)
+    mat = _mat(map_base=0, map_bump=1 if normal_map else -1, roughness=0.5, metalness=0.0)
+    V = O.make_lookat([0.4, 1.4, 2.2], [0.0, -0.4, -2.5])
+    Pm = O.make_perspective(45.0, np.float32(W_G) / np.float32(H_G), 0.01, 20.0)
+    Lv = O.make_lookat([2.0, 5.0, 5.0], [0, 0, 0])
+    Lp = O.make_ortho(np.float32(-3.0) * np.float32(W_G / H_G), np.float32(3.0) * np.float32(W_G / H_G),
+                      -3.0, 3.0, 0.1, 20.0)
+    L = O.mat_mul(Lp, Lv)
+    vp = O.make_viewport(0, 0, W_G, H_G)
+    fr = O.make_frame(V, Pm, L, vp, vp, [0.4, 1.4, 2.2], 1, None, [])
+    return dict(md=md, M=M, alb=alb, nmap=nmap, tex=[t_alb, t_nm], keep=[k1, k2], mat=mat, V=V, P=Pm, L=L,
+                vp=vp, fr=fr)
+
+
+def _unit_rows(a):
+    return a / np.linalg.norm(a, axis=-1, keepdims=True)
+
+
+def _fp64_frag(s, normal_map):
+    """float64 per-pixel G-buffer quantities of the quad, or None where a
+    pixel is covered by both triangles (the shared edge) or a vertex's window
+    coordinate is within 1e-3 of an integer (the float32 truncation could
+    round the other way)."""
+    md, M = s["md"], s["M"].astype(np.float64)
+    V, Pm, L, vp = (s[k].astype(np.float64) for k in ("V", "P", "L", "vp"))
+    Nmat = np.linalg.inv(M[:3, :3]).T
+    out = {}
+    tris = []
+    for f in range(2):
+        corners = []
+        for k in range(3):
+            p = np.append(md["pos"][md["vi"][f, k]].astype(np.float64), 1.0)
+            ws = M @ p
+            cs = Pm @ (V @ ws)
+            w = cs[3]
+            ndc = cs / w
+            win = vp @ ndc
+            assert min(abs(win[0] - round(win[0])), abs(win[1] - round(win[1]))) > 1e-3
+            ls = L @ ws
+            n = Nmat @ _unit(md["nrm"][md["ni"][f, k]])
+            t = Nmat @ _unit(md["tan"][md["vi"][f, k]]) if normal_map else None
+            corners.append(dict(X=math.trunc(win[0]), Y=math.trunc(win[1]), z=win[2], w=w, ws=ws[:3],
+                                uv=md["uv"][md["ti"][f, k]].astype(np.float64), n=n, t=t, l=ls[:3] / ls[3]))
+        tris.append(corners)
+    ws_range = [c["w"] for t in tris for c in t]
+    assert max(ws_range) / min(ws_range) > 3, "the test needs w to vary across the triangles"
+    for py in range(H_G):
+        for px in range(W_G):
+            hits = []
+            for f, cr in enumerate(tris):
+                (x0, y0), (x1, y1), (x2, y2) = [(c["X"], c["Y"]) for c in cr]
+                den = (x1 - x0) * (y2 - y0) - (x2 - x0) * (y1 - y0)
+                if abs(den) <= 1e-2:
+                    continue
+                b1 = ((px - x0) * (y2 - y0) - (x2 - x0) * (py - y0)) / den
+                b2 = ((x1 - x0) * (py - y0) - (px - x0) * (y1 - y0)) / den
+                b = np.array([1 - b1 - b2, b1, b2])
+                if np.all(b >= 0):
+                    hits.append((f, b))
+            if len(hits) != 1:
+                continue
+            f, b = hits[0]
+            cr = tris[f]
+            q = np.array([b[k] / cr[k]["w"] for k in range(3)])
+            pci = lambda key: sum(q[k] * cr[k][key] for k in range(3)) / q.sum()
+            Nn = _unit(pci("n"))
+            r = dict(pos=pci("ws"), uv=pci("uv"), lndc=pci("l"), N=Nn,
+                     depth=sum(b[k] * cr[k]["z"] for k in range(3)), b=b)
+            if normal_map:
+                T = _unit(pci("t") + 0.001)
+                T = _unit(T - np.dot(T, Nn) * Nn)
+                B = _unit(np.cross(Nn, T))
+                r["T"], r["B"] = T, B
+            out[(px, py)] = r
+    return out
+
+
+@pytest.mark.parametrize("normal_map", [False, True])
+def test_vertex_and_fragment_geometry_against_fp64(normal_map):
+    """Bounds (stated): world position, light-space NDC and the normal within
+    3e-5 absolute (values of size ~1..5); window depth within 2e-6; the
+    nearest albedo texel equals floor(u * (64 - 0.001)) of the float64
+    perspective-correct uv wherever that product is 1e-3 or more from an
+    integer.  Without perspective correction the world position is off by
+    up to 2.3 here; with the position index used for normals the normals
+    are off by up to 0.44 (the observed errors are below 1e-6)."""
+    s = _geometry_setup(normal_map)
+    sc = O.OracleScene([(s["md"], s["M"])], [s["mat"]], [])
+    tex = (O.Texture * 2)(*s["tex"])
+    sc.tex = tex
+    depth, g = O.geometry_pass(sc, s["fr"], W_G, H_G)
+    want = _fp64_frag(s, normal_map)
+    assert len(want) > 600
+    worst = {"pos": 0.0, "N": 0.0, "lndc": 0.0, "depth": 0.0, "normal": 0.0}
+    uv_checked = 0
+    for (px, py), r in want.items():
+        i = px + py * W_G
+        assert depth[i] < 3e38
+        worst["pos"] = max(worst["pos"], float(np.abs(g[3:6, i] - r["pos"]).max()))
+        worst["lndc"] = max(worst["lndc"], float(np.abs(g[6:9, i] - r["lndc"]).max()))
+        worst["depth"] = max(worst["depth"], abs(float(depth[i]) - r["depth"]))
+        tx = r["uv"] * (64 - 0.001)
+        if np.all(np.abs(tx - np.round(tx)) > 1e-3):
+            got = np.round(g[9:11, i] * 255 / 4).astype(int)
+            assert got[0] == math.floor(tx[0]) and got[1] == math.floor(tx[1]), ((px, py), got, tx)
+            uv_checked += 1
+            tn = r["uv"] * (16 - 0.001)       # the normal map is 16 x 16
+            if normal_map and np.all(np.abs(tn - np.round(tn)) > 1e-3):
+                s_px = s["nmap"][math.floor(tn[1]), math.floor(tn[0])].astype(np.float64) / 255
+                sn = _unit(s_px * 2 - 1)
+                TBN = np.stack([r["T"], r["B"], r["N"]], 1)     # columns T, B, N
+                n_want = _unit(TBN @ sn)
+                worst["normal"] = max(worst["normal"], float(np.abs(g[0:3, i] - n_want).max()))
+        if not normal_map:
+            worst["N"] = max(worst["N"], float(np.abs(g[0:3, i] - r["N"]).max()))
+    print("float64 re-evaluation, worst abs differences:", worst, "uv checked:", uv_checked)
+    assert uv_checked > 400
+    assert worst["pos"] < 3e-5 and worst["lndc"] < 3e-5 and worst["depth"] < 2e-6
+    assert worst["N"] < 3e-5 and worst["normal"] < 3e-5
+
+
+def test_shadow_pass_value_is_screen_space_ndc_z():
+    """ShadowBuffer = (vPositionNDC * bary).z * 0.5 + 0.5 (depthshader.h:30-36):
+    light-space NDC z interpolated with the screen-space weights (no
+    perspective correction), against float64 within 2e-6."""
+    s = _geometry_setup(False)
+    sc = O.OracleScene([(s["md"], s["M"])], [s["mat"]], [])
+    W, H = 64, 48
+    svp = O.make_viewport(0, 0, W, H)
+    fr = O.make_frame(I4, I4, s["L"], svp, svp, [0, 0, 0], 1, None, [])
+    _, shadow = O.shadow_pass(sc, fr, W, H)
+    LM = s["L"].astype(np.float64) @ s["M"].astype(np.float64)
+    md = s["md"]
+    checked = 0
+    worst = 0.0
+    for f in range(2):
+        ndc = [LM @ np.append(md["pos"][md["vi"][f, k]].astype(np.float64), 1.0) for k in range(3)]
+        ndc = [c / c[3] for c in ndc]
+        win = [svp.astype(np.float64) @ c for c in ndc]
+        X = [math.trunc(w[0]) for w in win]
+        Y = [math.trunc(w[1]) for w in win]
+        den = (X[1] - X[0]) * (Y[2] - Y[0]) - (X[2] - X[0]) * (Y[1] - Y[0])
+        for py in range(H):
+            for px in range(W):
+                b1 = ((px - X[0]) * (Y[2] - Y[0]) - (X[2] - X[0]) * (py - Y[0])) / den
+                b2 = ((X[1] - X[0]) * (py - Y[0]) - (px - X[0]) * (Y[1] - Y[0])) / den
+                b = np.array([1 - b1 - b2, b1, b2])
+                if np.all(b > 1e-9):
+                    want = sum(b[k] * ndc[k][2] for k in range(3)) * 0.5 + 0.5
+                    worst = max(worst, abs(float(shadow[px + py * W]) - want))
+                    checked += 1
+    assert checked > 300 and worst < 2e-6, (checked, worst)
+
+
+# ---------------------------------------------------------------------------
+# shadow filters
+
+
+class _SCfg:
+    def __init__(self, mode, grid=0, bd=(), pd=(), pcf_filter=0.0, bf=0.0, pf=0.0, area=0.0):
+        self.mode, self.grid = mode, grid
+        self.n_blocker, self.n_pcf = len(bd) // 2, len(pd) // 2
+        self.blocker_disk = list(bd) + [0.0] * (128 - len(bd))
+        self.pcf_disk = list(pd) + [0.0] * (128 - len(pd))
+        self.pcf_filter, self.blocker_filter, self.pcss_filter, self.area_light = pcf_filter, bf, pf, area
+
+
+SW = SH = 32
+
+
+def _vis(cfg, smap, ndc, n=(0, 0, 1), l=(0, 0, 1)):
+    vp = O.make_viewport(0, 0, SW, SH)
+    fr = O.make_frame(I4, I4, I4, vp, vp, [0, 0, 0], 1, cfg, [])
+    m = np.ascontiguousarray(smap, np.float32).reshape(-1)
+    a = [np.asarray(x, np.float32) for x in (ndc, n, l)]
+    return float(O.lib().orc_shadow_visibility(C.byref(fr), m.ctypes.data_as(P_), SW, SH,
+                                               *(x.ctypes.data_as(P_) for x in a)))
+
+
+def _step_map(left=0.3, right=0.8, edge=16):
+    m = np.full((SH, SW), right, np.float32)
+    m[:, :edge] = left
+    return m
+
+
+def _texel(u):
+    """SampleShadowMap's column: (int)((float)(int)(SW - 0.001) * u)."""
+    return math.floor((SW - 1) * u)
+
+
+def _ndc(u, v, z):
+    return (2 * u - 1, 2 * v - 1, 2 * z - 1)
+
+
+def test_hard_shadow_bias_and_lookup_closed_form():
+    """bias = max(0.009 (1 - N.L), 0.007); lit iff z <= s + bias; texel
+    column floor((SW - 1) u); outside [0, 1] the map reads +inf (lit); a
+    stored depth below 0.001 reads as 1."""
+    hard = _SCfg(0)
+    s0 = 0.6
+    flat = np.full((SH, SW), s0, np.float32)
+    for ndl in (1.0, 0.5, 0.25, 0.0, -0.6):
+        bias = max(0.009 * (1 - ndl), 0.007)
+        L = (math.sqrt(1 - ndl * ndl), 0.0, ndl)
+        for dz, lit in ((-2e-4, 1.0), (2e-4, 0.0)):
+            z = s0 + bias + dz
+            assert _vis(hard, flat, _ndc(0.5, 0.5, z), (0, 0, 1), L) == lit, (ndl, dz)
+    step = _step_map()
+    edge_u = 16 / (SW - 1)      # first u reading column 16 (a 32-wide lookup would put it at 0.5)
+    for u in np.linspace(0.3, 0.7, 81):
+        if abs(u - edge_u) < 1e-4:
+            continue
+        want = 1.0 if _texel(u) >= 16 else 0.0
+        assert _vis(hard, step, _ndc(u, 0.5, 0.5)) == want, u
+    assert _vis(hard, step, _ndc(0.505, 0.5, 0.5)) == 0.0     # column 15 under the (SW-1) rule
+    assert _vis(hard, step, _ndc(-0.05, 0.5, 0.5)) == 1.0     # off the map: +inf
+    assert _vis(hard, step, _ndc(0.1, 1.02, 0.5)) == 1.0
+    tiny = np.full((SH, SW), 0.0005, np.float32)              # below 0.001: reads 1
+    assert _vis(hard, tiny, _ndc(0.5, 0.5, 0.99)) == 1.0
+    assert _vis(hard, np.full((SH, SW), 0.002, np.float32), _ndc(0.5, 0.5, 0.99)) == 0.0
+
+
+def test_grid_pcf_on_a_step_edge():
+    """n x n texel grid: lit fraction = (rows) * #{dx : column of u + dx/SW is
+    right of the edge} / n^2, counted in float64."""
+    step = _step_map()
+    for n in (3, 5):
+        cfg = _SCfg(3, grid=n)
+        r = n // 2
+        for u in np.linspace(0.40, 0.62, 45):
+            cols = [(SW - 1) * (u + dx / SW) for dx in range(-r, r + 1)]
+            if any(abs(c - round(c)) < 1e-4 for c in cols):
+                continue
+            lit = sum(1 for c in cols if math.floor(c) >= 16)
+            want = n * lit / (n * n)
+            assert abs(_vis(cfg, step, _ndc(u, 0.5, 0.5)) - want) < 1e-6, (n, u)
+
+
+def _vogel(n):
+    g = 2.39996322972865332
+    return [x for i in range(n) for x in (math.sqrt((i + 0.5) / n) * math.cos(i * g),
+                                          math.sqrt((i + 0.5) / n) * math.sin(i * g))]
+
+
+def test_disk_pcf_on_a_step_edge():
+    step = _step_map()
+    disk = np.float32(_vogel(16))
+    filt = np.float32(0.05)
+    cfg = _SCfg(1, pd=disk.tolist(), pcf_filter=float(filt))
+    for u in np.linspace(0.42, 0.62, 41):
+        cols = [(SW - 1) * (u + float(disk[2 * i]) * float(filt)) for i in range(16)]
+        if any(abs(c - round(c)) < 1e-4 for c in cols):
+            continue
+        want = sum(1 for c in cols if math.floor(c) >= 16) / 16
+        assert abs(_vis(cfg, step, _ndc(u, 0.5, 0.5)) - want) < 1e-6, u
+
+
+def test_pcss_flat_blocker_penumbra_closed_form():
+    """Blocker at depth d_b left of the edge, receiver at z: every blocker
+    sample that lands left reads d_b, so the average blocker depth is d_b and
+    the PCF radius is pcss_filter * (z - d_b) * area / d_b; visibility is the
+    fraction of PCF taps at that radius landing right of the edge.  No
+    blocker in reach: visibility 1.  The parameters make the usual wrong
+    forms (d_b / (z - d_b), no pcss_filter factor) give other counts."""
+    d_b, z, area = 0.3, 0.5, 4.0
+    bd, pd = _vogel(16), _vogel(32)
+    bf, pf = 0.08, 0.05
+    cfg = _SCfg(2, bd=np.float32(bd).tolist(), pd=np.float32(pd).tolist(), bf=bf, pf=pf, area=area)
+    step = _step_map(left=d_b)
+    pen = (z - d_b) * area / d_b
+    radius = pf * pen
+    alt_radii = [pf * d_b * area / (z - d_b), pen]
+    pdf = np.float32(pd).astype(np.float64)
+    checked = 0
+    for u in np.linspace(0.45, 0.6, 31):
+        bcols = [(SW - 1) * (u + float(np.float32(bd[2 * i])) * bf) for i in range(16)]
+        pcols = [(SW - 1) * (u + pdf[2 * i] * radius) for i in range(32)]
+        if any(abs(c - round(c)) < 1e-3 for c in bcols + pcols):
+            continue
+        if not any(math.floor(c) < 16 for c in bcols):
+            continue   # no blocker found: covered below
+        want = sum(1 for c in pcols if math.floor(c) >= 16) / 32
+        got = _vis(cfg, step, _ndc(u, 0.5, z))
+        assert abs(got - want) < 1e-6, (u, got, want)
+        for r_alt in alt_radii:
+            alt = sum(1 for i in range(32) if math.floor((SW - 1) * (u + pdf[2 * i] * r_alt)) >= 16) / 32
+            checked += alt != want
+    assert checked > 5, "the cases do not separate the penumbra formula from its usual mistakes"
+    assert _vis(cfg, step, _ndc(0.9, 0.5, z)) == 1.0
+
+
+# ---------------------------------------------------------------------------
+# lighting as composed passes
+
+
+def _fp64_bp_term(L, H, N, dl, ao, ks, shin):
+    d = max(0.0, float(np.dot(L, N)))
+    sp = max(0.0, float(np.dot(H, N))) ** shin
+    return dl * d * ao, ks * sp * np.ones(3)
+
+
+def _tonemap(c):
+    return np.clip(np.power(c / (c + 1), 1 / 2.2), 0, 1)
+
+
+def _fp64_deferred_pixel(gp, ao, eye, lights, shadow, vis0):
+    """DrawScreenSpacePixels for one pixel in float64 (forkergl.cpp:326-380),
+    with the lights summed before ambient, emissive (light 0) and tonemap."""
+    N, Pw, al, em, pa, typ = gp["N"], gp["P"], gp["al"], gp["em"], gp["pa"].copy(), gp["type"]
+    pa[0] *= ao
+    Vd = _unit(np.asarray(eye) - Pw)
+    fac = lambda i: (1 - (1 - vis0) * 0.6) if (shadow and i == 0) else 1.0
+    if typ < 0.5:
+        dl, el = np.power(al, 2.2), np.power(em, 2.2)
+        c = 0.3 * dl * pa[0]
+        for i, (t, vec, col) in enumerate(lights):
+            L = _unit(np.asarray(vec) - Pw) if t == 0 else _unit(-np.asarray(vec))
+            dif, spe = _fp64_bp_term(L, Vd, N, dl, pa[0], pa[1], pa[2])   # viewDir in the halfway slot
+            if shadow:
+                dif, spe = dif * fac(i), spe * fac(i)
+            term = dif + spe + (el if i == 0 else 0.0)
+            c = c + term * np.asarray(col)
+        return _tonemap(c)
+    alp, el = np.power(al, 2.2), np.power(em, 2.2)
+    Lo = np.zeros(3)
+    for i, (t, vec, col) in enumerate(lights):
+        L = _unit(np.asarray(vec) - Pw) if t == 0 else _unit(-np.asarray(vec))
+        H = _unit(L + Vd)
+        nv, nl = max(float(np.dot(N, Vd)), 0), max(float(np.dot(N, L)), 0)
+        nh, hv = max(float(np.dot(N, H)), 0), max(float(np.dot(H, Vd)), 0)
+        m, r = pa[1], pa[2]
+        F0 = 0.04 * (1 - m) + alp * m
+        a2 = (r * r) ** 2
+        D = a2 / math.pi / (nh * nh * (a2 - 1) + 1) ** 2
+        k = (r + 1) ** 2 / 8
+        Gs = (nv / (nv * (1 - k) + k)) * (nl / (nl * (1 - k) + k))
+        Fr = F0 + (1 - F0) * max(1 - hv, 0) ** 5
+        spec = Fr * D * Gs / (4 * nv * nl + 0.001)
+        kd = (1 - Fr) * (1 - m)
+        lo = (kd * alp / math.pi + spec) * np.asarray(col) * nl
+        Lo = Lo + lo * fac(i)
+    return _tonemap(Lo + 0.3 * alp * pa[0] + el)
+
+
+@pytest.mark.parametrize("shadow", [False, True])
+def test_deferred_lighting_pass_composition(shadow):
+    """orc_lighting_pass over a G-buffer of random pixels (Blinn-Phong and PBR
+    shading types, per-pixel AO, a point and a directional light) against
+    the float64 pass.  With shadows on, the map is constant 0.5 and each
+    pixel's light-space depth puts it clearly lit or clearly shadowed.
+    Bound: 2e-5 absolute.  Blinn-Phong with the true halfway vector, AO not
+    multiplied into param.x, or the directional light shadowed, fail."""
+    rng = np.random.default_rng(21)
+    W, H = 16, 8
+    n = W * H
+    eye = [0.3, 1.2, 2.5]
+    lights = [(0, [2.0, 5.0, 5.0], [2.0, 1.8, 1.5]), (1, [-0.4, -1.0, -0.3], [0.6, 0.6, 0.7])]
+    g = np.zeros((19, n), np.float32)
+    Ns = _unit_rows(rng.normal([0, 1, 0.3], 0.5, (n, 3)))
+    g[0:3] = Ns.T
+    g[3:6] = rng.uniform(-1, 1, (3, n))
+    lz = np.where(rng.random(n) < 0.5, 0.2, 0.8)      # window depth 0.6 / 0.9 against the 0.5 map
+    g[6:9] = np.stack([rng.uniform(-0.8, 0.8, n), rng.uniform(-0.8, 0.8, n), lz])
+    g[9:12] = rng.uniform(0.05, 1, (3, n))
+    g[12:15] = rng.uniform(0, 0.1, (3, n))
+    g[15] = rng.uniform(0.4, 1, n)
+    typ = (rng.random(n) < 0.5).astype(np.float32)
+    g[16] = np.where(typ > 0.5, rng.uniform(0, 1, n), rng.uniform(0.1, 0.9, n))
+    g[17] = np.where(typ > 0.5, rng.uniform(0.05, 1, n), rng.uniform(5, 40, n))
+    g[18] = typ
+    ao = rng.uniform(0.2, 1, n).astype(np.float32)
+    vp = O.make_viewport(0, 0, W, H)
+    cfg = _SCfg(0)
+    fr = O.make_frame(I4, I4, I4, vp, vp, eye, int(shadow), cfg, lights)
+    smap = np.full(SW * SH, 0.5, np.float32)
+    fb = O.lighting_pass(fr, W, H, g, ao, smap, SW, SH)
+    worst = 0.0
+    for i in range(n):
+        gp = dict(N=g[0:3, i].astype(np.float64), P=g[3:6, i].astype(np.float64),
+                  al=g[9:12, i].astype(np.float64), em=g[12:15, i].astype(np.float64),
+                  pa=g[15:18, i].astype(np.float64), type=float(g[18, i]))
+        z = 0.5 * float(g[8, i]) + 0.5
+        vis0 = 1.0 if z <= 0.5 + 0.009 else 0.0     # bias <= 0.018 separates 0.6 / 0.9 from 0.5
+        want = _fp64_deferred_pixel(gp, float(ao[i]), eye, lights, shadow, vis0)
+        worst = max(worst, float(np.abs(fb[:, i] - want).max()))
+    assert worst < 2e-5, worst
+
+
+def test_forward_blinn_phong_uses_ka_x_as_ao():
+    """The forward Blinn-Phong fragment's param is (Ka.x, Ks.x, shininess 1
+    without a specular map) (phongshader.h:152-168): Ka.x scales ambient and
+    diffuse, Ka.y / Ka.z are unused, the halfway vector is normalize(L + V).
+    Flat quad facing the camera, shadows off, float64 within 2e-5."""
+    quad = np.array([(-2, -2, -3), (2, -2, -3), (2, 2, -3), (-2, 2, -3)], np.float32)
+    faces = np.array([(0, 1, 2), (0, 2, 3)], np.int32)
+    md = {"pos": quad, "uv": np.zeros((4, 2), np.float32), "nrm": np.tile([[0, 0, 1]], (4, 1)).astype(np.float32),
+          "tan": None, "vi": faces, "ti": faces, "ni": faces, "face_material": np.zeros(2, np.int32),
+          "support_pbr": False}
+    W, H = 24, 18
+    eye = [0.3, 0.2, 0.5]
+    V = O.make_lookat(eye, [0.0, 0.0, -3.0])
+    Pm = O.make_perspective(45.0, np.float32(W) / np.float32(H), 0.01, 20.0)
+    vp = O.make_viewport(0, 0, W, H)
+    light = (0, [1.0, 1.5, 1.0], [1.5, 1.2, 1.0])
+    kd, ka, ks = [0.6, 0.5, 0.4], [0.5, 0.9, 0.1], [0.35, 0.8, 0.8]
+    fr = O.make_frame(V, Pm, I4, vp, vp, eye, 0, None, [light])
+    sc = O.OracleScene([(md, I4)], [_mat(kd=kd, ka=ka, ks=ks)], [])
+    depth, fb = O.forward_pass(sc, fr, W, H, None, 1, 1)
+    _, g = O.geometry_pass(O.OracleScene([(md, I4)], [_mat(kd=kd, ka=ka, ks=ks)], []), fr, W, H)
+    cov = depth < 3e38
+    assert cov.sum() > 300
+    worst = 0.0
+    dl = np.power(np.float32(kd).astype(np.float64), 2.2)
+    for i in np.nonzero(cov)[0]:
+        Pw = g[3:6, i].astype(np.float64)
+        N = g[0:3, i].astype(np.float64)
+        L = _unit(np.asarray(light[1]) - Pw)
+        Hh = _unit(L + _unit(np.asarray(eye) - Pw))
+        dif, spe = _fp64_bp_term(L, Hh, N, dl, float(np.float32(ka[0])), float(np.float32(ks[0])), 1.0)
+        c = 0.3 * dl * float(np.float32(ka[0])) + (dif + spe) * np.asarray(light[2])
+        worst = max(worst, float(np.abs(fb[:, i] - _tonemap(c)).max()))
+    assert worst < 2e-5, worst
+
+
+# ---------------------------------------------------------------------------
+# textures
+
+
+def test_mirrored_repeat_against_the_textbook_mirror():
+    """MirroredRepeat: the coordinate reflected at every integer, m(c) = c mod
+    2 for c mod 2 <= 1, else 2 - (c mod 2); nearest texel floor(m * (W -
+    0.001)), negative coordinates included (C++ % keeps the sign)."""
+    W, H = 7, 5
+    yy, xx = np.mgrid[0:H, 0:W]
+    img = np.stack([30 * xx, 40 * yy, np.full_like(xx, 9)], -1).astype(np.uint8)
+    t, keep = _tex(img, 2, 0)
+    rng = np.random.default_rng(22)
+    checked = 0
+    for _ in range(2000):
+        u, v = np.float32(rng.uniform(-3.3, 3.3, 2))
+        mu = [math.fmod(float(c), 2.0) % 2.0 for c in (u, v)]
+        mu = [m if m <= 1 else 2 - m for m in mu]
+        px, py = mu[0] * (W - 0.001), mu[1] * (H - 0.001)
+        if min(abs(px - round(px)), abs(py - round(py))) < 1e-4 or min(abs(float(u) - round(float(u))),
+                                                                       abs(float(v) - round(float(v)))) < 1e-5:
+            continue
+        out = np.zeros(3, np.float32)
+        O.lib().orc_texture_sample(t, u, v, out.ctypes.data_as(P_))
+        got = np.round(out * 255).astype(int)
+        assert got[0] == 30 * math.floor(px) and got[1] == 40 * math.floor(py), (u, v, got, px, py)
+        checked += 1
+    assert checked > 1500
+
+
+# ---------------------------------------------------------------------------
+# the reference's stochastic SSAO
+
+
+def test_ssao_reference_counts_against_fp64_dossao():
+    """orc_ssao_reference (DoSSAO, render.cpp:214-289) per pixel: 32 samples
+    drawn in pixel row order from the stream, scaled by Lerp(0.1, 1, |d|^2),
+    radius 0.075, projected, depth compared with bias 0.0005 and range 0.01,
+    AO = (1 - k/32)^3.  The same samples are replayed from the (separately
+    pinned) stream and every occlusion decision re-evaluated in float64;
+    pixels with a sample within 1e-4 of a threshold or a texel edge are not
+    compared.  Scene: a sphere on a ground plane seen from close, so the
+    crease and the sphere's lower half occlude."""
+    d = G.tiny_scene(__import__("tempfile").mkdtemp(prefix="pin_ssao_"), seed=7, width=40, height=30, subdiv=3,
+                     tex=16, ssao=True)
+    txt = open(d).read().replace("camera persp 0.3 1.0 2.6 0 -0.2 0", "camera persp 0.5 -0.35 1.1 0 -0.8 0")
+    open(d, "w").write(txt)
+    sc = F.Scene(d)
+    I = sc.info
+    models = [sc.model(i) for i in range(I.n_models)]
+    mats = [sc.material(i) for i in range(I.n_materials)]
+    texs = [sc.texture(i) for i in range(I.n_textures)]
+    st = O.frame_setup(I)
+    W, H, fr = st["W"], st["H"], st["fr"]
+    depth, g = O.geometry_pass(O.OracleScene(models, mats, texs), fr, W, H)
+    O.rng_seed(5489)
+    ao = O.ssao_reference(fr, W, H, depth, g)
+    # replay the stream
+    O.rng_seed(5489)
+    VP = (st["proj"].astype(np.float64) @ st["view"].astype(np.float64))
+    vp = st["viewport"].astype(np.float64)
+    compared = occluded = 0
+    for y in range(H):
+        for x in range(W):
+            i = x + y * W
+            nn = np.ascontiguousarray(g[0:3, i], np.float32)
+            P = g[3:6, i].astype(np.float64)
+            k = 0
+            ambiguous = False
+            for _ in range(32):
+                dd = np.zeros(3, np.float32)
+                O.lib().orc_rng_hemisphere(nn.ctypes.data_as(P_), dd.ctypes.data_as(P_))
+                dv = dd.astype(np.float64)
+                scale = 0.9 + 0.1 * float(np.dot(dv, dv))
+                S = P + dv * scale * 0.075
+                cs = VP @ np.append(S, 1.0)
+                ss = vp @ (cs / cs[3])
+                if min(abs(ss[0] - round(ss[0])), abs(ss[1] - round(ss[1]))) < 1e-4:
+                    ambiguous = True
+                    continue
+                sx, sy = math.trunc(ss[0]), math.trunc(ss[1])
+                j = sx + sy * W
+                if not (0 <= j < W * H):
+                    continue
+                cached = float(depth[j])
+                t1 = ss[2] - (cached + 0.0005)
+                t2 = abs(float(depth[i]) - cached) - 0.01
+                if min(abs(t1), abs(t2)) < 1e-6:
+                    ambiguous = True
+                    continue
+                if t1 >= 0 and t2 < 0:
+                    k += 1
+            if ambiguous or depth[i] > 3e38:
+                continue
+            want = (1 - k / 32) ** 3
+            assert abs(float(ao[i]) - want) < 1e-6, ((x, y), float(ao[i]), k)
+            compared += 1
+            occluded += k > 0
+    print("DoSSAO pixels compared:", compared, "with occlusion:", occluded)
+    assert compared > 500 and occluded > 30
+
+
+# ---------------------------------------------------------------------------
+# sample sets generated for the extensions
+
+
+def test_extension_sample_sets(scene_dir):
+    """Vogel disks: point i at radius sqrt((i + 0.5) / n) and angle i times
+    the golden angle; equal-area rings.  SSAO kernel: points in the unit
+    hemisphere z >= 0 with length at most Lerp(0.1, 1, (i/n)^2); noise
+    vectors in the tangent plane (z = 0) inside [-1, 1]^2.  Gaussian blur:
+    weights exp(-i^2 / 2 sigma^2) normalised over the 2R + 1 taps, R = 3 sigma."""
+    import os
+    p = G.tiny_scene(os.path.join(scene_dir, "sets"), seed=3, ssao=True,
+                     extra="shadowfilter pcss 16 32 0.05\nssaokernel 64 0.5 0.0005 0.01 9\naoblur gaussian 2")
+    I = F.Scene(p).info
+    golden = math.pi * (3 - math.sqrt(5))
+    for n, disk in ((I.shadow.n_blocker, I.shadow.blocker_disk), (I.shadow.n_pcf, I.shadow.pcf_disk)):
+        assert n in (16, 32)
+        pts = np.array(list(disk)[:2 * n], np.float64).reshape(n, 2)
+        r2 = (pts ** 2).sum(1)
+        assert np.allclose(r2, (np.arange(n) + 0.5) / n, atol=1e-6)
+        ang = np.arctan2(pts[:, 1], pts[:, 0])
+        dang = np.angle(np.exp(1j * (ang[1:] - ang[:-1] - golden)))
+        assert np.abs(dang).max() < 1e-5
+        assert np.linalg.norm(pts.mean(0)) < 0.1
+    s = I.ssao
+    k = np.array(list(s.kernel)[:3 * s.n_kernel], np.float64).reshape(s.n_kernel, 3)
+    assert s.n_kernel == 64 and np.all(k[:, 2] >= 0)
+    lim = 0.1 + 0.9 * (np.arange(64) / 64) ** 2
+    ln = np.linalg.norm(k, axis=1)
+    assert np.all(ln <= lim * (1 + 1e-6)) and ln.max() > 0.5 and ln[:8].max() < 0.11
+    assert 0.3 < np.mean(k[:, 2] / ln) < 0.8          # directions spread over the hemisphere
+    nz = np.array(list(s.noise), np.float64).reshape(16, 3)
+    assert np.all(nz[:, 2] == 0) and np.all(np.abs(nz[:, :2]) <= 1) and np.linalg.norm(nz[:, :2], axis=1).min() > 0
+    assert s.blur == 2 and s.blur_radius == 6
+    w = np.array(list(s.blur_weights)[:7], np.float64)
+    assert abs(w[0] + 2 * w[1:].sum() - 1) < 1e-6
+    assert np.allclose(w / w[0], np.exp(-np.arange(7) ** 2 / 8.0), rtol=1e-6)
