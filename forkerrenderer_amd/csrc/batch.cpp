@@ -130,7 +130,10 @@ struct fgl_batch {
     bool have_shadow = false;
 
     // device
-    Buf d_views, d_draws[2], d_items[2], d_blk_view, d_pix_views, d_conv_items[4], d_vblocks[2], d_fblocks[2];
+    Buf d_views, d_draws[2], d_items[2], d_blk_view, d_lblk_view, d_pix_views, d_conv_items[4], d_vblocks[2],
+        d_fblocks[2];
+    size_t total_lpx = 0;      // network pixels of all views (neural AO)
+    uint32_t n_lblocks = 0;    // 64 x 4 network-pixel blocks of all views
     int n_vblocks[2] = {0, 0}, n_fblocks[2] = {0, 0};
     uint32_t n_conv_items[4] = {0, 0, 0, 0};
     Buf depth, vis, gbuf, ao, fb, img, shadow, shadow_depth;
@@ -161,8 +164,11 @@ static int plan(fgl_batch *b)
     b->hv.assign(b->n, DevView());
     std::vector<DevDraw> dd[2];
     std::vector<uint32_t> items[2];
-    std::vector<uint8_t> blk_view;
+    std::vector<uint8_t> blk_view, lblk_view;
     std::vector<BatchPixView> pix(b->n);
+    const int nscale = b->ao_mode == FGL_AO_NEURAL ? nao_scale(b->nao) : 1;
+    size_t lpx = 0;
+    uint32_t lblocks = 0;
     std::vector<uint2> conv_items[4];
     size_t px = 0, spx = 0;
     uint32_t tiles[2] = {0, 0}, blocks = 0;
@@ -203,11 +209,20 @@ static int plan(fgl_batch *b)
         blocks += nb;
         pix[i].W = v.width;
         pix[i].H = v.height;
+        pix[i].lW = (v.width + nscale - 1) / nscale;
+        pix[i].lH = (v.height + nscale - 1) / nscale;
         pix[i].blk_base = dv.blk_base;
-        pix[i].pad = 0;
+        pix[i].lblk_base = lblocks;
         pix[i].px_off = dv.px_off;
+        pix[i].lo_off = lpx;
         std::memcpy(pix[i].view, dv.u.view, sizeof(pix[i].view));
-        if (b->ao_mode == FGL_AO_NEURAL) nao_conv_items(i, v.width, v.height, conv_items);
+        {
+            const uint32_t nl = (uint32_t)(((pix[i].lW + 63) / 64) * ((pix[i].lH + 3) / 4));
+            lblk_view.insert(lblk_view.end(), nl, (uint8_t)i);
+            lblocks += nl;
+            lpx += (size_t)pix[i].lW * pix[i].lH;
+        }
+        if (b->ao_mode == FGL_AO_NEURAL) nao_conv_items(i, pix[i].lW, pix[i].lH, conv_items);
         // draw records: light * model with this view's light space
         for (int k = 0; k < v.n_draws; ++k) {
             DevDraw d;
@@ -238,10 +253,13 @@ static int plan(fgl_batch *b)
     b->total_px = px;
     b->total_spx = spx;
     b->n_blocks = blocks;
+    b->total_lpx = lpx;
+    b->n_lblocks = lblocks;
     if ((rc = ctx_tables(c, &b->materials, &b->textures))) return rc;
     const long long mt = std::max(b->nt[0], b->nt[1]), mv = std::max(b->nv[0], b->nv[1]);
     const uint32_t mtiles = std::max(tiles[0], tiles[1]);
     if (grow(b->d_views, b->n * sizeof(DevView)) || grow(b->d_blk_view, blk_view.size() + 1) ||
+        grow(b->d_lblk_view, lblk_view.size() + 1) ||
         grow(b->d_pix_views, b->n * sizeof(BatchPixView)) || grow(b->depth, px * 4) || grow(b->vis, px * 4) ||
         grow(b->gbuf, (size_t)kGbufPlanes * px * 4) || grow(b->ao, px * 4) || grow(b->fb, 3 * px * 4) ||
         grow(b->img, 3 * px) || grow(b->shadow, spx * 4) || grow(b->shadow_depth, spx * 4) ||
@@ -277,10 +295,12 @@ static int plan(fgl_batch *b)
     }
     if (hipMemcpy(b->d_views.p, b->hv.data(), b->n * sizeof(DevView), hipMemcpyHostToDevice) ||
         (!blk_view.empty() && hipMemcpy(b->d_blk_view.p, blk_view.data(), blk_view.size(), hipMemcpyHostToDevice)) ||
+        (!lblk_view.empty() &&
+         hipMemcpy(b->d_lblk_view.p, lblk_view.data(), lblk_view.size(), hipMemcpyHostToDevice)) ||
         hipMemcpy(b->d_pix_views.p, pix.data(), b->n * sizeof(BatchPixView), hipMemcpyHostToDevice))
         return bfail(b, FGL_ERR_DEVICE, "upload failed");
     if (b->ao_mode == FGL_AO_NEURAL) {
-        if ((rc = nao_reserve(b->nao, px))) return bfail(b, rc, "neural AO buffers");
+        if ((rc = nao_reserve(b->nao, px, lpx))) return bfail(b, rc, "neural AO buffers");
         for (int l = 0; l < 4; ++l) {
             b->n_conv_items[l] = (uint32_t)conv_items[l].size();
             if (grow(b->d_conv_items[l], (conv_items[l].size() + 1) * sizeof(uint2)))
@@ -434,11 +454,15 @@ static int enqueue(fgl_batch *b, bool measure, bool timed)
         nt.views = (const BatchPixView *)b->d_pix_views.p;
         nt.blk_view = pa.blk_view;
         nt.n_blocks = b->n_blocks;
+        nt.lblk_view = (const uint8_t *)b->d_lblk_view.p;
+        nt.n_lblocks = b->n_lblocks;
+        nt.total_px = b->total_px;
+        nt.total_lpx = b->total_lpx;
         for (int l = 0; l < 4; ++l) {
             nt.items[l] = (const uint2 *)b->d_conv_items[l].p;
             nt.n_items[l] = b->n_conv_items[l];
         }
-        if ((rc = nao_infer_batch(b->nao, s, nt, b->total_px, (const float *)b->depth.p, pa.gbuf, pa.ao)))
+        if ((rc = nao_infer_batch(b->nao, s, nt, (const float *)b->depth.p, pa.gbuf, pa.ao)))
             return bfail(b, rc, "batched neural AO failed");
     }
     {
@@ -561,6 +585,7 @@ void fgl_batch_destroy(fgl_batch *b)
         release(b->d_fblocks[p]);
     }
     Buf *bufs[] = {&b->d_views, &b->d_draws[0], &b->d_draws[1], &b->d_items[0], &b->d_items[1], &b->d_blk_view,
+                   &b->d_lblk_view,
                    &b->d_pix_views, &b->depth, &b->vis, &b->gbuf, &b->ao, &b->fb, &b->img, &b->shadow,
                    &b->shadow_depth, &b->screen, &b->ndc_z, &b->rect, &b->counts, &b->offs, &b->scan_tmp, &b->tri_v,
                    &b->big_list, &b->tile_start, &b->tile_end, &b->counters};
@@ -633,7 +658,7 @@ static int refresh_resources(fgl_batch *b)
         drop_graph(b);
     }
     if (b->ao_mode == FGL_AO_NEURAL) {
-        if ((rc = nao_reserve(b->nao, b->total_px))) return bfail(b, rc, "neural AO buffers");
+        if ((rc = nao_reserve(b->nao, b->total_px, b->total_lpx))) return bfail(b, rc, "neural AO buffers");
         const uint64_t g = nao_buffers_generation(b->nao);
         if (g != b->nao_gen) {
             b->nao_gen = g;

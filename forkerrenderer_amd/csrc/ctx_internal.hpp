@@ -43,35 +43,45 @@ hipStream_t ctx_stream(fgl_ctx *c);
 bool ctx_timing(fgl_ctx *c);
 
 // Per-view pixel layout of a batch of views, for the batched per-pixel
-// kernels outside the batch engine (neural AO): the view's targets start at
-// pixel px_off of the batch buffers; its 64 x 4 pixel blocks are blocks
-// blk_base .. of the batch-wide block index.
+// kernels outside the batch engine (neural AO): the view's frame targets
+// start at pixel px_off of the batch buffers and its 64 x 4 pixel blocks are
+// blocks blk_base.. of the batch-wide frame block table; its lW x lH network
+// grid (the frame divided by the network's scale, rounded up) starts at
+// lo_off of the network buffers, and its 64 x 4 network-pixel blocks are
+// blocks lblk_base.. of the network block table.
 struct BatchPixView {
-    int W, H;
-    uint32_t blk_base, pad;
-    size_t px_off;
+    int W, H, lW, lH;
+    uint32_t blk_base, lblk_base;
+    size_t px_off, lo_off;
     float view[16];
 };
 
-// Neural AO of every view of a batch (nao.hip): features from the batch
-// depth / G-buffer, four convolutions, head into the batch AO target.
-// nao_reserve allocates the batch-sized activations beforehand (the infer
-// call then allocates nothing, so it can be captured in a graph).
+// Neural AO of every view of a batch (nao.hip): down pass from the batch
+// depth / G-buffer, four convolutions on the network grids, head, up pass
+// into the batch AO target.  nao_reserve allocates the batch-sized buffers
+// beforehand (the infer call then allocates nothing, so it can be captured
+// in a graph).
 struct NaoBatchTables {
     const BatchPixView *views;
-    const uint8_t *blk_view;     // view of each 64 x 4 pixel block
+    const uint8_t *blk_view;     // view of each 64 x 4 frame-pixel block
     uint32_t n_blocks;
+    const uint8_t *lblk_view;    // view of each 64 x 4 network-pixel block
+    uint32_t n_lblocks;
+    size_t total_px, total_lpx;  // frame and network pixels of all views
     const uint2 *items[4];       // convolution tiles of each layer (nao_conv_items)
     uint32_t n_items[4];
 };
-int nao_reserve(fgl_nao *n, size_t total_px);
+int nao_reserve(fgl_nao *n, size_t frame_px, size_t net_px);
+// Frame pixels per network pixel, each way (1, 2 or 4).
+int nao_scale(const fgl_nao *n);
 // Changes whenever nao_reserve moves the inference buffers (a graph that
 // captured their addresses must then be captured again).
 uint64_t nao_buffers_generation(const fgl_nao *n);
-// Appends the convolution tiles of a W x H view to the four per-layer lists.
-void nao_conv_items(int view, int W, int H, std::vector<uint2> items[4]);
-int nao_infer_batch(fgl_nao *n, hipStream_t s, const NaoBatchTables &t, size_t total_px, const float *depth,
-                    const float *gbuf, float *ao);
+// Appends the convolution tiles of a view's lW x lH network grid to the four
+// per-layer lists.
+void nao_conv_items(int view, int lW, int lH, std::vector<uint2> items[4]);
+int nao_infer_batch(fgl_nao *n, hipStream_t s, const NaoBatchTables &t, const float *depth, const float *gbuf,
+                    float *ao);
 
 // Scoped HIP-event timing of the kernels enqueued on the context stream
 // while it lives, reported by fgl_read_timings under `name` (a string

@@ -1,12 +1,20 @@
 // nao.hip -- neural ambient occlusion for gfx950: a small convolutional
 // network that stands in for SSAO hemisphere sampling plus blur.
 //
-// Network (per pixel, 'same' 3x3 convolutions with zero padding):
-//   in  7 features (view depth, view normal xyz, view position xyz),
-//       stored as 16 BF16 channels (9 zero)
-//   L1  conv3x3 16->32 + ReLU      L2..L4 conv3x3 32->32 + ReLU
-//   L5  conv1x1 32->1 + sigmoid
-//   the 3x3 layers are dilated by 1, 2, 4, 8 (receptive field 31 x 31)
+// The network runs on a grid `scale` times coarser than the frame (scale 1,
+// 2 or 4; the shipped network uses 4): each network pixel stands for a
+// scale x scale block of frame pixels.
+//   down  per network pixel: the mean over the block's covered pixels of 7
+//         view-space features (depth, normal xyz, position xyz) and the
+//         covered fraction of the block -- 8 inputs, stored as 16 BF16
+//         channels (8 zero); a per-frame-pixel coverage mask
+//   L1    conv3x3 16->32 + ReLU      L2..L4 conv3x3 32->32 + ReLU
+//   L5    conv1x1 32->1 + sigmoid (fused into L4)
+//         the 3x3 layers are dilated by 1, 2, 6, 12: their taps reach every
+//         offset in [-21, 21] (receptive field 43 x 43 network pixels, 172 x
+//         172 frame pixels at scale 4)
+//   up    bilinear interpolation of the network's AO between block centres
+//         at every covered frame pixel; AO 1 elsewhere
 // Convolutions are implicit GEMMs on MFMA (v_mfma_f32_16x16x32_bf16):
 // M = pixels, N = 32 output channels, K = 9 taps x Cin, BF16 operands, FP32
 // accumulation, BF16 NHWC activations.  Backward: the data gradient is the
@@ -14,7 +22,8 @@
 // the weight gradient is an MFMA GEMM over pixels (co x pixels times
 // pixels x (tap, ci)) with per-block partial sums reduced in a fixed order
 // (bitwise reproducible training, no atomics).  Adam keeps FP32 master
-// weights.
+// weights.  Training runs on network-grid crops whose targets are the block
+// means of the plain pipeline's AO.
 #include <hip/hip_runtime.h>
 
 #include <cmath>
@@ -30,7 +39,8 @@ namespace nao {
 typedef short v4s __attribute__((ext_vector_type(4)));
 typedef float v4f __attribute__((ext_vector_type(4)));
 
-constexpr int C0 = 16;    // stored input channels (7 real)
+constexpr int C0 = 16;    // stored input channels (8 real)
+constexpr int NIN = 8;    // real input channels: 7 features + covered fraction
 constexpr int CH = 32;    // hidden channels
 constexpr int K1 = 9 * C0, K2 = 9 * CH;
 // flat FP32 parameter layout
@@ -41,10 +51,12 @@ constexpr int OFF_W4 = OFF_B3 + CH, OFF_B4 = OFF_W4 + CH * K2;
 constexpr int OFF_W5 = OFF_B4 + CH, OFF_B5 = OFF_W5 + CH;
 constexpr int NPARAM = OFF_B5 + 1;
 constexpr int WG_ROWS = 16;   // rows of pixels per weight-gradient workgroup
-// Dilation of the four 3x3 convolutions (taps at -d, 0, +d): a 31 x 31
-// receptive field at the cost of four plain 3x3 layers.
-constexpr int kMaxDil = 8;
-__host__ __device__ constexpr int layer_dil(int l) { return l == 0 ? 1 : l == 1 ? 2 : l == 2 ? 4 : 8; }
+// Dilation of the four 3x3 convolutions (taps at -d, 0, +d): sums of one tap
+// offset per layer cover every offset in [-21, 21] without gaps, a 43 x 43
+// receptive field at the cost of four plain 3x3 layers.  Every dilation
+// above 1 is even (the tile order pairs x-phases 2m, 2m + 1).
+constexpr int kMaxDil = 12;
+__host__ __device__ constexpr int layer_dil(int l) { return l == 0 ? 1 : l == 1 ? 2 : l == 2 ? 6 : 12; }
 
 __host__ __device__ __forceinline__ uint16_t f2bf(float f)
 {
@@ -66,16 +78,13 @@ __host__ __device__ __forceinline__ float bf2f(uint16_t h)
 // View-space features of one G-buffer pixel: depth d = -z_view / 10,
 // normal = V3 * N, position = V * (P,1) / 10.  Pixels where nothing was
 // drawn, or whose G-buffer holds a non-finite normal or position (a
-// degenerate tangent frame makes the mapped normal NaN), are uncovered: zero
-// features, AO 1 -- what kernel SSAO yields there too -- and no part in the
-// loss, so one such pixel cannot poison a weight gradient.
-__device__ __forceinline__ void pixel_features(const float *V, const float *depth, const float *g,
-                                               size_t P, size_t i, float f[7], bool &covered)
+// degenerate tangent frame makes the mapped normal NaN), are uncovered: no
+// part in the features or the loss, AO 1 -- what kernel SSAO yields there
+// too -- so one such pixel cannot poison a weight gradient.
+__device__ __forceinline__ bool features_of(const float *V, float d, Vec3 N, Vec3 Pw, float f[7])
 {
-    covered = depth[i] < 3.0e38f;
+    bool covered = d < 3.0e38f;
     if (covered) {
-        Vec3 N = vec3(g[i], g[P + i], g[2 * P + i]);
-        Vec3 Pw = vec3(g[3 * P + i], g[4 * P + i], g[5 * P + i]);
         Vec4 vp = mv4(V, vec4(Pw.x, Pw.y, Pw.z, 1.f));
         Vec3 vn = vec3(dot3(vec3(V[0], V[1], V[2]), N), dot3(vec3(V[4], V[5], V[6]), N),
                        dot3(vec3(V[8], V[9], V[10]), N));
@@ -84,46 +93,202 @@ __device__ __forceinline__ void pixel_features(const float *V, const float *dept
         f[4] = vp.x * 0.1f; f[5] = vp.y * 0.1f; f[6] = vp.z * 0.1f;
         for (int c = 0; c < 7; ++c) covered = covered && isfinite(f[c]);
     }
-    if (!covered)
-        for (int c = 0; c < 7; ++c) f[c] = 0.f;
+    return covered;
 }
 
 struct Mat16 {
     float m[16];
 };
 
-// One crop of the current frame into batch slot `slot`: features, AO target,
-// mask.  Crop origin (x0, y0), size S x S, inside the frame.
+// Inputs of one network pixel (X, Y): the block of frame pixels
+// [X*f, X*f+f) x [Y*f, Y*f+f) clipped to the W x H frame.  feat[0..6] = mean
+// of the covered pixels' features, feat[7] = covered / block pixels in the
+// frame.  fmask (optional) receives each frame pixel's coverage; ao
+// (optional) gives *ao_mean, the covered pixels' mean AO (1 when none).
+// Returns the covered count.  Rows of four pixels are read as 16-byte loads
+// when the block is four wide, inside the frame and the rows are 16-byte
+// aligned.
+__device__ __forceinline__ int block_features(const float *V, const float *__restrict__ depth,
+                                              const float *__restrict__ g, int W, int H, int X, int Y, int f,
+                                              float feat[NIN], uint8_t *__restrict__ fmask,
+                                              const float *__restrict__ ao, float *ao_mean)
+{
+    const size_t P = (size_t)W * H;
+    float acc[7] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+    float aos = 0.f;
+    int n = 0, m = 0;
+    const int x0 = X * f;
+    // (W % 4 == 0 keeps every row start, plane and ao offset on the same
+    // 16-byte alignment as `depth`; fmask then shares it at 4 bytes)
+    const bool vec4 = f == 4 && (W & 3) == 0 && x0 + 3 < W && ((uintptr_t)depth & 15) == 0 &&
+                      ((uintptr_t)g & 15) == 0 && (!ao || ((uintptr_t)ao & 15) == 0) &&
+                      (!fmask || ((uintptr_t)fmask & 3) == 0);
+    for (int dy = 0; dy < f; ++dy) {
+        const int y = Y * f + dy;
+        if (y >= H) break;
+        const size_t r = (size_t)y * W + x0;
+        float d[4], nx[4], ny[4], nz[4], px[4], py[4], pz[4], a[4];
+        const int cnt = vec4 ? 4 : (W - x0 < f ? W - x0 : f);
+        if (vec4) {
+            const float4 q0 = *(const float4 *)(depth + r), q1 = *(const float4 *)(g + r),
+                         q2 = *(const float4 *)(g + P + r), q3 = *(const float4 *)(g + 2 * P + r),
+                         q4 = *(const float4 *)(g + 3 * P + r), q5 = *(const float4 *)(g + 4 * P + r),
+                         q6 = *(const float4 *)(g + 5 * P + r);
+            d[0] = q0.x; d[1] = q0.y; d[2] = q0.z; d[3] = q0.w;
+            nx[0] = q1.x; nx[1] = q1.y; nx[2] = q1.z; nx[3] = q1.w;
+            ny[0] = q2.x; ny[1] = q2.y; ny[2] = q2.z; ny[3] = q2.w;
+            nz[0] = q3.x; nz[1] = q3.y; nz[2] = q3.z; nz[3] = q3.w;
+            px[0] = q4.x; px[1] = q4.y; px[2] = q4.z; px[3] = q4.w;
+            py[0] = q5.x; py[1] = q5.y; py[2] = q5.z; py[3] = q5.w;
+            pz[0] = q6.x; pz[1] = q6.y; pz[2] = q6.z; pz[3] = q6.w;
+            if (ao) {
+                const float4 qa = *(const float4 *)(ao + r);
+                a[0] = qa.x; a[1] = qa.y; a[2] = qa.z; a[3] = qa.w;
+            }
+        }
+        uint32_t mk = 0;
+        for (int dx = 0; dx < f && dx < 4; ++dx) {
+            if (dx >= cnt) break;
+            if (!vec4) {
+                const size_t i = r + dx;
+                d[dx] = depth[i];
+                nx[dx] = g[i]; ny[dx] = g[P + i]; nz[dx] = g[2 * P + i];
+                px[dx] = g[3 * P + i]; py[dx] = g[4 * P + i]; pz[dx] = g[5 * P + i];
+                if (ao) a[dx] = ao[i];
+            }
+            float pf[7];
+            const bool cov = features_of(V, d[dx], vec3(nx[dx], ny[dx], nz[dx]), vec3(px[dx], py[dx], pz[dx]), pf);
+            ++m;
+            if (cov) {
+                ++n;
+                for (int c = 0; c < 7; ++c) acc[c] += pf[c];
+                if (ao) aos += a[dx];
+            }
+            mk |= (cov ? 1u : 0u) << (8 * dx);
+            if (fmask && !vec4) fmask[r + dx] = cov ? 1 : 0;
+        }
+        if (fmask && vec4) *(uint32_t *)(fmask + r) = mk;
+    }
+    const float inv = n ? 1.f / (float)n : 0.f;
+    for (int c = 0; c < 7; ++c) feat[c] = acc[c] * inv;
+    feat[7] = m ? (float)n / (float)m : 0.f;
+    if (ao_mean) *ao_mean = n ? aos * inv : 1.f;
+    return n;
+}
+
+// The 16 stored BF16 input channels of one network pixel.
+__device__ __forceinline__ void store_inputs(uint16_t *__restrict__ dst, const float feat[NIN])
+{
+    uint16_t b[8];
+#pragma unroll
+    for (int c = 0; c < 8; ++c) b[c] = f2bf(feat[c]);
+    uint4 *o = (uint4 *)dst;
+    o[0] = make_uint4(b[0] | ((uint32_t)b[1] << 16), b[2] | ((uint32_t)b[3] << 16), b[4] | ((uint32_t)b[5] << 16),
+                      b[6] | ((uint32_t)b[7] << 16));
+    o[1] = make_uint4(0u, 0u, 0u, 0u);
+}
+
+// Per-view pixel layout: the view's frame pixels start at px_off, its
+// network pixels at lo_off; frame blocks of 64 x 4 pixels are blocks
+// blk_base.. of the batch's frame block table, network blocks of 64 x 4
+// network pixels blocks lblk_base.. of its network block table.
+__device__ __forceinline__ const BatchPixView &view_of_block(const BatchPixView *__restrict__ views,
+                                                             const uint8_t *__restrict__ blk_view, int b)
+{
+    return views[blk_view[b]];
+}
+
+// Down pass: network inputs, network-pixel coverage and frame-pixel
+// coverage.  One thread per network pixel, 64 x 4 network pixels per
+// workgroup.  Single frame: `one` describes it; batch: `views` and the
+// network block table.
+template <bool BATCH>
+__global__ __launch_bounds__(256) void k_nao_down(BatchPixView one, const BatchPixView *__restrict__ views,
+                                                  const uint8_t *__restrict__ lblk_view, int f,
+                                                  const float *__restrict__ depth, const float *__restrict__ gbuf,
+                                                  uint16_t *__restrict__ feat, uint8_t *__restrict__ lmask,
+                                                  uint8_t *__restrict__ fmask)
+{
+    const BatchPixView &v = BATCH ? view_of_block(views, lblk_view, blockIdx.x) : one;
+    const int k = BATCH ? (int)(blockIdx.x - v.lblk_base) : (int)blockIdx.x;
+    const int bx = (v.lW + 63) / 64;
+    const int X = (k % bx) * 64 + (int)(threadIdx.x & 63), Y = (k / bx) * 4 + (int)(threadIdx.x >> 6);
+    if (X >= v.lW || Y >= v.lH) return;
+    float in[NIN];
+    const int n = block_features(v.view, depth + v.px_off, gbuf + (size_t)kGbufPlanes * v.px_off, v.W, v.H, X, Y, f,
+                                 in, fmask + v.px_off, nullptr, nullptr);
+    const size_t o = v.lo_off + (size_t)Y * v.lW + X;
+    store_inputs(feat + o * C0, in);
+    lmask[o] = n > 0 ? 1 : 0;
+}
+
+// Up pass: AO of every frame pixel -- 1 where the pixel is uncovered, else
+// the network AO interpolated bilinearly between the centres of the blocks
+// around the pixel's centre (block (X, Y) has its centre at frame
+// coordinate f X + f/2; edges clamp).  One thread per frame pixel, 64 x 4
+// frame pixels per workgroup.
+template <bool BATCH>
+__global__ __launch_bounds__(256) void k_nao_up(BatchPixView one, const BatchPixView *__restrict__ views,
+                                                const uint8_t *__restrict__ blk_view, int f,
+                                                const float *__restrict__ lao, const uint8_t *__restrict__ fmask,
+                                                float *__restrict__ ao)
+{
+    const BatchPixView &v = BATCH ? view_of_block(views, blk_view, blockIdx.x) : one;
+    const int k = BATCH ? (int)(blockIdx.x - v.blk_base) : (int)blockIdx.x;
+    const int bx = (v.W + 63) / 64;
+    const int x = (k % bx) * 64 + (int)(threadIdx.x & 63), y = (k / bx) * 4 + (int)(threadIdx.x >> 6);
+    if (x >= v.W || y >= v.H) return;
+    const size_t i = v.px_off + (size_t)y * v.W + x;
+    if (!fmask[i]) {
+        ao[i] = 1.f;
+        return;
+    }
+    const float inv = 1.f / (float)f;
+    const float u = ((float)x + 0.5f) * inv - 0.5f, w = ((float)y + 0.5f) * inv - 0.5f;
+    const float fu = floorf(u), fw = floorf(w);
+    const float tx = u - fu, ty = w - fw;
+    const int X0 = (int)fu, Y0 = (int)fw;
+    const int xa = imax(0, imin(v.lW - 1, X0)), xb = imax(0, imin(v.lW - 1, X0 + 1));
+    const int ya = imax(0, imin(v.lH - 1, Y0)), yb = imax(0, imin(v.lH - 1, Y0 + 1));
+    const float *L = lao + v.lo_off;
+    const float top = L[(size_t)ya * v.lW + xa] * (1.f - tx) + L[(size_t)ya * v.lW + xb] * tx;
+    const float bot = L[(size_t)yb * v.lW + xa] * (1.f - tx) + L[(size_t)yb * v.lW + xb] * tx;
+    ao[i] = top * (1.f - ty) + bot * ty;
+}
+
+// Receptive-field radius of the network in network pixels (sum of the
+// dilations): a crop pixel closer than this to a crop edge that lies inside
+// the frame sees zero padding where the frame continues.
+constexpr int kReach = 1 + 2 + 6 + 12;
+
+// One crop of the current frame's network grid into batch slot `slot`:
+// inputs, target (the block mean of the frame's AO over covered pixels) and
+// mask: the block has a covered pixel and lies at least kReach network
+// pixels inside every crop edge that is not a frame edge (the loss then
+// only sees pixels whose inputs the crop holds as the whole frame does).
+// Crop origin (x0, y0) and size S x S in network pixels, inside the lW x lH
+// network grid.
 __global__ __launch_bounds__(256) void k_crop(Mat16 V, const float *__restrict__ depth,
                                               const float *__restrict__ g,
-                                              const float *__restrict__ ao, int W, int H, int x0,
-                                              int y0, int S, int slot,
+                                              const float *__restrict__ ao, int W, int H, int f, int lW, int lH,
+                                              int x0, int y0, int S, int slot,
                                               uint16_t *__restrict__ feat,
                                               float *__restrict__ target,
                                               uint8_t *__restrict__ mask)
 {
     const int cx = blockIdx.x * 64 + (threadIdx.x & 63), cy = blockIdx.y * 4 + (threadIdx.x >> 6);
     if (cx >= S || cy >= S) return;
-    const size_t P = (size_t)W * H, i = (size_t)(x0 + cx) + (size_t)(y0 + cy) * W;
     const size_t o = (size_t)slot * S * S + (size_t)cy * S + cx;
-    float f[7];
-    bool cov;
-    pixel_features(V.m, depth, g, P, i, f, cov);
-    for (int c = 0; c < C0; ++c) feat[o * C0 + c] = c < 7 ? f2bf(f[c]) : (uint16_t)0;
-    target[o] = ao[i];
-    mask[o] = cov ? 1 : 0;
+    float in[NIN], t;
+    const int n = block_features(V.m, depth, g, W, H, x0 + cx, y0 + cy, f, in, nullptr, ao, &t);
+    store_inputs(feat + o * C0, in);
+    target[o] = t;
+    const bool inside = (x0 == 0 || cx >= kReach) && (x0 + S == lW || cx < S - kReach) &&
+                        (y0 == 0 || cy >= kReach) && (y0 + S == lH || cy < S - kReach);
+    mask[o] = n > 0 && inside ? 1 : 0;
 }
 
 // ---------------- 3x3 convolution on MFMA ----------------
-
-// First layer with the features computed while staging (inference): the
-// G-buffer planes and depth of the image set, the view matrix (single-image
-// launches; batched views take their own), and the coverage mask it writes.
-struct FeatArgs {
-    const float *depth, *gbuf;
-    uint8_t *mask;
-    Mat16 V;
-};
 
 // Head pre-activation: conv1x1 32 -> 1 over one pixel's BF16 activations, in
 // channel order.
@@ -145,11 +310,6 @@ struct HeadArgs {
 // MODE 0: out = bf16(relu(conv(in) + bias)).
 // MODE 1 (data gradient): out = bf16(conv(in) * (act > 0)) with tap-flipped,
 // transposed weights; `in` is the layer's output gradient.
-// MODE 3 (first layer of inference): as MODE 0 with CIN 16, the input tile
-// computed from the G-buffer while staging (pixel_features, as the training
-// crops get them: 7 features, BF16, channels 7-15 zero except channel 15 =
-// coverage, whose weights are held at zero), and the coverage mask written
-// for the tile's pixels.
 // MODE 2 (last layer of inference): as MODE 0, but each row of outputs goes
 // to the wavefront's LDS row buffer and the head -- head_z, then
 // 1 / (1 + expf(-z)), AO 1 where the mask is 0 (the background keeps
@@ -204,7 +364,6 @@ constexpr int kStagePix = 40;  // bf16 per staged pixel (32 channels + 8 pad)
 struct ConvTile {
     size_t img;
     int W, H, phx, phy, bx, by;
-    const float *V;   // view matrix (first layer with features)
 };
 
 template <int CIN>
@@ -245,48 +404,6 @@ __device__ __forceinline__ void stage_store(const ConvStage<CIN> &st, uint16_t *
     for (int q = 0; q < ConvStage<CIN>::N; ++q) {
         const int e = (int)threadIdx.x + 256 * q;
         if (e < TOT) *(uint4 *)(s_in + (e / CPP) * kStagePix + (e % CPP) * 8) = st.v[q];
-    }
-}
-
-// MODE 3 staging: one thread per staged pixel (3 rounds over the 66 x 10
-// pixels), both 16-byte chunks of it.
-__device__ __forceinline__ void stage_load_feat(ConvStage<C0> &st, const FeatArgs &fa, const ConvTile &t, int dil)
-{
-    static_assert(ConvStage<C0>::N >= 6, "two chunks for each of three pixels");
-    const int Wp = (t.W - t.phx + dil - 1) / dil, Hp = (t.H - t.phy + dil - 1) / dil;
-    const int i0 = t.bx * kTileW - 1, j0 = t.by * kTileH - 1;
-    const size_t P = (size_t)t.W * t.H;
-#pragma unroll
-    for (int q = 0; q < 3; ++q) {
-        const int pix = (int)threadIdx.x + 256 * q;
-        st.v[2 * q] = make_uint4(0u, 0u, 0u, 0u);
-        st.v[2 * q + 1] = make_uint4(0u, 0u, 0u, 0u);
-        const int i = i0 + pix % kStageW, j = j0 + pix / kStageW;
-        if (pix < kStageW * kStageH && i >= 0 && i < Wp && j >= 0 && j < Hp) {
-            const size_t k = (size_t)(t.phy + j * dil) * t.W + t.phx + i * dil;
-            float f[7];
-            bool cov;
-            pixel_features(t.V, fa.depth + t.img, fa.gbuf + (size_t)kGbufPlanes * t.img, P, k, f, cov);
-            uint16_t b[8];
-#pragma unroll
-            for (int c = 0; c < 7; ++c) b[c] = f2bf(f[c]);
-            b[7] = 0;
-            st.v[2 * q] = make_uint4(b[0] | ((uint32_t)b[1] << 16), b[2] | ((uint32_t)b[3] << 16),
-                                     b[4] | ((uint32_t)b[5] << 16), b[6] | ((uint32_t)b[7] << 16));
-            st.v[2 * q + 1] = make_uint4(0u, 0u, 0u, cov ? 0x3f800000u : 0u);   // channel 15 = 1.0
-        }
-    }
-}
-
-__device__ __forceinline__ void stage_store_feat(const ConvStage<C0> &st, uint16_t *s_in)
-{
-#pragma unroll
-    for (int q = 0; q < 3; ++q) {
-        const int pix = (int)threadIdx.x + 256 * q;
-        if (pix < kStageW * kStageH) {
-            *(uint4 *)(s_in + pix * kStagePix) = st.v[2 * q];
-            *(uint4 *)(s_in + pix * kStagePix + 8) = st.v[2 * q + 1];
-        }
     }
 }
 
@@ -341,7 +458,7 @@ template <int CIN, int MODE>
 __device__ __forceinline__ void conv_tile_compute(const ConvWeights<CIN> &cw, const uint16_t *s_in,
                                                   const float *__restrict__ bias, const uint16_t *__restrict__ act,
                                                   uint16_t *__restrict__ out, const ConvTile &t, int dil,
-                                                  const HeadArgs &hd, uint16_t *s_head, const FeatArgs &fa)
+                                                  const HeadArgs &hd, uint16_t *s_head)
 {
     constexpr int K = ConvWeights<CIN>::K, KC = ConvWeights<CIN>::KC;
     const int Wp = (t.W - t.phx + dil - 1) / dil, Hp = (t.H - t.phy + dil - 1) / dil;
@@ -387,7 +504,7 @@ __device__ __forceinline__ void conv_tile_compute(const ConvWeights<CIN> &cw, co
                 const int co = cb * 16 + lq * 4;
                 const size_t o = (t.img + (size_t)y * t.W + x) * CH + co;
                 uint16_t rr[4];
-                if (MODE == 0 || MODE == 3) {
+                if (MODE == 0) {
                     const float4 bv = *(const float4 *)(bias + co);
                     const float bb[4] = {bv.x, bv.y, bv.z, bv.w};
 #pragma unroll
@@ -402,11 +519,6 @@ __device__ __forceinline__ void conv_tile_compute(const ConvWeights<CIN> &cw, co
                 }
                 *(uint64_t *)(out + o) = (uint64_t)rr[0] | ((uint64_t)rr[1] << 16) | ((uint64_t)rr[2] << 32) |
                                          ((uint64_t)rr[3] << 48);
-            }
-            if (MODE == 3 && lq == 0) {
-                // coverage of the pixel, staged as its channel 15
-                const uint16_t cv = s_in[((jr + 1) * kStageW + pb * 16 + lr + 1) * kStagePix + 15];
-                fa.mask[t.img + (size_t)y * t.W + x] = cv != 0 ? 1 : 0;
             }
         }
     }
@@ -459,7 +571,6 @@ __device__ __forceinline__ ConvTile single_tile(int t, int W, int H, int dil, in
     ct.phy = ph / dil;
     ct.bx = k % tiles_x;
     ct.by = k / tiles_x;
-    ct.V = nullptr;
     return ct;
 }
 
@@ -469,15 +580,14 @@ __device__ __forceinline__ ConvTile batch_tile(const BatchPixView *__restrict__ 
     const uint2 it = items[t];
     const BatchPixView &bv = views[it.x & 0xff];
     const int ph = (int)(it.x >> 8);
-    ConvTile ct;
-    ct.img = bv.px_off;
-    ct.W = bv.W;
-    ct.H = bv.H;
+    ConvTile ct;   // on the view's network grid
+    ct.img = bv.lo_off;
+    ct.W = bv.lW;
+    ct.H = bv.lH;
     ct.phx = ph % dil;
     ct.phy = ph / dil;
     ct.bx = (int)(it.y & 0xffff);
     ct.by = (int)(it.y >> 16);
-    ct.V = bv.view;
     return ct;
 }
 
@@ -491,8 +601,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
                                                        uint16_t *__restrict__ out, int W, int H, int dil,
                                                        int n_tiles, int tiles_x, int tiles_y,
                                                        const BatchPixView *__restrict__ views,
-                                                       const uint2 *__restrict__ items, HeadArgs hd,
-                                                       FeatArgs fa)
+                                                       const uint2 *__restrict__ items, HeadArgs hd)
 {
     __shared__ __attribute__((aligned(16))) uint16_t s_in[kStageH * kStageW * kStagePix];
     __shared__ __attribute__((aligned(16))) uint16_t s_head[MODE == 2 ? 4 * 32 * kStagePix : 8];
@@ -501,24 +610,19 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
     int t = blockIdx.x;
     if (t >= n_tiles) return;
     ConvTile cur = BATCH ? batch_tile(views, items, t, dil) : single_tile(t, W, H, dil, tiles_x, tiles_y);
-    if (MODE == 3 && !BATCH) cur.V = fa.V.m;
     ConvStage<CIN> st;
-    if constexpr (MODE == 3) stage_load_feat(st, fa, cur, dil);
-    else stage_load<CIN>(st, in, cur, dil);
+    stage_load<CIN>(st, in, cur, dil);
     for (;;) {
         __syncthreads();   // the previous tile is no longer read from LDS
-        if constexpr (MODE == 3) stage_store_feat(st, s_in);
-        else stage_store<CIN>(st, s_in);
+        stage_store<CIN>(st, s_in);
         __syncthreads();
         const int nt = t + (int)gridDim.x;
         ConvTile nxt = cur;
         if (nt < n_tiles) {
             nxt = BATCH ? batch_tile(views, items, nt, dil) : single_tile(nt, W, H, dil, tiles_x, tiles_y);
-            if (MODE == 3 && !BATCH) nxt.V = fa.V.m;
-            if constexpr (MODE == 3) stage_load_feat(st, fa, nxt, dil);
-            else stage_load<CIN>(st, in, nxt, dil);
+            stage_load<CIN>(st, in, nxt, dil);
         }
-        conv_tile_compute<CIN, MODE>(cw, s_in, bias, act, out, cur, dil, hd, s_head, fa);
+        conv_tile_compute<CIN, MODE>(cw, s_in, bias, act, out, cur, dil, hd, s_head);
         if (nt >= n_tiles) break;
         t = nt;
         cur = nxt;
@@ -560,19 +664,7 @@ static void launch_conv(hipStream_t s, const uint16_t *in, const uint16_t *Wt, c
     if (n == 0) return;
     hipLaunchKernelGGL((k_conv3x3_tiled<CIN, MODE, false>), dim3(conv_workgroups(n, CIN == C0 ? 3 : 2)), dim3(256), 0, s, in, Wt, bias,
                        act, out, W, H, dil, n, tx, ty, (const BatchPixView *)nullptr, (const uint2 *)nullptr,
-                       HeadArgs{nullptr, nullptr, nullptr}, FeatArgs{});
-}
-
-// First layer with the features computed while staging (single image set).
-static void launch_conv_feat(hipStream_t s, const FeatArgs &fa, const uint16_t *Wt, const float *bias, uint16_t *out,
-                             int W, int H, int dil)
-{
-    int tx, ty, n;
-    conv_tiles(W, H, 1, dil, tx, ty, n);
-    if (n == 0) return;
-    hipLaunchKernelGGL((k_conv3x3_tiled<C0, 3, false>), dim3(conv_workgroups(n, 3)), dim3(256), 0, s,
-                       (const uint16_t *)nullptr, Wt, bias, (const uint16_t *)nullptr, out, W, H, dil, n, tx, ty,
-                       (const BatchPixView *)nullptr, (const uint2 *)nullptr, HeadArgs{nullptr, nullptr, nullptr}, fa);
+                       HeadArgs{nullptr, nullptr, nullptr});
 }
 
 // Last layer with the head fused (single image set of N images).
@@ -584,18 +676,17 @@ static void launch_conv_head(hipStream_t s, const uint16_t *in, const uint16_t *
     if (n == 0) return;
     hipLaunchKernelGGL((k_conv3x3_tiled<CH, 2, false>), dim3(conv_workgroups(n, 2)), dim3(256), 0, s, in, Wt, bias,
                        (const uint16_t *)nullptr, (uint16_t *)nullptr, W, H, dil, n, tx, ty,
-                       (const BatchPixView *)nullptr, (const uint2 *)nullptr, hd, FeatArgs{});
+                       (const BatchPixView *)nullptr, (const uint2 *)nullptr, hd);
 }
 
 template <int CIN, int MODE = 0>
 static void launch_conv_batch(hipStream_t s, const BatchPixView *views, const uint2 *items, int n, const uint16_t *in,
                               const uint16_t *Wt, const float *bias, uint16_t *out, int dil,
-                              const HeadArgs &hd = HeadArgs{nullptr, nullptr, nullptr},
-                              const FeatArgs &fa = FeatArgs{})
+                              const HeadArgs &hd = HeadArgs{nullptr, nullptr, nullptr})
 {
     if (n == 0) return;
     hipLaunchKernelGGL((k_conv3x3_tiled<CIN, MODE, true>), dim3(conv_workgroups(n, CIN == C0 ? 3 : 2)), dim3(256), 0, s, in, Wt, bias,
-                       (const uint16_t *)nullptr, out, 0, 0, dil, n, 0, 0, views, items, hd, fa);
+                       (const uint16_t *)nullptr, out, 0, 0, dil, n, 0, 0, views, items, hd);
 }
 
 
@@ -758,10 +849,9 @@ __global__ __launch_bounds__(256) void k_prep_weights(const float *__restrict__ 
                                                       uint16_t *__restrict__ wt234)
 {
     const int i = blockIdx.x * 256 + threadIdx.x;
-    // layer 1: only the 7 real input channels; the padding channels' weights
-    // are held at zero (their inputs are zero, except channel 15, which
-    // carries the coverage in the fused inference layer)
-    if (i < CH * K1) w1[i] = (i % K1) % C0 < 7 ? f2bf(params[OFF_W1 + i]) : (uint16_t)0;
+    // layer 1: only the 8 real input channels; the padding channels' weights
+    // are held at zero (their inputs are zero)
+    if (i < CH * K1) w1[i] = (i % K1) % C0 < NIN ? f2bf(params[OFF_W1 + i]) : (uint16_t)0;
     if (i < 3 * CH * K2) {
         const int l = i / (CH * K2), r = i % (CH * K2);
         const int offs[3] = {OFF_W2, OFF_W3, OFF_W4};
@@ -802,6 +892,7 @@ using namespace fgl::nao;
 struct fgl_nao {
     fgl_ctx *ctx = nullptr;
     int batch = 0, crop = 0;
+    int scale = 4;   // frame pixels per network pixel, each way
     int step = 0;
     // parameters (FP32 master), gradients, Adam moments
     float *params = nullptr, *grads = nullptr, *m = nullptr, *v = nullptr;
@@ -813,10 +904,12 @@ struct fgl_nao {
     uint8_t *mask = nullptr;
     float *part = nullptr, *part_db = nullptr, *head_part = nullptr, *head_red = nullptr;
     size_t part_cap = 0;
-    // inference buffers (grow-only)
-    uint16_t *ia = nullptr, *ib = nullptr;
-    uint8_t *imask = nullptr;
-    size_t icap = 0;
+    // inference buffers (grow-only): network-grid inputs, activations,
+    // coverage and AO; frame-pixel coverage
+    uint16_t *ifeat = nullptr, *ia = nullptr, *ib = nullptr;
+    uint8_t *imask = nullptr, *fmask = nullptr;
+    float *lao = nullptr;
+    size_t icap = 0, fcap = 0;
     uint64_t igen = 0;   // bumped whenever the inference buffers move
     float *h_stats = nullptr;
 };
@@ -830,24 +923,37 @@ static inline dim3 g2(int W, int H) { return dim3((W + 63) / 64, (H + 3) / 4); }
 
 namespace fgl {
 
-int nao_reserve(fgl_nao *n, size_t px)
+int nao_reserve(fgl_nao *n, size_t frame_px, size_t net_px)
 {
-    if (px <= n->icap) return FGL_OK;
+    if (net_px <= n->icap && frame_px <= n->fcap) return FGL_OK;
     hipStreamSynchronize(ctx_stream(n->ctx));
-    if (n->ia) {
-        hipFree(n->ia); hipFree(n->ib); hipFree(n->imask);
-        n->ia = n->ib = nullptr;
-        n->imask = nullptr;
-        n->icap = 0;
-    }
-    // activations of layers 1-3 (layer 1 reads the G-buffer, layer 4 writes AO)
     ++n->igen;
-    if (nao_alloc((void **)&n->ia, px * CH * 2) || nao_alloc((void **)&n->ib, px * CH * 2) ||
-        nao_alloc((void **)&n->imask, px))
-        return FGL_ERR_NOMEM;
-    n->icap = px;
+    if (net_px > n->icap) {
+        void *ps[] = {n->ifeat, n->ia, n->ib, n->imask, n->lao};
+        for (void *q : ps)
+            if (q) hipFree(q);
+        n->ifeat = n->ia = n->ib = nullptr;
+        n->imask = nullptr;
+        n->lao = nullptr;
+        n->icap = 0;
+        // network inputs, activations of layers 1-3, coverage, AO
+        if (nao_alloc((void **)&n->ifeat, net_px * C0 * 2) || nao_alloc((void **)&n->ia, net_px * CH * 2) ||
+            nao_alloc((void **)&n->ib, net_px * CH * 2) || nao_alloc((void **)&n->imask, net_px) ||
+            nao_alloc((void **)&n->lao, net_px * 4))
+            return FGL_ERR_NOMEM;
+        n->icap = net_px;
+    }
+    if (frame_px > n->fcap) {
+        if (n->fmask) hipFree(n->fmask);
+        n->fmask = nullptr;
+        n->fcap = 0;
+        if (nao_alloc((void **)&n->fmask, frame_px)) return FGL_ERR_NOMEM;
+        n->fcap = frame_px;
+    }
     return FGL_OK;
 }
+
+int nao_scale(const fgl_nao *n) { return n->scale; }
 
 uint64_t nao_buffers_generation(const fgl_nao *n) { return n->igen; }
 
@@ -866,26 +972,27 @@ void nao_conv_items(int view, int W, int H, std::vector<uint2> items[4])
     }
 }
 
-int nao_infer_batch(fgl_nao *n, hipStream_t s, const NaoBatchTables &t, size_t total_px, const float *depth,
-                    const float *gbuf, float *ao)
+int nao_infer_batch(fgl_nao *n, hipStream_t s, const NaoBatchTables &t, const float *depth, const float *gbuf,
+                    float *ao)
 {
-    if (total_px > n->icap) return FGL_ERR_INVALID;
+    if (t.total_px > n->fcap || t.total_lpx > n->icap) return FGL_ERR_INVALID;
     if (t.n_blocks == 0) return FGL_OK;
-    FeatArgs fa;
-    fa.depth = depth;
-    fa.gbuf = gbuf;
-    fa.mask = n->imask;
-    std::memset(fa.V.m, 0, sizeof(fa.V.m));
-    launch_conv_batch<C0, 3>(s, t.views, t.items[0], (int)t.n_items[0], (const uint16_t *)nullptr, n->w1,
-                             n->params + OFF_B1, n->ia, layer_dil(0), HeadArgs{nullptr, nullptr, nullptr}, fa);
+    const int f = n->scale;
+    const BatchPixView none{};
+    hipLaunchKernelGGL(k_nao_down<true>, dim3(t.n_lblocks), dim3(256), 0, s, none, t.views, t.lblk_view, f, depth,
+                       gbuf, n->ifeat, n->imask, n->fmask);
+    launch_conv_batch<C0>(s, t.views, t.items[0], (int)t.n_items[0], (const uint16_t *)n->ifeat, n->w1,
+                          n->params + OFF_B1, n->ia, layer_dil(0));
     launch_conv_batch<CH>(s, t.views, t.items[1], (int)t.n_items[1], (const uint16_t *)n->ia, n->w234,
                           n->params + OFF_B2, n->ib, layer_dil(1));
     launch_conv_batch<CH>(s, t.views, t.items[2], (int)t.n_items[2], (const uint16_t *)n->ib,
                           n->w234 + (size_t)CH * K2, n->params + OFF_B3, n->ia, layer_dil(2));
-    // layer 4 with the head fused
+    // layer 4 with the head fused, into the network-grid AO
     launch_conv_batch<CH, 2>(s, t.views, t.items[3], (int)t.n_items[3], (const uint16_t *)n->ia,
                              n->w234 + (size_t)2 * CH * K2, n->params + OFF_B4, nullptr, layer_dil(3),
-                             HeadArgs{n->params + OFF_W5, n->imask, ao});
+                             HeadArgs{n->params + OFF_W5, n->imask, n->lao});
+    hipLaunchKernelGGL(k_nao_up<true>, dim3(t.n_blocks), dim3(256), 0, s, none, t.views, t.blk_view, f, n->lao,
+                       n->fmask, ao);
     return hipGetLastError() == hipSuccess ? FGL_OK : FGL_ERR_DEVICE;
 }
 
@@ -895,14 +1002,16 @@ extern "C" {
 
 int fgl_nao_param_count(void) { return NPARAM; }
 
-int fgl_nao_create(fgl_ctx *ctx, int batch, int crop, fgl_nao **out)
+int fgl_nao_create(fgl_ctx *ctx, int batch, int crop, int scale, fgl_nao **out)
 {
-    if (!ctx || !out || batch < 1 || crop < 16 || crop % 64 != 0 || crop % WG_ROWS != 0)
+    if (!ctx || !out || batch < 1 || crop < 16 || crop % 64 != 0 || crop % WG_ROWS != 0 ||
+        (scale != 1 && scale != 2 && scale != 4))
         return FGL_ERR_INVALID;
     fgl_nao *n = new fgl_nao();
     n->ctx = ctx;
     n->batch = batch;
     n->crop = crop;
+    n->scale = scale;
     const size_t px = (size_t)batch * crop * crop;
     int rc = FGL_OK;
     rc |= nao_alloc((void **)&n->params, NPARAM * 4);
@@ -945,8 +1054,8 @@ void fgl_nao_destroy(fgl_nao *n)
     if (!n) return;
     hipDeviceSynchronize();
     void *ps[] = {n->params, n->grads, n->m, n->v, n->w1, n->w234, n->wt234, n->feat, n->target, n->y,
-                  n->mask, n->part, n->part_db, n->head_part, n->head_red, n->ia, n->ib,
-                  n->imask};
+                  n->mask, n->part, n->part_db, n->head_part, n->head_red, n->ifeat, n->ia, n->ib,
+                  n->imask, n->fmask, n->lao};
     for (void *p : ps)
         if (p) hipFree(p);
     for (int l = 0; l < 4; ++l) {
@@ -997,7 +1106,8 @@ int fgl_nao_get_state(fgl_nao *n, float *state, int *step)
 }
 
 // Copies crops of the context's current frame (depth, G-buffer, AO as the
-// target) into batch slots slot0 .. slot0+count-1; xy holds crop origins.
+// target) on the network grid into batch slots slot0 .. slot0+count-1; xy
+// holds crop origins in network pixels.
 int fgl_nao_stage_crops(fgl_nao *n, int slot0, int count, const int *xy)
 {
     if (!n || !xy || slot0 < 0 || count < 0 || slot0 + count > n->batch) return FGL_ERR_INVALID;
@@ -1006,12 +1116,13 @@ int fgl_nao_stage_crops(fgl_nao *n, int slot0, int count, const int *xy)
     if (rc) return rc;
     Mat16 V;
     std::memcpy(V.m, cv.view, sizeof(V.m));
+    const int f = n->scale, lW = (cv.W + f - 1) / f, lH = (cv.H + f - 1) / f;
     for (int i = 0; i < count; ++i) {
         const int x0 = xy[2 * i], y0 = xy[2 * i + 1];
-        if (x0 < 0 || y0 < 0 || x0 + n->crop > cv.W || y0 + n->crop > cv.H)
-            return ctx_fail(n->ctx, FGL_ERR_INVALID, "crop outside the frame");
+        if (x0 < 0 || y0 < 0 || x0 + n->crop > lW || y0 + n->crop > lH)
+            return ctx_fail(n->ctx, FGL_ERR_INVALID, "crop outside the frame's network grid");
         hipLaunchKernelGGL(k_crop, g2(n->crop, n->crop), dim3(256), 0, cv.stream, V, cv.depth, cv.gbuf, cv.ao,
-                           cv.W, cv.H, x0, y0, n->crop, slot0 + i, n->feat, n->target, n->mask);
+                           cv.W, cv.H, f, lW, lH, x0, y0, n->crop, slot0 + i, n->feat, n->target, n->mask);
     }
     return hipGetLastError() == hipSuccess ? FGL_OK : FGL_ERR_DEVICE;
 }
@@ -1115,31 +1226,40 @@ int fgl_nao_read_batch(fgl_nao *n, uint16_t *feat, float *target, uint8_t *mask)
 }
 
 // Neural AO of the context's current frame (depth + G-buffer) into its AO
-// target: features, four MFMA convolutions, sigmoid head.
+// target: down pass, four MFMA convolutions with the head fused into the
+// last, up pass.
 int fgl_nao_infer(fgl_nao *n)
 {
     if (!n) return FGL_ERR_INVALID;
     CtxView cv;
     int rc = ctx_view(n->ctx, &cv);
     if (rc) return rc;
-    const size_t px = (size_t)cv.W * cv.H;
-    if ((rc = nao_reserve(n, px))) return rc;
-    FeatArgs fa;
-    fa.depth = cv.depth;
-    fa.gbuf = cv.gbuf;
-    fa.mask = n->imask;
-    std::memcpy(fa.V.m, cv.view, sizeof(fa.V.m));
+    const int f = n->scale, lW = (cv.W + f - 1) / f, lH = (cv.H + f - 1) / f;
+    if ((rc = nao_reserve(n, (size_t)cv.W * cv.H, (size_t)lW * lH))) return rc;
+    BatchPixView one{};
+    one.W = cv.W;
+    one.H = cv.H;
+    one.lW = lW;
+    one.lH = lH;
+    std::memcpy(one.view, cv.view, sizeof(one.view));
     {
-        // layer 1 with the features computed while staging, layers 2-3, then
-        // layer 4 with the head fused: G-buffer in, AO out
         CtxTimer t(n->ctx, "nao_conv");
-        launch_conv_feat(cv.stream, fa, n->w1, n->params + OFF_B1, n->ia, cv.W, cv.H, layer_dil(0));
-        launch_conv<CH, 0>(cv.stream, n->ia, n->w234, n->params + OFF_B2, nullptr, n->ib, cv.W, cv.H, 1,
-                           layer_dil(1));
-        launch_conv<CH, 0>(cv.stream, n->ib, n->w234 + (size_t)CH * K2, n->params + OFF_B3, nullptr, n->ia, cv.W,
-                           cv.H, 1, layer_dil(2));
+        hipLaunchKernelGGL(k_nao_down<false>, dim3(((lW + 63) / 64) * ((lH + 3) / 4)), dim3(256), 0, cv.stream, one,
+                           (const BatchPixView *)nullptr, (const uint8_t *)nullptr, f, cv.depth, cv.gbuf, n->ifeat,
+                           n->imask, n->fmask);
+        launch_conv<C0, 0>(cv.stream, n->ifeat, n->w1, n->params + OFF_B1, nullptr, n->ia, lW, lH, 1, layer_dil(0));
+        launch_conv<CH, 0>(cv.stream, n->ia, n->w234, n->params + OFF_B2, nullptr, n->ib, lW, lH, 1, layer_dil(1));
+        launch_conv<CH, 0>(cv.stream, n->ib, n->w234 + (size_t)CH * K2, n->params + OFF_B3, nullptr, n->ia, lW, lH,
+                           1, layer_dil(2));
+        // layer 4 with the head fused: the network-grid AO (the frame's AO
+        // itself when the grid is the frame)
+        float *dst = f == 1 ? cv.ao : n->lao;
         launch_conv_head(cv.stream, n->ia, n->w234 + (size_t)2 * CH * K2, n->params + OFF_B4,
-                         HeadArgs{n->params + OFF_W5, n->imask, cv.ao}, cv.W, cv.H, 1, layer_dil(3));
+                         HeadArgs{n->params + OFF_W5, n->imask, dst}, lW, lH, 1, layer_dil(3));
+        if (f > 1)
+            hipLaunchKernelGGL(k_nao_up<false>, dim3(((cv.W + 63) / 64) * ((cv.H + 3) / 4)), dim3(256), 0, cv.stream,
+                               one, (const BatchPixView *)nullptr, (const uint8_t *)nullptr, f, n->lao, n->fmask,
+                               cv.ao);
     }
     ctx_ao_written(n->ctx);
     return hipGetLastError() == hipSuccess ? FGL_OK : FGL_ERR_DEVICE;

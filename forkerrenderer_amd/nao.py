@@ -1,12 +1,18 @@
 """Neural ambient occlusion: dataset rendering, training, evaluation and
 inference driven through the C ABI (fgl_nao_*).
 
-The training pairs are G-buffer/AO frames the plain pipeline renders on the
+The network runs on a grid SCALE (4) times coarser than the frame: each
+network pixel takes the mean features of its 4 x 4 block of frame pixels,
+and the network's AO is interpolated back to the frame pixels.  The
+training pairs are G-buffer/AO frames the plain pipeline renders on the
 GPU: random layouts of displaced icospheres on a ground plane (the 1280x800
 deferred-PBR generator) seen from random camera poses; the target is the
-plain kernel SSAO + sigma=2 Gaussian blur.  A training step takes 16 random
-256x256 crops (4 frames x 4 crops), runs the network's forward and backward
-kernels and one Adam step (lr 1e-3, betas 0.9/0.999, eps 1e-8).
+plain kernel SSAO + sigma=2 Gaussian blur, averaged over each block's
+covered pixels.  A training step takes 16 random crops of 192 x 192 network
+pixels (768 x 768 frame pixels; 4 frames x 4 crops) -- the loss leaves out
+the pixels within the network's reach of a crop edge inside the frame --
+runs the network's forward and backward kernels and one Adam step (lr
+1e-3, betas 0.9/0.999, eps 1e-8).
 """
 import ctypes as C
 import math
@@ -25,7 +31,7 @@ def _bind():
     P = C.c_void_p
     sig = {
         "fgl_nao_param_count": (C.c_int, []),
-        "fgl_nao_create": (C.c_int, [P, C.c_int, C.c_int, C.POINTER(P)]),
+        "fgl_nao_create": (C.c_int, [P, C.c_int, C.c_int, C.c_int, C.POINTER(P)]),
         "fgl_nao_destroy": (None, [P]),
         "fgl_nao_set_state": (C.c_int, [P, P, C.c_int, C.c_int]),
         "fgl_nao_get_state": (C.c_int, [P, P, C.POINTER(C.c_int)]),
@@ -47,14 +53,18 @@ def param_count():
     return _bind().fgl_nao_param_count()
 
 
+SCALE = 4   # frame pixels per network pixel, each way
+NIN = 8     # real network inputs: 7 block-mean features + the covered fraction
+
+
 def init_params(seed):
-    """He-normal convolution weights (layer 1 fan-in counts its 7 real input
-    channels; the 9 padding channels stay zero), zero biases, head N(0, 1/32),
+    """He-normal convolution weights (layer 1 fan-in counts its 8 real input
+    channels; the 8 padding channels stay zero), zero biases, head N(0, 1/32),
     head bias 2 so the untrained AO starts near 0.88."""
     rng = np.random.default_rng(seed)
     ch = 32
     w1 = np.zeros((ch, 9, 16))
-    w1[:, :, :7] = rng.normal(0, math.sqrt(2.0 / 63), (ch, 9, 7))
+    w1[:, :, :NIN] = rng.normal(0, math.sqrt(2.0 / (9 * NIN)), (ch, 9, NIN))
     parts = [w1.reshape(-1), np.zeros(ch)]
     for _ in range(3):
         parts += [rng.normal(0, math.sqrt(2.0 / 288), ch * 288), np.zeros(ch)]
@@ -65,12 +75,15 @@ def init_params(seed):
 
 
 class NeuralAO:
-    def __init__(self, ctx, batch=16, crop=256, seed=0, params=None):
+    """The network on the device.  batch x crop x crop network pixels per
+    training step; scale: frame pixels per network pixel, each way."""
+
+    def __init__(self, ctx, batch=16, crop=192, seed=0, params=None, scale=SCALE):
         L = _bind()
         self.ctx = ctx
-        self.batch, self.crop = batch, crop
+        self.batch, self.crop, self.scale = batch, crop, scale
         self._h = C.c_void_p()
-        rc = L.fgl_nao_create(ctx._h, batch, crop, C.byref(self._h))
+        rc = L.fgl_nao_create(ctx._h, batch, crop, scale, C.byref(self._h))
         if rc != 0:
             raise FglError(f"fgl_nao_create failed: {rc}")
         self.set_params(init_params(seed) if params is None else params)
@@ -143,7 +156,7 @@ class NeuralAO:
 
 # ---------------- training data: the plain pipeline on random scenes ----------------
 
-def ssao_cfg(seed=2, n=64, radius=0.5, sigma=2.0):
+def ssao_cfg(seed=scenegen.SSAO_KERNEL_SEED, n=64, radius=0.5, sigma=2.0):
     """Config-2 SSAO: 64-sample kernel r=0.5, bias 5e-4, range 1e-2, 4x4 noise,
     sigma=2 separable blur (the same sets the .scene keys produce)."""
     import tempfile
@@ -159,10 +172,12 @@ def ssao_cfg(seed=2, n=64, radius=0.5, sigma=2.0):
 
 class SceneSet:
     """Assets of the deferred-PBR scene generator (displaced icosphere of
-    radius 1, ground plane, 2048^2-class albedo / roughness / metalness /
-    normal maps with generated tangents), uploaded once into a context."""
+    radius 1, ground plane, 2048^2 albedo / roughness / metalness / normal
+    maps with generated tangents), uploaded once into a context.  The normal
+    map's size matters: minified without mipmaps, it sets how rough the
+    per-pixel normals are, and with them the SSAO the network learns."""
 
-    def __init__(self, ctx, workdir, subdiv=5, seed=100, tex_size=512):
+    def __init__(self, ctx, workdir, subdiv=5, seed=100, tex_size=2048):
         from . import Scene
         os.makedirs(workdir, exist_ok=True)
         scenegen._pbr_scene_assets(workdir, seed, subdiv, tex_size)
@@ -176,12 +191,14 @@ class SceneSet:
 
 def layout(seed):
     """A random scene, seeded: a ground plane of half-size 4 and displaced
-    icospheres in one of three arrangements -- one large sphere (radius
-    0.8-1.2), 1-6 scattered spheres (radius 0.3-1.0), or an n x n grid
-    (n 3-8, spacing 0.8) of small spheres (radius 0.15-0.45) -- resting on
-    or sunk into the ground."""
+    icospheres in one of four arrangements -- one large sphere (radius
+    0.8-1.2), 1-6 scattered spheres (radius 0.3-1.0), an n x n grid (n 3-8,
+    spacing 0.8) of small spheres (radius 0.15-0.45) resting on or sunk into
+    the ground, or the heavy benchmark scene's family: an n x n grid (n 6-9,
+    spacing 0.8) of spheres of radius 0.3 x [0.5, 1.5] with their centres
+    0.45 above the ground."""
     rng = np.random.default_rng(seed)
-    kind = int(rng.integers(0, 3))
+    kind = int(rng.integers(0, 4))
     objs = []
     if kind == 0:
         r = float(rng.uniform(0.8, 1.2))
@@ -191,13 +208,20 @@ def layout(seed):
             r = float(rng.uniform(0.3, 1.0))
             x, z = rng.uniform(-2.2, 2.2, 2)
             objs.append(((float(x), float(rng.uniform(0.55, 1.0)) * r, float(z)), float(rng.uniform(0, 360)), r))
-    else:
+    elif kind == 2:
         n = int(rng.integers(3, 9))
         for gz in range(n):
             for gx in range(n):
                 r = float(rng.uniform(0.15, 0.45))
                 x, z = (gx - (n - 1) / 2) * 0.8, (gz - (n - 1) / 2) * 0.8
                 objs.append(((x, float(rng.uniform(0.55, 1.6)) * r, z), float(rng.uniform(0, 360)), r))
+    else:
+        n = int(rng.integers(6, 10))
+        for gz in range(n):
+            for gx in range(n):
+                r = 0.3 * float(rng.uniform(0.5, 1.5))
+                x, z = (gx - (n - 1) / 2) * 0.8, (gz - (n - 1) / 2) * 0.8
+                objs.append(((x, 0.45, z), float(rng.uniform(0, 360)), r))
     ground = make_model_matrix((0.0, 0.0, 0.0), 0.0, 4.0)
     return ground, [make_model_matrix(t, rot, r) for t, rot, r in objs]
 
@@ -237,6 +261,12 @@ def render_frame(ctx, sset, frame_seed, W, H, cfg, plain_ao=True):
         ctx.ssao(cfg)
 
 
+def frame_size(frame_seed):
+    """Training frames: one pose in four of every layout is rendered at
+    1920 x 1080, the others at 1280 x 800."""
+    return (1920, 1080) if (frame_seed % 1000) % 4 == 3 else (1280, 800)
+
+
 def frame_seeds(split, n):
     """Frame seeds layout * 1000 + pose.  Training: layouts 0..399, frame i
     is layout i % 400, pose i // 400 (4000 frames = 10 poses each).
@@ -247,17 +277,20 @@ def frame_seeds(split, n):
     return [(1000 + i % 20) * 1000 + i // 20 for i in range(n)]
 
 
-def step_batch(step, seed, frames, W, H, crop, batch):
+def step_batch(step, seed, frames, crop, batch, scale=SCALE):
     """The frames and crop origins of training step `step`: a pure function
     of (seed, step), so a run resumed from a checkpoint sees the same data.
-    4 frames per step, batch/4 crops each."""
+    4 frames per step, batch/4 crops each; crop origins in network pixels of
+    each frame's network grid (frame_size / scale)."""
     order = np.random.default_rng(seed).permutation(frame_seeds("train", frames))
     rng = np.random.default_rng([seed, step])
     per = batch // 4
     out = []
     for j in range(4):
         fs = int(order[(4 * step + j) % len(order)])
-        xy = np.stack([rng.integers(0, W - crop + 1, per), rng.integers(0, H - crop + 1, per)], 1)
+        W, H = frame_size(fs)
+        lW, lH = -(-W // scale), -(-H // scale)
+        xy = np.stack([rng.integers(0, lW - crop + 1, per), rng.integers(0, lH - crop + 1, per)], 1)
         out.append((fs, xy))
     return out
 
@@ -271,17 +304,17 @@ def lr_at(step, total, base=1e-3):
     return base * (1.0 - 0.99 * min(1.0, (step - t0) / max(total - t0, 1.0)))
 
 
-def train(nao, ctx, sset, steps, start=0, total=None, seed=0, W=1280, H=800, frames=4000, lr=1e-3, log=None,
-          cfg=None):
+def train(nao, ctx, sset, steps, start=0, total=None, seed=0, frames=4000, lr=1e-3, log=None, cfg=None):
     """Steps start .. start+steps-1 of a training run of `total` steps
     (default start+steps); returns per-step (sum sq err, sum |err|, covered
-    pixels)."""
+    network pixels)."""
     cfg = cfg if cfg is not None else ssao_cfg()
     total = start + steps if total is None else total
     per = nao.batch // 4
     hist = []
     for step in range(start, start + steps):
-        for j, (fs, xy) in enumerate(step_batch(step, seed, frames, W, H, nao.crop, nao.batch)):
+        for j, (fs, xy) in enumerate(step_batch(step, seed, frames, nao.crop, nao.batch, nao.scale)):
+            W, H = frame_size(fs)
             render_frame(ctx, sset, fs, W, H, cfg)
             nao.stage(j * per, xy)
         sse, sae, cnt = nao.train_step(lr_at(step, total, lr))
@@ -302,13 +335,16 @@ def save_checkpoint(path, nao, seed, frames, W=1280, H=800, evaluation=None):
     st, step = nao.state()
     np.savez(path, state=st, step=np.int64(step), seed=np.int64(seed), frames=np.int64(frames),
              size=np.array([W, H], np.int64), crop=np.int64(nao.crop), batch=np.int64(nao.batch),
-             nparam=np.int64(param_count()), eval_json=np.array(json.dumps(evaluation or {})))
+             scale=np.int64(nao.scale), nparam=np.int64(param_count()),
+             eval_json=np.array(json.dumps(evaluation or {})))
 
 
 def load_checkpoint(path):
     z = np.load(path)
     if int(z["nparam"]) != param_count():
         raise FglError(f"{path}: parameter count {int(z['nparam'])} != {param_count()}")
+    if "scale" not in z.files:
+        raise FglError(f"{path}: a checkpoint of the full-resolution network (no network scale recorded)")
     return {k: z[k] for k in z.files}
 
 

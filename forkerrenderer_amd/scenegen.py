@@ -195,6 +195,10 @@ def pbr_textures(d, seed, size):
     write_tga(os.path.join(d, "normal.tga"), np.round((n + 1) / 2 * 255).astype(np.uint8))
 
 
+# Seed of the 64-sample SSAO kernel and 4 x 4 noise of the deferred configs:
+# the heavy 1080p configuration uses the 1280 x 800 configuration's SSAO.
+SSAO_KERNEL_SEED = 2
+
 PBR_MTL = [("Ka", "1 1 1"), ("Kd", "1 1 1"), ("Ks", "0.5 0.5 0.5"), ("map_Kd", "albedo.tga"),
            ("map_Pr", "roughness.tga"), ("map_Pm", "metalness.tga"), ("map_Bump", "normal.tga")]
 
@@ -255,7 +259,7 @@ ssao on
 shadow on
 shadowmap 2048
 shadowfilter grid 5
-ssaokernel 64 0.5 0.0005 0.01 {seed}
+ssaokernel 64 0.5 0.0005 0.01 {SSAO_KERNEL_SEED}
 aoblur gaussian 2
 light point 2 5 5 3 3 3
 camera persp {eye[0]} {eye[1]} {eye[2]} {at[0]} {at[1]} {at[2]}
@@ -292,7 +296,7 @@ ssao on
 shadow on
 shadowmap 2048
 shadowfilter pcss 16 32 0.05
-ssaokernel 64 0.5 0.0005 0.01 {seed}
+ssaokernel 64 0.5 0.0005 0.01 {SSAO_KERNEL_SEED}
 aoblur gaussian 2
 multilight on
 light point 2 5 5 2.5 2.5 2.5

@@ -319,19 +319,25 @@ int fgl_render_scene(fgl_ctx *ctx, fgl_scene *s);
 
 /* ---- neural ambient occlusion (new; no counterpart in the reference) ----
  * A BF16 convolutional network that replaces Render::DoSSAO + blur
- * (src/render.cpp:214-289): 7 view-space features -> 4 x conv3x3 (32 ch,
- * ReLU) -> conv1x1 + sigmoid.  Training runs on the device (forward,
- * backward, Adam) on crops of frames the plain pipeline rendered. */
+ * (src/render.cpp:214-289).  It runs on a grid `scale` times coarser than
+ * the frame (1, 2 or 4): per network pixel, 8 inputs from its scale x scale
+ * block (the mean view-space depth, normal and position of the covered
+ * pixels, and the covered fraction) -> 4 x conv3x3 (32 ch, ReLU, dilated
+ * 1/2/4/8) -> conv1x1 + sigmoid, interpolated bilinearly back to the covered
+ * frame pixels (AO 1 elsewhere).  Training runs on the device (forward,
+ * backward, Adam) on network-grid crops of frames the plain pipeline
+ * rendered, against the block means of their AO. */
 typedef struct fgl_nao fgl_nao;
 int fgl_nao_param_count(void);
-int fgl_nao_create(fgl_ctx *ctx, int batch, int crop, fgl_nao **out);
+/* batch crops of crop x crop network pixels per training step */
+int fgl_nao_create(fgl_ctx *ctx, int batch, int crop, int scale, fgl_nao **out);
 void fgl_nao_destroy(fgl_nao *n);
 /* state = [params | Adam m | Adam v] (3 * param_count floats); with_adam = 0
  * loads params only and resets Adam */
 int fgl_nao_set_state(fgl_nao *n, const float *state, int with_adam, int step);
 int fgl_nao_get_state(fgl_nao *n, float *state, int *step);
-/* crops (origins xy[2*i]) of the context's current frame -- depth, G-buffer
- * and AO target -- into batch slots slot0.. */
+/* crops (origins xy[2*i], in network pixels) of the context's current frame
+ * -- depth, G-buffer and AO target -- into batch slots slot0.. */
 int fgl_nao_stage_crops(fgl_nao *n, int slot0, int count, const int *xy);
 /* one Adam step on the staged batch; stats = [sum sq err, sum |err|, covered] */
 int fgl_nao_train_step(fgl_nao *n, float lr, float *stats);

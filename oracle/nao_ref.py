@@ -1,9 +1,13 @@
 """CPU oracle of the neural-AO network (TEST INFRASTRUCTURE).
 
+The network runs on a grid `f` times coarser than the frame; block_inputs,
+block_targets and upsample restate the device's down pass, training
+targets and up pass in its float32 operation order.
+
 Forward and backward of: conv3x3 16->32 ReLU, 3x conv3x3 32->32 ReLU,
 conv1x1 32->1 sigmoid, 'same' zero padding, the four 3x3 convolutions
-dilated by DIL = (1, 2, 4, 8) (taps at offsets -d, 0, +d: receptive field
-31 x 31), loss = sum over covered pixels of (y - t)^2 / n_pixels.  Sums are evaluated in float64.  With bf16=True the
+dilated by DIL = (1, 2, 6, 12) (taps at offsets -d, 0, +d: receptive field
+43 x 43), loss = sum over covered pixels of (y - t)^2 / n_pixels.  Sums are evaluated in float64.  With bf16=True the
 values the device keeps in BF16 (weights of layers 1-4, activations, data
 gradients) are rounded to BF16 at the same points; with bf16=False the
 network is plain float64, which the finite-difference test uses to pin the
@@ -20,7 +24,7 @@ K1, K2 = 9 * C0, 9 * CH
 SIZES = [("W1", CH * K1), ("b1", CH), ("W2", CH * K2), ("b2", CH), ("W3", CH * K2), ("b3", CH),
          ("W4", CH * K2), ("b4", CH), ("W5", CH), ("b5", 1)]
 NPARAM = sum(s for _, s in SIZES)
-DIL = (1, 2, 4, 8)
+DIL = (1, 2, 6, 12)
 
 
 def split(params):
@@ -119,15 +123,110 @@ def loss_and_grads(params, feat, target, mask, bf16=True):
     return loss, flat, stats
 
 
+NIN = 8   # real inputs: 7 block-mean features + the covered fraction
+
+
 def init_params(seed):
-    """He-normal init for the convolutions (fan-in counts the 7 real input
+    """He-normal init for the convolutions (fan-in counts the 8 real input
     channels of layer 1), small head, b5 = 2 (AO starts near 0.88)."""
     rng = np.random.default_rng(seed)
     parts = []
     w1 = np.zeros((CH, 9, C0))
-    w1[:, :, :7] = rng.normal(0, np.sqrt(2.0 / (9 * 7)), (CH, 9, 7))
+    w1[:, :, :NIN] = rng.normal(0, np.sqrt(2.0 / (9 * NIN)), (CH, 9, NIN))
     parts += [w1.reshape(-1), np.zeros(CH)]
     for _ in range(3):
         parts += [rng.normal(0, np.sqrt(2.0 / K2), CH * K2), np.zeros(CH)]
     parts += [rng.normal(0, np.sqrt(1.0 / CH), CH), np.array([2.0])]
     return np.concatenate(parts).astype(np.float32)
+
+
+# ---------------- down pass, targets, up pass (float32, device order) ----------------
+
+def _blocks(W, H, f):
+    return -(-W // f), -(-H // f)
+
+
+def block_inputs(feat, cov, f):
+    """Network inputs from per-pixel features.  feat: (H, W, 7) float32 pixel
+    features (zero where uncovered), cov: (H, W) bool.  Per network pixel
+    (X, Y): the covered pixels of its block summed in row-major order from
+    0, times 1/n (n covered); channel 7 = n / (block pixels in the frame).
+    Returns (lH, lW, 16) float32 and the network coverage (n > 0)."""
+    H, W = cov.shape
+    lW, lH = _blocks(W, H, f)
+    f32 = np.float32
+    acc = np.zeros((lH, lW, 7), f32)
+    n = np.zeros((lH, lW), np.int64)
+    m = np.zeros((lH, lW), np.int64)
+    for dy in range(f):
+        for dx in range(f):
+            ys, xs = np.arange(lH) * f + dy, np.arange(lW) * f + dx
+            yv, xv = ys < H, xs < W
+            sub = np.zeros((lH, lW, 7), f32)
+            c = np.zeros((lH, lW), bool)
+            yy, xx = ys[yv], xs[xv]
+            sub[np.ix_(yv, xv)] = feat[np.ix_(yy, xx)]
+            c[np.ix_(yv, xv)] = cov[np.ix_(yy, xx)]
+            acc = np.where(c[..., None], acc + sub, acc)
+            n += c
+            m[np.ix_(yv, xv)] += 1
+    inv = np.where(n > 0, f32(1) / np.maximum(n, 1).astype(f32), f32(0))
+    out = np.zeros((lH, lW, 16), f32)
+    out[..., :7] = acc * inv[..., None]
+    out[..., 7] = np.where(m > 0, n.astype(f32) / np.maximum(m, 1).astype(f32), f32(0))
+    return out, n > 0
+
+
+def block_targets(ao, cov, f):
+    """Training targets: per network pixel the covered pixels' AO summed in
+    row-major order from 0, times 1/n; 1 where no pixel is covered."""
+    H, W = cov.shape
+    lW, lH = _blocks(W, H, f)
+    f32 = np.float32
+    acc = np.zeros((lH, lW), f32)
+    n = np.zeros((lH, lW), np.int64)
+    for dy in range(f):
+        for dx in range(f):
+            ys, xs = np.arange(lH) * f + dy, np.arange(lW) * f + dx
+            yv, xv = ys < H, xs < W
+            a = np.zeros((lH, lW), f32)
+            c = np.zeros((lH, lW), bool)
+            a[np.ix_(yv, xv)] = ao[np.ix_(ys[yv], xs[xv])]
+            c[np.ix_(yv, xv)] = cov[np.ix_(ys[yv], xs[xv])]
+            acc = np.where(c, acc + a, acc)
+            n += c
+    inv = np.where(n > 0, f32(1) / np.maximum(n, 1).astype(f32), f32(0))
+    return np.where(n > 0, acc * inv, f32(1)), n > 0
+
+
+def upsample(lao, cov, f):
+    """Frame AO from the network grid's AO: at covered pixel (x, y), bilinear
+    weights at u = (x + 0.5)/f - 0.5, w = (y + 0.5)/f - 0.5 between the
+    clamped neighbours floor(u), floor(u) + 1 (and in y); 1 elsewhere."""
+    H, W = cov.shape
+    lH, lW = lao.shape
+    f32 = np.float32
+    inv = f32(1) / f32(f)
+    u = (np.arange(W, dtype=f32) + f32(0.5)) * inv - f32(0.5)
+    w = (np.arange(H, dtype=f32) + f32(0.5)) * inv - f32(0.5)
+    fu, fw = np.floor(u), np.floor(w)
+    tx, ty = (u - fu).astype(f32), (w - fw).astype(f32)
+    xa, xb = np.clip(fu.astype(int), 0, lW - 1), np.clip(fu.astype(int) + 1, 0, lW - 1)
+    ya, yb = np.clip(fw.astype(int), 0, lH - 1), np.clip(fw.astype(int) + 1, 0, lH - 1)
+    L = lao.astype(f32)
+    top = L[ya][:, xa] * (f32(1) - tx)[None, :] + L[ya][:, xb] * tx[None, :]
+    bot = L[yb][:, xa] * (f32(1) - tx)[None, :] + L[yb][:, xb] * tx[None, :]
+    out = top * (f32(1) - ty)[:, None] + bot * ty[:, None]
+    return np.where(cov, out, f32(1)).astype(f32)
+
+
+REACH = sum(DIL)   # receptive-field radius in network pixels
+
+
+def crop_loss_mask(cov, x0, y0, S, lW, lH):
+    """Training mask of a crop: covered, and at least REACH pixels inside
+    every crop edge that is not an edge of the lW x lH network grid."""
+    cy, cx = np.mgrid[0:S, 0:S]
+    inside = ((x0 == 0) | (cx >= REACH)) & ((x0 + S == lW) | (cx < S - REACH)) & \
+             ((y0 == 0) | (cy >= REACH)) & ((y0 + S == lH) | (cy < S - REACH))
+    return cov & inside

@@ -1,6 +1,8 @@
 """Neural AO on the GPU against the CPU network oracle (oracle/nao_ref.py):
-staged features, forward, gradients, the Adam step, deterministic training,
-checkpoint resume, and the audit fallback."""
+the down pass (block inputs) and staged training crops, forward through the
+up pass, gradients, the Adam step, deterministic training, checkpoint
+resume, and the audit fallback.  The network runs on the 4x coarser grid
+(512 x 384 frame -> 128 x 96 network pixels)."""
 import numpy as np
 import pytest
 
@@ -10,13 +12,15 @@ from oracle import nao_ref as R
 
 pytestmark = pytest.mark.gpu
 
-W, H, CROP, BATCH = 320, 256, 64, 4
-XY = [(0, 0), (256, 192), (131, 70), (40, 150)]
+W, H, CROP, BATCH = 512, 384, 64, 4
+S = NA.SCALE
+LW, LH = W // S, H // S
+XY = [(0, 0), (64, 32), (31, 17), (10, 30)]    # crop origins on the network grid
 
 
 @pytest.fixture(scope="module")
 def world(ctx, tmp_path_factory):
-    sset = NA.SceneSet(ctx, str(tmp_path_factory.mktemp("nao")), subdiv=4)
+    sset = NA.SceneSet(ctx, str(tmp_path_factory.mktemp("nao")), subdiv=4, tex_size=256)
     cfg = NA.ssao_cfg()
     NA.render_frame(ctx, sset, 5003, W, H, cfg)
     frame = {"depth": ctx.read(F.TARGET_DEPTH).reshape(H, W),
@@ -63,9 +67,18 @@ def features_f32(frame):
     return out, cov
 
 
+def net_inputs(frame):
+    """The network grid's inputs and coverage (oracle down pass)."""
+    feat, cov = features_f32(frame)
+    x, lcov = R.block_inputs(feat[..., :7], cov, S)
+    return x, lcov, cov
+
+
 def test_nonfinite_gbuffer_pixels_are_uncovered(ctx, world):
-    """A NaN normal or position in the G-buffer gives zero features and mask
-    0 (it must not reach the weight gradients), and AO 1 at inference."""
+    """A NaN normal or position in the G-buffer makes the pixel uncovered: it
+    takes no part in its block's inputs or target (the staged crop equals
+    the oracle's block inputs with the pixel uncovered), the gradients stay
+    finite, and its AO at inference is 1."""
     sset, cfg, frame = world
     NA.render_frame(ctx, sset, 5003, W, H, cfg)
     g = ctx.read(F.TARGET_GBUFFER).reshape(F.GBUF_PLANES, H, W).copy()
@@ -76,15 +89,22 @@ def test_nonfinite_gbuffer_pixels_are_uncovered(ctx, world):
     g[4, picks[1][0], picks[1][1]] = np.inf      # position y
     g[2, picks[2][0], picks[2][1]] = np.nan      # normal z
     ctx.write(F.TARGET_GBUFFER, g)
+    bad = dict(frame)
+    bad["gbuf"] = g
+    x, lcov, cov = net_inputs(bad)
+    for y, xx in picks:
+        assert not cov[y, xx]
     n = NA.NeuralAO(ctx, batch=1, crop=64, params=_params())
     n.infer()
     ao = ctx.read(F.TARGET_AO).reshape(H, W)
-    for y, x in picks:
-        assert ao[y, x] == 1.0
-        x0, y0 = min(max(x - 32, 0), W - 64), min(max(y - 32, 0), H - 64)
+    for y, xx in picks:
+        assert ao[y, xx] == 1.0
+        X, Y = xx // S, y // S
+        x0, y0 = min(max(X - 32, 0), LW - 64), min(max(Y - 32, 0), LH - 64)
         n.stage(0, [(x0, y0)])
         feat, _, mask = n.read_batch()
-        assert mask[0, y - y0, x - x0] == 0 and not feat[0, y - y0, x - x0].any()
+        assert np.array_equal(feat[0], bf16_bits(x[y0:y0 + 64, x0:x0 + 64]))
+        assert np.array_equal(mask[0].astype(bool), R.crop_loss_mask(lcov[y0:y0 + 64, x0:x0 + 64], x0, y0, 64, LW, LH))
     g_grad, st = n.gradients()
     assert np.all(np.isfinite(g_grad))
     n.close()
@@ -107,15 +127,20 @@ def staged(ctx, world):
 
 
 def test_staged_crops_match_gbuffer(world, staged):
+    """Crops of the network grid: inputs (block means of the pixel features
+    and the covered fraction, in BF16), targets (block means of the plain
+    AO over covered pixels) and mask, bit for bit against the oracle."""
     _, _, frame = world
     _, feat, tgt, mask = staged
-    ref, cov = features_f32(frame)
-    refb = bf16_bits(ref)
-    for i, (x, y) in enumerate(XY):
-        sl = (slice(y, y + CROP), slice(x, x + CROP))
+    x, lcov, cov = net_inputs(frame)
+    t, _ = R.block_targets(frame["ao"], cov, S)
+    refb = bf16_bits(x)
+    assert 0.005 < (x[..., 7][lcov] < 1).mean() < 0.5, "blocks on silhouettes should be partly covered"
+    for i, (cx, cy) in enumerate(XY):
+        sl = (slice(cy, cy + CROP), slice(cx, cx + CROP))
         assert np.array_equal(feat[i], refb[sl]), f"crop {i}"
-        assert np.array_equal(tgt[i], frame["ao"][sl])
-        assert np.array_equal(mask[i].astype(bool), cov[sl])
+        assert np.array_equal(tgt[i].view(np.uint32), t[sl].view(np.uint32)), f"crop {i} target"
+        assert np.array_equal(mask[i].astype(bool), R.crop_loss_mask(lcov[sl], cx, cy, CROP, LW, LH))
 
 
 def test_gradients_match_oracle(staged):
@@ -145,12 +170,13 @@ def test_forward_matches_oracle(ctx, world, staged):
     NA.render_frame(ctx, sset, 5003, W, H, cfg)
     net.infer()
     y = ctx.read(F.TARGET_AO).reshape(H, W)
-    ref, cov = features_f32(frame)
-    featf = R.bf16_bits_to_float(bf16_bits(ref)).reshape(1, H, W, 16)
+    x, lcov, cov = net_inputs(frame)
+    featf = R.bf16_bits_to_float(bf16_bits(x)).reshape(1, LH, LW, 16)
     _, yr = R.forward(p, featf, bf16=True)
-    yr = yr[0]
+    low = np.where(lcov, yr[0], 1.0).astype(np.float32)
+    want = R.upsample(low, cov, S)
     assert np.all(y[~cov] == 1.0)
-    err = np.abs(y - yr)[cov]
+    err = np.abs(y - want)[cov]
     assert err.max() < 2e-2 and err.mean() < 2e-4, (err.max(), err.mean())
 
 

@@ -14,16 +14,22 @@ def test_param_layout_matches_oracle():
 
 
 def test_step_batch_is_a_function_of_seed_and_step():
-    a = NA.step_batch(17, 3, 4000, 1280, 800, 256, 16)
-    b = NA.step_batch(17, 3, 4000, 1280, 800, 256, 16)
-    c = NA.step_batch(18, 3, 4000, 1280, 800, 256, 16)
+    a = NA.step_batch(17, 3, 4000, 128, 16)
+    b = NA.step_batch(17, 3, 4000, 128, 16)
+    c = NA.step_batch(18, 3, 4000, 128, 16)
     for (fa, xa), (fb, xb) in zip(a, b):
         assert fa == fb and np.array_equal(xa, xb)
     assert [f for f, _ in a] != [f for f, _ in c]
-    for _, xy in a:
-        assert xy.shape == (4, 2)
-        assert (xy[:, 0] >= 0).all() and (xy[:, 0] <= 1280 - 256).all()
-        assert (xy[:, 1] >= 0).all() and (xy[:, 1] <= 800 - 256).all()
+    sizes = set()
+    for step in range(40):
+        for fs, xy in NA.step_batch(step, 3, 4000, 128, 16):
+            W, H = NA.frame_size(fs)
+            sizes.add((W, H))
+            lW, lH = W // NA.SCALE, H // NA.SCALE
+            assert xy.shape == (4, 2)
+            assert (xy[:, 0] >= 0).all() and (xy[:, 0] <= lW - 128).all()
+            assert (xy[:, 1] >= 0).all() and (xy[:, 1] <= lH - 128).all()
+    assert sizes == {(1280, 800), (1920, 1080)}
 
 
 def test_train_and_eval_frames_are_disjoint():

@@ -77,3 +77,73 @@ def test_dilated_convolution_against_direct_sum():
                             t = (dy + 1) * 3 + dx + 1
                             want[yy, xx] += Wt[:, t * 3:(t + 1) * 3] @ x[0, sy, sx]
         assert np.allclose(got, want, atol=1e-12), dil
+
+
+def test_block_inputs_and_targets_are_covered_block_means():
+    """Down pass and training targets against float64: the mean of each
+    block's covered pixels (7 features, AO), the covered fraction of the
+    block's pixels inside the frame (partial blocks at the right and top
+    edges), zero inputs and target 1 for blocks with nothing covered."""
+    rng = np.random.default_rng(31)
+    H, W, f = 37, 50, 4                       # partial blocks on both edges
+    feat = rng.normal(0, 1, (H, W, 7)).astype(np.float32)
+    cov = rng.random((H, W)) < 0.7
+    cov[0:4, 0:4] = False                     # one empty block
+    feat[~cov] = 0
+    ao = rng.uniform(0, 1, (H, W)).astype(np.float32)
+    x, lcov = R.block_inputs(feat, cov, f)
+    t, tcov = R.block_targets(ao, cov, f)
+    lH, lW = x.shape[:2]
+    assert (lH, lW) == (10, 13) and np.array_equal(lcov, tcov)
+    for Y in range(lH):
+        for X in range(lW):
+            blk = (slice(Y * f, min(Y * f + f, H)), slice(X * f, min(X * f + f, W)))
+            c = cov[blk]
+            npx = c.size
+            if not c.any():
+                assert not lcov[Y, X] and not x[Y, X].any() and t[Y, X] == 1.0
+                continue
+            m = feat[blk][c].astype(np.float64).mean(0)
+            assert np.allclose(x[Y, X, :7], m, rtol=1e-6, atol=1e-6)
+            assert x[Y, X, 7] == np.float32(c.sum()) / np.float32(npx)
+            assert abs(t[Y, X] - ao[blk][c].astype(np.float64).mean()) < 1e-6
+    assert not x[..., 8:].any()
+
+
+def test_upsample_is_bilinear_between_block_centres():
+    """Up pass: a linear field a + b X + c Y on the network grid comes back
+    as a + b u + c w at frame pixel (x, y), u = (x + 0.5)/f - 0.5, wherever
+    both neighbours exist (a wrong half-pixel offset or axis swap fails);
+    at the edges the nearest block's value; uncovered pixels 1."""
+    f, lW, lH = 4, 9, 6
+    W, H = lW * f, lH * f
+    Y, X = np.mgrid[0:lH, 0:lW]
+    a, b, c = 0.3, 0.05, -0.03
+    lao = (a + b * X + c * Y).astype(np.float32)
+    cov = np.ones((H, W), bool)
+    cov[5, 7] = False
+    up = R.upsample(lao, cov, f)
+    assert up[5, 7] == 1.0
+    for y in range(H):
+        for x in range(W):
+            if not cov[y, x]:
+                continue
+            u, w = (x + 0.5) / f - 0.5, (y + 0.5) / f - 0.5
+            uc, wc = min(max(u, 0), lW - 1), min(max(w, 0), lH - 1)
+            assert abs(up[y, x] - (a + b * uc + c * wc)) < 2e-6, (x, y)
+    const = R.upsample(np.full((lH, lW), 0.625, np.float32), cov, f)
+    assert np.all(const[cov] == np.float32(0.625))
+
+
+def test_crop_loss_mask_keeps_pixels_the_crop_sees_whole():
+    """Crop pixels within the network's reach (sum of the dilations) of a
+    crop edge inside the grid are left out of the loss; edges on the grid's
+    own border keep their pixels (the whole frame pads there too)."""
+    assert R.REACH == 21
+    cov = np.ones((64, 64), bool)
+    m = R.crop_loss_mask(cov, 10, 10, 64, 200, 100)          # every edge interior
+    assert m.sum() == (64 - 42) ** 2 and m[21, 21] and not m[20, 30] and not m[30, 43]
+    m = R.crop_loss_mask(cov, 0, 36, 64, 200, 100)           # left and top edges on the grid border
+    assert m[:, 0].any() and m[63, 30] and not m[0, 30] and not m[30, 63]
+    cov[40, 40] = False
+    assert not R.crop_loss_mask(cov, 0, 36, 64, 200, 100)[40, 40]
