@@ -108,6 +108,56 @@ struct Mat16 {
 // Returns the covered count.  Rows of four pixels are read as 16-byte loads
 // when the block is four wide, inside the frame and the rows are 16-byte
 // aligned.
+// A whole 4 x 4 block with 16-byte aligned rows: the 28 (32 with AO) row
+// loads are all issued before the first is used; the same sums, in the same
+// order, as the general path below.
+__device__ __forceinline__ int block_features_4x4(const float *V, const float *__restrict__ depth,
+                                                  const float *__restrict__ g, size_t P, int W, size_t r0,
+                                                  float feat[NIN], uint8_t *__restrict__ fmask,
+                                                  const float *__restrict__ ao, float *ao_mean)
+{
+    float4 q[4][7], qa[4];
+#pragma unroll
+    for (int dy = 0; dy < 4; ++dy) {
+        const size_t r = r0 + (size_t)dy * W;
+        q[dy][0] = *(const float4 *)(depth + r);
+#pragma unroll
+        for (int k = 0; k < 6; ++k) q[dy][k + 1] = *(const float4 *)(g + (size_t)k * P + r);
+        if (ao) qa[dy] = *(const float4 *)(ao + r);
+    }
+    float acc[7] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+    float aos = 0.f;
+    int n = 0;
+#pragma unroll
+    for (int dy = 0; dy < 4; ++dy) {
+        uint32_t mk = 0;
+        const float *c[7];
+        float a[4] = {qa[dy].x, qa[dy].y, qa[dy].z, qa[dy].w};
+#pragma unroll
+        for (int k = 0; k < 7; ++k) c[k] = &q[dy][k].x;
+#pragma unroll
+        for (int dx = 0; dx < 4; ++dx) {
+            float pf[7];
+            const bool cov = features_of(V, c[0][dx], vec3(c[1][dx], c[2][dx], c[3][dx]),
+                                         vec3(c[4][dx], c[5][dx], c[6][dx]), pf);
+            if (cov) {
+                ++n;
+#pragma unroll
+                for (int k = 0; k < 7; ++k) acc[k] += pf[k];
+                if (ao) aos += a[dx];
+            }
+            mk |= (cov ? 1u : 0u) << (8 * dx);
+        }
+        if (fmask) *(uint32_t *)(fmask + r0 + (size_t)dy * W) = mk;
+    }
+    const float inv = n ? 1.f / (float)n : 0.f;
+#pragma unroll
+    for (int k = 0; k < 7; ++k) feat[k] = acc[k] * inv;
+    feat[7] = (float)n / 16.f;
+    if (ao_mean) *ao_mean = n ? aos * inv : 1.f;
+    return n;
+}
+
 __device__ __forceinline__ int block_features(const float *V, const float *__restrict__ depth,
                                               const float *__restrict__ g, int W, int H, int X, int Y, int f,
                                               float feat[NIN], uint8_t *__restrict__ fmask,
@@ -123,6 +173,8 @@ __device__ __forceinline__ int block_features(const float *V, const float *__res
     const bool vec4 = f == 4 && (W & 3) == 0 && x0 + 3 < W && ((uintptr_t)depth & 15) == 0 &&
                       ((uintptr_t)g & 15) == 0 && (!ao || ((uintptr_t)ao & 15) == 0) &&
                       (!fmask || ((uintptr_t)fmask & 3) == 0);
+    if (vec4 && Y * 4 + 3 < H)
+        return block_features_4x4(V, depth, g, P, W, (size_t)(Y * 4) * W + x0, feat, fmask, ao, ao_mean);
     for (int dy = 0; dy < f; ++dy) {
         const int y = Y * f + dy;
         if (y >= H) break;
@@ -225,24 +277,11 @@ __global__ __launch_bounds__(256) void k_nao_down(BatchPixView one, const BatchP
 // Up pass: AO of every frame pixel -- 1 where the pixel is uncovered, else
 // the network AO interpolated bilinearly between the centres of the blocks
 // around the pixel's centre (block (X, Y) has its centre at frame
-// coordinate f X + f/2; edges clamp).  One thread per frame pixel, 64 x 4
-// frame pixels per workgroup.
-template <bool BATCH>
-__global__ __launch_bounds__(256) void k_nao_up(BatchPixView one, const BatchPixView *__restrict__ views,
-                                                const uint8_t *__restrict__ blk_view, int f,
-                                                const float *__restrict__ lao, const uint8_t *__restrict__ fmask,
-                                                float *__restrict__ ao)
+// coordinate f X + f/2; edges clamp).  64 threads per 64 x 4 frame-pixel
+// block, four consecutive pixels of a row each (one 4-byte mask load and
+// one 16-byte store where the row is aligned).
+__device__ __forceinline__ float up_value(const BatchPixView &v, const float *__restrict__ L, int f, int x, int y)
 {
-    const BatchPixView &v = BATCH ? view_of_block(views, blk_view, blockIdx.x) : one;
-    const int k = BATCH ? (int)(blockIdx.x - v.blk_base) : (int)blockIdx.x;
-    const int bx = (v.W + 63) / 64;
-    const int x = (k % bx) * 64 + (int)(threadIdx.x & 63), y = (k / bx) * 4 + (int)(threadIdx.x >> 6);
-    if (x >= v.W || y >= v.H) return;
-    const size_t i = v.px_off + (size_t)y * v.W + x;
-    if (!fmask[i]) {
-        ao[i] = 1.f;
-        return;
-    }
     const float inv = 1.f / (float)f;
     const float u = ((float)x + 0.5f) * inv - 0.5f, w = ((float)y + 0.5f) * inv - 0.5f;
     const float fu = floorf(u), fw = floorf(w);
@@ -250,10 +289,35 @@ __global__ __launch_bounds__(256) void k_nao_up(BatchPixView one, const BatchPix
     const int X0 = (int)fu, Y0 = (int)fw;
     const int xa = imax(0, imin(v.lW - 1, X0)), xb = imax(0, imin(v.lW - 1, X0 + 1));
     const int ya = imax(0, imin(v.lH - 1, Y0)), yb = imax(0, imin(v.lH - 1, Y0 + 1));
-    const float *L = lao + v.lo_off;
     const float top = L[(size_t)ya * v.lW + xa] * (1.f - tx) + L[(size_t)ya * v.lW + xb] * tx;
     const float bot = L[(size_t)yb * v.lW + xa] * (1.f - tx) + L[(size_t)yb * v.lW + xb] * tx;
-    ao[i] = top * (1.f - ty) + bot * ty;
+    return top * (1.f - ty) + bot * ty;
+}
+
+template <bool BATCH>
+__global__ __launch_bounds__(64) void k_nao_up(BatchPixView one, const BatchPixView *__restrict__ views,
+                                               const uint8_t *__restrict__ blk_view, int f,
+                                               const float *__restrict__ lao, const uint8_t *__restrict__ fmask,
+                                               float *__restrict__ ao)
+{
+    const BatchPixView &v = BATCH ? view_of_block(views, blk_view, blockIdx.x) : one;
+    const int k = BATCH ? (int)(blockIdx.x - v.blk_base) : (int)blockIdx.x;
+    const int bx = (v.W + 63) / 64;
+    const int x = (k % bx) * 64 + 4 * (int)(threadIdx.x & 15), y = (k / bx) * 4 + (int)(threadIdx.x >> 4);
+    if (x >= v.W || y >= v.H) return;
+    const size_t i = v.px_off + (size_t)y * v.W + x;
+    const float *L = lao + v.lo_off;
+    if (x + 3 < v.W && (i & 3) == 0) {
+        const uint32_t mk = *(const uint32_t *)(fmask + i);
+        float4 o;
+        o.x = (mk & 0xffu) ? up_value(v, L, f, x, y) : 1.f;
+        o.y = (mk & 0xff00u) ? up_value(v, L, f, x + 1, y) : 1.f;
+        o.z = (mk & 0xff0000u) ? up_value(v, L, f, x + 2, y) : 1.f;
+        o.w = (mk & 0xff000000u) ? up_value(v, L, f, x + 3, y) : 1.f;
+        *(float4 *)(ao + i) = o;
+        return;
+    }
+    for (int e = 0; e < 4 && x + e < v.W; ++e) ao[i + e] = fmask[i + e] ? up_value(v, L, f, x + e, y) : 1.f;
 }
 
 // Receptive-field radius of the network in network pixels (sum of the
@@ -991,7 +1055,7 @@ int nao_infer_batch(fgl_nao *n, hipStream_t s, const NaoBatchTables &t, const fl
     launch_conv_batch<CH, 2>(s, t.views, t.items[3], (int)t.n_items[3], (const uint16_t *)n->ia,
                              n->w234 + (size_t)2 * CH * K2, n->params + OFF_B4, nullptr, layer_dil(3),
                              HeadArgs{n->params + OFF_W5, n->imask, n->lao});
-    hipLaunchKernelGGL(k_nao_up<true>, dim3(t.n_blocks), dim3(256), 0, s, none, t.views, t.blk_view, f, n->lao,
+    hipLaunchKernelGGL(k_nao_up<true>, dim3(t.n_blocks), dim3(64), 0, s, none, t.views, t.blk_view, f, n->lao,
                        n->fmask, ao);
     return hipGetLastError() == hipSuccess ? FGL_OK : FGL_ERR_DEVICE;
 }
@@ -1257,7 +1321,7 @@ int fgl_nao_infer(fgl_nao *n)
         launch_conv_head(cv.stream, n->ia, n->w234 + (size_t)2 * CH * K2, n->params + OFF_B4,
                          HeadArgs{n->params + OFF_W5, n->imask, dst}, lW, lH, 1, layer_dil(3));
         if (f > 1)
-            hipLaunchKernelGGL(k_nao_up<false>, dim3(((cv.W + 63) / 64) * ((cv.H + 3) / 4)), dim3(256), 0, cv.stream,
+            hipLaunchKernelGGL(k_nao_up<false>, dim3(((cv.W + 63) / 64) * ((cv.H + 3) / 4)), dim3(64), 0, cv.stream,
                                one, (const BatchPixView *)nullptr, (const uint8_t *)nullptr, f, n->lao, n->fmask,
                                cv.ao);
     }
