@@ -9,8 +9,8 @@ geometry pass, SSAO + blur, lighting, quantise + 2x2 SSAA resolve, with every
 input resident in HBM.
 
 Prints one JSON line (see the repository's contract).  The CPU baseline is the
-oracle (the C restatement of the reference path) timed single-threaded on a
-horizontal band of the same frame and scaled to the whole frame.
+oracle (the C restatement of the reference path) rendering the same whole
+frame single-threaded on one pinned host core, after the GPU work.
 """
 import argparse
 import json
@@ -127,10 +127,12 @@ def run_gpu(sc, steps, warmup, tile, tiles=None, dominant=None):
     return sec, per, nb, sec_all, dom, per_dom[dom], img, tuned
 
 
-def run_neural_ao(steps, warmup):
+def run_neural_ao(steps, warmup, heldout_frames=200):
     """Neural AO against plain SSAO + blur on the 1920x1080 frame without SSAA
-    (the default scene at SSAA 1x): per-frame kernel time of each, and the
-    AO error of the trained network on that frame (covered pixels)."""
+    (the default scene at SSAA 1x): per-frame kernel time of each, the AO
+    error of the trained network on that frame (covered pixels), and its
+    error on the held-out frames (the plain pipeline's SSAO + blur on
+    held-out seeded layouts and poses, evaluated here)."""
     import forkerrenderer_amd as F
     from forkerrenderer_amd import nao as NA
     if not os.path.exists(NA.DEFAULT_WEIGHTS):
@@ -162,19 +164,27 @@ def run_neural_ao(steps, warmup):
     pred = ctx.read(F.TARGET_AO).reshape(H, W)
     err = np.abs(pred - plain)[cover]
     fixed, audited, failed = NA.audit_fallback(pred, plain, cover, 0)
+    plain_ms = per.get("ssao", 0.0) + per.get("blur_gaussian", 0.0) + per.get("blur_reference", 0.0)
+    nao_ms = per.get("nao_conv", 0.0)   # the whole network: down pass, four convolutions + head, up pass
+    scale = int(ck["scale"])
+    net_px = -(-W // scale) * -(-H // scale)
+    flop = net_px * 59392           # four 3x3 layers + head per network pixel
+    conv_tflops = flop / (nao_ms * 1e-3) / 1e12 if nao_ms else None
+    # held-out frames, evaluated in this run against the current plain SSAO
+    t0 = time.perf_counter()
+    sset = NA.SceneSet(ctx, os.path.join(ROOT, "_scenes", "_nao"))
+    held = NA.evaluate(net, ctx, sset, heldout_frames, unblurred=True)
+    held["seconds"] = time.perf_counter() - t0
     net.close()
     ctx.close()
-    plain_ms = per.get("ssao", 0.0) + per.get("blur_gaussian", 0.0) + per.get("blur_reference", 0.0)
-    nao_ms = per.get("nao_conv", 0.0)   # the whole network: features, four convolutions, head
-    px = W * H
-    conv_tflops = px * 59392 / (per["nao_conv"] * 1e-3) / 1e12 if per.get("nao_conv") else None
-    held = json.loads(str(ck["eval_json"])) if "eval_json" in ck else None
-    return {"resolution": [W, H], "plain_ssao_blur_ms": plain_ms, "neural_ao_ms": nao_ms,
+    return {"resolution": [W, H], "network_grid": [-(-W // scale), -(-H // scale)],
+            "plain_ssao_blur_ms": plain_ms, "neural_ao_ms": nao_ms,
             "speedup": plain_ms / nao_ms if nao_ms else None,
             "kernels_ms": {k: v for k, v in per.items() if k.startswith("nao") or k in ("ssao", "blur_gaussian")},
             "conv_roofline": {"bound": "mfma", "achieved": conv_tflops, "peak": 2500.0, "unit": "TFLOP/s",
                               "frac": conv_tflops / 2500.0 if conv_tflops else None,
-                              "flop_per_px": 59392},
+                              "flop": flop, "note": "network FLOPs over the whole inference time (down and up "
+                                                    "passes included)"},
             "frame_mae": float(err.mean()), "frame_max_err": float(err.max()),
             "frame_fallback_tiles": [failed, audited],
             "heldout_eval": held, "train_steps": int(ck["step"])}
@@ -255,57 +265,68 @@ def run_batched_views(reps, n_views=64):
             "batch_speedup_vs_sequential": float(lat.sum() / (sec * 1e3)), "setup_s": t_setup}
 
 
+def host_cpu():
+    """The host CPU's model name (/proc/cpuinfo) and logical core count."""
+    model = platform.processor() or platform.machine()
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return model, os.cpu_count()
+
+
 def cpu_baseline(sc, band_frac):
-    """Oracle on a band of rows, scaled: est = t_empty + (t_band - t_empty) * rows_total/rows_band
-    per pass, where t_empty is the same pass with an empty row window (per-triangle work)."""
+    """The oracle renders the whole frame -- shadow pass, geometry pass, SSAO,
+    blur, lighting, quantise + SSAA -- single-threaded with the process
+    pinned to one core; each pass timed.  A band estimate (the passes on a
+    horizontal band of rows, scaled to all rows) is kept beside it."""
     import oracle as O
     I = sc.info
     models = [sc.model(i) for i in range(I.n_models)]
     mats = [sc.material(i) for i in range(I.n_materials)]
     texs = [sc.texture(i) for i in range(I.n_textures)]
+    core = sorted(os.sched_getaffinity(0))[0]
+    os.sched_setaffinity(0, {core})
     osc = O.OracleScene(models, mats, texs)
-    k = I.ssaa_k if I.ssaa_on else 1
-    W, H = I.width * k, I.height * k
-    S = I.shadow_map_size if I.shadow_map_size > 0 else 0
-    SW, SH = (S, S) if S else (W, H)
-    ratio = np.float32(I.width) / np.float32(I.height)
-    lights = [(I.lights[i].type, list(I.lights[i].vec), list(I.lights[i].color)) for i in range(I.n_lights)]
-    V = O.make_lookat(lights[0][1], (0, 0, 0))
-    Pm = O.make_ortho(np.float32(-3.0) * ratio, np.float32(3.0) * ratio, -3.0, 3.0, 0.1, 20.0)
-    L = O.mat_mul(Pm, V)
-    vp, svp = O.make_viewport(0, 0, W, H), O.make_viewport(0, 0, SW, SH)
-    fr_s = O.make_frame(np.eye(4), np.eye(4), L, vp, svp, I.eye, 1, I.shadow, lights)
-    Vc = O.make_lookat(list(I.eye), list(I.look_at))
-    Pc = O.make_perspective(45.0, ratio, 0.01, 20.0)
-    fr = O.make_frame(Vc, Pc, L, vp, svp, I.eye, I.shadow_on, I.shadow, lights)
+    st = O.frame_setup(I)
+    W, H, SW, SH, k = st["W"], st["H"], st["SW"], st["SH"], st["k"]
+    fr = st["fr"]
     lib = O.lib()
-    total = 0.0
     detail = {}
 
-    def timed(fn, rows_total, band):
+    def timed(name, fn):
+        t = time.perf_counter()
+        r = fn()
+        detail[name] = time.perf_counter() - t
+        return r
+    t_all = time.perf_counter()
+    sd, smap = timed("shadow", lambda: O.shadow_pass(osc, st["fr_shadow"], SW, SH))
+    depth, g = timed("geometry", lambda: O.geometry_pass(osc, fr, W, H))
+    ao = timed("ssao", lambda: O.ssao_kernel(fr, W, H, depth, g, I.ssao))
+    ao = timed("blur", lambda: O.blur_separable(ao, W, H, I.ssao.blur_weights, I.ssao.blur_radius))
+    fb = timed("lighting", lambda: O.lighting_pass(fr, W, H, g, ao, smap, SW, SH))
+    timed("resolve", lambda: O.ssaa_resolve(O.quantize_rgb(fb, W, H), W, H, k))
+    total = time.perf_counter() - t_all
+    # band estimate of the same frame (secondary): est = t_empty + (t_band - t_empty) * rows / band
+    band = max(1, int(H * band_frac))
+    est = detail["blur"] + detail["resolve"]
+
+    def band_time(fn, rows, b):
         lib.orc_set_rows(0, 0)
         t = time.perf_counter(); fn(); te = time.perf_counter() - t
-        y0 = rows_total // 2 - band // 2
-        lib.orc_set_rows(y0, y0 + band)
-        t = time.perf_counter(); r = fn(); tb = time.perf_counter() - t
+        y0 = rows // 2 - b // 2
+        lib.orc_set_rows(y0, y0 + b)
+        t = time.perf_counter(); fn(); tb = time.perf_counter() - t
         lib.orc_set_rows(0, 1 << 30)
-        return te + max(tb - te, 0.0) * rows_total / band, r
-
-    band_s = max(1, int(SH * band_frac))
-    band = max(1, int(H * band_frac))
-    t, (sd, smap) = timed(lambda: O.shadow_pass(osc, fr_s, SW, SH), SH, band_s)
-    detail["shadow"] = t; total += t
-    t, (depth, g) = timed(lambda: O.geometry_pass(osc, fr, W, H), H, band)
-    detail["geometry"] = t; total += t
-    t, ao = timed(lambda: O.ssao_kernel(fr, W, H, depth, g, I.ssao), H, band)
-    detail["ssao"] = t; total += t
-    t0 = time.perf_counter(); O.blur_separable(ao, W, H, I.ssao.blur_weights, I.ssao.blur_radius)
-    detail["blur"] = time.perf_counter() - t0; total += detail["blur"]
-    t, fb = timed(lambda: O.lighting_pass(fr, W, H, g, ao, smap, SW, SH), H, band)
-    detail["lighting"] = t; total += t
-    t0 = time.perf_counter(); q = O.quantize_rgb(fb, W, H); O.ssaa_resolve(q, W, H, k)
-    detail["resolve"] = time.perf_counter() - t0; total += detail["resolve"]
-    return total, detail, band
+        return te + max(tb - te, 0.0) * rows / b
+    est += band_time(lambda: O.shadow_pass(osc, st["fr_shadow"], SW, SH), SH, max(1, int(SH * band_frac)))
+    est += band_time(lambda: O.geometry_pass(osc, fr, W, H), H, band)
+    est += band_time(lambda: O.ssao_kernel(fr, W, H, depth, g, I.ssao), H, band)
+    est += band_time(lambda: O.lighting_pass(fr, W, H, g, ao, smap, SW, SH), H, band)
+    return total, detail, core, est, band
 
 
 def main():
@@ -358,13 +379,6 @@ def main():
     bytes_launch = algorithmic_bytes(dom, I, ntri, npos)
     achieved = bytes_launch / (ms_launch * 1e-3) / 1e9
     k = I.ssaa_k if I.ssaa_on else 1
-    cpu = None
-    if not a.no_cpu:
-        t_cpu, detail, band = cpu_baseline(sc, a.cpu_band)
-        cpu = {"value": 1.0 / t_cpu, "unit": "frames/s", "cores": 1, "kind": "port",
-               "sample": f"oracle single-threaded on a {band}-row band ({a.cpu_band:.4f} of the rows) of each "
-                         f"pass of the same frame, scaled per pass to all rows plus its per-triangle cost",
-               "seconds_per_frame_est": t_cpu, "per_pass_s": detail, "host": platform.processor() or platform.machine()}
     line = {
         "metric": METRIC, "value": 1.0 / sec, "unit": "frames/s", "n_gpus": 1, "steps": a.steps,
         "warmup": a.warmup, "ms_per_step": sec * 1e3, "higher_is_better": True, "scaling": "weak",
@@ -382,13 +396,25 @@ def main():
         "kernels_run": {"frames": nb, "ms_per_frame": sec_all * 1e3,
                         "note": "every kernel timed with HIP events (a separate run; the events add to the "
                                 "frame time); the throughput run times only the roofline kernel"},
-        "cpu_baseline": cpu,
-        "speedup_vs_cpu": (None if cpu is None else (1.0 / sec) / cpu["value"]),
     }
     if not a.no_nao:
         line["neural_ao"] = run_neural_ao(max(5, a.steps // 2), a.warmup)
     if not a.no_views:
         line["batched_views"] = run_batched_views(max(3, a.steps // 4))
+    cpu = None
+    if not a.no_cpu:
+        # after the GPU work: the process is pinned to one core from here on
+        print("[bench] CPU baseline: the oracle on the whole frame, one pinned core", file=sys.stderr, flush=True)
+        t_cpu, detail, core, est, band = cpu_baseline(sc, a.cpu_band)
+        model, ncpu = host_cpu()
+        cpu = {"value": 1.0 / t_cpu, "unit": "frames/s", "cores": 1, "kind": "port",
+               "sample": "the whole frame (shadow, geometry, SSAO, blur, lighting, quantise + SSAA) rendered "
+                         "once by the oracle, single-threaded, process pinned to one core",
+               "seconds_per_frame": t_cpu, "per_pass_s": detail, "pinned_core": core,
+               "host_cpu": model, "host_logical_cpus": ncpu,
+               "band_estimate_s": est, "band_rows": band}
+    line["cpu_baseline"] = cpu
+    line["speedup_vs_cpu"] = None if cpu is None else (1.0 / sec) / cpu["value"]
     print(json.dumps(line))
 
 

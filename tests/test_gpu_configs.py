@@ -121,3 +121,54 @@ def test_config3_full_size(ctx, scene_dir):
     assert occluded.mean() > 0.05, "middle band has too little occlusion to test"
     assert_bitexact("blurred AO", g_ao, O.blur_separable(g_raw, W, H, I.ssao.blur_weights, I.ssao.blur_radius))
     assert np.array_equal(g_img, O.ssaa_resolve(O.quantize_rgb(g_fb, W, H), W, H, k)), "SSAA resolve"
+
+
+def test_config4_neural_ao_full_frame(ctx, scene_dir):
+    """Config 4: the config-3 scene at 1920x1080 without SSAA, neural AO in
+    place of SSAO + blur, with the shipped trained weights.  Depth and
+    G-buffer bit-exact against the oracle; the device's down pass, network
+    and up pass against the network oracle (block inputs, BF16 forward,
+    bilinear up pass) over the whole frame -- max 2e-2, mean 2e-4 absolute
+    (FP32 accumulation order inside the MFMA tiles, expf); and the frame's
+    AO error against plain SSAO + blur, bounded to catch regressions of the
+    trained network."""
+    from forkerrenderer_amd import nao as NA
+    from oracle import nao_ref as R
+    from tests.test_gpu_nao import bf16_bits
+    p = G.config4(os.path.join(scene_dir, "c4"))
+    sc = F.Scene(p)
+    I = sc.info
+    st = O.frame_setup(I)
+    W, H = st["W"], st["H"]
+    assert (W, H) == (1920, 1080)
+    ctx.render_scene(sc)
+    depth = ctx.read(F.TARGET_DEPTH)
+    gbuf = ctx.read(F.TARGET_GBUFFER).reshape(F.GBUF_PLANES, -1)
+    plain = ctx.read(F.TARGET_AO).reshape(H, W)
+    ck = NA.load_checkpoint(NA.DEFAULT_WEIGHTS)
+    params = ck["state"][:NA.param_count()]
+    net = NA.NeuralAO(ctx, batch=1, crop=64, params=params)
+    net.infer()
+    pred = ctx.read(F.TARGET_AO).reshape(H, W)
+    net.close()
+    print("config 4: oracle geometry pass", flush=True)
+    models, mats, texs = scene_arrays(sc)
+    od, og = O.geometry_pass(O.OracleScene(models, mats, texs), st["fr"], W, H)
+    assert_bitexact("depth", depth, od)
+    assert_bitexact("gbuf", gbuf, og)
+    # network oracle on the whole network grid
+    from tests.test_gpu_nao import features_f32
+    frame = {"depth": depth.reshape(H, W), "gbuf": gbuf.reshape(F.GBUF_PLANES, H, W),
+             "view": st["view"].astype(np.float32)}
+    feat, cov = features_f32(frame)
+    S = int(ck["scale"])
+    x, lcov = R.block_inputs(feat[..., :7], cov, S)
+    _, y = R.forward(params, R.bf16_bits_to_float(bf16_bits(x))[None], bf16=True)
+    want = R.upsample(np.where(lcov, y[0], 1.0).astype(np.float32), cov, S)
+    err = np.abs(pred - want)[cov]
+    print(f"config 4 network vs oracle: max {err.max():.2e} mean {err.mean():.2e}", flush=True)
+    assert err.max() < 2e-2 and err.mean() < 2e-4
+    assert np.all(pred[~cov] == 1.0)
+    mae = float(np.abs(pred - plain)[cov].mean())
+    print(f"config 4 neural AO vs plain SSAO + blur: MAE {mae:.4f}", flush=True)
+    assert mae < 0.035, mae

@@ -61,7 +61,7 @@ def features_f32(frame):
     with np.errstate(invalid="ignore"):
         for c in range(7):
             cov &= np.isfinite(feat[c])   # non-finite G-buffer values count as uncovered
-    out = np.zeros((H, W, 16), np.float32)
+    out = np.zeros(cov.shape + (16,), np.float32)
     for c in range(7):
         out[..., c] = np.where(cov, feat[c], f32(0))
     return out, cov
