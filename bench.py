@@ -191,23 +191,28 @@ def run_neural_ao(steps, warmup, heldout_frames=200):
 
 
 def run_batched_views(reps, n_views=64):
-    """The uneven-views workload (64 views, 256^2 .. 1920x1080, icosphere
-    subdivision 4..8, deferred PBR + 5x5 PCF + neural AO) as one batch:
-    aggregate Mpix/s and Mtri/s of graph replays after a small autotune over
-    tile sizes and rasteriser occupancy, and per-view latency of the same
-    views rendered one at a time (render_view, synchronised per view)."""
+    """The batched-views configuration: 64 views of distinct seeded scenes of
+    the 1280x800 generator (2048^2 maps and shadow maps, 5x5 PCF, icosphere
+    subdivision 4..8), 256^2 .. 1920x1080, deferred PBR + neural AO, as one
+    batch: aggregate Mpix/s and Mtri/s of graph replays after the batch
+    autotuner (tiles, rasteriser workgroups per CU, pair capacity), and the
+    per-view latency of the same views rendered one at a time (render_view,
+    synchronised per view)."""
     import forkerrenderer_amd as F
     from forkerrenderer_amd import nao as NA
     from forkerrenderer_amd import views as VW
+    from forkerrenderer_amd.autotune import autotune_batch
     ctx = F.Context(0)
     params = NA.load_weights() if os.path.exists(NA.DEFAULT_WEIGHTS) else NA.init_params(0)
     net = NA.NeuralAO(ctx, batch=1, crop=64, params=params)
     t0 = time.perf_counter()
-    views = VW.uneven_views(ctx, n_views=n_views)
+    specs = VW.view_specs(n_views)
+    views = VW.distinct_views(ctx, specs)
     t_setup = time.perf_counter() - t0
     b = VW.Batch(ctx, views, VW.AO_NEURAL, net)
     b.render()
     b.sync()
+    best, tune = autotune_batch(b)
 
     def timed(k):
         b.render()
@@ -217,21 +222,6 @@ def run_batched_views(reps, n_views=64):
             b.render()
         b.sync()
         return (time.perf_counter() - t) / k
-
-    tune = {}
-    for tiles in ((16, 16), (8, 16), (16, 8), (8, 8), (32, 16)):
-        for occ in (2, 4, 8):
-            b.set_tiles(*tiles)
-            b.set_occupancy(occ)
-            b.render()
-            b.sync()
-            tune[f"{tiles[0]}/{tiles[1]}/{occ}"] = timed(2) * 1e3
-    best = min(tune, key=tune.get)
-    ts, tm, occ = (int(x) for x in best.split("/"))
-    b.set_tiles(ts, tm)
-    b.set_occupancy(occ)
-    b.render()
-    b.sync()
     sec = timed(reps)
     st = b.stats()
     # per-pass kernel time of one eager batch render
@@ -255,10 +245,12 @@ def run_batched_views(reps, n_views=64):
     net.close()
     ctx.close()
     lat = np.array(lat)
-    return {"views": n_views, "pixels": st["pixels"], "triangles_per_pass": st["tris_main"],
+    return {"views": n_views, "scenes": "distinct seeded scenes (view seeds 30000..), 2048^2 maps and shadow maps",
+            "pixels": st["pixels"], "triangles_per_pass": st["tris_main"],
             "ms_per_batch": sec * 1e3, "mpix_per_s": st["pixels"] / sec / 1e6,
-            "mtri_per_s": st["tris_main"] / sec / 1e6, "tiles_shadow_main_occupancy": best,
+            "mtri_per_s": st["tris_main"] / sec / 1e6, "autotuned": best,
             "autotune_ms": tune, "pairs": [st["pairs_shadow"], st["pairs_main"]],
+            "pair_capacity": [st["cap_shadow"], st["cap_main"]],
             "kernels_ms": per,
             "single_view_latency_ms": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99)),
                                        "sum": float(lat.sum())},

@@ -107,18 +107,26 @@ def value_noise(p, seed, octaves=3, freq=2.0, lacunarity=2.0, gain=0.5):
     return total / norm
 
 
+def scatter_add(index, values, n):
+    """Sum of the rows of `values` (m, c) into n rows by `index` (m,)."""
+    return np.stack([np.bincount(index, weights=values[:, c], minlength=n) for c in range(values.shape[1])], 1)
+
+
 def vertex_normals(v, f):
     fn = np.cross(v[f[:, 1]] - v[f[:, 0]], v[f[:, 2]] - v[f[:, 0]])
-    n = np.zeros_like(v)
-    for k in range(3):
-        np.add.at(n, f[:, k], fn)
+    n = sum(scatter_add(f[:, k], fn, len(v)) for k in range(3))
     ln = np.linalg.norm(n, axis=1, keepdims=True)
     ln[ln == 0] = 1
     return n / ln
 
 
+_ICO = {}
+
+
 def displaced_icosphere(subdiv, seed, amplitude=0.1, radius=1.0):
-    v, f = icosphere(subdiv)
+    if subdiv not in _ICO:
+        _ICO[subdiv] = icosphere(subdiv)
+    v, f = _ICO[subdiv]
     d = value_noise(v, seed)
     v = v * (radius * (1 + amplitude * d))[:, None]
     n = vertex_normals(v, f)

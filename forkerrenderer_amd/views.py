@@ -170,9 +170,7 @@ def tangents(v, uv, f):
     det = d1[:, 0] * d2[:, 1] - d2[:, 0] * d1[:, 1]
     det[np.abs(det) < 1e-12] = 1e-12
     T = (e1 * d2[:, 1:2] - e2 * d1[:, 1:2]) / det[:, None]
-    acc = np.zeros_like(v)
-    for k in range(3):
-        np.add.at(acc, f[:, k], T)
+    acc = sum(scenegen.scatter_add(f[:, k], T, len(v)) for k in range(3))
     n = v / np.maximum(np.linalg.norm(v, axis=1, keepdims=True), 1e-12)
     acc -= n * np.sum(acc * n, axis=1, keepdims=True)
     ln = np.linalg.norm(acc, axis=1, keepdims=True)
@@ -181,10 +179,11 @@ def tangents(v, uv, f):
     return acc.astype(np.float32)
 
 
-def pbr_material(ctx, seed, size=512):
-    """A PBR material with generated albedo / roughness / metalness / normal
-    maps (repeat, bilinear)."""
-    from . import FILTER_LINEAR, WRAP_REPEAT
+def pbr_maps(seed, size):
+    """The generator's PBR maps (host bytes): albedo uniform RGB, roughness
+    round(255 U[0.05, 1]), metalness {0, 255} with probability 1/2, tangent-
+    space normal map normalize(N(0, 0.2), N(0, 0.2), 1) encoded (n + 1)/2,
+    B, G, R byte order for the colour maps."""
     rng = np.random.default_rng(seed)
     alb = rng.integers(0, 256, (size, size, 3), dtype=np.uint8)
     rough = np.round(255 * rng.uniform(0.05, 1.0, (size, size))).astype(np.uint8)
@@ -193,6 +192,30 @@ def pbr_material(ctx, seed, size=512):
     n = np.concatenate([nxy, np.ones((size, size, 1))], -1)
     n /= np.linalg.norm(n, axis=-1, keepdims=True)
     nrm = np.round((n + 1) / 2 * 255).astype(np.uint8)[..., ::-1].copy()   # B, G, R bytes
+    return alb, rough, metal, nrm
+
+
+def pbr_material_desc(ids):
+    """The PBR Material record over uploaded (or oracle-indexed) maps
+    [albedo, roughness, metalness, normal]."""
+    m = Material()
+    m.ka[:] = (1, 1, 1)
+    m.kd[:] = (1, 1, 1)
+    m.ks[:] = (0.5, 0.5, 0.5)
+    m.albedo[:] = (1, 1, 1)
+    m.roughness, m.metalness = 0.5, 0.0
+    for fld in ("map_ks", "map_ke", "map_ao", "map_pbr_ke"):
+        setattr(m, fld, -1)
+    m.map_kd = m.map_base = ids[0]
+    m.map_rough, m.map_metal, m.map_bump = ids[1], ids[2], ids[3]
+    return m
+
+
+def pbr_material(ctx, seed, size=512):
+    """A PBR material with generated albedo / roughness / metalness / normal
+    maps (repeat, bilinear)."""
+    from . import FILTER_LINEAR, WRAP_REPEAT
+    alb, rough, metal, nrm = pbr_maps(seed, size)
     ids = [ctx.upload_texture(t, WRAP_REPEAT, FILTER_LINEAR) for t in (alb, rough, metal, nrm)]
     m = Material()
     m.ka[:] = (1, 1, 1)
@@ -207,19 +230,28 @@ def pbr_material(ctx, seed, size=512):
     return ctx.upload_material(m)
 
 
-def upload_sphere(ctx, subdiv, seed, mat):
+def sphere_model(subdiv, seed, mat):
+    """The generator's displaced icosphere (radius 1, 3-octave value noise of
+    amplitude 0.1) as a model dict with generated tangents."""
     v, n, uv, f = scenegen.displaced_icosphere(subdiv, seed, 0.1, 1.0)
     f32 = np.float32
-    return ctx.upload_model({"pos": v.astype(f32), "uv": uv.astype(f32), "nrm": n.astype(f32),
-                             "tan": tangents(v, uv, f), "vi": f, "ti": f, "ni": f,
-                             "face_material": np.full(len(f), mat, np.int32), "support_pbr": True})
+    return {"pos": v.astype(f32), "uv": uv.astype(f32), "nrm": n.astype(f32), "tan": tangents(v, uv, f),
+            "vi": f, "ti": f, "ni": f, "face_material": np.full(len(f), mat, np.int32), "support_pbr": True}
+
+
+def ground_model(mat):
+    p, n, uv, f = scenegen.plane_group(4.0)
+    t = np.tile(np.array([[1.0, 0.0, 0.0]], np.float32), (len(p), 1))
+    return {"pos": p, "uv": uv, "nrm": n, "tan": t, "vi": f, "ti": f, "ni": f,
+            "face_material": np.full(len(f), mat, np.int32), "support_pbr": True}
+
+
+def upload_sphere(ctx, subdiv, seed, mat):
+    return ctx.upload_model(sphere_model(subdiv, seed, mat))
 
 
 def upload_ground(ctx, mat):
-    p, n, uv, f = scenegen.plane_group(4.0)
-    t = np.tile(np.array([[1.0, 0.0, 0.0]], np.float32), (len(p), 1))
-    return ctx.upload_model({"pos": p, "uv": uv, "nrm": n, "tan": t, "vi": f, "ti": f, "ni": f,
-                             "face_material": np.full(len(f), mat, np.int32), "support_pbr": True})
+    return ctx.upload_model(ground_model(mat))
 
 
 def pcf_grid(n=5):
@@ -256,4 +288,70 @@ def uneven_views(ctx, n_views=64, seed=5, subdivs=(4, 5, 6, 7, 8), min_px=256 * 
                  (spheres[s], make_model_matrix((0.0, 0.0, 0.0), float(rng.uniform(0, 360)), 1.0))]
         lights = [(0, (2.0, 5.0, 5.0), (3.0, 3.0, 3.0))]
         views.append(View(w, h, eye, (0.0, -0.2, 0.0), draws, lights, pcf_grid(5), shadow_size))
+    return views
+
+
+# ---------------- the batched-views configuration: distinct seeded scenes ----------------
+
+class ViewSpec:
+    """Everything that fixes one view of the batched-views configuration: its
+    seed, frame size, sphere subdivision, camera and sphere rotation.  The
+    scene itself (sphere, ground, maps) follows from the seed."""
+
+    def __init__(self, seed, width, height, subdiv, eye, look_at, roty, tex, shadow_size):
+        self.seed, self.width, self.height, self.subdiv = seed, width, height, subdiv
+        self.eye, self.look_at, self.roty, self.tex, self.shadow_size = eye, look_at, roty, tex, shadow_size
+        self.light = (0, (2.0, 5.0, 5.0), (3.0, 3.0, 3.0))
+
+    def matrices(self):
+        return (make_model_matrix((0.0, -1.1, 0.0), 0.0, 3.0), make_model_matrix((0.0, 0.0, 0.0), self.roty, 1.0))
+
+
+def view_specs(n_views=64, seed=30000, subdivs=(4, 5, 6, 7, 8), min_px=256 * 256, max_px=1920 * 1080, tex=2048,
+               shadow_size=2048):
+    """The batched-views configuration: view i has seed seed + i; pixel count
+    log-uniform in [min_px, max_px] with aspect 1, 4:3, 16:10 or 16:9;
+    icosphere subdivision drawn from `subdivs`; a random orbit camera."""
+    out = []
+    aspects = [1.0, 4 / 3, 16 / 10, 16 / 9]
+    for i in range(n_views):
+        rng = np.random.default_rng(seed + i)
+        px = math.exp(rng.uniform(math.log(min_px), math.log(max_px)))
+        a = aspects[int(rng.integers(0, 4))]
+        h = max(16, int(round(math.sqrt(px / a))))
+        w = max(16, min(1920, int(round(h * a))))
+        h = min(h, 1080)
+        sd = int(rng.choice(subdivs))
+        r = rng.uniform(2.6, 4.0)
+        el = math.radians(rng.uniform(10, 50))
+        az = rng.uniform(0, 2 * math.pi)
+        eye = (r * math.cos(el) * math.cos(az), r * math.sin(el), r * math.cos(el) * math.sin(az))
+        out.append(ViewSpec(seed + i, w, h, sd, eye, (0.0, -0.2, 0.0), float(rng.uniform(0, 360)), tex, shadow_size))
+    return out
+
+
+def scene_host(spec):
+    """Host arrays of a view's scene, material index 0 and texture indices
+    0..3 (albedo, roughness, metalness, normal; repeat, bilinear)."""
+    return {"sphere": sphere_model(spec.subdiv, spec.seed, 0), "ground": ground_model(0),
+            "maps": pbr_maps(spec.seed + 1000, spec.tex)}
+
+
+def distinct_views(ctx, specs, workers=8):
+    """Uploads every view's own scene (the host arrays are generated in a
+    thread pool) and returns the View list."""
+    from concurrent.futures import ThreadPoolExecutor
+    from . import FILTER_LINEAR, WRAP_REPEAT
+    views = []
+    with ThreadPoolExecutor(max_workers=workers) as ex:
+        for spec, host in zip(specs, ex.map(scene_host, specs)):
+            ids = [ctx.upload_texture(t, WRAP_REPEAT, FILTER_LINEAR) for t in host["maps"]]
+            mat = ctx.upload_material(pbr_material_desc(ids))
+            for k in ("sphere", "ground"):
+                host[k]["face_material"] = np.full(len(host[k]["vi"]), mat, np.int32)
+            sphere = ctx.upload_model(host["sphere"])
+            ground = ctx.upload_model(host["ground"])
+            Mg, Ms = spec.matrices()
+            views.append(View(spec.width, spec.height, spec.eye, spec.look_at, [(ground, Mg), (sphere, Ms)],
+                              [spec.light], pcf_grid(5), spec.shadow_size))
     return views

@@ -25,6 +25,7 @@ import pytest
 import forkerrenderer_amd as F
 import oracle as O
 from forkerrenderer_amd import scenegen as G
+from forkerrenderer_amd import views as VW
 from tests.util import assert_bitexact, lsb_stats, oracle_frame, scene_arrays
 
 pytestmark = pytest.mark.gpu
@@ -172,3 +173,91 @@ def test_config4_neural_ao_full_frame(ctx, scene_dir):
     mae = float(np.abs(pred - plain)[cov].mean())
     print(f"config 4 neural AO vs plain SSAO + blur: MAE {mae:.4f}", flush=True)
     assert mae < 0.035, mae
+
+
+def _oracle_view(spec, host):
+    """One view of the batched-views configuration through the oracle, set
+    up as fgl_render_view does: the shadow pass from the point light (ortho
+    +-3 x aspect, 0.1..20, looking at the origin) on the S x S map, the
+    geometry pass (perspective 45 degrees, 0.01..20)."""
+    W, H, S = spec.width, spec.height, spec.shadow_size
+    ratio = np.float32(W) / np.float32(H)
+    light = spec.light
+    Lm = O.mat_mul(O.make_ortho(np.float32(-3.0) * ratio, np.float32(3.0) * ratio, -3.0, 3.0, 0.1, 20.0),
+                   O.make_lookat(list(light[1]), (0, 0, 0)))
+    V = O.make_lookat(list(spec.eye), list(spec.look_at))
+    P = O.make_perspective(45.0, ratio, 0.01, 20.0)
+    vp, svp = O.make_viewport(0, 0, W, H), O.make_viewport(0, 0, S, S)
+    cfg = VW.pcf_grid(5)
+    lights = [(light[0], list(light[1]), list(light[2]))]
+    fr_s = O.make_frame(np.eye(4), np.eye(4), Lm, vp, svp, spec.eye, 1, cfg, lights)
+    fr = O.make_frame(V, P, Lm, vp, svp, spec.eye, 1, cfg, lights)
+    Mg, Ms = spec.matrices()
+    texs = [{"bytes": t, "wrap": F.WRAP_REPEAT, "filter": F.FILTER_LINEAR} for t in host["maps"]]
+    osc = O.OracleScene([(host["ground"], Mg), (host["sphere"], Ms)], [VW.pbr_material_desc([0, 1, 2, 3])], texs)
+    sdepth, smap = O.shadow_pass(osc, fr_s, S, S)
+    depth, g = O.geometry_pass(osc, fr, W, H)
+    return dict(fr=fr, V=V, sdepth=sdepth, smap=smap, depth=depth, gbuf=g)
+
+
+@pytest.mark.timeout(900)
+def test_config5_batched_views(ctx):
+    """Config 5: 64 views of distinct seeded scenes of the 1280x800 generator
+    (2048^2 maps, 2048^2 shadow maps with 5x5 PCF, subdivision 4-8), sizes
+    log-uniform 256^2 .. 1920x1080, neural AO with the shipped weights, one
+    batch replayed from its HIP graph.  Every target of every view equals
+    the view rendered alone (fgl_render_view) bit for bit; the four smallest
+    views against the oracle: shadow map, depth and G-buffer bit-exact,
+    neural AO against the network oracle (max 2e-2, mean 2e-4), image
+    within +-1 LSB (oracle lighting of the device's G-buffer and AO)."""
+    from forkerrenderer_amd import nao as NA
+    from oracle import nao_ref as R
+    from tests.test_gpu_nao import bf16_bits, features_f32
+    specs = VW.view_specs()
+    assert len(specs) == 64 and {s.subdiv for s in specs} == {4, 5, 6, 7, 8}
+    print("config 5: generating and uploading 64 scenes", flush=True)
+    views = VW.distinct_views(ctx, specs)
+    ck = NA.load_checkpoint(NA.DEFAULT_WEIGHTS)
+    params = ck["state"][:NA.param_count()]
+    net = NA.NeuralAO(ctx, batch=1, crop=64, params=params)
+    b = VW.Batch(ctx, views, VW.AO_NEURAL, net)
+    b.render()
+    assert b.sync() == 0
+    b.render()                      # graph replay
+    assert b.sync() == 0
+    st = b.stats()
+    print(f"config 5: {st['pixels']} pixels, {st['tris_main']} triangles per pass", flush=True)
+    targets = [F.TARGET_DEPTH, F.TARGET_GBUFFER, F.TARGET_AO, F.TARGET_FRAME, F.TARGET_IMAGE, F.TARGET_SHADOW,
+               F.TARGET_SHADOW_DEPTH]
+    for i, v in enumerate(views):
+        VW.render_view(ctx, v, VW.AO_NEURAL, net)
+        for t in targets:
+            got, want = b.read(i, t), ctx.read(t)
+            assert np.array_equal(got.view(np.uint8), np.ascontiguousarray(want).view(np.uint8)), \
+                f"view {i} ({v.desc.width}x{v.desc.height}) target {t}"
+    print("config 5: all 64 views equal their single-view renders", flush=True)
+    small = sorted(range(64), key=lambda i: specs[i].width * specs[i].height)[:4]
+    for i in small:
+        spec = specs[i]
+        W, H = spec.width, spec.height
+        print(f"config 5: view {i} ({W}x{H}, subdivision {spec.subdiv}) against the oracle", flush=True)
+        o = _oracle_view(spec, VW.scene_host(spec))
+        assert_bitexact(f"view {i} shadow", b.read(i, F.TARGET_SHADOW), o["smap"])
+        assert_bitexact(f"view {i} shadow depth", b.read(i, F.TARGET_SHADOW_DEPTH), o["sdepth"])
+        depth = b.read(i, F.TARGET_DEPTH)
+        gbuf = b.read(i, F.TARGET_GBUFFER).reshape(F.GBUF_PLANES, -1)
+        assert_bitexact(f"view {i} depth", depth, o["depth"])
+        assert_bitexact(f"view {i} gbuf", gbuf, o["gbuf"])
+        ao = b.read(i, F.TARGET_AO).reshape(H, W)
+        feat, cov = features_f32({"depth": depth.reshape(H, W), "gbuf": gbuf.reshape(F.GBUF_PLANES, H, W),
+                                  "view": o["V"].astype(np.float32)})
+        x, lcov = R.block_inputs(feat[..., :7], cov, NA.SCALE)
+        _, y = R.forward(params, R.bf16_bits_to_float(bf16_bits(x))[None], bf16=True)
+        want_ao = R.upsample(np.where(lcov, y[0], 1.0).astype(np.float32), cov, NA.SCALE)
+        err = np.abs(ao - want_ao)[cov]
+        assert err.max() < 2e-2 and err.mean() < 2e-4, (i, err.max(), err.mean())
+        fb = O.lighting_pass(o["fr"], W, H, gbuf, ao.reshape(-1), o["smap"], spec.shadow_size, spec.shadow_size)
+        mx, frac = lsb_stats(b.read(i, F.TARGET_IMAGE), O.quantize_rgb(fb, W, H))
+        assert frac >= 0.999, (i, mx, frac)
+    b.close()
+    net.close()
