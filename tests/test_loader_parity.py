@@ -250,3 +250,30 @@ def test_texture_modes_are_captured_at_load(scene_dir):
     sc = F.Scene(p, F.WRAP_REPEAT, F.FILTER_NEAREST)
     assert all((sc.texture(i)["wrap"], sc.texture(i)["filter"]) == (F.WRAP_CLAMP, F.FILTER_LINEAR)
                for i in range(sc.info.n_textures))
+
+
+# The reference's sample assets from the committed fixtures (tests/golden/assets):
+# these run everywhere, the reference tree is not needed.
+ASSET_SETS = [["plane", "diablo_pose"], ["plane", "mary"], ["plane", "great_sword"],
+              ["african_head", "african_head/eye"], ["brickwall"]]
+
+
+@pytest.mark.parametrize("names", ASSET_SETS, ids=lambda n: n[-1].split("/")[0])
+def test_asset_fixture_loader(scene_dir, names):
+    from tests import assets as A
+    p = A.scene(os.path.join(scene_dir, "fixtures", names[-1].split("/")[0] + ".scene"), names,
+                mode="deferred", width=64, height=48)
+    sc = _cmp_scene(p)
+    assert sc.info.n_models == len(names)
+
+
+@pytest.mark.skipif(not os.path.isdir(REF), reason="reference assets not present on this machine")
+def test_asset_fixtures_are_the_reference_files():
+    from tests import assets as A
+    for f in sorted(os.listdir(A.FIXTURES)):
+        name = f[:-4]
+        with np.load(os.path.join(A.FIXTURES, f)) as z:
+            assert z.files
+            for k in z.files:
+                with open(os.path.join(REF, name, k), "rb") as fh:
+                    assert z[k].tobytes() == fh.read(), (name, k)
