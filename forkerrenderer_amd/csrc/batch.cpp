@@ -130,12 +130,12 @@ struct fgl_batch {
     bool have_shadow = false;
 
     // device
-    Buf d_views, d_draws[2], d_items[2], d_blk_view, d_lblk_view, d_pix_views, d_conv_items[4], d_vblocks[2],
+    Buf d_views, d_draws[2], d_items[2], d_blk_view, d_lblk_view, d_pix_views, d_conv_items[2], d_vblocks[2],
         d_fblocks[2];
     size_t total_lpx = 0;      // network pixels of all views (neural AO)
     uint32_t n_lblocks = 0;    // 64 x 4 network-pixel blocks of all views
     int n_vblocks[2] = {0, 0}, n_fblocks[2] = {0, 0};
-    uint32_t n_conv_items[4] = {0, 0, 0, 0};
+    uint32_t n_conv_items[2] = {0, 0};
     Buf depth, vis, gbuf, ao, fb, img, shadow, shadow_depth;
     Buf screen, ndc_z, rect, counts, offs, scan_tmp, tri_v, big_list, tile_start, tile_end;
     Buf keys[2], vals[2], keys_alt[2], vals_alt[2], sort_tmp[2];
@@ -169,7 +169,7 @@ static int plan(fgl_batch *b)
     const int nscale = b->ao_mode == FGL_AO_NEURAL ? nao_scale(b->nao) : 1;
     size_t lpx = 0;
     uint32_t lblocks = 0;
-    std::vector<uint2> conv_items[4];
+    std::vector<uint2> conv_items[2];
     size_t px = 0, spx = 0;
     uint32_t tiles[2] = {0, 0}, blocks = 0;
     b->have_shadow = false;
@@ -301,7 +301,7 @@ static int plan(fgl_batch *b)
         return bfail(b, FGL_ERR_DEVICE, "upload failed");
     if (b->ao_mode == FGL_AO_NEURAL) {
         if ((rc = nao_reserve(b->nao, px, lpx))) return bfail(b, rc, "neural AO buffers");
-        for (int l = 0; l < 4; ++l) {
+        for (int l = 0; l < 2; ++l) {
             b->n_conv_items[l] = (uint32_t)conv_items[l].size();
             if (grow(b->d_conv_items[l], (conv_items[l].size() + 1) * sizeof(uint2)))
                 return bfail(b, FGL_ERR_NOMEM, "batch allocation failed");
@@ -458,7 +458,7 @@ static int enqueue(fgl_batch *b, bool measure, bool timed)
         nt.n_lblocks = b->n_lblocks;
         nt.total_px = b->total_px;
         nt.total_lpx = b->total_lpx;
-        for (int l = 0; l < 4; ++l) {
+        for (int l = 0; l < 2; ++l) {
             nt.items[l] = (const uint2 *)b->d_conv_items[l].p;
             nt.n_items[l] = b->n_conv_items[l];
         }
@@ -579,7 +579,7 @@ void fgl_batch_destroy(fgl_batch *b)
     if (!b) return;
     hipStreamSynchronize(ctx_stream(b->ctx));
     drop_graph(b);
-    for (int l = 0; l < 4; ++l) release(b->d_conv_items[l]);
+    for (int l = 0; l < 2; ++l) release(b->d_conv_items[l]);
     for (int p = 0; p < 2; ++p) {
         release(b->d_vblocks[p]);
         release(b->d_fblocks[p]);

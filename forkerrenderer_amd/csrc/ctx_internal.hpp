@@ -57,7 +57,7 @@ struct BatchPixView {
 };
 
 // Neural AO of every view of a batch (nao.hip): down pass from the batch
-// depth / G-buffer, four convolutions on the network grids, head, up pass
+// depth / G-buffer, two fused kernels for the four convolutions and the head, up pass
 // into the batch AO target.  nao_reserve allocates the batch-sized buffers
 // beforehand (the infer call then allocates nothing, so it can be captured
 // in a graph).
@@ -68,8 +68,8 @@ struct NaoBatchTables {
     const uint8_t *lblk_view;    // view of each 64 x 4 network-pixel block
     uint32_t n_lblocks;
     size_t total_px, total_lpx;  // frame and network pixels of all views
-    const uint2 *items[4];       // convolution tiles of each layer (nao_conv_items)
-    uint32_t n_items[4];
+    const uint2 *items[2];       // tiles of the fused layers 1-2 and 3-4 (nao_conv_items)
+    uint32_t n_items[2];
 };
 int nao_reserve(fgl_nao *n, size_t frame_px, size_t net_px);
 // Frame pixels per network pixel, each way (1, 2 or 4).
@@ -77,9 +77,9 @@ int nao_scale(const fgl_nao *n);
 // Changes whenever nao_reserve moves the inference buffers (a graph that
 // captured their addresses must then be captured again).
 uint64_t nao_buffers_generation(const fgl_nao *n);
-// Appends the convolution tiles of a view's lW x lH network grid to the four
-// per-layer lists.
-void nao_conv_items(int view, int lW, int lH, std::vector<uint2> items[4]);
+// Appends the tiles of a view's lW x lH network grid to the lists of the two
+// fused inference kernels (layers 1-2, layers 3-4 with the head).
+void nao_conv_items(int view, int lW, int lH, std::vector<uint2> items[2]);
 int nao_infer_batch(fgl_nao *n, hipStream_t s, const NaoBatchTables &t, const float *depth, const float *gbuf,
                     float *ao);
 

@@ -277,8 +277,8 @@ __global__ __launch_bounds__(256) void k_nao_down(BatchPixView one, const BatchP
 // Up pass: AO of every frame pixel -- 1 where the pixel is uncovered, else
 // the network AO interpolated bilinearly between the centres of the blocks
 // around the pixel's centre (block (X, Y) has its centre at frame
-// coordinate f X + f/2; edges clamp).  64 threads per 64 x 4 frame-pixel
-// block, four consecutive pixels of a row each (one 4-byte mask load and
+// coordinate f X + f/2; edges clamp).  One wavefront per 64 x 4 frame-pixel
+// block, four consecutive pixels of a row per lane (one 4-byte mask load and
 // one 16-byte store where the row is aligned).
 __device__ __forceinline__ float up_value(const BatchPixView &v, const float *__restrict__ L, int f, int x, int y)
 {
@@ -294,26 +294,65 @@ __device__ __forceinline__ float up_value(const BatchPixView &v, const float *__
     return top * (1.f - ty) + bot * ty;
 }
 
-template <bool BATCH>
-__global__ __launch_bounds__(64) void k_nao_up(BatchPixView one, const BatchPixView *__restrict__ views,
-                                               const uint8_t *__restrict__ blk_view, int f,
-                                               const float *__restrict__ lao, const uint8_t *__restrict__ fmask,
-                                               float *__restrict__ ao)
+// Four pixels x .. x+3 of row y, x a multiple of 4, f = 4: the sample
+// points share the row weights and fall between network columns X - 1 and X
+// (first two) or X and X + 1 (last two), X = x / 4 -- six network values
+// instead of sixteen, and the same products and sums as up_value.
+__device__ __forceinline__ void up_value4(const BatchPixView &v, const float *__restrict__ L, int x, int y,
+                                          float out[4])
 {
-    const BatchPixView &v = BATCH ? view_of_block(views, blk_view, blockIdx.x) : one;
-    const int k = BATCH ? (int)(blockIdx.x - v.blk_base) : (int)blockIdx.x;
+    const float inv = 1.f / 4.f;
+    const float w = ((float)y + 0.5f) * inv - 0.5f;
+    const float fw = floorf(w);
+    const float ty = w - fw;
+    const int Y0 = (int)fw;
+    const int ya = imax(0, imin(v.lH - 1, Y0)), yb = imax(0, imin(v.lH - 1, Y0 + 1));
+    const int X = x >> 2;
+    const int c0 = imax(0, imin(v.lW - 1, X - 1)), c1 = imax(0, imin(v.lW - 1, X)), c2 = imax(0, imin(v.lW - 1, X + 1));
+    const float *ra = L + (size_t)ya * v.lW, *rb = L + (size_t)yb * v.lW;
+    const float a0 = ra[c0], a1 = ra[c1], a2 = ra[c2], b0 = rb[c0], b1 = rb[c1], b2 = rb[c2];
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+        const float u = ((float)(x + e) + 0.5f) * inv - 0.5f;
+        const float tx = u - floorf(u);
+        const float al = e < 2 ? a0 : a1, ar = e < 2 ? a1 : a2, bl = e < 2 ? b0 : b1, br = e < 2 ? b1 : b2;
+        const float top = al * (1.f - tx) + ar * tx;
+        const float bot = bl * (1.f - tx) + br * tx;
+        out[e] = top * (1.f - ty) + bot * ty;
+    }
+}
+
+// Four waves per workgroup, one 64 x 4 frame-pixel block each.
+template <bool BATCH>
+__global__ __launch_bounds__(256) void k_nao_up(BatchPixView one, const BatchPixView *__restrict__ views,
+                                                const uint8_t *__restrict__ blk_view, int n_blocks, int f,
+                                                const float *__restrict__ lao, const uint8_t *__restrict__ fmask,
+                                                float *__restrict__ ao)
+{
+    const int b = (int)blockIdx.x * 4 + (int)(threadIdx.x >> 6);
+    if (b >= n_blocks) return;
+    const BatchPixView &v = BATCH ? view_of_block(views, blk_view, b) : one;
+    const int k = BATCH ? (int)(b - v.blk_base) : b;
+    const int lane = threadIdx.x & 63;
     const int bx = (v.W + 63) / 64;
-    const int x = (k % bx) * 64 + 4 * (int)(threadIdx.x & 15), y = (k / bx) * 4 + (int)(threadIdx.x >> 4);
+    const int x = (k % bx) * 64 + 4 * (lane & 15), y = (k / bx) * 4 + (lane >> 4);
     if (x >= v.W || y >= v.H) return;
     const size_t i = v.px_off + (size_t)y * v.W + x;
     const float *L = lao + v.lo_off;
     if (x + 3 < v.W && (i & 3) == 0) {
         const uint32_t mk = *(const uint32_t *)(fmask + i);
+        float u[4];
+        if (f == 4) {
+            up_value4(v, L, x, y, u);
+        } else {
+#pragma unroll
+            for (int e = 0; e < 4; ++e) u[e] = up_value(v, L, f, x + e, y);
+        }
         float4 o;
-        o.x = (mk & 0xffu) ? up_value(v, L, f, x, y) : 1.f;
-        o.y = (mk & 0xff00u) ? up_value(v, L, f, x + 1, y) : 1.f;
-        o.z = (mk & 0xff0000u) ? up_value(v, L, f, x + 2, y) : 1.f;
-        o.w = (mk & 0xff000000u) ? up_value(v, L, f, x + 3, y) : 1.f;
+        o.x = (mk & 0xffu) ? u[0] : 1.f;
+        o.y = (mk & 0xff00u) ? u[1] : 1.f;
+        o.z = (mk & 0xff0000u) ? u[2] : 1.f;
+        o.w = (mk & 0xff000000u) ? u[3] : 1.f;
         *(float4 *)(ao + i) = o;
         return;
     }
@@ -356,11 +395,26 @@ __global__ __launch_bounds__(256) void k_crop(Mat16 V, const float *__restrict__
 
 // Head pre-activation: conv1x1 32 -> 1 over one pixel's BF16 activations, in
 // channel order.
+// z = b5 + ((p0 + p1) + (p2 + p3)), p_g the sum, in channel order, over the
+// channels g*4 .. g*4+3 and 16+g*4 .. 16+g*4+3 -- the channels one lane group
+// holds after an MFMA, so the fused inference kernels form the same sum with
+// two lane exchanges.
 __device__ __forceinline__ float head_z(const float *w5, float b5, const uint16_t *a)
 {
-    float z = b5;
-    for (int c = 0; c < CH; ++c) z += w5[c] * bf2f(a[c]);
-    return z;
+    float p[4];
+#pragma unroll
+    for (int g = 0; g < 4; ++g) {
+        float sg = 0.f;
+#pragma unroll
+        for (int cb = 0; cb < 2; ++cb)
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                const int c = cb * 16 + g * 4 + i;
+                sg += w5[c] * bf2f(a[c]);
+            }
+        p[g] = sg;
+    }
+    return b5 + ((p[0] + p[1]) + (p[2] + p[3]));
 }
 
 // Head of the network fused into the last convolution (inference): w5 = the
@@ -422,6 +476,7 @@ __device__ __forceinline__ void load_weights(ConvWeights<CIN> &cw, const uint16_
 // the banks.
 constexpr int kTileW = 64, kTileH = 8;
 constexpr int kStageW = kTileW + 2, kStageH = kTileH + 2;
+constexpr int kTilePB = kTileW / 16;   // 16-pixel blocks per tile row
 constexpr int kStagePix = 40;  // bf16 per staged pixel (32 channels + 8 pad)
 
 // Tile geometry: image base pixel, phase, tile column / row.
@@ -475,13 +530,13 @@ __device__ __forceinline__ void stage_store(const ConvStage<CIN> &st, uint16_t *
 // BF16 activations bf16(relu(acc + bias)) of 32 pixels at a time go to this
 // wavefront's LDS rows (32 x 80 B), then lanes 0-31 take one pixel each
 // through the head.  LDS operations of one wavefront execute in order.
-__device__ __forceinline__ void head_row(const v4f (&acc)[4][2], const float *__restrict__ bias,
+__device__ __forceinline__ void head_row(const v4f (&acc)[kTilePB][2], const float *__restrict__ bias,
                                          const HeadArgs &hd, uint16_t *sh, const ConvTile &t, int y, int Wp,
                                          int dil)
 {
     const int lane = threadIdx.x & 63, lr = lane & 15, lq = lane >> 4;
 #pragma unroll
-    for (int h = 0; h < 2; ++h) {
+    for (int h = 0; h < kTilePB / 2; ++h) {
 #pragma unroll
         for (int q2 = 0; q2 < 2; ++q2) {
             const int pb = 2 * h + q2;
@@ -531,9 +586,9 @@ __device__ __forceinline__ void conv_tile_compute(const ConvWeights<CIN> &cw, co
     for (int r = 0; r < kTileH / 4; ++r) {
         const int jr = wave * (kTileH / 4) + r, j = t.by * kTileH + jr;
         if (j >= Hp) break;
-        v4f acc[4][2];
+        v4f acc[kTilePB][2];
 #pragma unroll
-        for (int pb = 0; pb < 4; ++pb)
+        for (int pb = 0; pb < kTilePB; ++pb)
 #pragma unroll
             for (int cb = 0; cb < 2; ++cb) acc[pb][cb] = v4f{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
@@ -541,14 +596,14 @@ __device__ __forceinline__ void conv_tile_compute(const ConvWeights<CIN> &cw, co
             const int k0 = c * 32 + lq * 8;
             const int tap = imin(k0 / CIN, 8), ci0 = k0 % CIN;
             const int srow = jr + tap / 3, scol = lr + tap % 3;
-            v8bf b[4];
+            v8bf b[kTilePB];
 #pragma unroll
-            for (int pb = 0; pb < 4; ++pb) {
+            for (int pb = 0; pb < kTilePB; ++pb) {
                 b[pb] = v8bf{};
                 if (k0 < K) b[pb] = *(const v8bf *)(s_in + (srow * kStageW + pb * 16 + scol) * kStagePix + ci0);
             }
 #pragma unroll
-            for (int pb = 0; pb < 4; ++pb) {
+            for (int pb = 0; pb < kTilePB; ++pb) {
                 acc[pb][0] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(cw.w[c][0], b[pb], acc[pb][0], 0, 0, 0);
                 acc[pb][1] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(cw.w[c][1], b[pb], acc[pb][1], 0, 0, 0);
             }
@@ -559,7 +614,7 @@ __device__ __forceinline__ void conv_tile_compute(const ConvWeights<CIN> &cw, co
             continue;
         }
 #pragma unroll
-        for (int pb = 0; pb < 4; ++pb) {
+        for (int pb = 0; pb < kTilePB; ++pb) {
             const int i = t.bx * kTileW + pb * 16 + lr;
             if (i >= Wp) continue;
             const int x = t.phx + i * dil;
@@ -751,6 +806,300 @@ static void launch_conv_batch(hipStream_t s, const BatchPixView *views, const ui
     if (n == 0) return;
     hipLaunchKernelGGL((k_conv3x3_tiled<CIN, MODE, true>), dim3(conv_workgroups(n, CIN == C0 ? 3 : 2)), dim3(256), 0, s, in, Wt, bias,
                        (const uint16_t *)nullptr, out, 0, 0, dil, n, 0, 0, views, items, hd);
+}
+
+// ---------------- fused inference: layers 1+2, then 3+4 with the head ----------------
+//
+// Inference needs no activation in HBM but the one between the two fused
+// kernels.  Layers 1 (dilation 1) and 2 (dilation 2) run on one tile of the
+// network grid: a workgroup stages its 16-channel inputs with a 3-pixel halo,
+// computes layer 1 on the tile plus a 2-pixel halo into LDS (zero where that
+// pixel lies outside the frame: the padding layer 2 sees), then layer 2 on the
+// tile.  Layers 3 (dilation 6) and 4 (dilation 12) only connect pixels of one
+// phase of the 6 x 6 grid, where they are 3x3 convolutions dilated by 1 and 2:
+// a workgroup takes a tile of one phase grid the same way and ends with the
+// head, summed across lanes.  Every output is the same sum in the same order
+// as the layer-by-layer kernels (same K chunks, same BF16 rounding between
+// layers, head_z's order): the results are identical bit for bit.  Each
+// layer is computed over a flat list of its region's pixels, 16 per MFMA
+// column block.  One tile per workgroup, two workgroups per CU (LDS): the
+// tile sizes put a 1920 x 1080 frame's grids (480 x 270) in one round of the
+// 256 CUs (432 and 504 tiles).
+constexpr int kF12W = 64, kF12H = 5;                    // layer-2 output tile
+constexpr int kF12OW = kF12W + 4, kF12OH = kF12H + 4;   // layer-1 outputs it reads
+constexpr int kF12IW = kF12W + 6, kF12IH = kF12H + 6;   // layer-1 inputs
+constexpr int kF34W = 40, kF34H = 7;                    // layer-4 output tile on a phase grid
+constexpr int kF34OW = kF34W + 4, kF34OH = kF34H + 4;   // layer-3 outputs it reads
+constexpr int kF34IW = kF34W + 6, kF34IH = kF34H + 6;   // layer-3 inputs
+constexpr int kF34Dil = 6;                              // phase grid of layers 3 and 4
+
+// LDS pixel layouts of the fused kernels: 16 bf16 per 16-byte chunk, no
+// padding; the chunks of pixel q are permuted by an XOR of bits of q, so the
+// 16-byte reads of one chunk of 16 consecutive pixels (an MFMA B fragment)
+// touch distinct banks.  Offsets in bf16.
+template <int CPP>
+__device__ __forceinline__ int lds_px(int q, int c)
+{
+    return q * (CPP * 8) + c * 8;
+}
+
+// Staging of a (RW x RH) region of a (phase) grid, global -> registers ->
+// LDS: pixel (i, j) of the region is grid pixel (i0 + i, j0 + j), i.e.
+// network pixel (phx + step * (i0 + i), phy + step * (j0 + j)); zeros outside
+// the Wg x Hg grid.  CPP 16-byte chunks per pixel.  All loads are issued
+// before the first store.
+template <int RW, int RH, int CPP>
+__device__ __forceinline__ void stage_region(uint16_t *s, const uint16_t *__restrict__ in, size_t base, int lW,
+                                             int phx, int phy, int step, int i0, int j0, int Wg, int Hg)
+{
+    constexpr int TOT = RW * RH * CPP, N = (TOT + 255) / 256;
+    uint4 v[N];
+#pragma unroll
+    for (int q = 0; q < N; ++q) {
+        const int e = (int)threadIdx.x + 256 * q;
+        v[q] = make_uint4(0u, 0u, 0u, 0u);
+        if (e < TOT) {
+            const int pix = e / CPP, c = e % CPP;
+            const int i = i0 + pix % RW, j = j0 + pix / RW;
+            if (i >= 0 && i < Wg && j >= 0 && j < Hg)
+                v[q] = *(const uint4 *)(in + (base + (size_t)(phy + j * step) * lW + phx + (size_t)i * step) *
+                                                 (CPP * 8) +
+                                        c * 8);
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < N; ++q) {
+        const int e = (int)threadIdx.x + 256 * q;
+        if (e < TOT) *(uint4 *)(s + lds_px<CPP>(e / CPP, e % CPP)) = v[q];
+    }
+}
+
+// bf16(relu(acc + bias)) of one lane's 2 x 4 outputs (channels cb*16 + lq*4
+// + 0..3), packed.
+__device__ __forceinline__ uint64_t relu_pack(const v4f &acc, const float (&bb)[4])
+{
+    uint16_t rr[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        const float v = acc[q] + bb[q];
+        rr[q] = f2bf(v > 0.f ? v : 0.f);
+    }
+    return (uint64_t)rr[0] | ((uint64_t)rr[1] << 16) | ((uint64_t)rr[2] << 32) | ((uint64_t)rr[3] << 48);
+}
+
+// One layer's 16-pixel column block: output pixel p of a region of row pitch
+// RW reads input pixel (p / RW + dy * D, p % RW + dx * D) of an IW-wide LDS
+// region for tap (dy, dx).  Returns the two 16-channel accumulators.
+template <int CIN, int RW, int NPIX, int IW, int D>
+__device__ __forceinline__ void conv_block(const ConvWeights<CIN> &cw, const uint16_t *s_in, int p, v4f (&acc)[2])
+{
+    constexpr int K = ConvWeights<CIN>::K, KC = ConvWeights<CIN>::KC, CPP = CIN / 8;
+    const int lq = (threadIdx.x & 63) >> 4;
+    const int pc = p < NPIX ? p : NPIX - 1;
+    const int ry = pc / RW, rx = pc % RW;
+    acc[0] = v4f{0.f, 0.f, 0.f, 0.f};
+    acc[1] = v4f{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int c = 0; c < KC; ++c) {
+        const int k0 = c * 32 + lq * 8;
+        const int tap = imin(k0 / CIN, 8), ci0 = k0 % CIN;
+        v8bf b = v8bf{};
+        if (k0 < K) b = *(const v8bf *)(s_in + lds_px<CPP>((ry + (tap / 3) * D) * IW + rx + (tap % 3) * D, ci0 / 8));
+        acc[0] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(cw.w[c][0], b, acc[0], 0, 0, 0);
+        acc[1] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(cw.w[c][1], b, acc[1], 0, 0, 0);
+    }
+}
+
+// A layer over the NPIX pixels of a region into LDS (32 channels, lds_px
+// layout), zero where `inside` says the pixel is off the grid.  Wavefront w
+// takes column blocks w, w + 4, ...
+template <int CIN, int RW, int NPIX, int IW, int D, typename Inside>
+__device__ __forceinline__ void layer_to_lds(const ConvWeights<CIN> &cw, const uint16_t *s_in,
+                                             const float *__restrict__ bias, uint16_t *s_out, Inside inside)
+{
+    constexpr int NBLK = (NPIX + 15) / 16;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, lr = lane & 15, lq = lane >> 4;
+    float bb[2][4];
+#pragma unroll
+    for (int cb = 0; cb < 2; ++cb) {
+        const float4 bv = *(const float4 *)(bias + cb * 16 + lq * 4);
+        bb[cb][0] = bv.x; bb[cb][1] = bv.y; bb[cb][2] = bv.z; bb[cb][3] = bv.w;
+    }
+#pragma unroll 1
+    for (int blk = wave; blk < NBLK; blk += 4) {
+        const int p = blk * 16 + lr;
+        v4f acc[2];
+        conv_block<CIN, RW, NPIX, IW, D>(cw, s_in, p, acc);
+        if (p >= NPIX) continue;
+        const bool in = inside(p / RW, p % RW);
+#pragma unroll
+        for (int cb = 0; cb < 2; ++cb)
+            *(uint64_t *)(s_out + lds_px<4>(p, cb * 2 + (lq >> 1)) + (lq & 1) * 4) = in ? relu_pack(acc[cb], bb[cb]) : 0ull;
+    }
+}
+
+// Tile of a fused launch: image base, grid size, phase, tile column / row.
+struct FusedTile {
+    size_t base;
+    int lW, lH, phx, phy, bx, by;
+};
+
+template <bool BATCH>
+__device__ __forceinline__ FusedTile fused_tile(const BatchPixView &one, const BatchPixView *__restrict__ views,
+                                                const uint2 *__restrict__ items, int t, int tiles_x, int tiles_y,
+                                                int dil)
+{
+    FusedTile ft;
+    if (BATCH) {
+        const uint2 it = items[t];
+        const BatchPixView &bv = views[it.x & 0xff];
+        const int ph = (int)(it.x >> 8);
+        ft.base = bv.lo_off;
+        ft.lW = bv.lW;
+        ft.lH = bv.lH;
+        ft.phx = ph % dil;
+        ft.phy = ph / dil;
+        ft.bx = (int)(it.y & 0xffff);
+        ft.by = (int)(it.y >> 16);
+    } else {
+        int ph, k;
+        tile_order(t, tiles_x * tiles_y, dil, ph, k);
+        ft.base = 0;
+        ft.lW = one.lW;
+        ft.lH = one.lH;
+        ft.phx = ph % dil;
+        ft.phy = ph / dil;
+        ft.bx = k % tiles_x;
+        ft.by = k / tiles_x;
+    }
+    return ft;
+}
+
+// Layers 1 and 2, one tile per workgroup: `in` 16-channel inputs -> `out`
+// layer-2 activations.
+template <bool BATCH>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k_nao_l12(
+    BatchPixView one, const BatchPixView *__restrict__ views, const uint2 *__restrict__ items, int n_tiles,
+    int tiles_x, int tiles_y, const uint16_t *__restrict__ in, const uint16_t *__restrict__ W1,
+    const float *__restrict__ b1, const uint16_t *__restrict__ W2, const float *__restrict__ b2,
+    uint16_t *__restrict__ out)
+{
+    __shared__ __attribute__((aligned(16))) uint16_t s_x[kF12IW * kF12IH * C0];
+    __shared__ __attribute__((aligned(16))) uint16_t s_a[kF12OW * kF12OH * CH];
+    constexpr int NPIX = kF12W * kF12H, NBLK = (NPIX + 15) / 16;
+    if ((int)blockIdx.x >= n_tiles) return;
+    const FusedTile t = fused_tile<BATCH>(one, views, items, (int)blockIdx.x, tiles_x, tiles_y, 1);
+    const int x0 = t.bx * kF12W, y0 = t.by * kF12H, lW = t.lW, lH = t.lH;
+    ConvWeights<C0> w1;
+    load_weights<C0>(w1, W1);
+    stage_region<kF12IW, kF12IH, C0 / 8>(s_x, in, t.base, lW, 0, 0, 1, x0 - 3, y0 - 3, lW, lH);
+    ConvWeights<CH> w2;
+    load_weights<CH>(w2, W2);
+    __syncthreads();
+    layer_to_lds<C0, kF12OW, kF12OW * kF12OH, kF12IW, 1>(w1, s_x, b1, s_a, [=](int ry, int rx) {
+        const int gx = x0 - 2 + rx, gy = y0 - 2 + ry;
+        return gx >= 0 && gx < lW && gy >= 0 && gy < lH;
+    });
+    __syncthreads();
+    // layer 2 (dilation 2) over the tile's pixels: output (oy, ox) reads
+    // layer-1 region pixel (oy + 2 dy, ox + 2 dx)
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, lr = lane & 15, lq = lane >> 4;
+    float bb[2][4];
+#pragma unroll
+    for (int cb = 0; cb < 2; ++cb) {
+        const float4 bv = *(const float4 *)(b2 + cb * 16 + lq * 4);
+        bb[cb][0] = bv.x; bb[cb][1] = bv.y; bb[cb][2] = bv.z; bb[cb][3] = bv.w;
+    }
+#pragma unroll 1
+    for (int blk = wave; blk < NBLK; blk += 4) {
+        const int p = blk * 16 + lr;
+        v4f acc[2];
+        conv_block<CH, kF12W, NPIX, kF12OW, 2>(w2, s_a, p, acc);
+        const int gx = x0 + p % kF12W, gy = y0 + p / kF12W;
+        if (p >= NPIX || gx >= lW || gy >= lH) continue;
+#pragma unroll
+        for (int cb = 0; cb < 2; ++cb)
+            *(uint64_t *)(out + (t.base + (size_t)gy * lW + gx) * CH + cb * 16 + lq * 4) = relu_pack(acc[cb], bb[cb]);
+    }
+}
+
+// Layers 3 and 4 and the head, one tile of a phase grid per workgroup: `in`
+// layer-2 activations -> network-grid AO (1 where the network pixel has no
+// covered frame pixel).  Layer 4's weights are loaded while layer 3 runs.
+template <bool BATCH>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k_nao_l34(
+    BatchPixView one, const BatchPixView *__restrict__ views, const uint2 *__restrict__ items, int n_tiles,
+    int tiles_x, int tiles_y, const uint16_t *__restrict__ in, const uint16_t *__restrict__ W3,
+    const float *__restrict__ b3, const uint16_t *__restrict__ W4, const float *__restrict__ b4, HeadArgs hd)
+{
+    __shared__ __attribute__((aligned(16))) uint16_t s_x[kF34IW * kF34IH * CH];
+    __shared__ __attribute__((aligned(16))) uint16_t s_a[kF34OW * kF34OH * CH];
+    constexpr int NPIX = kF34W * kF34H, NBLK = (NPIX + 15) / 16;
+    if ((int)blockIdx.x >= n_tiles) return;
+    const FusedTile t = fused_tile<BATCH>(one, views, items, (int)blockIdx.x, tiles_x, tiles_y, kF34Dil);
+    const int Wp = (t.lW - t.phx + kF34Dil - 1) / kF34Dil, Hp = (t.lH - t.phy + kF34Dil - 1) / kF34Dil;
+    const int i0 = t.bx * kF34W, j0 = t.by * kF34H;
+    ConvWeights<CH> w3;
+    load_weights<CH>(w3, W3);
+    stage_region<kF34IW, kF34IH, CH / 8>(s_x, in, t.base, t.lW, t.phx, t.phy, kF34Dil, i0 - 3, j0 - 3, Wp, Hp);
+    ConvWeights<CH> w4;
+    load_weights<CH>(w4, W4);
+    __syncthreads();
+    layer_to_lds<CH, kF34OW, kF34OW * kF34OH, kF34IW, 1>(w3, s_x, b3, s_a, [=](int ry, int rx) {
+        const int i = i0 - 2 + rx, j = j0 - 2 + ry;
+        return i >= 0 && i < Wp && j >= 0 && j < Hp;
+    });
+    __syncthreads();
+    // layer 4 (dilation 2 on the phase grid) over the tile's pixels, then the
+    // head: each lane sums its 8 channels, two exchanges across the lane
+    // groups give head_z's sum
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, lr = lane & 15, lq = lane >> 4;
+    float bb[2][4], w5[2][4];
+#pragma unroll
+    for (int cb = 0; cb < 2; ++cb) {
+        const float4 bv = *(const float4 *)(b4 + cb * 16 + lq * 4);
+        bb[cb][0] = bv.x; bb[cb][1] = bv.y; bb[cb][2] = bv.z; bb[cb][3] = bv.w;
+        const float4 wv = *(const float4 *)(hd.w5 + cb * 16 + lq * 4);
+        w5[cb][0] = wv.x; w5[cb][1] = wv.y; w5[cb][2] = wv.z; w5[cb][3] = wv.w;
+    }
+    const float b5 = hd.w5[CH];
+#pragma unroll 1
+    for (int blk = wave; blk < NBLK; blk += 4) {
+        const int p = blk * 16 + lr;
+        v4f acc[2];
+        conv_block<CH, kF34W, NPIX, kF34OW, 2>(w4, s_a, p, acc);
+        float sg = 0.f;
+#pragma unroll
+        for (int cb = 0; cb < 2; ++cb)
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const float v = acc[cb][q] + bb[cb][q];
+                sg += w5[cb][q] * bf2f(f2bf(v > 0.f ? v : 0.f));
+            }
+        const float t1 = sg + __shfl_xor(sg, 16);
+        const float t2 = t1 + __shfl_xor(t1, 32);
+        const int i = i0 + p % kF34W, j = j0 + p / kF34W;
+        if (lq == 0 && p < NPIX && i < Wp && j < Hp) {
+            const size_t q = t.base + (size_t)(t.phy + j * kF34Dil) * t.lW + t.phx + (size_t)i * kF34Dil;
+            const float z = b5 + t2;
+            float v = 1.f / (1.f + expf(-z));
+            if (!hd.mask[q]) v = 1.f;
+            hd.ao[q] = v;
+        }
+    }
+}
+
+static inline void fused12_tiles(int lW, int lH, int &tx, int &ty)
+{
+    tx = (lW + kF12W - 1) / kF12W;
+    ty = (lH + kF12H - 1) / kF12H;
+}
+
+// Tiles per phase grid of layers 3-4: the widest phase grid (phase 0) sets them.
+static inline void fused34_tiles(int lW, int lH, int &tx, int &ty)
+{
+    tx = ((lW + kF34Dil - 1) / kF34Dil + kF34W - 1) / kF34W;
+    ty = ((lH + kF34Dil - 1) / kF34Dil + kF34H - 1) / kF34H;
 }
 
 
@@ -975,6 +1324,7 @@ struct fgl_nao {
     float *lao = nullptr;
     size_t icap = 0, fcap = 0;
     uint64_t igen = 0;   // bumped whenever the inference buffers move
+    int fused = 1;       // single-frame inference: fused kernels (1) or one kernel per layer (0)
     float *h_stats = nullptr;
 };
 
@@ -1021,12 +1371,16 @@ int nao_scale(const fgl_nao *n) { return n->scale; }
 
 uint64_t nao_buffers_generation(const fgl_nao *n) { return n->igen; }
 
-void nao_conv_items(int view, int W, int H, std::vector<uint2> items[4])
+void nao_conv_items(int view, int W, int H, std::vector<uint2> items[2])
 {
-    for (int l = 0; l < 4; ++l) {
-        const int d = layer_dil(l);
-        int tx, ty, n;
-        conv_tiles(W, H, 1, d, tx, ty, n);
+    // items[0]: tiles of the fused layers 1-2 (network grid), items[1]: of
+    // layers 3-4 (phase grids), in the single-frame launches' tile order
+    for (int l = 0; l < 2; ++l) {
+        const int d = l == 0 ? 1 : kF34Dil;
+        int tx, ty;
+        if (l == 0) fused12_tiles(W, H, tx, ty);
+        else fused34_tiles(W, H, tx, ty);
+        const int n = tx * ty * d * d;
         for (int u = 0; u < n; ++u) {
             int ph, k;
             tile_order(u, tx * ty, d, ph, k);
@@ -1045,18 +1399,18 @@ int nao_infer_batch(fgl_nao *n, hipStream_t s, const NaoBatchTables &t, const fl
     const BatchPixView none{};
     hipLaunchKernelGGL(k_nao_down<true>, dim3(t.n_lblocks), dim3(256), 0, s, none, t.views, t.lblk_view, f, depth,
                        gbuf, n->ifeat, n->imask, n->fmask);
-    launch_conv_batch<C0>(s, t.views, t.items[0], (int)t.n_items[0], (const uint16_t *)n->ifeat, n->w1,
-                          n->params + OFF_B1, n->ia, layer_dil(0));
-    launch_conv_batch<CH>(s, t.views, t.items[1], (int)t.n_items[1], (const uint16_t *)n->ia, n->w234,
-                          n->params + OFF_B2, n->ib, layer_dil(1));
-    launch_conv_batch<CH>(s, t.views, t.items[2], (int)t.n_items[2], (const uint16_t *)n->ib,
-                          n->w234 + (size_t)CH * K2, n->params + OFF_B3, n->ia, layer_dil(2));
-    // layer 4 with the head fused, into the network-grid AO
-    launch_conv_batch<CH, 2>(s, t.views, t.items[3], (int)t.n_items[3], (const uint16_t *)n->ia,
-                             n->w234 + (size_t)2 * CH * K2, n->params + OFF_B4, nullptr, layer_dil(3),
-                             HeadArgs{n->params + OFF_W5, n->imask, n->lao});
-    hipLaunchKernelGGL(k_nao_up<true>, dim3(t.n_blocks), dim3(64), 0, s, none, t.views, t.blk_view, f, n->lao,
-                       n->fmask, ao);
+    if (t.n_items[0])
+        hipLaunchKernelGGL(k_nao_l12<true>, dim3(t.n_items[0]), dim3(256), 0, s, none,
+                           t.views, t.items[0], (int)t.n_items[0], 0, 0, (const uint16_t *)n->ifeat, n->w1,
+                           n->params + OFF_B1, n->w234, n->params + OFF_B2, n->ia);
+    // layers 3-4 with the head, into the network-grid AO
+    if (t.n_items[1])
+        hipLaunchKernelGGL(k_nao_l34<true>, dim3(t.n_items[1]), dim3(256), 0, s, none,
+                           t.views, t.items[1], (int)t.n_items[1], 0, 0, (const uint16_t *)n->ia,
+                           n->w234 + (size_t)CH * K2, n->params + OFF_B3, n->w234 + (size_t)2 * CH * K2,
+                           n->params + OFF_B4, HeadArgs{n->params + OFF_W5, n->imask, n->lao});
+    hipLaunchKernelGGL(k_nao_up<true>, dim3((t.n_blocks + 3) / 4), dim3(256), 0, s, none, t.views, t.blk_view,
+                       (int)t.n_blocks, f, n->lao, n->fmask, ao);
     return hipGetLastError() == hipSuccess ? FGL_OK : FGL_ERR_DEVICE;
 }
 
@@ -1292,6 +1646,13 @@ int fgl_nao_read_batch(fgl_nao *n, uint16_t *feat, float *target, uint8_t *mask)
 // Neural AO of the context's current frame (depth + G-buffer) into its AO
 // target: down pass, four MFMA convolutions with the head fused into the
 // last, up pass.
+int fgl_nao_set_fused(fgl_nao *n, int fused)
+{
+    if (!n || (fused != 0 && fused != 1)) return FGL_ERR_INVALID;
+    n->fused = fused;
+    return FGL_OK;
+}
+
 int fgl_nao_infer(fgl_nao *n)
 {
     if (!n) return FGL_ERR_INVALID;
@@ -1311,19 +1672,40 @@ int fgl_nao_infer(fgl_nao *n)
         hipLaunchKernelGGL(k_nao_down<false>, dim3(((lW + 63) / 64) * ((lH + 3) / 4)), dim3(256), 0, cv.stream, one,
                            (const BatchPixView *)nullptr, (const uint8_t *)nullptr, f, cv.depth, cv.gbuf, n->ifeat,
                            n->imask, n->fmask);
-        launch_conv<C0, 0>(cv.stream, n->ifeat, n->w1, n->params + OFF_B1, nullptr, n->ia, lW, lH, 1, layer_dil(0));
-        launch_conv<CH, 0>(cv.stream, n->ia, n->w234, n->params + OFF_B2, nullptr, n->ib, lW, lH, 1, layer_dil(1));
-        launch_conv<CH, 0>(cv.stream, n->ib, n->w234 + (size_t)CH * K2, n->params + OFF_B3, nullptr, n->ia, lW, lH,
-                           1, layer_dil(2));
-        // layer 4 with the head fused: the network-grid AO (the frame's AO
-        // itself when the grid is the frame)
+        // the network-grid AO (the frame's AO itself when the grid is the frame)
         float *dst = f == 1 ? cv.ao : n->lao;
-        launch_conv_head(cv.stream, n->ia, n->w234 + (size_t)2 * CH * K2, n->params + OFF_B4,
-                         HeadArgs{n->params + OFF_W5, n->imask, dst}, lW, lH, 1, layer_dil(3));
-        if (f > 1)
-            hipLaunchKernelGGL(k_nao_up<false>, dim3(((cv.W + 63) / 64) * ((cv.H + 3) / 4)), dim3(64), 0, cv.stream,
-                               one, (const BatchPixView *)nullptr, (const uint8_t *)nullptr, f, n->lao, n->fmask,
+        const HeadArgs hd{n->params + OFF_W5, n->imask, dst};
+        if (n->fused) {
+            int tx, ty;
+            fused12_tiles(lW, lH, tx, ty);
+            int nt = tx * ty;
+            hipLaunchKernelGGL(k_nao_l12<false>, dim3(nt), dim3(256), 0, cv.stream, one,
+                               (const BatchPixView *)nullptr, (const uint2 *)nullptr, nt, tx, ty,
+                               (const uint16_t *)n->ifeat, n->w1, n->params + OFF_B1, n->w234, n->params + OFF_B2,
+                               n->ia);
+            fused34_tiles(lW, lH, tx, ty);
+            nt = tx * ty * kF34Dil * kF34Dil;
+            hipLaunchKernelGGL(k_nao_l34<false>, dim3(nt), dim3(256), 0, cv.stream, one,
+                               (const BatchPixView *)nullptr, (const uint2 *)nullptr, nt, tx, ty,
+                               (const uint16_t *)n->ia, n->w234 + (size_t)CH * K2, n->params + OFF_B3,
+                               n->w234 + (size_t)2 * CH * K2, n->params + OFF_B4, hd);
+        } else {
+            launch_conv<C0, 0>(cv.stream, n->ifeat, n->w1, n->params + OFF_B1, nullptr, n->ia, lW, lH, 1,
+                               layer_dil(0));
+            launch_conv<CH, 0>(cv.stream, n->ia, n->w234, n->params + OFF_B2, nullptr, n->ib, lW, lH, 1,
+                               layer_dil(1));
+            launch_conv<CH, 0>(cv.stream, n->ib, n->w234 + (size_t)CH * K2, n->params + OFF_B3, nullptr, n->ia,
+                               lW, lH, 1, layer_dil(2));
+            // layer 4 with the head fused
+            launch_conv_head(cv.stream, n->ia, n->w234 + (size_t)2 * CH * K2, n->params + OFF_B4, hd, lW, lH, 1,
+                             layer_dil(3));
+        }
+        if (f > 1) {
+            const int nb = ((cv.W + 63) / 64) * ((cv.H + 3) / 4);
+            hipLaunchKernelGGL(k_nao_up<false>, dim3((nb + 3) / 4), dim3(256), 0, cv.stream, one,
+                               (const BatchPixView *)nullptr, (const uint8_t *)nullptr, nb, f, n->lao, n->fmask,
                                cv.ao);
+        }
     }
     ctx_ao_written(n->ctx);
     return hipGetLastError() == hipSuccess ? FGL_OK : FGL_ERR_DEVICE;

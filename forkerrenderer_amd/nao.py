@@ -40,6 +40,7 @@ def _bind():
         "fgl_nao_gradients": (C.c_int, [P, P, P]),
         "fgl_nao_read_batch": (C.c_int, [P, P, P, P]),
         "fgl_nao_infer": (C.c_int, [P]),
+        "fgl_nao_set_fused": (C.c_int, [P, C.c_int]),
     }
     for k, (r, a) in sig.items():
         f = getattr(L, k)
@@ -152,6 +153,11 @@ class NeuralAO:
     def infer(self):
         """Neural AO of the context's current frame into its AO target."""
         self._chk(_bind().fgl_nao_infer(self._h), "infer")
+
+    def set_fused(self, fused):
+        """Inference through the two fused kernels (True, the default) or one
+        kernel per layer (False); the results are the same bits."""
+        self._chk(_bind().fgl_nao_set_fused(self._h, int(bool(fused))), "set_fused")
 
 
 # ---------------- training data: the plain pipeline on random scenes ----------------

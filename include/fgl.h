@@ -323,7 +323,7 @@ int fgl_render_scene(fgl_ctx *ctx, fgl_scene *s);
  * the frame (1, 2 or 4): per network pixel, 8 inputs from its scale x scale
  * block (the mean view-space depth, normal and position of the covered
  * pixels, and the covered fraction) -> 4 x conv3x3 (32 ch, ReLU, dilated
- * 1/2/4/8) -> conv1x1 + sigmoid, interpolated bilinearly back to the covered
+ * 1/2/6/12) -> conv1x1 + sigmoid, interpolated bilinearly back to the covered
  * frame pixels (AO 1 elsewhere).  Training runs on the device (forward,
  * backward, Adam) on network-grid crops of frames the plain pipeline
  * rendered, against the block means of their AO. */
@@ -345,6 +345,10 @@ int fgl_nao_gradients(fgl_nao *n, float *grads, float *stats);
 int fgl_nao_read_batch(fgl_nao *n, uint16_t *feat_bf16, float *target, uint8_t *mask);
 /* neural AO of the context's current frame into its AO target */
 int fgl_nao_infer(fgl_nao *n);
+/* fgl_nao_infer's kernels: 1 (default) two fused kernels (layers 1-2, layers
+ * 3-4 + head; activations of layers 1 and 3 stay on chip), 0 one kernel per
+ * layer (the training forward's); both give the same bits */
+int fgl_nao_set_fused(fgl_nao *n, int fused);
 
 /* ---- views and batched views (new; no counterpart in the reference) ----
  * A view is one deferred frame of Render::Render (src/render.cpp:24-58)

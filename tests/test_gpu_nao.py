@@ -250,3 +250,29 @@ def test_audit_falls_back_on_bad_weights(ctx, world):
     fixed, audited, failed = NA.audit_fallback(pred, plain, cover, 0, every=1)
     assert audited > 0 and failed >= 0.9 * audited, (audited, failed)
     assert np.abs(fixed - plain)[cover].mean() < np.abs(pred - plain)[cover].mean()
+
+
+@pytest.mark.parametrize("size", [(512, 384), (333, 201), (40, 24), (1920, 1080)],
+                         ids=lambda s: f"{s[0]}x{s[1]}")
+def test_fused_inference_equals_layered(ctx, world, size):
+    """The two fused inference kernels (layers 1-2; layers 3-4 + head on the
+    6 x 6 phase grids, halos recomputed in LDS) give the same bits as one
+    kernel per layer, on grids that split into whole, ragged and partial
+    tiles (network grids 128x96, 84x51, 10x6, 480x270)."""
+    sset, cfg, _ = world
+    w, h = size
+    net = NA.NeuralAO(ctx, batch=1, crop=64, params=_params(11))
+    try:
+        NA.render_frame(ctx, sset, 5011, w, h, cfg)
+        cov = ctx.read(F.TARGET_DEPTH) < 3e38
+        assert cov.any()
+        out = {}
+        for fused in (True, False):
+            net.set_fused(fused)
+            net.infer()
+            out[fused] = ctx.read(F.TARGET_AO).copy()
+        assert np.array_equal(out[True].view(np.uint32), out[False].view(np.uint32)), \
+            int((out[True] != out[False]).sum())
+        assert np.all(out[True][~cov] == 1.0) and (out[True][cov] < 1.0).any()
+    finally:
+        net.close()
