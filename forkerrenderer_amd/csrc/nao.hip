@@ -60,10 +60,17 @@ __host__ __device__ constexpr int layer_dil(int l) { return l == 0 ? 1 : l == 1 
 
 __host__ __device__ __forceinline__ uint16_t f2bf(float f)
 {
+#if defined(__HIP_DEVICE_COMPILE__)
+    // v_cvt_pk_bf16_f32 (round to nearest even): the same bits as the integer
+    // rounding below for every non-NaN float (an exhaustive comparison of all
+    // 2^32 patterns on gfx950 found no difference), in one instruction
+    return __builtin_bit_cast(uint16_t, (__bf16)f);
+#else
     uint32_t u;
     std::memcpy(&u, &f, 4);
     u += 0x7FFFu + ((u >> 16) & 1u);
     return (uint16_t)(u >> 16);
+#endif
 }
 __host__ __device__ __forceinline__ float bf2f(uint16_t h)
 {
@@ -75,39 +82,60 @@ __host__ __device__ __forceinline__ float bf2f(uint16_t h)
 
 // ---------------- features ----------------
 
-// View-space features of one G-buffer pixel: depth d = -z_view / 10,
-// normal = V3 * N, position = V * (P,1) / 10.  Pixels where nothing was
-// drawn, or whose G-buffer holds a non-finite normal or position (a
-// degenerate tangent frame makes the mapped normal NaN), are uncovered: no
-// part in the features or the loss, AO 1 -- what kernel SSAO yields there
+// Network coverage of one frame pixel: something was drawn (depth below
+// 3e38) and its G-buffer normal and position are finite (a degenerate
+// tangent frame makes the mapped normal NaN).  Uncovered pixels take no part
+// in the inputs or the loss and get AO 1 -- what kernel SSAO yields there
 // too -- so one such pixel cannot poison a weight gradient.
-__device__ __forceinline__ bool features_of(const float *V, float d, Vec3 N, Vec3 Pw, float f[7])
+__device__ __forceinline__ bool px_covered(float d, const float g[6])
 {
     bool covered = d < 3.0e38f;
-    if (covered) {
-        Vec4 vp = mv4(V, vec4(Pw.x, Pw.y, Pw.z, 1.f));
-        Vec3 vn = vec3(dot3(vec3(V[0], V[1], V[2]), N), dot3(vec3(V[4], V[5], V[6]), N),
-                       dot3(vec3(V[8], V[9], V[10]), N));
-        f[0] = -vp.z * 0.1f;
-        f[1] = vn.x; f[2] = vn.y; f[3] = vn.z;
-        f[4] = vp.x * 0.1f; f[5] = vp.y * 0.1f; f[6] = vp.z * 0.1f;
-        for (int c = 0; c < 7; ++c) covered = covered && isfinite(f[c]);
-    }
+#pragma unroll
+    for (int k = 0; k < 6; ++k) covered = covered && isfinite(g[k]);
     return covered;
+}
+
+// View-space features of a block from the means m of its covered pixels'
+// normal N (m[0..2]) and position P (m[3..5]): depth -z_view / 10, V3 N,
+// V (P, 1) / 10.  The view transform is affine, so these are the means of
+// the pixels' own features up to rounding, at one transform per block.
+__device__ __forceinline__ void block_view_features(const float *V, const float m[6], float f[7])
+{
+    const Vec4 vp = mv4(V, vec4(m[3], m[4], m[5], 1.f));
+    const Vec3 N = vec3(m[0], m[1], m[2]);
+    f[0] = -vp.z * 0.1f;
+    f[1] = dot3(vec3(V[0], V[1], V[2]), N);
+    f[2] = dot3(vec3(V[4], V[5], V[6]), N);
+    f[3] = dot3(vec3(V[8], V[9], V[10]), N);
+    f[4] = vp.x * 0.1f;
+    f[5] = vp.y * 0.1f;
+    f[6] = vp.z * 0.1f;
 }
 
 struct Mat16 {
     float m[16];
 };
 
-// Inputs of one network pixel (X, Y): the block of frame pixels
-// [X*f, X*f+f) x [Y*f, Y*f+f) clipped to the W x H frame.  feat[0..6] = mean
-// of the covered pixels' features, feat[7] = covered / block pixels in the
-// frame.  fmask (optional) receives each frame pixel's coverage; ao
-// (optional) gives *ao_mean, the covered pixels' mean AO (1 when none).
-// Returns the covered count.  Rows of four pixels are read as 16-byte loads
-// when the block is four wide, inside the frame and the rows are 16-byte
-// aligned.
+// Inputs of one network pixel (X, Y) from the block of frame pixels
+// [X*f, X*f+f) x [Y*f, Y*f+f) clipped to the W x H frame: the covered
+// pixels' normals and positions summed in row-major order from 0, times 1/n
+// (n covered), through block_view_features; feat[7] = covered / block
+// pixels in the frame; zeros when none is covered.  fmask (optional)
+// receives each frame pixel's coverage; ao (optional) gives *ao_mean, the
+// covered pixels' mean AO (1 when none).  Returns n.
+__device__ __forceinline__ void block_finish(const float *V, const float acc[6], int n, int m, float feat[NIN])
+{
+    const float inv = n ? 1.f / (float)n : 0.f;
+    float mean[6];
+#pragma unroll
+    for (int k = 0; k < 6; ++k) mean[k] = acc[k] * inv;
+    float fv[7];
+    block_view_features(V, mean, fv);
+#pragma unroll
+    for (int k = 0; k < 7; ++k) feat[k] = n ? fv[k] : 0.f;
+    feat[7] = m ? (float)n / (float)m : 0.f;
+}
+
 // A whole 4 x 4 block with 16-byte aligned rows: the 28 (32 with AO) row
 // loads are all issued before the first is used; the same sums, in the same
 // order, as the general path below.
@@ -125,7 +153,7 @@ __device__ __forceinline__ int block_features_4x4(const float *V, const float *_
         for (int k = 0; k < 6; ++k) q[dy][k + 1] = *(const float4 *)(g + (size_t)k * P + r);
         if (ao) qa[dy] = *(const float4 *)(ao + r);
     }
-    float acc[7] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+    float acc[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
     float aos = 0.f;
     int n = 0;
 #pragma unroll
@@ -137,24 +165,20 @@ __device__ __forceinline__ int block_features_4x4(const float *V, const float *_
         for (int k = 0; k < 7; ++k) c[k] = &q[dy][k].x;
 #pragma unroll
         for (int dx = 0; dx < 4; ++dx) {
-            float pf[7];
-            const bool cov = features_of(V, c[0][dx], vec3(c[1][dx], c[2][dx], c[3][dx]),
-                                         vec3(c[4][dx], c[5][dx], c[6][dx]), pf);
+            const float px[6] = {c[1][dx], c[2][dx], c[3][dx], c[4][dx], c[5][dx], c[6][dx]};
+            const bool cov = px_covered(c[0][dx], px);
             if (cov) {
                 ++n;
 #pragma unroll
-                for (int k = 0; k < 7; ++k) acc[k] += pf[k];
+                for (int k = 0; k < 6; ++k) acc[k] += px[k];
                 if (ao) aos += a[dx];
             }
             mk |= (cov ? 1u : 0u) << (8 * dx);
         }
         if (fmask) *(uint32_t *)(fmask + r0 + (size_t)dy * W) = mk;
     }
-    const float inv = n ? 1.f / (float)n : 0.f;
-#pragma unroll
-    for (int k = 0; k < 7; ++k) feat[k] = acc[k] * inv;
-    feat[7] = (float)n / 16.f;
-    if (ao_mean) *ao_mean = n ? aos * inv : 1.f;
+    block_finish(V, acc, n, 16, feat);
+    if (ao_mean) *ao_mean = n ? aos * (1.f / (float)n) : 1.f;
     return n;
 }
 
@@ -164,7 +188,7 @@ __device__ __forceinline__ int block_features(const float *V, const float *__res
                                               const float *__restrict__ ao, float *ao_mean)
 {
     const size_t P = (size_t)W * H;
-    float acc[7] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+    float acc[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
     float aos = 0.f;
     int n = 0, m = 0;
     const int x0 = X * f;
@@ -179,52 +203,22 @@ __device__ __forceinline__ int block_features(const float *V, const float *__res
         const int y = Y * f + dy;
         if (y >= H) break;
         const size_t r = (size_t)y * W + x0;
-        float d[4], nx[4], ny[4], nz[4], px[4], py[4], pz[4], a[4];
-        const int cnt = vec4 ? 4 : (W - x0 < f ? W - x0 : f);
-        if (vec4) {
-            const float4 q0 = *(const float4 *)(depth + r), q1 = *(const float4 *)(g + r),
-                         q2 = *(const float4 *)(g + P + r), q3 = *(const float4 *)(g + 2 * P + r),
-                         q4 = *(const float4 *)(g + 3 * P + r), q5 = *(const float4 *)(g + 4 * P + r),
-                         q6 = *(const float4 *)(g + 5 * P + r);
-            d[0] = q0.x; d[1] = q0.y; d[2] = q0.z; d[3] = q0.w;
-            nx[0] = q1.x; nx[1] = q1.y; nx[2] = q1.z; nx[3] = q1.w;
-            ny[0] = q2.x; ny[1] = q2.y; ny[2] = q2.z; ny[3] = q2.w;
-            nz[0] = q3.x; nz[1] = q3.y; nz[2] = q3.z; nz[3] = q3.w;
-            px[0] = q4.x; px[1] = q4.y; px[2] = q4.z; px[3] = q4.w;
-            py[0] = q5.x; py[1] = q5.y; py[2] = q5.z; py[3] = q5.w;
-            pz[0] = q6.x; pz[1] = q6.y; pz[2] = q6.z; pz[3] = q6.w;
-            if (ao) {
-                const float4 qa = *(const float4 *)(ao + r);
-                a[0] = qa.x; a[1] = qa.y; a[2] = qa.z; a[3] = qa.w;
-            }
-        }
-        uint32_t mk = 0;
-        for (int dx = 0; dx < f && dx < 4; ++dx) {
-            if (dx >= cnt) break;
-            if (!vec4) {
-                const size_t i = r + dx;
-                d[dx] = depth[i];
-                nx[dx] = g[i]; ny[dx] = g[P + i]; nz[dx] = g[2 * P + i];
-                px[dx] = g[3 * P + i]; py[dx] = g[4 * P + i]; pz[dx] = g[5 * P + i];
-                if (ao) a[dx] = ao[i];
-            }
-            float pf[7];
-            const bool cov = features_of(V, d[dx], vec3(nx[dx], ny[dx], nz[dx]), vec3(px[dx], py[dx], pz[dx]), pf);
+        const int cnt = W - x0 < f ? W - x0 : f;
+        for (int dx = 0; dx < cnt; ++dx) {
+            const size_t i = r + dx;
+            const float px[6] = {g[i], g[P + i], g[2 * P + i], g[3 * P + i], g[4 * P + i], g[5 * P + i]};
+            const bool cov = px_covered(depth[i], px);
             ++m;
             if (cov) {
                 ++n;
-                for (int c = 0; c < 7; ++c) acc[c] += pf[c];
-                if (ao) aos += a[dx];
+                for (int k = 0; k < 6; ++k) acc[k] += px[k];
+                if (ao) aos += ao[i];
             }
-            mk |= (cov ? 1u : 0u) << (8 * dx);
-            if (fmask && !vec4) fmask[r + dx] = cov ? 1 : 0;
+            if (fmask) fmask[i] = cov ? 1 : 0;
         }
-        if (fmask && vec4) *(uint32_t *)(fmask + r) = mk;
     }
-    const float inv = n ? 1.f / (float)n : 0.f;
-    for (int c = 0; c < 7; ++c) feat[c] = acc[c] * inv;
-    feat[7] = m ? (float)n / (float)m : 0.f;
-    if (ao_mean) *ao_mean = n ? aos * inv : 1.f;
+    block_finish(V, acc, n, m, feat);
+    if (ao_mean) *ao_mean = n ? aos * (1.f / (float)n) : 1.f;
     return n;
 }
 

@@ -1,8 +1,9 @@
 """CPU oracle of the neural-AO network (TEST INFRASTRUCTURE).
 
-The network runs on a grid `f` times coarser than the frame; block_inputs,
-block_targets and upsample restate the device's down pass, training
-targets and up pass in its float32 operation order.
+The network runs on a grid `f` times coarser than the frame;
+block_inputs_gbuf, block_targets and upsample restate the device's down
+pass, training targets and up pass in its float32 operation order
+(block_inputs is the generic covered-block mean).
 
 Forward and backward of: conv3x3 16->32 ReLU, 3x conv3x3 32->32 ReLU,
 conv1x1 32->1 sigmoid, 'same' zero padding, the four 3x3 convolutions
@@ -173,6 +174,69 @@ def block_inputs(feat, cov, f):
     inv = np.where(n > 0, f32(1) / np.maximum(n, 1).astype(f32), f32(0))
     out = np.zeros((lH, lW, 16), f32)
     out[..., :7] = acc * inv[..., None]
+    out[..., 7] = np.where(m > 0, n.astype(f32) / np.maximum(m, 1).astype(f32), f32(0))
+    return out, n > 0
+
+
+def gbuffer_coverage(depth, gbuf):
+    """Network coverage of each frame pixel: something drawn (depth below
+    3e38) with a finite G-buffer normal and position (planes 0-5)."""
+    with np.errstate(invalid="ignore"):
+        cov = np.asarray(depth) < 3e38
+        for k in range(6):
+            cov = cov & np.isfinite(gbuf[k])
+    return cov
+
+
+def _dot(row, v):
+    """Dot product in float32, summed from 0 in order (the device's dot3/dot4)."""
+    f32 = np.float32
+    r = np.zeros_like(v[0], dtype=f32)
+    for a, b in zip(row, v):
+        r = r + f32(a) * b
+    return r
+
+
+def block_inputs_gbuf(depth, gbuf, V, f):
+    """The device's down pass: per network pixel the block means of the
+    covered pixels' G-buffer normal N and position P (planes 0-5; summed in
+    row-major order from 0, times 1/n), then the view-space features of the
+    means -- depth -z/10, V3 N, V (P, 1) / 10 (V row-major 4x4; one dot
+    product per row) -- and the covered fraction.  The view transform is
+    affine, so this is block_inputs of the per-pixel features up to
+    rounding.  depth (H, W), gbuf (>= 6, H, W) float32.  Returns
+    (lH, lW, 16) float32 and the network coverage."""
+    f32 = np.float32
+    cov = gbuffer_coverage(depth, gbuf)
+    raw = np.stack([np.where(cov, gbuf[k], f32(0)).astype(f32) for k in range(6)], -1)
+    H, W = cov.shape
+    lW, lH = _blocks(W, H, f)
+    acc = np.zeros((lH, lW, 6), f32)
+    n = np.zeros((lH, lW), np.int64)
+    m = np.zeros((lH, lW), np.int64)
+    for dy in range(f):
+        for dx in range(f):
+            ys, xs = np.arange(lH) * f + dy, np.arange(lW) * f + dx
+            yv, xv = ys < H, xs < W
+            sub = np.zeros((lH, lW, 6), f32)
+            c = np.zeros((lH, lW), bool)
+            yy, xx = ys[yv], xs[xv]
+            sub[np.ix_(yv, xv)] = raw[np.ix_(yy, xx)]
+            c[np.ix_(yv, xv)] = cov[np.ix_(yy, xx)]
+            acc = np.where(c[..., None], acc + sub, acc)
+            n += c
+            m[np.ix_(yv, xv)] += 1
+    inv = np.where(n > 0, f32(1) / np.maximum(n, 1).astype(f32), f32(0))
+    mean = acc * inv[..., None]
+    Vm = np.asarray(V, f32).reshape(4, 4)
+    N = [mean[..., 0], mean[..., 1], mean[..., 2]]
+    P = [mean[..., 3], mean[..., 4], mean[..., 5], np.ones_like(mean[..., 0])]
+    vp = [_dot(Vm[r], P) for r in range(3)]
+    vn = [_dot(Vm[r, :3], N) for r in range(3)]
+    feat = [-vp[2] * f32(0.1), vn[0], vn[1], vn[2], vp[0] * f32(0.1), vp[1] * f32(0.1), vp[2] * f32(0.1)]
+    out = np.zeros((lH, lW, 16), f32)
+    for k in range(7):
+        out[..., k] = np.where(n > 0, feat[k], f32(0))
     out[..., 7] = np.where(m > 0, n.astype(f32) / np.maximum(m, 1).astype(f32), f32(0))
     return out, n > 0
 

@@ -158,12 +158,10 @@ def test_config4_neural_ao_full_frame(ctx, scene_dir):
     assert_bitexact("depth", depth, od)
     assert_bitexact("gbuf", gbuf, og)
     # network oracle on the whole network grid
-    from tests.test_gpu_nao import features_f32
-    frame = {"depth": depth.reshape(H, W), "gbuf": gbuf.reshape(F.GBUF_PLANES, H, W),
-             "view": st["view"].astype(np.float32)}
-    feat, cov = features_f32(frame)
+    g3 = gbuf.reshape(F.GBUF_PLANES, H, W)
+    cov = R.gbuffer_coverage(depth.reshape(H, W), g3)
     S = int(ck["scale"])
-    x, lcov = R.block_inputs(feat[..., :7], cov, S)
+    x, lcov = R.block_inputs_gbuf(depth.reshape(H, W), g3, st["view"].astype(np.float32), S)
     _, y = R.forward(params, R.bf16_bits_to_float(bf16_bits(x))[None], bf16=True)
     want = R.upsample(np.where(lcov, y[0], 1.0).astype(np.float32), cov, S)
     err = np.abs(pred - want)[cov]
@@ -212,7 +210,7 @@ def test_config5_batched_views(ctx):
     within +-1 LSB (oracle lighting of the device's G-buffer and AO)."""
     from forkerrenderer_amd import nao as NA
     from oracle import nao_ref as R
-    from tests.test_gpu_nao import bf16_bits, features_f32
+    from tests.test_gpu_nao import bf16_bits
     specs = VW.view_specs()
     assert len(specs) == 64 and {s.subdiv for s in specs} == {4, 5, 6, 7, 8}
     print("config 5: generating and uploading 64 scenes", flush=True)
@@ -249,9 +247,9 @@ def test_config5_batched_views(ctx):
         assert_bitexact(f"view {i} depth", depth, o["depth"])
         assert_bitexact(f"view {i} gbuf", gbuf, o["gbuf"])
         ao = b.read(i, F.TARGET_AO).reshape(H, W)
-        feat, cov = features_f32({"depth": depth.reshape(H, W), "gbuf": gbuf.reshape(F.GBUF_PLANES, H, W),
-                                  "view": o["V"].astype(np.float32)})
-        x, lcov = R.block_inputs(feat[..., :7], cov, NA.SCALE)
+        g3 = gbuf.reshape(F.GBUF_PLANES, H, W)
+        cov = R.gbuffer_coverage(depth.reshape(H, W), g3)
+        x, lcov = R.block_inputs_gbuf(depth.reshape(H, W), g3, o["V"].astype(np.float32), NA.SCALE)
         _, y = R.forward(params, R.bf16_bits_to_float(bf16_bits(x))[None], bf16=True)
         want_ao = R.upsample(np.where(lcov, y[0], 1.0).astype(np.float32), cov, NA.SCALE)
         err = np.abs(ao - want_ao)[cov]

@@ -41,36 +41,10 @@ def _params(seed=7):
     return p.astype(np.float32)
 
 
-def features_f32(frame):
-    """7 view-space features in float32, operation order of the device."""
-    f32 = np.float32
-    V = frame["view"].reshape(4, 4)
-    g = frame["gbuf"]
-    N = [g[0], g[1], g[2]]
-    P = [g[3], g[4], g[5], np.ones_like(g[3])]
-
-    def dot(row, v):
-        r = np.zeros_like(v[0], dtype=f32)
-        for a, b in zip(row, v):
-            r = r + f32(a) * b
-        return r
-    vp = [dot(V[r], P) for r in range(3)]
-    vn = [dot(V[r, :3], N) for r in range(3)]
-    feat = [-vp[2] * f32(0.1), vn[0], vn[1], vn[2], vp[0] * f32(0.1), vp[1] * f32(0.1), vp[2] * f32(0.1)]
-    cov = frame["depth"] < 3e38
-    with np.errstate(invalid="ignore"):
-        for c in range(7):
-            cov &= np.isfinite(feat[c])   # non-finite G-buffer values count as uncovered
-    out = np.zeros(cov.shape + (16,), np.float32)
-    for c in range(7):
-        out[..., c] = np.where(cov, feat[c], f32(0))
-    return out, cov
-
-
 def net_inputs(frame):
     """The network grid's inputs and coverage (oracle down pass)."""
-    feat, cov = features_f32(frame)
-    x, lcov = R.block_inputs(feat[..., :7], cov, S)
+    cov = R.gbuffer_coverage(frame["depth"], frame["gbuf"])
+    x, lcov = R.block_inputs_gbuf(frame["depth"], frame["gbuf"], frame["view"], S)
     return x, lcov, cov
 
 

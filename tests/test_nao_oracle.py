@@ -147,3 +147,55 @@ def test_crop_loss_mask_keeps_pixels_the_crop_sees_whole():
     assert m[:, 0].any() and m[63, 30] and not m[0, 30] and not m[30, 63]
     cov[40, 40] = False
     assert not R.crop_loss_mask(cov, 0, 36, 64, 200, 100)[40, 40]
+
+
+def test_block_inputs_gbuf_pins():
+    """The down pass from the G-buffer (block means of normal and position,
+    then the view transform): with V = identity the channels are exactly the
+    generic covered-block means of N and of P / 10 (and -Pz / 10); with a
+    rigid view transform they equal the block means of the per-pixel
+    features in float64 to float32 rounding (the transform is affine); a
+    non-finite normal or position, or depth >= 3e38, uncovers the pixel."""
+    rng = np.random.default_rng(5)
+    H, W, f = 21, 30, 4
+    g = rng.normal(0, 1, (19, H, W)).astype(np.float32)
+    depth = rng.uniform(0, 1, (H, W)).astype(np.float32)
+    depth[rng.random((H, W)) < 0.2] = np.float32(3.4e38)
+    depth[3, 5] = depth[10, 11] = np.float32(0.5)
+    g[0, 3, 5] = np.nan
+    g[4, 10, 11] = np.inf
+    cov = R.gbuffer_coverage(depth, g)
+    assert not cov[3, 5] and not cov[10, 11] and not cov[depth > 3e38].any()
+    assert cov.sum() == ((depth < 3e38).sum() - 2)
+    f32 = np.float32
+    x, lcov = R.block_inputs_gbuf(depth, g, np.eye(4, dtype=f32), f)
+    raw = np.stack([np.where(cov, g[k], 0) for k in range(6)], -1).astype(f32)
+    mN, lc2 = R.block_inputs(np.concatenate([raw[..., :3], np.zeros((H, W, 4), f32)], -1), cov, f)
+    mP, _ = R.block_inputs(np.concatenate([raw[..., 3:6], np.zeros((H, W, 4), f32)], -1), cov, f)
+    assert np.array_equal(lcov, lc2)
+    assert np.array_equal(x[..., 1:4], mN[..., 0:3])
+    assert np.array_equal(x[..., 4:7], np.where(lcov[..., None], mP[..., 0:3] * f32(0.1), 0))
+    assert np.array_equal(x[..., 0], np.where(lcov, -mP[..., 2] * f32(0.1), 0))
+    assert np.array_equal(x[..., 7], mN[..., 7])
+    # a rigid view transform: rotation about y and x, translation
+    a, b = 0.7, -0.3
+    Ry = np.array([[np.cos(a), 0, np.sin(a)], [0, 1, 0], [-np.sin(a), 0, np.cos(a)]])
+    Rx = np.array([[1, 0, 0], [0, np.cos(b), -np.sin(b)], [0, np.sin(b), np.cos(b)]])
+    V = np.eye(4)
+    V[:3, :3] = Rx @ Ry
+    V[:3, 3] = [0.2, -1.5, -3.0]
+    x, lcov = R.block_inputs_gbuf(depth, g, V.astype(f32), f)
+    V32 = V.astype(f32).astype(np.float64)
+    P = np.concatenate([raw[..., 3:6].astype(np.float64), np.ones((H, W, 1))], -1)
+    vp = P @ V32.T
+    vn = raw[..., :3].astype(np.float64) @ V32[:3, :3].T
+    per = np.concatenate([-vp[..., 2:3] * 0.1, vn, vp[..., :3] * 0.1], -1)
+    for Y in range(x.shape[0]):
+        for X in range(x.shape[1]):
+            blk = (slice(Y * f, min(Y * f + f, H)), slice(X * f, min(X * f + f, W)))
+            c = cov[blk]
+            if not c.any():
+                assert not lcov[Y, X] and not x[Y, X].any()
+                continue
+            want = per[blk][c].mean(0)
+            assert np.allclose(x[Y, X, :7], want, rtol=2e-6, atol=2e-6), (Y, X)
