@@ -1033,6 +1033,8 @@ int fgl_ssao(fgl_ctx *c, const fgl_ssao_cfg *cfg)
         const float a = std::fabs(sa.kernel[k]);
         if (!(a <= sa.kmax)) sa.kmax = a;   // a NaN component makes kmax NaN
     }
+    // the whole AO stage (SSAO and blur, with the gap between them) as one span
+    KernelTimer stage(c, "ssao_blur");
     if (cfg->stochastic) {
         // 32 hemisphere samples per pixel, pixels in row order, from the stream
         const size_t n = (size_t)c->W * c->H * 32;

@@ -222,8 +222,10 @@ __device__ __forceinline__ int block_features(const float *V, const float *__res
     return n;
 }
 
-// The 16 stored BF16 input channels of one network pixel.
-__device__ __forceinline__ void store_inputs(uint16_t *__restrict__ dst, const float feat[NIN])
+// The 16 stored BF16 input channels of one network pixel; with
+// zero_pad = false only the 8 real ones (the buffer's channels 8-15 were
+// zeroed when it was allocated and are never written).
+__device__ __forceinline__ void store_inputs(uint16_t *__restrict__ dst, const float feat[NIN], bool zero_pad = true)
 {
     uint16_t b[8];
 #pragma unroll
@@ -231,7 +233,7 @@ __device__ __forceinline__ void store_inputs(uint16_t *__restrict__ dst, const f
     uint4 *o = (uint4 *)dst;
     o[0] = make_uint4(b[0] | ((uint32_t)b[1] << 16), b[2] | ((uint32_t)b[3] << 16), b[4] | ((uint32_t)b[5] << 16),
                       b[6] | ((uint32_t)b[7] << 16));
-    o[1] = make_uint4(0u, 0u, 0u, 0u);
+    if (zero_pad) o[1] = make_uint4(0u, 0u, 0u, 0u);
 }
 
 // Per-view pixel layout: the view's frame pixels start at px_off, its
@@ -264,7 +266,7 @@ __global__ __launch_bounds__(256) void k_nao_down(BatchPixView one, const BatchP
     const int n = block_features(v.view, depth + v.px_off, gbuf + (size_t)kGbufPlanes * v.px_off, v.W, v.H, X, Y, f,
                                  in, fmask + v.px_off, nullptr, nullptr);
     const size_t o = v.lo_off + (size_t)Y * v.lW + X;
-    store_inputs(feat + o * C0, in);
+    store_inputs(feat + o * C0, in, false);
     lmask[o] = n > 0 ? 1 : 0;
 }
 
@@ -842,18 +844,20 @@ __device__ __forceinline__ int lds_px(int q, int c)
 // network pixel (phx + step * (i0 + i), phy + step * (j0 + j)); zeros outside
 // the Wg x Hg grid.  CPP 16-byte chunks per pixel.  All loads are issued
 // before the first store.
-template <int RW, int RH, int CPP>
+// LOAD = the leading chunks of each pixel read from memory (the rest are the
+// input's zero channels and staged as zeros).
+template <int RW, int RH, int CPP, int LOAD = CPP>
 __device__ __forceinline__ void stage_region(uint16_t *s, const uint16_t *__restrict__ in, size_t base, int lW,
                                              int phx, int phy, int step, int i0, int j0, int Wg, int Hg)
 {
-    constexpr int TOT = RW * RH * CPP, N = (TOT + 255) / 256;
+    constexpr int TOT = RW * RH * LOAD, N = (TOT + 255) / 256;
     uint4 v[N];
 #pragma unroll
     for (int q = 0; q < N; ++q) {
         const int e = (int)threadIdx.x + 256 * q;
         v[q] = make_uint4(0u, 0u, 0u, 0u);
         if (e < TOT) {
-            const int pix = e / CPP, c = e % CPP;
+            const int pix = e / LOAD, c = e % LOAD;
             const int i = i0 + pix % RW, j = j0 + pix / RW;
             if (i >= 0 && i < Wg && j >= 0 && j < Hg)
                 v[q] = *(const uint4 *)(in + (base + (size_t)(phy + j * step) * lW + phx + (size_t)i * step) *
@@ -864,7 +868,11 @@ __device__ __forceinline__ void stage_region(uint16_t *s, const uint16_t *__rest
 #pragma unroll
     for (int q = 0; q < N; ++q) {
         const int e = (int)threadIdx.x + 256 * q;
-        if (e < TOT) *(uint4 *)(s + lds_px<CPP>(e / CPP, e % CPP)) = v[q];
+        if (e < TOT) {
+            *(uint4 *)(s + lds_px<CPP>(e / LOAD, e % LOAD)) = v[q];
+#pragma unroll
+            for (int z = LOAD; z < CPP; ++z) *(uint4 *)(s + lds_px<CPP>(e / LOAD, z)) = make_uint4(0u, 0u, 0u, 0u);
+        }
     }
 }
 
@@ -986,7 +994,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
     const int x0 = t.bx * kF12W, y0 = t.by * kF12H, lW = t.lW, lH = t.lH;
     ConvWeights<C0> w1;
     load_weights<C0>(w1, W1);
-    stage_region<kF12IW, kF12IH, C0 / 8>(s_x, in, t.base, lW, 0, 0, 1, x0 - 3, y0 - 3, lW, lH);
+    stage_region<kF12IW, kF12IH, C0 / 8, NIN / 8>(s_x, in, t.base, lW, 0, 0, 1, x0 - 3, y0 - 3, lW, lH);
     ConvWeights<CH> w2;
     load_weights<CH>(w2, W2);
     __syncthreads();
@@ -1349,6 +1357,8 @@ int nao_reserve(fgl_nao *n, size_t frame_px, size_t net_px)
             nao_alloc((void **)&n->ib, net_px * CH * 2) || nao_alloc((void **)&n->imask, net_px) ||
             nao_alloc((void **)&n->lao, net_px * 4))
             return FGL_ERR_NOMEM;
+        // input channels 8-15 stay zero: the down pass writes channels 0-7 only
+        if (hipMemset(n->ifeat, 0, net_px * C0 * 2) != hipSuccess) return FGL_ERR_DEVICE;
         n->icap = net_px;
     }
     if (frame_px > n->fcap) {
