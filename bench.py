@@ -148,20 +148,26 @@ def run_neural_ao(steps, warmup, heldout_frames=200):
     ctx.sync()
 
     def timed_frames(only):
-        # frames of the plain pipeline, each followed by the network; HIP
-        # events only around `only` (None: around every kernel)
+        # frames of the plain pipeline (SSAO + blur, lighting), each followed
+        # by the network and the lighting pass again, now reading the network
+        # AO from its grid; HIP events only around `only` (None: every kernel)
         ctx.read_timings()
         ctx.enable_timing(True, only)
         for _ in range(steps):
             ctx.render_scene(sc)
             net.infer()
+            ctx.draw_screen_space_pixels()
         ctx.sync()
         got = ctx.read_timings()
         ctx.enable_timing(False)
         return got
-    # each AO stage as one span: SSAO + blur (two kernels), the network (four)
+    # each AO stage as one span: SSAO + blur (two kernels), the network (down
+    # pass, two fused kernels); the lighting pass's difference between reading
+    # the AO plane and interpolating the network grid counts to the network
     plain_v = [ms for name, ms in timed_frames("ssao_blur") if name == "ssao_blur"]
     nao_v = [ms for name, ms in timed_frames("nao_conv") if name == "nao_conv"]
+    light_v = [ms for name, ms in timed_frames("lighting") if name == "lighting"]
+    light_plain, light_net = float(np.mean(light_v[0::2])), float(np.mean(light_v[1::2]))
     per = {}
     for name, ms in timed_frames(None):
         per[name] = per.get(name, 0.0) + ms / steps
@@ -174,7 +180,8 @@ def run_neural_ao(steps, warmup, heldout_frames=200):
     err = np.abs(pred - plain)[cover]
     fixed, audited, failed = NA.audit_fallback(pred, plain, cover, 0)
     plain_ms = float(np.mean(plain_v))
-    nao_ms = float(np.mean(nao_v))   # the whole network: down pass, two fused kernels, up pass
+    nao_span_ms = float(np.mean(nao_v))   # down pass and the two fused kernels
+    nao_ms = nao_span_ms + (light_net - light_plain)
     scale = int(ck["scale"])
     net_px = -(-W // scale) * -(-H // scale)
     # per network pixel: layer 1 on the 8 real inputs, three 32-channel
@@ -194,7 +201,10 @@ def run_neural_ao(steps, warmup, heldout_frames=200):
             "plain_ssao_blur_ms": plain_ms, "neural_ao_ms": nao_ms,
             "speedup": plain_ms / nao_ms if nao_ms else None,
             "timing": "each stage as one span of HIP events on the frame's stream (SSAO + blur: two kernels; "
-                      "network: down pass, two fused convolution kernels, up pass), mean over the frames",
+                      "network: down pass and two fused convolution kernels, which leave the AO on the network "
+                      "grid) plus, for the network, the lighting pass's time reading the grid minus its time "
+                      "reading the AO plane; means over the frames",
+            "neural_ao_span_ms": nao_span_ms, "lighting_ms": {"ao_plane": light_plain, "network_grid": light_net},
             "kernels_ms": {k: v for k, v in per.items() if k.startswith("nao") or k in ("ssao", "blur_gaussian")},
             "conv_roofline": {"bound": "mfma", "achieved": conv_tflops, "peak": 2500.0, "unit": "TFLOP/s",
                               "frac": conv_tflops / 2500.0 if conv_tflops else None,

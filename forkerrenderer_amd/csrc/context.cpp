@@ -100,6 +100,12 @@ struct fgl_ctx {
 
     // device address of the allocation-free entry point's scratch array
     float *noalloc_scratch = nullptr;
+    // neural AO still on its network grid (lao != nullptr): the AO target's
+    // content until something reads or replaces it; owner = the network
+    // whose buffers hold it
+    NetAo ao_net{};
+    const void *ao_net_owner = nullptr;
+    std::vector<void *> naos;   // networks created on this context
 
     // timing
     bool timing = false;
@@ -305,6 +311,7 @@ int fgl_ctx_create(int device, fgl_ctx **out)
 void fgl_ctx_destroy(fgl_ctx *c)
 {
     if (!c) return;
+    for (void *n : c->naos) nao_detach(n);   // networks outliving their context stop using it
     hipSetDevice(c->device);
     hipStreamSynchronize(c->stream);
     for (void *p : c->texture_mem) hipFree(p);
@@ -472,6 +479,11 @@ static int materialize(fgl_ctx *c, bool depth, bool gbuf, bool fb, bool ao, bool
         for (int k = 0; k < 3; ++k) launch_fill_f32((float *)c->fb.p + k * P, c->fb_clear[k], P, c->stream);
         c->pend_fb = false;
     }
+    if (ao && c->ao_net.lao) {
+        nao_launch_up(c->ao_net, c->W, c->H, (float *)c->ao.p, c->stream);
+        c->ao_net = NetAo{};
+        c->ao_net_owner = nullptr;
+    }
     if (ao && c->pend_ao) { launch_fill_f32((float *)c->ao.p, 1.f, P, c->stream); c->pend_ao = false; }
     if (shadow && c->pend_shadow) {
         launch_fill_f32((float *)c->shadow.p, 0.f, SP, c->stream);
@@ -499,6 +511,8 @@ int fgl_init_targets(fgl_ctx *c, int w, int h)
     c->W = w;
     c->H = h;
     c->pend_depth = c->pend_gbuf = c->pend_fb = c->pend_ao = true;
+    c->ao_net = NetAo{};
+    c->ao_net_owner = nullptr;
     c->fb_clear[0] = c->fb_clear[1] = c->fb_clear[2] = 0.f;
     return FGL_OK;
 }
@@ -522,6 +536,8 @@ int fgl_init_buffer(fgl_ctx *c, int which, int w, int h)
         c->pend_depth = true;
     } else {
         c->pend_gbuf = c->pend_ao = true;
+        c->ao_net = NetAo{};
+        c->ao_net_owner = nullptr;
     }
     return FGL_OK;
 }
@@ -1000,7 +1016,9 @@ int fgl_draw_screen_space_pixels(fgl_ctx *c)
     }
     {
         KernelTimer t(c, "lighting");
-        launch_lighting(u, (const float *)c->gbuf.p, (const float *)c->ao.p, c->pend_ao ? 1 : 0,
+        // a pending network AO is interpolated from its grid by the lighting
+        // pass itself (the same values the up pass would write)
+        launch_lighting(u, (const float *)c->gbuf.p, (const float *)c->ao.p, c->pend_ao ? 1 : 0, c->ao_net,
                         (const float *)c->shadow.p, ss, (float *)c->fb.p, c->stream);
     }
     c->pend_fb = false;  // every pixel written
@@ -1019,6 +1037,8 @@ int fgl_ssao(fgl_ctx *c, const fgl_ssao_cfg *cfg)
     if (!c->gbuf.p) return fail(c, FGL_ERR_INVALID, "targets not initialised");
     if ((rc = materialize(c, true, true, false, false, false))) return rc;
     c->pend_ao = false;  // every pixel written
+    c->ao_net = NetAo{};
+    c->ao_net_owner = nullptr;
     FrameUniforms u;
     build_uniforms(c, u);
     SsaoArgs sa;
@@ -1188,7 +1208,11 @@ int fgl_write_target(fgl_ctx *c, int which, const void *src, size_t bytes)
     if (which == FGL_TARGET_DEPTH) c->pend_depth = false;
     if (which == FGL_TARGET_GBUFFER) c->pend_gbuf = false;
     if (which == FGL_TARGET_FRAME) c->pend_fb = false;
-    if (which == FGL_TARGET_AO) c->pend_ao = false;
+    if (which == FGL_TARGET_AO) {
+        c->pend_ao = false;
+        c->ao_net = NetAo{};
+        c->ao_net_owner = nullptr;
+    }
     HIPCHK(c, hipStreamSynchronize(c->stream));
     HIPCHK(c, hipMemcpy(p, src, n, hipMemcpyHostToDevice));
     return FGL_OK;
@@ -1349,12 +1373,12 @@ int fgl_sort_pairs(fgl_ctx *c, uint32_t *keys, uint32_t *vals, size_t n, int bit
 
 namespace fgl {
 
-int ctx_view(fgl_ctx *c, CtxView *v)
+int ctx_view(fgl_ctx *c, CtxView *v, bool need_ao)
 {
     int rc = flush_pass(c);
     if (rc) return rc;
     if (!c->depth.p) return fail(c, FGL_ERR_INVALID, "targets not initialised");
-    if ((rc = materialize(c, true, true, false, true, false))) return rc;
+    if ((rc = materialize(c, true, true, false, need_ao, false))) return rc;
     v->stream = c->stream;
     v->W = c->W;
     v->H = c->H;
@@ -1368,7 +1392,43 @@ int ctx_view(fgl_ctx *c, CtxView *v)
     return FGL_OK;
 }
 
-void ctx_ao_written(fgl_ctx *c) { c->pend_ao = false; }
+void ctx_ao_written(fgl_ctx *c)
+{
+    c->pend_ao = false;
+    c->ao_net = NetAo{};
+    c->ao_net_owner = nullptr;
+}
+
+void ctx_ao_from_network(fgl_ctx *c, const NetAo &net, const void *owner)
+{
+    c->pend_ao = false;
+    c->ao_net = net;
+    c->ao_net_owner = owner;
+}
+
+void ctx_drop_network_ao(fgl_ctx *c, const void *owner)
+{
+    if (c->ao_net_owner != owner) return;
+    c->ao_net = NetAo{};
+    c->ao_net_owner = nullptr;
+}
+
+int ctx_settle_network_ao(fgl_ctx *c, const void *owner)
+{
+    if (!c->ao_net.lao || c->ao_net_owner != owner) return FGL_OK;
+    return materialize(c, false, false, false, true, false);
+}
+
+void ctx_register_nao(fgl_ctx *c, void *n) { c->naos.push_back(n); }
+
+void ctx_unregister_nao(fgl_ctx *c, void *n)
+{
+    for (size_t k = 0; k < c->naos.size(); ++k)
+        if (c->naos[k] == n) {
+            c->naos.erase(c->naos.begin() + (long)k);
+            break;
+        }
+}
 
 int ctx_fail(fgl_ctx *c, int code, const char *msg) { return fail(c, code, msg); }
 

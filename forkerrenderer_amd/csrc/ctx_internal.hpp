@@ -25,7 +25,7 @@ struct CtxView {
 
 // Fills v from the context; flushes pending draws and materialises pending
 // clears of depth, G-buffer and AO first.
-int ctx_view(fgl_ctx *c, CtxView *v);
+int ctx_view(fgl_ctx *c, CtxView *v, bool need_ao = true);
 // Marks the AO target as written in full (no pending clear).
 void ctx_ao_written(fgl_ctx *c);
 int ctx_fail(fgl_ctx *c, int code, const char *msg);
@@ -72,6 +72,20 @@ struct NaoBatchTables {
     uint32_t n_items[2];
 };
 int nao_reserve(fgl_nao *n, size_t frame_px, size_t net_px);
+// The up pass of a network AO still on its grid into a W x H AO plane.
+void nao_launch_up(const NetAo &net, int W, int H, float *ao, hipStream_t s);
+// A network whose context is being destroyed: it stops using the context.
+void nao_detach(void *n);
+// The context's AO target is `net` (the lighting pass reads it from its grid;
+// reading or replacing the target first runs the up pass).  owner: the
+// network whose buffers hold it.
+void ctx_ao_from_network(fgl_ctx *c, const NetAo &net, const void *owner);
+// Forget / write out a pending network AO held by `owner`'s buffers (before
+// they are overwritten / moved or freed).
+void ctx_drop_network_ao(fgl_ctx *c, const void *owner);
+int ctx_settle_network_ao(fgl_ctx *c, const void *owner);
+void ctx_register_nao(fgl_ctx *c, void *n);
+void ctx_unregister_nao(fgl_ctx *c, void *n);
 // Frame pixels per network pixel, each way (1, 2 or 4).
 int nao_scale(const fgl_nao *n);
 // Changes whenever nao_reserve moves the inference buffers (a graph that

@@ -43,6 +43,31 @@ struct ResolveArgs {
     const float *shadow_map;
 };
 
+// Neural AO still on its network grid, the up pass not yet run: the AO of
+// frame pixel (x, y) is 1 where the pixel is uncovered (fmask 0), else the
+// network AO interpolated bilinearly between the centres of the f x f blocks
+// around the pixel's centre (block (X, Y) centred at frame coordinate
+// f X + f/2; edges clamp).  lao == nullptr: not in use.
+struct NetAo {
+    const float *lao;       // lW x lH network-grid AO
+    const uint8_t *fmask;   // frame-pixel coverage
+    int lW, lH, f;
+};
+
+__device__ __forceinline__ float net_ao_interp(const float *__restrict__ L, int lW, int lH, int f, int x, int y)
+{
+    const float inv = 1.f / (float)f;
+    const float u = ((float)x + 0.5f) * inv - 0.5f, w = ((float)y + 0.5f) * inv - 0.5f;
+    const float fu = floorf(u), fw = floorf(w);
+    const float tx = u - fu, ty = w - fw;
+    const int X0 = (int)fu, Y0 = (int)fw;
+    const int xa = imax(0, imin(lW - 1, X0)), xb = imax(0, imin(lW - 1, X0 + 1));
+    const int ya = imax(0, imin(lH - 1, Y0)), yb = imax(0, imin(lH - 1, Y0 + 1));
+    const float top = L[(size_t)ya * lW + xa] * (1.f - tx) + L[(size_t)ya * lW + xb] * tx;
+    const float bot = L[(size_t)yb * lW + xa] * (1.f - tx) + L[(size_t)yb * lW + xb] * tx;
+    return top * (1.f - ty) + bot * ty;
+}
+
 struct SsaoArgs {
     int n_kernel;
     float kernel[3 * 64];
@@ -113,7 +138,7 @@ struct ShadowSamples {
 };
 
 // shade.hip
-void launch_lighting(const FrameUniforms &u, const float *gbuf, const float *ao, int ao_is_one,
+void launch_lighting(const FrameUniforms &u, const float *gbuf, const float *ao, int ao_is_one, const NetAo &net,
                      const float *shadow_map, const ShadowSamples &ss, float *fb, hipStream_t s);
 // Render::DoSSAO with the stream's hemisphere samples (32 (x, y, z) per
 // pixel in row order); pow3[c] = powf(1 - c / 32, 3) from the host libm.

@@ -278,22 +278,13 @@ __global__ __launch_bounds__(256) void k_nao_down(BatchPixView one, const BatchP
 // one 16-byte store where the row is aligned).
 __device__ __forceinline__ float up_value(const BatchPixView &v, const float *__restrict__ L, int f, int x, int y)
 {
-    const float inv = 1.f / (float)f;
-    const float u = ((float)x + 0.5f) * inv - 0.5f, w = ((float)y + 0.5f) * inv - 0.5f;
-    const float fu = floorf(u), fw = floorf(w);
-    const float tx = u - fu, ty = w - fw;
-    const int X0 = (int)fu, Y0 = (int)fw;
-    const int xa = imax(0, imin(v.lW - 1, X0)), xb = imax(0, imin(v.lW - 1, X0 + 1));
-    const int ya = imax(0, imin(v.lH - 1, Y0)), yb = imax(0, imin(v.lH - 1, Y0 + 1));
-    const float top = L[(size_t)ya * v.lW + xa] * (1.f - tx) + L[(size_t)ya * v.lW + xb] * tx;
-    const float bot = L[(size_t)yb * v.lW + xa] * (1.f - tx) + L[(size_t)yb * v.lW + xb] * tx;
-    return top * (1.f - ty) + bot * ty;
+    return net_ao_interp(L, v.lW, v.lH, f, x, y);
 }
 
 // Four pixels x .. x+3 of row y, x a multiple of 4, f = 4: the sample
 // points share the row weights and fall between network columns X - 1 and X
 // (first two) or X and X + 1 (last two), X = x / 4 -- six network values
-// instead of sixteen, and the same products and sums as up_value.
+// instead of sixteen, and the same products and sums as net_ao_interp.
 __device__ __forceinline__ void up_value4(const BatchPixView &v, const float *__restrict__ L, int x, int y,
                                           float out[4])
 {
@@ -1342,6 +1333,8 @@ namespace fgl {
 int nao_reserve(fgl_nao *n, size_t frame_px, size_t net_px)
 {
     if (net_px <= n->icap && frame_px <= n->fcap) return FGL_OK;
+    int rc = ctx_settle_network_ao(n->ctx, n);   // its buffers are about to move
+    if (rc) return rc;
     hipStreamSynchronize(ctx_stream(n->ctx));
     ++n->igen;
     if (net_px > n->icap) {
@@ -1394,10 +1387,25 @@ void nao_conv_items(int view, int W, int H, std::vector<uint2> items[2])
     }
 }
 
+void nao_launch_up(const NetAo &net, int W, int H, float *ao, hipStream_t s)
+{
+    if (W <= 0 || H <= 0) return;
+    BatchPixView one{};
+    one.W = W;
+    one.H = H;
+    one.lW = net.lW;
+    one.lH = net.lH;
+    const int nb = ((W + 63) / 64) * ((H + 3) / 4);
+    hipLaunchKernelGGL(k_nao_up<false>, dim3((nb + 3) / 4), dim3(256), 0, s, one, (const BatchPixView *)nullptr,
+                       (const uint8_t *)nullptr, nb, net.f, net.lao, net.fmask, ao);
+}
+
 int nao_infer_batch(fgl_nao *n, hipStream_t s, const NaoBatchTables &t, const float *depth, const float *gbuf,
                     float *ao)
 {
     if (t.total_px > n->fcap || t.total_lpx > n->icap) return FGL_ERR_INVALID;
+    // the batch overwrites the network's buffers, which the context's AO may still be
+    if (int rc = ctx_settle_network_ao(n->ctx, n)) return rc;
     if (t.n_blocks == 0) return FGL_OK;
     const int f = n->scale;
     const BatchPixView none{};
@@ -1418,6 +1426,8 @@ int nao_infer_batch(fgl_nao *n, hipStream_t s, const NaoBatchTables &t, const fl
     return hipGetLastError() == hipSuccess ? FGL_OK : FGL_ERR_DEVICE;
 }
 
+void nao_detach(void *p) { static_cast<fgl_nao *>(p)->ctx = nullptr; }
+
 }  // namespace fgl
 
 extern "C" {
@@ -1431,6 +1441,7 @@ int fgl_nao_create(fgl_ctx *ctx, int batch, int crop, int scale, fgl_nao **out)
         return FGL_ERR_INVALID;
     fgl_nao *n = new fgl_nao();
     n->ctx = ctx;
+    ctx_register_nao(ctx, n);
     n->batch = batch;
     n->crop = crop;
     n->scale = scale;
@@ -1474,6 +1485,10 @@ int fgl_nao_create(fgl_ctx *ctx, int batch, int crop, int scale, fgl_nao **out)
 void fgl_nao_destroy(fgl_nao *n)
 {
     if (!n) return;
+    if (n->ctx) {
+        ctx_settle_network_ao(n->ctx, n);   // the context's AO may still be this network's grid
+        ctx_unregister_nao(n->ctx, n);
+    }
     hipDeviceSynchronize();
     void *ps[] = {n->params, n->grads, n->m, n->v, n->w1, n->w234, n->wt234, n->feat, n->target, n->y,
                   n->mask, n->part, n->part_db, n->head_part, n->head_red, n->ifeat, n->ia, n->ib,
@@ -1517,7 +1532,7 @@ int fgl_nao_set_state(fgl_nao *n, const float *state, int with_adam, int step)
 
 int fgl_nao_get_state(fgl_nao *n, float *state, int *step)
 {
-    if (!n || !state) return FGL_ERR_INVALID;
+    if (!n || !n->ctx || !state) return FGL_ERR_INVALID;
     CtxView cv;
     if (ctx_view(n->ctx, &cv) == FGL_OK) hipStreamSynchronize(cv.stream);
     hipMemcpy(state, n->params, NPARAM * 4, hipMemcpyDeviceToHost);
@@ -1532,7 +1547,7 @@ int fgl_nao_get_state(fgl_nao *n, float *state, int *step)
 // holds crop origins in network pixels.
 int fgl_nao_stage_crops(fgl_nao *n, int slot0, int count, const int *xy)
 {
-    if (!n || !xy || slot0 < 0 || count < 0 || slot0 + count > n->batch) return FGL_ERR_INVALID;
+    if (!n || !n->ctx || !xy || slot0 < 0 || count < 0 || slot0 + count > n->batch) return FGL_ERR_INVALID;
     CtxView cv;
     int rc = ctx_view(n->ctx, &cv);
     if (rc) return rc;
@@ -1622,14 +1637,14 @@ static int run_batch(fgl_nao *n, bool update, float lr, float *stats)
 // One Adam step on the staged batch.  stats: [sum sq err, sum abs err, covered].
 int fgl_nao_train_step(fgl_nao *n, float lr, float *stats)
 {
-    if (!n) return FGL_ERR_INVALID;
+    if (!n || !n->ctx) return FGL_ERR_INVALID;
     return run_batch(n, true, lr, stats);
 }
 
 // Gradients of the staged batch without updating (for gradient checks).
 int fgl_nao_gradients(fgl_nao *n, float *grads, float *stats)
 {
-    if (!n || !grads) return FGL_ERR_INVALID;
+    if (!n || !n->ctx || !grads) return FGL_ERR_INVALID;
     int rc = run_batch(n, false, 0.f, stats);
     if (rc) return rc;
     return hipMemcpy(grads, n->grads, NPARAM * 4, hipMemcpyDeviceToHost) == hipSuccess ? FGL_OK : FGL_ERR_DEVICE;
@@ -1659,9 +1674,12 @@ int fgl_nao_set_fused(fgl_nao *n, int fused)
 
 int fgl_nao_infer(fgl_nao *n)
 {
-    if (!n) return FGL_ERR_INVALID;
+    if (!n || !n->ctx) return FGL_ERR_INVALID;
+    ctx_drop_network_ao(n->ctx, n);   // replaced below: no need to write it out
     CtxView cv;
-    int rc = ctx_view(n->ctx, &cv);
+    // a coarser grid leaves the AO target alone (it becomes the network's);
+    // the grid of the frame itself writes it
+    int rc = ctx_view(n->ctx, &cv, n->scale == 1);
     if (rc) return rc;
     const int f = n->scale, lW = (cv.W + f - 1) / f, lH = (cv.H + f - 1) / f;
     if ((rc = nao_reserve(n, (size_t)cv.W * cv.H, (size_t)lW * lH))) return rc;
@@ -1704,14 +1722,11 @@ int fgl_nao_infer(fgl_nao *n)
             launch_conv_head(cv.stream, n->ia, n->w234 + (size_t)2 * CH * K2, n->params + OFF_B4, hd, lW, lH, 1,
                              layer_dil(3));
         }
-        if (f > 1) {
-            const int nb = ((cv.W + 63) / 64) * ((cv.H + 3) / 4);
-            hipLaunchKernelGGL(k_nao_up<false>, dim3((nb + 3) / 4), dim3(256), 0, cv.stream, one,
-                               (const BatchPixView *)nullptr, (const uint8_t *)nullptr, nb, f, n->lao, n->fmask,
-                               cv.ao);
-        }
     }
-    ctx_ao_written(n->ctx);
+    // a coarser grid stays there: the lighting pass interpolates it, and the
+    // up pass writes the AO target only when the target itself is read
+    if (f > 1) ctx_ao_from_network(n->ctx, NetAo{n->lao, n->fmask, lW, lH, f}, n);
+    else ctx_ao_written(n->ctx);
     return hipGetLastError() == hipSuccess ? FGL_OK : FGL_ERR_DEVICE;
 }
 

@@ -6,14 +6,14 @@
 namespace fgl {
 
 __global__ __launch_bounds__(256) void k_lighting(FrameUniforms u, const float *__restrict__ g,
-                                                  const float *__restrict__ ao, int ao_is_one,
+                                                  const float *__restrict__ ao, int ao_is_one, NetAo net,
                                                   const float *__restrict__ smap, ShadowSamples ss,
                                                   float *__restrict__ fb)
 {
     const int x = blockIdx.x * 64 + (threadIdx.x & 63);
     const int y = blockIdx.y * 4 + (threadIdx.x >> 6);
     if (x >= u.W || y >= u.H) return;
-    lighting_px(u, g, ao, ao_is_one, smap, ss, fb, x, y);
+    lighting_px(u, g, ao, ao_is_one, smap, ss, fb, x, y, net);
 }
 
 __global__ __launch_bounds__(256) void k_ssao_ref(FrameUniforms u, const float *__restrict__ samples,
@@ -344,11 +344,11 @@ void launch_copy(const void *src, void *dst, size_t bytes, hipStream_t s)
 
 static inline dim3 grid2(int W, int H) { return dim3((W + 63) / 64, (H + 3) / 4); }
 
-void launch_lighting(const FrameUniforms &u, const float *gbuf, const float *ao, int ao_is_one,
+void launch_lighting(const FrameUniforms &u, const float *gbuf, const float *ao, int ao_is_one, const NetAo &net,
                      const float *shadow_map, const ShadowSamples &ss, float *fb, hipStream_t s)
 {
     if (u.W <= 0 || u.H <= 0) return;
-    hipLaunchKernelGGL(k_lighting, grid2(u.W, u.H), dim3(256), 0, s, u, gbuf, ao, ao_is_one,
+    hipLaunchKernelGGL(k_lighting, grid2(u.W, u.H), dim3(256), 0, s, u, gbuf, ao, ao_is_one, net,
                        shadow_map, ss, fb);
 }
 

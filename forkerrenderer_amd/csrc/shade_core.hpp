@@ -10,12 +10,13 @@
 namespace fgl {
 
 // ForkerGL::DrawScreenSpacePixels (src/forkergl.cpp:326-380) at pixel (x,y):
-// read the G-buffer (76 B/px) + AO (4 B/px), light, write the frame buffer
-// (12 B/px).
+// read the G-buffer (76 B/px) + AO (4 B/px, or the network AO still on its
+// coarse grid: 1 B of coverage and four cached grid values), light, write
+// the frame buffer (12 B/px).
 __device__ __forceinline__ void lighting_px(const FrameUniforms &u, const float *__restrict__ g,
                                             const float *__restrict__ ao, int ao_is_one,
                                             const float *__restrict__ smap, const ShadowSamples &ss,
-                                            float *__restrict__ fb, int x, int y)
+                                            float *__restrict__ fb, int x, int y, const NetAo &net = NetAo{})
 {
     const size_t P = (size_t)u.W * u.H;
     const size_t i = (size_t)x + (size_t)y * u.W;
@@ -27,7 +28,8 @@ __device__ __forceinline__ void lighting_px(const FrameUniforms &u, const float 
     Vec3 pa = vec3(g[15 * P + i], g[16 * P + i], g[17 * P + i]);
     float type = g[18 * P + i];
     // AmbientOcclusionGBuffer starts at 1 (forkergl.cpp:80); x * 1.f == x.
-    if (!ao_is_one) pa.x *= ao[i];
+    if (net.lao) pa.x *= net.fmask[i] ? net_ao_interp(net.lao, net.lW, net.lH, net.f, x, y) : 1.f;
+    else if (!ao_is_one) pa.x *= ao[i];
     const float *rdisk = nullptr;
     if (ss.pairs) rdisk = ss.pairs + 2 * (ss.offs ? (size_t)ss.offs[i] : i * (size_t)ss.stride);
     Vec3 c = shade_point(u, smap, !(type < 0.5f), true, Pw, N, L, al, em, pa, rdisk);

@@ -250,3 +250,40 @@ def test_fused_inference_equals_layered(ctx, world, size):
         assert np.all(out[True][~cov] == 1.0) and (out[True][cov] < 1.0).any()
     finally:
         net.close()
+
+
+def test_network_ao_stays_on_its_grid_until_read(ctx, world):
+    """fgl_nao_infer leaves the AO on the network grid: the deferred lighting
+    pass interpolates it there, and reading the AO target runs the up pass.
+    Lighting from the grid and from the written-out plane give the same
+    frame bit for bit; an SSAO pass replaces the pending network AO; closing
+    the network writes its pending AO out first."""
+    sset, cfg, frame = world
+    net = NA.NeuralAO(ctx, batch=1, crop=64, params=_params())
+    NA.render_frame(ctx, sset, 5003, W, H, cfg)
+    plain = ctx.read(F.TARGET_AO).copy()
+    cur = {"depth": ctx.read(F.TARGET_DEPTH).reshape(H, W),
+           "gbuf": ctx.read(F.TARGET_GBUFFER).reshape(F.GBUF_PLANES, H, W), "view": frame["view"]}
+    net.infer()
+    ctx.draw_screen_space_pixels()                       # AO from the grid
+    fb_grid = ctx.read(F.TARGET_FRAME).copy()
+    ao = ctx.read(F.TARGET_AO).copy()                    # the up pass runs here
+    ctx.draw_screen_space_pixels()                       # AO from the plane
+    fb_plane = ctx.read(F.TARGET_FRAME).copy()
+    assert np.array_equal(fb_grid.view(np.uint32), fb_plane.view(np.uint32))
+    x, lcov, c2 = net_inputs(cur)
+    cov = c2.reshape(-1)
+    assert np.all(ao[~cov] == 1.0) and not np.array_equal(ao, plain)
+    # the oracle's network and up pass on this frame give the same AO
+    featf = R.bf16_bits_to_float(bf16_bits(x)).reshape(1, LH, LW, 16)
+    _, yr = R.forward(_params(), featf, bf16=True)
+    want = R.upsample(np.where(lcov, yr[0], 1.0).astype(np.float32), c2, S).reshape(-1)
+    assert np.abs(ao - want)[cov].max() < 2e-2
+    # SSAO replaces the pending network AO
+    net.infer()
+    ctx.ssao(cfg)
+    assert np.array_equal(ctx.read(F.TARGET_AO).view(np.uint32), plain.view(np.uint32))
+    # a network closed with its AO pending writes it out first
+    net.infer()
+    net.close()
+    assert np.array_equal(ctx.read(F.TARGET_AO).view(np.uint32), ao.view(np.uint32))
