@@ -26,7 +26,7 @@ struct Buf {
 int grow(Buf &b, size_t bytes)
 {
     if (b.p && bytes <= b.cap) return FGL_OK;
-    if (b.p) hipFree(b.p);
+    if (b.p) (void)hipFree(b.p);
     b.p = nullptr;
     b.cap = 0;
     size_t want = bytes < 256 ? 256 : bytes;
@@ -40,7 +40,7 @@ int grow(Buf &b, size_t bytes)
 
 void release(Buf &b)
 {
-    if (b.p) hipFree(b.p);
+    if (b.p) (void)hipFree(b.p);
     b.p = nullptr;
     b.cap = 0;
 }
@@ -151,7 +151,7 @@ static int bfail(fgl_batch *b, int code, const char *msg) { return ctx_fail(b->c
 
 static void drop_graph(fgl_batch *b)
 {
-    if (b->exec) hipGraphExecDestroy(b->exec);
+    if (b->exec) (void)hipGraphExecDestroy(b->exec);
     b->exec = nullptr;
 }
 
@@ -488,12 +488,12 @@ static int capture(fgl_batch *b)
     int rc = enqueue(b, false, false);
     hipError_t e = hipStreamEndCapture(s, &g);
     if (rc) {
-        if (g) hipGraphDestroy(g);
+        if (g) (void)hipGraphDestroy(g);
         return rc;
     }
     if (e != hipSuccess || !g) return bfail(b, FGL_ERR_DEVICE, "graph capture failed");
     e = hipGraphInstantiate(&b->exec, g, nullptr, nullptr, 0);
-    hipGraphDestroy(g);
+    (void)hipGraphDestroy(g);
     if (e != hipSuccess) {
         b->exec = nullptr;
         return bfail(b, FGL_ERR_DEVICE, "graph instantiation failed");
@@ -558,7 +558,7 @@ int fgl_batch_create(fgl_ctx *c, int n, const fgl_view_desc *views, int ao_mode,
         vc.d.matrices = vc.mats.data();
     }
     int dev = 0;
-    hipGetDevice(&dev);
+    if (hipGetDevice(&dev) != hipSuccess) dev = 0;
     if (hipDeviceGetAttribute(&b->n_cu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || b->n_cu < 1)
         b->n_cu = 256;
     if (hipHostMalloc((void **)&b->h_flags, 16, hipHostMallocDefault) != hipSuccess ||
@@ -577,7 +577,7 @@ int fgl_batch_create(fgl_ctx *c, int n, const fgl_view_desc *views, int ao_mode,
 void fgl_batch_destroy(fgl_batch *b)
 {
     if (!b) return;
-    hipStreamSynchronize(ctx_stream(b->ctx));
+    (void)hipStreamSynchronize(ctx_stream(b->ctx));   // teardown: nothing to report to
     drop_graph(b);
     for (int l = 0; l < 2; ++l) release(b->d_conv_items[l]);
     for (int p = 0; p < 2; ++p) {
@@ -594,8 +594,8 @@ void fgl_batch_destroy(fgl_batch *b)
         release(b->keys[p]); release(b->vals[p]); release(b->keys_alt[p]); release(b->vals_alt[p]);
         release(b->sort_tmp[p]);
     }
-    if (b->h_flags) hipHostFree(b->h_flags);
-    if (b->h_tot) hipHostFree(b->h_tot);
+    if (b->h_flags) (void)hipHostFree(b->h_flags);
+    if (b->h_tot) (void)hipHostFree(b->h_tot);
     delete b;
 }
 
@@ -605,7 +605,7 @@ int fgl_batch_set_tiles(fgl_batch *b, int shadow_tile, int main_tile)
     for (int t : {shadow_tile, main_tile})
         if (t != 8 && t != 16 && t != 32) return bfail(b, FGL_ERR_INVALID, "tile must be 8, 16 or 32");
     if (b->TS[0] == shadow_tile && b->TS[1] == main_tile) return FGL_OK;
-    hipStreamSynchronize(ctx_stream(b->ctx));
+    if (hipStreamSynchronize(ctx_stream(b->ctx)) != hipSuccess) return bfail(b, FGL_ERR_DEVICE, "stream failed");
     b->TS[0] = shadow_tile;
     b->TS[1] = main_tile;
     return plan(b);
@@ -623,7 +623,7 @@ int fgl_batch_set_occupancy(fgl_batch *b, int wg_per_cu)
 int fgl_batch_set_capacity(fgl_batch *b, uint32_t shadow_pairs, uint32_t main_pairs)
 {
     if (!b) return FGL_ERR_INVALID;
-    hipStreamSynchronize(ctx_stream(b->ctx));
+    if (hipStreamSynchronize(ctx_stream(b->ctx)) != hipSuccess) return bfail(b, FGL_ERR_DEVICE, "stream failed");
     b->cap_req[0] = shadow_pairs;
     b->cap_req[1] = main_pairs;
     int rc;

@@ -158,8 +158,8 @@ static int ensure(fgl_ctx *c, DevBuf &b, size_t bytes)
 {
     if (bytes <= b.cap && b.p) return FGL_OK;
     if (b.p) {
-        hipStreamSynchronize(c->stream);
-        hipFree(b.p);
+        HIPCHK(c, hipStreamSynchronize(c->stream));   // no kernel may still use it
+        (void)hipFree(b.p);
         b.p = nullptr;
         b.cap = 0;
     }
@@ -174,7 +174,7 @@ static int ensure(fgl_ctx *c, DevBuf &b, size_t bytes)
 
 static void free_buf(DevBuf &b)
 {
-    if (b.p) hipFree(b.p);
+    if (b.p) (void)hipFree(b.p);
     b.p = nullptr;
     b.cap = 0;
 }
@@ -190,14 +190,20 @@ struct KernelTimer {
     {
         if (!on) return;
         tk.name = name;
-        hipEventCreate(&tk.a);
-        hipEventCreate(&tk.b);
-        hipEventRecord(tk.a, c->stream);
+        // a diagnostic: when its events cannot be made, the span is not timed
+        if (hipEventCreate(&tk.a) != hipSuccess) {
+            on = false;
+            return;
+        }
+        if (hipEventCreate(&tk.b) != hipSuccess || hipEventRecord(tk.a, c->stream) != hipSuccess) {
+            (void)hipEventDestroy(tk.a);
+            on = false;
+        }
     }
     ~KernelTimer()
     {
         if (!on) return;
-        hipEventRecord(tk.b, c->stream);
+        (void)hipEventRecord(tk.b, c->stream);
         c->timed.push_back(tk);
     }
 };
@@ -269,15 +275,15 @@ int fgl_ctx_create(int device, fgl_ctx **out)
         return FGL_ERR_DEVICE;
     }
     if (hipHostMalloc((void **)&c->h_total, sizeof(uint32_t) * 4, hipHostMallocDefault) != hipSuccess) {
-        hipStreamDestroy(c->stream);
+        (void)hipStreamDestroy(c->stream);
         delete c;
         return FGL_ERR_NOMEM;
     }
     if (hipHostMalloc((void **)&c->h_pub, 64, hipHostMallocCoherent | hipHostMallocMapped) != hipSuccess ||
         hipHostGetDevicePointer((void **)&c->d_pub, c->h_pub, 0) != hipSuccess) {
-        if (c->h_pub) hipHostFree(c->h_pub);
-        hipHostFree(c->h_total);
-        hipStreamDestroy(c->stream);
+        if (c->h_pub) (void)hipHostFree(c->h_pub);
+        (void)hipHostFree(c->h_total);
+        (void)hipStreamDestroy(c->stream);
         delete c;
         return FGL_ERR_NOMEM;
     }
@@ -312,13 +318,13 @@ void fgl_ctx_destroy(fgl_ctx *c)
 {
     if (!c) return;
     for (void *n : c->naos) nao_detach(n);   // networks outliving their context stop using it
-    hipSetDevice(c->device);
-    hipStreamSynchronize(c->stream);
-    for (void *p : c->texture_mem) hipFree(p);
+    (void)hipSetDevice(c->device);
+    (void)hipStreamSynchronize(c->stream);
+    for (void *p : c->texture_mem) (void)hipFree(p);
     for (auto &m : c->models) {
-        hipFree(m.pos); hipFree(m.uv); hipFree(m.nrm);
-        if (m.tan) hipFree(m.tan);
-        hipFree(m.vi); hipFree(m.ti); hipFree(m.ni); hipFree(m.fmat); hipFree(m.corner);
+        (void)hipFree(m.pos); (void)hipFree(m.uv); (void)hipFree(m.nrm);
+        if (m.tan) (void)hipFree(m.tan);
+        (void)hipFree(m.vi); (void)hipFree(m.ti); (void)hipFree(m.ni); (void)hipFree(m.fmat); (void)hipFree(m.corner);
     }
     DevBuf *bufs[] = {&c->d_textures, &c->d_materials, &c->depth, &c->fb, &c->gbuf, &c->ao,
                       &c->blur_tmp, &c->shadow, &c->shadow_depth, &c->img, &c->vis, &c->d_draws,
@@ -327,10 +333,10 @@ void fgl_ctx_destroy(fgl_ctx *c)
                       &c->vals, &c->keys_alt, &c->vals_alt, &c->sort_tmp, &c->tile_start,
                       &c->tile_end, &c->tri_v, &c->big_list, &c->counters, &c->ndc_z};
     for (DevBuf *b : bufs) free_buf(*b);
-    for (auto &t : c->timed) { hipEventDestroy(t.a); hipEventDestroy(t.b); }
-    if (c->h_total) hipHostFree(c->h_total);
-    if (c->h_pub) hipHostFree(c->h_pub);
-    hipStreamDestroy(c->stream);
+    for (auto &t : c->timed) { (void)hipEventDestroy(t.a); (void)hipEventDestroy(t.b); }
+    if (c->h_total) (void)hipHostFree(c->h_total);
+    if (c->h_pub) (void)hipHostFree(c->h_pub);
+    (void)hipStreamDestroy(c->stream);
     delete c;
 }
 
@@ -1295,7 +1301,7 @@ int fgl_copy_benchmark(fgl_ctx *c, size_t bytes, int iters, float *ms_per_iter)
     void *a = nullptr, *b = nullptr;
     HIPCHK(c, hipMalloc(&a, bytes));
     if (hipMalloc(&b, bytes) != hipSuccess) {
-        hipFree(a);
+        (void)hipFree(a);
         return fail(c, FGL_ERR_NOMEM, "copy buffer");
     }
     launch_fill_f32((float *)a, 1.f, bytes / 4, c->stream);
@@ -1312,8 +1318,8 @@ int fgl_copy_benchmark(fgl_ctx *c, size_t bytes, int iters, float *ms_per_iter)
     if (ms_per_iter) *ms_per_iter = ms / iters;
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
-    hipFree(a);
-    hipFree(b);
+    (void)hipFree(a);
+    (void)hipFree(b);
     return FGL_OK;
 }
 
@@ -1331,7 +1337,7 @@ int fgl_check_recip(fgl_ctx *c, uint64_t *compared, uint64_t *mismatches, uint64
     }
     if (e == hipSuccess) e = hipMemcpyAsync(h, d, sizeof(h), hipMemcpyDeviceToHost, c->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
-    hipFree(d);
+    (void)hipFree(d);
     if (e != hipSuccess) return fail(c, FGL_ERR_DEVICE, hipGetErrorString(e));
     if (compared) *compared = h[0];
     if (mismatches) *mismatches = h[1];
@@ -1351,7 +1357,7 @@ int fgl_sort_pairs(fgl_ctx *c, uint32_t *keys, uint32_t *vals, size_t n, int bit
     const size_t sz[5] = {bytes, bytes, bytes, bytes, tmp_bytes};
     for (int i = 0; i < 5; ++i)
         if (hipMalloc(&buf[i], sz[i]) != hipSuccess) {
-            for (int j = 0; j < i; ++j) hipFree(buf[j]);
+            for (int j = 0; j < i; ++j) (void)hipFree(buf[j]);
             return fail(c, FGL_ERR_NOMEM, "sort buffers");
         }
     hipError_t e = hipMemcpyAsync(buf[0], keys, bytes, hipMemcpyHostToDevice, c->stream);
@@ -1364,7 +1370,7 @@ int fgl_sort_pairs(fgl_ctx *c, uint32_t *keys, uint32_t *vals, size_t n, int bit
     if (e == hipSuccess) e = hipMemcpyAsync(keys, buf[0], bytes, hipMemcpyDeviceToHost, c->stream);
     if (e == hipSuccess) e = hipMemcpyAsync(vals, buf[1], bytes, hipMemcpyDeviceToHost, c->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
-    for (int i = 0; i < 5; ++i) hipFree(buf[i]);
+    for (int i = 0; i < 5; ++i) (void)hipFree(buf[i]);
     if (e != hipSuccess) return fail(c, FGL_ERR_DEVICE, hipGetErrorString(e));
     return FGL_OK;
 }

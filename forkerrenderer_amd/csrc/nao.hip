@@ -753,7 +753,7 @@ static inline unsigned conv_workgroups(int n_tiles, int per_cu)
 {
     if (g_cus == 0) {
         int dev = 0, cu = 0;
-        hipGetDevice(&dev);
+        if (hipGetDevice(&dev) != hipSuccess) dev = 0;
         if (hipDeviceGetAttribute(&cu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cu < 1) cu = 256;
         g_cus = cu;
     }
@@ -1335,12 +1335,12 @@ int nao_reserve(fgl_nao *n, size_t frame_px, size_t net_px)
     if (net_px <= n->icap && frame_px <= n->fcap) return FGL_OK;
     int rc = ctx_settle_network_ao(n->ctx, n);   // its buffers are about to move
     if (rc) return rc;
-    hipStreamSynchronize(ctx_stream(n->ctx));
+    if (hipStreamSynchronize(ctx_stream(n->ctx)) != hipSuccess) return FGL_ERR_DEVICE;
     ++n->igen;
     if (net_px > n->icap) {
         void *ps[] = {n->ifeat, n->ia, n->ib, n->imask, n->lao};
         for (void *q : ps)
-            if (q) hipFree(q);
+            if (q) (void)hipFree(q);
         n->ifeat = n->ia = n->ib = nullptr;
         n->imask = nullptr;
         n->lao = nullptr;
@@ -1355,7 +1355,7 @@ int nao_reserve(fgl_nao *n, size_t frame_px, size_t net_px)
         n->icap = net_px;
     }
     if (frame_px > n->fcap) {
-        if (n->fmask) hipFree(n->fmask);
+        if (n->fmask) (void)hipFree(n->fmask);
         n->fmask = nullptr;
         n->fcap = 0;
         if (nao_alloc((void **)&n->fmask, frame_px)) return FGL_ERR_NOMEM;
@@ -1471,13 +1471,14 @@ int fgl_nao_create(fgl_ctx *ctx, int batch, int crop, int scale, fgl_nao **out)
     rc |= nao_alloc((void **)&n->head_red, HEAD_PART * 4);
     if (hipHostMalloc((void **)&n->h_stats, 64 * 4, hipHostMallocDefault) != hipSuccess) rc = FGL_ERR_NOMEM;
     if (rc != FGL_OK) {
-        delete n;
+        fgl_nao_destroy(n);   // frees what was allocated, leaves the context
         return FGL_ERR_NOMEM;
     }
-    hipMemset(n->m, 0, NPARAM * 4);
-    hipMemset(n->v, 0, NPARAM * 4);
-    hipMemset(n->params, 0, NPARAM * 4);
-    hipDeviceSynchronize();
+    if (hipMemset(n->m, 0, NPARAM * 4) != hipSuccess || hipMemset(n->v, 0, NPARAM * 4) != hipSuccess ||
+        hipMemset(n->params, 0, NPARAM * 4) != hipSuccess || hipDeviceSynchronize() != hipSuccess) {
+        fgl_nao_destroy(n);
+        return FGL_ERR_DEVICE;
+    }
     *out = n;
     return FGL_OK;
 }
@@ -1486,20 +1487,22 @@ void fgl_nao_destroy(fgl_nao *n)
 {
     if (!n) return;
     if (n->ctx) {
-        ctx_settle_network_ao(n->ctx, n);   // the context's AO may still be this network's grid
+        // the context's AO may still be this network's grid (teardown: a
+        // failure here has no caller to go to)
+        (void)ctx_settle_network_ao(n->ctx, n);
         ctx_unregister_nao(n->ctx, n);
     }
-    hipDeviceSynchronize();
+    (void)hipDeviceSynchronize();
     void *ps[] = {n->params, n->grads, n->m, n->v, n->w1, n->w234, n->wt234, n->feat, n->target, n->y,
                   n->mask, n->part, n->part_db, n->head_part, n->head_red, n->ifeat, n->ia, n->ib,
                   n->imask, n->fmask, n->lao};
     for (void *p : ps)
-        if (p) hipFree(p);
+        if (p) (void)hipFree(p);
     for (int l = 0; l < 4; ++l) {
-        hipFree(n->a[l]);
-        hipFree(n->d[l]);
+        if (n->a[l]) (void)hipFree(n->a[l]);
+        if (n->d[l]) (void)hipFree(n->d[l]);
     }
-    if (n->h_stats) hipHostFree(n->h_stats);
+    if (n->h_stats) (void)hipHostFree(n->h_stats);
     delete n;
 }
 
@@ -1517,27 +1520,28 @@ int fgl_nao_set_state(fgl_nao *n, const float *state, int with_adam, int step)
 {
     if (!n || !state) return FGL_ERR_INVALID;
     if (hipMemcpy(n->params, state, NPARAM * 4, hipMemcpyHostToDevice) != hipSuccess) return FGL_ERR_DEVICE;
+    hipError_t e;
     if (with_adam) {
-        hipMemcpy(n->m, state + NPARAM, NPARAM * 4, hipMemcpyHostToDevice);
-        hipMemcpy(n->v, state + 2 * NPARAM, NPARAM * 4, hipMemcpyHostToDevice);
+        e = hipMemcpy(n->m, state + NPARAM, NPARAM * 4, hipMemcpyHostToDevice);
+        if (e == hipSuccess) e = hipMemcpy(n->v, state + 2 * NPARAM, NPARAM * 4, hipMemcpyHostToDevice);
         n->step = step;
     } else {
-        hipMemset(n->m, 0, NPARAM * 4);
-        hipMemset(n->v, 0, NPARAM * 4);
+        e = hipMemset(n->m, 0, NPARAM * 4);
+        if (e == hipSuccess) e = hipMemset(n->v, 0, NPARAM * 4);
         n->step = 0;
     }
-    hipDeviceSynchronize();
+    if (e != hipSuccess || hipDeviceSynchronize() != hipSuccess) return FGL_ERR_DEVICE;
     return prep(n, 0) == FGL_OK && hipDeviceSynchronize() == hipSuccess ? FGL_OK : FGL_ERR_DEVICE;
 }
 
 int fgl_nao_get_state(fgl_nao *n, float *state, int *step)
 {
     if (!n || !n->ctx || !state) return FGL_ERR_INVALID;
-    CtxView cv;
-    if (ctx_view(n->ctx, &cv) == FGL_OK) hipStreamSynchronize(cv.stream);
-    hipMemcpy(state, n->params, NPARAM * 4, hipMemcpyDeviceToHost);
-    hipMemcpy(state + NPARAM, n->m, NPARAM * 4, hipMemcpyDeviceToHost);
-    hipMemcpy(state + 2 * NPARAM, n->v, NPARAM * 4, hipMemcpyDeviceToHost);
+    if (hipStreamSynchronize(ctx_stream(n->ctx)) != hipSuccess ||
+        hipMemcpy(state, n->params, NPARAM * 4, hipMemcpyDeviceToHost) != hipSuccess ||
+        hipMemcpy(state + NPARAM, n->m, NPARAM * 4, hipMemcpyDeviceToHost) != hipSuccess ||
+        hipMemcpy(state + 2 * NPARAM, n->v, NPARAM * 4, hipMemcpyDeviceToHost) != hipSuccess)
+        return FGL_ERR_DEVICE;
     if (step) *step = n->step;
     return FGL_OK;
 }
@@ -1589,9 +1593,10 @@ static int run_batch(fgl_nao *n, bool update, float lr, float *stats)
     hipLaunchKernelGGL(k_head_bwd, dim3(hb), dim3(256), 0, s, n->params, n->a[3], n->target, n->mask, px,
                        n->d[3], n->head_part);
     hipLaunchKernelGGL(k_reduce_cols, dim3(1), dim3(256), 0, s, n->head_part, hb, HEAD_PART, n->head_red);
-    hipMemcpyAsync(n->grads + OFF_W5, n->head_red, CH * 4, hipMemcpyDeviceToDevice, s);
-    hipMemcpyAsync(n->grads + OFF_B5, n->head_red + CH, 4, hipMemcpyDeviceToDevice, s);
-    hipMemcpyAsync(n->h_stats, n->head_red + CH + 1, 3 * 4, hipMemcpyDeviceToHost, s);
+    if (hipMemcpyAsync(n->grads + OFF_W5, n->head_red, CH * 4, hipMemcpyDeviceToDevice, s) != hipSuccess ||
+        hipMemcpyAsync(n->grads + OFF_B5, n->head_red + CH, 4, hipMemcpyDeviceToDevice, s) != hipSuccess ||
+        hipMemcpyAsync(n->h_stats, n->head_red + CH + 1, 3 * 4, hipMemcpyDeviceToHost, s) != hipSuccess)
+        return FGL_ERR_DEVICE;
     // layers 4..1: weight gradient from (input, output gradient), then the
     // data gradient of the layer input (not needed for layer 1)
     const dim3 wg(S / 64, S / WG_ROWS, N);
@@ -1654,17 +1659,17 @@ int fgl_nao_gradients(fgl_nao *n, float *grads, float *stats)
 int fgl_nao_read_batch(fgl_nao *n, uint16_t *feat, float *target, uint8_t *mask)
 {
     if (!n) return FGL_ERR_INVALID;
-    hipDeviceSynchronize();
+    if (hipDeviceSynchronize() != hipSuccess) return FGL_ERR_DEVICE;
     const size_t px = (size_t)n->batch * n->crop * n->crop;
-    if (feat) hipMemcpy(feat, n->feat, px * C0 * 2, hipMemcpyDeviceToHost);
-    if (target) hipMemcpy(target, n->target, px * 4, hipMemcpyDeviceToHost);
-    if (mask) hipMemcpy(mask, n->mask, px, hipMemcpyDeviceToHost);
+    if ((feat && hipMemcpy(feat, n->feat, px * C0 * 2, hipMemcpyDeviceToHost) != hipSuccess) ||
+        (target && hipMemcpy(target, n->target, px * 4, hipMemcpyDeviceToHost) != hipSuccess) ||
+        (mask && hipMemcpy(mask, n->mask, px, hipMemcpyDeviceToHost) != hipSuccess))
+        return FGL_ERR_DEVICE;
     return FGL_OK;
 }
 
-// Neural AO of the context's current frame (depth + G-buffer) into its AO
-// target: down pass, four MFMA convolutions with the head fused into the
-// last, up pass.
+// Inference kernels: fused (two kernels for the four convolutions and the
+// head) or one kernel per layer.
 int fgl_nao_set_fused(fgl_nao *n, int fused)
 {
     if (!n || (fused != 0 && fused != 1)) return FGL_ERR_INVALID;
@@ -1672,6 +1677,10 @@ int fgl_nao_set_fused(fgl_nao *n, int fused)
     return FGL_OK;
 }
 
+// Neural AO of the context's current frame (depth + G-buffer): down pass,
+// the convolutions and the head; on a coarser grid the AO stays there as the
+// context's AO target (the lighting pass interpolates it, a read of the
+// target writes it out).
 int fgl_nao_infer(fgl_nao *n)
 {
     if (!n || !n->ctx) return FGL_ERR_INVALID;

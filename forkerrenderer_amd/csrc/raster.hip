@@ -139,9 +139,9 @@ __global__ __launch_bounds__(256) void k_tile_ranges(const uint32_t *__restrict_
 // (from 88, five wavefronts) with an 8-byte spill, and the extra wavefront
 // hides more of the LDS atomics' latency (main pass 0.325 -> 0.294 ms at
 // 3840x2160); the 8-pixel tiles already fit seven, the 32-pixel ones stay
-// at four.
+// at four (their LDS allows no more).
 template <int TS, int BLOCK, bool SHADOW>
-__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(6))) void k_raster(RasterArgs a)
+__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(TS == 32 ? 4 : 6))) void k_raster(RasterArgs a)
 {
     __shared__ RasterLds<TS, BLOCK> lds;
     const int tile = xcd_remap(blockIdx.x, a.tiles_x * a.tiles_y);

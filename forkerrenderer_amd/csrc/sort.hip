@@ -299,8 +299,9 @@ void launch_radix_sort(uint32_t *keys, uint32_t *vals, uint32_t *keys_alt, uint3
                        size_t n, int bits, uint32_t *tmp, hipStream_t s)
 {
     if (launch_radix_sort_impl(keys, vals, keys_alt, vals_alt, n, nullptr, bits, tmp, s)) {
-        hipMemcpyAsync(keys, keys_alt, n * sizeof(uint32_t), hipMemcpyDeviceToDevice, s);
-        hipMemcpyAsync(vals, vals_alt, n * sizeof(uint32_t), hipMemcpyDeviceToDevice, s);
+        // a failure here is the stream's last error, which the caller checks
+        (void)hipMemcpyAsync(keys, keys_alt, n * sizeof(uint32_t), hipMemcpyDeviceToDevice, s);
+        (void)hipMemcpyAsync(vals, vals_alt, n * sizeof(uint32_t), hipMemcpyDeviceToDevice, s);
     }
 }
 
