@@ -170,6 +170,7 @@ __global__ __launch_bounds__(256) void k_bv_gbuffer(BvPixelArgs b)
     a.materials = b.materials;
     a.textures = b.textures;
     a.shadow_map = nullptr;
+    a.ss = ShadowSamples{nullptr, nullptr, 0u};
     gbuffer_px(a, v.u, x, y);
 }
 

@@ -87,6 +87,12 @@ struct fgl_ctx {
     // pass scratch
     DevBuf d_draws, screen, rect, counts, offs, scan_tmp, keys, vals, keys_alt, vals_alt, sort_tmp,
         tile_start, tile_end, tri_v, big_list, counters, ndc_z;
+    // forward pass with a stochastic shadow filter: per-pixel passing counts
+    // (+ scan scratch), the passing fragments (triangle ids then pixels) and
+    // the sort's other copy, its scratch, their shadow inputs, and each
+    // winner's sample offset
+    DevBuf frag_cnt, frag_tri, frag_alt, frag_tmp, frag_planes, frag_offs_px;
+    bool frag_sorted_alt = false;
     std::vector<DevDraw> last_draws;
     uint32_t *h_total = nullptr;
     // pass counts published by the device into coherent pinned memory
@@ -331,7 +337,9 @@ void fgl_ctx_destroy(fgl_ctx *c)
                       &c->rng, &c->rng_saved, &c->rng_samples, &c->rng_offs, &c->ssao_pow3,
                       &c->screen, &c->rect, &c->counts, &c->offs, &c->scan_tmp, &c->keys,
                       &c->vals, &c->keys_alt, &c->vals_alt, &c->sort_tmp, &c->tile_start,
-                      &c->tile_end, &c->tri_v, &c->big_list, &c->counters, &c->ndc_z};
+                      &c->tile_end, &c->tri_v, &c->big_list, &c->counters, &c->ndc_z,
+                      &c->frag_cnt, &c->frag_tri, &c->frag_alt, &c->frag_tmp, &c->frag_planes,
+                      &c->frag_offs_px};
     for (DevBuf *b : bufs) free_buf(*b);
     for (auto &t : c->timed) { (void)hipEventDestroy(t.a); (void)hipEventDestroy(t.b); }
     if (c->h_total) (void)hipHostFree(c->h_total);
@@ -776,6 +784,60 @@ static void fill_draw(fgl_ctx *c, const ModelRes &m, const float M[16], DevDraw 
     mat_mul(c->light_space, d.model, d.lm);
 }
 
+// The stochastic shadow filters' unit-disk samples for n shading points in
+// the order they shade -- the lighting pass's pixels in row order, or the
+// forward pass's passing fragments in raster order -- drawn from the
+// context's stream, which ends past exactly the samples the points use.  g:
+// the points' normal, world position and light NDC as planes of stride n
+// (read for PCSS only).  ss: point i's samples start at pair ss.offs[i]
+// (PCSS; offs[n] = the total) or at i * ss.stride (PCF).
+static int stochastic_shadow_samples(fgl_ctx *c, const FrameUniforms &u, const float *g, size_t n, ShadowSamples &ss)
+{
+    int rc;
+    const int mode = c->shadow_cfg.mode;
+    const uint32_t nb = (uint32_t)c->shadow_cfg.n_blocker, np = (uint32_t)c->shadow_cfg.n_pcf;
+    const size_t most = n * (mode == FGL_SHADOW_REF_PCF ? np : nb + np);
+    if (most > 0xffffffffull) return fail(c, FGL_ERR_CAPACITY, "stochastic shadows: too many samples");
+    // never fewer than one point's worth, so every offset in [0, most) reads
+    // inside the buffer
+    if ((rc = ensure(c, c->rng_samples, (most > nb + np ? most : nb + np) * 2 * sizeof(float)))) return rc;
+    MtState *st = (MtState *)c->rng.p;
+    float *pairs = (float *)c->rng_samples.p;
+    if (mode == FGL_SHADOW_REF_PCF) {
+        launch_rng_accept(2, st, pairs, (uint32_t)most, c->stream);
+        ss = {pairs, nullptr, np};
+        return FGL_OK;
+    }
+    // PCSS takes n_pcf more samples only where its blocker search finds a
+    // blocker.  Points the shadow map decides alone are counted in parallel;
+    // the others are walked in order over the most samples the points can
+    // need; then the stream is restored and replayed to stop after exactly
+    // the samples used (the replay rewrites the same values).
+    const size_t words = scan_tmp_words(n + 1);
+    if ((rc = ensure(c, c->rng_offs, (5 * (n + 1) + words) * sizeof(uint32_t)))) return rc;
+    uint32_t *base = (uint32_t *)c->rng_offs.p, *upos = base + (n + 1), *list = upos + (n + 1);
+    uint32_t *found = list + (n + 1), *offs = found + (n + 1), *tmp = offs + (n + 1);
+    HIPCHK(c, hipMemcpyAsync(c->rng_saved.p, st, sizeof(MtState), hipMemcpyDeviceToDevice, c->stream));
+    launch_rng_accept(2, st, pairs, (uint32_t)most, c->stream);
+    launch_pcss_classify(u, g, n, (const float *)c->shadow.p, base, upos, c->stream);
+    launch_exclusive_scan(base, base, n, tmp, c->stream);
+    launch_exclusive_scan(upos, upos, n, tmp, c->stream);
+    launch_pcss_compact(upos, n, list, c->stream);
+    HIPCHK(c, hipMemcpyAsync(c->h_total, upos + n, 4, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    const uint32_t n_unc = c->h_total[0];
+    launch_pcss_walk_list(u, g, n, (const float *)c->shadow.p, pairs, base, list, n_unc, found, c->stream);
+    launch_exclusive_scan(found, found, n_unc, tmp, c->stream);
+    launch_pcss_offsets(base, upos, found, np, n, offs, c->stream);
+    HIPCHK(c, hipMemcpyAsync(c->h_total, offs + n, 4, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    const uint32_t used = c->h_total[0];
+    HIPCHK(c, hipMemcpyAsync(st, c->rng_saved.p, sizeof(MtState), hipMemcpyDeviceToDevice, c->stream));
+    launch_rng_accept(2, st, pairs, used, c->stream);
+    ss = {pairs, offs, 0};
+    return FGL_OK;
+}
+
 static int flush_pass(fgl_ctx *c)
 {
     if (c->draws.empty()) return FGL_OK;
@@ -791,11 +853,14 @@ static int flush_pass(fgl_ctx *c)
         c->draws.clear();
         return fail(c, FGL_ERR_INVALID, "forward pass with shadows needs a shadow buffer");
     }
-    if (pass == kPassForward && c->shadow_on && c->shadow_cfg.mode >= FGL_SHADOW_REF_PCF) {
-        // the forward shaders draw per depth-passing fragment in raster
-        // order, a sequence this tile rasteriser does not replay
+    // The forward shaders with a stochastic shadow filter draw samples for
+    // every fragment that passes the depth test, in raster order: the pass
+    // replays that sequence (k_frag_walk) to find each winner's samples.
+    const bool stoch = pass == kPassForward && c->shadow_on && !c->lights.empty() &&
+                       (c->shadow_cfg.mode == FGL_SHADOW_REF_PCF || c->shadow_cfg.mode == FGL_SHADOW_REF_PCSS);
+    if (stoch && (size_t)c->W * c->H > 0xffffffffull) {
         c->draws.clear();
-        return fail(c, FGL_ERR_INVALID, "stochastic shadow filters run in the deferred lighting pass only");
+        return fail(c, FGL_ERR_CAPACITY, "stochastic shadows in the forward pass: too many pixels");
     }
     int rc = upload_tables(c);
     if (rc) return rc;
@@ -902,6 +967,46 @@ static int flush_pass(fgl_ctx *c)
         launch_tile_ranges((const uint32_t *)c->keys.p, npairs, nullptr, (uint32_t *)c->tile_start.p,
                            (uint32_t *)c->tile_end.p, c->stream);
     }
+    uint32_t n_frag = 0;
+    if (stoch) {
+        // the passing fragments in raster order: count, scan, emit, sort
+        KernelTimer t(c, "frag_walk");
+        const size_t P = (size_t)W * H;
+        if ((rc = ensure(c, c->frag_cnt, (P + 1 + scan_tmp_words(P)) * 4))) return rc;
+        uint32_t *cnt = (uint32_t *)c->frag_cnt.p;
+        FragWalkArgs fw;
+        std::memset(&fw, 0, sizeof(fw));
+        fw.screen = (const int4 *)c->screen.p;
+        fw.tri_v = (const int4 *)c->tri_v.p;
+        fw.vals = (const uint32_t *)c->vals.p;
+        fw.tile_start = (const uint32_t *)c->tile_start.p;
+        fw.tile_end = (const uint32_t *)c->tile_end.p;
+        fw.W = W;
+        fw.H = H;
+        fw.tiles_x = tiles_x;
+        fw.tiles_y = tiles_y;
+        fw.fresh = fresh;
+        fw.depth = (const float *)c->depth.p;
+        fw.count = cnt;
+        launch_frag_walk(false, TS, fw, c->stream);
+        launch_exclusive_scan(cnt, cnt, P, cnt + P + 1, c->stream);
+        HIPCHK(c, hipMemcpyAsync(c->h_total, cnt + P, 4, hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        n_frag = c->h_total[0];
+        if (n_frag == 0xffffffffu) return fail(c, FGL_ERR_CAPACITY, "stochastic shadows: too many passing fragments");
+        if ((rc = ensure(c, c->frag_tri, (size_t)n_frag * 8 + 8)) ||
+            (rc = ensure(c, c->frag_alt, (size_t)n_frag * 8 + 8)) ||
+            (rc = ensure(c, c->frag_tmp, radix_tmp_words(n_frag) * 4)))
+            return rc;
+        uint32_t *ftri = (uint32_t *)c->frag_tri.p, *fq = ftri + n_frag;
+        uint32_t *atri = (uint32_t *)c->frag_alt.p, *aq = atri + n_frag;
+        fw.base = cnt;
+        fw.tri_out = ftri;
+        fw.q_out = fq;
+        launch_frag_walk(true, TS, fw, c->stream);
+        c->frag_sorted_alt = launch_radix_sort_impl(ftri, fq, atri, aq, n_frag, nullptr, bits_for((uint32_t)(nt - 1)),
+                                                    (uint32_t *)c->frag_tmp.p, c->stream) != 0;
+    }
     RasterArgs a;
     std::memset(&a, 0, sizeof(a));
     a.draws = ddraws;
@@ -947,6 +1052,26 @@ static int flush_pass(fgl_ctx *c)
             launch_gbuffer_resolve(r, u, c->stream);
             c->pend_gbuf = false;
         } else {
+            if (stoch) {
+                // each winner's samples: the offset of its fragment in the sequence
+                KernelTimer t(c, "shadow_samples");
+                const size_t P = (size_t)W * H;
+                if ((rc = ensure(c, c->frag_offs_px, P * 4))) return rc;
+                uint32_t *offs_px = (uint32_t *)c->frag_offs_px.p;
+                launch_fill_u32(offs_px, 0u, P, c->stream);
+                uint32_t *ftri = (uint32_t *)(c->frag_sorted_alt ? c->frag_alt.p : c->frag_tri.p), *fq = ftri + n_frag;
+                const float *planes = nullptr;
+                if (c->shadow_cfg.mode == FGL_SHADOW_REF_PCSS) {
+                    if ((rc = ensure(c, c->frag_planes, (size_t)9 * n_frag * sizeof(float) + 4))) return rc;
+                    launch_frag_shadow_inputs(r, u, ftri, fq, n_frag, (float *)c->frag_planes.p, c->stream);
+                    planes = (const float *)c->frag_planes.p;
+                }
+                ShadowSamples ss = {nullptr, nullptr, 0};
+                if ((rc = stochastic_shadow_samples(c, u, planes, n_frag, ss))) return rc;
+                launch_frag_pixel_offsets(ftri, fq, n_frag, W, H, (const int *)c->vis.p, ss.offs, ss.stride, offs_px,
+                                          c->stream);
+                r.ss = {ss.pairs, offs_px, 0};
+            }
             KernelTimer t(c, "forward_resolve");
             launch_forward_resolve(r, u, c->stream);
             c->pend_fb = false;
@@ -976,49 +1101,9 @@ int fgl_draw_screen_space_pixels(fgl_ctx *c)
     ShadowSamples ss = {nullptr, nullptr, 0};
     const int mode = c->shadow_cfg.mode;
     if (c->shadow_on && !c->lights.empty() && (mode == FGL_SHADOW_REF_PCF || mode == FGL_SHADOW_REF_PCSS)) {
-        // The filters' unit-disk samples from the stream, pixels in row order.
+        // the filters' unit-disk samples from the stream, pixels in row order
         KernelTimer t(c, "shadow_samples");
-        const size_t P = (size_t)c->W * c->H;
-        const uint32_t nb = (uint32_t)c->shadow_cfg.n_blocker, np = (uint32_t)c->shadow_cfg.n_pcf;
-        const size_t most = P * (mode == FGL_SHADOW_REF_PCF ? np : nb + np);
-        if (most > 0xffffffffull) return fail(c, FGL_ERR_CAPACITY, "stochastic shadows: too many samples");
-        if ((rc = ensure(c, c->rng_samples, most * 2 * sizeof(float)))) return rc;
-        MtState *st = (MtState *)c->rng.p;
-        float *pairs = (float *)c->rng_samples.p;
-        if (mode == FGL_SHADOW_REF_PCF) {
-            launch_rng_accept(2, st, pairs, (uint32_t)most, c->stream);
-            ss = {pairs, nullptr, np};
-        } else {
-            // PCSS takes n_pcf more samples only where its blocker search
-            // finds a blocker.  Pixels the shadow map decides alone are
-            // counted in parallel; the others are walked in order over the
-            // most samples the frame can need; then the stream is restored and
-            // replayed to stop after exactly the samples used (the replay
-            // rewrites the same values).
-            const size_t words = scan_tmp_words(P + 1);
-            if ((rc = ensure(c, c->rng_offs, (5 * (P + 1) + words) * sizeof(uint32_t)))) return rc;
-            uint32_t *base = (uint32_t *)c->rng_offs.p, *upos = base + (P + 1), *list = upos + (P + 1);
-            uint32_t *found = list + (P + 1), *offs = found + (P + 1), *tmp = offs + (P + 1);
-            HIPCHK(c, hipMemcpyAsync(c->rng_saved.p, st, sizeof(MtState), hipMemcpyDeviceToDevice, c->stream));
-            launch_rng_accept(2, st, pairs, (uint32_t)most, c->stream);
-            launch_pcss_classify(u, (const float *)c->gbuf.p, (const float *)c->shadow.p, base, upos, c->stream);
-            launch_exclusive_scan(base, base, P, tmp, c->stream);
-            launch_exclusive_scan(upos, upos, P, tmp, c->stream);
-            launch_pcss_compact(upos, P, list, c->stream);
-            HIPCHK(c, hipMemcpyAsync(c->h_total, upos + P, 4, hipMemcpyDeviceToHost, c->stream));
-            HIPCHK(c, hipStreamSynchronize(c->stream));
-            const uint32_t n_unc = c->h_total[0];
-            launch_pcss_walk_list(u, (const float *)c->gbuf.p, (const float *)c->shadow.p, pairs, base, list, n_unc,
-                                  found, c->stream);
-            launch_exclusive_scan(found, found, n_unc, tmp, c->stream);
-            launch_pcss_offsets(base, upos, found, np, P, offs, c->stream);
-            HIPCHK(c, hipMemcpyAsync(c->h_total, offs + P, 4, hipMemcpyDeviceToHost, c->stream));
-            HIPCHK(c, hipStreamSynchronize(c->stream));
-            const uint32_t used = c->h_total[0];
-            HIPCHK(c, hipMemcpyAsync(st, c->rng_saved.p, sizeof(MtState), hipMemcpyDeviceToDevice, c->stream));
-            launch_rng_accept(2, st, pairs, used, c->stream);
-            ss = {pairs, offs, 0};
-        }
+        if ((rc = stochastic_shadow_samples(c, u, (const float *)c->gbuf.p, (size_t)c->W * c->H, ss))) return rc;
     }
     {
         KernelTimer t(c, "lighting");

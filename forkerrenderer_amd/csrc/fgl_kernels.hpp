@@ -29,6 +29,15 @@ struct RasterArgs {
     const float *ndc_z;        // shadow pass: per vertex, z of DepthShader::ProcessVertex's NDC
 };
 
+// Per-pixel unit-disk samples of the stochastic shadow filters: pixel i's
+// samples start at pair offs[i] of `pairs`, or at i * stride when offs is
+// null (fixed count per pixel).
+struct ShadowSamples {
+    const float *pairs;
+    const uint32_t *offs;
+    uint32_t stride;
+};
+
 struct ResolveArgs {
     const DevDraw *draws;
     const int4 *screen;
@@ -41,6 +50,23 @@ struct ResolveArgs {
     const DevMaterial *materials;
     const DevTexture *textures;
     const float *shadow_map;
+    ShadowSamples ss;          // forward pass, stochastic shadow filters: the winners' samples
+};
+
+// The forward pass's depth test replayed in submission order (k_frag_walk):
+// per tile, triangles in pass-wide id order, each pixel's running depth in a
+// register.  Pixels are indexed column-major, q = px * H + py.
+struct FragWalkArgs {
+    const int4 *screen;
+    const int4 *tri_v;
+    const uint32_t *vals;      // per-tile triangle lists (sorted pairs)
+    const uint32_t *tile_start, *tile_end;
+    int W, H, tiles_x, tiles_y;
+    int fresh;                 // the depth target still holds its clear value
+    const float *depth;        // depth target before the pass
+    uint32_t *count;           // count walk: passing fragments per pixel (q)
+    const uint32_t *base;      // emit walk: exclusive scan of count
+    uint32_t *tri_out, *q_out; // emit walk: the fragments' triangle ids and pixels
 };
 
 // Neural AO still on its network grid, the up pass not yet run: the AO of
@@ -99,6 +125,17 @@ void launch_tile_ranges(const uint32_t *keys, uint32_t n, const uint32_t *n_dev,
 void launch_raster(bool shadow, int TS, const RasterArgs &a, hipStream_t s);
 void launch_gbuffer_resolve(const ResolveArgs &a, const FrameUniforms &u, hipStream_t s);
 void launch_forward_resolve(const ResolveArgs &a, const FrameUniforms &u, hipStream_t s);
+// emit = false: the count walk; true: the emit walk.
+void launch_frag_walk(bool emit, int TS, const FragWalkArgs &a, hipStream_t s);
+// Light 0's shadow inputs of the n passing fragments (triangle tri[k] at
+// pixel q[k], in raster order): shading normal, world position and light
+// NDC as planes of n floats (9 n in all), as the forward shaders compute them.
+void launch_frag_shadow_inputs(const ResolveArgs &a, const FrameUniforms &u, const uint32_t *tri,
+                               const uint32_t *q, uint32_t n, float *planes, hipStream_t s);
+// offs_px[p] for each pixel whose winner vis[p] is passing fragment k:
+// offs[k], or k * stride when offs is null.
+void launch_frag_pixel_offsets(const uint32_t *tri, const uint32_t *q, uint32_t n, int W, int H, const int *vis,
+                               const uint32_t *offs, uint32_t stride, uint32_t *offs_px, hipStream_t s);
 
 // sort.hip
 // Exclusive scan of n uint32 values into out (out may alias in); the total
@@ -128,15 +165,6 @@ void mt_seed_state(uint32_t seed, MtState &st);
 // RandomVectorInUnitSphere (x, y, z), 2 RandomVectorInUnitDisk (x, y).
 void launch_rng_accept(int draws_per_try, MtState *st, float *out, uint32_t n_accept, hipStream_t s);
 
-// Per-pixel unit-disk samples of the stochastic shadow filters: pixel i's
-// samples start at pair offs[i] of `pairs`, or at i * stride when offs is
-// null (fixed count per pixel).
-struct ShadowSamples {
-    const float *pairs;
-    const uint32_t *offs;
-    uint32_t stride;
-};
-
 // shade.hip
 void launch_lighting(const FrameUniforms &u, const float *gbuf, const float *ao, int ao_is_one, const NetAo &net,
                      const float *shadow_map, const ShadowSamples &ss, float *fb, hipStream_t s);
@@ -144,18 +172,22 @@ void launch_lighting(const FrameUniforms &u, const float *gbuf, const float *ao,
 // pixel in row order); pow3[c] = powf(1 - c / 32, 3) from the host libm.
 void launch_ssao_reference(const FrameUniforms &u, const float *samples, const float *pow3,
                            const float *depth, const float *gbuf, float *ao, hipStream_t s);
-// Offsets of the PCSS filter's samples (a pixel takes n_blocker pairs, plus
-// n_pcf when its blocker search finds a blocker): classify every pixel as
-// decided by the shadow map alone or not (cnt: its pairs when decided,
-// n_blocker otherwise; unc: 1 when undecided), compact the undecided pixels
-// after a scan of unc (upos), walk them in order from the scanned cnt (base),
-// then offs[i] = base[i] + n_pcf * (blockers found by undecided pixels
-// before i) for i = 0..P.
-void launch_pcss_classify(const FrameUniforms &u, const float *gbuf, const float *shadow_map, uint32_t *cnt,
+// Offsets of the PCSS filter's samples for P shading points in the order
+// they shade (a point takes n_blocker pairs, plus n_pcf when its blocker
+// search finds a blocker); g holds each point's normal, world position and
+// light NDC as planes of stride P (the first nine G-buffer planes, or the
+// forward pass's fragment planes).  Classify every point as decided by the
+// shadow map alone or not (cnt: its pairs when decided, n_blocker otherwise;
+// unc: 1 when undecided), compact the undecided points after a scan of unc
+// (upos), walk them in order from the scanned cnt (base), then
+// offs[i] = base[i] + n_pcf * (blockers found by undecided points before i)
+// for i = 0..P.
+void launch_pcss_classify(const FrameUniforms &u, const float *g, size_t P, const float *shadow_map, uint32_t *cnt,
                           uint32_t *unc, hipStream_t s);
 void launch_pcss_compact(const uint32_t *upos, size_t P, uint32_t *list, hipStream_t s);
-void launch_pcss_walk_list(const FrameUniforms &u, const float *gbuf, const float *shadow_map, const float *pairs,
-                           const uint32_t *base, const uint32_t *list, uint32_t n, uint32_t *found, hipStream_t s);
+void launch_pcss_walk_list(const FrameUniforms &u, const float *g, size_t P, const float *shadow_map,
+                           const float *pairs, const uint32_t *base, const uint32_t *list, uint32_t n,
+                           uint32_t *found, hipStream_t s);
 void launch_pcss_offsets(const uint32_t *base, const uint32_t *upos, const uint32_t *fpos, uint32_t n_pcf, size_t P,
                          uint32_t *offs, hipStream_t s);
 void launch_ssao(const FrameUniforms &u, const SsaoArgs &sa, const float *depth,

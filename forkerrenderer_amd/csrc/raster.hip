@@ -167,6 +167,9 @@ __device__ __forceinline__ void forward_shade(const ResolveArgs &a, const FrameU
     Varying v1 = main_vertex(dr, u, f, 1);
     Varying v2 = main_vertex(dr, u, f, 2);
     FragGeom fg = frag_geometry(dr, u, a.textures, v0, v1, v2, b, m.map_bump);
+    // the stochastic shadow filters: this fragment's samples in the stream
+    const float *rdisk = nullptr;
+    if (a.ss.pairs) rdisk = a.ss.pairs + 2 * (a.ss.offs ? (size_t)a.ss.offs[i] : i * (size_t)a.ss.stride);
     Vec3 c;
     if (!dr.support_pbr) {
         Vec3 dc = m.map_kd >= 0 ? tex_sample(a.textures, m.map_kd, fg.uv)
@@ -176,7 +179,7 @@ __device__ __forceinline__ void forward_shade(const ResolveArgs &a, const FrameU
         float shin = 1.f;
         if (m.map_ks >= 0) shin = tex_samplef(a.textures, m.map_ks, fg.uv) + 5;
         c = shade_point(u, a.shadow_map, false, false, fg.pos, fg.normal, fg.lndc, dc, em,
-                        vec3(m.ka[0], m.ks[0], shin));
+                        vec3(m.ka[0], m.ks[0], shin), rdisk);
     } else {
         Vec3 al = m.map_base >= 0 ? tex_sample(a.textures, m.map_base, fg.uv)
                                   : vec3(m.albedo[0], m.albedo[1], m.albedo[2]);
@@ -186,7 +189,7 @@ __device__ __forceinline__ void forward_shade(const ResolveArgs &a, const FrameU
         float metal = m.map_metal >= 0 ? tex_samplef(a.textures, m.map_metal, fg.uv) : m.metalness;
         float ao = m.map_ao >= 0 ? tex_samplef(a.textures, m.map_ao, fg.uv) : 1.f;
         c = shade_point(u, a.shadow_map, true, false, fg.pos, fg.normal, fg.lndc, al, em,
-                        vec3(ao, metal, rough));
+                        vec3(ao, metal, rough), rdisk);
     }
     a.fb[i] = c.x;
     a.fb[P + i] = c.y;
@@ -227,6 +230,142 @@ __global__ __launch_bounds__(256) void k_forward_resolve(ResolveArgs a, FrameUni
     const DevDraw &dr = a.draws[tv.w];
     const int f = g - dr.tri_base;
     forward_shade(a, u, dr, a.materials[dr.face_mat[f]], f, b, i, P);
+}
+
+// ---------------- the forward pass's fragment sequence ----------------
+//
+// The forward shaders run for every fragment that passes the depth test at
+// the moment it is drawn (src/forkergl.cpp:169-204): triangles in
+// submission order, each over its bounding box with px outer and py inner,
+// a fragment passing when !(depth >= the depth buffer's value then).  With
+// the stochastic shadow filters every such fragment draws from the random
+// stream, so the samples a pixel's final colour uses depend on every passing
+// fragment drawn before it, at any pixel.  k_frag_walk replays the depth
+// test in that order inside a tile (the tile's triangles one after another
+// in id order, a thread per pixel, the running depth in a register): the
+// count walk counts each pixel's passing fragments; the emit walk writes
+// them as (triangle id, q) from the pixel's offset in the exclusive scan of
+// the counts over the pixels in column-major order (q = px * H + py).  A
+// stable sort by triangle id then leaves them in the reference's order: by
+// triangle, then px, then py.  Coverage and depth are the rasteriser's own
+// (stage_tri, cover, tri_depth), so the last passing fragment of a pixel is
+// the winner k_raster finds.
+template <int TS, int BLOCK, bool EMIT>
+__global__ __launch_bounds__(BLOCK) void k_frag_walk(FragWalkArgs a)
+{
+    constexpr int PPT = TS * TS / BLOCK;
+    static_assert(PPT * BLOCK == TS * TS, "whole pixels per thread");
+    __shared__ TriSoA<BLOCK> tri;
+    const int tile = blockIdx.x;
+    const int x0 = (tile % a.tiles_x) * TS, y0 = (tile / a.tiles_x) * TS;
+    const int tid = threadIdx.x;
+    int px[PPT], py[PPT];
+    float run[PPT];
+    uint32_t pos[PPT];
+#pragma unroll
+    for (int k = 0; k < PPT; ++k) {
+        const int p = tid + k * BLOCK;
+        px[k] = x0 + p % TS;
+        py[k] = y0 + p / TS;
+        const bool in = px[k] < a.W && py[k] < a.H;
+        run[k] = 3.40282347e+38f;
+        if (in && !a.fresh) run[k] = a.depth[(size_t)px[k] + (size_t)py[k] * a.W];
+        pos[k] = (EMIT && in) ? a.base[(size_t)px[k] * a.H + py[k]] : 0u;
+        if (!in) px[k] = -1;   // inside no bounding box
+    }
+    const uint32_t start = a.tile_start[tile], end = a.tile_end[tile];
+    for (uint32_t base = start; base < end; base += BLOCK) {
+        const int n = imin(BLOCK, (int)(end - base));
+        __syncthreads();   // the previous chunk fully walked
+        if (tid < n) {
+            const int g = (int)a.vals[base + tid];
+            const int4 tv = a.tri_v[g];
+            TriL t;
+            stage_tri(t, g, a.screen[tv.x], a.screen[tv.y], a.screen[tv.z], a.W, a.H);
+            tri.put(tid, t);
+        }
+        __syncthreads();
+        for (int i = 0; i < n; ++i) {
+            const int bb0 = tri.bb0[i], bb1 = tri.bb1[i];
+            const int minx = bb0 & 0xffff, miny = bb0 >> 16, maxx = bb1 & 0xffff, maxy = bb1 >> 16;
+            bool hit[PPT];
+            bool any = false;
+#pragma unroll
+            for (int k = 0; k < PPT; ++k) {
+                hit[k] = px[k] >= minx && px[k] <= maxx && py[k] >= miny && py[k] <= maxy;
+                any |= hit[k];
+            }
+            if (!any) continue;
+            const TriL t = tri.get(i);
+#pragma unroll
+            for (int k = 0; k < PPT; ++k) {
+                float b0, b1, b2;
+                if (!hit[k] || !cover(t, px[k], py[k], b0, b1, b2)) continue;
+                const float dd = tri_depth(t, b0, b1, b2);
+                if (dd >= run[k]) continue;
+                run[k] = dd;
+                if (EMIT) {
+                    a.tri_out[pos[k]] = (uint32_t)t.g;
+                    a.q_out[pos[k]] = (uint32_t)px[k] * (uint32_t)a.H + (uint32_t)py[k];
+                }
+                ++pos[k];
+            }
+        }
+    }
+    if (!EMIT) {
+#pragma unroll
+        for (int k = 0; k < PPT; ++k)
+            if (px[k] >= 0) a.count[(size_t)px[k] * a.H + py[k]] = pos[k];
+    }
+}
+
+// Light 0's shadow inputs of passing fragment k, as forward_shade computes
+// them: shading normal, world position, light NDC (planes of n floats).
+__global__ __launch_bounds__(256) void k_frag_shadow_inputs(ResolveArgs a, FrameUniforms u,
+                                                            const uint32_t *__restrict__ tri,
+                                                            const uint32_t *__restrict__ q, uint32_t n,
+                                                            float *__restrict__ out)
+{
+    const uint32_t k = blockIdx.x * 256 + threadIdx.x;
+    if (k >= n) return;
+    const int g = (int)tri[k];
+    const uint32_t qq = q[k];
+    const int px = (int)(qq / (uint32_t)a.H), py = (int)(qq - (uint32_t)px * (uint32_t)a.H);
+    int4 tv;
+    const Vec3 b = winner_bary(a.screen, a.tri_v, g, a.W, a.H, px, py, tv);
+    const DevDraw &dr = a.draws[tv.w];
+    const int f = g - dr.tri_base;
+    const DevMaterial &m = a.materials[dr.face_mat[f]];
+    Varying v0 = main_vertex(dr, u, f, 0);
+    Varying v1 = main_vertex(dr, u, f, 1);
+    Varying v2 = main_vertex(dr, u, f, 2);
+    const FragGeom fg = frag_geometry(dr, u, a.textures, v0, v1, v2, b, m.map_bump);
+    const size_t N = n;
+    out[k] = fg.normal.x;
+    out[N + k] = fg.normal.y;
+    out[2 * N + k] = fg.normal.z;
+    out[3 * N + k] = fg.pos.x;
+    out[4 * N + k] = fg.pos.y;
+    out[5 * N + k] = fg.pos.z;
+    out[6 * N + k] = fg.lndc.x;
+    out[7 * N + k] = fg.lndc.y;
+    out[8 * N + k] = fg.lndc.z;
+}
+
+// A pixel's triangles pass at most once each, so exactly one passing
+// fragment of a pixel with a winner carries the winner's id.
+__global__ __launch_bounds__(256) void k_frag_pixel_offsets(const uint32_t *__restrict__ tri,
+                                                            const uint32_t *__restrict__ q, uint32_t n, int W,
+                                                            int H, const int *__restrict__ vis,
+                                                            const uint32_t *__restrict__ offs, uint32_t stride,
+                                                            uint32_t *__restrict__ offs_px)
+{
+    const uint32_t k = blockIdx.x * 256 + threadIdx.x;
+    if (k >= n) return;
+    const uint32_t qq = q[k];
+    const uint32_t px = qq / (uint32_t)H, py = qq - px * (uint32_t)H;
+    const size_t p = (size_t)px + (size_t)py * W;
+    if (vis[p] == (int)tri[k]) offs_px[p] = offs ? offs[k] : k * stride;
 }
 
 // ---------------- launch wrappers ----------------
@@ -320,6 +459,37 @@ void launch_forward_resolve(const ResolveArgs &a, const FrameUniforms &u, hipStr
 {
     if (a.W <= 0 || a.H <= 0) return;
     hipLaunchKernelGGL(k_forward_resolve, grid2(a.W, a.H), dim3(256), 0, s, a, u);
+}
+
+template <bool EMIT>
+static void launch_frag_walk_t(int TS, const FragWalkArgs &a, hipStream_t s)
+{
+    const unsigned nt = (unsigned)(a.tiles_x * a.tiles_y);
+    if (TS == 8) hipLaunchKernelGGL((k_frag_walk<8, 64, EMIT>), dim3(nt), dim3(64), 0, s, a);
+    else if (TS == 16) hipLaunchKernelGGL((k_frag_walk<16, 256, EMIT>), dim3(nt), dim3(256), 0, s, a);
+    else hipLaunchKernelGGL((k_frag_walk<32, 256, EMIT>), dim3(nt), dim3(256), 0, s, a);
+}
+
+void launch_frag_walk(bool emit, int TS, const FragWalkArgs &a, hipStream_t s)
+{
+    if (a.tiles_x * a.tiles_y == 0) return;
+    if (emit) launch_frag_walk_t<true>(TS, a, s);
+    else launch_frag_walk_t<false>(TS, a, s);
+}
+
+void launch_frag_shadow_inputs(const ResolveArgs &a, const FrameUniforms &u, const uint32_t *tri,
+                               const uint32_t *q, uint32_t n, float *planes, hipStream_t s)
+{
+    if (n == 0) return;
+    hipLaunchKernelGGL(k_frag_shadow_inputs, dim3(blocks_for(n, 256)), dim3(256), 0, s, a, u, tri, q, n, planes);
+}
+
+void launch_frag_pixel_offsets(const uint32_t *tri, const uint32_t *q, uint32_t n, int W, int H, const int *vis,
+                               const uint32_t *offs, uint32_t stride, uint32_t *offs_px, hipStream_t s)
+{
+    if (n == 0) return;
+    hipLaunchKernelGGL(k_frag_pixel_offsets, dim3(blocks_for(n, 256)), dim3(256), 0, s, tri, q, n, W, H, vis,
+                       offs, stride, offs_px);
 }
 
 }  // namespace fgl

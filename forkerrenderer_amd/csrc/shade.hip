@@ -29,17 +29,19 @@ __global__ __launch_bounds__(256) void k_ssao_ref(FrameUniforms u, const float *
 
 // ---- PCSS sample offsets for the stochastic filter (FGL_SHADOW_REF_PCSS) ----
 //
-// The lighting pass visits pixels in row order and each pixel draws n_blocker
+// The lighting pass visits pixels in row order (the forward pass its passing
+// fragments in raster order) and each shading point draws n_blocker
 // unit-disk samples, plus n_pcf more when its blocker search finds a blocker
-// (PCSS, shadow.cpp:93-106), so a pixel's offset into the stream depends on
+// (PCSS, shadow.cpp:93-106), so a point's offset into the stream depends on
 // the outcome of every search before it.  Most searches are decided by the
-// shadow map alone: k_pcss_classify checks every texel a sample of the pixel
-// could reach (the square around the disk) and marks the pixel "no blocker
+// shadow map alone: k_pcss_classify checks every texel a sample of the point
+// could reach (the square around the disk) and marks the point "no blocker
 // for any sample set", "a blocker for any sample set" or "depends on the
 // samples".  Only the last kind is walked in order (k_pcss_walk_list, one
-// wavefront), at offsets that count the decided pixels before it.
+// wavefront), at offsets that count the decided points before it.  The
+// points' inputs are planes of stride P: normal, world position, light NDC.
 
-// Light 0's shadow coordinate and bias at pixel i, as shade_point computes them.
+// Light 0's shadow coordinate and bias at point i, as shade_point computes them.
 __device__ __forceinline__ void pcss_pixel(const FrameUniforms &u, const float *__restrict__ g, size_t P, size_t i,
                                            Vec3 &sc, float &bias)
 {
@@ -53,11 +55,10 @@ __device__ __forceinline__ void pcss_pixel(const FrameUniforms &u, const float *
 // cnt[i] = n_blocker + n_pcf for "a blocker for any sample set", n_blocker
 // otherwise; unc[i] = 1 for "depends on the samples".  Nothing is drawn with
 // shadows off or without a light.
-__global__ __launch_bounds__(256) void k_pcss_classify(FrameUniforms u, const float *__restrict__ g,
+__global__ __launch_bounds__(256) void k_pcss_classify(FrameUniforms u, const float *__restrict__ g, size_t P,
                                                        const float *__restrict__ smap, uint32_t *__restrict__ cnt,
                                                        uint32_t *__restrict__ unc)
 {
-    const size_t P = (size_t)u.W * u.H;
     const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= P) return;
     const DevShadowCfg &c = u.shadow;
@@ -115,7 +116,7 @@ __global__ __launch_bounds__(256) void k_pcss_compact(const uint32_t *__restrict
 // lanes evaluate its blocker samples in parallel and lane order sums them as
 // FindAverageBlockDepth does (shadow.cpp:65-91).  found[k] = 1 when it finds
 // a blocker.
-__global__ __launch_bounds__(64) void k_pcss_walk_list(FrameUniforms u, const float *__restrict__ g,
+__global__ __launch_bounds__(64) void k_pcss_walk_list(FrameUniforms u, const float *__restrict__ g, size_t P,
                                                        const float *__restrict__ smap,
                                                        const float *__restrict__ pairs,
                                                        const uint32_t *__restrict__ base,
@@ -123,7 +124,6 @@ __global__ __launch_bounds__(64) void k_pcss_walk_list(FrameUniforms u, const fl
                                                        uint32_t *__restrict__ found)
 {
     const int lane = threadIdx.x;
-    const size_t P = (size_t)u.W * u.H;
     const DevShadowCfg &c = u.shadow;
     uint32_t F = 0;
     for (uint32_t k = 0; k < n; ++k) {
@@ -359,12 +359,11 @@ void launch_ssao_reference(const FrameUniforms &u, const float *samples, const f
     hipLaunchKernelGGL(k_ssao_ref, grid2(u.W, u.H), dim3(256), 0, s, u, samples, pow3, depth, gbuf, ao);
 }
 
-void launch_pcss_classify(const FrameUniforms &u, const float *gbuf, const float *shadow_map, uint32_t *cnt,
+void launch_pcss_classify(const FrameUniforms &u, const float *g, size_t P, const float *shadow_map, uint32_t *cnt,
                           uint32_t *unc, hipStream_t s)
 {
-    const size_t P = (size_t)u.W * u.H;
     if (P == 0) return;
-    hipLaunchKernelGGL(k_pcss_classify, dim3((unsigned)((P + 255) / 256)), dim3(256), 0, s, u, gbuf, shadow_map, cnt,
+    hipLaunchKernelGGL(k_pcss_classify, dim3((unsigned)((P + 255) / 256)), dim3(256), 0, s, u, g, P, shadow_map, cnt,
                        unc);
 }
 
@@ -374,11 +373,12 @@ void launch_pcss_compact(const uint32_t *upos, size_t P, uint32_t *list, hipStre
     hipLaunchKernelGGL(k_pcss_compact, dim3((unsigned)((P + 255) / 256)), dim3(256), 0, s, upos, P, list);
 }
 
-void launch_pcss_walk_list(const FrameUniforms &u, const float *gbuf, const float *shadow_map, const float *pairs,
-                           const uint32_t *base, const uint32_t *list, uint32_t n, uint32_t *found, hipStream_t s)
+void launch_pcss_walk_list(const FrameUniforms &u, const float *g, size_t P, const float *shadow_map,
+                           const float *pairs, const uint32_t *base, const uint32_t *list, uint32_t n,
+                           uint32_t *found, hipStream_t s)
 {
     if (n == 0) return;
-    hipLaunchKernelGGL(k_pcss_walk_list, dim3(1), dim3(64), 0, s, u, gbuf, shadow_map, pairs, base, list, n, found);
+    hipLaunchKernelGGL(k_pcss_walk_list, dim3(1), dim3(64), 0, s, u, g, P, shadow_map, pairs, base, list, n, found);
 }
 
 void launch_pcss_offsets(const uint32_t *base, const uint32_t *upos, const uint32_t *fpos, uint32_t n_pcf, size_t P,

@@ -58,8 +58,11 @@ extern "C" {
 /* The reference's own filters with their samples drawn from the reference's
  * random stream (SOFT_SHADOW_PCF / SOFT_SHADOW_PCSS, src/shaders/shadow.h:14-23,
  * shadow.cpp:47-106): n_pcf (and n_blocker) RandomVectorInUnitDisk samples per
- * pixel from the context's std::mt19937 stream; the disk arrays are unused.
- * Deferred lighting only. */
+ * shading point from the context's std::mt19937 stream; the disk arrays are
+ * unused.  The deferred lighting pass draws them for every pixel in row
+ * order; the forward pass for every fragment that passes the depth test when
+ * drawn (overdrawn ones included), triangles in submission order, px outer,
+ * py inner (src/forkergl.cpp:169-204).  Batched views refuse them. */
 #define FGL_SHADOW_REF_PCF 4
 #define FGL_SHADOW_REF_PCSS 5
 /* Render targets (ForkerGL's public buffers, src/forkergl.h:41-54) */

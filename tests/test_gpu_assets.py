@@ -19,6 +19,7 @@ import numpy as np
 import pytest
 
 import forkerrenderer_amd as F
+import oracle as O
 from tests import assets as A
 from tests.util import assert_bitexact, gpu_frame, lsb_stats, oracle_frame
 
@@ -63,6 +64,19 @@ def test_readme_diablo_frame(ctx, scene_dir):
     I = sc.info
     m, _ = sc.model(0)
     assert (I.width, I.height, I.deferred, I.shadow_on, I.shadow.mode, len(m["vi"])) == (1280, 800, 0, 1, 2, 5022)
+
+
+def test_readme_diablo_reference_stream(ctx, scene_dir):
+    """The README's run as the reference builds it by default: forward
+    Blinn-Phong with SOFT_SHADOW_PCSS drawing its samples from the random
+    stream (shadow.h:15-16, shadow.cpp:65-106) for every fragment that passes
+    the depth test.  Diablo alone at 1280x800; depth and shadow map
+    bit-exact, the stream consumed to the same draw, image within 1 LSB."""
+    p = A.scene(os.path.join(scene_dir, "assets", "readme_diablo_ref.scene"), ["diablo_pose"],
+                extra=("shadowfilter refpcss",))
+    sc = _check(ctx, p, "README Diablo 1280x800 forward, reference stream", 0.1)
+    assert (sc.info.deferred, sc.info.shadow.mode) == (0, F.SHADOW_REF_PCSS)
+    assert ctx.rng_draws() == O.rng_draws(), (ctx.rng_draws(), O.rng_draws())
 
 
 def test_diablo_deferred_ssaa(ctx, scene_dir):

@@ -1,9 +1,9 @@
 """The reference's random stream on the device against the oracle: the
 generator's unit-sphere and unit-disk vectors and draw counts (bit-exact,
 across calls that end mid-block), Render::DoSSAO with the stream (AO
-bit-exact), and whole deferred frames with the reference's PCF / PCSS filters
-drawing from the stream (G-buffer and AO bit-exact, the stream consumed to
-the same draw, image within the usual +-1 LSB)."""
+bit-exact), and whole deferred and forward frames with the reference's PCF /
+PCSS filters drawing from the stream (G-buffer, AO and depth bit-exact, the
+stream consumed to the same draw, image within the usual +-1 LSB)."""
 import os
 
 import numpy as np
@@ -111,18 +111,80 @@ def test_stochastic_frames_match_the_oracle(ctx, scene_dir, filt):
     assert device_draws > 32 * P * 3
 
 
-def test_stochastic_modes_are_rejected_where_they_cannot_replay(ctx, scene_dir):
-    """The forward pass draws per depth-passing fragment in raster order, which
-    the tile rasteriser does not replay, and batched views take the
-    deterministic filters: both refuse FGL_SHADOW_REF_* with an error (no
-    device work is started)."""
-    p = G.tiny_scene(os.path.join(scene_dir, "rfwd"), seed=33, mode="forward", extra="shadowfilter refpcss")
+@pytest.mark.parametrize("tile", [8, 16, 32])
+@pytest.mark.parametrize("filt", ["refpcf", "refpcss"])
+def test_forward_stochastic_frames_match_the_oracle(ctx, scene_dir, filt, tile):
+    """Forward pass with the reference's PCF / PCSS drawing from the stream:
+    the shaders draw for every fragment that passes the depth test when it is
+    drawn, overdrawn ones included, in raster order (triangles in order, px
+    outer, py inner); the device replays that sequence in each tile.  Depth
+    and shadow map bit-exact, the stream consumed to the same draw, image
+    within 1 LSB; every tile size (each has its own walk kernel)."""
+    p = G.tiny_scene(os.path.join(scene_dir, "f" + filt), seed=33, mode="forward", width=96, height=72, subdiv=3,
+                     tex=16, extra=f"shadowfilter {filt}")
     sc = F.Scene(p)
-    with pytest.raises(F.FglError, match="deferred lighting pass only"):
-        ctx.render_scene(sc)
+    assert not sc.info.deferred
+    print(f"forward stochastic {filt} tile {tile}", flush=True)
+    got = gpu_frame(ctx, sc, tile)
+    device_draws = ctx.rng_draws()
+    want = oracle_frame(sc)
+    assert device_draws == O.rng_draws(), (device_draws, O.rng_draws())
+    assert_bitexact("depth", got["depth"], want["depth"])
+    assert_bitexact("shadow", got["shadow"], want["shadow"])
+    mx, frac = lsb_stats(got["image"], want["image"])
+    print(f"  draws {device_draws}, image max diff {mx}, within 1 LSB {frac:.6f}", flush=True)
+    assert frac >= 0.999, (mx, frac)
+    # every shaded pixel drew at least one sample set (2 draws per try)
+    covered = int((want["depth"] < 3e38).sum())
+    assert device_draws > 2 * 32 * covered
+
+
+def test_forward_stochastic_passes_split_over_flushes_equal_one_pass(ctx, scene_dir):
+    """The depth buffer and the stream carry over between forward passes as
+    in the reference: the models drawn in one pass, or in two passes (the
+    second starting from the first's depth), give the same depth, frame and
+    stream position, equal to the scene render."""
+    p = G.tiny_scene(os.path.join(scene_dir, "fsplit"), seed=34, mode="forward", width=80, height=60, subdiv=3,
+                     tex=16, extra="shadowfilter refpcss")
+    sc = F.Scene(p)
+    full = gpu_frame(ctx, sc)   # sets the scene's matrices, lights and shadow map
+    full_draws = ctx.rng_draws()
+    ids = ctx.upload_scene(sc)
+    assert len(ids) >= 2
+    I = sc.info
+    outs = []
+    for split in (False, True):
+        ctx.init_targets(I.width, I.height)
+        ctx.clear_color(0.12, 0.12, 0.12)
+        ctx.rng_seed(5489)
+        ctx.begin_pass(F.PASS_FORWARD)
+        for j, (mid, M) in enumerate(ids):
+            if split and j == 1:
+                ctx.end_pass()
+                ctx.begin_pass(F.PASS_FORWARD)
+            ctx.draw_model(mid, M)
+        ctx.end_pass()
+        outs.append((ctx.read(F.TARGET_DEPTH), ctx.read(F.TARGET_FRAME), ctx.rng_draws()))
+    for name, k in (("depth", 0), ("frame", 1)):
+        assert_bitexact(f"{name} split", outs[1][k], outs[0][k])
+    assert outs[0][2] == outs[1][2] == full_draws
+    assert_bitexact("depth vs scene", outs[0][0], full["depth"])
+    assert_bitexact("frame vs scene", outs[0][1], full["fb"].reshape(-1))
+
+
+def test_batched_views_refuse_the_stochastic_filters(ctx, scene_dir):
+    """Batched views take the deterministic filters only: a view with
+    FGL_SHADOW_REF_* is refused when the batch is made (no device work is
+    started), and the context still works afterwards."""
+    from forkerrenderer_amd import views as VW
+    v = VW.uneven_views(ctx, n_views=1, seed=3, subdivs=(2,), min_px=32 * 32, max_px=64 * 64, shadow_size=64,
+                        tex=16)[0]
+    assert v.desc.shadow_on
+    v.desc.shadow.mode = F.SHADOW_REF_PCSS
+    with pytest.raises(F.FglError, match="deterministic shadow filters"):
+        VW.Batch(ctx, [v])
     with pytest.raises(F.FglError):
         ctx.rng_vectors(4, 10)
-    # the context still works afterwards
     p2 = G.tiny_scene(os.path.join(scene_dir, "rok"), seed=33)
     sc2 = F.Scene(p2)
     got = gpu_frame(ctx, sc2)

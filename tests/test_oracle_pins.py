@@ -18,6 +18,11 @@ a transposed operand, a wrong constant, an off-by-one texel) fails.
   shadowed.
 * orc_forward_pass's Blinn-Phong inputs (phongshader.h:152-168): ao = Ka.x,
   ks = Ks.x, shininess 1 without a specular map.
+* orc_forward_pass with the reference's random stream (forkergl.cpp:169-204,
+  shadow.cpp:47-106): every fragment that passes the depth test when drawn
+  -- overdrawn ones included -- draws its PCF / PCSS samples, triangles in
+  order, px outer, py inner; against the order read off two triangles'
+  coverage masks, numpy's MT19937 and a step-edge shadow map.
 * MirroredRepeat (texture.h:69-75) against the textbook mirror.
 * orc_ssao_reference (render.cpp:214-289): per-pixel occlusion counts
   against a float64 evaluation of DoSSAO with the same random samples.
@@ -507,6 +512,134 @@ def test_forward_blinn_phong_uses_ka_x_as_ao():
         c = 0.3 * dl * float(np.float32(ka[0])) + (dif + spe) * np.asarray(light[2])
         worst = max(worst, float(np.abs(fb[:, i] - _tonemap(c)).max()))
     assert worst < 2e-5, worst
+
+
+# ---------------------------------------------------------------------------
+# the forward pass with the reference's random stream
+
+_FW, _FH, _EDGE = 40, 30, 16
+_TRI_POS = np.array([(-1.5, -1.2, -3.0), (1.5, -1.2, -3.0), (0.0, 1.4, -3.0),        # far, drawn first
+                     (-0.6, -0.5, -2.5), (0.7, -0.4, -2.5), (0.1, 0.6, -2.5)], np.float32)   # near
+
+
+def _tri_model(faces):
+    faces = np.asarray(faces, np.int32)
+    return {"pos": _TRI_POS, "uv": np.zeros((6, 2), np.float32),
+            "nrm": np.tile([[0, 0, 1]], (6, 1)).astype(np.float32), "tan": None, "vi": faces, "ti": faces,
+            "ni": faces, "face_material": np.zeros(len(faces), np.int32), "support_pbr": False}
+
+
+def _tri_frame(cfg):
+    eye = [0.0, 0.0, 0.0]
+    V = O.make_lookat(eye, [0.0, 0.0, -1.0])
+    Pm = O.make_perspective(45.0, np.float32(_FW) / np.float32(_FH), 0.01, 20.0)
+    vp = O.make_viewport(0, 0, _FW, _FH)
+    # light NDC = (0.3 x, 0.3 y, 0): shadow u = 0.5 + 0.15 x across the far
+    # triangle, receivers at depth 0.5
+    Lm = np.diag([0.3, 0.3, 0.0, 1.0]).astype(np.float32)
+    return O.make_frame(V, Pm, Lm, vp, O.make_viewport(0, 0, SW, SH), eye, 1, cfg,
+                        [(0, [0.5, 1.0, 1.0], [1.0, 1.0, 1.0])])
+
+
+def _tri_forward(cfg, smap, faces=((0, 1, 2), (3, 4, 5))):
+    sc = O.OracleScene([(_tri_model(faces), I4)], [_mat(kd=[0.7, 0.6, 0.5], ka=[1, 1, 1], ks=[0.2, 0.2, 0.2])], [])
+    return O.forward_pass(sc, _tri_frame(cfg), _FW, _FH, smap, SW, SH)
+
+
+def _tri_gbuffer(faces):
+    sc = O.OracleScene([(_tri_model(faces), I4)], [_mat(kd=[0.7, 0.6, 0.5])], [])
+    return O.geometry_pass(sc, _tri_frame(_SCfg(0)), _FW, _FH)
+
+
+def _fragments_in_raster_order():
+    """The forward pass's passing fragments, read off each triangle's coverage
+    alone: the far triangle first (every covered pixel passes), then the near
+    one (nearer everywhere, so every covered pixel passes again); inside a
+    triangle px outer, py inner.  Returns [(shadow u, shadow z, pixel)] and
+    the pixels' final fragment index."""
+    frags, final = [], {}
+    for faces in (((0, 1, 2),), ((3, 4, 5),)):
+        depth, g = _tri_gbuffer(faces)
+        cov = np.nonzero(depth < 3e38)[0]
+        px, py = cov % _FW, cov // _FW
+        for i in cov[np.lexsort((py, px))]:
+            sx = np.float32(g[6, i]) * np.float32(0.5) + np.float32(0.5)
+            sz = np.float32(g[8, i]) * np.float32(0.5) + np.float32(0.5)
+            final[int(i)] = len(frags)
+            frags.append((sx, sz, int(i)))
+    return frags, final
+
+
+def _lit_fraction(u, samples, radius):
+    """PCF on the step map: a sample lights the receiver when its column
+    floor((float)31 * u) is right of the edge; (float) arithmetic as
+    pcf_disk's u = sc.x + d.x * filter."""
+    lit = 0
+    for dx in samples[:, 0]:
+        uu = np.float32(u) + np.float32(dx) * np.float32(radius)
+        lit += int(np.float32(SW - 1) * uu) >= _EDGE
+    return lit / len(samples)
+
+
+def _vis_from_frames(fb, fb1, fb0):
+    """Visibility from the shaded colour: c = A + K (0.4 + 0.6 vis) before the
+    tone map (phongshader.h:171-215), so vis = (c - c(vis 0)) / (c(vis 1) - c(vis 0))."""
+    def c(y):
+        t = np.asarray(y, np.float64) ** 2.2
+        return t / (1 - t)
+    with np.errstate(invalid="ignore", divide="ignore"):   # uncovered pixels: 0 / 0
+        return (c(fb[0]) - c(fb0[0])) / (c(fb1[0]) - c(fb0[0]))
+
+
+@pytest.mark.parametrize("mode", [4, 5])
+def test_forward_stochastic_shadows_draw_per_passing_fragment(mode):
+    from tests.test_oracle_rng import mt_raw, stream_disk
+    frags, final = _fragments_in_raster_order()
+    overdrawn = len(frags) - len(final)
+    assert len(final) > 300 and overdrawn > 40, (len(final), overdrawn)
+    nb, npcf = 16, 16
+    bf, pf, area, d_b = 0.05, 0.05, 2.5, 0.2
+    cfg = _SCfg(mode, bd=[0.0] * (2 * nb if mode == 5 else 0), pd=[0.0] * 2 * npcf, pcf_filter=0.05, bf=bf, pf=pf,
+                area=area)
+    raw = mt_raw(400000)
+    disk, _ = stream_disk(raw, len(frags) * (nb + npcf))
+    # the expected walk: each fragment's samples at its offset in the stream
+    vis_of, off = {}, 0
+    for k, (sx, sz, i) in enumerate(frags):
+        if mode == 4:
+            vis_of[k] = _lit_fraction(sx, disk[off:off + npcf], 0.05)
+            off += npcf
+            continue
+        bs = disk[off:off + nb]
+        off += nb
+        blocker, nbk = np.float32(0), np.float32(0)
+        for dx in bs[:, 0]:
+            uu = np.float32(sx) + np.float32(np.float64(dx) * bf)
+            if int(np.float32(SW - 1) * uu) < _EDGE:      # depth 0.2 < 0.5 - bias: a blocker
+                blocker = np.float32(blocker + np.float32(d_b))
+                nbk = np.float32(nbk + np.float32(1))
+        if nbk < 1:
+            vis_of[k] = 1.0
+            continue
+        dB = np.float32(blocker / nbk)
+        pen = np.float32(np.float32(np.float32(sz - dB) * np.float32(area)) / dB)
+        vis_of[k] = _lit_fraction(sx, disk[off:off + npcf], np.float32(pf * np.float64(pen)))
+        off += npcf
+    used = stream_disk(raw, off)[1] if off else 0
+    step = _step_map(left=d_b, right=1.0)
+    O.rng_seed(5489)
+    depth, fb = _tri_forward(cfg, step)
+    assert O.rng_draws() == used, (O.rng_draws(), used)
+    _, fb1 = _tri_forward(_SCfg(0), np.full((SH, SW), 1.0, np.float32))
+    _, fb0 = _tri_forward(_SCfg(0), np.full((SH, SW), d_b, np.float32))
+    vis = _vis_from_frames(fb, fb1, fb0)
+    want = np.full(_FW * _FH, np.nan)
+    for i, k in final.items():
+        want[i] = vis_of[k]
+    idx = np.array(sorted(final))
+    assert np.all(depth[idx] < 3e38)
+    assert np.abs(vis[idx] - want[idx]).max() < 2e-3, np.abs(vis[idx] - want[idx]).max()
+    assert len(np.unique(want[idx])) > 5, "the visibilities do not tell sample sets apart"
 
 
 # ---------------------------------------------------------------------------
