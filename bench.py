@@ -218,20 +218,25 @@ def run_neural_ao(steps, warmup, heldout_frames=200):
 README_DIABLO_S = 0.106988 + 3.97755   # shadow pass + forward pass, the reference's README.md:206,213
 
 
-def readme_scene():
+def readme_scene(stream=True):
     """The README's console-output run (README.md:190-223): obj/diablo_pose
     (5,022 faces, three 1024^2 textures) alone, forward Blinn-Phong,
-    1280x800, shadow on with the default PCSS filter, SSAA and SSAO off;
-    the asset comes from the committed byte-for-byte fixtures."""
+    1280x800, shadow on, SSAA and SSAO off; the asset comes from the
+    committed byte-for-byte fixtures.  stream: PCSS as the reference builds it
+    (SOFT_SHADOW_PCSS, shadow.h:15-16), its disk samples drawn from the
+    random stream for every fragment that passes the depth test; otherwise
+    the same counts and sizes on one fixed disk."""
     from tests import assets as A
-    return A.scene(os.path.join(ROOT, "_scenes", "readme", "diablo.scene"), ["diablo_pose"])
+    name = "diablo_stream.scene" if stream else "diablo.scene"
+    return A.scene(os.path.join(ROOT, "_scenes", "readme", name), ["diablo_pose"],
+                   extra=("shadowfilter refpcss",) if stream else ())
 
 
-def run_readme_scene(steps, warmup):
+def run_readme_scene(steps, warmup, stream=True):
     """Whole frames of the README scene on the device (the scene resident),
     with the per-kernel breakdown from a separate run timed with HIP events."""
     import forkerrenderer_amd as F
-    sc = F.Scene(readme_scene())
+    sc = F.Scene(readme_scene(stream))
     ctx = F.Context(0)
     ctx.set_tile_size(16)
     for _ in range(warmup):
@@ -242,6 +247,7 @@ def run_readme_scene(steps, warmup):
         ctx.render_scene(sc)
     ctx.sync()
     sec = (time.perf_counter() - t0) / steps
+    draws = ctx.rng_draws()
     ctx.read_timings()
     ctx.enable_timing(True, None)
     for _ in range(steps):
@@ -253,15 +259,20 @@ def run_readme_scene(steps, warmup):
     ctx.enable_timing(False)
     ctx.close()
     ntri, _ = scene_stats(sc)
-    return sc, {"scene": "README console-output run: obj/diablo_pose alone, forward Blinn-Phong, 1280x800, "
-                         "shadow on: PCSS with the reference's sample counts and sizes (32 blocker-search / 64 "
-                         "PCF samples, area 2.5) on a fixed disk instead of its per-fragment random disk; "
-                         "SSAA off, SSAO off", "triangles": ntri, "frames": steps,
-                "ms_per_frame": sec * 1e3, "frames_per_s": 1.0 / sec, "kernel_ms_per_frame": per,
-                "published_s_per_frame": README_DIABLO_S,
-                "published_note": "shadow pass 0.106988 s + forward pass 3.97755 s (README.md:206,213); "
-                                  "hardware and build type unstated",
-                "speedup_vs_published": README_DIABLO_S / sec}
+    shadow = ("PCSS as the reference builds it: 32 blocker-search and 64 PCF unit-disk samples drawn from its "
+              "mt19937 stream for every fragment that passes the depth test, in raster order" if stream else
+              "PCSS with the reference's sample counts and sizes (32 blocker-search / 64 PCF samples, area 2.5) "
+              "on one fixed disk instead of its per-fragment random disk")
+    out = {"scene": "README console-output run: obj/diablo_pose alone, forward Blinn-Phong, 1280x800, shadow on: "
+                    + shadow + "; SSAA off, SSAO off", "triangles": ntri, "frames": steps,
+           "ms_per_frame": sec * 1e3, "frames_per_s": 1.0 / sec, "kernel_ms_per_frame": per,
+           "published_s_per_frame": README_DIABLO_S,
+           "published_note": "shadow pass 0.106988 s + forward pass 3.97755 s (README.md:206,213), the "
+                             "reference's default build (random-stream PCSS); hardware and build type unstated",
+           "speedup_vs_published": README_DIABLO_S / sec}
+    if stream:
+        out["stream_draws_per_frame"] = draws
+    return sc, out
 
 
 def readme_cpu(sc, reps):
@@ -488,9 +499,10 @@ def main():
         line["neural_ao"] = run_neural_ao(max(5, a.steps // 2), a.warmup, a.nao_heldout)
     if not a.no_views:
         line["batched_views"] = run_batched_views(max(3, a.steps // 4))
-    rsc = None
+    rsc = rsc_fixed = None
     if not a.no_readme:
-        rsc, line["readme_scene"] = run_readme_scene(a.steps, a.warmup)
+        rsc, line["readme_scene"] = run_readme_scene(a.steps, a.warmup, True)
+        rsc_fixed, line["readme_scene_fixed_disk"] = run_readme_scene(a.steps, a.warmup, False)
     cpu = None
     if not a.no_cpu:
         # after the GPU work: the process is pinned to one core from here on
@@ -503,11 +515,13 @@ def main():
                "seconds_per_frame": t_cpu, "per_pass_s": detail, "pinned_core": core,
                "host_cpu": model, "host_logical_cpus": ncpu,
                "band_estimate_s": est, "band_rows": band}
-        if rsc is not None:
+        for key, r in (("readme_scene", rsc), ("readme_scene_fixed_disk", rsc_fixed)):
+            if r is None:
+                continue
             # the README scene's whole frame by the oracle, same pinned core, best of 3
-            t_r = readme_cpu(rsc, 3)
-            line["readme_scene"].update({"cpu_port_s_per_frame": t_r, "cpu_port_cores": 1,
-                                         "speedup_vs_cpu_port": t_r / (line["readme_scene"]["ms_per_frame"] * 1e-3)})
+            t_r = readme_cpu(r, 3)
+            line[key].update({"cpu_port_s_per_frame": t_r, "cpu_port_cores": 1,
+                              "speedup_vs_cpu_port": t_r / (line[key]["ms_per_frame"] * 1e-3)})
     line["cpu_baseline"] = cpu
     line["speedup_vs_cpu"] = None if cpu is None else (1.0 / sec) / cpu["value"]
     print(json.dumps(line))

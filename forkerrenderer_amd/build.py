@@ -30,10 +30,17 @@ COMMON = [
 ]
 
 
+# host-only sources (no device code), built by the host compiler
+HOST_SOURCES = ["mt_jump.cpp"]
+
+
 def _compile(src):
     path = os.path.join(CSRC, src)
     obj = os.path.join(BUILD, src + ".o")
-    cmd = [HIPCC] + COMMON + ["-x", "hip", "-c", path, "-o", obj]
+    if src in HOST_SOURCES:
+        cmd = ["g++", "-O2", "-std=c++17", "-fPIC", "-I", CSRC, "-c", path, "-o", obj]
+    else:
+        cmd = [HIPCC] + COMMON + ["-x", "hip", "-c", path, "-o", obj]
     r = subprocess.run(cmd, capture_output=True, text=True)
     return src, obj, r.returncode, r.stdout + r.stderr
 
@@ -41,7 +48,7 @@ def _compile(src):
 def build(verbose=False):
     os.makedirs(BUILD, exist_ok=True)
     os.makedirs(LIBDIR, exist_ok=True)
-    sources = [s for s in SOURCES if os.path.exists(os.path.join(CSRC, s))]
+    sources = [s for s in SOURCES + HOST_SOURCES if os.path.exists(os.path.join(CSRC, s))]
     with ThreadPoolExecutor(max_workers=min(8, len(sources))) as ex:
         results = list(ex.map(_compile, sources))
     failed = [r for r in results if r[2] != 0]

@@ -15,6 +15,7 @@
 
 #include "../../include/fgl.h"
 #include "ctx_internal.hpp"
+#include "mt_jump.hpp"
 #include "fgl_kernels.hpp"
 #include "host_io.hpp"
 
@@ -99,10 +100,17 @@ struct fgl_ctx {
     // (pair count, big-triangle count, sequence number), polled by the host
     uint32_t *h_pub = nullptr, *d_pub = nullptr;
     uint32_t pub_seq = 0;
-    // the reference's random stream (device MtState), a copy for the PCSS
-    // replay, its sample buffers, the DoSSAO powf table
-    DevBuf rng, rng_saved, rng_samples, rng_offs, ssao_pow3;
+    // the reference's random stream (device MtState), its sample buffers,
+    // the PCSS sample offsets, the DoSSAO powf table
+    DevBuf rng, rng_samples, rng_offs, ssao_pow3;
     uint32_t rng_seed = 5489u;
+    // the parallel generator: base sequence, jumped chunk arrays, raw draws,
+    // try flags / their scan, the jump polynomials uploaded so far; the last
+    // fill's tries (T of D draws) for its commit
+    DevBuf mt_base, mt_states, mt_raw, mt_pos, mt_polys;
+    int mt_polys_n = 0;
+    uint32_t fill_T = 0, fill_n = 0;
+    int fill_D = 0;
 
     // device address of the allocation-free entry point's scratch array
     float *noalloc_scratch = nullptr;
@@ -310,7 +318,6 @@ int fgl_ctx_create(int device, fgl_ctx **out)
         pow3[k] = powf(1.f - occ, 3);
     }
     if (ensure(c, c->ssao_pow3, sizeof(pow3)) || ensure(c, c->rng, sizeof(MtState)) ||
-        ensure(c, c->rng_saved, sizeof(MtState)) ||
         hipMemcpy(c->ssao_pow3.p, pow3, sizeof(pow3), hipMemcpyHostToDevice) != hipSuccess ||
         rng_restart(c) != FGL_OK) {
         fgl_ctx_destroy(c);
@@ -334,7 +341,8 @@ void fgl_ctx_destroy(fgl_ctx *c)
     }
     DevBuf *bufs[] = {&c->d_textures, &c->d_materials, &c->depth, &c->fb, &c->gbuf, &c->ao,
                       &c->blur_tmp, &c->shadow, &c->shadow_depth, &c->img, &c->vis, &c->d_draws,
-                      &c->rng, &c->rng_saved, &c->rng_samples, &c->rng_offs, &c->ssao_pow3,
+                      &c->rng, &c->rng_samples, &c->rng_offs, &c->ssao_pow3,
+                      &c->mt_base, &c->mt_states, &c->mt_raw, &c->mt_pos, &c->mt_polys,
                       &c->screen, &c->rect, &c->counts, &c->offs, &c->scan_tmp, &c->keys,
                       &c->vals, &c->keys_alt, &c->vals_alt, &c->sort_tmp, &c->tile_start,
                       &c->tile_end, &c->tri_v, &c->big_list, &c->counters, &c->ndc_z,
@@ -784,6 +792,86 @@ static void fill_draw(fgl_ctx *c, const ModelRes &m, const float M[16], DevDraw 
     mat_mul(c->light_space, d.model, d.lm);
 }
 
+// The stream's next n vectors (RandomVectorInUnitDisk for D = 2,
+// RandomVectorInUnitSphere for D = 3) into out, in stream order, without
+// consuming them: rng_take then moves the stream past the first m <= n.  The
+// draws are generated in parallel chunks of kMtChunkBlocks blocks, each
+// chunk's array reached by a jump from the context's (rng.hip, mt_jump.hpp);
+// the draws needed are estimated from the acceptance rate (pi/4 for the
+// disk, pi/6 for the sphere) with a margin, and doubled in the rare case
+// that too few tries are kept.
+static constexpr int kMtChunkBlocks = 256;
+
+static int rng_fill(fgl_ctx *c, int D, float *out, uint32_t n)
+{
+    int rc;
+    c->fill_T = 0;
+    c->fill_n = 0;
+    c->fill_D = D;
+    if (n == 0) return FGL_OK;
+    const double keep = D == 2 ? 0.78539816339744831 : 0.52359877559829887;
+    const uint64_t C = 624ull * kMtChunkBlocks;
+    uint64_t R = (uint64_t)((double)n * D / keep * 1.01) + 4096;
+    for (;;) {
+        uint64_t Nc = (R + C - 1) / C;
+        int K = kMtChunkBlocks;
+        if (Nc == 1) K = (int)((R + 623) / 624);   // one chunk: only the blocks needed
+        const uint64_t len = Nc * (uint64_t)K * 624;
+        if (len + 624 >= 0xffffffffull || Nc > 0x7fffffffull)
+            return fail(c, FGL_ERR_CAPACITY, "random stream: too many draws in one request");
+        const uint32_t T = (uint32_t)(len / (uint64_t)D);
+        if ((rc = ensure(c, c->mt_raw, (len + 624) * 4)) ||
+            (rc = ensure(c, c->mt_pos, ((size_t)T + 1 + scan_tmp_words(T)) * 4)))
+            return rc;
+        MtState *st = (MtState *)c->rng.p;
+        uint32_t *raw = (uint32_t *)c->mt_raw.p, *pos = (uint32_t *)c->mt_pos.p;
+        const uint32_t *states = nullptr;
+        if (Nc > 1) {
+            const int nj = (int)Nc - 1;
+            if (nj > c->mt_polys_n) {
+                std::vector<uint64_t> polys;
+                if (!mt_jump_polys(C, nj, polys))
+                    return fail(c, FGL_ERR_DEVICE, "random stream: no characteristic polynomial");
+                if ((rc = ensure(c, c->mt_polys, polys.size() * sizeof(uint64_t)))) return rc;
+                HIPCHK(c, hipMemcpyAsync(c->mt_polys.p, polys.data(), polys.size() * sizeof(uint64_t),
+                                         hipMemcpyHostToDevice, c->stream));
+                HIPCHK(c, hipStreamSynchronize(c->stream));
+                c->mt_polys_n = nj;
+            }
+            if ((rc = ensure(c, c->mt_base, (size_t)kMtBaseWords * 4)) ||
+                (rc = ensure(c, c->mt_states, (size_t)Nc * 624 * 4)))
+                return rc;
+            launch_mt_base(st, (uint32_t *)c->mt_base.p, c->stream);
+            launch_mt_jump((const uint32_t *)c->mt_base.p, (const uint64_t *)c->mt_polys.p, nj,
+                           (uint32_t *)c->mt_states.p, c->stream);
+            states = (const uint32_t *)c->mt_states.p;
+        }
+        launch_mt_gen(st, states, (int)Nc, K, raw, c->stream);
+        launch_mt_flags(D, raw, T, pos, c->stream);
+        launch_exclusive_scan(pos, pos, T, pos + T + 1, c->stream);
+        HIPCHK(c, hipMemcpyAsync(c->h_total, pos + T, 4, hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        if (c->h_total[0] >= n) {
+            launch_mt_take(D, raw, T, pos, n, out, c->stream);
+            c->fill_T = T;
+            c->fill_n = n;
+            HIPCHK(c, hipGetLastError());
+            return FGL_OK;
+        }
+        R *= 2;
+    }
+}
+
+static int rng_take(fgl_ctx *c, uint32_t m)
+{
+    if (m == 0) return FGL_OK;
+    if (m > c->fill_n) return fail(c, FGL_ERR_INVALID, "random stream: taking more vectors than were drawn");
+    launch_mt_commit(c->fill_D, (MtState *)c->rng.p, (const uint32_t *)c->mt_raw.p, (const uint32_t *)c->mt_pos.p,
+                     c->fill_T, m, c->stream);
+    HIPCHK(c, hipGetLastError());
+    return FGL_OK;
+}
+
 // The stochastic shadow filters' unit-disk samples for n shading points in
 // the order they shade -- the lighting pass's pixels in row order, or the
 // forward pass's passing fragments in raster order -- drawn from the
@@ -801,24 +889,24 @@ static int stochastic_shadow_samples(fgl_ctx *c, const FrameUniforms &u, const f
     // never fewer than one point's worth, so every offset in [0, most) reads
     // inside the buffer
     if ((rc = ensure(c, c->rng_samples, (most > nb + np ? most : nb + np) * 2 * sizeof(float)))) return rc;
-    MtState *st = (MtState *)c->rng.p;
     float *pairs = (float *)c->rng_samples.p;
     if (mode == FGL_SHADOW_REF_PCF) {
-        launch_rng_accept(2, st, pairs, (uint32_t)most, c->stream);
+        if ((rc = rng_fill(c, 2, pairs, (uint32_t)most)) || (rc = rng_take(c, (uint32_t)most))) return rc;
         ss = {pairs, nullptr, np};
         return FGL_OK;
     }
     // PCSS takes n_pcf more samples only where its blocker search finds a
-    // blocker.  Points the shadow map decides alone are counted in parallel;
-    // the others are walked in order over the most samples the points can
-    // need; then the stream is restored and replayed to stop after exactly
-    // the samples used (the replay rewrites the same values).
-    const size_t words = scan_tmp_words(n + 1);
-    if ((rc = ensure(c, c->rng_offs, (5 * (n + 1) + words) * sizeof(uint32_t)))) return rc;
-    uint32_t *base = (uint32_t *)c->rng_offs.p, *upos = base + (n + 1), *list = upos + (n + 1);
-    uint32_t *found = list + (n + 1), *offs = found + (n + 1), *tmp = offs + (n + 1);
-    HIPCHK(c, hipMemcpyAsync(c->rng_saved.p, st, sizeof(MtState), hipMemcpyDeviceToDevice, c->stream));
-    launch_rng_accept(2, st, pairs, (uint32_t)most, c->stream);
+    // blocker.  The most samples the points can need are drawn without
+    // consuming them; points the shadow map decides alone are counted in
+    // parallel, the others walked in order; then the stream moves past
+    // exactly the samples used.
+    const size_t words = scan_tmp_words(n + 1), n4 = (n + 4) & ~(size_t)3;
+    if ((rc = ensure(c, c->rng_offs, (6 * n4 + 4 * n4 + words) * sizeof(uint32_t)))) return rc;
+    uint32_t *base = (uint32_t *)c->rng_offs.p, *upos = base + n4, *list = upos + n4;
+    uint32_t *found = list + n4, *offs = found + n4, *pbase = offs + n4;
+    float4 *pt = (float4 *)(pbase + n4);
+    uint32_t *tmp = (uint32_t *)(pt + n4);
+    if ((rc = rng_fill(c, 2, pairs, (uint32_t)most))) return rc;
     launch_pcss_classify(u, g, n, (const float *)c->shadow.p, base, upos, c->stream);
     launch_exclusive_scan(base, base, n, tmp, c->stream);
     launch_exclusive_scan(upos, upos, n, tmp, c->stream);
@@ -826,14 +914,13 @@ static int stochastic_shadow_samples(fgl_ctx *c, const FrameUniforms &u, const f
     HIPCHK(c, hipMemcpyAsync(c->h_total, upos + n, 4, hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
     const uint32_t n_unc = c->h_total[0];
-    launch_pcss_walk_list(u, g, n, (const float *)c->shadow.p, pairs, base, list, n_unc, found, c->stream);
+    launch_pcss_walk_list(u, g, n, (const float *)c->shadow.p, pairs, base, list, n_unc, pt, pbase, found,
+                          c->stream);
     launch_exclusive_scan(found, found, n_unc, tmp, c->stream);
     launch_pcss_offsets(base, upos, found, np, n, offs, c->stream);
     HIPCHK(c, hipMemcpyAsync(c->h_total, offs + n, 4, hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
-    const uint32_t used = c->h_total[0];
-    HIPCHK(c, hipMemcpyAsync(st, c->rng_saved.p, sizeof(MtState), hipMemcpyDeviceToDevice, c->stream));
-    launch_rng_accept(2, st, pairs, used, c->stream);
+    if ((rc = rng_take(c, c->h_total[0]))) return rc;
     ss = {pairs, offs, 0};
     return FGL_OK;
 }
@@ -1152,7 +1239,8 @@ int fgl_ssao(fgl_ctx *c, const fgl_ssao_cfg *cfg)
         if (n > 0xffffffffull) return fail(c, FGL_ERR_CAPACITY, "stochastic SSAO: too many samples");
         if ((rc = ensure(c, c->rng_samples, n * 3 * sizeof(float)))) return rc;
         KernelTimer t(c, "ssao_reference");
-        launch_rng_accept(3, (MtState *)c->rng.p, (float *)c->rng_samples.p, (uint32_t)n, c->stream);
+        if ((rc = rng_fill(c, 3, (float *)c->rng_samples.p, (uint32_t)n)) || (rc = rng_take(c, (uint32_t)n)))
+            return rc;
         launch_ssao_reference(u, (const float *)c->rng_samples.p, (const float *)c->ssao_pow3.p,
                               (const float *)c->depth.p, (const float *)c->gbuf.p, (float *)c->ao.p, c->stream);
     } else {
@@ -1340,7 +1428,7 @@ int fgl_rng_vectors(fgl_ctx *c, int kind, uint32_t n, float *out)
     if (n == 0) return FGL_OK;
     const size_t bytes = (size_t)n * kind * sizeof(float);
     if ((rc = ensure(c, c->rng_samples, bytes))) return rc;
-    launch_rng_accept(kind, (MtState *)c->rng.p, (float *)c->rng_samples.p, n, c->stream);
+    if ((rc = rng_fill(c, kind, (float *)c->rng_samples.p, n)) || (rc = rng_take(c, n))) return rc;
     HIPCHK(c, hipGetLastError());
     HIPCHK(c, hipStreamSynchronize(c->stream));
     HIPCHK(c, hipMemcpy(out, c->rng_samples.p, bytes, hipMemcpyDeviceToHost));

@@ -161,9 +161,25 @@ struct MtState {
     unsigned long long draws;
 };
 void mt_seed_state(uint32_t seed, MtState &st);
-// Rejection-sampled unit vectors in stream order: draws_per_try 3 gives
-// RandomVectorInUnitSphere (x, y, z), 2 RandomVectorInUnitDisk (x, y).
-void launch_rng_accept(int draws_per_try, MtState *st, float *out, uint32_t n_accept, hipStream_t s);
+// The stream's next draws in parallel chunks of K blocks (624 K draws each):
+// base = the context's array and the 32 blocks after it (kMtBaseWords
+// words); states[c] (c = 1..n_jumps, 624 words each) = chunk c's starting
+// array from jump polynomial c (mt_jump.hpp; kMtPolyWords words each);
+// raw = the tempered draws from the context's next one on,
+// (624 - idx) + n_chunks 624 K of them.
+constexpr int kMtBaseWords = 33 * 624;
+void launch_mt_base(const MtState *st, uint32_t *base, hipStream_t s);
+void launch_mt_jump(const uint32_t *base, const uint64_t *polys, int n_jumps, uint32_t *states, hipStream_t s);
+void launch_mt_gen(const MtState *st, const uint32_t *states, int n_chunks, int K, uint32_t *raw, hipStream_t s);
+// Rejection-sampled unit vectors from raw's T tries of D draws (D = 3:
+// RandomVectorInUnitSphere (x, y, z); 2: RandomVectorInUnitDisk (x, y)):
+// flag[t] = try t is kept; after an exclusive scan of the flags into pos,
+// the first n kept vectors go to out in stream order, and a commit moves the
+// stream past the try that kept the m-th.
+void launch_mt_flags(int D, const uint32_t *raw, uint32_t T, uint32_t *flag, hipStream_t s);
+void launch_mt_take(int D, const uint32_t *raw, uint32_t T, const uint32_t *pos, uint32_t n, float *out, hipStream_t s);
+void launch_mt_commit(int D, MtState *st, const uint32_t *raw, const uint32_t *pos, uint32_t T, uint32_t m,
+                      hipStream_t s);
 
 // shade.hip
 void launch_lighting(const FrameUniforms &u, const float *gbuf, const float *ao, int ao_is_one, const NetAo &net,
@@ -185,9 +201,10 @@ void launch_ssao_reference(const FrameUniforms &u, const float *samples, const f
 void launch_pcss_classify(const FrameUniforms &u, const float *g, size_t P, const float *shadow_map, uint32_t *cnt,
                           uint32_t *unc, hipStream_t s);
 void launch_pcss_compact(const uint32_t *upos, size_t P, uint32_t *list, hipStream_t s);
+// (pt, pbase: scratch of n entries for the undecided points' inputs)
 void launch_pcss_walk_list(const FrameUniforms &u, const float *g, size_t P, const float *shadow_map,
-                           const float *pairs, const uint32_t *base, const uint32_t *list, uint32_t n,
-                           uint32_t *found, hipStream_t s);
+                           const float *pairs, const uint32_t *base, const uint32_t *list, uint32_t n, float4 *pt,
+                           uint32_t *pbase, uint32_t *found, hipStream_t s);
 void launch_pcss_offsets(const uint32_t *base, const uint32_t *upos, const uint32_t *fpos, uint32_t n_pcf, size_t P,
                          uint32_t *offs, hipStream_t s);
 void launch_ssao(const FrameUniforms &u, const SsaoArgs &sa, const float *depth,

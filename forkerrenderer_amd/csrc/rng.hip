@@ -4,10 +4,11 @@
 // std::mt19937 (default seed 5489) through uniform_real_distribution<float>
 // (src/utility.h:90-103), and builds its random vectors by rejection
 // (src/geometry.h:947-990).  Its consumers (Render::DoSSAO, the PCF / PCSS
-// shadow filters) take the stream strictly in pixel order, so the device
-// replays it in two steps: one workgroup runs the generator and writes the
-// accepted vectors in stream order (k_mt_accept); the per-pixel kernels then
-// read the vectors of their pixel at a known offset.
+// shadow filters) take the stream strictly in pixel (or fragment) order, so
+// the device replays it in two steps: the generator's draws are produced in
+// parallel chunks reached by jumping ahead, and the accepted vectors written
+// in stream order; the per-pixel kernels then read the vectors of their
+// pixel at a known offset.
 //
 // Arithmetic, fixed by the C++ library the reference is built with:
 //  * MT19937: the standard twist and tempering on 624 words;
@@ -20,6 +21,7 @@
 //    takes z, y, x from three consecutive draws, a unit-disk try y, x from
 //    two; a try is kept when (x*x + y*y) + z*z < 1.
 #include "fgl_kernels.hpp"
+#include "mt_jump.hpp"
 
 namespace fgl {
 
@@ -83,106 +85,226 @@ __device__ __forceinline__ void mt_twist(uint32_t *mt)
     __syncthreads();
 }
 
-// One workgroup of 256 threads: draws tries of D draws (3: unit sphere,
-// 2: unit disk) until n_accept are kept, writes the kept vectors (D floats
-// each) to out in stream order, and leaves the stream right after the last
-// kept try -- where the reference's next draw would come from.
+__device__ __forceinline__ uint32_t mt_untemper(uint32_t y)
+{
+    y ^= y >> 18;
+    y ^= (y << 15) & 0xefc60000u;
+    uint32_t x = y;
+#pragma unroll
+    for (int i = 0; i < 5; ++i) x = y ^ ((x << 7) & 0x9d2c5680u);
+    return x ^ (x >> 11) ^ (x >> 22);
+}
+
+// One try of D draws at p: the vector (x, y, z) and whether it is kept.
 template <int D>
-__global__ __launch_bounds__(256) void k_mt_accept(MtState *__restrict__ st, float *__restrict__ out,
-                                                   uint32_t n_accept)
+__device__ __forceinline__ bool mt_try(const uint32_t *__restrict__ p, float &x, float &y, float &z)
+{
+    z = 0.f;
+    if (D == 3) {
+        z = random_pm1(p[0]);
+        y = random_pm1(p[1]);
+        x = random_pm1(p[2]);
+    } else {
+        y = random_pm1(p[0]);
+        x = random_pm1(p[1]);
+    }
+    return (x * x + y * y) + z * z < 1.f;
+}
+
+// ---- the stream in parallel: jump ahead, generate, keep ----
+//
+// A fill takes the stream's next draws in chunks of K blocks (624 K draws).
+// Chunk 0 continues from the context's array; chunk c > 0 starts from the
+// array c 624 K words further on, which the jump gives (mt_jump.hpp): the XOR
+// of the windows x_{B+i} .. x_{B+i+623} of the base sequence over the set
+// bits i of t^(c 624 K) mod phi, x_B .. being the context's array and the 32
+// blocks after it.  The tries are then flagged, scanned and kept in parallel,
+// and a commit moves the context's stream past the tries a caller used.
+constexpr int kMtBaseBlocks = kMtBaseWords / (int)kMtN;   // every window of a jump: x_{B + 19967 + 623} at most
+static_assert(kMtBaseWords >= 64 * kMtPolyWords + 623, "the jump's windows stay in the base sequence");
+
+// One workgroup: the context's array and the 32 blocks after it, untempered.
+__global__ __launch_bounds__(256) void k_mt_base(const MtState *__restrict__ st, uint32_t *__restrict__ base)
 {
     __shared__ uint32_t mt[kMtN];
-    __shared__ uint32_t buf[kMtN + 4];
-    __shared__ uint32_t wcnt[4];
-    __shared__ uint32_t s_last;
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    for (int i = tid; i < (int)kMtN; i += 256) mt[i] = st->mt[i];
-    uint32_t idx = st->idx;
-    unsigned long long draws = st->draws;
-    uint32_t carry = 0, accepted = 0;
-    __syncthreads();
-    while (accepted < n_accept) {
-        if (idx >= kMtN) {
-            mt_twist(mt);
-            idx = 0;
-        }
-        const uint32_t avail = kMtN - idx;
-        for (uint32_t i = tid; i < avail; i += 256) buf[carry + i] = mt_temper(mt[idx + i]);
-        __syncthreads();
-        const uint32_t total = carry + avail, ntries = total / D, need = n_accept - accepted;
-        uint32_t base = 0;
-        bool done = false;
-        for (uint32_t c0 = 0; c0 < ntries; c0 += 256) {
-            const uint32_t t = c0 + tid;
-            bool ok = false;
-            float x = 0.f, y = 0.f, z = 0.f;
-            if (t < ntries) {
-                if (D == 3) {
-                    z = random_pm1(buf[3 * t]);
-                    y = random_pm1(buf[3 * t + 1]);
-                    x = random_pm1(buf[3 * t + 2]);
-                } else {
-                    y = random_pm1(buf[2 * t]);
-                    x = random_pm1(buf[2 * t + 1]);
-                }
-                ok = (x * x + y * y) + z * z < 1.f;
-            }
-            const unsigned long long m = __ballot(ok);
-            if (lane == 0) wcnt[wave] = (uint32_t)__popcll(m);
-            __syncthreads();
-            uint32_t pre = 0, tot = 0;
-#pragma unroll
-            for (int w = 0; w < 4; ++w) {
-                const uint32_t cw = wcnt[w];
-                if (w < wave) pre += cw;
-                tot += cw;
-            }
-            const uint32_t rank = base + pre + (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
-            if (ok && rank < need) {
-                float *o = out + (size_t)(accepted + rank) * D;
-                o[0] = x;
-                o[1] = y;
-                if (D == 3) o[2] = z;
-                if (rank == need - 1) s_last = t;
-            }
-            __syncthreads();   // wcnt is rewritten by the next chunk; s_last is visible
-            if (base + tot >= need) {
-                done = true;
-                break;
-            }
-            base += tot;
-        }
-        if (done) {
-            const uint32_t used = (s_last + 1) * D - carry;   // draws of this block up to the last kept try
-            idx += used;
-            draws += used;
-            accepted = n_accept;
-            break;
-        }
-        accepted += base;
-        draws += avail;
-        idx = kMtN;
-        const uint32_t rem = total - ntries * D;   // a try started here ends in the next block
-        uint32_t keep = 0;
-        if ((uint32_t)tid < rem) keep = buf[ntries * D + tid];
-        __syncthreads();
-        if ((uint32_t)tid < rem) buf[tid] = keep;
-        carry = rem;
-        __syncthreads();
+    const int tid = threadIdx.x;
+    for (int i = tid; i < (int)kMtN; i += 256) {
+        const uint32_t v = st->mt[i];
+        mt[i] = v;
+        base[i] = v;
     }
     __syncthreads();
-    for (int i = tid; i < (int)kMtN; i += 256) st->mt[i] = mt[i];
-    if (tid == 0) {
-        st->idx = idx;
-        st->draws = draws;
+    for (int b = 1; b < kMtBaseBlocks; ++b) {
+        mt_twist(mt);   // ends with a barrier
+        for (int i = tid; i < (int)kMtN; i += 256) base[(size_t)b * kMtN + i] = mt[i];
     }
 }
 
-void launch_rng_accept(int draws_per_try, MtState *st, float *out, uint32_t n_accept, hipStream_t s)
+// Workgroup c - 1 jumps to chunk c's array: states[c][k] = XOR of base[i + k]
+// over the set bits i of polynomial c, the base sequence in LDS (82 KB), a
+// thread per word k; the set bits are found by the scalar unit (the
+// polynomial is the same for the whole workgroup) and their reads issued
+// eight at a time.
+__global__ __launch_bounds__(640) void k_mt_jump(const uint32_t *__restrict__ base, const uint64_t *__restrict__ polys,
+                                                 uint32_t *__restrict__ states)
 {
-    if (n_accept == 0) return;
-    if (draws_per_try == 3) hipLaunchKernelGGL(k_mt_accept<3>, dim3(1), dim3(256), 0, s, st, out, n_accept);
-    else hipLaunchKernelGGL(k_mt_accept<2>, dim3(1), dim3(256), 0, s, st, out, n_accept);
+    __shared__ uint32_t b[kMtBaseWords + 1];   // + a zero word for the padding reads
+    const int tid = threadIdx.x;
+    for (int i = tid; i < kMtBaseWords; i += 640) b[i] = base[i];
+    if (tid == 0) b[kMtBaseWords] = 0u;
+    __syncthreads();
+    const int c = blockIdx.x + 1;
+    if (tid >= (int)kMtN) return;
+    const uint64_t *g = polys + (size_t)(c - 1) * kMtPolyWords;
+    uint32_t acc = 0;
+    for (int q = 0; q < kMtPolyWords; ++q) {
+        uint64_t bits = g[q];
+        while (bits) {
+            int ix[8];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {
+                if (bits) {
+                    ix[u] = q * 64 + __builtin_ctzll(bits) + tid;
+                    bits &= bits - 1;
+                } else {
+                    ix[u] = kMtBaseWords;
+                }
+            }
+            uint32_t v[8];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) v[u] = b[ix[u]];
+            acc ^= ((v[0] ^ v[1]) ^ (v[2] ^ v[3])) ^ ((v[4] ^ v[5]) ^ (v[6] ^ v[7]));
+        }
+    }
+    states[(size_t)c * kMtN + tid] = acc;
+}
+
+// Workgroup c generates chunk c: K blocks of tempered draws at
+// raw[(624 - idx) + c 624 K ..]; chunk 0 first writes the rest of the
+// context's current block at raw[0 .. 624 - idx).
+__global__ __launch_bounds__(256) void k_mt_gen(const MtState *__restrict__ st, const uint32_t *__restrict__ states,
+                                                int K, uint32_t *__restrict__ raw)
+{
+    __shared__ uint32_t mt[kMtN];
+    const int tid = threadIdx.x, c = blockIdx.x;
+    const uint32_t idx = st->idx;
+    const uint32_t *src = c == 0 ? st->mt : states + (size_t)c * kMtN;
+    for (int i = tid; i < (int)kMtN; i += 256) mt[i] = src[i];
+    __syncthreads();
+    const size_t a = kMtN - idx;
+    if (c == 0)
+        for (uint32_t i = tid; i < a; i += 256) raw[i] = mt_temper(mt[idx + i]);
+    uint32_t *out = raw + a + (size_t)c * K * kMtN;
+    for (int b = 0; b < K; ++b) {
+        mt_twist(mt);   // starts by reading, and writes only after a barrier
+        for (int i = tid; i < (int)kMtN; i += 256) out[(size_t)b * kMtN + i] = mt_temper(mt[i]);
+    }
+}
+
+template <int D>
+__global__ __launch_bounds__(256) void k_mt_flags(const uint32_t *__restrict__ raw, uint32_t T,
+                                                  uint32_t *__restrict__ flag)
+{
+    const uint32_t t = blockIdx.x * 256 + threadIdx.x;
+    if (t >= T) return;
+    float x, y, z;
+    flag[t] = mt_try<D>(raw + (size_t)D * t, x, y, z) ? 1u : 0u;
+}
+
+// pos: exclusive scan of the flags (pos[T] = the kept tries); the first n
+// kept vectors to out in stream order.
+template <int D>
+__global__ __launch_bounds__(256) void k_mt_take(const uint32_t *__restrict__ raw, uint32_t T,
+                                                 const uint32_t *__restrict__ pos, uint32_t n, float *__restrict__ out)
+{
+    const uint32_t t = blockIdx.x * 256 + threadIdx.x;
+    if (t >= T) return;
+    const uint32_t p = pos[t];
+    if (pos[t + 1] == p || p >= n) return;
+    float x, y, z;
+    mt_try<D>(raw + (size_t)D * t, x, y, z);
+    float *o = out + (size_t)p * D;
+    o[0] = x;
+    o[1] = y;
+    if (D == 3) o[2] = z;
+}
+
+// The stream moves past the try that kept the m-th vector (m >= 1, at most
+// pos[T]): the draws up to it are consumed, and the array becomes the block
+// holding the next draw (its words untempered from raw), idx its position
+// there -- 624 when the next draw starts a new block, as the generator
+// itself leaves it.
+__global__ __launch_bounds__(256) void k_mt_commit(MtState *__restrict__ st, int D, const uint32_t *__restrict__ raw,
+                                                   const uint32_t *__restrict__ pos, uint32_t T, uint32_t m)
+{
+    __shared__ uint32_t s_t;
+    const int tid = threadIdx.x;
+    const uint32_t idx0 = st->idx;
+    if (tid == 0) {
+        uint32_t lo = 0, hi = T - 1;   // the smallest t with pos[t + 1] >= m
+        while (lo < hi) {
+            const uint32_t mid = lo + (hi - lo) / 2;
+            if (pos[mid + 1] >= m) hi = mid;
+            else lo = mid + 1;
+        }
+        s_t = lo;
+    }
+    __syncthreads();
+    const unsigned long long e = (unsigned long long)D * (s_t + 1ull);
+    unsigned long long blk = (idx0 + e) / kMtN;
+    uint32_t idx1 = (uint32_t)((idx0 + e) % kMtN);
+    if (idx1 == 0) {
+        blk -= 1;
+        idx1 = kMtN;
+    }
+    if (blk > 0) {
+        const uint32_t *src = raw + (kMtN - idx0) + (size_t)(blk - 1) * kMtN;
+        for (int i = tid; i < (int)kMtN; i += 256) st->mt[i] = mt_untemper(src[i]);
+    }
+    __syncthreads();   // every thread has read idx before it changes
+    if (tid == 0) {
+        st->idx = idx1;
+        st->draws += e;
+    }
+}
+
+static inline unsigned blocks256(size_t n) { return (unsigned)((n + 255) / 256); }
+
+void launch_mt_base(const MtState *st, uint32_t *base, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_mt_base, dim3(1), dim3(256), 0, s, st, base);
+}
+
+void launch_mt_jump(const uint32_t *base, const uint64_t *polys, int n_jumps, uint32_t *states, hipStream_t s)
+{
+    if (n_jumps <= 0) return;
+    hipLaunchKernelGGL(k_mt_jump, dim3(n_jumps), dim3(640), 0, s, base, polys, states);
+}
+
+void launch_mt_gen(const MtState *st, const uint32_t *states, int n_chunks, int K, uint32_t *raw, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_mt_gen, dim3(n_chunks), dim3(256), 0, s, st, states, K, raw);
+}
+
+void launch_mt_flags(int D, const uint32_t *raw, uint32_t T, uint32_t *flag, hipStream_t s)
+{
+    if (T == 0) return;
+    if (D == 3) hipLaunchKernelGGL(k_mt_flags<3>, dim3(blocks256(T)), dim3(256), 0, s, raw, T, flag);
+    else hipLaunchKernelGGL(k_mt_flags<2>, dim3(blocks256(T)), dim3(256), 0, s, raw, T, flag);
+}
+
+void launch_mt_take(int D, const uint32_t *raw, uint32_t T, const uint32_t *pos, uint32_t n, float *out, hipStream_t s)
+{
+    if (T == 0 || n == 0) return;
+    if (D == 3) hipLaunchKernelGGL(k_mt_take<3>, dim3(blocks256(T)), dim3(256), 0, s, raw, T, pos, n, out);
+    else hipLaunchKernelGGL(k_mt_take<2>, dim3(blocks256(T)), dim3(256), 0, s, raw, T, pos, n, out);
+}
+
+void launch_mt_commit(int D, MtState *st, const uint32_t *raw, const uint32_t *pos, uint32_t T, uint32_t m,
+                      hipStream_t s)
+{
+    if (T == 0 || m == 0) return;
+    hipLaunchKernelGGL(k_mt_commit, dim3(1), dim3(256), 0, s, st, D, raw, pos, T, m);
 }
 
 }  // namespace fgl

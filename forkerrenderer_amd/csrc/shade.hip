@@ -111,48 +111,100 @@ __global__ __launch_bounds__(256) void k_pcss_compact(const uint32_t *__restrict
     if (i < P && upos[i + 1] != upos[i]) list[upos[i]] = (uint32_t)i;
 }
 
-// The undecided pixels in row order: pixel list[k] draws from
-// base[list[k]] + n_pcf * (blockers found by the undecided pixels before it);
-// lanes evaluate its blocker samples in parallel and lane order sums them as
-// FindAverageBlockDepth does (shadow.cpp:65-91).  found[k] = 1 when it finds
-// a blocker.
-__global__ __launch_bounds__(64) void k_pcss_walk_list(FrameUniforms u, const float *__restrict__ g, size_t P,
-                                                       const float *__restrict__ smap,
-                                                       const float *__restrict__ pairs,
-                                                       const uint32_t *__restrict__ base,
-                                                       const uint32_t *__restrict__ list, uint32_t n,
-                                                       uint32_t *__restrict__ found)
+// The undecided points' inputs that do not depend on the stream: light 0's
+// shadow coordinate and bias, and the offset of their samples were no earlier
+// undecided point to find a blocker (base of the point).
+__global__ __launch_bounds__(256) void k_pcss_points(FrameUniforms u, const float *__restrict__ g, size_t P,
+                                                     const uint32_t *__restrict__ base,
+                                                     const uint32_t *__restrict__ list, uint32_t n,
+                                                     float4 *__restrict__ pt, uint32_t *__restrict__ pbase)
 {
-    const int lane = threadIdx.x;
+    const uint32_t k = blockIdx.x * 256 + threadIdx.x;
+    if (k >= n) return;
+    const size_t i = list[k];
+    Vec3 sc;
+    float bias;
+    pcss_pixel(u, g, P, i, sc, bias);
+    pt[k] = make_float4(sc.x, sc.y, sc.z, bias);
+    pbase[k] = base[i];
+}
+
+// The undecided points in order: point k draws from
+// pbase[k] + n_pcf * (blockers found by the undecided points before it), so
+// each outcome moves every later point's samples.  One workgroup walks them
+// in rounds of kWalkW points: thread 16 j + m evaluates point p + j as if m
+// of the j points before it (in this round) found no blocker -- the search
+// exactly as FindAverageBlockDepth runs it (shadow.cpp:65-91: the samples in
+// order, blocker depths summed in that order) -- and one thread then follows
+// the actual outcomes through the round's table until a point needs more
+// than kWalkM - 1 misses before it.  Every round settles at least its first
+// point.  found[k] = 1 when point k finds a blocker.
+constexpr uint32_t kWalkW = 64, kWalkM = 16;
+static_assert(kWalkW * kWalkM == 1024, "a thread per (point, miss count)");
+
+__global__ __launch_bounds__(1024) void k_pcss_walk_spec(FrameUniforms u, const float *__restrict__ smap,
+                                                         const float *__restrict__ pairs,
+                                                         const float4 *__restrict__ pt,
+                                                         const uint32_t *__restrict__ pbase, uint32_t n,
+                                                         uint32_t *__restrict__ found)
+{
+    __shared__ unsigned long long masks[kWalkW / 4];   // a wavefront's ballot: 4 points x 16 miss counts
+    __shared__ uint32_t s_p, s_F;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const DevShadowCfg &c = u.shadow;
-    uint32_t F = 0;
-    for (uint32_t k = 0; k < n; ++k) {
-        const size_t i = list[k];
-        const uint32_t o = base[i] + (uint32_t)c.n_pcf * F;
-        Vec3 sc;
-        float bias;
-        pcss_pixel(u, g, P, i, sc, bias);
-        float sv = 0.f;
-        int blk = 0;
-        if (lane < c.n_blocker) {
-            const float *d = pairs + 2 * ((size_t)o + lane);
-            const float su = sc.x + (float)((double)d[0] * c.blocker_filter);
-            const float sw = sc.y + (float)((double)d[1] * c.blocker_filter);
-            sv = sample_shadow(smap, u.SW, u.SH, su, sw);
-            blk = sc.z > sv + bias;
-        }
-        float blocker = 0.f, nb = 0.f;
-        for (int j = 0; j < c.n_blocker; ++j) {
-            const float sj = __shfl(sv, j);
-            if (__shfl(blk, j)) {
-                blocker += sj;
-                nb += 1.f;
+    if (tid == 0) {
+        s_p = 0;
+        s_F = 0;
+    }
+    __syncthreads();
+    for (;;) {
+        const uint32_t p = s_p, F = s_F;
+        if (p >= n) break;
+        const uint32_t j = (uint32_t)tid / kWalkM, m = (uint32_t)tid % kWalkM;
+        const uint32_t k = p + j;
+        bool f = false;
+        if (k < n && m <= j) {
+            const float4 q = pt[k];
+            const float2 *d = (const float2 *)pairs + (pbase[k] + (uint32_t)c.n_pcf * (F + j - m));
+            float blocker = 0.f, nb = 0.f;
+#pragma unroll 8
+            for (int s = 0; s < c.n_blocker; ++s) {
+                const float2 ds = d[s];
+                const float su = q.x + (float)((double)ds.x * c.blocker_filter);
+                const float sw = q.y + (float)((double)ds.y * c.blocker_filter);
+                const float sv = sample_shadow(smap, u.SW, u.SH, su, sw);
+                if (q.z > sv + q.w) {
+                    blocker += sv;
+                    nb += 1.f;
+                }
             }
+            const float dB = (nb < 1.f) ? 0.f : blocker / nb;
+            f = !((double)dB < 0.001);
         }
-        const float dB = (nb < 1.f) ? 0.f : blocker / nb;
-        const uint32_t f = ((double)dB < 0.001) ? 0u : 1u;
-        if (lane == 0) found[k] = f;
-        F += f;
+        const unsigned long long b = __ballot(f);
+        if (lane == 0) masks[wave] = b;
+        __syncthreads();
+        if (tid == 0) {
+            uint32_t r = 0, miss = 0;
+            bool more = true;
+#pragma unroll
+            for (uint32_t w = 0; w < kWalkW / 4; ++w) {
+                const unsigned long long mw = masks[w];
+#pragma unroll
+                for (uint32_t e = 0; e < 4; ++e) {
+                    if (more && (p + r >= n || miss >= kWalkM)) more = false;
+                    if (more) {
+                        const uint32_t bit = (uint32_t)(mw >> (e * kWalkM + miss)) & 1u;
+                        found[p + r] = bit;
+                        miss += 1u - bit;
+                        ++r;
+                    }
+                }
+            }
+            s_p = p + r;
+            s_F = F + (r - miss);
+        }
+        __syncthreads();
     }
 }
 
@@ -374,11 +426,12 @@ void launch_pcss_compact(const uint32_t *upos, size_t P, uint32_t *list, hipStre
 }
 
 void launch_pcss_walk_list(const FrameUniforms &u, const float *g, size_t P, const float *shadow_map,
-                           const float *pairs, const uint32_t *base, const uint32_t *list, uint32_t n,
-                           uint32_t *found, hipStream_t s)
+                           const float *pairs, const uint32_t *base, const uint32_t *list, uint32_t n, float4 *pt,
+                           uint32_t *pbase, uint32_t *found, hipStream_t s)
 {
     if (n == 0) return;
-    hipLaunchKernelGGL(k_pcss_walk_list, dim3(1), dim3(64), 0, s, u, g, P, shadow_map, pairs, base, list, n, found);
+    hipLaunchKernelGGL(k_pcss_points, dim3((n + 255) / 256), dim3(256), 0, s, u, g, P, base, list, n, pt, pbase);
+    hipLaunchKernelGGL(k_pcss_walk_spec, dim3(1), dim3(1024), 0, s, u, shadow_map, pairs, pt, pbase, n, found);
 }
 
 void launch_pcss_offsets(const uint32_t *base, const uint32_t *upos, const uint32_t *fpos, uint32_t n_pcf, size_t P,

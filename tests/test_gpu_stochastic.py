@@ -40,6 +40,27 @@ def test_device_stream_matches_the_oracle(ctx):
     ctx.rng_seed(5489)
 
 
+@pytest.mark.parametrize("seed", [5489, 77])
+def test_device_stream_in_parallel_chunks_matches_numpy(ctx, seed):
+    """Requests of more than one chunk (256 blocks, 159,744 draws) run as
+    parallel chunks, each reached by jumping ahead from the context's array:
+    vectors bit-exact with numpy's MT19937 through the reference's rejection
+    loops, draw counts equal, and the stream continues exactly from the
+    array the commit rebuilds (small requests before, between and after,
+    so the big ones start and end inside a block)."""
+    ctx.rng_seed(seed)
+    raw = mt_raw(26_000_000, seed=seed)
+    at = 0
+    for kind, n in ((3, 7), (2, 3_000_000), (2, 5), (3, 2_000_000), (2, 1), (3, 1)):
+        print(f"seed {seed}: kind {kind} n {n}", flush=True)
+        got = ctx.rng_vectors(kind, n)
+        want, used = (stream_sphere if kind == 3 else stream_disk)(raw[at:], n)
+        assert_bitexact(f"kind {kind} n {n}", got, want)
+        at += used
+        assert ctx.rng_draws() == at, (kind, n, ctx.rng_draws(), at)
+    ctx.rng_seed(5489)
+
+
 def _deferred_targets(ctx, sc):
     """Geometry pass of a scene through the context (the frame's matrices and
     lights set as fgl_render_scene sets them); the frame's stream starts at
