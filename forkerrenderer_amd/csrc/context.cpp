@@ -879,6 +879,8 @@ static int rng_take(fgl_ctx *c, uint32_t m)
 // the points' normal, world position and light NDC as planes of stride n
 // (read for PCSS only).  ss: point i's samples start at pair ss.offs[i]
 // (PCSS; offs[n] = the total) or at i * ss.stride (PCF).
+static constexpr uint32_t kSegmentMax = 32768;   // undecided PCSS points per walk segment, at most
+
 static int stochastic_shadow_samples(fgl_ctx *c, const FrameUniforms &u, const float *g, size_t n, ShadowSamples &ss)
 {
     int rc;
@@ -901,11 +903,15 @@ static int stochastic_shadow_samples(fgl_ctx *c, const FrameUniforms &u, const f
     // parallel, the others walked in order; then the stream moves past
     // exactly the samples used.
     const size_t words = scan_tmp_words(n + 1), n4 = (n + 4) & ~(size_t)3;
-    if ((rc = ensure(c, c->rng_offs, (6 * n4 + 4 * n4 + words) * sizeof(uint32_t)))) return rc;
+    const size_t seg_words = 4 * (size_t)kSegmentMax + 2 * ((size_t)kSegmentMax + 2) + 4;
+    if ((rc = ensure(c, c->rng_offs, (6 * n4 + 4 * n4 + seg_words + words) * sizeof(uint32_t)))) return rc;
     uint32_t *base = (uint32_t *)c->rng_offs.p, *upos = base + n4, *list = upos + n4;
     uint32_t *found = list + n4, *offs = found + n4, *pbase = offs + n4;
     float4 *pt = (float4 *)(pbase + n4);
-    uint32_t *tmp = (uint32_t *)(pt + n4);
+    unsigned long long *masks = (unsigned long long *)(pt + n4);
+    uint32_t *pred = (uint32_t *)(masks + 2 * kSegmentMax);
+    uint32_t *seg_out = pred + 2 * ((size_t)kSegmentMax + 2);
+    uint32_t *tmp = seg_out + 4;
     if ((rc = rng_fill(c, 2, pairs, (uint32_t)most))) return rc;
     launch_pcss_classify(u, g, n, (const float *)c->shadow.p, base, upos, c->stream);
     launch_exclusive_scan(base, base, n, tmp, c->stream);
@@ -914,8 +920,27 @@ static int stochastic_shadow_samples(fgl_ctx *c, const FrameUniforms &u, const f
     HIPCHK(c, hipMemcpyAsync(c->h_total, upos + n, 4, hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
     const uint32_t n_unc = c->h_total[0];
-    launch_pcss_walk_list(u, g, n, (const float *)c->shadow.p, pairs, base, list, n_unc, pt, pbase, found,
-                          c->stream);
+    // the undecided points in order, a segment at a time (shade.hip): the
+    // band of miss counts each point is evaluated at follows the miss rate
+    // seen so far
+    launch_pcss_points(u, g, n, base, list, n_unc, pt, pbase, c->stream);
+    uint32_t s0 = 0, M0 = 0, seg = kSegmentMax / 4;
+    float rate = 0.25f;
+    while (s0 < n_unc) {
+        const uint32_t L = n_unc - s0 < seg ? n_unc - s0 : seg;
+        launch_pcss_segment(u, (const float *)c->shadow.p, pairs, pt, pbase, s0, L, M0, rate, pred, tmp, masks, found,
+                            seg_out, c->stream);
+        HIPCHK(c, hipMemcpyAsync(c->h_total, seg_out, 8, hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        const uint32_t s1 = c->h_total[0], M1 = c->h_total[1];
+        if (s1 <= s0 || s1 > n_unc) return fail(c, FGL_ERR_DEVICE, "stochastic shadows: the PCSS walk did not advance");
+        if (s1 - s0 >= 64) rate = (float)(M1 - M0) / (float)(s1 - s0);
+        // the next segment: longer while whole segments settle, shorter when few of it did
+        if (s1 - s0 == L && seg < kSegmentMax) seg *= 2;
+        else if (4 * (s1 - s0) < L && seg > 1024) seg /= 2;
+        s0 = s1;
+        M0 = M1;
+    }
     launch_exclusive_scan(found, found, n_unc, tmp, c->stream);
     launch_pcss_offsets(base, upos, found, np, n, offs, c->stream);
     HIPCHK(c, hipMemcpyAsync(c->h_total, offs + n, 4, hipMemcpyDeviceToHost, c->stream));

@@ -195,16 +195,23 @@ void launch_ssao_reference(const FrameUniforms &u, const float *samples, const f
 // forward pass's fragment planes).  Classify every point as decided by the
 // shadow map alone or not (cnt: its pairs when decided, n_blocker otherwise;
 // unc: 1 when undecided), compact the undecided points after a scan of unc
-// (upos), walk them in order from the scanned cnt (base), then
+// (upos), walk them in order from the scanned cnt (base; launch_pcss_points,
+// launch_pcss_segment), then
 // offs[i] = base[i] + n_pcf * (blockers found by undecided points before i)
 // for i = 0..P.
 void launch_pcss_classify(const FrameUniforms &u, const float *g, size_t P, const float *shadow_map, uint32_t *cnt,
                           uint32_t *unc, hipStream_t s);
 void launch_pcss_compact(const uint32_t *upos, size_t P, uint32_t *list, hipStream_t s);
-// (pt, pbase: scratch of n entries for the undecided points' inputs)
-void launch_pcss_walk_list(const FrameUniforms &u, const float *g, size_t P, const float *shadow_map,
-                           const float *pairs, const uint32_t *base, const uint32_t *list, uint32_t n, float4 *pt,
-                           uint32_t *pbase, uint32_t *found, hipStream_t s);
+// The n undecided points' stream-independent inputs (shadow coordinate and
+// bias, sample base), then segments [s0, s0 + L) of the ordered walk from
+// M0 misses before s0 (rate: the miss rate so far): scratch pred (2 L + 2
+// words), scan_tmp (scan_tmp_words(L)), masks (2 L 64-bit words); out =
+// (first unsettled point, misses before it).
+void launch_pcss_points(const FrameUniforms &u, const float *g, size_t P, const uint32_t *base, const uint32_t *list,
+                        uint32_t n, float4 *pt, uint32_t *pbase, hipStream_t s);
+void launch_pcss_segment(const FrameUniforms &u, const float *shadow_map, const float *pairs, const float4 *pt,
+                         const uint32_t *pbase, uint32_t s0, uint32_t L, uint32_t M0, float rate, uint32_t *pred,
+                         uint32_t *scan_tmp, unsigned long long *masks, uint32_t *found, uint32_t *out, hipStream_t s);
 void launch_pcss_offsets(const uint32_t *base, const uint32_t *upos, const uint32_t *fpos, uint32_t n_pcf, size_t P,
                          uint32_t *offs, hipStream_t s);
 void launch_ssao(const FrameUniforms &u, const SsaoArgs &sa, const float *depth,

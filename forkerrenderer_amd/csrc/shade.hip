@@ -130,81 +130,148 @@ __global__ __launch_bounds__(256) void k_pcss_points(FrameUniforms u, const floa
 }
 
 // The undecided points in order: point k draws from
-// pbase[k] + n_pcf * (blockers found by the undecided points before it), so
-// each outcome moves every later point's samples.  One workgroup walks them
-// in rounds of kWalkW points: thread 16 j + m evaluates point p + j as if m
-// of the j points before it (in this round) found no blocker -- the search
-// exactly as FindAverageBlockDepth runs it (shadow.cpp:65-91: the samples in
-// order, blocker depths summed in that order) -- and one thread then follows
-// the actual outcomes through the round's table until a point needs more
-// than kWalkM - 1 misses before it.  Every round settles at least its first
-// point.  found[k] = 1 when point k finds a blocker.
-constexpr uint32_t kWalkW = 64, kWalkM = 16;
-static_assert(kWalkW * kWalkM == 1024, "a thread per (point, miss count)");
+// pbase[k] + n_pcf * (k - M_k), M_k the undecided points before it that find
+// no blocker, so each outcome moves every later point's samples.  They are
+// settled a segment [s0, s0 + L) at a time, M_{s0} = M0 known:
+//  * a prediction of M_k: every point evaluated once at the current
+//    prediction (k_pcss_guess), its misses scanned (twice, from a straight
+//    line at the miss rate so far) -- exact wherever the outcomes do not
+//    depend on the samples' offset, which is most points;
+//  * k_pcss_band evaluates every point at the 127 values of M_k around its
+//    prediction, a wavefront per 64 values, the outcomes as bit masks;
+//  * k_pcss_resolve follows the actual M_k through the masks in order,
+//    writes found[k] (1: a blocker) and stops where M_k leaves the band or
+//    the segment ends; the host starts the next segment there.
+// Every evaluation is the blocker search exactly as FindAverageBlockDepth
+// runs it (shadow.cpp:65-91: the samples in order, blocker depths summed in
+// that order).  Every segment settles its first point (its prediction is
+// exact).
+constexpr int kBandHalf = 63;   // M_k in [prediction - 63, prediction + 63]
 
-__global__ __launch_bounds__(1024) void k_pcss_walk_spec(FrameUniforms u, const float *__restrict__ smap,
-                                                         const float *__restrict__ pairs,
-                                                         const float4 *__restrict__ pt,
-                                                         const uint32_t *__restrict__ pbase, uint32_t n,
-                                                         uint32_t *__restrict__ found)
+__device__ __forceinline__ bool pcss_search(const FrameUniforms &u, const float *__restrict__ smap,
+                                            const float *__restrict__ pairs, float4 q, uint32_t o)
 {
-    __shared__ unsigned long long masks[kWalkW / 4];   // a wavefront's ballot: 4 points x 16 miss counts
-    __shared__ uint32_t s_p, s_F;
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const DevShadowCfg &c = u.shadow;
-    if (tid == 0) {
-        s_p = 0;
-        s_F = 0;
+    const float2 *d = (const float2 *)pairs + o;
+    float blocker = 0.f, nb = 0.f;
+    for (int s = 0; s < c.n_blocker; ++s) {
+        const float2 ds = d[s];
+        const float su = q.x + (float)((double)ds.x * c.blocker_filter);
+        const float sw = q.y + (float)((double)ds.y * c.blocker_filter);
+        const float sv = sample_shadow(smap, u.SW, u.SH, su, sw);
+        if (q.z > sv + q.w) {
+            blocker += sv;
+            nb += 1.f;
+        }
     }
-    __syncthreads();
-    for (;;) {
-        const uint32_t p = s_p, F = s_F;
-        if (p >= n) break;
-        const uint32_t j = (uint32_t)tid / kWalkM, m = (uint32_t)tid % kWalkM;
-        const uint32_t k = p + j;
-        bool f = false;
-        if (k < n && m <= j) {
-            const float4 q = pt[k];
-            const float2 *d = (const float2 *)pairs + (pbase[k] + (uint32_t)c.n_pcf * (F + j - m));
-            float blocker = 0.f, nb = 0.f;
-#pragma unroll 8
-            for (int s = 0; s < c.n_blocker; ++s) {
-                const float2 ds = d[s];
-                const float su = q.x + (float)((double)ds.x * c.blocker_filter);
-                const float sw = q.y + (float)((double)ds.y * c.blocker_filter);
-                const float sv = sample_shadow(smap, u.SW, u.SH, su, sw);
-                if (q.z > sv + q.w) {
-                    blocker += sv;
-                    nb += 1.f;
-                }
-            }
-            const float dB = (nb < 1.f) ? 0.f : blocker / nb;
-            f = !((double)dB < 0.001);
+    const float dB = (nb < 1.f) ? 0.f : blocker / nb;
+    return !((double)dB < 0.001);
+}
+
+// pred == nullptr: the prediction M0 + round(i * rate); else M0 + pred[i]
+// (the scanned misses of the previous guess).  Clamped to the possible
+// [M0, M0 + i].
+__device__ __forceinline__ uint32_t pcss_predict(const uint32_t *__restrict__ pred, uint32_t M0, float rate,
+                                                 uint32_t i)
+{
+    const uint32_t e = pred ? pred[i] : (uint32_t)((float)i * rate + 0.5f);
+    return M0 + (e < i ? e : i);
+}
+
+__global__ __launch_bounds__(256) void k_pcss_guess(FrameUniforms u, const float *__restrict__ smap,
+                                                    const float *__restrict__ pairs, const float4 *__restrict__ pt,
+                                                    const uint32_t *__restrict__ pbase, uint32_t s0, uint32_t L,
+                                                    uint32_t M0, float rate, const uint32_t *__restrict__ pred,
+                                                    uint32_t *__restrict__ miss)
+{
+    const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= L) return;
+    const uint32_t k = s0 + i, M = pcss_predict(pred, M0, rate, i);
+    miss[i] = pcss_search(u, smap, pairs, pt[k], pbase[k] + (uint32_t)u.shadow.n_pcf * (k - M)) ? 0u : 1u;
+}
+
+__global__ __launch_bounds__(256) void k_pcss_band(FrameUniforms u, const float *__restrict__ smap,
+                                                   const float *__restrict__ pairs, const float4 *__restrict__ pt,
+                                                   const uint32_t *__restrict__ pbase, uint32_t s0, uint32_t L,
+                                                   uint32_t M0, const uint32_t *__restrict__ pred,
+                                                   unsigned long long *__restrict__ masks)
+{
+    const uint32_t gw = (blockIdx.x * 256 + threadIdx.x) >> 6, lane = threadIdx.x & 63;
+    const uint32_t i = gw >> 1, h = gw & 1;
+    if (i >= L) return;   // whole wavefronts
+    const int cand = (int)(h * 64 + lane);
+    const long long M = (long long)pcss_predict(pred, M0, 0.f, i) - kBandHalf + cand;
+    bool f = false;
+    if (cand <= 2 * kBandHalf && M >= (long long)M0 && M <= (long long)M0 + i) {
+        const uint32_t k = s0 + i;
+        f = pcss_search(u, smap, pairs, pt[k], pbase[k] + (uint32_t)u.shadow.n_pcf * (k - (uint32_t)M));
+    }
+    const unsigned long long b = __ballot(f);
+    if (lane == 0) masks[2 * (size_t)i + h] = b;
+}
+
+// One wavefront, lane l on point base + l of the current group of 64.  With
+// P_i the prediction (misses before point i in the segment, pred[i]) and
+// D_i = 63 + M_i - M0 - P_i the actual M_i's place in point i's band,
+// D_{i+1} = D_i + (miss_i(D_i) - (P_{i+1} - P_i)): D stays put while the
+// prediction holds.  So the lanes from the first unsettled one on all test
+// the same D; a ballot finds the first lane whose outcome breaks the
+// prediction, the lanes up to it are settled, and D moves by one from
+// there.  A group takes one step per broken prediction, and the next
+// group's masks are loaded while this one is walked.  out = (first
+// unsettled point, M there).
+__global__ __launch_bounds__(64) void k_pcss_resolve(const unsigned long long *__restrict__ masks,
+                                                     const uint32_t *__restrict__ pred, uint32_t s0, uint32_t L,
+                                                     uint32_t M0, uint32_t *__restrict__ found,
+                                                     uint32_t *__restrict__ out)
+{
+    const uint32_t lane = threadIdx.x;
+    int D = kBandHalf;
+    uint32_t base = 0;
+    bool stop = false;
+    auto load = [&](uint32_t b, unsigned long long &m0, unsigned long long &m1, int &dl) {
+        const uint32_t i = b + lane;
+        m0 = m1 = 0;
+        dl = 0;
+        if (i < L) {
+            m0 = masks[2 * (size_t)i];
+            m1 = masks[2 * (size_t)i + 1];
+            dl = (int)(pred[i + 1] - pred[i]);   // pred[L]: the segment's predicted total
         }
-        const unsigned long long b = __ballot(f);
-        if (lane == 0) masks[wave] = b;
-        __syncthreads();
-        if (tid == 0) {
-            uint32_t r = 0, miss = 0;
-            bool more = true;
-#pragma unroll
-            for (uint32_t w = 0; w < kWalkW / 4; ++w) {
-                const unsigned long long mw = masks[w];
-#pragma unroll
-                for (uint32_t e = 0; e < 4; ++e) {
-                    if (more && (p + r >= n || miss >= kWalkM)) more = false;
-                    if (more) {
-                        const uint32_t bit = (uint32_t)(mw >> (e * kWalkM + miss)) & 1u;
-                        found[p + r] = bit;
-                        miss += 1u - bit;
-                        ++r;
-                    }
-                }
+    };
+    unsigned long long m0, m1;
+    int dl;
+    load(0, m0, m1, dl);
+    while (base < L && !stop) {
+        const uint32_t cnt = L - base < 64 ? L - base : 64;
+        unsigned long long n0, n1;
+        int ndl;
+        load(base + 64, n0, n1, ndl);
+        uint32_t start = 0;
+        unsigned long long fb = 0;
+        while (start < cnt) {
+            if (D < 0 || D > 2 * kBandHalf) {
+                stop = true;
+                break;
             }
-            s_p = p + r;
-            s_F = F + (r - miss);
+            const uint32_t bit = (uint32_t)((D < 64 ? m0 : m1) >> (D & 63)) & 1u;
+            const int dev = (int)(1u - bit) - dl;
+            const unsigned long long brk = __ballot(lane >= start && lane < cnt && dev != 0);
+            const uint32_t j = brk ? (uint32_t)__builtin_ctzll(brk) : cnt - 1;
+            const unsigned long long span = (j == 63 ? ~0ull : ((1ull << (j + 1)) - 1ull)) & ~((1ull << start) - 1ull);
+            fb |= __ballot(bit) & span;
+            if (brk) D += __builtin_amdgcn_readlane(dev, j);
+            start = j + 1;
         }
-        __syncthreads();
+        if (lane < start) found[s0 + base + lane] = (uint32_t)(fb >> lane) & 1u;
+        base += start;
+        m0 = n0;
+        m1 = n1;
+        dl = ndl;
+    }
+    if (lane == 0) {
+        out[0] = s0 + base;
+        out[1] = (uint32_t)((int)M0 + (int)pred[base] + D - kBandHalf);
     }
 }
 
@@ -425,13 +492,30 @@ void launch_pcss_compact(const uint32_t *upos, size_t P, uint32_t *list, hipStre
     hipLaunchKernelGGL(k_pcss_compact, dim3((unsigned)((P + 255) / 256)), dim3(256), 0, s, upos, P, list);
 }
 
-void launch_pcss_walk_list(const FrameUniforms &u, const float *g, size_t P, const float *shadow_map,
-                           const float *pairs, const uint32_t *base, const uint32_t *list, uint32_t n, float4 *pt,
-                           uint32_t *pbase, uint32_t *found, hipStream_t s)
+void launch_pcss_points(const FrameUniforms &u, const float *g, size_t P, const uint32_t *base, const uint32_t *list,
+                        uint32_t n, float4 *pt, uint32_t *pbase, hipStream_t s)
 {
     if (n == 0) return;
     hipLaunchKernelGGL(k_pcss_points, dim3((n + 255) / 256), dim3(256), 0, s, u, g, P, base, list, n, pt, pbase);
-    hipLaunchKernelGGL(k_pcss_walk_spec, dim3(1), dim3(1024), 0, s, u, shadow_map, pairs, pt, pbase, n, found);
+}
+
+void launch_pcss_segment(const FrameUniforms &u, const float *shadow_map, const float *pairs, const float4 *pt,
+                         const uint32_t *pbase, uint32_t s0, uint32_t L, uint32_t M0, float rate, uint32_t *pred,
+                         uint32_t *scan_tmp, unsigned long long *masks, uint32_t *found, uint32_t *out, hipStream_t s)
+{
+    if (L == 0) return;
+    // two guesses: from the straight line, then from the first guess's misses
+    hipLaunchKernelGGL(k_pcss_guess, dim3((L + 255) / 256), dim3(256), 0, s, u, shadow_map, pairs, pt, pbase, s0, L, M0,
+                       rate, (const uint32_t *)nullptr, pred);
+    launch_exclusive_scan(pred, pred, L, scan_tmp, s);
+    uint32_t *pred2 = pred + (L + 1);
+    hipLaunchKernelGGL(k_pcss_guess, dim3((L + 255) / 256), dim3(256), 0, s, u, shadow_map, pairs, pt, pbase, s0, L, M0,
+                       rate, (const uint32_t *)pred, pred2);
+    launch_exclusive_scan(pred2, pred2, L, scan_tmp, s);
+    hipLaunchKernelGGL(k_pcss_band, dim3((2 * L + 3) / 4), dim3(256), 0, s, u, shadow_map, pairs, pt, pbase, s0, L,
+                       M0, (const uint32_t *)pred2, masks);
+    hipLaunchKernelGGL(k_pcss_resolve, dim3(1), dim3(64), 0, s, masks, (const uint32_t *)pred2, s0, L, M0, found,
+                       out);
 }
 
 void launch_pcss_offsets(const uint32_t *base, const uint32_t *upos, const uint32_t *fpos, uint32_t n_pcf, size_t P,
