@@ -142,40 +142,72 @@ __global__ __launch_bounds__(256) void k_mt_base(const MtState *__restrict__ st,
 
 // Workgroup c - 1 jumps to chunk c's array: states[c][k] = XOR of base[i + k]
 // over the set bits i of polynomial c, the base sequence in LDS (82 KB), a
-// thread per word k; the set bits are found by the scalar unit (the
-// polynomial is the same for the whole workgroup) and their reads issued
-// eight at a time.
+// thread per word k.  The set bits are first listed in LDS (16-bit
+// positions, in order: per-word counts scanned by one wavefront), so every
+// wavefront reads eight positions with one broadcast load and spends no
+// scalar work per bit.
 __global__ __launch_bounds__(640) void k_mt_jump(const uint32_t *__restrict__ base, const uint64_t *__restrict__ polys,
                                                  uint32_t *__restrict__ states)
 {
-    __shared__ uint32_t b[kMtBaseWords + 1];   // + a zero word for the padding reads
-    const int tid = threadIdx.x;
-    for (int i = tid; i < kMtBaseWords; i += 640) b[i] = base[i];
-    if (tid == 0) b[kMtBaseWords] = 0u;
-    __syncthreads();
+    __shared__ uint32_t b[kMtBaseWords];
+    __shared__ __align__(16) uint16_t ix[kMtPolyWords * 64];
+    __shared__ uint32_t off[kMtPolyWords + 1];
+    const int tid = threadIdx.x, lane = tid & 63;
     const int c = blockIdx.x + 1;
-    if (tid >= (int)kMtN) return;
     const uint64_t *g = polys + (size_t)(c - 1) * kMtPolyWords;
-    uint32_t acc = 0;
-    for (int q = 0; q < kMtPolyWords; ++q) {
-        uint64_t bits = g[q];
+    for (int i = tid; i < kMtBaseWords; i += 640) b[i] = base[i];
+    const uint64_t w = tid < kMtPolyWords ? g[tid] : 0ull;
+    if (tid < kMtPolyWords) off[tid] = (uint32_t)__popcll(w);
+    __syncthreads();
+    if (tid < 64) {
+        // exclusive scan of the 312 counts: 5 words a lane, then across lanes
+        constexpr int kPer = (kMtPolyWords + 63) / 64;
+        uint32_t v[kPer], sum = 0;
+#pragma unroll
+        for (int e = 0; e < kPer; ++e) {
+            const int q = lane * kPer + e;
+            v[e] = q < kMtPolyWords ? off[q] : 0u;
+            sum += v[e];
+        }
+        uint32_t incl = sum;
+#pragma unroll
+        for (int d = 1; d < 64; d <<= 1) {
+            const uint32_t o = __shfl_up(incl, d);
+            if (lane >= d) incl += o;
+        }
+        uint32_t run = incl - sum;
+#pragma unroll
+        for (int e = 0; e < kPer; ++e) {
+            const int q = lane * kPer + e;
+            if (q < kMtPolyWords) off[q] = run;
+            run += v[e];
+        }
+        if (lane == 63) off[kMtPolyWords] = incl;
+    }
+    __syncthreads();
+    if (tid < kMtPolyWords) {
+        uint32_t pos = off[tid];
+        uint64_t bits = w;
         while (bits) {
-            int ix[8];
-#pragma unroll
-            for (int u = 0; u < 8; ++u) {
-                if (bits) {
-                    ix[u] = q * 64 + __builtin_ctzll(bits) + tid;
-                    bits &= bits - 1;
-                } else {
-                    ix[u] = kMtBaseWords;
-                }
-            }
-            uint32_t v[8];
-#pragma unroll
-            for (int u = 0; u < 8; ++u) v[u] = b[ix[u]];
-            acc ^= ((v[0] ^ v[1]) ^ (v[2] ^ v[3])) ^ ((v[4] ^ v[5]) ^ (v[6] ^ v[7]));
+            ix[pos++] = (uint16_t)(tid * 64 + __builtin_ctzll(bits));
+            bits &= bits - 1;
         }
     }
+    __syncthreads();
+    if (tid >= (int)kMtN) return;
+    const uint32_t T = off[kMtPolyWords];
+    const uint4 *ix8 = (const uint4 *)ix;
+    uint32_t acc = 0;
+    uint32_t r = 0;
+    for (; r + 8 <= T; r += 8) {
+        const uint4 v = ix8[r / 8];
+        const uint32_t a0 = b[(v.x & 0xffffu) + tid], a1 = b[(v.x >> 16) + tid];
+        const uint32_t a2 = b[(v.y & 0xffffu) + tid], a3 = b[(v.y >> 16) + tid];
+        const uint32_t a4 = b[(v.z & 0xffffu) + tid], a5 = b[(v.z >> 16) + tid];
+        const uint32_t a6 = b[(v.w & 0xffffu) + tid], a7 = b[(v.w >> 16) + tid];
+        acc ^= ((a0 ^ a1) ^ (a2 ^ a3)) ^ ((a4 ^ a5) ^ (a6 ^ a7));
+    }
+    for (; r < T; ++r) acc ^= b[ix[r] + tid];
     states[(size_t)c * kMtN + tid] = acc;
 }
 
