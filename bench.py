@@ -440,7 +440,15 @@ def main():
     ap.add_argument("--nao-heldout", type=int, default=200, help="held-out frames the neural-AO leg evaluates")
     ap.add_argument("--copy-calibrate", type=int, default=0, metavar="MIB",
                     help="only run a plain copy of MIB MiB (calibrates HBM byte counters) and exit")
+    ap.add_argument("--readme-only", action="store_true",
+                    help="only render the README's Diablo frame (both shadow variants) and print their figures")
     a = ap.parse_args()
+    if a.readme_only:
+        out = {}
+        for stream, key in ((True, "readme_scene"), (False, "readme_scene_fixed_disk")):
+            out[key] = run_readme_scene(a.steps, a.warmup, stream)[1]
+        print(json.dumps(out))
+        return
     if a.copy_calibrate:
         import forkerrenderer_amd as F
         ctx = F.Context(0)

@@ -903,13 +903,13 @@ static int stochastic_shadow_samples(fgl_ctx *c, const FrameUniforms &u, const f
     // parallel, the others walked in order; then the stream moves past
     // exactly the samples used.
     const size_t words = scan_tmp_words(n + 1), n4 = (n + 4) & ~(size_t)3;
-    const size_t seg_words = 4 * (size_t)kSegmentMax + 2 * ((size_t)kSegmentMax + 2) + 4;
+    const size_t seg_words = 2 * (size_t)kSegmentMax + 2 * ((size_t)kSegmentMax + 2) + 4;
     if ((rc = ensure(c, c->rng_offs, (6 * n4 + 4 * n4 + seg_words + words) * sizeof(uint32_t)))) return rc;
     uint32_t *base = (uint32_t *)c->rng_offs.p, *upos = base + n4, *list = upos + n4;
     uint32_t *found = list + n4, *offs = found + n4, *pbase = offs + n4;
     float4 *pt = (float4 *)(pbase + n4);
     unsigned long long *masks = (unsigned long long *)(pt + n4);
-    uint32_t *pred = (uint32_t *)(masks + 2 * kSegmentMax);
+    uint32_t *pred = (uint32_t *)(masks + kSegmentMax);
     uint32_t *seg_out = pred + 2 * ((size_t)kSegmentMax + 2);
     uint32_t *tmp = seg_out + 4;
     if ((rc = rng_fill(c, 2, pairs, (uint32_t)most))) return rc;

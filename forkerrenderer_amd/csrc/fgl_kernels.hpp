@@ -205,7 +205,7 @@ void launch_pcss_compact(const uint32_t *upos, size_t P, uint32_t *list, hipStre
 // The n undecided points' stream-independent inputs (shadow coordinate and
 // bias, sample base), then segments [s0, s0 + L) of the ordered walk from
 // M0 misses before s0 (rate: the miss rate so far): scratch pred (2 L + 2
-// words), scan_tmp (scan_tmp_words(L)), masks (2 L 64-bit words); out =
+// words), scan_tmp (scan_tmp_words(L)), masks (L 64-bit words); out =
 // (first unsettled point, misses before it).
 void launch_pcss_points(const FrameUniforms &u, const float *g, size_t P, const uint32_t *base, const uint32_t *list,
                         uint32_t n, float4 *pt, uint32_t *pbase, hipStream_t s);

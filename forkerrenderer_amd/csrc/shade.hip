@@ -137,8 +137,8 @@ __global__ __launch_bounds__(256) void k_pcss_points(FrameUniforms u, const floa
 //    prediction (k_pcss_guess), its misses scanned (twice, from a straight
 //    line at the miss rate so far) -- exact wherever the outcomes do not
 //    depend on the samples' offset, which is most points;
-//  * k_pcss_band evaluates every point at the 127 values of M_k around its
-//    prediction, a wavefront per 64 values, the outcomes as bit masks;
+//  * k_pcss_band evaluates every point at the 63 values of M_k around its
+//    prediction, a wavefront per point, the outcomes as a bit mask;
 //  * k_pcss_resolve follows the actual M_k through the masks in order,
 //    writes found[k] (1: a blocker) and stops where M_k leaves the band or
 //    the segment ends; the host starts the next segment there.
@@ -146,7 +146,7 @@ __global__ __launch_bounds__(256) void k_pcss_points(FrameUniforms u, const floa
 // runs it (shadow.cpp:65-91: the samples in order, blocker depths summed in
 // that order).  Every segment settles its first point (its prediction is
 // exact).
-constexpr int kBandHalf = 63;   // M_k in [prediction - 63, prediction + 63]
+constexpr int kBandHalf = 31;   // M_k in [prediction - 31, prediction + 31]: a wavefront per point
 
 __device__ __forceinline__ bool pcss_search(const FrameUniforms &u, const float *__restrict__ smap,
                                             const float *__restrict__ pairs, float4 q, uint32_t o)
@@ -196,10 +196,9 @@ __global__ __launch_bounds__(256) void k_pcss_band(FrameUniforms u, const float 
                                                    uint32_t M0, const uint32_t *__restrict__ pred,
                                                    unsigned long long *__restrict__ masks)
 {
-    const uint32_t gw = (blockIdx.x * 256 + threadIdx.x) >> 6, lane = threadIdx.x & 63;
-    const uint32_t i = gw >> 1, h = gw & 1;
+    const uint32_t i = (blockIdx.x * 256 + threadIdx.x) >> 6, lane = threadIdx.x & 63;
     if (i >= L) return;   // whole wavefronts
-    const int cand = (int)(h * 64 + lane);
+    const int cand = (int)lane;
     const long long M = (long long)pcss_predict(pred, M0, 0.f, i) - kBandHalf + cand;
     bool f = false;
     if (cand <= 2 * kBandHalf && M >= (long long)M0 && M <= (long long)M0 + i) {
@@ -207,7 +206,7 @@ __global__ __launch_bounds__(256) void k_pcss_band(FrameUniforms u, const float 
         f = pcss_search(u, smap, pairs, pt[k], pbase[k] + (uint32_t)u.shadow.n_pcf * (k - (uint32_t)M));
     }
     const unsigned long long b = __ballot(f);
-    if (lane == 0) masks[2 * (size_t)i + h] = b;
+    if (lane == 0) masks[i] = b;
 }
 
 // One wavefront, lane l on point base + l of the current group of 64.  With
@@ -229,24 +228,23 @@ __global__ __launch_bounds__(64) void k_pcss_resolve(const unsigned long long *_
     int D = kBandHalf;
     uint32_t base = 0;
     bool stop = false;
-    auto load = [&](uint32_t b, unsigned long long &m0, unsigned long long &m1, int &dl) {
+    auto load = [&](uint32_t b, unsigned long long &m, int &dl) {
         const uint32_t i = b + lane;
-        m0 = m1 = 0;
+        m = 0;
         dl = 0;
         if (i < L) {
-            m0 = masks[2 * (size_t)i];
-            m1 = masks[2 * (size_t)i + 1];
+            m = masks[i];
             dl = (int)(pred[i + 1] - pred[i]);   // pred[L]: the segment's predicted total
         }
     };
-    unsigned long long m0, m1;
+    unsigned long long m;
     int dl;
-    load(0, m0, m1, dl);
+    load(0, m, dl);
     while (base < L && !stop) {
         const uint32_t cnt = L - base < 64 ? L - base : 64;
-        unsigned long long n0, n1;
+        unsigned long long nm;
         int ndl;
-        load(base + 64, n0, n1, ndl);
+        load(base + 64, nm, ndl);
         uint32_t start = 0;
         unsigned long long fb = 0;
         while (start < cnt) {
@@ -254,7 +252,7 @@ __global__ __launch_bounds__(64) void k_pcss_resolve(const unsigned long long *_
                 stop = true;
                 break;
             }
-            const uint32_t bit = (uint32_t)((D < 64 ? m0 : m1) >> (D & 63)) & 1u;
+            const uint32_t bit = (uint32_t)(m >> D) & 1u;
             const int dev = (int)(1u - bit) - dl;
             const unsigned long long brk = __ballot(lane >= start && lane < cnt && dev != 0);
             const uint32_t j = brk ? (uint32_t)__builtin_ctzll(brk) : cnt - 1;
@@ -265,8 +263,7 @@ __global__ __launch_bounds__(64) void k_pcss_resolve(const unsigned long long *_
         }
         if (lane < start) found[s0 + base + lane] = (uint32_t)(fb >> lane) & 1u;
         base += start;
-        m0 = n0;
-        m1 = n1;
+        m = nm;
         dl = ndl;
     }
     if (lane == 0) {
@@ -512,8 +509,8 @@ void launch_pcss_segment(const FrameUniforms &u, const float *shadow_map, const 
     hipLaunchKernelGGL(k_pcss_guess, dim3((L + 255) / 256), dim3(256), 0, s, u, shadow_map, pairs, pt, pbase, s0, L, M0,
                        rate, (const uint32_t *)pred, pred2);
     launch_exclusive_scan(pred2, pred2, L, scan_tmp, s);
-    hipLaunchKernelGGL(k_pcss_band, dim3((2 * L + 3) / 4), dim3(256), 0, s, u, shadow_map, pairs, pt, pbase, s0, L,
-                       M0, (const uint32_t *)pred2, masks);
+    hipLaunchKernelGGL(k_pcss_band, dim3((L + 3) / 4), dim3(256), 0, s, u, shadow_map, pairs, pt, pbase, s0, L, M0,
+                       (const uint32_t *)pred2, masks);
     hipLaunchKernelGGL(k_pcss_resolve, dim3(1), dim3(64), 0, s, masks, (const uint32_t *)pred2, s0, L, M0, found,
                        out);
 }
