@@ -134,9 +134,9 @@ __global__ __launch_bounds__(256) void k_pcss_points(FrameUniforms u, const floa
 // no blocker, so each outcome moves every later point's samples.  They are
 // settled a segment [s0, s0 + L) at a time, M_{s0} = M0 known:
 //  * a prediction of M_k: every point evaluated once at the current
-//    prediction (k_pcss_guess), its misses scanned (twice, from a straight
-//    line at the miss rate so far) -- exact wherever the outcomes do not
-//    depend on the samples' offset, which is most points;
+//    prediction (k_pcss_guess), its misses scanned (kGuesses times, from a
+//    straight line at the miss rate so far) -- exact wherever the outcomes
+//    do not depend on the samples' offset, which is most points;
 //  * k_pcss_band evaluates every point at the 63 values of M_k around its
 //    prediction, a wavefront per point, the outcomes as a bit mask;
 //  * k_pcss_resolve follows the actual M_k through the masks in order,
@@ -147,6 +147,7 @@ __global__ __launch_bounds__(256) void k_pcss_points(FrameUniforms u, const floa
 // that order).  Every segment settles its first point (its prediction is
 // exact).
 constexpr int kBandHalf = 31;   // M_k in [prediction - 31, prediction + 31]: a wavefront per point
+constexpr int kGuesses = 2;     // prediction rounds per segment (more do not pay)
 
 __device__ __forceinline__ bool pcss_search(const FrameUniforms &u, const float *__restrict__ smap,
                                             const float *__restrict__ pairs, float4 q, uint32_t o)
@@ -209,66 +210,67 @@ __global__ __launch_bounds__(256) void k_pcss_band(FrameUniforms u, const float 
     if (lane == 0) masks[i] = b;
 }
 
-// One wavefront, lane l on point base + l of the current group of 64.  With
+// One wavefront, lane l on point g + l of the current group of 64.  With
 // P_i the prediction (misses before point i in the segment, pred[i]) and
-// D_i = 63 + M_i - M0 - P_i the actual M_i's place in point i's band,
+// D_i = 31 + M_i - M0 - P_i the actual M_i's place in point i's band,
 // D_{i+1} = D_i + (miss_i(D_i) - (P_{i+1} - P_i)): D stays put while the
 // prediction holds.  So the lanes from the first unsettled one on all test
 // the same D; a ballot finds the first lane whose outcome breaks the
 // prediction, the lanes up to it are settled, and D moves by one from
-// there.  A group takes one step per broken prediction, and the next
-// group's masks are loaded while this one is walked.  out = (first
+// there.  A group takes one step per broken prediction.  The masks and
+// prediction steps of 1024 points at a time are first copied to LDS (one
+// coalesced sweep), so the walk waits on LDS, not on memory.  out = (first
 // unsettled point, M there).
+constexpr uint32_t kResolveBlock = 1024;
+
 __global__ __launch_bounds__(64) void k_pcss_resolve(const unsigned long long *__restrict__ masks,
                                                      const uint32_t *__restrict__ pred, uint32_t s0, uint32_t L,
                                                      uint32_t M0, uint32_t *__restrict__ found,
                                                      uint32_t *__restrict__ out)
 {
+    __shared__ unsigned long long sm[kResolveBlock];
+    __shared__ int sdl[kResolveBlock];
     const uint32_t lane = threadIdx.x;
     int D = kBandHalf;
-    uint32_t base = 0;
+    uint32_t base = 0, settled = 0;
     bool stop = false;
-    auto load = [&](uint32_t b, unsigned long long &m, int &dl) {
-        const uint32_t i = b + lane;
-        m = 0;
-        dl = 0;
-        if (i < L) {
-            m = masks[i];
-            dl = (int)(pred[i + 1] - pred[i]);   // pred[L]: the segment's predicted total
-        }
-    };
-    unsigned long long m;
-    int dl;
-    load(0, m, dl);
     while (base < L && !stop) {
-        const uint32_t cnt = L - base < 64 ? L - base : 64;
-        unsigned long long nm;
-        int ndl;
-        load(base + 64, nm, ndl);
-        uint32_t start = 0;
-        unsigned long long fb = 0;
-        while (start < cnt) {
-            if (D < 0 || D > 2 * kBandHalf) {
-                stop = true;
-                break;
-            }
-            const uint32_t bit = (uint32_t)(m >> D) & 1u;
-            const int dev = (int)(1u - bit) - dl;
-            const unsigned long long brk = __ballot(lane >= start && lane < cnt && dev != 0);
-            const uint32_t j = brk ? (uint32_t)__builtin_ctzll(brk) : cnt - 1;
-            const unsigned long long span = (j == 63 ? ~0ull : ((1ull << (j + 1)) - 1ull)) & ~((1ull << start) - 1ull);
-            fb |= __ballot(bit) & span;
-            if (brk) D += __builtin_amdgcn_readlane(dev, j);
-            start = j + 1;
+        const uint32_t nb = L - base < kResolveBlock ? L - base : kResolveBlock;
+        for (uint32_t j = lane; j < nb; j += 64) {
+            sm[j] = masks[base + j];
+            sdl[j] = (int)(pred[base + j + 1] - pred[base + j]);   // pred[L]: the segment's predicted total
         }
-        if (lane < start) found[s0 + base + lane] = (uint32_t)(fb >> lane) & 1u;
-        base += start;
-        m = nm;
-        dl = ndl;
+        __syncthreads();
+        for (uint32_t g = 0; g < nb && !stop; g += 64) {
+            const uint32_t cnt = nb - g < 64 ? nb - g : 64;
+            const unsigned long long m = lane < cnt ? sm[g + lane] : 0ull;
+            const int dl = lane < cnt ? sdl[g + lane] : 0;
+            uint32_t start = 0;
+            unsigned long long fb = 0;
+            while (start < cnt) {
+                if (D < 0 || D > 2 * kBandHalf) {
+                    stop = true;
+                    break;
+                }
+                const uint32_t bit = (uint32_t)(m >> D) & 1u;
+                const int dev = (int)(1u - bit) - dl;
+                const unsigned long long brk = __ballot(lane >= start && lane < cnt && dev != 0);
+                const uint32_t j = brk ? (uint32_t)__builtin_ctzll(brk) : cnt - 1;
+                const unsigned long long span =
+                    (j == 63 ? ~0ull : ((1ull << (j + 1)) - 1ull)) & ~((1ull << start) - 1ull);
+                fb |= __ballot(bit) & span;
+                if (brk) D += __builtin_amdgcn_readlane(dev, j);
+                start = j + 1;
+            }
+            if (lane < start) found[s0 + base + g + lane] = (uint32_t)(fb >> lane) & 1u;
+            settled = base + g + start;
+        }
+        base += nb;
+        __syncthreads();   // the block's LDS copy is read before the next one overwrites it
     }
     if (lane == 0) {
-        out[0] = s0 + base;
-        out[1] = (uint32_t)((int)M0 + (int)pred[base] + D - kBandHalf);
+        out[0] = s0 + settled;
+        out[1] = (uint32_t)((int)M0 + (int)pred[settled] + D - kBandHalf);
     }
 }
 
@@ -501,14 +503,18 @@ void launch_pcss_segment(const FrameUniforms &u, const float *shadow_map, const 
                          uint32_t *scan_tmp, unsigned long long *masks, uint32_t *found, uint32_t *out, hipStream_t s)
 {
     if (L == 0) return;
-    // two guesses: from the straight line, then from the first guess's misses
-    hipLaunchKernelGGL(k_pcss_guess, dim3((L + 255) / 256), dim3(256), 0, s, u, shadow_map, pairs, pt, pbase, s0, L, M0,
-                       rate, (const uint32_t *)nullptr, pred);
-    launch_exclusive_scan(pred, pred, L, scan_tmp, s);
-    uint32_t *pred2 = pred + (L + 1);
-    hipLaunchKernelGGL(k_pcss_guess, dim3((L + 255) / 256), dim3(256), 0, s, u, shadow_map, pairs, pt, pbase, s0, L, M0,
-                       rate, (const uint32_t *)pred, pred2);
-    launch_exclusive_scan(pred2, pred2, L, scan_tmp, s);
+    // kGuesses guesses: from the straight line, then each from the misses of
+    // the one before; the last lands in pred + (L + 1)
+    uint32_t *bufs[2] = {pred + (L + 1), pred};
+    const uint32_t *prev = nullptr;
+    for (int q = 0; q < kGuesses; ++q) {
+        uint32_t *dst = bufs[(kGuesses - 1 - q) & 1];
+        hipLaunchKernelGGL(k_pcss_guess, dim3((L + 255) / 256), dim3(256), 0, s, u, shadow_map, pairs, pt, pbase, s0, L,
+                           M0, rate, prev, dst);
+        launch_exclusive_scan(dst, dst, L, scan_tmp, s);
+        prev = dst;
+    }
+    const uint32_t *pred2 = prev;
     hipLaunchKernelGGL(k_pcss_band, dim3((L + 3) / 4), dim3(256), 0, s, u, shadow_map, pairs, pt, pbase, s0, L, M0,
                        (const uint32_t *)pred2, masks);
     hipLaunchKernelGGL(k_pcss_resolve, dim3(1), dim3(64), 0, s, masks, (const uint32_t *)pred2, s0, L, M0, found,
