@@ -283,12 +283,27 @@ def frame_seeds(split, n):
     return [(1000 + i % 20) * 1000 + i // 20 for i in range(n)]
 
 
-def step_batch(step, seed, frames, crop, batch, scale=SCALE):
+def layout_kind(layout_seed):
+    """The arrangement `layout` draws for a layout seed (0: one sphere,
+    1: scattered, 2: small-sphere grid, 3: the benchmark scene's grid)."""
+    return int(np.random.default_rng(layout_seed).integers(0, 4))
+
+
+def train_frames(frames, favour_grid=1):
+    """The training frame seeds, the grid layouts (kinds 2 and 3) listed
+    `favour_grid` times."""
+    fl = frame_seeds("train", frames)
+    if favour_grid > 1:
+        fl = fl + [f for f in fl if layout_kind(f // 1000) >= 2] * (favour_grid - 1)
+    return fl
+
+
+def step_batch(step, seed, frames, crop, batch, scale=SCALE, favour_grid=1):
     """The frames and crop origins of training step `step`: a pure function
     of (seed, step), so a run resumed from a checkpoint sees the same data.
     4 frames per step, batch/4 crops each; crop origins in network pixels of
     each frame's network grid (frame_size / scale)."""
-    order = np.random.default_rng(seed).permutation(frame_seeds("train", frames))
+    order = np.random.default_rng(seed).permutation(train_frames(frames, favour_grid))
     rng = np.random.default_rng([seed, step])
     per = batch // 4
     out = []
@@ -310,16 +325,18 @@ def lr_at(step, total, base=1e-3):
     return base * (1.0 - 0.99 * min(1.0, (step - t0) / max(total - t0, 1.0)))
 
 
-def train(nao, ctx, sset, steps, start=0, total=None, seed=0, frames=4000, lr=1e-3, log=None, cfg=None):
+def train(nao, ctx, sset, steps, start=0, total=None, seed=0, frames=4000, lr=1e-3, log=None, cfg=None,
+          favour_grid=1):
     """Steps start .. start+steps-1 of a training run of `total` steps
     (default start+steps); returns per-step (sum sq err, sum |err|, covered
-    network pixels)."""
+    network pixels).  favour_grid: how many times the grid layouts are
+    listed among the training frames."""
     cfg = cfg if cfg is not None else ssao_cfg()
     total = start + steps if total is None else total
     per = nao.batch // 4
     hist = []
     for step in range(start, start + steps):
-        for j, (fs, xy) in enumerate(step_batch(step, seed, frames, nao.crop, nao.batch, nao.scale)):
+        for j, (fs, xy) in enumerate(step_batch(step, seed, frames, nao.crop, nao.batch, nao.scale, favour_grid)):
             W, H = frame_size(fs)
             render_frame(ctx, sset, fs, W, H, cfg)
             nao.stage(j * per, xy)
@@ -443,6 +460,8 @@ def main(argv=None):
     ap.add_argument("--out", required=True)
     ap.add_argument("--resume", default=None)
     ap.add_argument("--workdir", default="_scenes/_nao")
+    ap.add_argument("--favour-grid", type=int, default=1,
+                    help="list the grid layouts this many times among the training frames")
     a = ap.parse_args(argv)
     ctx = Context(0)
     sset = SceneSet(ctx, a.workdir)
@@ -460,7 +479,7 @@ def main(argv=None):
     for s0 in range(start, a.total, chunk):
         n = min(chunk, a.total - s0)
         hist += train(nao, ctx, sset, n, start=s0, total=a.total, seed=a.seed, frames=a.frames, lr=a.lr, cfg=cfg,
-                      log=lambda s: print(s, f"({time.time() - t0:.1f}s)", flush=True))
+                      log=lambda s: print(s, f"({time.time() - t0:.1f}s)", flush=True), favour_grid=a.favour_grid)
         ev = evaluate(nao, ctx, sset, 20, cfg=cfg)
         print(f"step {s0 + n}: held-out MAE (20 frames) {ev['mae']:.4f}", flush=True)
     t_train = time.time() - t0
