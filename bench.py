@@ -224,12 +224,13 @@ def readme_scene(stream=True):
     1280x800, shadow on, SSAA and SSAO off; the asset comes from the
     committed byte-for-byte fixtures.  stream: PCSS as the reference builds it
     (SOFT_SHADOW_PCSS, shadow.h:15-16), its disk samples drawn from the
-    random stream for every fragment that passes the depth test; otherwise
-    the same counts and sizes on one fixed disk."""
+    random stream for every fragment that passes the depth test -- the
+    scene's default; otherwise the same counts and sizes on one fixed disk
+    (`shadowfilter pcss 32 64 2.5`)."""
     from tests import assets as A
     name = "diablo_stream.scene" if stream else "diablo.scene"
     return A.scene(os.path.join(ROOT, "_scenes", "readme", name), ["diablo_pose"],
-                   extra=("shadowfilter refpcss",) if stream else ())
+                   extra=() if stream else ("shadowfilter pcss 32 64 2.5",))
 
 
 def run_readme_scene(steps, warmup, stream=True):

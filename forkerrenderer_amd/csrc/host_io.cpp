@@ -689,13 +689,19 @@ bool load_scene(const std::string &path, HostScene &s, std::string &err, int wra
     I.shadow_on = 1;     // Shadow's static default (src/shaders/shadow.cpp:9)
     I.deferred = 0;      // ForkerGL's static default render mode
     I.camera_ortho = 0;
-    // Reference-equivalent deterministic defaults for the RNG-driven filters.
+    // The reference's own filters by default: SOFT_SHADOW_PCSS
+    // (src/shaders/shadow.h:15-23) and Render::DoSSAO with its recursive blur,
+    // drawing from the random stream as the reference does; the
+    // `shadowfilter` and `ssaokernel` keys select the deterministic ones (the
+    // kernel below, 32 taps of radius 0.075, is what `ssaokernel` replaces).
     set_shadow_pcss(I.shadow, 32, 64, 2.5f);
+    I.shadow.mode = FGL_SHADOW_REF_PCSS;
     ssao_kernel(I.ssao, 32, 0x55A0u);
     I.ssao.radius = 0.075f;
     I.ssao.bias = 0.0005f;
     I.ssao.range = 0.01f;
     I.ssao.blur = 1;
+    I.ssao.stochastic = 1;
     bool have_point = false, have_dir = false;
     fgl_light point{}, dir{};
     LoadState st;

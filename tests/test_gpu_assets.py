@@ -3,7 +3,9 @@ african_head, brickwall, plane; kept byte for byte under tests/golden/assets)
 through the whole device path: the product's loader, shadow pass, forward
 Blinn-Phong or deferred G-buffer + SSAO + blur + lighting, SSAA resolve.
 
-These exercise what the generated scenes do not: mary's two groups drawn in
+Without filter keys the scenes take the reference's own filters: PCSS and
+DoSSAO drawing from the random stream.  These exercise what the generated
+scenes do not: mary's two groups drawn in
 name order with a material whose texture file is missing, great_sword's
 quads (fan triangulation) and RGBA texture, the head and its eye model with
 1024^2 diffuse / specular / normal maps, brickwall's normal map with
@@ -56,27 +58,29 @@ def test_asset_frame(ctx, scene_dir, names, mode):
 
 
 def test_readme_diablo_frame(ctx, scene_dir):
-    """The README's console-output run (README.md:190-223): Diablo alone,
-    forward Blinn-Phong, 1280x800, shadow on with the default PCSS filter,
-    SSAA and SSAO off."""
+    """The README's console-output run (README.md:190-223) as the reference
+    builds it by default, and as the scene loads here without filter keys:
+    Diablo alone, forward Blinn-Phong, 1280x800, shadow on with
+    SOFT_SHADOW_PCSS drawing its samples from the random stream
+    (shadow.h:15-16, shadow.cpp:65-106) for every fragment that passes the
+    depth test, SSAA and SSAO off.  Depth and shadow map bit-exact, the
+    stream consumed to the same draw, image within 1 LSB."""
     p = A.scene(os.path.join(scene_dir, "assets", "readme_diablo.scene"), ["diablo_pose"])
-    sc = _check(ctx, p, "README Diablo 1280x800 forward", 0.1)
+    sc = _check(ctx, p, "README Diablo 1280x800 forward, reference stream", 0.1)
     I = sc.info
     m, _ = sc.model(0)
-    assert (I.width, I.height, I.deferred, I.shadow_on, I.shadow.mode, len(m["vi"])) == (1280, 800, 0, 1, 2, 5022)
-
-
-def test_readme_diablo_reference_stream(ctx, scene_dir):
-    """The README's run as the reference builds it by default: forward
-    Blinn-Phong with SOFT_SHADOW_PCSS drawing its samples from the random
-    stream (shadow.h:15-16, shadow.cpp:65-106) for every fragment that passes
-    the depth test.  Diablo alone at 1280x800; depth and shadow map
-    bit-exact, the stream consumed to the same draw, image within 1 LSB."""
-    p = A.scene(os.path.join(scene_dir, "assets", "readme_diablo_ref.scene"), ["diablo_pose"],
-                extra=("shadowfilter refpcss",))
-    sc = _check(ctx, p, "README Diablo 1280x800 forward, reference stream", 0.1)
-    assert (sc.info.deferred, sc.info.shadow.mode) == (0, F.SHADOW_REF_PCSS)
+    assert (I.width, I.height, I.deferred, I.shadow_on, I.shadow.mode, len(m["vi"])) == \
+        (1280, 800, 0, 1, F.SHADOW_REF_PCSS, 5022)
     assert ctx.rng_draws() == O.rng_draws(), (ctx.rng_draws(), O.rng_draws())
+
+
+def test_readme_diablo_fixed_disk(ctx, scene_dir):
+    """The same frame with the reference's PCSS sample counts and sizes on
+    one fixed disk (`shadowfilter pcss 32 64 2.5`)."""
+    p = A.scene(os.path.join(scene_dir, "assets", "readme_diablo_fixed.scene"), ["diablo_pose"],
+                extra=("shadowfilter pcss 32 64 2.5",))
+    sc = _check(ctx, p, "README Diablo 1280x800 forward, fixed disk", 0.1)
+    assert (sc.info.deferred, sc.info.shadow.mode) == (0, F.SHADOW_PCSS)
 
 
 def test_diablo_deferred_ssaa(ctx, scene_dir):

@@ -136,7 +136,9 @@ def test_stochastic_scene_keys(scene_dir):
     """`ssaokernel reference` selects Render::DoSSAO with the random stream;
     `shadowfilter refpcf|refpcss` the reference's PCF / PCSS with its
     parameters (shadow.h:18-22: 64 PCF and 32 blocker samples, filter sizes
-    0.007 and 0.005, area light 2.5)."""
+    0.007 and 0.005, area light 2.5).  A scene without filter keys gets the
+    reference's defaults: PCSS and DoSSAO on the random stream, the
+    recursive blur."""
     for filt, mode in (("refpcf", F.SHADOW_REF_PCF), ("refpcss", F.SHADOW_REF_PCSS)):
         p = G.tiny_scene(os.path.join(scene_dir, "k" + filt), seed=1, ssao=True,
                          extra=f"ssaokernel reference\nshadowfilter {filt}")
@@ -147,7 +149,18 @@ def test_stochastic_scene_keys(scene_dir):
         assert s.pcf_filter == np.float32(0.007) and s.blocker_filter == 0.005 and s.pcss_filter == 0.007
         assert s.area_light == np.float32(2.5)
     p = G.tiny_scene(os.path.join(scene_dir, "kdet"), seed=1, ssao=True)
-    assert F.Scene(p).info.ssao.stochastic == 0
+    I = F.Scene(p).info
+    assert I.ssao.stochastic == 0 and I.shadow.mode == F.SHADOW_HARD
+    # no filter keys: the reference's own filters, as its default build has them
+    d = os.path.join(scene_dir, "kdefault")
+    os.makedirs(d, exist_ok=True)
+    p = os.path.join(d, "plain.scene")
+    with open(p, "w") as fh:
+        fh.write("mode deferred\nssao on\nlight point 2 5 5 1 1 1\ncamera persp -1 1 1 0 0 -1\n")
+    I = F.Scene(p).info
+    assert (I.shadow_on, I.shadow.mode, I.shadow.n_blocker, I.shadow.n_pcf) == (1, F.SHADOW_REF_PCSS, 32, 64)
+    assert I.shadow.blocker_filter == 0.005 and I.shadow.pcss_filter == 0.007 and I.shadow.area_light == np.float32(2.5)
+    assert (I.ssao_on, I.ssao.stochastic, I.ssao.blur) == (1, 1, 1)
 
 
 def test_rle_stream_equals_the_reference_encoder(tmp_path):
