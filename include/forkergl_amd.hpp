@@ -331,8 +331,23 @@ private:
 };
 
 // Shadow::SetShadowStatus / GetShadowStatus and the filter parameters of
-// src/shaders/shadow.h:12-23 (src/shaders/shadow.cpp:9-21).
+// src/shaders/shadow.h:12-23 (src/shaders/shadow.cpp:9-21).  The filter is
+// the reference's default build's until a Scene or a caller sets another:
+// SOFT_SHADOW_PCSS with its random disk samples (32 blocker-search samples
+// over 0.005, 64 PCF samples over 0.007 times the penumbra, area light 2.5).
 struct Shadow {
+    static fgl_shadow_cfg ReferencePCSS()
+    {
+        fgl_shadow_cfg c{};
+        c.mode = FGL_SHADOW_REF_PCSS;
+        c.n_blocker = 32;
+        c.n_pcf = 64;
+        c.pcf_filter = 0.007f;
+        c.blocker_filter = 0.005;
+        c.pcss_filter = 0.007;
+        c.area_light = 2.5f;
+        return c;
+    }
     static void SetShadowStatus(bool on) { SetShadowStatus(on, cfg_); }
     static void SetShadowStatus(bool on, const fgl_shadow_cfg &cfg)
     {
@@ -344,7 +359,7 @@ struct Shadow {
 
 private:
     inline static bool on_ = true;
-    inline static fgl_shadow_cfg cfg_{};
+    inline static fgl_shadow_cfg cfg_ = ReferencePCSS();
 };
 
 // ---------------- Scene (src/scene.h) ----------------
