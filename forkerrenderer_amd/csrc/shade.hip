@@ -37,9 +37,10 @@ __global__ __launch_bounds__(256) void k_ssao_ref(FrameUniforms u, const float *
 // shadow map alone: k_pcss_classify checks every texel a sample of the point
 // could reach (the square around the disk) and marks the point "no blocker
 // for any sample set", "a blocker for any sample set" or "depends on the
-// samples".  Only the last kind is walked in order (k_pcss_walk_list, one
-// wavefront), at offsets that count the decided points before it.  The
-// points' inputs are planes of stride P: normal, world position, light NDC.
+// samples".  Only the last kind is walked in order (k_pcss_points,
+// k_pcss_guess, k_pcss_band, k_pcss_resolve below), at offsets that count
+// the decided points before it.  The points' inputs are planes of stride P:
+// normal, world position, light NDC.
 
 // Light 0's shadow coordinate and bias at point i, as shade_point computes them.
 __device__ __forceinline__ void pcss_pixel(const FrameUniforms &u, const float *__restrict__ g, size_t P, size_t i,
