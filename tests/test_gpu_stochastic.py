@@ -160,6 +160,26 @@ def test_forward_stochastic_frames_match_the_oracle(ctx, scene_dir, filt, tile):
     assert device_draws > 2 * 32 * covered
 
 
+@pytest.mark.parametrize("pbr", [False, True], ids=["phong", "pbr"])
+def test_forward_stochastic_ssaa_and_two_lights(ctx, scene_dir, pbr):
+    """The forward pass's fragment sequence at SSAA 2x (the internal frame's
+    pixels) with a point and a directional light (only the point light is
+    shadowed and draws samples), Blinn-Phong and Cook-Torrance."""
+    p = G.tiny_scene(os.path.join(scene_dir, f"fssaa{int(pbr)}"), seed=35, mode="forward", width=60, height=44,
+                     subdiv=3, tex=16, ssaa=2, pbr=pbr,
+                     extra="shadowfilter refpcss\nmultilight on\nlight dir -0.3 -1 -0.2 0.5 0.5 0.6")
+    sc = F.Scene(p)
+    assert sc.info.ssaa_on == 1 and sc.info.n_lights == 2
+    got = gpu_frame(ctx, sc)
+    device_draws = ctx.rng_draws()
+    want = oracle_frame(sc)
+    assert device_draws == O.rng_draws(), (device_draws, O.rng_draws())
+    assert_bitexact("depth", got["depth"], want["depth"])
+    assert_bitexact("shadow", got["shadow"], want["shadow"])
+    mx, frac = lsb_stats(got["image"], want["image"])
+    assert frac >= 0.999, (mx, frac)
+
+
 def test_forward_stochastic_passes_split_over_flushes_equal_one_pass(ctx, scene_dir):
     """The depth buffer and the stream carry over between forward passes as
     in the reference: the models drawn in one pass, or in two passes (the
