@@ -800,7 +800,7 @@ static void fill_draw(fgl_ctx *c, const ModelRes &m, const float M[16], DevDraw 
 // the draws needed are estimated from the acceptance rate (pi/4 for the
 // disk, pi/6 for the sphere) with a margin, and doubled in the rare case
 // that too few tries are kept.
-static constexpr int kMtChunkBlocks = 256;
+static constexpr int kMtChunkBlocks = 512;
 
 static int rng_fill(fgl_ctx *c, int D, float *out, uint32_t n)
 {

@@ -42,7 +42,7 @@ def test_device_stream_matches_the_oracle(ctx):
 
 @pytest.mark.parametrize("seed", [5489, 77])
 def test_device_stream_in_parallel_chunks_matches_numpy(ctx, seed):
-    """Requests of more than one chunk (256 blocks, 159,744 draws) run as
+    """Requests of more than one chunk (512 blocks, 319,488 draws) run as
     parallel chunks, each reached by jumping ahead from the context's array:
     vectors bit-exact with numpy's MT19937 through the reference's rejection
     loops, draw counts equal, and the stream continues exactly from the
