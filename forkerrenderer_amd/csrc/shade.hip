@@ -350,35 +350,83 @@ __global__ __launch_bounds__(1024) void k_ssao_tiled(FrameUniforms u, SsaoArgs s
 __constant__ float c_ref_blur_w[5] = {(float)0.227027, (float)0.1945946, (float)0.1216216,
                                       (float)0.054054, (float)0.016216};
 
+// Buffer1f::TwoPassGaussianBlurDenoised (src/buffer.cpp:59-98) along one
+// line of n samples at stride s, in place: each output is the reference's
+// sum in its order -- the centre, then for i = 1..4 the sample i ahead
+// (not yet blurred) and the sample i behind (already blurred), indices
+// clamped to the line.  The last four outputs and the next four inputs stay
+// in registers, and the inputs are loaded kPF steps ahead (samples ahead are
+// never written before they are read), so a step waits on its own
+// arithmetic, not on memory.
+constexpr int kPF = 16;
+
+__device__ __forceinline__ void blur_rec_line(float *__restrict__ p, int n, size_t s)
+{
+    const float w0 = c_ref_blur_w[0], w1 = c_ref_blur_w[1], w2 = c_ref_blur_w[2], w3 = c_ref_blur_w[3],
+                w4 = c_ref_blur_w[4];
+    auto at = [&](int x) { return p[(size_t)(x < n - 1 ? x : n - 1) * s]; };
+    float a0 = at(0), a1 = at(1), a2 = at(2), a3 = at(3), a4 = at(4);
+    // x = 0: every sample behind clamps to sample 0, not yet blurred
+    float r = a0 * w0;
+    r += a1 * w1;
+    r += a0 * w1;
+    r += a2 * w2;
+    r += a0 * w2;
+    r += a3 * w3;
+    r += a0 * w3;
+    r += a4 * w4;
+    r += a0 * w4;
+    p[0] = r;
+    float o1 = r, o2 = r, o3 = r, o4 = r;   // from here on they clamp to the blurred sample 0
+    a0 = a1;
+    a1 = a2;
+    a2 = a3;
+    a3 = a4;
+    float ring[kPF];
+#pragma unroll
+    for (int j = 0; j < kPF; ++j) ring[j] = at(5 + j);
+    for (int x0 = 1; x0 < n; x0 += kPF) {
+#pragma unroll
+        for (int j = 0; j < kPF; ++j) {
+            const int x = x0 + j;
+            if (x < n) {
+                a4 = ring[j];                   // sample x + 4
+                ring[j] = at(x + 4 + kPF);      // for step x + kPF
+                float q = a0 * w0;
+                q += a1 * w1;
+                q += o1 * w1;
+                q += a2 * w2;
+                q += o2 * w2;
+                q += a3 * w3;
+                q += o3 * w3;
+                q += a4 * w4;
+                q += o4 * w4;
+                p[(size_t)x * s] = q;
+                o4 = o3;
+                o3 = o2;
+                o2 = o1;
+                o1 = q;
+                a0 = a1;
+                a1 = a2;
+                a2 = a3;
+                a3 = a4;
+            }
+        }
+    }
+}
+
 __global__ __launch_bounds__(64) void k_blur_rec_rows(float *__restrict__ a, int W, int H)
 {
     const int h = blockIdx.x * 64 + threadIdx.x;
     if (h >= H) return;
-    float *row = a + (size_t)h * W;
-    for (int x = 0; x < W; ++x) {
-        float r = row[x] * c_ref_blur_w[0];
-#pragma unroll
-        for (int i = 1; i < 5; ++i) {
-            r += row[clampi(x + i, 0, W - 1)] * c_ref_blur_w[i];
-            r += row[clampi(x - i, 0, W - 1)] * c_ref_blur_w[i];
-        }
-        row[x] = r;
-    }
+    blur_rec_line(a + (size_t)h * W, W, 1);
 }
 
 __global__ __launch_bounds__(64) void k_blur_rec_cols(float *__restrict__ a, int W, int H)
 {
     const int x = blockIdx.x * 64 + threadIdx.x;
     if (x >= W) return;
-    for (int h = 0; h < H; ++h) {
-        float r = a[(size_t)x + (size_t)h * W] * c_ref_blur_w[0];
-#pragma unroll
-        for (int i = 1; i < 5; ++i) {
-            r += a[(size_t)x + (size_t)clampi(h + i, 0, H - 1) * W] * c_ref_blur_w[i];
-            r += a[(size_t)x + (size_t)clampi(h - i, 0, H - 1) * W] * c_ref_blur_w[i];
-        }
-        a[(size_t)x + (size_t)h * W] = r;
-    }
+    blur_rec_line(a + x, H, (size_t)W);
 }
 
 struct BlurWeights {
