@@ -886,22 +886,21 @@ static int stochastic_shadow_samples(fgl_ctx *c, const FrameUniforms &u, const f
     int rc;
     const int mode = c->shadow_cfg.mode;
     const uint32_t nb = (uint32_t)c->shadow_cfg.n_blocker, np = (uint32_t)c->shadow_cfg.n_pcf;
-    const size_t most = n * (mode == FGL_SHADOW_REF_PCF ? np : nb + np);
-    if (most > 0xffffffffull) return fail(c, FGL_ERR_CAPACITY, "stochastic shadows: too many samples");
-    // never fewer than one point's worth, so every offset in [0, most) reads
-    // inside the buffer
-    if ((rc = ensure(c, c->rng_samples, (most > nb + np ? most : nb + np) * 2 * sizeof(float)))) return rc;
-    float *pairs = (float *)c->rng_samples.p;
     if (mode == FGL_SHADOW_REF_PCF) {
-        if ((rc = rng_fill(c, 2, pairs, (uint32_t)most)) || (rc = rng_take(c, (uint32_t)most))) return rc;
+        const size_t total = n * np;
+        if (total > 0xffffffffull) return fail(c, FGL_ERR_CAPACITY, "stochastic shadows: too many samples");
+        if ((rc = ensure(c, c->rng_samples, (total > np ? total : np) * 2 * sizeof(float)))) return rc;
+        float *pairs = (float *)c->rng_samples.p;
+        if ((rc = rng_fill(c, 2, pairs, (uint32_t)total)) || (rc = rng_take(c, (uint32_t)total))) return rc;
         ss = {pairs, nullptr, np};
         return FGL_OK;
     }
     // PCSS takes n_pcf more samples only where its blocker search finds a
-    // blocker.  The most samples the points can need are drawn without
-    // consuming them; points the shadow map decides alone are counted in
-    // parallel, the others walked in order; then the stream moves past
-    // exactly the samples used.
+    // blocker.  Points the shadow map decides alone are counted in parallel;
+    // the most samples the points can need (n_pcf more for every undecided
+    // one) are drawn without consuming them; the undecided points are walked
+    // in order; then the stream moves past exactly the samples used.
+    if (n * (nb + np) > 0xffffffffull) return fail(c, FGL_ERR_CAPACITY, "stochastic shadows: too many samples");
     const size_t words = scan_tmp_words(n + 1), n4 = (n + 4) & ~(size_t)3;
     const size_t seg_words = 2 * (size_t)kSegmentMax + 2 * ((size_t)kSegmentMax + 2) + 4;
     if ((rc = ensure(c, c->rng_offs, (6 * n4 + 4 * n4 + seg_words + words) * sizeof(uint32_t)))) return rc;
@@ -912,14 +911,21 @@ static int stochastic_shadow_samples(fgl_ctx *c, const FrameUniforms &u, const f
     uint32_t *pred = (uint32_t *)(masks + kSegmentMax);
     uint32_t *seg_out = pred + 2 * ((size_t)kSegmentMax + 2);
     uint32_t *tmp = seg_out + 4;
-    if ((rc = rng_fill(c, 2, pairs, (uint32_t)most))) return rc;
     launch_pcss_classify(u, g, n, (const float *)c->shadow.p, base, upos, c->stream);
     launch_exclusive_scan(base, base, n, tmp, c->stream);
     launch_exclusive_scan(upos, upos, n, tmp, c->stream);
     launch_pcss_compact(upos, n, list, c->stream);
-    HIPCHK(c, hipMemcpyAsync(c->h_total, upos + n, 4, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipMemcpyAsync(c->h_total, base + n, 4, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipMemcpyAsync(c->h_total + 1, upos + n, 4, hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
-    const uint32_t n_unc = c->h_total[0];
+    const uint32_t n_unc = c->h_total[1];
+    // the decided points' samples and n_blocker per undecided point, plus
+    // n_pcf per undecided point: every offset the walk evaluates reads
+    // inside it (point k of the undecided ones at most n_pcf k past its base)
+    const size_t most = (size_t)c->h_total[0] + (size_t)np * n_unc;
+    if ((rc = ensure(c, c->rng_samples, (most > nb + np ? most : nb + np) * 2 * sizeof(float)))) return rc;
+    float *pairs = (float *)c->rng_samples.p;
+    if ((rc = rng_fill(c, 2, pairs, (uint32_t)most))) return rc;
     // the undecided points in order, a segment at a time (shade.hip): the
     // band of miss counts each point is evaluated at follows the miss rate
     // seen so far

@@ -60,29 +60,32 @@ __device__ __forceinline__ float random_pm1(uint32_t u)
     return -1.f + 2.f * r;
 }
 
-// The twist of std::mt19937 on 624 LDS words in three dependent phases:
-// words [0,227) read old words only; [227,454) read the new [0,227);
-// [454,624) read the new [227,397) (and word 623 the new word 0).  Each
-// phase reads, waits for every read, then writes.
-__device__ __forceinline__ void mt_twist(uint32_t *mt)
+// The generator as one sequence: x_{n+624} = twist(x_n, x_{n+1}, x_{n+397})
+// (a block's twist is this recurrence, new words used as they appear).  The
+// 227 words x_n .. x_{n+226} read only words at least 227 back, so a
+// workgroup makes them together, one barrier per step, keeping the last 1024
+// words in an LDS ring (word n in slot n % 1024): a step writes the slots of
+// words 1024 back, none of which it reads (its reads lie 227 to 624 back).
+constexpr uint32_t kMtRing = 1024, kMtLag = kMtN - kMtM;
+
+// ring holds x_0 .. x_623 (slots 0..623); the next `total` words x_624 ..
+// go through emit(j, x_{624+j}) once each, in steps of 227.  Needs at least
+// 227 threads; every thread of the workgroup calls it.
+template <typename Emit>
+__device__ __forceinline__ void mt_run(uint32_t *ring, uint32_t total, Emit emit)
 {
-    const int tid = threadIdx.x;
-    uint32_t v = 0;
-    if (tid < 227) v = mt_twist_word(mt[tid], mt[tid + 1], mt[tid + kMtM]);
-    __syncthreads();
-    if (tid < 227) mt[tid] = v;
-    __syncthreads();
-    if (tid < 227) v = mt_twist_word(mt[227 + tid], mt[228 + tid], mt[tid]);
-    __syncthreads();
-    if (tid < 227) mt[227 + tid] = v;
-    __syncthreads();
-    if (tid < 170) {
-        const int i = 454 + tid;
-        v = mt_twist_word(mt[i], mt[(i + 1) % kMtN], mt[i - 227]);
+    const uint32_t tid = threadIdx.x;
+    for (uint32_t j0 = 0; j0 < total; j0 += kMtLag) {
+        const uint32_t j = j0 + tid;
+        if (tid < kMtLag && j < total) {
+            // x_{624+j} from x_j, x_{j+1}, x_{j+397}
+            const uint32_t v = mt_twist_word(ring[j & (kMtRing - 1)], ring[(j + 1) & (kMtRing - 1)],
+                                             ring[(j + kMtM) & (kMtRing - 1)]);
+            ring[(j + kMtN) & (kMtRing - 1)] = v;
+            emit(j, v);
+        }
+        __syncthreads();
     }
-    __syncthreads();
-    if (tid < 170) mt[454 + tid] = v;
-    __syncthreads();
 }
 
 __device__ __forceinline__ uint32_t mt_untemper(uint32_t y)
@@ -120,24 +123,21 @@ __device__ __forceinline__ bool mt_try(const uint32_t *__restrict__ p, float &x,
 // bits i of t^(c 624 K) mod phi, x_B .. being the context's array and the 32
 // blocks after it.  The tries are then flagged, scanned and kept in parallel,
 // and a commit moves the context's stream past the tries a caller used.
-constexpr int kMtBaseBlocks = kMtBaseWords / (int)kMtN;   // every window of a jump: x_{B + 19967 + 623} at most
+// every window of a jump, x_{B + 19967 + 623} at most, lies in the base sequence
 static_assert(kMtBaseWords >= 64 * kMtPolyWords + 623, "the jump's windows stay in the base sequence");
 
 // One workgroup: the context's array and the 32 blocks after it, untempered.
 __global__ __launch_bounds__(256) void k_mt_base(const MtState *__restrict__ st, uint32_t *__restrict__ base)
 {
-    __shared__ uint32_t mt[kMtN];
+    __shared__ uint32_t ring[kMtRing];
     const int tid = threadIdx.x;
     for (int i = tid; i < (int)kMtN; i += 256) {
         const uint32_t v = st->mt[i];
-        mt[i] = v;
+        ring[i] = v;
         base[i] = v;
     }
     __syncthreads();
-    for (int b = 1; b < kMtBaseBlocks; ++b) {
-        mt_twist(mt);   // ends with a barrier
-        for (int i = tid; i < (int)kMtN; i += 256) base[(size_t)b * kMtN + i] = mt[i];
-    }
+    mt_run(ring, (uint32_t)(kMtBaseWords - (int)kMtN), [&](uint32_t j, uint32_t v) { base[kMtN + j] = v; });
 }
 
 // Workgroup c - 1 jumps to chunk c's array: states[c][k] = XOR of base[i + k]
@@ -217,20 +217,17 @@ __global__ __launch_bounds__(640) void k_mt_jump(const uint32_t *__restrict__ ba
 __global__ __launch_bounds__(256) void k_mt_gen(const MtState *__restrict__ st, const uint32_t *__restrict__ states,
                                                 int K, uint32_t *__restrict__ raw)
 {
-    __shared__ uint32_t mt[kMtN];
+    __shared__ uint32_t ring[kMtRing];
     const int tid = threadIdx.x, c = blockIdx.x;
     const uint32_t idx = st->idx;
     const uint32_t *src = c == 0 ? st->mt : states + (size_t)c * kMtN;
-    for (int i = tid; i < (int)kMtN; i += 256) mt[i] = src[i];
+    for (int i = tid; i < (int)kMtN; i += 256) ring[i] = src[i];
     __syncthreads();
     const size_t a = kMtN - idx;
     if (c == 0)
-        for (uint32_t i = tid; i < a; i += 256) raw[i] = mt_temper(mt[idx + i]);
+        for (uint32_t i = tid; i < a; i += 256) raw[i] = mt_temper(ring[idx + i]);
     uint32_t *out = raw + a + (size_t)c * K * kMtN;
-    for (int b = 0; b < K; ++b) {
-        mt_twist(mt);   // starts by reading, and writes only after a barrier
-        for (int i = tid; i < (int)kMtN; i += 256) out[(size_t)b * kMtN + i] = mt_temper(mt[i]);
-    }
+    mt_run(ring, (uint32_t)K * kMtN, [&](uint32_t j, uint32_t v) { out[j] = mt_temper(v); });
 }
 
 template <int D>
