@@ -107,8 +107,9 @@ struct fgl_ctx {
     // the parallel generator: base sequence, jumped chunk arrays, raw draws,
     // try flags / their scan, the jump polynomials uploaded so far; the last
     // fill's tries (T of D draws) for its commit
-    DevBuf mt_base, mt_states, mt_raw, mt_pos, mt_polys;
-    int mt_polys_n = 0;
+    DevBuf mt_base, mt_states, mt_raw, mt_pos;
+    DevBuf mt_polys[2];          // jump polynomials for the two chunk lengths
+    int mt_polys_n[2] = {0, 0};
     uint32_t fill_T = 0, fill_n = 0;
     int fill_D = 0;
 
@@ -342,7 +343,7 @@ void fgl_ctx_destroy(fgl_ctx *c)
     DevBuf *bufs[] = {&c->d_textures, &c->d_materials, &c->depth, &c->fb, &c->gbuf, &c->ao,
                       &c->blur_tmp, &c->shadow, &c->shadow_depth, &c->img, &c->vis, &c->d_draws,
                       &c->rng, &c->rng_samples, &c->rng_offs, &c->ssao_pow3,
-                      &c->mt_base, &c->mt_states, &c->mt_raw, &c->mt_pos, &c->mt_polys,
+                      &c->mt_base, &c->mt_states, &c->mt_raw, &c->mt_pos, &c->mt_polys[0], &c->mt_polys[1],
                       &c->screen, &c->rect, &c->counts, &c->offs, &c->scan_tmp, &c->keys,
                       &c->vals, &c->keys_alt, &c->vals_alt, &c->sort_tmp, &c->tile_start,
                       &c->tile_end, &c->tri_v, &c->big_list, &c->counters, &c->ndc_z,
@@ -795,12 +796,14 @@ static void fill_draw(fgl_ctx *c, const ModelRes &m, const float M[16], DevDraw 
 // The stream's next n vectors (RandomVectorInUnitDisk for D = 2,
 // RandomVectorInUnitSphere for D = 3) into out, in stream order, without
 // consuming them: rng_take then moves the stream past the first m <= n.  The
-// draws are generated in parallel chunks of kMtChunkBlocks blocks, each
-// chunk's array reached by a jump from the context's (rng.hip, mt_jump.hpp);
-// the draws needed are estimated from the acceptance rate (pi/4 for the
-// disk, pi/6 for the sphere) with a margin, and doubled in the rare case
-// that too few tries are kept.
-static constexpr int kMtChunkBlocks = 512;
+// draws are generated in parallel chunks of K blocks, each chunk's array
+// reached by a jump from the context's (rng.hip, mt_jump.hpp); the draws
+// needed are estimated from the acceptance rate (pi/4 for the disk, pi/6 for
+// the sphere) with a margin, and doubled in the rare case that too few tries
+// are kept.  A chunk's generation takes time in proportion to K, and the
+// jumps (a workgroup each, one per CU) take a fixed time per 256 chunks:
+// K = 256 while 256 chunks of it cover the request, else 512.
+static constexpr int kMtChunkShort = 256, kMtChunkLong = 512;
 
 static int rng_fill(fgl_ctx *c, int D, float *out, uint32_t n)
 {
@@ -810,11 +813,12 @@ static int rng_fill(fgl_ctx *c, int D, float *out, uint32_t n)
     c->fill_D = D;
     if (n == 0) return FGL_OK;
     const double keep = D == 2 ? 0.78539816339744831 : 0.52359877559829887;
-    const uint64_t C = 624ull * kMtChunkBlocks;
     uint64_t R = (uint64_t)((double)n * D / keep * 1.01) + 4096;
     for (;;) {
+        const int kind = R <= 256ull * 624 * kMtChunkShort ? 0 : 1;
+        const uint64_t C = 624ull * (kind ? kMtChunkLong : kMtChunkShort);
         uint64_t Nc = (R + C - 1) / C;
-        int K = kMtChunkBlocks;
+        int K = kind ? kMtChunkLong : kMtChunkShort;
         if (Nc == 1) K = (int)((R + 623) / 624);   // one chunk: only the blocks needed
         const uint64_t len = Nc * (uint64_t)K * 624;
         if (len + 624 >= 0xffffffffull || Nc > 0x7fffffffull)
@@ -828,21 +832,21 @@ static int rng_fill(fgl_ctx *c, int D, float *out, uint32_t n)
         const uint32_t *states = nullptr;
         if (Nc > 1) {
             const int nj = (int)Nc - 1;
-            if (nj > c->mt_polys_n) {
+            if (nj > c->mt_polys_n[kind]) {
                 std::vector<uint64_t> polys;
                 if (!mt_jump_polys(C, nj, polys))
                     return fail(c, FGL_ERR_DEVICE, "random stream: no characteristic polynomial");
-                if ((rc = ensure(c, c->mt_polys, polys.size() * sizeof(uint64_t)))) return rc;
-                HIPCHK(c, hipMemcpyAsync(c->mt_polys.p, polys.data(), polys.size() * sizeof(uint64_t),
+                if ((rc = ensure(c, c->mt_polys[kind], polys.size() * sizeof(uint64_t)))) return rc;
+                HIPCHK(c, hipMemcpyAsync(c->mt_polys[kind].p, polys.data(), polys.size() * sizeof(uint64_t),
                                          hipMemcpyHostToDevice, c->stream));
                 HIPCHK(c, hipStreamSynchronize(c->stream));
-                c->mt_polys_n = nj;
+                c->mt_polys_n[kind] = nj;
             }
             if ((rc = ensure(c, c->mt_base, (size_t)kMtBaseWords * 4)) ||
                 (rc = ensure(c, c->mt_states, (size_t)Nc * 624 * 4)))
                 return rc;
             launch_mt_base(st, (uint32_t *)c->mt_base.p, c->stream);
-            launch_mt_jump((const uint32_t *)c->mt_base.p, (const uint64_t *)c->mt_polys.p, nj,
+            launch_mt_jump((const uint32_t *)c->mt_base.p, (const uint64_t *)c->mt_polys[kind].p, nj,
                            (uint32_t *)c->mt_states.p, c->stream);
             states = (const uint32_t *)c->mt_states.p;
         }

@@ -10,6 +10,7 @@
 #include <immintrin.h>
 
 #include <cstring>
+#include <map>
 #include <mutex>
 
 namespace fgl {
@@ -167,12 +168,15 @@ void reduce(const Phi &phi, uint64_t *p, uint64_t *out)
     out[kW - 1] &= (1ull << (kMtDegree - 64 * (kW - 1))) - 1;
 }
 
+struct Series {
+    std::vector<uint64_t> g1, polys;   // t^step mod phi; t^(c step) mod phi for c = 1..
+};
+
 struct Table {
     std::mutex mu;
     bool tried = false, ok = false, hw = false;
     Phi phi;
-    uint64_t step = 0;
-    std::vector<uint64_t> g1, polys;   // t^step mod phi; t^(c step) mod phi for c = 1..
+    std::map<uint64_t, Series> series;   // by step
 };
 
 Table &table()
@@ -223,26 +227,25 @@ bool mt_jump_polys(uint64_t step, int n, std::vector<uint64_t> &out)
         T.ok = init_phi(T);
     }
     if (!T.ok) return false;
-    if (T.step != step) {
+    Series &S = T.series[step];
+    if (S.g1.empty()) {
         // t^step by square-and-multiply from the top bit of step
-        T.step = step;
-        T.polys.clear();
-        T.g1.assign(kW, 0);
-        T.g1[0] = 1;
+        S.g1.assign(kW, 0);
+        S.g1[0] = 1;
         std::vector<uint64_t> t1(kW, 0);
         t1[0] = 2;   // t
         for (int b = 63; b >= 0; --b) {
-            mulmod(T, T.g1.data(), T.g1.data(), T.g1.data());
-            if ((step >> b) & 1u) mulmod(T, T.g1.data(), t1.data(), T.g1.data());
+            mulmod(T, S.g1.data(), S.g1.data(), S.g1.data());
+            if ((step >> b) & 1u) mulmod(T, S.g1.data(), t1.data(), S.g1.data());
         }
-        T.polys = T.g1;
+        S.polys = S.g1;
     }
-    while ((int)(T.polys.size() / kW) < n) {
-        const size_t have = T.polys.size();
-        T.polys.resize(have + kW);
-        mulmod(T, T.polys.data() + have - kW, T.g1.data(), T.polys.data() + have);
+    while ((int)(S.polys.size() / kW) < n) {
+        const size_t have = S.polys.size();
+        S.polys.resize(have + kW);
+        mulmod(T, S.polys.data() + have - kW, S.g1.data(), S.polys.data() + have);
     }
-    out.assign(T.polys.begin(), T.polys.begin() + (size_t)n * kW);
+    out.assign(S.polys.begin(), S.polys.begin() + (size_t)n * kW);
     return true;
 }
 

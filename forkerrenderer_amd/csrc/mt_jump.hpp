@@ -22,9 +22,9 @@ constexpr int kMtPolyWords = 312;   // uint64 words holding the coefficients of 
 
 // out = the polynomials t^(c * step) mod phi(t) for c = 1..n, kMtPolyWords
 // words each (bit b of word w: the coefficient of t^(64 w + b)).  phi is
-// found once per process (Berlekamp-Massey on the generator's low bits) and
-// the table is kept and extended as larger n are asked for.  Returns false
-// when phi does not come out at degree 19937.
+// found once per process (Berlekamp-Massey on the generator's low bits); a
+// table per step is kept and extended as larger n are asked for.  Returns
+// false when phi does not come out at degree 19937.
 bool mt_jump_polys(uint64_t step, int n, std::vector<uint64_t> &out);
 
 }  // namespace fgl

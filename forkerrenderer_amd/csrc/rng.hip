@@ -64,25 +64,40 @@ __device__ __forceinline__ float random_pm1(uint32_t u)
 // (a block's twist is this recurrence, new words used as they appear).  The
 // 227 words x_n .. x_{n+226} read only words at least 227 back, so a
 // workgroup makes them together, one barrier per step, keeping the last 1024
-// words in an LDS ring (word n in slot n % 1024): a step writes the slots of
-// words 1024 back, none of which it reads (its reads lie 227 to 624 back).
+// words in an LDS ring (word n in slot n % 1024).  In step s thread t makes
+// x_{624+j}, j = 227 s + t, from x_j and x_{j+1} (made two steps back or
+// earlier, so read during the step before, while it computes) and x_{j+397}
+// (the word the thread made in the step before).  A step writes the slots of
+// words 1024 back, none of which it or the next step's reads touch, so the
+// next step's reads are issued first and their latency overlaps the twist.
 constexpr uint32_t kMtRing = 1024, kMtLag = kMtN - kMtM;
 
 // ring holds x_0 .. x_623 (slots 0..623); the next `total` words x_624 ..
 // go through emit(j, x_{624+j}) once each, in steps of 227.  Needs at least
-// 227 threads; every thread of the workgroup calls it.
+// 227 threads; every thread of the workgroup calls it.  Past the end the
+// last step computes and stores words nobody reads, and emits only j < total.
 template <typename Emit>
 __device__ __forceinline__ void mt_run(uint32_t *ring, uint32_t total, Emit emit)
 {
+    constexpr uint32_t R = kMtRing - 1;
     const uint32_t tid = threadIdx.x;
+    const bool lane_on = tid < kMtLag;
+    uint32_t cur = 0, nxt = 0, far = 0;
+    if (lane_on) {
+        cur = ring[tid];
+        nxt = ring[tid + 1];
+        far = ring[tid + kMtM];
+    }
     for (uint32_t j0 = 0; j0 < total; j0 += kMtLag) {
-        const uint32_t j = j0 + tid;
-        if (tid < kMtLag && j < total) {
-            // x_{624+j} from x_j, x_{j+1}, x_{j+397}
-            const uint32_t v = mt_twist_word(ring[j & (kMtRing - 1)], ring[(j + 1) & (kMtRing - 1)],
-                                             ring[(j + kMtM) & (kMtRing - 1)]);
-            ring[(j + kMtN) & (kMtRing - 1)] = v;
-            emit(j, v);
+        if (lane_on) {
+            const uint32_t j = j0 + tid;
+            const uint32_t ncur = ring[(j + kMtLag) & R], nnxt = ring[(j + kMtLag + 1) & R];
+            const uint32_t v = mt_twist_word(cur, nxt, far);
+            ring[(j + kMtN) & R] = v;
+            if (j < total) emit(j, v);
+            cur = ncur;
+            nxt = nnxt;
+            far = v;
         }
         __syncthreads();
     }

@@ -219,10 +219,13 @@ __global__ __launch_bounds__(256) void k_pcss_band(FrameUniforms u, const float 
 // the same D; a ballot finds the first lane whose outcome breaks the
 // prediction, the lanes up to it are settled, and D moves by one from
 // there.  A group takes one step per broken prediction.  The masks and
-// prediction steps of 1024 points at a time are first copied to LDS (one
-// coalesced sweep), so the walk waits on LDS, not on memory.  out = (first
+// prediction steps of 1024 points at a time are copied to LDS, so the walk
+// waits on LDS, not on memory; the next block's loads are all issued into
+// registers before the walk of the current one starts, so their latency
+// (the masks come from other XCDs' caches) overlaps the walk.  out = (first
 // unsettled point, M there).
 constexpr uint32_t kResolveBlock = 1024;
+constexpr int kResolvePer = kResolveBlock / 64;   // points a lane stages per block
 
 __global__ __launch_bounds__(64) void k_pcss_resolve(const unsigned long long *__restrict__ masks,
                                                      const uint32_t *__restrict__ pred, uint32_t s0, uint32_t L,
@@ -235,13 +238,32 @@ __global__ __launch_bounds__(64) void k_pcss_resolve(const unsigned long long *_
     int D = kBandHalf;
     uint32_t base = 0, settled = 0;
     bool stop = false;
+    // the block at `at` (< L) into registers; nothing waits on these loads
+    // here.  Straight-line code (indices past the block repeat its last
+    // point; their LDS slots are never read), so no branch leaves the
+    // compiler unsure which loads are still in flight.
+    unsigned long long rm[kResolvePer];
+    uint32_t rp0[kResolvePer], rp1[kResolvePer];
+    auto fetch = [&](uint32_t at) {
+        const uint32_t last = (L - at < kResolveBlock ? L - at : kResolveBlock) - 1;
+#pragma unroll
+        for (int e = 0; e < kResolvePer; ++e) {
+            const uint32_t j0 = lane + 64u * (uint32_t)e, j = at + (j0 < last ? j0 : last);
+            rm[e] = masks[j];
+            rp0[e] = pred[j];
+            rp1[e] = pred[j + 1];   // pred[L]: the segment's predicted total
+        }
+    };
+    if (L > 0) fetch(0);
     while (base < L && !stop) {
         const uint32_t nb = L - base < kResolveBlock ? L - base : kResolveBlock;
-        for (uint32_t j = lane; j < nb; j += 64) {
-            sm[j] = masks[base + j];
-            sdl[j] = (int)(pred[base + j + 1] - pred[base + j]);   // pred[L]: the segment's predicted total
+#pragma unroll
+        for (int e = 0; e < kResolvePer; ++e) {
+            sm[lane + 64u * (uint32_t)e] = rm[e];
+            sdl[lane + 64u * (uint32_t)e] = (int)(rp1[e] - rp0[e]);
         }
         __syncthreads();
+        if (base + nb < L) fetch(base + nb);
         for (uint32_t g = 0; g < nb && !stop; g += 64) {
             const uint32_t cnt = nb - g < 64 ? nb - g : 64;
             const unsigned long long m = lane < cnt ? sm[g + lane] : 0ull;

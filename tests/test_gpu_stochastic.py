@@ -42,8 +42,9 @@ def test_device_stream_matches_the_oracle(ctx):
 
 @pytest.mark.parametrize("seed", [5489, 77])
 def test_device_stream_in_parallel_chunks_matches_numpy(ctx, seed):
-    """Requests of more than one chunk (512 blocks, 319,488 draws) run as
-    parallel chunks, each reached by jumping ahead from the context's array:
+    """Requests of more than one chunk (256 blocks, 159,744 draws, while 256
+    such chunks cover the request) run as parallel chunks, each reached by
+    jumping ahead from the context's array:
     vectors bit-exact with numpy's MT19937 through the reference's rejection
     loops, draw counts equal, and the stream continues exactly from the
     array the commit rebuilds (small requests before, between and after,
@@ -58,6 +59,25 @@ def test_device_stream_in_parallel_chunks_matches_numpy(ctx, seed):
         assert_bitexact(f"kind {kind} n {n}", got, want)
         at += used
         assert ctx.rng_draws() == at, (kind, n, ctx.rng_draws(), at)
+    ctx.rng_seed(5489)
+
+
+def test_device_stream_long_chunks_match_numpy(ctx):
+    """A request of more than 256 chunks of 256 blocks (17 M disk vectors,
+    ~43 M draws) runs as chunks of 512 blocks, jumping with their own
+    polynomials: vectors bit-exact with numpy's MT19937, the draw count equal,
+    and the stream continuing exactly after it."""
+    ctx.rng_seed(5489)
+    raw = mt_raw(44_000_000)
+    at = 0
+    for kind, n in ((2, 3), (2, 17_000_000), (3, 5)):
+        print(f"kind {kind} n {n}", flush=True)
+        got = ctx.rng_vectors(kind, n)
+        want, used = (stream_sphere if kind == 3 else stream_disk)(raw[at:], n)
+        assert_bitexact(f"kind {kind} n {n}", got, want)
+        at += used
+        assert ctx.rng_draws() == at, (kind, n, ctx.rng_draws(), at)
+    assert at > 256 * 256 * 624, at
     ctx.rng_seed(5489)
 
 
