@@ -303,25 +303,35 @@ __global__ __launch_bounds__(64) void k_pcss_resolve(const unsigned long long *_
         if (base + nb < L) fetch(base + nb);
         for (uint32_t g = 0; g < nb && !stop; g += 64) {
             const uint32_t cnt = nb - g < 64 ? nb - g : 64;
-            const unsigned long long m = lane < cnt ? sm[g + lane] : 0ull;
-            const int dl = lane < cnt ? sdl[g + lane] : 0;
+            const unsigned long long valid = cnt == 64 ? ~0ull : ((1ull << cnt) - 1ull);
+            // With the prediction step dl in {0, 1}, a point breaks the
+            // prediction at D exactly when its outcome there equals dl (a
+            // blocker found where a miss was predicted, or the reverse):
+            // bm marks those D.  D then moves by +1 (dl = 0) or -1 (dl = 1);
+            // a point settled at a break has outcome dl, every other settled
+            // point 1 - dl.
+            const unsigned long long m = sm[g + lane];
+            const int dl = sdl[g + lane];
+            const unsigned long long bm = dl ? m : ~m;
+            const unsigned long long dlm = __ballot(dl != 0);
             uint32_t start = 0;
-            unsigned long long fb = 0;
+            unsigned long long brks = 0;
             while (start < cnt) {
                 if (D < 0 || D > 2 * kBandHalf) {
                     stop = true;
                     break;
                 }
-                const uint32_t bit = (uint32_t)(m >> D) & 1u;
-                const int dev = (int)(1u - bit) - dl;
-                const unsigned long long brk = __ballot(lane >= start && lane < cnt && dev != 0);
-                const uint32_t j = brk ? (uint32_t)__builtin_ctzll(brk) : cnt - 1;
-                const unsigned long long span =
-                    (j == 63 ? ~0ull : ((1ull << (j + 1)) - 1ull)) & ~((1ull << start) - 1ull);
-                fb |= __ballot(bit) & span;
-                if (brk) D += __builtin_amdgcn_readlane(dev, j);
+                const unsigned long long b = __ballot((uint32_t)(bm >> D) & 1u) & valid & (~0ull << start);
+                if (!b) {
+                    start = cnt;
+                    break;
+                }
+                const uint32_t j = (uint32_t)__builtin_ctzll(b);
+                brks |= 1ull << j;
+                D += ((dlm >> j) & 1ull) ? -1 : 1;
                 start = j + 1;
             }
+            const unsigned long long fb = ~(dlm ^ brks);
             if (lane < start) found[s0 + base + g + lane] = (uint32_t)(fb >> lane) & 1u;
             settled = base + g + start;
         }
