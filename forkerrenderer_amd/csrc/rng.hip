@@ -156,32 +156,55 @@ __global__ __launch_bounds__(256) void k_mt_base(const MtState *__restrict__ st,
 }
 
 // Workgroup c - 1 jumps to chunk c's array: states[c][k] = XOR of base[i + k]
-// over the set bits i of polynomial c, the base sequence in LDS (82 KB), a
-// thread per word k.  The set bits are first listed in LDS (16-bit
-// positions, in order: per-word counts scanned by one wavefront), so every
-// wavefront reads eight positions with one broadcast load and spends no
-// scalar work per bit.
-__global__ __launch_bounds__(640) void k_mt_jump(const uint32_t *__restrict__ base, const uint64_t *__restrict__ polys,
+// over the set bits i of polynomial c, the base sequence in LDS.  The set
+// bits are split by i mod 4: for the bits i = 4 q + r, thread t gathers the
+// 16-byte group base[4 (q + t) .. 4 (q + t) + 3], words k + r of the result
+// for k + r in [4 t, 4 t + 4), so each bit costs a thread one 16-byte LDS
+// read; three wavefronts per class, so each class's chain of dependent LDS
+// reads runs beside the others; the four partial arrays are combined at the
+// end (word k = XOR over r of partial r at k + r).  Each class's q values are listed in LDS (per-word
+// counts scanned by one wavefront per class), padded to a multiple of eight
+// with a q whose groups are zero, so every wavefront reads eight q values
+// with one broadcast load.
+constexpr int kJumpGroups = ((int)kMtN + 3 + 3) / 4;                       // 157: k + r < 627
+constexpr int kJumpClassThreads = 192;                                     // three wavefronts per class
+constexpr int kJumpThreads = 4 * kJumpClassThreads;
+static_assert(kJumpGroups <= kJumpClassThreads, "a thread per 16-byte group of a class");
+constexpr int kJumpZeroQ = kMtBaseWords / 4;                               // groups past the base: zero
+constexpr int kJumpLdsWords = kMtBaseWords + 4 * kJumpGroups;              // base, then the zero groups
+constexpr int kJumpListMax = kMtPolyWords * 64 + 4 * 7 + 8;                // every bit, padding, one read ahead
+static_assert(kMtBaseWords % 4 == 0 && kJumpZeroQ < 65536, "16-bit q values; aligned zero groups");
+static_assert(4 * (kMtDegree / 4 + kJumpGroups - 1) + 3 < kMtBaseWords, "the groups a bit reads lie in the base");
+
+__global__ __launch_bounds__(kJumpThreads) void k_mt_jump(const uint32_t *__restrict__ base, const uint64_t *__restrict__ polys,
                                                  uint32_t *__restrict__ states)
 {
-    __shared__ uint32_t b[kMtBaseWords];
-    __shared__ __align__(16) uint16_t ix[kMtPolyWords * 64];
-    __shared__ uint32_t off[kMtPolyWords + 1];
-    const int tid = threadIdx.x, lane = tid & 63;
+    __shared__ __align__(16) uint32_t b[kJumpLdsWords];
+    __shared__ __align__(16) uint16_t qs[kJumpListMax];
+    __shared__ uint32_t off[4][kMtPolyWords];
+    __shared__ uint32_t ncls[4], cls[5];                // class sizes; starts (multiples of 8), cls[4] = the end
+    __shared__ uint32_t part[4][4 * kJumpGroups];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int c = blockIdx.x + 1;
     const uint64_t *g = polys + (size_t)(c - 1) * kMtPolyWords;
-    for (int i = tid; i < kMtBaseWords; i += 640) b[i] = base[i];
-    const uint64_t w = tid < kMtPolyWords ? g[tid] : 0ull;
-    if (tid < kMtPolyWords) off[tid] = (uint32_t)__popcll(w);
+    constexpr uint64_t kRes = 0x1111111111111111ull;   // bits i with i mod 4 = 0
+    for (int i = tid; i < kMtBaseWords / 4; i += kJumpThreads) ((uint4 *)b)[i] = ((const uint4 *)base)[i];
+    for (int i = tid; i < kJumpGroups; i += kJumpThreads) ((uint4 *)b)[kJumpZeroQ + i] = make_uint4(0u, 0u, 0u, 0u);
+    for (int w = tid; w < kMtPolyWords; w += kJumpThreads) {
+        const uint64_t v = g[w];
+#pragma unroll
+        for (int r = 0; r < 4; ++r) off[r][w] = (uint32_t)__popcll(v & (kRes << r));
+    }
     __syncthreads();
-    if (tid < 64) {
-        // exclusive scan of the 312 counts: 5 words a lane, then across lanes
+    if (wave < 4) {
+        // wavefront r: exclusive scan of class r's 312 counts, 5 words a lane
         constexpr int kPer = (kMtPolyWords + 63) / 64;
+        const int r = wave;
         uint32_t v[kPer], sum = 0;
 #pragma unroll
         for (int e = 0; e < kPer; ++e) {
             const int q = lane * kPer + e;
-            v[e] = q < kMtPolyWords ? off[q] : 0u;
+            v[e] = q < kMtPolyWords ? off[r][q] : 0u;
             sum += v[e];
         }
         uint32_t incl = sum;
@@ -194,36 +217,62 @@ __global__ __launch_bounds__(640) void k_mt_jump(const uint32_t *__restrict__ ba
 #pragma unroll
         for (int e = 0; e < kPer; ++e) {
             const int q = lane * kPer + e;
-            if (q < kMtPolyWords) off[q] = run;
+            if (q < kMtPolyWords) off[r][q] = run;
             run += v[e];
         }
-        if (lane == 63) off[kMtPolyWords] = incl;
+        if (lane == 63) ncls[r] = incl;
     }
     __syncthreads();
-    if (tid < kMtPolyWords) {
-        uint32_t pos = off[tid];
-        uint64_t bits = w;
-        while (bits) {
-            ix[pos++] = (uint16_t)(tid * 64 + __builtin_ctzll(bits));
-            bits &= bits - 1;
+    if (tid == 0) {
+        uint32_t at = 0;
+        for (int r = 0; r < 4; ++r) {
+            cls[r] = at;
+            at += (ncls[r] + 7u) & ~7u;
+        }
+        cls[4] = at;
+    }
+    __syncthreads();
+    for (int w = tid; w < kMtPolyWords; w += kJumpThreads) {
+        const uint64_t v = g[w];
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            uint32_t pos = cls[r] + off[r][w];
+            uint64_t bits = v & (kRes << r);
+            while (bits) {
+                qs[pos++] = (uint16_t)((w * 64 + __builtin_ctzll(bits)) >> 2);
+                bits &= bits - 1;
+            }
+        }
+    }
+    if (tid < 4)   // pad class tid to its rounded length with the zero groups
+        for (uint32_t e = cls[tid] + ncls[tid]; e < cls[tid + 1]; ++e) qs[e] = (uint16_t)kJumpZeroQ;
+    __syncthreads();
+    const int r = tid / kJumpClassThreads, t = tid - r * kJumpClassThreads;
+    if (t < kJumpGroups) {
+        const uint4 *b4 = (const uint4 *)b + t;
+        {
+            uint4 acc = make_uint4(0u, 0u, 0u, 0u);
+            const uint4 *q8 = (const uint4 *)(qs + cls[r]);
+            const uint32_t n8 = (cls[r + 1] - cls[r]) / 8;
+            uint4 v = q8[0];
+            for (uint32_t e = 0; e < n8; ++e) {
+                const uint4 next = q8[e + 1];   // read ahead (past the class: unused)
+                const uint4 a0 = b4[v.x & 0xffffu], a1 = b4[v.x >> 16];
+                const uint4 a2 = b4[v.y & 0xffffu], a3 = b4[v.y >> 16];
+                const uint4 a4 = b4[v.z & 0xffffu], a5 = b4[v.z >> 16];
+                const uint4 a6 = b4[v.w & 0xffffu], a7 = b4[v.w >> 16];
+                acc.x ^= ((a0.x ^ a1.x) ^ (a2.x ^ a3.x)) ^ ((a4.x ^ a5.x) ^ (a6.x ^ a7.x));
+                acc.y ^= ((a0.y ^ a1.y) ^ (a2.y ^ a3.y)) ^ ((a4.y ^ a5.y) ^ (a6.y ^ a7.y));
+                acc.z ^= ((a0.z ^ a1.z) ^ (a2.z ^ a3.z)) ^ ((a4.z ^ a5.z) ^ (a6.z ^ a7.z));
+                acc.w ^= ((a0.w ^ a1.w) ^ (a2.w ^ a3.w)) ^ ((a4.w ^ a5.w) ^ (a6.w ^ a7.w));
+                v = next;
+            }
+            ((uint4 *)part[r])[t] = acc;
         }
     }
     __syncthreads();
-    if (tid >= (int)kMtN) return;
-    const uint32_t T = off[kMtPolyWords];
-    const uint4 *ix8 = (const uint4 *)ix;
-    uint32_t acc = 0;
-    uint32_t r = 0;
-    for (; r + 8 <= T; r += 8) {
-        const uint4 v = ix8[r / 8];
-        const uint32_t a0 = b[(v.x & 0xffffu) + tid], a1 = b[(v.x >> 16) + tid];
-        const uint32_t a2 = b[(v.y & 0xffffu) + tid], a3 = b[(v.y >> 16) + tid];
-        const uint32_t a4 = b[(v.z & 0xffffu) + tid], a5 = b[(v.z >> 16) + tid];
-        const uint32_t a6 = b[(v.w & 0xffffu) + tid], a7 = b[(v.w >> 16) + tid];
-        acc ^= ((a0 ^ a1) ^ (a2 ^ a3)) ^ ((a4 ^ a5) ^ (a6 ^ a7));
-    }
-    for (; r < T; ++r) acc ^= b[ix[r] + tid];
-    states[(size_t)c * kMtN + tid] = acc;
+    for (int k = tid; k < (int)kMtN; k += kJumpThreads)
+        states[(size_t)c * kMtN + k] = part[0][k] ^ part[1][k + 1] ^ part[2][k + 2] ^ part[3][k + 3];
 }
 
 // Workgroup c generates chunk c: K blocks of tempered draws at
@@ -322,7 +371,7 @@ void launch_mt_base(const MtState *st, uint32_t *base, hipStream_t s)
 void launch_mt_jump(const uint32_t *base, const uint64_t *polys, int n_jumps, uint32_t *states, hipStream_t s)
 {
     if (n_jumps <= 0) return;
-    hipLaunchKernelGGL(k_mt_jump, dim3(n_jumps), dim3(640), 0, s, base, polys, states);
+    hipLaunchKernelGGL(k_mt_jump, dim3(n_jumps), dim3(kJumpThreads), 0, s, base, polys, states);
 }
 
 void launch_mt_gen(const MtState *st, const uint32_t *states, int n_chunks, int K, uint32_t *raw, hipStream_t s)
