@@ -824,8 +824,9 @@ static int rng_fill(fgl_ctx *c, int D, float *out, uint32_t n)
         if (len + 624 >= 0xffffffffull || Nc > 0x7fffffffull)
             return fail(c, FGL_ERR_CAPACITY, "random stream: too many draws in one request");
         const uint32_t T = (uint32_t)(len / (uint64_t)D);
+        const uint32_t nb = mt_try_blocks(T);
         if ((rc = ensure(c, c->mt_raw, (len + 624) * 4)) ||
-            (rc = ensure(c, c->mt_pos, ((size_t)T + 1 + scan_tmp_words(T)) * 4)))
+            (rc = ensure(c, c->mt_pos, ((size_t)nb + 1 + scan_tmp_words(nb)) * 4)))
             return rc;
         MtState *st = (MtState *)c->rng.p;
         uint32_t *raw = (uint32_t *)c->mt_raw.p, *pos = (uint32_t *)c->mt_pos.p;
@@ -851,9 +852,9 @@ static int rng_fill(fgl_ctx *c, int D, float *out, uint32_t n)
             states = (const uint32_t *)c->mt_states.p;
         }
         launch_mt_gen(st, states, (int)Nc, K, raw, c->stream);
-        launch_mt_flags(D, raw, T, pos, c->stream);
-        launch_exclusive_scan(pos, pos, T, pos + T + 1, c->stream);
-        HIPCHK(c, hipMemcpyAsync(c->h_total, pos + T, 4, hipMemcpyDeviceToHost, c->stream));
+        launch_mt_count(D, raw, T, pos, c->stream);
+        launch_exclusive_scan(pos, pos, nb, pos + nb + 1, c->stream);
+        HIPCHK(c, hipMemcpyAsync(c->h_total, pos + nb, 4, hipMemcpyDeviceToHost, c->stream));
         HIPCHK(c, hipStreamSynchronize(c->stream));
         if (c->h_total[0] >= n) {
             launch_mt_take(D, raw, T, pos, n, out, c->stream);

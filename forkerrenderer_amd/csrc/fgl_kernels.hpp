@@ -172,13 +172,17 @@ void launch_mt_base(const MtState *st, uint32_t *base, hipStream_t s);
 void launch_mt_jump(const uint32_t *base, const uint64_t *polys, int n_jumps, uint32_t *states, hipStream_t s);
 void launch_mt_gen(const MtState *st, const uint32_t *states, int n_chunks, int K, uint32_t *raw, hipStream_t s);
 // Rejection-sampled unit vectors from raw's T tries of D draws (D = 3:
-// RandomVectorInUnitSphere (x, y, z); 2: RandomVectorInUnitDisk (x, y)):
-// flag[t] = try t is kept; after an exclusive scan of the flags into pos,
-// the first n kept vectors go to out in stream order, and a commit moves the
-// stream past the try that kept the m-th.
-void launch_mt_flags(int D, const uint32_t *raw, uint32_t T, uint32_t *flag, hipStream_t s);
-void launch_mt_take(int D, const uint32_t *raw, uint32_t T, const uint32_t *pos, uint32_t n, float *out, hipStream_t s);
-void launch_mt_commit(int D, MtState *st, const uint32_t *raw, const uint32_t *pos, uint32_t T, uint32_t m,
+// RandomVectorInUnitSphere (x, y, z); 2: RandomVectorInUnitDisk (x, y)),
+// kMtTryTile tries to a workgroup: part[b] = workgroup b's count of kept
+// tries (mt_try_blocks(T) of them); after an exclusive scan of part (the
+// total at part[nb]) the first n kept vectors go to out in stream order, and
+// a commit moves the stream past the try that kept the m-th.
+constexpr uint32_t kMtTryTile = 2048;
+uint32_t mt_try_blocks(uint32_t T);
+void launch_mt_count(int D, const uint32_t *raw, uint32_t T, uint32_t *part, hipStream_t s);
+void launch_mt_take(int D, const uint32_t *raw, uint32_t T, const uint32_t *part, uint32_t n, float *out,
+                    hipStream_t s);
+void launch_mt_commit(int D, MtState *st, const uint32_t *raw, const uint32_t *part, uint32_t T, uint32_t m,
                       hipStream_t s);
 
 // shade.hip

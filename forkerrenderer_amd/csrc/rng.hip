@@ -294,53 +294,151 @@ __global__ __launch_bounds__(256) void k_mt_gen(const MtState *__restrict__ st, 
     mt_run(ring, (uint32_t)K * kMtN, [&](uint32_t j, uint32_t v) { out[j] = mt_temper(v); });
 }
 
+// Tries are taken 2048 to a workgroup, 8 consecutive ones to a thread (its
+// 8 D draws read as 16-byte loads): k_mt_count writes each workgroup's count
+// of kept tries, a scan of the counts gives each workgroup its first
+// vector's index, and k_mt_take recounts its own tries and writes the kept
+// vectors in order.  No per-try array is written.
+constexpr uint32_t kTryItems = 8, kTryTile = kTryItems * 256;
+static_assert(kTryTile == kMtTryTile, "tries per workgroup");
+
+// The kept flags (bit k) of tries t0 .. t0 + 7 (those below T), their
+// vectors in xyz[3 k ..].
 template <int D>
-__global__ __launch_bounds__(256) void k_mt_flags(const uint32_t *__restrict__ raw, uint32_t T,
-                                                  uint32_t *__restrict__ flag)
+__device__ __forceinline__ uint32_t mt_tries8(const uint32_t *__restrict__ raw, uint32_t T, uint32_t t0, float *xyz)
 {
-    const uint32_t t = blockIdx.x * 256 + threadIdx.x;
-    if (t >= T) return;
-    float x, y, z;
-    flag[t] = mt_try<D>(raw + (size_t)D * t, x, y, z) ? 1u : 0u;
+    uint32_t w[kTryItems * D];
+    if (t0 + kTryItems <= T) {
+        // D t0 words from a 16-byte aligned array: t0 is a multiple of 8
+        const uint4 *p = (const uint4 *)(raw + (size_t)D * t0);
+#pragma unroll
+        for (int e = 0; e < 2 * D; ++e) {
+            const uint4 q = p[e];
+            w[4 * e] = q.x;
+            w[4 * e + 1] = q.y;
+            w[4 * e + 2] = q.z;
+            w[4 * e + 3] = q.w;
+        }
+    } else {
+#pragma unroll
+        for (int e = 0; e < (int)kTryItems * D; ++e) w[e] = t0 + (uint32_t)(e / D) < T ? raw[(size_t)D * t0 + e] : 0u;
+    }
+    uint32_t kept = 0;
+#pragma unroll
+    for (int k = 0; k < (int)kTryItems; ++k) {
+        float x, y, z;
+        const bool ok = mt_try<D>(w + D * k, x, y, z);
+        xyz[3 * k] = x;
+        xyz[3 * k + 1] = y;
+        xyz[3 * k + 2] = z;
+        if (ok && t0 + (uint32_t)k < T) kept |= 1u << k;
+    }
+    return kept;
 }
 
-// pos: exclusive scan of the flags (pos[T] = the kept tries); the first n
-// kept vectors to out in stream order.
+// Exclusive scan of one value per thread over a 256-thread workgroup.
+__device__ __forceinline__ uint32_t try_block_scan(uint32_t v, uint32_t *lds4, uint32_t &total)
+{
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    uint32_t inc = v;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const uint32_t y = __shfl_up(inc, o, 64);
+        if (lane >= o) inc += y;
+    }
+    if (lane == 63) lds4[wave] = inc;
+    __syncthreads();
+    uint32_t prefix = 0, tot = 0;
+#pragma unroll
+    for (int w = 0; w < 4; ++w) {
+        const uint32_t c = lds4[w];
+        if (w < wave) prefix += c;
+        tot += c;
+    }
+    __syncthreads();
+    total = tot;
+    return prefix + inc - v;
+}
+
+template <int D>
+__global__ __launch_bounds__(256) void k_mt_count(const uint32_t *__restrict__ raw, uint32_t T,
+                                                  uint32_t *__restrict__ part)
+{
+    __shared__ uint32_t lds4[4];
+    float xyz[3 * kTryItems];
+    const uint32_t t0 = blockIdx.x * kTryTile + threadIdx.x * kTryItems;
+    const uint32_t c = (uint32_t)__popc(mt_tries8<D>(raw, T, t0, xyz));
+    uint32_t total;
+    try_block_scan(c, lds4, total);
+    if (threadIdx.x == 0) part[blockIdx.x] = total;
+}
+
+// part: exclusive scan of the workgroups' counts (part[nb] = every kept
+// try); the first n kept vectors to out in stream order.
 template <int D>
 __global__ __launch_bounds__(256) void k_mt_take(const uint32_t *__restrict__ raw, uint32_t T,
-                                                 const uint32_t *__restrict__ pos, uint32_t n, float *__restrict__ out)
+                                                 const uint32_t *__restrict__ part, uint32_t n, float *__restrict__ out)
 {
-    const uint32_t t = blockIdx.x * 256 + threadIdx.x;
-    if (t >= T) return;
-    const uint32_t p = pos[t];
-    if (pos[t + 1] == p || p >= n) return;
-    float x, y, z;
-    mt_try<D>(raw + (size_t)D * t, x, y, z);
-    float *o = out + (size_t)p * D;
-    o[0] = x;
-    o[1] = y;
-    if (D == 3) o[2] = z;
+    __shared__ uint32_t lds4[4];
+    const uint32_t first = part[blockIdx.x];
+    if (first >= n) return;   // the whole workgroup: its vectors are not asked for
+    float xyz[3 * kTryItems];
+    const uint32_t t0 = blockIdx.x * kTryTile + threadIdx.x * kTryItems;
+    const uint32_t kept = mt_tries8<D>(raw, T, t0, xyz);
+    uint32_t total;
+    uint32_t p = first + try_block_scan((uint32_t)__popc(kept), lds4, total);
+#pragma unroll
+    for (int k = 0; k < (int)kTryItems; ++k)
+        if ((kept >> k) & 1u) {
+            if (p < n) {
+                float *o = out + (size_t)p * D;
+                o[0] = xyz[3 * k];
+                o[1] = xyz[3 * k + 1];
+                if (D == 3) o[2] = xyz[3 * k + 2];
+            }
+            ++p;
+        }
 }
 
 // The stream moves past the try that kept the m-th vector (m >= 1, at most
-// pos[T]): the draws up to it are consumed, and the array becomes the block
-// holding the next draw (its words untempered from raw), idx its position
-// there -- 624 when the next draw starts a new block, as the generator
-// itself leaves it.
-__global__ __launch_bounds__(256) void k_mt_commit(MtState *__restrict__ st, int D, const uint32_t *__restrict__ raw,
-                                                   const uint32_t *__restrict__ pos, uint32_t T, uint32_t m)
+// part[nb]): the draws up to it are consumed, and the array becomes the
+// block holding the next draw (its words untempered from raw), idx its
+// position there -- 624 when the next draw starts a new block, as the
+// generator itself leaves it.  The workgroup holding the m-th kept try is
+// found by a binary search over part, the try by recounting its tries.
+template <int D>
+__global__ __launch_bounds__(256) void k_mt_commit(MtState *__restrict__ st, const uint32_t *__restrict__ raw,
+                                                   const uint32_t *__restrict__ part, uint32_t nb, uint32_t T,
+                                                   uint32_t m)
 {
-    __shared__ uint32_t s_t;
+    __shared__ uint32_t s_b, s_t;
+    __shared__ uint32_t lds4[4];
     const int tid = threadIdx.x;
     const uint32_t idx0 = st->idx;
     if (tid == 0) {
-        uint32_t lo = 0, hi = T - 1;   // the smallest t with pos[t + 1] >= m
+        uint32_t lo = 0, hi = nb - 1;   // the smallest b with part[b + 1] >= m
         while (lo < hi) {
             const uint32_t mid = lo + (hi - lo) / 2;
-            if (pos[mid + 1] >= m) hi = mid;
+            if (part[mid + 1] >= m) hi = mid;
             else lo = mid + 1;
         }
-        s_t = lo;
+        s_b = lo;
+    }
+    __syncthreads();
+    {
+        float xyz[3 * kTryItems];
+        const uint32_t t0 = s_b * kTryTile + (uint32_t)tid * kTryItems;
+        const uint32_t kept = mt_tries8<D>(raw, T, t0, xyz);
+        const uint32_t cnt = (uint32_t)__popc(kept);
+        uint32_t total;
+        const uint32_t ex = part[s_b] + try_block_scan(cnt, lds4, total);
+        if (ex < m && m <= ex + cnt) {
+            // the (m - ex)-th kept try of this thread's eight
+            uint32_t j = m - ex, k = 0;
+            for (; k < kTryItems; ++k)
+                if (((kept >> k) & 1u) && --j == 0) break;
+            s_t = t0 + k;
+        }
     }
     __syncthreads();
     const unsigned long long e = (unsigned long long)D * (s_t + 1ull);
@@ -379,25 +477,30 @@ void launch_mt_gen(const MtState *st, const uint32_t *states, int n_chunks, int 
     hipLaunchKernelGGL(k_mt_gen, dim3(n_chunks), dim3(256), 0, s, st, states, K, raw);
 }
 
-void launch_mt_flags(int D, const uint32_t *raw, uint32_t T, uint32_t *flag, hipStream_t s)
+uint32_t mt_try_blocks(uint32_t T) { return (T + kTryTile - 1) / kTryTile; }
+
+void launch_mt_count(int D, const uint32_t *raw, uint32_t T, uint32_t *part, hipStream_t s)
 {
     if (T == 0) return;
-    if (D == 3) hipLaunchKernelGGL(k_mt_flags<3>, dim3(blocks256(T)), dim3(256), 0, s, raw, T, flag);
-    else hipLaunchKernelGGL(k_mt_flags<2>, dim3(blocks256(T)), dim3(256), 0, s, raw, T, flag);
+    if (D == 3) hipLaunchKernelGGL(k_mt_count<3>, dim3(mt_try_blocks(T)), dim3(256), 0, s, raw, T, part);
+    else hipLaunchKernelGGL(k_mt_count<2>, dim3(mt_try_blocks(T)), dim3(256), 0, s, raw, T, part);
 }
 
-void launch_mt_take(int D, const uint32_t *raw, uint32_t T, const uint32_t *pos, uint32_t n, float *out, hipStream_t s)
+void launch_mt_take(int D, const uint32_t *raw, uint32_t T, const uint32_t *part, uint32_t n, float *out,
+                    hipStream_t s)
 {
     if (T == 0 || n == 0) return;
-    if (D == 3) hipLaunchKernelGGL(k_mt_take<3>, dim3(blocks256(T)), dim3(256), 0, s, raw, T, pos, n, out);
-    else hipLaunchKernelGGL(k_mt_take<2>, dim3(blocks256(T)), dim3(256), 0, s, raw, T, pos, n, out);
+    if (D == 3) hipLaunchKernelGGL(k_mt_take<3>, dim3(mt_try_blocks(T)), dim3(256), 0, s, raw, T, part, n, out);
+    else hipLaunchKernelGGL(k_mt_take<2>, dim3(mt_try_blocks(T)), dim3(256), 0, s, raw, T, part, n, out);
 }
 
-void launch_mt_commit(int D, MtState *st, const uint32_t *raw, const uint32_t *pos, uint32_t T, uint32_t m,
+void launch_mt_commit(int D, MtState *st, const uint32_t *raw, const uint32_t *part, uint32_t T, uint32_t m,
                       hipStream_t s)
 {
     if (T == 0 || m == 0) return;
-    hipLaunchKernelGGL(k_mt_commit, dim3(1), dim3(256), 0, s, st, D, raw, pos, T, m);
+    const uint32_t nb = mt_try_blocks(T);
+    if (D == 3) hipLaunchKernelGGL(k_mt_commit<3>, dim3(1), dim3(256), 0, s, st, raw, part, nb, T, m);
+    else hipLaunchKernelGGL(k_mt_commit<2>, dim3(1), dim3(256), 0, s, st, raw, part, nb, T, m);
 }
 
 }  // namespace fgl
