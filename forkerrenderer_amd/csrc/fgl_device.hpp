@@ -305,18 +305,16 @@ __device__ __forceinline__ float pcf_disk(const float *map, int SW, int SH, Vec3
 }
 
 // PCSS blocker term (FindAverageBlockDepth, shadow.cpp:65-91): the average
-// depth of the samples that block, 0 when none does.  SCALED: `disk` holds
-// the offsets already scaled (blocker_off), else unit-disk samples scaled
-// here in double as the reference does.
-template <bool SCALED>
+// depth of the samples that block, 0 when none does.  `disk` holds the
+// offsets already scaled (blocker_off: the unit-disk samples times
+// blocker_filter in double, as the reference scales them).
 __device__ __forceinline__ float pcss_blocker_depth(const FrameUniforms &u, const float *map, Vec3 sc, float bias,
                                                     const float *disk, int n)
 {
-    const DevShadowCfg &c = u.shadow;
     float blocker = 0.f, nb = 0.f;
     for (int i = 0; i < n; ++i) {
-        const float ou = SCALED ? disk[2 * i] : (float)((double)disk[2 * i] * c.blocker_filter);
-        const float ov = SCALED ? disk[2 * i + 1] : (float)((double)disk[2 * i + 1] * c.blocker_filter);
+        const float ou = disk[2 * i];
+        const float ov = disk[2 * i + 1];
         float su = sc.x + ou;
         float sv = sc.y + ov;
         float s = sample_shadow(map, u.SW, u.SH, su, sv);
@@ -326,6 +324,99 @@ __device__ __forceinline__ float pcss_blocker_depth(const FrameUniforms &u, cons
         }
     }
     return (nb < 1.f) ? 0.f : blocker / nb;
+}
+
+// The random stream's samples (modes 4 and 5) are each pixel's own, 512
+// bytes or more apart between neighbouring lanes: read one sample per load,
+// every load would fetch a cache line per lane.  So they are taken
+// kStreamGroup at a time: the group's pairs, then its texels, are loaded
+// before any is used (a texel of a sample off the map is read at index 0 and
+// replaced by +inf, as sample_shadow returns it), and a pixel's lines are
+// used while they are cached.  The arithmetic and its order per sample are
+// those of the reference's FindAverageBlockDepth and PCF (pcss_blocker_depth
+// with the offsets scaled per sample, pcf_disk).
+constexpr int kStreamGroup = 8;
+
+// Shadow-map texel of (su, sv) as sample_shadow indexes it; in = on the map.
+__device__ __forceinline__ size_t shadow_texel(int SW, int w, int h, float su, float sv, bool &in)
+{
+    in = su >= 0.f && su <= 1.f && sv >= 0.f && sv <= 1.f;
+    const int x = in ? (int)((float)w * su) : 0, y = in ? (int)((float)h * sv) : 0;
+    return (size_t)x + (size_t)y * SW;
+}
+
+__device__ __forceinline__ float shadow_value(bool in, float dd) { return !in ? __builtin_inff() : ((dd < 0.001f) ? 1.f : dd); }
+
+// pcss_blocker_depth over stream samples (unit-disk samples scaled here in
+// double, as the reference scales them).
+__device__ __forceinline__ float pcss_blocker_stream(const FrameUniforms &u, const float *map, Vec3 sc, float bias,
+                                                     const float *disk, int n)
+{
+    const DevShadowCfg &c = u.shadow;
+    const float2 *d = (const float2 *)disk;
+    const int w = (int)((float)u.SW - 0.001f), h = (int)((float)u.SH - 0.001f);
+    float blocker = 0.f, nb = 0.f;
+    int i = 0;
+    for (; i + kStreamGroup <= n; i += kStreamGroup) {
+        float2 ds[kStreamGroup];
+#pragma unroll
+        for (int e = 0; e < kStreamGroup; ++e) ds[e] = d[i + e];
+        float dd[kStreamGroup];
+        bool in[kStreamGroup];
+#pragma unroll
+        for (int e = 0; e < kStreamGroup; ++e) {
+            const float su = sc.x + (float)((double)ds[e].x * c.blocker_filter);
+            const float sv = sc.y + (float)((double)ds[e].y * c.blocker_filter);
+            dd[e] = map[shadow_texel(u.SW, w, h, su, sv, in[e])];
+        }
+#pragma unroll
+        for (int e = 0; e < kStreamGroup; ++e) {
+            const float s = shadow_value(in[e], dd[e]);
+            if (sc.z > s + bias) {
+                blocker += s;
+                nb += 1.f;
+            }
+        }
+    }
+    for (; i < n; ++i) {
+        const float su = sc.x + (float)((double)d[i].x * c.blocker_filter);
+        const float sv = sc.y + (float)((double)d[i].y * c.blocker_filter);
+        const float s = sample_shadow(map, u.SW, u.SH, su, sv);
+        if (sc.z > s + bias) {
+            blocker += s;
+            nb += 1.f;
+        }
+    }
+    return (nb < 1.f) ? 0.f : blocker / nb;
+}
+
+// pcf_disk over stream samples.
+__device__ __forceinline__ float pcf_stream(const float *map, int SW, int SH, Vec3 sc, float bias, float filter,
+                                            const float *disk, int n)
+{
+    const float2 *d = (const float2 *)disk;
+    const int w = (int)((float)SW - 0.001f), h = (int)((float)SH - 0.001f);
+    float vis = 0.f;
+    const float inv = 1.f / (float)n;
+    int i = 0;
+    for (; i + kStreamGroup <= n; i += kStreamGroup) {
+        float2 ds[kStreamGroup];
+#pragma unroll
+        for (int e = 0; e < kStreamGroup; ++e) ds[e] = d[i + e];
+        float dd[kStreamGroup];
+        bool in[kStreamGroup];
+#pragma unroll
+        for (int e = 0; e < kStreamGroup; ++e)
+            dd[e] = map[shadow_texel(SW, w, h, sc.x + ds[e].x * filter, sc.y + ds[e].y * filter, in[e])];
+#pragma unroll
+        for (int e = 0; e < kStreamGroup; ++e)
+            if (sc.z <= shadow_value(in[e], dd[e]) + bias) vis += inv;
+    }
+    for (; i < n; ++i) {
+        const float s = sample_shadow(map, SW, SH, sc.x + d[i].x * filter, sc.y + d[i].y * filter);
+        if (sc.z <= s + bias) vis += inv;
+    }
+    return vis;
 }
 
 // Shadow coordinate and slope bias (CalculateShadowVisibility, shadow.cpp:109-117).
@@ -341,17 +432,17 @@ __device__ __forceinline__ float shadow_visibility(const FrameUniforms &u, const
     Vec3 sc = shadow_coord(ndc);
     float bias = shadow_bias(n, l);
     if (c.mode == 1) return pcf_disk(map, u.SW, u.SH, sc, bias, c.pcf_filter, c.pcf_disk, c.n_pcf);
-    if (c.mode == 4) return pcf_disk(map, u.SW, u.SH, sc, bias, c.pcf_filter, rdisk, c.n_pcf);
+    if (c.mode == 4) return pcf_stream(map, u.SW, u.SH, sc, bias, c.pcf_filter, rdisk, c.n_pcf);
     if (c.mode == 2 || c.mode == 5) {
         // (two calls rather than a select between the argument-space sample
         // arrays and the global stream samples)
         float dB;
-        if (c.mode == 5) dB = pcss_blocker_depth<false>(u, map, sc, bias, rdisk, c.n_blocker);
-        else dB = pcss_blocker_depth<true>(u, map, sc, bias, c.blocker_off, c.n_blocker);
+        if (c.mode == 5) dB = pcss_blocker_stream(u, map, sc, bias, rdisk, c.n_blocker);
+        else dB = pcss_blocker_depth(u, map, sc, bias, c.blocker_off, c.n_blocker);
         if ((double)dB < 0.001) return 1.f;
         float pen = (sc.z - dB) * c.area_light / dB;
         float filter = (float)(c.pcss_filter * (double)pen);
-        if (c.mode == 5) return pcf_disk(map, u.SW, u.SH, sc, bias, filter, rdisk + 2 * c.n_blocker, c.n_pcf);
+        if (c.mode == 5) return pcf_stream(map, u.SW, u.SH, sc, bias, filter, rdisk + 2 * c.n_blocker, c.n_pcf);
         return pcf_disk(map, u.SW, u.SH, sc, bias, filter, c.pcf_disk, c.n_pcf);
     }
     if (c.mode == 3) {

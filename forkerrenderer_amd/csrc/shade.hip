@@ -150,60 +150,12 @@ __global__ __launch_bounds__(256) void k_pcss_points(FrameUniforms u, const floa
 constexpr int kBandHalf = 31;   // M_k in [prediction - 31, prediction + 31]: a wavefront per point
 constexpr int kGuesses = 2;     // prediction rounds per segment (more do not pay)
 
-// The blocker test of one sample, as sample_shadow and the search compare it.
-__device__ __forceinline__ void pcss_blocker_step(float4 q, bool in, float dd, float &blocker, float &nb)
-{
-    const float sv = !in ? __builtin_inff() : ((dd < 0.001f) ? 1.f : dd);
-    if (q.z > sv + q.w) {
-        blocker += sv;
-        nb += 1.f;
-    }
-}
-
-// The samples are taken kPcssGroup at a time: the group's disk pairs, then
-// its shadow-map texels, are all loaded before any is used (a texel index of
-// a sample off the map is replaced by 0 and its value by +inf, as
-// sample_shadow returns it), so a search waits on memory twice per group
-// instead of twice per sample; the blocker depths are still summed in sample
-// order.
-constexpr int kPcssGroup = 8;
-
+// The blocker search of an undecided point at sample offset o (the stream
+// samples' loads grouped as in the shading, fgl_device.hpp).
 __device__ __forceinline__ bool pcss_search(const FrameUniforms &u, const float *__restrict__ smap,
                                             const float *__restrict__ pairs, float4 q, uint32_t o)
 {
-    const DevShadowCfg &c = u.shadow;
-    const float2 *d = (const float2 *)pairs + o;
-    const int w = (int)((float)u.SW - 0.001f), h = (int)((float)u.SH - 0.001f);
-    float blocker = 0.f, nb = 0.f;
-    int s = 0;
-    for (; s + kPcssGroup <= c.n_blocker; s += kPcssGroup) {
-        float2 ds[kPcssGroup];
-#pragma unroll
-        for (int e = 0; e < kPcssGroup; ++e) ds[e] = d[s + e];
-        float dd[kPcssGroup];
-        bool in[kPcssGroup];
-#pragma unroll
-        for (int e = 0; e < kPcssGroup; ++e) {
-            const float su = q.x + (float)((double)ds[e].x * c.blocker_filter);
-            const float sw = q.y + (float)((double)ds[e].y * c.blocker_filter);
-            in[e] = su >= 0.f && su <= 1.f && sw >= 0.f && sw <= 1.f;
-            const int x = in[e] ? (int)((float)w * su) : 0, y = in[e] ? (int)((float)h * sw) : 0;
-            dd[e] = smap[(size_t)x + (size_t)y * u.SW];
-        }
-#pragma unroll
-        for (int e = 0; e < kPcssGroup; ++e) pcss_blocker_step(q, in[e], dd[e], blocker, nb);
-    }
-    for (; s < c.n_blocker; ++s) {
-        const float2 ds = d[s];
-        const float su = q.x + (float)((double)ds.x * c.blocker_filter);
-        const float sw = q.y + (float)((double)ds.y * c.blocker_filter);
-        const float sv = sample_shadow(smap, u.SW, u.SH, su, sw);
-        if (q.z > sv + q.w) {
-            blocker += sv;
-            nb += 1.f;
-        }
-    }
-    const float dB = (nb < 1.f) ? 0.f : blocker / nb;
+    const float dB = pcss_blocker_stream(u, smap, vec3(q.x, q.y, q.z), q.w, pairs + 2 * (size_t)o, u.shadow.n_blocker);
     return !((double)dB < 0.001);
 }
 
