@@ -140,7 +140,9 @@ __device__ __forceinline__ void ssao_px_with(const FrameUniforms &u, const SsaoA
 // truncated; the depth read is data[x + y * W] in 32-bit int arithmetic, so
 // an off-screen sample reads another pixel, or lies outside the buffer
 // (undefined in the reference; here no occlusion).  Each occluding sample
-// adds 1/32 (exact); ao = powf(1 - occlusion, 3) by table.
+// adds 1/32 (exact); ao = powf(1 - occlusion, 3) by table.  The samples are
+// the pixel's own (384 bytes a pixel), so they are read 8 at a time (16-byte
+// loads), and the 8 depths they project to are loaded together.
 __device__ __forceinline__ void ssao_ref_px(const FrameUniforms &u, const float *__restrict__ samples,
                                             const float *__restrict__ pow3, const float *__restrict__ depth,
                                             const float *__restrict__ g, float *__restrict__ ao, int x, int y)
@@ -150,23 +152,44 @@ __device__ __forceinline__ void ssao_ref_px(const FrameUniforms &u, const float 
     const Vec3 N = vec3(g[i], g[P + i], g[2 * P + i]);
     const Vec3 Pw = vec3(g[3 * P + i], g[4 * P + i], g[5 * P + i]);
     const float fd = depth[i];
-    const float *smp = samples + i * 96;
+    const float4 *smp = (const float4 *)(samples + i * 96);   // 16-byte aligned: 384 bytes a pixel
     int count = 0;
-    for (int s = 0; s < 32; ++s) {
-        Vec3 d = vec3(smp[3 * s], smp[3 * s + 1], smp[3 * s + 2]);
-        if (!(dot3(d, N) > 0.f)) d = vec3(-d.x, -d.y, -d.z);
-        float sc = len3(d);
-        sc = (1 - 0.1f) * 1.0f + 0.1f * (sc * sc);
-        d = scale3(d, sc);
-        const Vec3 ws = add3(Pw, scale3(d, 0.075f));
-        const Vec4 cs = mv4(u.vp, vec4(ws.x, ws.y, ws.z, 1.f));
-        const Vec4 ndc = divs4(cs, cs.w);
-        const Vec4 ss = mv4(u.viewport, ndc);
-        const int sx = trunc_int(ss.x), sy = trunc_int(ss.y);
-        const int idx = (int)((uint32_t)sx + (uint32_t)sy * (uint32_t)u.W);
-        if (idx < 0 || (size_t)idx >= P) continue;
-        const float cached = depth[idx];
-        if (ss.z >= cached + 0.0005f && fabsf(fd - cached) < 0.01f) ++count;
+    for (int s0 = 0; s0 < 32; s0 += 8) {
+        float v[24];
+#pragma unroll
+        for (int e = 0; e < 6; ++e) {
+            const float4 q = smp[(s0 * 3) / 4 + e];
+            v[4 * e] = q.x;
+            v[4 * e + 1] = q.y;
+            v[4 * e + 2] = q.z;
+            v[4 * e + 3] = q.w;
+        }
+        size_t at[8];
+        float sz[8];
+        bool on[8];
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            Vec3 d = vec3(v[3 * k], v[3 * k + 1], v[3 * k + 2]);
+            if (!(dot3(d, N) > 0.f)) d = vec3(-d.x, -d.y, -d.z);
+            float sc = len3(d);
+            sc = (1 - 0.1f) * 1.0f + 0.1f * (sc * sc);
+            d = scale3(d, sc);
+            const Vec3 ws = add3(Pw, scale3(d, 0.075f));
+            const Vec4 cs = mv4(u.vp, vec4(ws.x, ws.y, ws.z, 1.f));
+            const Vec4 ndc = divs4(cs, cs.w);
+            const Vec4 ss = mv4(u.viewport, ndc);
+            const int sx = trunc_int(ss.x), sy = trunc_int(ss.y);
+            const int idx = (int)((uint32_t)sx + (uint32_t)sy * (uint32_t)u.W);
+            on[k] = !(idx < 0 || (size_t)idx >= P);
+            at[k] = on[k] ? (size_t)idx : 0;
+            sz[k] = ss.z;
+        }
+        float cached[8];
+#pragma unroll
+        for (int k = 0; k < 8; ++k) cached[k] = depth[at[k]];
+#pragma unroll
+        for (int k = 0; k < 8; ++k)
+            if (on[k] && sz[k] >= cached[k] + 0.0005f && fabsf(fd - cached[k]) < 0.01f) ++count;
     }
     ao[i] = pow3[count];
 }
