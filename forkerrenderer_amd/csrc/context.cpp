@@ -105,8 +105,9 @@ struct fgl_ctx {
     DevBuf rng, rng_samples, rng_offs, ssao_pow3;
     uint32_t rng_seed = 5489u;
     // the parallel generator: base sequence, jumped chunk arrays, raw draws,
-    // try flags / their scan, the jump polynomials uploaded so far; the last
-    // fill's tries (T of D draws) for its commit
+    // the per-workgroup counts of kept tries and their scan, the jump
+    // polynomials uploaded so far; the last fill's tries (T of D draws) for
+    // its commit
     DevBuf mt_base, mt_states, mt_raw, mt_pos;
     DevBuf mt_polys[2];          // jump polynomials for the two chunk lengths
     int mt_polys_n[2] = {0, 0};

@@ -136,9 +136,10 @@ __device__ __forceinline__ bool mt_try(const uint32_t *__restrict__ p, float &x,
 // array c 624 K words further on, which the jump gives (mt_jump.hpp): the XOR
 // of the windows x_{B+i} .. x_{B+i+623} of the base sequence over the set
 // bits i of t^(c 624 K) mod phi, x_B .. being the context's array and the 32
-// blocks after it.  The tries are then flagged, scanned and kept in parallel,
-// and a commit moves the context's stream past the tries a caller used.
-// every window of a jump, x_{B + 19967 + 623} at most, lies in the base sequence
+// blocks after it.  The tries are then counted, scanned and kept in
+// parallel, and a commit moves the context's stream past the tries a caller
+// used.  Every window of a jump, x_{B + 19967 + 623} at most, lies in the
+// base sequence.
 static_assert(kMtBaseWords >= 64 * kMtPolyWords + 623, "the jump's windows stay in the base sequence");
 
 // One workgroup: the context's array and the 32 blocks after it, untempered.
