@@ -885,7 +885,7 @@ static int rng_take(fgl_ctx *c, uint32_t m)
 // the points' normal, world position and light NDC as planes of stride n
 // (read for PCSS only).  ss: point i's samples start at pair ss.offs[i]
 // (PCSS; offs[n] = the total) or at i * ss.stride (PCF).
-static constexpr uint32_t kSegmentMax = 32768;   // undecided PCSS points per walk segment, at most
+static constexpr uint32_t kSegmentMax = kPcssSegmentMax;   // undecided PCSS points per walk segment, at most
 
 static int stochastic_shadow_samples(fgl_ctx *c, const FrameUniforms &u, const float *g, size_t n, ShadowSamples &ss)
 {
@@ -908,13 +908,15 @@ static int stochastic_shadow_samples(fgl_ctx *c, const FrameUniforms &u, const f
     // in order; then the stream moves past exactly the samples used.
     if (n * (nb + np) > 0xffffffffull) return fail(c, FGL_ERR_CAPACITY, "stochastic shadows: too many samples");
     const size_t words = scan_tmp_words(n + 1), n4 = (n + 4) & ~(size_t)3;
-    const size_t seg_words = 2 * (size_t)kSegmentMax + 2 * ((size_t)kSegmentMax + 2) + 4;
+    const size_t seg_words = 2 * (size_t)kSegmentMax + 2 * kPcssChainFbits + kPcssChainRes + 2 * ((size_t)kSegmentMax + 2) + 4;
     if ((rc = ensure(c, c->rng_offs, (6 * n4 + 4 * n4 + seg_words + words) * sizeof(uint32_t)))) return rc;
     uint32_t *base = (uint32_t *)c->rng_offs.p, *upos = base + n4, *list = upos + n4;
     uint32_t *found = list + n4, *offs = found + n4, *pbase = offs + n4;
     float4 *pt = (float4 *)(pbase + n4);
     unsigned long long *masks = (unsigned long long *)(pt + n4);
-    uint32_t *pred = (uint32_t *)(masks + kSegmentMax);
+    unsigned long long *fbits = masks + kSegmentMax;
+    uint32_t *res = (uint32_t *)(fbits + kPcssChainFbits);
+    uint32_t *pred = res + kPcssChainRes;
     uint32_t *seg_out = pred + 2 * ((size_t)kSegmentMax + 2);
     uint32_t *tmp = seg_out + 4;
     launch_pcss_classify(u, g, n, (const float *)c->shadow.p, base, upos, c->stream);
@@ -940,8 +942,8 @@ static int stochastic_shadow_samples(fgl_ctx *c, const FrameUniforms &u, const f
     float rate = 0.25f;
     while (s0 < n_unc) {
         const uint32_t L = n_unc - s0 < seg ? n_unc - s0 : seg;
-        launch_pcss_segment(u, (const float *)c->shadow.p, pairs, pt, pbase, s0, L, M0, rate, pred, tmp, masks, found,
-                            seg_out, c->stream);
+        launch_pcss_segment(u, (const float *)c->shadow.p, pairs, pt, pbase, s0, L, M0, rate, pred, tmp, masks, fbits,
+                            res, found, seg_out, c->stream);
         HIPCHK(c, hipMemcpyAsync(c->h_total, seg_out, 8, hipMemcpyDeviceToHost, c->stream));
         HIPCHK(c, hipStreamSynchronize(c->stream));
         const uint32_t s1 = c->h_total[0], M1 = c->h_total[1];

@@ -208,14 +208,21 @@ void launch_pcss_classify(const FrameUniforms &u, const float *g, size_t P, cons
 void launch_pcss_compact(const uint32_t *upos, size_t P, uint32_t *list, hipStream_t s);
 // The n undecided points' stream-independent inputs (shadow coordinate and
 // bias, sample base), then segments [s0, s0 + L) of the ordered walk from
-// M0 misses before s0 (rate: the miss rate so far): scratch pred (2 L + 2
-// words), scan_tmp (scan_tmp_words(L)), masks (L 64-bit words); out =
-// (first unsettled point, misses before it).
+// M0 misses before s0 (rate: the miss rate so far; L <= kPcssSegmentMax):
+// scratch pred (2 L + 2 words), scan_tmp (scan_tmp_words(L)), masks (L
+// 64-bit words), fbits (kPcssChainFbits 64-bit words) and res
+// (kPcssChainRes words) for the walks of the segment's blocks; out = (first
+// unsettled point, misses before it).
+constexpr uint32_t kPcssSegmentMax = 32768, kPcssChainBlock = 1024;
+constexpr int kPcssBandWidth = 63;
+constexpr size_t kPcssChainFbits = (size_t)(kPcssSegmentMax / kPcssChainBlock) * kPcssBandWidth * (kPcssChainBlock / 64);
+constexpr size_t kPcssChainRes = 2 * (size_t)(kPcssSegmentMax / kPcssChainBlock) * kPcssBandWidth;
 void launch_pcss_points(const FrameUniforms &u, const float *g, size_t P, const uint32_t *base, const uint32_t *list,
                         uint32_t n, float4 *pt, uint32_t *pbase, hipStream_t s);
 void launch_pcss_segment(const FrameUniforms &u, const float *shadow_map, const float *pairs, const float4 *pt,
                          const uint32_t *pbase, uint32_t s0, uint32_t L, uint32_t M0, float rate, uint32_t *pred,
-                         uint32_t *scan_tmp, unsigned long long *masks, uint32_t *found, uint32_t *out, hipStream_t s);
+                         uint32_t *scan_tmp, unsigned long long *masks, unsigned long long *fbits, uint32_t *res,
+                         uint32_t *found, uint32_t *out, hipStream_t s);
 void launch_pcss_offsets(const uint32_t *base, const uint32_t *upos, const uint32_t *fpos, uint32_t n_pcf, size_t P,
                          uint32_t *offs, hipStream_t s);
 void launch_ssao(const FrameUniforms &u, const SsaoArgs &sa, const float *depth,

@@ -38,8 +38,8 @@ __global__ __launch_bounds__(256) void k_ssao_ref(FrameUniforms u, const float *
 // could reach (the square around the disk) and marks the point "no blocker
 // for any sample set", "a blocker for any sample set" or "depends on the
 // samples".  Only the last kind is walked in order (k_pcss_points,
-// k_pcss_guess, k_pcss_band, k_pcss_resolve below), at offsets that count
-// the decided points before it.  The points' inputs are planes of stride P:
+// k_pcss_guess, k_pcss_band, k_pcss_walk_blocks, k_pcss_chain below), at
+// offsets that count the decided points before it.  The points' inputs are planes of stride P:
 // normal, world position, light NDC.
 
 // Light 0's shadow coordinate and bias at point i, as shade_point computes them.
@@ -140,9 +140,10 @@ __global__ __launch_bounds__(256) void k_pcss_points(FrameUniforms u, const floa
 //    do not depend on the samples' offset, which is most points;
 //  * k_pcss_band evaluates every point at the 63 values of M_k around its
 //    prediction, a wavefront per point, the outcomes as a bit mask;
-//  * k_pcss_resolve follows the actual M_k through the masks in order,
-//    writes found[k] (1: a blocker) and stops where M_k leaves the band or
-//    the segment ends; the host starts the next segment there.
+//  * k_pcss_walk_blocks and k_pcss_chain follow the actual M_k through the
+//    masks in order, write found[k] (1: a blocker) and stop where M_k
+//    leaves the band or the segment ends; the host starts the next segment
+//    there.
 // Every evaluation is the blocker search exactly as FindAverageBlockDepth
 // runs it (shadow.cpp:65-91: the samples in order, blocker depths summed in
 // that order).  Every segment settles its first point (its prediction is
@@ -200,99 +201,125 @@ __global__ __launch_bounds__(256) void k_pcss_band(FrameUniforms u, const float 
     if (lane == 0) masks[i] = b;
 }
 
-// One wavefront, lane l on point g + l of the current group of 64.  With
-// P_i the prediction (misses before point i in the segment, pred[i]) and
-// D_i = 31 + M_i - M0 - P_i the actual M_i's place in point i's band,
-// D_{i+1} = D_i + (miss_i(D_i) - (P_{i+1} - P_i)): D stays put while the
-// prediction holds.  So the lanes from the first unsettled one on all test
-// the same D; a ballot finds the first lane whose outcome breaks the
-// prediction, the lanes up to it are settled, and D moves by one from
-// there.  A group takes one step per broken prediction.  The masks and
-// prediction steps of 1024 points at a time are copied to LDS, so the walk
-// waits on LDS, not on memory; the next block's loads are all issued into
-// registers before the walk of the current one starts, so their latency
-// (the masks come from other XCDs' caches) overlaps the walk.  out = (first
-// unsettled point, M there).
-constexpr uint32_t kResolveBlock = 1024;
-constexpr int kResolvePer = kResolveBlock / 64;   // points a lane stages per block
+// Following the actual M_k through the masks.  With P_i the prediction
+// (misses before point i in the segment, pred[i]) and D_i = 31 + M_i - M0 -
+// P_i the actual M_i's place in point i's band, D_{i+1} = D_i + (miss_i(D_i)
+// - (P_{i+1} - P_i)): D stays put while the prediction holds.  So a
+// wavefront walking 64 points tests them all at one D; a ballot finds the
+// first point whose outcome breaks the prediction (equals the predicted
+// step), the points up to it settle (every one before it with the opposite
+// outcome), and D moves by one from there.
+//
+// The segment is cut into blocks of 1024 points, and every block is walked
+// from each of the 63 places D can enter it with (k_pcss_walk_blocks: a
+// workgroup per block and 16 places, a wavefront per place; block 0 only
+// from 31), each walk writing its outcomes, the points it settled and the D
+// it left with.  k_pcss_chain then follows the actual D from block to block
+// through those results and copies the chosen walks' outcomes.  A walk stops
+// where D leaves the band; so does the chain, at the first block whose walk
+// stopped or which D enters from outside the band.
+constexpr int kBandWidth = 2 * kBandHalf + 1;
+constexpr int kChainWaves = 16;   // start places per workgroup
+static_assert(kPcssChainBlock == 1024 && kPcssBandWidth == kBandWidth, "chain layout");
 
-__global__ __launch_bounds__(64) void k_pcss_resolve(const unsigned long long *__restrict__ masks,
-                                                     const uint32_t *__restrict__ pred, uint32_t s0, uint32_t L,
-                                                     uint32_t M0, uint32_t *__restrict__ found,
-                                                     uint32_t *__restrict__ out)
+__global__ __launch_bounds__(64 * kChainWaves) void k_pcss_walk_blocks(const unsigned long long *__restrict__ masks,
+                                                                      const uint32_t *__restrict__ pred, uint32_t L,
+                                                                      unsigned long long *__restrict__ fbits,
+                                                                      uint32_t *__restrict__ res)
 {
-    __shared__ unsigned long long sm[kResolveBlock];
-    __shared__ int sdl[kResolveBlock];
-    const uint32_t lane = threadIdx.x;
-    int D = kBandHalf;
-    uint32_t base = 0, settled = 0;
-    bool stop = false;
-    // the block at `at` (< L) into registers; nothing waits on these loads
-    // here.  Straight-line code (indices past the block repeat its last
-    // point; their LDS slots are never read), so no branch leaves the
-    // compiler unsure which loads are still in flight.
-    unsigned long long rm[kResolvePer];
-    uint32_t rp0[kResolvePer], rp1[kResolvePer];
-    auto fetch = [&](uint32_t at) {
-        const uint32_t last = (L - at < kResolveBlock ? L - at : kResolveBlock) - 1;
-#pragma unroll
-        for (int e = 0; e < kResolvePer; ++e) {
-            const uint32_t j0 = lane + 64u * (uint32_t)e, j = at + (j0 < last ? j0 : last);
-            rm[e] = masks[j];
-            rp0[e] = pred[j];
-            rp1[e] = pred[j + 1];   // pred[L]: the segment's predicted total
-        }
-    };
-    if (L > 0) fetch(0);
-    while (base < L && !stop) {
-        const uint32_t nb = L - base < kResolveBlock ? L - base : kResolveBlock;
-#pragma unroll
-        for (int e = 0; e < kResolvePer; ++e) {
-            sm[lane + 64u * (uint32_t)e] = rm[e];
-            sdl[lane + 64u * (uint32_t)e] = (int)(rp1[e] - rp0[e]);
-        }
-        __syncthreads();
-        if (base + nb < L) fetch(base + nb);
-        for (uint32_t g = 0; g < nb && !stop; g += 64) {
-            const uint32_t cnt = nb - g < 64 ? nb - g : 64;
-            const unsigned long long valid = cnt == 64 ? ~0ull : ((1ull << cnt) - 1ull);
-            // With the prediction step dl in {0, 1}, a point breaks the
-            // prediction at D exactly when its outcome there equals dl (a
-            // blocker found where a miss was predicted, or the reverse):
-            // bm marks those D.  D then moves by +1 (dl = 0) or -1 (dl = 1);
-            // a point settled at a break has outcome dl, every other settled
-            // point 1 - dl.
-            const unsigned long long m = sm[g + lane];
-            const int dl = sdl[g + lane];
-            const unsigned long long bm = dl ? m : ~m;
-            const unsigned long long dlm = __ballot(dl != 0);
-            uint32_t start = 0;
-            unsigned long long brks = 0;
-            while (start < cnt) {
-                if (D < 0 || D > 2 * kBandHalf) {
-                    stop = true;
-                    break;
-                }
-                const unsigned long long b = __ballot((uint32_t)(bm >> D) & 1u) & valid & (~0ull << start);
-                if (!b) {
-                    start = cnt;
-                    break;
-                }
-                const uint32_t j = (uint32_t)__builtin_ctzll(b);
-                brks |= 1ull << j;
-                D += ((dlm >> j) & 1ull) ? -1 : 1;
-                start = j + 1;
+    __shared__ unsigned long long sm[kPcssChainBlock];
+    __shared__ int sdl[kPcssChainBlock];
+    const uint32_t b = blockIdx.x, at = b * kPcssChainBlock;
+    const uint32_t nb = L - at < kPcssChainBlock ? L - at : kPcssChainBlock;
+    for (uint32_t j = threadIdx.x; j < nb; j += 64 * kChainWaves) {
+        sm[j] = masks[at + j];
+        sdl[j] = (int)(pred[at + j + 1] - pred[at + j]);   // pred[L]: the segment's predicted total
+    }
+    __syncthreads();
+    const int lane = (int)(threadIdx.x & 63), d0 = (int)blockIdx.y * kChainWaves + (int)(threadIdx.x >> 6);
+    if (d0 >= kBandWidth || (b == 0 && d0 != kBandHalf)) return;   // whole wavefronts, after the only barrier
+    unsigned long long *fo = fbits + ((size_t)b * kBandWidth + d0) * (kPcssChainBlock / 64);
+    int D = d0;
+    uint32_t settled = 0;
+    for (uint32_t g = 0; g < nb; g += 64) {
+        const uint32_t cnt = nb - g < 64 ? nb - g : 64;
+        const unsigned long long valid = cnt == 64 ? ~0ull : ((1ull << cnt) - 1ull);
+        // (slots past the block are never settled: valid masks them out)
+        const unsigned long long m = sm[g + lane];
+        const int dl = sdl[g + lane];
+        const unsigned long long bm = dl ? m : ~m;
+        const unsigned long long dlm = __ballot(dl != 0);
+        uint32_t start = 0;
+        unsigned long long brks = 0;
+        bool stop = false;
+        while (start < cnt) {
+            if (D < 0 || D > 2 * kBandHalf) {
+                stop = true;
+                break;
             }
-            const unsigned long long fb = ~(dlm ^ brks);
-            if (lane < start) found[s0 + base + g + lane] = (uint32_t)(fb >> lane) & 1u;
-            settled = base + g + start;
+            const unsigned long long hit = __ballot((uint32_t)(bm >> D) & 1u) & valid & (~0ull << start);
+            if (!hit) {
+                start = cnt;
+                break;
+            }
+            const uint32_t j = (uint32_t)__builtin_ctzll(hit);
+            brks |= 1ull << j;
+            D += ((dlm >> j) & 1ull) ? -1 : 1;
+            start = j + 1;
         }
-        base += nb;
-        __syncthreads();   // the block's LDS copy is read before the next one overwrites it
+        if (lane == 0) fo[g / 64] = ~(dlm ^ brks);   // outcomes of the settled points
+        settled += start;
+        if (stop) break;
     }
     if (lane == 0) {
+        uint32_t *r = res + 2 * ((size_t)b * kBandWidth + d0);
+        r[0] = settled;
+        r[1] = (uint32_t)D;
+    }
+}
+
+// One workgroup: the chain through the blocks' walks (their results staged
+// in LDS), then found[] for the settled points.  out = (first unsettled
+// point, M there).
+__global__ __launch_bounds__(256) void k_pcss_chain(const unsigned long long *__restrict__ fbits,
+                                                    const uint32_t *__restrict__ res, const uint32_t *__restrict__ pred,
+                                                    uint32_t s0, uint32_t L, uint32_t M0, uint32_t *__restrict__ found,
+                                                    uint32_t *__restrict__ out)
+{
+    constexpr uint32_t kMaxBlocks = kPcssSegmentMax / kPcssChainBlock;
+    __shared__ uint32_t sres[2 * kBandWidth * kMaxBlocks];
+    __shared__ int pick[kMaxBlocks];
+    __shared__ uint32_t s_settled;
+    __shared__ int s_D;
+    const uint32_t tid = threadIdx.x;
+    const uint32_t nblk = (L + kPcssChainBlock - 1) / kPcssChainBlock;
+    for (uint32_t i = tid; i < 2 * kBandWidth * nblk; i += 256) sres[i] = res[i];
+    __syncthreads();
+    if (tid == 0) {
+        int D = kBandHalf;
+        uint32_t settled = 0;
+        for (uint32_t b = 0; b < nblk; ++b) {
+            if (D < 0 || D > 2 * kBandHalf) break;
+            const uint32_t len = L - b * kPcssChainBlock < kPcssChainBlock ? L - b * kPcssChainBlock : kPcssChainBlock;
+            const uint32_t r = sres[2 * (b * kBandWidth + (uint32_t)D)];
+            pick[b] = D;
+            D = (int)sres[2 * (b * kBandWidth + (uint32_t)D) + 1];
+            settled += r;
+            if (r < len) break;
+        }
+        s_settled = settled;
+        s_D = D;
+    }
+    __syncthreads();
+    const uint32_t settled = s_settled;
+    for (uint32_t k = tid; k < settled; k += 256) {
+        const uint32_t b = k / kPcssChainBlock, kk = k - b * kPcssChainBlock;
+        const unsigned long long w = fbits[((size_t)b * kBandWidth + (uint32_t)pick[b]) * (kPcssChainBlock / 64) + kk / 64];
+        found[s0 + k] = (uint32_t)(w >> (kk & 63)) & 1u;
+    }
+    if (tid == 0) {
         out[0] = s0 + settled;
-        out[1] = (uint32_t)((int)M0 + (int)pred[settled] + D - kBandHalf);
+        out[1] = (uint32_t)((int)M0 + (int)pred[settled] + s_D - kBandHalf);
     }
 }
 
@@ -570,9 +597,10 @@ void launch_pcss_points(const FrameUniforms &u, const float *g, size_t P, const 
 
 void launch_pcss_segment(const FrameUniforms &u, const float *shadow_map, const float *pairs, const float4 *pt,
                          const uint32_t *pbase, uint32_t s0, uint32_t L, uint32_t M0, float rate, uint32_t *pred,
-                         uint32_t *scan_tmp, unsigned long long *masks, uint32_t *found, uint32_t *out, hipStream_t s)
+                         uint32_t *scan_tmp, unsigned long long *masks, unsigned long long *fbits, uint32_t *res,
+                         uint32_t *found, uint32_t *out, hipStream_t s)
 {
-    if (L == 0) return;
+    if (L == 0 || L > kPcssSegmentMax) return;
     // kGuesses guesses: from the straight line, then each from the misses of
     // the one before; the last lands in pred + (L + 1)
     uint32_t *bufs[2] = {pred + (L + 1), pred};
@@ -587,8 +615,10 @@ void launch_pcss_segment(const FrameUniforms &u, const float *shadow_map, const 
     const uint32_t *pred2 = prev;
     hipLaunchKernelGGL(k_pcss_band, dim3((L + 3) / 4), dim3(256), 0, s, u, shadow_map, pairs, pt, pbase, s0, L, M0,
                        (const uint32_t *)pred2, masks);
-    hipLaunchKernelGGL(k_pcss_resolve, dim3(1), dim3(64), 0, s, masks, (const uint32_t *)pred2, s0, L, M0, found,
-                       out);
+    const uint32_t nblk = (L + kPcssChainBlock - 1) / kPcssChainBlock;
+    hipLaunchKernelGGL(k_pcss_walk_blocks, dim3(nblk, (kBandWidth + kChainWaves - 1) / kChainWaves),
+                       dim3(64 * kChainWaves), 0, s, masks, pred2, L, fbits, res);
+    hipLaunchKernelGGL(k_pcss_chain, dim3(1), dim3(256), 0, s, fbits, res, pred2, s0, L, M0, found, out);
 }
 
 void launch_pcss_offsets(const uint32_t *base, const uint32_t *upos, const uint32_t *fpos, uint32_t n_pcf, size_t P,
