@@ -406,7 +406,7 @@ __constant__ float c_ref_blur_w[5] = {(float)0.227027, (float)0.1945946, (float)
 // in registers, and the inputs are loaded kPF steps ahead (samples ahead are
 // never written before they are read), so a step waits on its own
 // arithmetic, not on memory.
-constexpr int kPF = 16;
+constexpr int kPF = 64;
 
 __device__ __forceinline__ void blur_rec_line(float *__restrict__ p, int n, size_t s)
 {
