@@ -423,8 +423,13 @@ __device__ __forceinline__ float pcf_stream(const float *map, int SW, int SH, Ve
 __device__ __forceinline__ Vec3 shadow_coord(Vec3 ndc) { return add3(scale3(ndc, 0.5f), vec3(0.5f, 0.5f, 0.5f)); }
 __device__ __forceinline__ float shadow_bias(Vec3 n, Vec3 l) { return smax(0.009f * (1.f - dot3(n, l)), 0.007f); }
 
-// `rdisk`: this pixel's unit-disk samples from the random stream (modes 4 and
-// 5 only): the PCF samples, or the blocker samples followed by the PCF ones.
+// STREAM: the kernel runs with the random stream's samples (modes 4 and 5),
+// `rdisk` being this pixel's: the PCF samples, or the blocker samples
+// followed by the PCF ones.  The kernels without them are compiled without
+// the stream filters (their registers would cost the other modes occupancy);
+// there modes 4 and 5 do not occur (the host launches the STREAM kernels for
+// them) and would read as lit.
+template <bool STREAM>
 __device__ __forceinline__ float shadow_visibility(const FrameUniforms &u, const float *map,
                                                    Vec3 ndc, Vec3 n, Vec3 l, const float *rdisk)
 {
@@ -432,17 +437,25 @@ __device__ __forceinline__ float shadow_visibility(const FrameUniforms &u, const
     Vec3 sc = shadow_coord(ndc);
     float bias = shadow_bias(n, l);
     if (c.mode == 1) return pcf_disk(map, u.SW, u.SH, sc, bias, c.pcf_filter, c.pcf_disk, c.n_pcf);
-    if (c.mode == 4) return pcf_stream(map, u.SW, u.SH, sc, bias, c.pcf_filter, rdisk, c.n_pcf);
-    if (c.mode == 2 || c.mode == 5) {
-        // (two calls rather than a select between the argument-space sample
-        // arrays and the global stream samples)
-        float dB;
-        if (c.mode == 5) dB = pcss_blocker_stream(u, map, sc, bias, rdisk, c.n_blocker);
-        else dB = pcss_blocker_depth(u, map, sc, bias, c.blocker_off, c.n_blocker);
+    if (c.mode == 4) {
+        if constexpr (STREAM) return pcf_stream(map, u.SW, u.SH, sc, bias, c.pcf_filter, rdisk, c.n_pcf);
+        return 1.f;
+    }
+    if (c.mode == 5) {
+        if constexpr (STREAM) {
+            const float dB = pcss_blocker_stream(u, map, sc, bias, rdisk, c.n_blocker);
+            if ((double)dB < 0.001) return 1.f;
+            float pen = (sc.z - dB) * c.area_light / dB;
+            float filter = (float)(c.pcss_filter * (double)pen);
+            return pcf_stream(map, u.SW, u.SH, sc, bias, filter, rdisk + 2 * c.n_blocker, c.n_pcf);
+        }
+        return 1.f;
+    }
+    if (c.mode == 2) {
+        const float dB = pcss_blocker_depth(u, map, sc, bias, c.blocker_off, c.n_blocker);
         if ((double)dB < 0.001) return 1.f;
         float pen = (sc.z - dB) * c.area_light / dB;
         float filter = (float)(c.pcss_filter * (double)pen);
-        if (c.mode == 5) return pcf_stream(map, u.SW, u.SH, sc, bias, filter, rdisk + 2 * c.n_blocker, c.n_pcf);
         return pcf_disk(map, u.SW, u.SH, sc, bias, filter, c.pcf_disk, c.n_pcf);
     }
     if (c.mode == 3) {
@@ -556,6 +569,7 @@ __device__ __forceinline__ Vec3 light_dir(const DevLight &l, Vec3 P)
 // One surface point lit by every light.  With a single point light this is
 // the reference's CalculateLight; `deferred_quirk` passes viewDir in the
 // Blinn-Phong halfway slot as DrawScreenSpacePixels does (forkergl.cpp:369).
+template <bool STREAM = false>
 __device__ __forceinline__ Vec3 shade_point(const FrameUniforms &u, const float *smap, bool pbr,
                                             bool deferred_quirk, Vec3 P, Vec3 N, Vec3 lndc,
                                             Vec3 albedo, Vec3 emissive, Vec3 param,
@@ -572,7 +586,7 @@ __device__ __forceinline__ Vec3 shade_point(const FrameUniforms &u, const float 
             Vec3 L = light_dir(l, P);
             float vis = 1.f;
             if (!u.shadow_on) vis = 0.f;
-            else if (i == 0) vis = shadow_visibility(u, smap, lndc, N, L, rdisk);
+            else if (i == 0) vis = shadow_visibility<STREAM>(u, smap, lndc, N, L, rdisk);
             Vec3 H = deferred_quirk ? Vd : norm3(add3(L, Vd));
             c = add3(c, bp_term(L, H, N, vis, dl, el, param.x, param.y, param.z,
                                 vec3(l.color[0], l.color[1], l.color[2]), u.shadow_on, i == 0));
@@ -587,7 +601,7 @@ __device__ __forceinline__ Vec3 shade_point(const FrameUniforms &u, const float 
         Vec3 L = light_dir(l, P);
         float vis = 1.f;
         if (!u.shadow_on) vis = 0.f;
-        else if (i == 0) vis = shadow_visibility(u, smap, lndc, N, L, rdisk);
+        else if (i == 0) vis = shadow_visibility<STREAM>(u, smap, lndc, N, L, rdisk);
         Vec3 H = norm3(add3(L, Vd));
         Vec3 lo = pbr_lo(L, Vd, H, N, vis, al, param.y, param.z,
                          vec3(l.color[0], l.color[1], l.color[2]), u.shadow_on);

@@ -160,6 +160,7 @@ __global__ __launch_bounds__(256) void k_gbuffer_resolve(ResolveArgs a, FrameUni
 // (phongshader.h:90-169, pbrshader.h:90-180) once for the visible fragment.
 // The forward fragment program for the winner (face f of draw dr, material
 // m, barycentrics b) at pixel i.
+template <bool STREAM>
 __device__ __forceinline__ void forward_shade(const ResolveArgs &a, const FrameUniforms &u, const DevDraw &dr,
                                               const DevMaterial &m, int f, const Vec3 &b, size_t i, size_t P)
 {
@@ -169,7 +170,7 @@ __device__ __forceinline__ void forward_shade(const ResolveArgs &a, const FrameU
     FragGeom fg = frag_geometry(dr, u, a.textures, v0, v1, v2, b, m.map_bump);
     // the stochastic shadow filters: this fragment's samples in the stream
     const float *rdisk = nullptr;
-    if (a.ss.pairs) rdisk = a.ss.pairs + 2 * (a.ss.offs ? (size_t)a.ss.offs[i] : i * (size_t)a.ss.stride);
+    if (STREAM && a.ss.pairs) rdisk = a.ss.pairs + 2 * (a.ss.offs ? (size_t)a.ss.offs[i] : i * (size_t)a.ss.stride);
     Vec3 c;
     if (!dr.support_pbr) {
         Vec3 dc = m.map_kd >= 0 ? tex_sample(a.textures, m.map_kd, fg.uv)
@@ -178,7 +179,7 @@ __device__ __forceinline__ void forward_shade(const ResolveArgs &a, const FrameU
                                 : vec3(m.ke[0], m.ke[1], m.ke[2]);
         float shin = 1.f;
         if (m.map_ks >= 0) shin = tex_samplef(a.textures, m.map_ks, fg.uv) + 5;
-        c = shade_point(u, a.shadow_map, false, false, fg.pos, fg.normal, fg.lndc, dc, em,
+        c = shade_point<STREAM>(u, a.shadow_map, false, false, fg.pos, fg.normal, fg.lndc, dc, em,
                         vec3(m.ka[0], m.ks[0], shin), rdisk);
     } else {
         Vec3 al = m.map_base >= 0 ? tex_sample(a.textures, m.map_base, fg.uv)
@@ -188,7 +189,7 @@ __device__ __forceinline__ void forward_shade(const ResolveArgs &a, const FrameU
         float rough = m.map_rough >= 0 ? tex_samplef(a.textures, m.map_rough, fg.uv) : m.roughness;
         float metal = m.map_metal >= 0 ? tex_samplef(a.textures, m.map_metal, fg.uv) : m.metalness;
         float ao = m.map_ao >= 0 ? tex_samplef(a.textures, m.map_ao, fg.uv) : 1.f;
-        c = shade_point(u, a.shadow_map, true, false, fg.pos, fg.normal, fg.lndc, al, em,
+        c = shade_point<STREAM>(u, a.shadow_map, true, false, fg.pos, fg.normal, fg.lndc, al, em,
                         vec3(ao, metal, rough), rdisk);
     }
     a.fb[i] = c.x;
@@ -198,6 +199,7 @@ __device__ __forceinline__ void forward_shade(const ResolveArgs &a, const FrameU
 
 // Forward resolve, one thread per pixel; draw and material records through
 // readfirstlane when the wavefront shares them (as in gbuffer_px).
+template <bool STREAM>
 __global__ __launch_bounds__(256) void k_forward_resolve(ResolveArgs a, FrameUniforms u)
 {
     const int x = blockIdx.x * 64 + (threadIdx.x & 63);
@@ -223,13 +225,13 @@ __global__ __launch_bounds__(256) void k_forward_resolve(ResolveArgs a, FrameUni
         const int mi = dr.face_mat[f];
         const int m0 = __builtin_amdgcn_readfirstlane(mi);
         if (__ballot(mi != m0) == 0ull) {
-            forward_shade(a, u, dr, a.materials[m0], f, b, i, P);
+            forward_shade<STREAM>(a, u, dr, a.materials[m0], f, b, i, P);
             return;
         }
     }
     const DevDraw &dr = a.draws[tv.w];
     const int f = g - dr.tri_base;
-    forward_shade(a, u, dr, a.materials[dr.face_mat[f]], f, b, i, P);
+    forward_shade<STREAM>(a, u, dr, a.materials[dr.face_mat[f]], f, b, i, P);
 }
 
 // ---------------- the forward pass's fragment sequence ----------------
@@ -458,7 +460,8 @@ void launch_gbuffer_resolve(const ResolveArgs &a, const FrameUniforms &u, hipStr
 void launch_forward_resolve(const ResolveArgs &a, const FrameUniforms &u, hipStream_t s)
 {
     if (a.W <= 0 || a.H <= 0) return;
-    hipLaunchKernelGGL(k_forward_resolve, grid2(a.W, a.H), dim3(256), 0, s, a, u);
+    if (a.ss.pairs) hipLaunchKernelGGL(k_forward_resolve<true>, grid2(a.W, a.H), dim3(256), 0, s, a, u);
+    else hipLaunchKernelGGL(k_forward_resolve<false>, grid2(a.W, a.H), dim3(256), 0, s, a, u);
 }
 
 template <bool EMIT>

@@ -5,6 +5,7 @@
 
 namespace fgl {
 
+template <bool STREAM>
 __global__ __launch_bounds__(256) void k_lighting(FrameUniforms u, const float *__restrict__ g,
                                                   const float *__restrict__ ao, int ao_is_one, NetAo net,
                                                   const float *__restrict__ smap, ShadowSamples ss,
@@ -13,7 +14,7 @@ __global__ __launch_bounds__(256) void k_lighting(FrameUniforms u, const float *
     const int x = blockIdx.x * 64 + (threadIdx.x & 63);
     const int y = blockIdx.y * 4 + (threadIdx.x >> 6);
     if (x >= u.W || y >= u.H) return;
-    lighting_px(u, g, ao, ao_is_one, smap, ss, fb, x, y, net);
+    lighting_px<STREAM>(u, g, ao, ao_is_one, smap, ss, fb, x, y, net);
 }
 
 __global__ __launch_bounds__(256) void k_ssao_ref(FrameUniforms u, const float *__restrict__ samples,
@@ -563,8 +564,12 @@ void launch_lighting(const FrameUniforms &u, const float *gbuf, const float *ao,
                      const float *shadow_map, const ShadowSamples &ss, float *fb, hipStream_t s)
 {
     if (u.W <= 0 || u.H <= 0) return;
-    hipLaunchKernelGGL(k_lighting, grid2(u.W, u.H), dim3(256), 0, s, u, gbuf, ao, ao_is_one, net,
-                       shadow_map, ss, fb);
+    if (ss.pairs)
+        hipLaunchKernelGGL(k_lighting<true>, grid2(u.W, u.H), dim3(256), 0, s, u, gbuf, ao, ao_is_one, net,
+                           shadow_map, ss, fb);
+    else
+        hipLaunchKernelGGL(k_lighting<false>, grid2(u.W, u.H), dim3(256), 0, s, u, gbuf, ao, ao_is_one, net,
+                           shadow_map, ss, fb);
 }
 
 void launch_ssao_reference(const FrameUniforms &u, const float *samples, const float *pow3,

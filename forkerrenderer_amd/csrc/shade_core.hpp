@@ -13,6 +13,7 @@ namespace fgl {
 // read the G-buffer (76 B/px) + AO (4 B/px, or the network AO still on its
 // coarse grid: 1 B of coverage and four cached grid values), light, write
 // the frame buffer (12 B/px).
+template <bool STREAM = false>
 __device__ __forceinline__ void lighting_px(const FrameUniforms &u, const float *__restrict__ g,
                                             const float *__restrict__ ao, int ao_is_one,
                                             const float *__restrict__ smap, const ShadowSamples &ss,
@@ -31,8 +32,8 @@ __device__ __forceinline__ void lighting_px(const FrameUniforms &u, const float 
     if (net.lao) pa.x *= net.fmask[i] ? net_ao_interp(net.lao, net.lW, net.lH, net.f, x, y) : 1.f;
     else if (!ao_is_one) pa.x *= ao[i];
     const float *rdisk = nullptr;
-    if (ss.pairs) rdisk = ss.pairs + 2 * (ss.offs ? (size_t)ss.offs[i] : i * (size_t)ss.stride);
-    Vec3 c = shade_point(u, smap, !(type < 0.5f), true, Pw, N, L, al, em, pa, rdisk);
+    if (STREAM && ss.pairs) rdisk = ss.pairs + 2 * (ss.offs ? (size_t)ss.offs[i] : i * (size_t)ss.stride);
+    Vec3 c = shade_point<STREAM>(u, smap, !(type < 0.5f), true, Pw, N, L, al, em, pa, rdisk);
     fb[i] = c.x;
     fb[P + i] = c.y;
     fb[2 * P + i] = c.z;
