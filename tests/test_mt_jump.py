@@ -1,8 +1,9 @@
 """The host jump-ahead for std::mt19937 (forkerrenderer_amd/csrc/mt_jump.cpp):
 the characteristic polynomial found by Berlekamp-Massey and the jump
-polynomials t^(c 624 256) mod phi, checked against brute-force generation of
-the word sequence for a seed the polynomial was not derived from
-(tests/mt_jump_check.cpp, built with the host compiler)."""
+polynomials t^(c 624 K) mod phi for the two chunk lengths K = 256 and 512
+(asked for in turn, one table extended after the other), checked against
+brute-force generation of the word sequence for a seed the polynomial was
+not derived from (tests/mt_jump_check.cpp, built with the host compiler)."""
 import os
 import subprocess
 
